@@ -1,0 +1,471 @@
+"""ctypes binding of the ASC::sparse_hip C ABI (include/asc_sparse_hip.h).
+
+This is the binding a Python caller would write against the C ABI: plain
+structs, raw device pointers, status codes mapped to exceptions.  It loads the
+in-tree ``lib/libasc_sparse_hip.so`` and fails loudly when it is missing --
+there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(os.path.dirname(_HERE), "lib")
+LIB_PATH = os.path.join(LIB_DIR, "libasc_sparse_hip.so")
+GEN_PATH = os.path.join(LIB_DIR, "libasc_sparse_gen.so")
+
+# ---- enums (values of include/asc_sparse_hip.h) ---------------------------
+OK, INVALID_ARGUMENT, SHAPE, INDEX, OVERFLOW, INVALID_STATE, ALLOCATION = 0, 1, 2, 3, 4, 5, 6
+MEMORY_ACCESS, MEMORY_TRANSFER, UNSUPPORTED, UNAVAILABLE, PROVIDER = 7, 8, 9, 10, 11
+NUMERICAL, INTERNAL = 12, 18
+
+F32, F64, C64, C128, BF16, F16 = 0, 1, 2, 3, 4, 5
+COO, CSR, CSC = 0, 1, 2
+HOST, PINNED_HOST, DEVICE, MANAGED = 0, 1, 2, 3
+OP_N, OP_T, OP_H = 0, 1, 2
+COLUMN_MAJOR, ROW_MAJOR = 0, 1
+DETERMINISTIC, BACKEND_DEFAULT = 0, 1
+
+ALG_AUTO, ALG_ORDERED, ALG_MERGE = 0, 1, 2
+ALG_VECTOR = {1: 3, 2: 4, 4: 5, 8: 6, 16: 7, 32: 8, 64: 9}
+ALG_NAMES = {2: "merge", 3: "vec1", 4: "vec2", 5: "vec4", 6: "vec8", 7: "vec16",
+             8: "vec32", 9: "vec64", 1: "ordered"}
+TUNABLE_ALGS = (2, 3, 4, 5, 6, 7, 8, 9)
+ACC_PLAIN, ACC_TWOSUM = 0, 1
+INJECT_ALLOCATION, INJECT_EVENT = 0, 1
+
+ELEM_BYTES = {F32: 4, F64: 8, C64: 8, C128: 16, BF16: 2, F16: 2}
+NP_STORAGE = {F32: np.float32, F64: np.float64, BF16: np.uint16, F16: np.float16}
+
+
+class Status(C.Structure):
+    _fields_ = [("code", C.c_int32), ("provider", C.c_int32), ("native", C.c_int64)]
+
+
+class SparseDesc(C.Structure):
+    _fields_ = [("outer_offsets", C.c_void_p), ("inner_indices", C.c_void_p),
+                ("values", C.c_void_p), ("rows", C.c_int64), ("cols", C.c_int64),
+                ("nnz", C.c_int64), ("format", C.c_uint8), ("element_kind", C.c_uint8),
+                ("index_bytes", C.c_uint8), ("offset_bytes", C.c_uint8),
+                ("memory_space", C.c_uint8), ("trusted", C.c_uint8),
+                ("reserved", C.c_uint8 * 2)]
+
+
+class VectorDesc(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("extent", C.c_int64), ("stride", C.c_int64),
+                ("element_kind", C.c_uint8), ("memory_space", C.c_uint8),
+                ("writable", C.c_uint8), ("reserved", C.c_uint8 * 5)]
+
+
+class MatrixDesc(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("rows", C.c_int64), ("cols", C.c_int64),
+                ("leading_dimension", C.c_int64), ("layout", C.c_uint8),
+                ("element_kind", C.c_uint8), ("memory_space", C.c_uint8),
+                ("writable", C.c_uint8), ("reserved", C.c_uint8 * 4)]
+
+
+class Scalar(C.Structure):
+    _fields_ = [("real", C.c_double), ("imaginary", C.c_double)]
+
+
+class SpmvOptions(C.Structure):
+    _fields_ = [("algorithm", C.c_uint8), ("accumulate", C.c_uint8),
+                ("reserved", C.c_uint8 * 6)]
+
+
+class AscHipError(RuntimeError):
+    def __init__(self, code: int, message: str, native: int = 0):
+        super().__init__(f"asc_sparse_hip error {code}: {message}")
+        self.code = code
+        self.native = native
+        self.message = message
+
+
+P = C.POINTER
+_VP = C.c_void_p
+_SIGS = {
+    "asc_sparse_hip_context_create": [C.c_int32, _VP, C.c_uint8, P(_VP)],
+    "asc_sparse_hip_context_destroy": [_VP],
+    "asc_sparse_hip_context_stream": [_VP, P(_VP)],
+    "asc_sparse_hip_context_synchronize": [_VP],
+    "asc_sparse_hip_allocate": [_VP, C.c_size_t, C.c_uint8, P(_VP)],
+    "asc_sparse_hip_deallocate": [_VP, _VP, C.c_uint8],
+    "asc_sparse_hip_copy": [_VP, _VP, _VP, C.c_size_t],
+    "asc_sparse_hip_fill": [_VP, _VP, C.c_uint8, C.c_size_t],
+    "asc_sparse_hip_event_record": [_VP, P(_VP)],
+    "asc_sparse_hip_event_wait": [_VP],
+    "asc_sparse_hip_event_elapsed_ms": [_VP, _VP, P(C.c_float)],
+    "asc_sparse_hip_event_destroy": [_VP],
+    "asc_sparse_hip_clone_compressed": [_VP, P(SparseDesc), C.c_uint8, _VP, _VP, _VP, P(SparseDesc)],
+    "asc_sparse_hip_clone_coordinate": [_VP, P(SparseDesc), _VP, _VP, P(SparseDesc)],
+    "asc_sparse_hip_narrow_indices": [_VP, P(SparseDesc), _VP, P(SparseDesc)],
+    "asc_sparse_hip_convert_values": [_VP, P(SparseDesc), C.c_uint8, _VP, P(SparseDesc)],
+    "asc_sparse_hip_csr_spmv_workspace_size": [_VP, P(SparseDesc), P(VectorDesc), P(VectorDesc), P(SpmvOptions), P(C.c_size_t)],
+    "asc_sparse_hip_csr_spmv": [_VP, C.c_double, P(SparseDesc), P(VectorDesc), C.c_double, P(VectorDesc), _VP, C.c_size_t, P(SpmvOptions)],
+    "asc_sparse_hip_standard_spmv_workspace_size": [_VP, C.c_uint8, P(SparseDesc), P(VectorDesc), P(VectorDesc), P(SpmvOptions), P(C.c_size_t)],
+    "asc_sparse_hip_standard_spmv": [_VP, C.c_uint8, Scalar, P(SparseDesc), P(VectorDesc), P(VectorDesc), _VP, C.c_size_t, P(SpmvOptions)],
+    "asc_sparse_hip_standard_spmm_workspace_size": [_VP, C.c_uint8, P(SparseDesc), P(MatrixDesc), P(MatrixDesc), P(C.c_size_t)],
+    "asc_sparse_hip_standard_spmm": [_VP, C.c_uint8, Scalar, P(SparseDesc), P(MatrixDesc), P(MatrixDesc), _VP, C.c_size_t],
+    "asc_sparse_hip_convert_coordinate_workspace_size": [_VP, P(SparseDesc), C.c_uint8, C.c_uint8, P(C.c_size_t)],
+    "asc_sparse_hip_convert_coordinate": [_VP, P(SparseDesc), C.c_uint8, C.c_uint8, _VP, _VP, _VP, P(SparseDesc), _VP, C.c_size_t],
+    "asc_sparse_hip_convert_compressed_workspace_size": [_VP, P(SparseDesc), P(C.c_size_t)],
+    "asc_sparse_hip_convert_compressed": [_VP, P(SparseDesc), _VP, _VP, _VP, P(SparseDesc), _VP, C.c_size_t],
+    "asc_sparse_hip_test_inject": [C.c_uint32, C.c_int32],
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Loads the HIP library (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+    lib = C.CDLL(LIB_PATH)
+    for name, args in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = Status
+    lib.asc_sparse_hip_last_message.argtypes = []
+    lib.asc_sparse_hip_last_message.restype = C.c_char_p
+    lib.asc_sparse_hip_device_count.argtypes = []
+    lib.asc_sparse_hip_device_count.restype = C.c_int32
+    _lib = lib
+    return lib
+
+
+def exported_symbols() -> list[str]:
+    return list(_SIGS) + ["asc_sparse_hip_last_message", "asc_sparse_hip_device_count"]
+
+
+def check(st: Status) -> None:
+    if st.code != OK:
+        msg = load().asc_sparse_hip_last_message().decode()
+        raise AscHipError(st.code, msg, st.native)
+
+
+def device_count() -> int:
+    return int(load().asc_sparse_hip_device_count())
+
+
+class Context:
+    """An asc_hip_context on one device (device=-1: validation only)."""
+
+    def __init__(self, device: int = 0, stream: int | None = None,
+                 determinism: int = DETERMINISTIC):
+        self.lib = load()
+        h = C.c_void_p()
+        check(self.lib.asc_sparse_hip_context_create(device, stream, determinism, C.byref(h)))
+        self.handle = h
+        self.device = device
+
+    def close(self):
+        if self.handle:
+            check(self.lib.asc_sparse_hip_context_destroy(self.handle))
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def synchronize(self):
+        check(self.lib.asc_sparse_hip_context_synchronize(self.handle))
+
+    @property
+    def stream(self) -> int:
+        s = C.c_void_p()
+        check(self.lib.asc_sparse_hip_context_stream(self.handle, C.byref(s)))
+        return s.value or 0
+
+    # ---- memory --------------------------------------------------------------
+    def alloc(self, nbytes: int, space: int = DEVICE) -> "Buffer":
+        p = C.c_void_p()
+        check(self.lib.asc_sparse_hip_allocate(self.handle, nbytes, space, C.byref(p)))
+        return Buffer(self, p.value or 0, nbytes, space)
+
+    def upload(self, arr: np.ndarray) -> "Buffer":
+        arr = np.ascontiguousarray(arr)
+        buf = self.alloc(max(arr.nbytes, 1))
+        if arr.nbytes:
+            check(self.lib.asc_sparse_hip_copy(self.handle, buf.ptr, arr.ctypes.data, arr.nbytes))
+            self.synchronize()
+        buf.dtype, buf.shape = arr.dtype, arr.shape
+        return buf
+
+    def download(self, buf: "Buffer", dtype, count: int, offset_bytes: int = 0) -> np.ndarray:
+        out = np.empty(count, dtype=dtype)
+        if out.nbytes:
+            check(self.lib.asc_sparse_hip_copy(self.handle, out.ctypes.data,
+                                               buf.ptr + offset_bytes, out.nbytes))
+            self.synchronize()
+        return out
+
+    def copy(self, dst: int, src: int, nbytes: int):
+        check(self.lib.asc_sparse_hip_copy(self.handle, dst, src, nbytes))
+
+    def fill(self, buf: "Buffer", byte: int = 0, nbytes: int | None = None):
+        check(self.lib.asc_sparse_hip_fill(self.handle, buf.ptr, byte,
+                                           buf.nbytes if nbytes is None else nbytes))
+
+    # ---- events --------------------------------------------------------------
+    def record(self) -> "Event":
+        e = C.c_void_p()
+        check(self.lib.asc_sparse_hip_event_record(self.handle, C.byref(e)))
+        return Event(self.lib, e)
+
+
+class Event:
+    def __init__(self, lib, handle):
+        self.lib, self.handle = lib, handle
+
+    def wait(self):
+        check(self.lib.asc_sparse_hip_event_wait(self.handle))
+
+    def elapsed_ms(self, later: "Event") -> float:
+        v = C.c_float()
+        check(self.lib.asc_sparse_hip_event_elapsed_ms(self.handle, later.handle, C.byref(v)))
+        return float(v.value)
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            self.lib.asc_sparse_hip_event_destroy(self.handle)
+            self.handle = None
+
+
+class Buffer:
+    def __init__(self, ctx: Context, ptr: int, nbytes: int, space: int):
+        self.ctx, self.ptr, self.nbytes, self.space = ctx, ptr, nbytes, space
+        self.dtype, self.shape = None, None
+
+    def free(self):
+        if self.ptr and self.ctx.handle:
+            check(self.ctx.lib.asc_sparse_hip_deallocate(self.ctx.handle, self.ptr, self.space))
+        self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def numpy(self, dtype=None, count=None) -> np.ndarray:
+        dtype = dtype or self.dtype
+        count = count if count is not None else self.nbytes // np.dtype(dtype).itemsize
+        return self.ctx.download(self, dtype, count)
+
+
+# ---- descriptor helpers ----------------------------------------------------------
+
+def vector(buf_or_ptr, extent: int, kind: int, stride: int = 1, writable: bool = True,
+           space: int = DEVICE) -> VectorDesc:
+    ptr = buf_or_ptr.ptr if isinstance(buf_or_ptr, Buffer) else int(buf_or_ptr or 0)
+    return VectorDesc(ptr, extent, stride, kind, space, 1 if writable else 0)
+
+
+def dense(buf_or_ptr, rows: int, cols: int, ld: int, layout: int, kind: int,
+          writable: bool = True, space: int = DEVICE) -> MatrixDesc:
+    ptr = buf_or_ptr.ptr if isinstance(buf_or_ptr, Buffer) else int(buf_or_ptr or 0)
+    return MatrixDesc(ptr, rows, cols, ld, layout, kind, space, 1 if writable else 0)
+
+
+def host_sparse(fmt: int, rows: int, cols: int, offsets: np.ndarray, indices: np.ndarray | None,
+                values: np.ndarray, kind: int) -> SparseDesc:
+    """Descriptor over host numpy arrays (kept alive by the caller)."""
+    nnz = int(values.shape[0])
+    ib = indices.dtype.itemsize if indices is not None else 8
+    return SparseDesc(offsets.ctypes.data, indices.ctypes.data if indices is not None else None,
+                      values.ctypes.data, rows, cols, nnz, fmt, kind, ib, 8, HOST, 0)
+
+
+@dataclass
+class DeviceSparse:
+    """Device CSR/CSC/COO arrays owned by Buffers plus their descriptor."""
+    desc: SparseDesc
+    buffers: list
+
+    @property
+    def rows(self):
+        return self.desc.rows
+
+    @property
+    def cols(self):
+        return self.desc.cols
+
+    @property
+    def nnz(self):
+        return self.desc.nnz
+
+
+def clone_compressed(ctx: Context, fmt: int, rows: int, cols: int, offsets: np.ndarray,
+                     indices: np.ndarray, values: np.ndarray, kind: int,
+                     index_bytes: int = 8) -> DeviceSparse:
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    indices = np.ascontiguousarray(indices)
+    values = np.ascontiguousarray(values)
+    src = host_sparse(fmt, rows, cols, offsets, indices, values, kind)
+    outer = rows if fmt == CSR else cols
+    nnz = values.shape[0]
+    bo = ctx.alloc(8 * (outer + 1))
+    bi = ctx.alloc(max(index_bytes * nnz, 1))
+    bv = ctx.alloc(max(ELEM_BYTES[kind] * nnz, 1))
+    out = SparseDesc()
+    check(ctx.lib.asc_sparse_hip_clone_compressed(ctx.handle, C.byref(src), index_bytes,
+                                                  bo.ptr, bi.ptr, bv.ptr, C.byref(out)))
+    return DeviceSparse(out, [bo, bi, bv])
+
+
+def clone_coordinate(ctx: Context, rows: int, cols: int, coords: np.ndarray,
+                     values: np.ndarray, kind: int) -> DeviceSparse:
+    coords = np.ascontiguousarray(coords, dtype=np.int64).reshape(-1)
+    values = np.ascontiguousarray(values)
+    nnz = values.shape[0]
+    src = SparseDesc(coords.ctypes.data, None, values.ctypes.data, rows, cols, nnz, COO, kind,
+                     8, 8, HOST, 0)
+    bc = ctx.alloc(max(16 * nnz, 1))
+    bv = ctx.alloc(max(ELEM_BYTES[kind] * nnz, 1))
+    out = SparseDesc()
+    check(ctx.lib.asc_sparse_hip_clone_coordinate(ctx.handle, C.byref(src), bc.ptr, bv.ptr,
+                                                  C.byref(out)))
+    return DeviceSparse(out, [bc, bv])
+
+
+def narrow_indices(ctx: Context, a: DeviceSparse) -> DeviceSparse:
+    b = ctx.alloc(max(4 * a.nnz, 1))
+    out = SparseDesc()
+    check(ctx.lib.asc_sparse_hip_narrow_indices(ctx.handle, C.byref(a.desc), b.ptr, C.byref(out)))
+    return DeviceSparse(out, a.buffers + [b])
+
+
+def convert_values(ctx: Context, a: DeviceSparse, kind: int) -> DeviceSparse:
+    b = ctx.alloc(max(ELEM_BYTES[kind] * a.nnz, 1))
+    out = SparseDesc()
+    check(ctx.lib.asc_sparse_hip_convert_values(ctx.handle, C.byref(a.desc), kind, b.ptr,
+                                                C.byref(out)))
+    return DeviceSparse(out, a.buffers + [b])
+
+
+def download_compressed(ctx: Context, a: DeviceSparse):
+    d = a.desc
+    outer = d.rows if d.format == CSR else d.cols
+    off = ctx.download(_ptr_buf(ctx, d.outer_offsets), np.int64, outer + 1)
+    idx = ctx.download(_ptr_buf(ctx, d.inner_indices), np.int32 if d.index_bytes == 4 else np.int64, d.nnz)
+    val = ctx.download(_ptr_buf(ctx, d.values), NP_STORAGE[d.element_kind], d.nnz)
+    return off, idx, val
+
+
+class _PtrView:
+    def __init__(self, ptr):
+        self.ptr = ptr
+
+
+def _ptr_buf(ctx, ptr):
+    return _PtrView(ptr or 0)
+
+
+def options(alg: int = ALG_AUTO, acc: int = ACC_PLAIN) -> SpmvOptions:
+    return SpmvOptions(alg, acc)
+
+
+class Workspace:
+    """Grows-only device workspace reused across calls."""
+
+    def __init__(self, ctx: Context):
+        self.ctx, self.buf = ctx, None
+
+    def get(self, nbytes: int):
+        if nbytes == 0:
+            return None, 0
+        if self.buf is None or self.buf.nbytes < nbytes:
+            if self.buf is not None:
+                self.buf.free()
+            self.buf = self.ctx.alloc(nbytes)
+        return self.buf.ptr, self.buf.nbytes
+
+
+def csr_spmv(ctx: Context, alpha: float, a: DeviceSparse, x: VectorDesc, beta: float,
+             y: VectorDesc, alg: int = ALG_AUTO, acc: int = ACC_PLAIN,
+             ws: Workspace | None = None):
+    opt = options(alg, acc)
+    need = C.c_size_t()
+    check(ctx.lib.asc_sparse_hip_csr_spmv_workspace_size(ctx.handle, C.byref(a.desc), C.byref(x),
+                                                         C.byref(y), C.byref(opt), C.byref(need)))
+    ws = ws or Workspace(ctx)
+    p, n = ws.get(need.value)
+    check(ctx.lib.asc_sparse_hip_csr_spmv(ctx.handle, alpha, C.byref(a.desc), C.byref(x), beta,
+                                          C.byref(y), p, n, C.byref(opt)))
+    return ws
+
+
+def standard_spmv(ctx: Context, transpose: int, alpha: float, a: DeviceSparse, x: VectorDesc,
+                  y: VectorDesc, alg: int = ALG_AUTO, ws: Workspace | None = None):
+    opt = options(alg)
+    need = C.c_size_t()
+    check(ctx.lib.asc_sparse_hip_standard_spmv_workspace_size(
+        ctx.handle, transpose, C.byref(a.desc), C.byref(x), C.byref(y), C.byref(opt),
+        C.byref(need)))
+    ws = ws or Workspace(ctx)
+    p, n = ws.get(need.value)
+    check(ctx.lib.asc_sparse_hip_standard_spmv(ctx.handle, transpose, Scalar(alpha, 0.0),
+                                               C.byref(a.desc), C.byref(x), C.byref(y), p, n,
+                                               C.byref(opt)))
+    return ws
+
+
+def standard_spmm(ctx: Context, transpose: int, alpha: float, a: DeviceSparse, b: MatrixDesc,
+                  c: MatrixDesc, ws: Workspace | None = None):
+    need = C.c_size_t()
+    check(ctx.lib.asc_sparse_hip_standard_spmm_workspace_size(
+        ctx.handle, transpose, C.byref(a.desc), C.byref(b), C.byref(c), C.byref(need)))
+    ws = ws or Workspace(ctx)
+    p, n = ws.get(need.value)
+    check(ctx.lib.asc_sparse_hip_standard_spmm(ctx.handle, transpose, Scalar(alpha, 0.0),
+                                               C.byref(a.desc), C.byref(b), C.byref(c), p, n))
+    return ws
+
+
+def convert_coordinate(ctx: Context, coo: DeviceSparse, out_format: int, index_bytes: int = 8,
+                       ws: Workspace | None = None) -> DeviceSparse:
+    d = coo.desc
+    need = C.c_size_t()
+    check(ctx.lib.asc_sparse_hip_convert_coordinate_workspace_size(
+        ctx.handle, C.byref(d), out_format, index_bytes, C.byref(need)))
+    outer = d.rows if out_format == CSR else d.cols
+    bo = ctx.alloc(8 * (outer + 1))
+    bi = ctx.alloc(max(index_bytes * d.nnz, 1))
+    bv = ctx.alloc(max(ELEM_BYTES[d.element_kind] * d.nnz, 1))
+    ws = ws or Workspace(ctx)
+    p, n = ws.get(need.value)
+    out = SparseDesc()
+    check(ctx.lib.asc_sparse_hip_convert_coordinate(ctx.handle, C.byref(d), out_format,
+                                                    index_bytes, bo.ptr, bi.ptr, bv.ptr,
+                                                    C.byref(out), p, n))
+    return DeviceSparse(out, [bo, bi, bv])
+
+
+def convert_compressed(ctx: Context, a: DeviceSparse, ws: Workspace | None = None) -> DeviceSparse:
+    d = a.desc
+    need = C.c_size_t()
+    check(ctx.lib.asc_sparse_hip_convert_compressed_workspace_size(ctx.handle, C.byref(d),
+                                                                   C.byref(need)))
+    outer = d.cols if d.format == CSR else d.rows
+    bo = ctx.alloc(8 * (outer + 1))
+    bi = ctx.alloc(max(d.index_bytes * d.nnz, 1))
+    bv = ctx.alloc(max(ELEM_BYTES[d.element_kind] * d.nnz, 1))
+    ws = ws or Workspace(ctx)
+    p, n = ws.get(need.value)
+    out = SparseDesc()
+    check(ctx.lib.asc_sparse_hip_convert_compressed(ctx.handle, C.byref(d), bo.ptr, bi.ptr,
+                                                    bv.ptr, C.byref(out), p, n))
+    return DeviceSparse(out, [bo, bi, bv])
+
+
+def test_inject(what: int, countdown: int):
+    check(load().asc_sparse_hip_test_inject(what, countdown))

@@ -1,0 +1,84 @@
+// Shared internals of the C ABI translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "asc_sparse_hip.h"
+#include "../kernels/launch.h"
+
+constexpr uint64_t kContextMagic = 0x4153435f48495031ull;  // "ASC_HIP1"
+
+struct asc_hip_context {
+  uint64_t magic = kContextMagic;
+  int device = -1;  // -1: validation-only context
+  hipStream_t stream = nullptr;
+  bool owns_stream = false;
+  uint8_t determinism = ASC_HIP_DETERMINISTIC;
+  std::mutex plan_mutex;  // guards the tuned-plan cache below
+  struct Plan {
+    const void* offsets;
+    const void* indices;
+    int64_t rows, cols, nnz;
+    uint8_t element_kind;
+    uint8_t algorithm;
+  };
+  std::vector<Plan> plans;
+};
+
+namespace asc_hip_abi {
+
+extern thread_local std::string g_message;
+
+asc_hip_status ok();
+asc_hip_status fail(int32_t code, const std::string& message);
+asc_hip_status fail_hip(hipError_t e, const std::string& what);
+
+struct DeviceGuard {
+  int previous = -1;
+  hipError_t error = hipSuccess;
+  explicit DeviceGuard(int device);
+  ~DeviceGuard();
+};
+
+struct Span {
+  uintptr_t begin = 0;
+  uintptr_t end = 0;
+};
+bool overlaps(Span a, Span b);
+Span span_of(const void* p, size_t bytes);
+Span vector_span(const asc_hip_vector_desc& v);
+Span matrix_span(const asc_hip_matrix_desc& m);
+struct SparseSpans {
+  Span first, second, values;
+};
+SparseSpans sparse_spans(const asc_hip_sparse_desc& d);
+bool sparse_overlaps(const asc_hip_sparse_desc& d, Span out);
+bool checked_mul(int64_t a, int64_t b, int64_t* out);
+size_t elem_size(uint8_t kind);
+
+asc_hip_status check_context(const asc_hip_context* ctx);
+asc_hip_status check_device_pointer(const asc_hip_context* ctx, const void* p,
+                                    const char* what);
+asc_hip_status check_sparse_meta(const asc_hip_sparse_desc* d, const char* what);
+asc_hip_status check_device_sparse(const asc_hip_context* ctx,
+                                   const asc_hip_sparse_desc* d, const char* what);
+asc_hip_status check_vector(const asc_hip_context* ctx, const asc_hip_vector_desc* v,
+                            uint8_t kind, int64_t extent, bool writable,
+                            const char* what);
+asc_hip_status check_workspace(const asc_hip_context* ctx, void* ws, size_t ws_bytes,
+                               size_t needed, const std::vector<Span>& operands);
+asc_hip_status unavailable_if_validation_only(const asc_hip_context* ctx);
+asc_hip_status finish_launch(asc_hip_context* ctx, hipError_t e, const char* what);
+asc_hip::Compressed as_compressed(const asc_hip_sparse_desc& d, bool row_view_of_csc);
+
+// Host-forced fault injection (test hooks, see asc_sparse_hip_test_inject).
+bool test_hook_consume_allocation_failure();
+bool test_hook_consume_event_failure();
+
+}  // namespace asc_hip_abi

@@ -1,0 +1,707 @@
+// C ABI operations: staging, SpMV, SpMM and conversions.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "abi_internal.h"
+#include "asc_sparse_hip.h"
+
+using asc_hip::Elem;
+using namespace asc_hip_abi;
+
+namespace {
+
+constexpr int64_t kU32Max = std::numeric_limits<uint32_t>::max();
+
+inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+int64_t outer_of(const asc_hip_sparse_desc& d) {
+  return d.format == ASC_HIP_CSC ? d.cols : d.rows;
+}
+int64_t inner_of(const asc_hip_sparse_desc& d) {
+  return d.format == ASC_HIP_CSC ? d.rows : d.cols;
+}
+
+template <typename I>
+asc_hip_status validate_host_compressed_typed(const asc_hip_sparse_desc& d) {
+  const int64_t outer = outer_of(d), inner = inner_of(d);
+  const int64_t* off = static_cast<const int64_t*>(d.outer_offsets);
+  const I* idx = static_cast<const I*>(d.inner_indices);
+  // compressed.h:711-746
+  if (off[0] != 0 || off[outer] != d.nnz)
+    return fail(ASC_HIP_INVALID_ARGUMENT, "offsets must start at 0 and end at nnz");
+  for (int64_t o = 0; o < outer; ++o) {
+    const int64_t b = off[o], e = off[o + 1];
+    if (e < b || e > d.nnz) return fail(ASC_HIP_INVALID_ARGUMENT, "offsets must be nondecreasing");
+    for (int64_t p = b; p < e; ++p) {
+      const int64_t v = static_cast<int64_t>(idx[p]);
+      if (v < 0 || v >= inner) return fail(ASC_HIP_INDEX, "inner index out of range");
+      if (p > b && v <= static_cast<int64_t>(idx[p - 1]))
+        return fail(ASC_HIP_INVALID_ARGUMENT, "inner indices must increase strictly within a segment");
+    }
+  }
+  return ok();
+}
+
+asc_hip_status validate_host_compressed(const asc_hip_sparse_desc& d) {
+  if (d.index_bytes == 4) return validate_host_compressed_typed<int32_t>(d);
+  return validate_host_compressed_typed<int64_t>(d);
+}
+
+bool host_space(uint8_t s) { return s == ASC_HIP_HOST || s == ASC_HIP_PINNED_HOST; }
+
+// ---- SpMV preparation -----------------------------------------------------
+struct SpmvPlan {
+  asc_hip::SpmvArgs args{};
+  bool transposed = false;  // op(A) needs a device transpose first
+  asc_hip::Compressed source{};
+  int64_t t_outer = 0;      // transposed operand's outer extent
+  size_t t_offsets = 0, t_indices = 0, t_values = 0, t_ws = 0, spmv_ws = 0;
+  size_t total = 0;
+  bool noop = false;
+};
+
+uint8_t resolve_alg(asc_hip_context* ctx, const asc_hip_sparse_desc& a,
+                    const asc_hip_spmv_options* opt) {
+  const uint8_t alg = opt ? opt->algorithm : ASC_HIP_ALG_AUTO;
+  if (alg != ASC_HIP_ALG_AUTO) return alg;
+  std::lock_guard<std::mutex> lock(ctx->plan_mutex);
+  for (const auto& p : ctx->plans) {
+    if (p.offsets == a.outer_offsets && p.indices == a.inner_indices &&
+        p.rows == a.rows && p.cols == a.cols && p.nnz == a.nnz &&
+        p.element_kind == a.element_kind)
+      return p.algorithm;
+  }
+  return ASC_HIP_ALG_MERGE;
+}
+
+asc_hip_status check_options(const asc_hip_sparse_desc& a, const asc_hip_spmv_options* opt,
+                             uint8_t alg) {
+  if (alg == ASC_HIP_ALG_AUTO || alg >= ASC_HIP_ALG_COUNT)
+    return fail(ASC_HIP_INVALID_ARGUMENT, "unknown SpMV algorithm");
+  const uint8_t acc = opt ? opt->accumulate : ASC_HIP_ACC_PLAIN;
+  if (acc > ASC_HIP_ACC_TWOSUM) return fail(ASC_HIP_INVALID_ARGUMENT, "unknown accumulation mode");
+  if (acc == ASC_HIP_ACC_TWOSUM && a.element_kind == ASC_HIP_F64)
+    return fail(ASC_HIP_UNSUPPORTED, "compensated accumulation applies to FP32-accumulated storage");
+  if (alg == ASC_HIP_ALG_ORDERED &&
+      (acc != ASC_HIP_ACC_PLAIN ||
+       (a.element_kind != ASC_HIP_F32 && a.element_kind != ASC_HIP_F64)))
+    return fail(ASC_HIP_UNSUPPORTED, "ordered summation is defined for FP32/FP64 storage");
+  return ok();
+}
+
+asc_hip_status prepare_compat_spmv(asc_hip_context* ctx, const asc_hip_sparse_desc* a,
+                                   const asc_hip_vector_desc* x,
+                                   const asc_hip_vector_desc* y,
+                                   const asc_hip_spmv_options* opt, SpmvPlan* plan) {
+  asc_hip_status st = check_context(ctx);
+  if (st.code) return st;
+  if ((st = check_device_sparse(ctx, a, "matrix")).code) return st;
+  if (a->format != ASC_HIP_CSR) return fail(ASC_HIP_INVALID_ARGUMENT, "CSR SpMV requires CSR format");
+  if (a->element_kind == ASC_HIP_C64 || a->element_kind == ASC_HIP_C128)
+    return fail(ASC_HIP_UNSUPPORTED, "compat SpMV is defined for real storage");
+  const uint8_t vk = a->element_kind == ASC_HIP_F64 ? ASC_HIP_F64 : ASC_HIP_F32;
+  if ((st = check_vector(ctx, x, vk, a->cols, false, "input")).code) return st;
+  if ((st = check_vector(ctx, y, vk, a->rows, true, "output")).code) return st;
+  const Span ys = vector_span(*y);
+  if (overlaps(vector_span(*x), ys) || sparse_overlaps(*a, ys))
+    return fail(ASC_HIP_INVALID_ARGUMENT, "SpMV rejects output operand overlap");
+  const uint8_t alg = resolve_alg(ctx, *a, opt);
+  if ((st = check_options(*a, opt, alg)).code) return st;
+  asc_hip::SpmvArgs& g = plan->args;
+  g.a = as_compressed(*a, false);
+  g.x = x->data;
+  g.incx = x->stride;
+  g.y = const_cast<void*>(y->data);
+  g.incy = y->stride;
+  g.form = asc_hip::SpmvForm::Compat;
+  g.alg = static_cast<asc_hip::SpmvAlg>(alg);
+  g.twosum = opt && opt->accumulate == ASC_HIP_ACC_TWOSUM;
+  plan->spmv_ws = asc_hip::spmv_workspace_bytes(g);
+  plan->total = plan->spmv_ws;
+  plan->noop = a->rows == 0;
+  return ok();
+}
+
+// Standardized SpMV/SpMM: the kernels consume op(A) as a row view.  CSR/N
+// and CSC/T,H are row views already; CSR/T,H and CSC/N are transposed into
+// the workspace first.
+asc_hip_status plan_row_view(const asc_hip_sparse_desc& a, uint8_t transpose,
+                             SpmvPlan* plan) {
+  const bool csr = a.format == ASC_HIP_CSR;
+  const bool direct = (csr && transpose == ASC_HIP_OP_N) || (!csr && transpose != ASC_HIP_OP_N);
+  plan->source = as_compressed(a, false);
+  if (direct) {
+    plan->args.a = plan->source;
+    return ok();
+  }
+  const int64_t n = a.nnz, outer = outer_of(a), inner = inner_of(a);
+  if (n > kU32Max || outer > kU32Max || inner > kU32Max)
+    return fail(ASC_HIP_UNSUPPORTED, "device transpose supports extents and nnz below 2^32");
+  plan->transposed = true;
+  plan->t_outer = inner;
+  const size_t esz = elem_size(a.element_kind);
+  plan->t_offsets = align256(sizeof(int64_t) * (inner + 1));
+  plan->t_indices = align256(static_cast<size_t>(a.index_bytes) * n);
+  plan->t_values = align256(esz * n);
+  plan->t_ws = align256(asc_hip::transpose_workspace_bytes(plan->source));
+  asc_hip::Compressed rv;
+  rv.offsets = nullptr;
+  rv.indices = nullptr;
+  rv.values = nullptr;
+  rv.outer = inner;
+  rv.inner = outer;
+  rv.nnz = n;
+  rv.index_bytes = a.index_bytes;
+  rv.elem = plan->source.elem;
+  plan->args.a = rv;
+  return ok();
+}
+
+// Points the row view at the transposed arrays inside the workspace and
+// enqueues the transpose.
+hipError_t run_transpose(SpmvPlan* plan, char* ws, hipStream_t s) {
+  int64_t* off = reinterpret_cast<int64_t*>(ws);
+  void* idx = ws + plan->t_offsets;
+  void* val = ws + plan->t_offsets + plan->t_indices;
+  void* tws = ws + plan->t_offsets + plan->t_indices + plan->t_values;
+  plan->args.a.offsets = off;
+  plan->args.a.indices = idx;
+  plan->args.a.values = val;
+  return asc_hip::transpose_launch(plan->source, off, idx, plan->source.index_bytes,
+                                   val, tws, s);
+}
+
+size_t transpose_prefix(const SpmvPlan& p) {
+  return p.transposed ? p.t_offsets + p.t_indices + p.t_values + p.t_ws : 0;
+}
+
+asc_hip_status prepare_standard_spmv(asc_hip_context* ctx, uint8_t transpose,
+                                     const asc_hip_sparse_desc* a,
+                                     const asc_hip_vector_desc* x,
+                                     const asc_hip_vector_desc* y,
+                                     const asc_hip_spmv_options* opt, SpmvPlan* plan) {
+  asc_hip_status st = check_context(ctx);
+  if (st.code) return st;
+  if (transpose > ASC_HIP_OP_H) return fail(ASC_HIP_INVALID_ARGUMENT, "unknown transpose");
+  if ((st = check_device_sparse(ctx, a, "matrix")).code) return st;
+  if (a->format == ASC_HIP_COO) return fail(ASC_HIP_INVALID_ARGUMENT, "standard SpMV requires CSR or CSC");
+  if (a->element_kind != ASC_HIP_F32 && a->element_kind != ASC_HIP_F64)
+    return fail(ASC_HIP_UNSUPPORTED, "standard SpMV supports real FP32/FP64 storage");
+  const int64_t in_ext = transpose == ASC_HIP_OP_N ? a->cols : a->rows;
+  const int64_t out_ext = transpose == ASC_HIP_OP_N ? a->rows : a->cols;
+  if ((st = check_vector(ctx, x, a->element_kind, in_ext, false, "input")).code) return st;
+  if ((st = check_vector(ctx, y, a->element_kind, out_ext, true, "output")).code) return st;
+  const Span ys = vector_span(*y);
+  if (overlaps(vector_span(*x), ys) || sparse_overlaps(*a, ys))
+    return fail(ASC_HIP_INVALID_ARGUMENT, "Sparse matrix-vector input and output overlap");
+  const uint8_t alg = resolve_alg(ctx, *a, opt);
+  if ((st = check_options(*a, opt, alg)).code) return st;
+  if ((st = plan_row_view(*a, transpose, plan)).code) return st;
+  asc_hip::SpmvArgs& g = plan->args;
+  g.x = x->data;
+  g.incx = x->stride;
+  g.y = const_cast<void*>(y->data);
+  g.incy = y->stride;
+  g.alpha = 0.0;
+  g.beta = 0.0;
+  g.form = asc_hip::SpmvForm::Standard;
+  g.alg = static_cast<asc_hip::SpmvAlg>(alg);
+  g.twosum = false;
+  plan->spmv_ws = align256(asc_hip::spmv_workspace_bytes(g));
+  plan->total = transpose_prefix(*plan) + plan->spmv_ws;
+  plan->noop = out_ext == 0;
+  return ok();
+}
+
+std::vector<Span> operand_spans(const asc_hip_sparse_desc& a, std::initializer_list<Span> more) {
+  const SparseSpans s = sparse_spans(a);
+  std::vector<Span> v{s.first, s.second, s.values};
+  v.insert(v.end(), more.begin(), more.end());
+  return v;
+}
+
+asc_hip_status run_spmv(asc_hip_context* ctx, SpmvPlan& plan, void* ws) {
+  DeviceGuard guard(ctx->device);
+  if (guard.error != hipSuccess) return fail_hip(guard.error, "hipSetDevice");
+  char* w = static_cast<char*>(ws);
+  if (plan.transposed) {
+    const hipError_t e = run_transpose(&plan, w, ctx->stream);
+    if (e != hipSuccess) return finish_launch(ctx, e, "transpose");
+  }
+  const hipError_t e =
+      asc_hip::spmv_launch(plan.args, w + transpose_prefix(plan), ctx->stream);
+  return finish_launch(ctx, e, "SpMV launch");
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---- staging ---------------------------------------------------------------
+
+asc_hip_status asc_sparse_hip_clone_compressed(asc_hip_context* ctx,
+                                               const asc_hip_sparse_desc* src,
+                                               uint8_t index_bytes, void* d_off,
+                                               void* d_idx, void* d_val,
+                                               asc_hip_sparse_desc* out) {
+  asc_hip_status st = check_context(ctx);
+  if (st.code) return st;
+  if (out == nullptr) return fail(ASC_HIP_INVALID_ARGUMENT, "output descriptor is null");
+  if ((st = check_sparse_meta(src, "source")).code) return st;
+  if (src->format == ASC_HIP_COO) return fail(ASC_HIP_INVALID_ARGUMENT, "use asc_sparse_hip_clone_coordinate for COO");
+  if (!host_space(src->memory_space)) return fail(ASC_HIP_MEMORY_ACCESS, "clone source must be host memory");
+  if (index_bytes != 4 && index_bytes != 8) return fail(ASC_HIP_INVALID_ARGUMENT, "index width must be 4 or 8 bytes");
+  const int64_t outer = outer_of(*src), inner = inner_of(*src), n = src->nnz;
+  if (index_bytes == 4 && inner > std::numeric_limits<int32_t>::max())
+    return fail(ASC_HIP_OVERFLOW, "narrowing clone: inner extent exceeds INT32_MAX");
+  if ((st = validate_host_compressed(*src)).code) return st;
+  const size_t esz = elem_size(src->element_kind);
+  const size_t off_b = sizeof(int64_t) * (outer + 1), idx_b = index_bytes * n, val_b = esz * n;
+  if ((st = check_device_pointer(ctx, d_off, "offsets destination")).code) return st;
+  if (n > 0) {
+    if ((st = check_device_pointer(ctx, d_idx, "indices destination")).code) return st;
+    if ((st = check_device_pointer(ctx, d_val, "values destination")).code) return st;
+  }
+  if (overlaps(span_of(d_off, off_b), span_of(d_idx, idx_b)) ||
+      overlaps(span_of(d_off, off_b), span_of(d_val, val_b)) ||
+      overlaps(span_of(d_idx, idx_b), span_of(d_val, val_b)))
+    return fail(ASC_HIP_INVALID_ARGUMENT, "clone destinations overlap");
+  if ((st = unavailable_if_validation_only(ctx)).code) return st;
+  DeviceGuard guard(ctx->device);
+  hipError_t e = hipMemcpyAsync(d_off, src->outer_offsets, off_b, hipMemcpyHostToDevice, ctx->stream);
+  std::vector<int32_t> narrow32;
+  std::vector<int64_t> widen64;
+  if (e == hipSuccess && n > 0) {
+    const void* idx_src = src->inner_indices;
+    if (index_bytes != src->index_bytes) {
+      if (index_bytes == 4) {
+        narrow32.resize(n);
+        const int64_t* s64 = static_cast<const int64_t*>(src->inner_indices);
+        for (int64_t i = 0; i < n; ++i) narrow32[i] = static_cast<int32_t>(s64[i]);
+        idx_src = narrow32.data();
+      } else {
+        widen64.resize(n);
+        const int32_t* s32 = static_cast<const int32_t*>(src->inner_indices);
+        for (int64_t i = 0; i < n; ++i) widen64[i] = s32[i];
+        idx_src = widen64.data();
+      }
+    }
+    e = hipMemcpyAsync(d_idx, idx_src, idx_b, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(d_val, src->values, val_b, hipMemcpyHostToDevice, ctx->stream);
+  }
+  // Host staging buffers die with this call: complete the copies first.
+  const hipError_t sync = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess) e = sync;
+  if (e != hipSuccess) return finish_launch(ctx, e, "clone copy");
+  *out = *src;
+  out->outer_offsets = d_off;
+  out->inner_indices = d_idx;
+  out->values = d_val;
+  out->index_bytes = index_bytes;
+  out->offset_bytes = 8;
+  out->memory_space = ASC_HIP_DEVICE;
+  out->trusted = 1;
+  return ok();
+}
+
+asc_hip_status asc_sparse_hip_clone_coordinate(asc_hip_context* ctx,
+                                               const asc_hip_sparse_desc* src,
+                                               void* d_coords, void* d_val,
+                                               asc_hip_sparse_desc* out) {
+  asc_hip_status st = check_context(ctx);
+  if (st.code) return st;
+  if (out == nullptr) return fail(ASC_HIP_INVALID_ARGUMENT, "output descriptor is null");
+  if ((st = check_sparse_meta(src, "source")).code) return st;
+  if (src->format != ASC_HIP_COO) return fail(ASC_HIP_INVALID_ARGUMENT, "coordinate clone requires COO");
+  if (!host_space(src->memory_space)) return fail(ASC_HIP_MEMORY_ACCESS, "clone source must be host memory");
+  // coordinate.h:751-782: in bounds and strictly lexicographic.
+  const int64_t* c = static_cast<const int64_t*>(src->outer_offsets);
+  for (int64_t p = 0; p < src->nnz; ++p) {
+    const int64_t r = c[2 * p], col = c[2 * p + 1];
+    if (r < 0 || r >= src->rows || col < 0 || col >= src->cols)
+      return fail(ASC_HIP_INDEX, "coordinate out of range");
+    if (p > 0) {
+      const int64_t pr = c[2 * p - 2], pc = c[2 * p - 1];
+      if (r < pr || (r == pr && col <= pc))
+        return fail(ASC_HIP_INVALID_ARGUMENT, "coordinates must be canonical (sorted, unique)");
+    }
+  }
+  const size_t cb = 16 * static_cast<size_t>(src->nnz), vb = elem_size(src->element_kind) * src->nnz;
+  if (src->nnz > 0) {
+    if ((st = check_device_pointer(ctx, d_coords, "coordinates destination")).code) return st;
+    if ((st = check_device_pointer(ctx, d_val, "values destination")).code) return st;
+    if (overlaps(span_of(d_coords, cb), span_of(d_val, vb)))
+      return fail(ASC_HIP_INVALID_ARGUMENT, "clone destinations overlap");
+  }
+  if ((st = unavailable_if_validation_only(ctx)).code) return st;
+  DeviceGuard guard(ctx->device);
+  hipError_t e = hipSuccess;
+  if (src->nnz > 0) {
+    e = hipMemcpyAsync(d_coords, c, cb, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_val, src->values, vb, hipMemcpyHostToDevice, ctx->stream);
+  }
+  const hipError_t sync = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess) e = sync;
+  if (e != hipSuccess) return finish_launch(ctx, e, "clone copy");
+  *out = *src;
+  out->outer_offsets = d_coords;
+  out->inner_indices = nullptr;
+  out->values = d_val;
+  out->memory_space = ASC_HIP_DEVICE;
+  out->trusted = 1;
+  return ok();
+}
+
+asc_hip_status asc_sparse_hip_narrow_indices(asc_hip_context* ctx,
+                                             const asc_hip_sparse_desc* src,
+                                             void* d_idx32, asc_hip_sparse_desc* out) {
+  asc_hip_status st = check_context(ctx);
+  if (st.code) return st;
+  if (out == nullptr) return fail(ASC_HIP_INVALID_ARGUMENT, "output descriptor is null");
+  if ((st = check_device_sparse(ctx, src, "source")).code) return st;
+  if (src->format == ASC_HIP_COO) return fail(ASC_HIP_INVALID_ARGUMENT, "narrowing applies to CSR/CSC");
+  if (src->index_bytes != 8) return fail(ASC_HIP_INVALID_ARGUMENT, "source indices are not int64");
+  if (inner_of(*src) > std::numeric_limits<int32_t>::max())
+    return fail(ASC_HIP_OVERFLOW, "inner extent exceeds INT32_MAX");
+  if (src->nnz > 0) {
+    if ((st = check_device_pointer(ctx, d_idx32, "indices destination")).code) return st;
+    if (sparse_overlaps(*src, span_of(d_idx32, 4 * static_cast<size_t>(src->nnz))))
+      return fail(ASC_HIP_INVALID_ARGUMENT, "destination overlaps the source");
+  }
+  if ((st = unavailable_if_validation_only(ctx)).code) return st;
+  DeviceGuard guard(ctx->device);
+  st = finish_launch(ctx, asc_hip::narrow_indices_launch(
+      static_cast<const int64_t*>(src->inner_indices), static_cast<int32_t*>(d_idx32),
+      src->nnz, ctx->stream), "narrow indices");
+  if (st.code) return st;
+  *out = *src;
+  out->inner_indices = d_idx32;
+  out->index_bytes = 4;
+  return ok();
+}
+
+asc_hip_status asc_sparse_hip_convert_values(asc_hip_context* ctx,
+                                             const asc_hip_sparse_desc* src,
+                                             uint8_t kind, void* d_val,
+                                             asc_hip_sparse_desc* out) {
+  asc_hip_status st = check_context(ctx);
+  if (st.code) return st;
+  if (out == nullptr) return fail(ASC_HIP_INVALID_ARGUMENT, "output descriptor is null");
+  if ((st = check_device_sparse(ctx, src, "source")).code) return st;
+  if (src->element_kind != ASC_HIP_F64) return fail(ASC_HIP_UNSUPPORTED, "value conversion reads FP64 storage");
+  if (kind != ASC_HIP_F32 && kind != ASC_HIP_BF16 && kind != ASC_HIP_F16)
+    return fail(ASC_HIP_INVALID_ARGUMENT, "target must be FP32, BF16 or F16");
+  if (src->nnz > 0) {
+    if ((st = check_device_pointer(ctx, d_val, "values destination")).code) return st;
+    if (sparse_overlaps(*src, span_of(d_val, elem_size(kind) * src->nnz)))
+      return fail(ASC_HIP_INVALID_ARGUMENT, "destination overlaps the source");
+  }
+  if ((st = unavailable_if_validation_only(ctx)).code) return st;
+  DeviceGuard guard(ctx->device);
+  st = finish_launch(ctx, asc_hip::convert_values_launch(
+      static_cast<const double*>(src->values), d_val, static_cast<Elem>(kind), src->nnz,
+      ctx->stream), "convert values");
+  if (st.code) return st;
+  *out = *src;
+  out->values = d_val;
+  out->element_kind = kind;
+  return ok();
+}
+
+// ---- SpMV --------------------------------------------------------------------
+
+asc_hip_status asc_sparse_hip_csr_spmv_workspace_size(
+    asc_hip_context* ctx, const asc_hip_sparse_desc* a, const asc_hip_vector_desc* x,
+    const asc_hip_vector_desc* y, const asc_hip_spmv_options* opt, size_t* out) {
+  if (out == nullptr) return fail(ASC_HIP_INVALID_ARGUMENT, "output pointer is null");
+  SpmvPlan plan;
+  asc_hip_status st = prepare_compat_spmv(ctx, a, x, y, opt, &plan);
+  if (st.code) return st;
+  *out = plan.total;
+  return ok();
+}
+
+asc_hip_status asc_sparse_hip_csr_spmv(asc_hip_context* ctx, double alpha,
+                                       const asc_hip_sparse_desc* a,
+                                       const asc_hip_vector_desc* x, double beta,
+                                       const asc_hip_vector_desc* y, void* ws,
+                                       size_t ws_bytes, const asc_hip_spmv_options* opt) {
+  SpmvPlan plan;
+  asc_hip_status st = prepare_compat_spmv(ctx, a, x, y, opt, &plan);
+  if (st.code) return st;
+  if ((st = check_workspace(ctx, ws, ws_bytes, plan.total,
+                            operand_spans(*a, {vector_span(*x), vector_span(*y)}))).code)
+    return st;
+  if ((st = unavailable_if_validation_only(ctx)).code) return st;
+  if (plan.noop) return ok();
+  plan.args.alpha = alpha;
+  plan.args.beta = beta;
+  return run_spmv(ctx, plan, ws);
+}
+
+asc_hip_status asc_sparse_hip_standard_spmv_workspace_size(
+    asc_hip_context* ctx, uint8_t transpose, const asc_hip_sparse_desc* a,
+    const asc_hip_vector_desc* x, const asc_hip_vector_desc* y,
+    const asc_hip_spmv_options* opt, size_t* out) {
+  if (out == nullptr) return fail(ASC_HIP_INVALID_ARGUMENT, "output pointer is null");
+  SpmvPlan plan;
+  asc_hip_status st = prepare_standard_spmv(ctx, transpose, a, x, y, opt, &plan);
+  if (st.code) return st;
+  *out = plan.total;
+  return ok();
+}
+
+asc_hip_status asc_sparse_hip_standard_spmv(asc_hip_context* ctx, uint8_t transpose,
+                                            asc_hip_scalar alpha,
+                                            const asc_hip_sparse_desc* a,
+                                            const asc_hip_vector_desc* x,
+                                            const asc_hip_vector_desc* y, void* ws,
+                                            size_t ws_bytes,
+                                            const asc_hip_spmv_options* opt) {
+  SpmvPlan plan;
+  asc_hip_status st = prepare_standard_spmv(ctx, transpose, a, x, y, opt, &plan);
+  if (st.code) return st;
+  if ((st = check_workspace(ctx, ws, ws_bytes, plan.total,
+                            operand_spans(*a, {vector_span(*x), vector_span(*y)}))).code)
+    return st;
+  if ((st = unavailable_if_validation_only(ctx)).code) return st;
+  // blas.h:1650-1652: alpha == 0 returns without reading anything.
+  if (alpha.real == 0.0 || plan.noop) return ok();
+  plan.args.alpha = alpha.real;
+  return run_spmv(ctx, plan, ws);
+}
+
+// ---- SpMM --------------------------------------------------------------------
+
+namespace {
+
+asc_hip_status check_matrix(const asc_hip_context* ctx, const asc_hip_matrix_desc* m,
+                            uint8_t kind, int64_t rows, int64_t cols, bool writable,
+                            const char* what) {
+  if (m == nullptr) return fail(ASC_HIP_INVALID_ARGUMENT, std::string(what) + " descriptor is null");
+  if (m->element_kind != kind) return fail(ASC_HIP_INVALID_ARGUMENT, std::string(what) + ": element kind does not match");
+  if (m->layout > ASC_HIP_ROW_MAJOR) return fail(ASC_HIP_INVALID_ARGUMENT, std::string(what) + ": unknown layout");
+  if (m->rows != rows || (cols >= 0 && m->cols != cols))
+    return fail(ASC_HIP_SHAPE, "Sparse matrix-matrix operand extents do not match");
+  const int64_t minor = m->layout == ASC_HIP_ROW_MAJOR ? m->cols : m->rows;
+  if (m->leading_dimension < (minor > 1 ? minor : 1))
+    return fail(ASC_HIP_INVALID_ARGUMENT, std::string(what) + ": leading dimension is too small");
+  if (writable && !m->writable) return fail(ASC_HIP_INVALID_ARGUMENT, std::string(what) + ": output is not writable");
+  if (m->memory_space != ASC_HIP_DEVICE && m->memory_space != ASC_HIP_MANAGED)
+    return fail(ASC_HIP_MEMORY_ACCESS, std::string(what) + ": block is not device memory");
+  if (m->rows > 0 && m->cols > 0) return check_device_pointer(ctx, m->data, what);
+  return ok();
+}
+
+asc_hip_status prepare_spmm(asc_hip_context* ctx, uint8_t transpose,
+                            const asc_hip_sparse_desc* a, const asc_hip_matrix_desc* b,
+                            const asc_hip_matrix_desc* c, SpmvPlan* plan) {
+  asc_hip_status st = check_context(ctx);
+  if (st.code) return st;
+  if (transpose > ASC_HIP_OP_H) return fail(ASC_HIP_INVALID_ARGUMENT, "unknown transpose");
+  if ((st = check_device_sparse(ctx, a, "matrix")).code) return st;
+  if (a->format == ASC_HIP_COO) return fail(ASC_HIP_INVALID_ARGUMENT, "SpMM requires CSR or CSC");
+  if (a->element_kind != ASC_HIP_F32 && a->element_kind != ASC_HIP_F64)
+    return fail(ASC_HIP_UNSUPPORTED, "SpMM supports real FP32/FP64 storage");
+  const int64_t inner = transpose == ASC_HIP_OP_N ? a->cols : a->rows;
+  const int64_t rows = transpose == ASC_HIP_OP_N ? a->rows : a->cols;
+  if (b == nullptr || c == nullptr) return fail(ASC_HIP_INVALID_ARGUMENT, "dense descriptor is null");
+  if ((st = check_matrix(ctx, b, a->element_kind, inner, -1, false, "input")).code) return st;
+  if ((st = check_matrix(ctx, c, a->element_kind, rows, b->cols, true, "output")).code) return st;
+  const Span cs = matrix_span(*c);
+  if (overlaps(matrix_span(*b), cs) || sparse_overlaps(*a, cs))
+    return fail(ASC_HIP_INVALID_ARGUMENT, "Sparse matrix-matrix input and output overlap");
+  if ((st = plan_row_view(*a, transpose, plan)).code) return st;
+  plan->total = transpose_prefix(*plan);
+  plan->noop = rows == 0 || b->cols == 0;
+  return ok();
+}
+
+}  // namespace
+
+asc_hip_status asc_sparse_hip_standard_spmm_workspace_size(
+    asc_hip_context* ctx, uint8_t transpose, const asc_hip_sparse_desc* a,
+    const asc_hip_matrix_desc* b, const asc_hip_matrix_desc* c, size_t* out) {
+  if (out == nullptr) return fail(ASC_HIP_INVALID_ARGUMENT, "output pointer is null");
+  SpmvPlan plan;
+  asc_hip_status st = prepare_spmm(ctx, transpose, a, b, c, &plan);
+  if (st.code) return st;
+  *out = plan.total;
+  return ok();
+}
+
+asc_hip_status asc_sparse_hip_standard_spmm(asc_hip_context* ctx, uint8_t transpose,
+                                            asc_hip_scalar alpha,
+                                            const asc_hip_sparse_desc* a,
+                                            const asc_hip_matrix_desc* b,
+                                            const asc_hip_matrix_desc* c, void* ws,
+                                            size_t ws_bytes) {
+  SpmvPlan plan;
+  asc_hip_status st = prepare_spmm(ctx, transpose, a, b, c, &plan);
+  if (st.code) return st;
+  if ((st = check_workspace(ctx, ws, ws_bytes, plan.total,
+                            operand_spans(*a, {matrix_span(*b), matrix_span(*c)}))).code)
+    return st;
+  if ((st = unavailable_if_validation_only(ctx)).code) return st;
+  if (alpha.real == 0.0 || plan.noop) return ok();  // blas.h:1730-1732
+  DeviceGuard guard(ctx->device);
+  if (plan.transposed) {
+    const hipError_t e = run_transpose(&plan, static_cast<char*>(ws), ctx->stream);
+    if (e != hipSuccess) return finish_launch(ctx, e, "transpose");
+  }
+  asc_hip::SpmmArgs m;
+  m.a = plan.args.a;
+  m.b = b->data;
+  m.ldb = b->leading_dimension;
+  m.b_row_major = b->layout == ASC_HIP_ROW_MAJOR;
+  m.c = const_cast<void*>(c->data);
+  m.ldc = c->leading_dimension;
+  m.c_row_major = c->layout == ASC_HIP_ROW_MAJOR;
+  m.k = b->cols;
+  m.alpha = alpha.real;
+  return finish_launch(ctx, asc_hip::spmm_launch(m, ctx->stream), "SpMM launch");
+}
+
+// ---- conversions ---------------------------------------------------------------
+
+namespace {
+
+asc_hip_status prepare_coo_convert(asc_hip_context* ctx, const asc_hip_sparse_desc* src,
+                                   uint8_t out_format, uint8_t index_bytes, size_t* ws) {
+  asc_hip_status st = check_context(ctx);
+  if (st.code) return st;
+  if ((st = check_device_sparse(ctx, src, "source")).code) return st;
+  if (src->format != ASC_HIP_COO) return fail(ASC_HIP_INVALID_ARGUMENT, "source must be canonical COO");
+  if (out_format != ASC_HIP_CSR && out_format != ASC_HIP_CSC)
+    return fail(ASC_HIP_INVALID_ARGUMENT, "target format must be CSR or CSC");
+  if (index_bytes != 4 && index_bytes != 8) return fail(ASC_HIP_INVALID_ARGUMENT, "index width must be 4 or 8 bytes");
+  const int64_t inner = out_format == ASC_HIP_CSR ? src->cols : src->rows;
+  if (index_bytes == 4 && inner > std::numeric_limits<int32_t>::max())
+    return fail(ASC_HIP_OVERFLOW, "int32 indices cannot address the inner extent");
+  const bool csc = out_format == ASC_HIP_CSC;
+  if (csc && (src->nnz > kU32Max || src->rows > kU32Max || src->cols > kU32Max))
+    return fail(ASC_HIP_UNSUPPORTED, "device COO->CSC supports extents and nnz below 2^32");
+  *ws = asc_hip::coo_convert_workspace_bytes(src->rows, src->cols, src->nnz, csc,
+                                             static_cast<Elem>(src->element_kind));
+  return ok();
+}
+
+asc_hip_status check_outputs(const asc_hip_context* ctx, const asc_hip_sparse_desc& src,
+                             int64_t out_outer, uint8_t index_bytes, void* off,
+                             void* idx, void* val) {
+  asc_hip_status st;
+  const size_t ob = sizeof(int64_t) * (out_outer + 1);
+  const size_t ib = static_cast<size_t>(index_bytes) * src.nnz;
+  const size_t vb = elem_size(src.element_kind) * src.nnz;
+  if ((st = check_device_pointer(ctx, off, "offsets destination")).code) return st;
+  if (src.nnz > 0) {
+    if ((st = check_device_pointer(ctx, idx, "indices destination")).code) return st;
+    if ((st = check_device_pointer(ctx, val, "values destination")).code) return st;
+  }
+  const Span so = span_of(off, ob), si = span_of(idx, ib), sv = span_of(val, vb);
+  if (overlaps(so, si) || overlaps(so, sv) || overlaps(si, sv))
+    return fail(ASC_HIP_INVALID_ARGUMENT, "conversion destinations overlap");
+  if (sparse_overlaps(src, so) || sparse_overlaps(src, si) || sparse_overlaps(src, sv))
+    return fail(ASC_HIP_INVALID_ARGUMENT, "conversion destination overlaps the source");
+  return ok();
+}
+
+}  // namespace
+
+asc_hip_status asc_sparse_hip_convert_coordinate_workspace_size(
+    asc_hip_context* ctx, const asc_hip_sparse_desc* src, uint8_t out_format,
+    uint8_t index_bytes, size_t* out) {
+  if (out == nullptr) return fail(ASC_HIP_INVALID_ARGUMENT, "output pointer is null");
+  return prepare_coo_convert(ctx, src, out_format, index_bytes, out);
+}
+
+asc_hip_status asc_sparse_hip_convert_coordinate(
+    asc_hip_context* ctx, const asc_hip_sparse_desc* src, uint8_t out_format,
+    uint8_t index_bytes, void* off, void* idx, void* val, asc_hip_sparse_desc* out,
+    void* ws, size_t ws_bytes) {
+  size_t need = 0;
+  asc_hip_status st = prepare_coo_convert(ctx, src, out_format, index_bytes, &need);
+  if (st.code) return st;
+  if (out == nullptr) return fail(ASC_HIP_INVALID_ARGUMENT, "output descriptor is null");
+  const bool csc = out_format == ASC_HIP_CSC;
+  const int64_t out_outer = csc ? src->cols : src->rows;
+  if ((st = check_outputs(ctx, *src, out_outer, index_bytes, off, idx, val)).code) return st;
+  const size_t ob = sizeof(int64_t) * (out_outer + 1);
+  if ((st = check_workspace(ctx, ws, ws_bytes, need,
+                            operand_spans(*src, {span_of(off, ob),
+                                                 span_of(idx, index_bytes * src->nnz),
+                                                 span_of(val, elem_size(src->element_kind) * src->nnz)}))).code)
+    return st;
+  if ((st = unavailable_if_validation_only(ctx)).code) return st;
+  DeviceGuard guard(ctx->device);
+  const hipError_t e = asc_hip::coo_convert_launch(
+      static_cast<const int64_t*>(src->outer_offsets), src->values,
+      static_cast<Elem>(src->element_kind), src->rows, src->cols, src->nnz, csc,
+      static_cast<int64_t*>(off), idx, index_bytes, val, ws, ctx->stream);
+  if ((st = finish_launch(ctx, e, "COO conversion")).code) return st;
+  *out = *src;
+  out->format = out_format;
+  out->outer_offsets = off;
+  out->inner_indices = idx;
+  out->values = val;
+  out->index_bytes = index_bytes;
+  out->offset_bytes = 8;
+  out->trusted = 1;
+  return ok();
+}
+
+asc_hip_status asc_sparse_hip_convert_compressed_workspace_size(
+    asc_hip_context* ctx, const asc_hip_sparse_desc* src, size_t* out) {
+  if (out == nullptr) return fail(ASC_HIP_INVALID_ARGUMENT, "output pointer is null");
+  asc_hip_status st = check_context(ctx);
+  if (st.code) return st;
+  if ((st = check_device_sparse(ctx, src, "source")).code) return st;
+  if (src->format == ASC_HIP_COO) return fail(ASC_HIP_INVALID_ARGUMENT, "source must be CSR or CSC");
+  if (src->nnz > kU32Max || src->rows > kU32Max || src->cols > kU32Max)
+    return fail(ASC_HIP_UNSUPPORTED, "device transpose supports extents and nnz below 2^32");
+  if (src->index_bytes == 4 && outer_of(*src) > std::numeric_limits<int32_t>::max())
+    return fail(ASC_HIP_OVERFLOW, "int32 indices cannot address the transposed inner extent");
+  *out = asc_hip::transpose_workspace_bytes(as_compressed(*src, false));
+  return ok();
+}
+
+asc_hip_status asc_sparse_hip_convert_compressed(asc_hip_context* ctx,
+                                                 const asc_hip_sparse_desc* src,
+                                                 void* off, void* idx, void* val,
+                                                 asc_hip_sparse_desc* out, void* ws,
+                                                 size_t ws_bytes) {
+  size_t need = 0;
+  asc_hip_status st = asc_sparse_hip_convert_compressed_workspace_size(ctx, src, &need);
+  if (st.code) return st;
+  if (out == nullptr) return fail(ASC_HIP_INVALID_ARGUMENT, "output descriptor is null");
+  const int64_t out_outer = inner_of(*src);
+  if ((st = check_outputs(ctx, *src, out_outer, src->index_bytes, off, idx, val)).code) return st;
+  const size_t ob = sizeof(int64_t) * (out_outer + 1);
+  if ((st = check_workspace(ctx, ws, ws_bytes, need,
+                            operand_spans(*src, {span_of(off, ob),
+                                                 span_of(idx, src->index_bytes * src->nnz),
+                                                 span_of(val, elem_size(src->element_kind) * src->nnz)}))).code)
+    return st;
+  if ((st = unavailable_if_validation_only(ctx)).code) return st;
+  DeviceGuard guard(ctx->device);
+  const hipError_t e = asc_hip::transpose_launch(as_compressed(*src, false),
+                                                 static_cast<int64_t*>(off), idx,
+                                                 src->index_bytes, val, ws, ctx->stream);
+  if ((st = finish_launch(ctx, e, "compressed conversion")).code) return st;
+  *out = *src;
+  out->format = src->format == ASC_HIP_CSR ? ASC_HIP_CSC : ASC_HIP_CSR;
+  out->outer_offsets = off;
+  out->inner_indices = idx;
+  out->values = val;
+  out->offset_bytes = 8;
+  out->trusted = 1;
+  return ok();
+}
+
+}  // extern "C"

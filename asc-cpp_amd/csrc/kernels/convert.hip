@@ -1,0 +1,513 @@
+// Device conversions for gfx950: canonical COO -> CSR/CSC and the stable
+// CSR <-> CSC transpose.
+//
+// The reference builds every compressed array by scanning the source once
+// per output outer index (compressed.h:1082-1131, 1151-1221); its output is
+// the unique canonical order, i.e. a *stable* sort of the source entries by
+// the new outer index.  Here that sort is an LSD radix sort:
+//   - 8-bit digits, one histogram + scan + scatter round per digit;
+//   - tiles of kTile entries; inside a tile each wave64 ranks its entries
+//     with a ballot "match" on the digit bits, which keeps equal digits in
+//     source order (stability), so the result is bit-exact with the
+//     reference on indices and values;
+//   - keys travel with their payload (other coordinate + raw value bits);
+//   - offsets come from the sorted keys by a boundary pass
+//     (offsets[c] = first position whose key >= c), no atomics anywhere.
+// Canonical COO -> CSR is already sorted by row: one streaming pass copies
+// columns and values and derives the offsets by the same boundary rule.
+#include "device_common.h"
+#include "launch.h"
+
+namespace asc_hip {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kIpt = 16;
+constexpr int kTile = kBlock * kIpt;  // entries per radix tile
+constexpr int kRadixBits = 8;
+constexpr int kRadix = 1 << kRadixBits;
+constexpr int kWaves = kBlock / kWave;
+constexpr int kWaveItems = kWave * kIpt;
+
+inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+unsigned blocks_for(int64_t n, int64_t per) {
+  return static_cast<unsigned>((n + per - 1) / per);
+}
+
+// ---- exclusive scan of uint32 counts (reduce, scan partials, downsweep) ----
+constexpr int kScanIpt = 16;
+constexpr int kScanTile = kBlock * kScanIpt;
+
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v,
+                                                         uint32_t* s_tmp,
+                                                         uint32_t* total) {
+  const int lane = threadIdx.x % kWave;
+  const int wave = threadIdx.x / kWave;
+  uint32_t x = v;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off);
+    if (lane >= off) x += y;
+  }
+  if (lane == kWave - 1) s_tmp[wave] = x;
+  __syncthreads();
+  uint32_t base = 0, all = 0;
+  for (int w = 0; w < kWaves; ++w) {
+    if (w < wave) base += s_tmp[w];
+    all += s_tmp[w];
+  }
+  __syncthreads();
+  *total = all;
+  return base + x - v;
+}
+
+__global__ __launch_bounds__(kBlock) void scan_reduce_kernel(
+    const uint32_t* __restrict__ in, int64_t n, uint32_t* __restrict__ partial) {
+  __shared__ uint32_t s_tmp[kWaves];
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile;
+  uint32_t sum = 0;
+#pragma unroll
+  for (int u = 0; u < kScanIpt; ++u) {
+    const int64_t i = base + u * kBlock + threadIdx.x;
+    if (i < n) sum += in[i];
+  }
+  uint32_t total;
+  block_exclusive_scan(sum, s_tmp, &total);
+  if (threadIdx.x == 0) partial[blockIdx.x] = total;
+}
+
+// Exclusive scan in place of a short array by one block.
+__global__ __launch_bounds__(kBlock) void scan_single_kernel(uint32_t* data,
+                                                             int64_t n) {
+  __shared__ uint32_t s_tmp[kWaves];
+  uint32_t carry = 0;
+  for (int64_t base = 0; base < n; base += kBlock) {
+    const int64_t i = base + threadIdx.x;
+    const uint32_t v = i < n ? data[i] : 0u;
+    uint32_t total;
+    const uint32_t ex = block_exclusive_scan(v, s_tmp, &total);
+    if (i < n) data[i] = carry + ex;
+    carry += total;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void scan_downsweep_kernel(
+    const uint32_t* __restrict__ in, int64_t n,
+    const uint32_t* __restrict__ partial, uint32_t* __restrict__ out) {
+  __shared__ uint32_t s_tmp[kWaves];
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile;
+  // Each thread owns kScanIpt consecutive entries.
+  uint32_t v[kScanIpt];
+  uint32_t sum = 0;
+#pragma unroll
+  for (int u = 0; u < kScanIpt; ++u) {
+    const int64_t i = base + threadIdx.x * kScanIpt + u;
+    v[u] = i < n ? in[i] : 0u;
+    sum += v[u];
+  }
+  uint32_t total;
+  uint32_t run = partial[blockIdx.x] + block_exclusive_scan(sum, s_tmp, &total);
+#pragma unroll
+  for (int u = 0; u < kScanIpt; ++u) {
+    const int64_t i = base + threadIdx.x * kScanIpt + u;
+    if (i < n) out[i] = run;
+    run += v[u];
+  }
+}
+
+size_t scan_workspace_bytes(int64_t n) {
+  const int64_t nb = (n + kScanTile - 1) / kScanTile;
+  return align256(sizeof(uint32_t) * (nb > 0 ? nb : 1));
+}
+
+// Exclusive scan of n uint32 (n * kScanTile-sized partials must fit one
+// block pass; the single-block step loops, so any n works).
+hipError_t scan_exclusive(const uint32_t* in, uint32_t* out, int64_t n,
+                          uint32_t* partial, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const unsigned nb = blocks_for(n, kScanTile);
+  scan_reduce_kernel<<<nb, kBlock, 0, s>>>(in, n, partial);
+  scan_single_kernel<<<1, kBlock, 0, s>>>(partial, nb);
+  scan_downsweep_kernel<<<nb, kBlock, 0, s>>>(in, n, partial, out);
+  return hipGetLastError();
+}
+
+// ---- row expansion: rows[p] = outer index owning position p --------------
+template <int G>
+__global__ __launch_bounds__(kBlock) void expand_kernel(
+    const int64_t* __restrict__ offsets, int64_t outer,
+    uint32_t* __restrict__ rows) {
+  const int64_t o = xcd_remap(blockIdx.x, gridDim.x) * (kBlock / G) + threadIdx.x / G;
+  if (o >= outer) return;
+  const int64_t b = offsets[o], e = offsets[o + 1];
+  for (int64_t p = b + threadIdx.x % G; p < e; p += G)
+    rows[p] = static_cast<uint32_t>(o);
+}
+
+hipError_t expand_launch(const int64_t* offsets, int64_t outer, int64_t nnz,
+                         uint32_t* rows, hipStream_t s) {
+  if (outer == 0 || nnz == 0) return hipSuccess;
+  const int64_t avg = nnz / outer;
+  if (avg >= 32) {
+    expand_kernel<64><<<blocks_for(outer, kBlock / 64), kBlock, 0, s>>>(offsets, outer, rows);
+  } else if (avg >= 4) {
+    expand_kernel<8><<<blocks_for(outer, kBlock / 8), kBlock, 0, s>>>(offsets, outer, rows);
+  } else {
+    expand_kernel<1><<<blocks_for(outer, kBlock), kBlock, 0, s>>>(offsets, outer, rows);
+  }
+  return hipGetLastError();
+}
+
+// ---- radix passes ---------------------------------------------------------
+// Key sources: a uint32 array, or an int32/int64 array read with a stride
+// (COO coordinates are interleaved).
+template <typename K>
+__device__ __forceinline__ uint32_t read_key(const K* keys, int64_t stride,
+                                             int64_t i) {
+  return static_cast<uint32_t>(ld_stream(keys + i * stride));
+}
+
+template <typename K>
+__global__ __launch_bounds__(kBlock) void radix_hist_kernel(
+    const K* __restrict__ keys, int64_t key_stride, int64_t n, int shift,
+    uint32_t* __restrict__ counts, int64_t ntiles) {
+  __shared__ uint32_t s_hist[kRadix];
+  s_hist[threadIdx.x] = 0;  // kBlock == kRadix
+  __syncthreads();
+  const int64_t tile = blockIdx.x;
+  const int64_t base = tile * kTile;
+#pragma unroll
+  for (int u = 0; u < kIpt; ++u) {
+    const int64_t i = base + u * kBlock + threadIdx.x;
+    if (i < n) {
+      const uint32_t d = (read_key(keys, key_stride, i) >> shift) & (kRadix - 1);
+      atomicAdd(&s_hist[d], 1u);
+    }
+  }
+  __syncthreads();
+  counts[static_cast<int64_t>(threadIdx.x) * ntiles + tile] = s_hist[threadIdx.x];
+}
+
+template <typename T>
+__device__ __forceinline__ T ld_plain(const T* p) { return *p; }
+
+// One stable scatter round.  Entry i of the tile is handled by wave
+// (i / kWaveItems), chunk ((i % kWaveItems) / 64), lane (i % 64), so chunk
+// order and lane order are source order.
+template <typename K, typename R, typename VB, typename RO>
+__global__ __launch_bounds__(kBlock) void radix_scatter_kernel(
+    const K* __restrict__ keys, int64_t key_stride, const R* __restrict__ rows,
+    int64_t row_stride, const VB* __restrict__ vals, int64_t n, int shift,
+    const uint32_t* __restrict__ offsets, int64_t ntiles,
+    uint32_t* __restrict__ keys_out, RO* __restrict__ rows_out,
+    VB* __restrict__ vals_out) {
+  __shared__ uint32_t s_count[kWaves][kRadix];
+  const int lane = threadIdx.x % kWave;
+  const int wave = threadIdx.x / kWave;
+  for (int d = threadIdx.x; d < kWaves * kRadix; d += kBlock)
+    (&s_count[0][0])[d] = 0;
+  __syncthreads();
+  const int64_t tile = blockIdx.x;
+  const int64_t base = tile * kTile + static_cast<int64_t>(wave) * kWaveItems;
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+
+  uint32_t key[kIpt];
+  uint32_t rank[kIpt];
+#pragma unroll
+  for (int u = 0; u < kIpt; ++u) {
+    const int64_t i = base + u * kWave + lane;
+    const bool valid = i < n;
+    key[u] = valid ? read_key(keys, key_stride, i) : 0u;
+    const uint32_t d = (key[u] >> shift) & (kRadix - 1);
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < kRadixBits; ++b) {
+      const uint64_t bal = __ballot((d >> b) & 1u);
+      peers &= ((d >> b) & 1u) ? bal : ~bal;
+    }
+    const uint32_t lower = __popcll(peers & lt_mask);
+    const uint32_t total = __popcll(peers);
+    uint32_t r = 0;
+    if (valid) r = s_count[wave][d] + lower;
+    rank[u] = r;
+    // The highest peer publishes the new running count after every peer
+    // has read the old one (LDS operations of one wave retire in order).
+    if (valid && lower == total - 1) s_count[wave][d] += total;
+  }
+  __syncthreads();
+  // Per-digit base of this wave: global tile offset + earlier waves' counts.
+  __shared__ uint32_t s_base[kWaves][kRadix];
+  {
+    const int d = threadIdx.x;  // kBlock == kRadix
+    uint32_t run = offsets[static_cast<int64_t>(d) * ntiles + tile];
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+      s_base[w][d] = run;
+      run += s_count[w][d];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < kIpt; ++u) {
+    const int64_t i = base + u * kWave + lane;
+    if (i < n) {
+      const uint32_t d = (key[u] >> shift) & (kRadix - 1);
+      const uint32_t dst = s_base[wave][d] + rank[u];
+      keys_out[dst] = key[u];
+      rows_out[dst] = static_cast<RO>(ld_stream(rows + i * row_stride));
+      vals_out[dst] = ld_stream(vals + i);
+    }
+  }
+}
+
+// offsets[c] = first sorted position with key >= c, for c in [0, outer].
+__global__ __launch_bounds__(kBlock) void boundary_kernel(
+    const uint32_t* __restrict__ keys, int64_t n, int64_t outer,
+    int64_t* __restrict__ offsets) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
+  for (int64_t p = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+       p <= n; p += stride) {
+    const int64_t k = p < n ? static_cast<int64_t>(keys[p]) : outer;
+    const int64_t kprev = p > 0 ? static_cast<int64_t>(keys[p - 1]) : -1;
+    for (int64_t c = kprev + 1; c <= k; ++c) offsets[c] = p;
+  }
+}
+
+// Same rule over interleaved COO rows (row of entry p = coords[2p]).
+__global__ __launch_bounds__(kBlock) void coo_rows_kernel(
+    const int64_t* __restrict__ coords, const uint8_t* __restrict__ vals,
+    int value_bytes, int64_t nnz, int64_t outer, int64_t* __restrict__ offsets,
+    void* __restrict__ out_indices, int index_bytes,
+    uint8_t* __restrict__ out_vals) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
+  for (int64_t p = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+       p <= nnz; p += stride) {
+    const int64_t k = p < nnz ? coords[2 * p] : outer;
+    const int64_t kprev = p > 0 ? coords[2 * p - 2] : -1;
+    for (int64_t c = kprev + 1; c <= k; ++c) offsets[c] = p;
+    if (p < nnz) {
+      const int64_t col = coords[2 * p + 1];
+      if (index_bytes == 4) static_cast<int32_t*>(out_indices)[p] = static_cast<int32_t>(col);
+      else static_cast<int64_t*>(out_indices)[p] = col;
+      if (value_bytes == 8) reinterpret_cast<uint64_t*>(out_vals)[p] = reinterpret_cast<const uint64_t*>(vals)[p];
+      else if (value_bytes == 4) reinterpret_cast<uint32_t*>(out_vals)[p] = reinterpret_cast<const uint32_t*>(vals)[p];
+      else reinterpret_cast<uint16_t*>(out_vals)[p] = reinterpret_cast<const uint16_t*>(vals)[p];
+    }
+  }
+}
+
+unsigned grid_stride_blocks(int64_t n) {
+  int64_t g = (n + kBlock - 1) / kBlock;
+  if (g > 65536) g = 65536;
+  if (g < 1) g = 1;
+  return static_cast<unsigned>(g);
+}
+
+int key_bits(int64_t outer) {
+  int bits = 0;
+  while (bits < 63 && (int64_t(1) << bits) < outer) ++bits;
+  return bits;
+}
+
+int passes_for(int64_t outer) {
+  const int bits = key_bits(outer);
+  return bits == 0 ? 1 : (bits + kRadixBits - 1) / kRadixBits;
+}
+
+// Workspace layout of a stable sort of n entries keyed below `outer`.
+struct SortLayout {
+  size_t keys[2], rows[2], vals[2], counts, scanned, partial, total;
+};
+
+SortLayout sort_layout(int64_t n, int64_t outer, size_t vbytes) {
+  SortLayout L{};
+  const int64_t ntiles = (n + kTile - 1) / kTile;
+  const int64_t ncounts = ntiles * kRadix;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = off;
+    off += align256(bytes);
+    return o;
+  };
+  L.keys[0] = take(sizeof(uint32_t) * n);
+  L.keys[1] = take(sizeof(uint32_t) * n);
+  L.rows[0] = take(sizeof(uint32_t) * n);
+  L.rows[1] = take(sizeof(uint32_t) * n);
+  L.vals[0] = take(vbytes * n);
+  L.vals[1] = take(vbytes * n);
+  L.counts = take(sizeof(uint32_t) * ncounts);
+  L.scanned = take(sizeof(uint32_t) * ncounts);
+  L.partial = take(scan_workspace_bytes(ncounts));
+  L.total = off;
+  (void)outer;
+  return L;
+}
+
+// Stable sort by key (< outer) with payload (row, value).  Round 0 reads the
+// caller's arrays (key/row strided); later rounds ping-pong in the
+// workspace; the last round writes rows/values to the outputs.
+template <typename K0, typename R0, typename VB>
+hipError_t stable_sort(const K0* key0, int64_t key_stride, const R0* row0,
+                       int64_t row_stride, const VB* val0, int64_t n,
+                       int64_t outer, void* out_rows, int out_row_bytes,
+                       VB* out_vals, int64_t* out_offsets, char* ws,
+                       hipStream_t s) {
+  const SortLayout L = sort_layout(n, outer, sizeof(VB));
+  const int64_t ntiles = (n + kTile - 1) / kTile;
+  const int passes = passes_for(outer);
+  uint32_t* counts = reinterpret_cast<uint32_t*>(ws + L.counts);
+  uint32_t* scanned = reinterpret_cast<uint32_t*>(ws + L.scanned);
+  uint32_t* partial = reinterpret_cast<uint32_t*>(ws + L.partial);
+  uint32_t* kbuf[2] = {reinterpret_cast<uint32_t*>(ws + L.keys[0]),
+                       reinterpret_cast<uint32_t*>(ws + L.keys[1])};
+  uint32_t* rbuf[2] = {reinterpret_cast<uint32_t*>(ws + L.rows[0]),
+                       reinterpret_cast<uint32_t*>(ws + L.rows[1])};
+  VB* vbuf[2] = {reinterpret_cast<VB*>(ws + L.vals[0]),
+                 reinterpret_cast<VB*>(ws + L.vals[1])};
+  hipError_t err = hipSuccess;
+  for (int pass = 0; pass < passes && n > 0; ++pass) {
+    const int shift = pass * kRadixBits;
+    const int out = pass & 1;
+    const bool last = pass == passes - 1;
+    if (pass == 0) {
+      radix_hist_kernel<K0><<<static_cast<unsigned>(ntiles), kBlock, 0, s>>>(
+          key0, key_stride, n, shift, counts, ntiles);
+    } else {
+      radix_hist_kernel<uint32_t><<<static_cast<unsigned>(ntiles), kBlock, 0, s>>>(
+          kbuf[out ^ 1], 1, n, shift, counts, ntiles);
+    }
+    if ((err = hipGetLastError()) != hipSuccess) return err;
+    if ((err = scan_exclusive(counts, scanned, ntiles * kRadix, partial, s)) != hipSuccess)
+      return err;
+    const unsigned g = static_cast<unsigned>(ntiles);
+#define ASC_SCATTER(KT, KP, KS, RT, RP, RS, VP, ROT, ROP)                        \
+  radix_scatter_kernel<KT, RT, VB, ROT><<<g, kBlock, 0, s>>>(                    \
+      KP, KS, RP, RS, VP, n, shift, scanned, ntiles, kbuf[out], ROP,             \
+      last ? out_vals : vbuf[out])
+    if (pass == 0) {
+      if (last) {
+        if (out_row_bytes == 4)
+          ASC_SCATTER(K0, key0, key_stride, R0, row0, row_stride, val0, int32_t,
+                      static_cast<int32_t*>(out_rows));
+        else
+          ASC_SCATTER(K0, key0, key_stride, R0, row0, row_stride, val0, int64_t,
+                      static_cast<int64_t*>(out_rows));
+      } else {
+        ASC_SCATTER(K0, key0, key_stride, R0, row0, row_stride, val0, uint32_t,
+                    rbuf[out]);
+      }
+    } else {
+      if (last) {
+        if (out_row_bytes == 4)
+          ASC_SCATTER(uint32_t, kbuf[out ^ 1], 1, uint32_t, rbuf[out ^ 1], 1,
+                      vbuf[out ^ 1], int32_t, static_cast<int32_t*>(out_rows));
+        else
+          ASC_SCATTER(uint32_t, kbuf[out ^ 1], 1, uint32_t, rbuf[out ^ 1], 1,
+                      vbuf[out ^ 1], int64_t, static_cast<int64_t*>(out_rows));
+      } else {
+        ASC_SCATTER(uint32_t, kbuf[out ^ 1], 1, uint32_t, rbuf[out ^ 1], 1,
+                    vbuf[out ^ 1], uint32_t, rbuf[out]);
+      }
+    }
+#undef ASC_SCATTER
+    if ((err = hipGetLastError()) != hipSuccess) return err;
+  }
+  const uint32_t* sorted_keys = kbuf[(passes - 1) & 1];
+  boundary_kernel<<<grid_stride_blocks(n + 1), kBlock, 0, s>>>(
+      sorted_keys, n, outer, out_offsets);
+  return hipGetLastError();
+}
+
+template <typename VB>
+hipError_t transpose_typed(const Compressed& src, int64_t* out_offsets,
+                           void* out_indices, int out_index_bytes,
+                           void* out_values, char* ws, hipStream_t s) {
+  const int64_t n = src.nnz;
+  const SortLayout L = sort_layout(n, src.inner, sizeof(VB));
+  // Source rows for every entry go in the second row buffer: round 0 reads
+  // them, later rounds use both buffers only after round 0 consumed them...
+  // so give round 0 its own region at the tail of the workspace.
+  uint32_t* rows0 = reinterpret_cast<uint32_t*>(ws + L.total);
+  hipError_t err = expand_launch(src.offsets, src.outer, n, rows0, s);
+  if (err != hipSuccess) return err;
+  const VB* vals = static_cast<const VB*>(src.values);
+  if (src.index_bytes == 4) {
+    return stable_sort<int32_t, uint32_t, VB>(
+        static_cast<const int32_t*>(src.indices), 1, rows0, 1, vals, n,
+        src.inner, out_indices, out_index_bytes, static_cast<VB*>(out_values),
+        out_offsets, ws, s);
+  }
+  return stable_sort<int64_t, uint32_t, VB>(
+      static_cast<const int64_t*>(src.indices), 1, rows0, 1, vals, n, src.inner,
+      out_indices, out_index_bytes, static_cast<VB*>(out_values), out_offsets,
+      ws, s);
+}
+
+}  // namespace
+
+size_t transpose_workspace_bytes(const Compressed& src) {
+  const SortLayout L = sort_layout(src.nnz, src.inner, elem_bytes(src.elem));
+  return L.total + align256(sizeof(uint32_t) * src.nnz);
+}
+
+hipError_t transpose_launch(const Compressed& src, int64_t* out_offsets,
+                            void* out_indices, int out_index_bytes,
+                            void* out_values, void* workspace, hipStream_t s) {
+  char* ws = static_cast<char*>(workspace);
+  if (src.nnz == 0) {
+    // No entries: every output offset is zero.
+    return fill_launch(out_offsets, 0, sizeof(int64_t) * (src.inner + 1), s);
+  }
+  switch (elem_bytes(src.elem)) {
+    case 8: return transpose_typed<uint64_t>(src, out_offsets, out_indices, out_index_bytes, out_values, ws, s);
+    case 4: return transpose_typed<uint32_t>(src, out_offsets, out_indices, out_index_bytes, out_values, ws, s);
+    default: return transpose_typed<uint16_t>(src, out_offsets, out_indices, out_index_bytes, out_values, ws, s);
+  }
+}
+
+size_t coo_convert_workspace_bytes(int64_t rows, int64_t cols, int64_t nnz,
+                                   bool to_csc, Elem elem) {
+  if (!to_csc) return 0;
+  (void)rows;
+  return sort_layout(nnz, cols, elem_bytes(elem)).total;
+}
+
+hipError_t coo_convert_launch(const int64_t* coords, const void* values,
+                              Elem elem, int64_t rows, int64_t cols,
+                              int64_t nnz, bool to_csc, int64_t* out_offsets,
+                              void* out_indices, int out_index_bytes,
+                              void* out_values, void* workspace,
+                              hipStream_t s) {
+  const int64_t outer = to_csc ? cols : rows;
+  if (nnz == 0) return fill_launch(out_offsets, 0, sizeof(int64_t) * (outer + 1), s);
+  if (!to_csc) {
+    coo_rows_kernel<<<grid_stride_blocks(nnz + 1), kBlock, 0, s>>>(
+        coords, static_cast<const uint8_t*>(values),
+        static_cast<int>(elem_bytes(elem)), nnz, rows, out_offsets, out_indices,
+        out_index_bytes, static_cast<uint8_t*>(out_values));
+    return hipGetLastError();
+  }
+  // CSC: stable sort by column; the row travels as payload.  Canonical COO
+  // is row-major, so equal columns keep increasing rows.
+  char* ws = static_cast<char*>(workspace);
+  switch (elem_bytes(elem)) {
+    case 8:
+      return stable_sort<int64_t, int64_t, uint64_t>(
+          coords + 1, 2, coords, 2, static_cast<const uint64_t*>(values), nnz,
+          cols, out_indices, out_index_bytes, static_cast<uint64_t*>(out_values),
+          out_offsets, ws, s);
+    case 4:
+      return stable_sort<int64_t, int64_t, uint32_t>(
+          coords + 1, 2, coords, 2, static_cast<const uint32_t*>(values), nnz,
+          cols, out_indices, out_index_bytes, static_cast<uint32_t*>(out_values),
+          out_offsets, ws, s);
+    default:
+      return stable_sort<int64_t, int64_t, uint16_t>(
+          coords + 1, 2, coords, 2, static_cast<const uint16_t*>(values), nnz,
+          cols, out_indices, out_index_bytes, static_cast<uint16_t*>(out_values),
+          out_offsets, ws, s);
+  }
+}
+
+}  // namespace asc_hip

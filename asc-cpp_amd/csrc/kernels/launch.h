@@ -1,0 +1,100 @@
+// Host-side launch interface between the C ABI (csrc/abi) and the kernels.
+// Every launcher assumes its arguments were validated by the ABI layer
+// (shapes, element kinds, trust, workspace size) and only enqueues work.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace asc_hip {
+
+enum class Elem : uint8_t { F32 = 0, F64 = 1, BF16 = 4, F16 = 5 };
+
+// A compressed operand seen as "rows" of the kernel: outer = row-like
+// dimension.  Offsets are int64; indices int32 or int64.
+struct Compressed {
+  const int64_t* offsets;
+  const void* indices;
+  const void* values;
+  int64_t outer;
+  int64_t inner;
+  int64_t nnz;
+  int index_bytes;
+  Elem elem;
+};
+
+enum class SpmvAlg : uint8_t {
+  Ordered = 1, Merge = 2,
+  Vec1 = 3, Vec2 = 4, Vec4 = 5, Vec8 = 6, Vec16 = 7, Vec32 = 8, Vec64 = 9
+};
+
+enum class SpmvForm : uint8_t {
+  Compat = 0,    // y = alpha*sum + beta*y (y unread when beta == 0)
+  Standard = 1,  // y += alpha*v*x per entry (ordered) / y += alpha*sum (plain)
+};
+
+struct SpmvArgs {
+  Compressed a;
+  const void* x;
+  int64_t incx;
+  void* y;
+  int64_t incy;
+  double alpha;
+  double beta;
+  SpmvForm form;
+  SpmvAlg alg;
+  bool twosum;  // reduced-precision compensated accumulation
+};
+
+// Workspace bytes the algorithm needs for this operand (0 for most).
+size_t spmv_workspace_bytes(const SpmvArgs& args);
+hipError_t spmv_launch(const SpmvArgs& args, void* workspace,
+                       hipStream_t stream);
+
+struct SpmmArgs {
+  Compressed a;
+  const void* b;
+  int64_t ldb;
+  bool b_row_major;
+  void* c;
+  int64_t ldc;
+  bool c_row_major;
+  int64_t k;  // right-hand sides
+  double alpha;
+};
+hipError_t spmm_launch(const SpmmArgs& args, hipStream_t stream);
+
+// ---- conversions ---------------------------------------------------------
+// Stable transpose of a compressed operand (CSR<->CSC): out_offsets has
+// inner+1 entries, out_indices nnz entries of out_index_bytes, out_values
+// nnz values of the same element kind.
+size_t transpose_workspace_bytes(const Compressed& src);
+hipError_t transpose_launch(const Compressed& src, int64_t* out_offsets,
+                            void* out_indices, int out_index_bytes,
+                            void* out_values, void* workspace,
+                            hipStream_t stream);
+
+// Canonical COO (interleaved int64 (row, col)) -> CSR (to_csc false) or CSC.
+size_t coo_convert_workspace_bytes(int64_t rows, int64_t cols, int64_t nnz,
+                                   bool to_csc, Elem elem);
+hipError_t coo_convert_launch(const int64_t* coords, const void* values,
+                              Elem elem, int64_t rows, int64_t cols,
+                              int64_t nnz, bool to_csc, int64_t* out_offsets,
+                              void* out_indices, int out_index_bytes,
+                              void* out_values, void* workspace,
+                              hipStream_t stream);
+
+// int64 -> int32 index narrowing (caller guarantees inner <= INT32_MAX).
+hipError_t narrow_indices_launch(const int64_t* src, int32_t* dst, int64_t n,
+                                 hipStream_t stream);
+// FP64 -> FP32/BF16/F16 with round-to-nearest-even.
+hipError_t convert_values_launch(const double* src, void* dst, Elem to,
+                                 int64_t n, hipStream_t stream);
+// Byte fill as a kernel (graph-capture safe, unlike hipMemsetAsync).
+hipError_t fill_launch(void* dst, uint8_t byte, size_t bytes,
+                       hipStream_t stream);
+
+size_t elem_bytes(Elem e);
+
+}  // namespace asc_hip

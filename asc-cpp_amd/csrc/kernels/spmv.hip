@@ -1,0 +1,618 @@
+// CSR SpMV kernels for gfx950.
+//
+//   vector<G>  G lanes (1..64) per row, coalesced strided walk of the row,
+//              butterfly reduction, fused alpha/beta epilogue.
+//   merge      merge-path split (Merrill & Garland, SC'16): every tile owns
+//              exactly kTile items of the rows+nnz merge sequence, so row
+//              length skew cannot unbalance the grid.  Products are staged
+//              in LDS, each thread walks kIpt consecutive merge items, a
+//              wave64 segmented scan hands partial rows to their owners and
+//              rows that cross tiles are finished by a small fix-up kernel.
+//   ordered    one lane per row, strict left-to-right unfused chain: the
+//              serial reference's arithmetic bit for bit
+//              (src/sparse/reference_blas.cc:13-39, blas.h:1653-1665).
+//
+// Value storage V in {double, float, bf16, f16}; accumulation in double for
+// double and in float otherwise (plain, or compensated with TwoSum plus the
+// FMA product error).
+#include "device_common.h"
+#include "launch.h"
+
+namespace asc_hip {
+namespace {
+
+constexpr int kBlock = 256;
+
+// ---- accumulators ---------------------------------------------------------
+template <typename A>
+struct PlainAcc {
+  A s;  // trivially constructible so it can live in LDS; start from zero()
+  __device__ __forceinline__ static PlainAcc zero() {
+    PlainAcc r;
+    r.s = A(0);
+    return r;
+  }
+  __device__ __forceinline__ void add(A p) { s += p; }
+  __device__ __forceinline__ void add_product(A v, A x) { s += v * x; }
+  __device__ __forceinline__ void merge_after(const PlainAcc& later) {
+    s += later.s;
+  }
+  __device__ __forceinline__ PlainAcc shfl_xor(int off, int width) const {
+    PlainAcc r;
+    r.s = __shfl_xor(s, off, width);
+    return r;
+  }
+  __device__ __forceinline__ PlainAcc shfl_up(int off) const {
+    PlainAcc r;
+    r.s = __shfl_up(s, off);
+    return r;
+  }
+  __device__ __forceinline__ double total_f64() const {
+    return static_cast<double>(s);
+  }
+  __device__ __forceinline__ A total() const { return s; }
+};
+
+// Compensated FP32 accumulation: s + c carries the running sum; every
+// product's rounding error is captured with an FMA and every addition's with
+// Knuth's TwoSum.
+struct TwoSumAcc {
+  float s;
+  float c;
+  __device__ __forceinline__ static TwoSumAcc zero() {
+    TwoSumAcc r;
+    r.s = 0.0f;
+    r.c = 0.0f;
+    return r;
+  }
+  __device__ __forceinline__ static void two_sum(float a, float b, float& t,
+                                                 float& e) {
+#pragma clang fp contract(off)
+    t = a + b;
+    const float bp = t - a;
+    e = (a - (t - bp)) + (b - bp);
+  }
+  __device__ __forceinline__ void add(float p) {
+#pragma clang fp contract(off)
+    float t, e;
+    two_sum(s, p, t, e);
+    s = t;
+    c += e;
+  }
+  __device__ __forceinline__ void add_pair(float p, float pe) {
+#pragma clang fp contract(off)
+    float t, e;
+    two_sum(s, p, t, e);
+    s = t;
+    c += e + pe;
+  }
+  __device__ __forceinline__ void add_product(float v, float x) {
+#pragma clang fp contract(off)
+    const float p = v * x;
+    const float pe = __builtin_fmaf(v, x, -p);
+    add_pair(p, pe);
+  }
+  __device__ __forceinline__ void merge_after(const TwoSumAcc& later) {
+#pragma clang fp contract(off)
+    float t, e;
+    two_sum(s, later.s, t, e);
+    s = t;
+    c += e + later.c;
+  }
+  __device__ __forceinline__ TwoSumAcc shfl_xor(int off, int width) const {
+    TwoSumAcc r;
+    r.s = __shfl_xor(s, off, width);
+    r.c = __shfl_xor(c, off, width);
+    return r;
+  }
+  __device__ __forceinline__ TwoSumAcc shfl_up(int off) const {
+    TwoSumAcc r;
+    r.s = __shfl_up(s, off);
+    r.c = __shfl_up(c, off);
+    return r;
+  }
+  __device__ __forceinline__ double total_f64() const {
+    return static_cast<double>(s) + static_cast<double>(c);
+  }
+  __device__ __forceinline__ float total() const { return s + c; }
+};
+
+template <typename V, bool TWOSUM>
+struct AccSel {
+  using type = PlainAcc<typename acc_of<V>::type>;
+};
+template <typename V>
+struct AccSel<V, true> {
+  using type = TwoSumAcc;
+};
+
+struct KParams {
+  const int64_t* offsets;
+  const void* indices;
+  const void* values;
+  int64_t rows;
+  int64_t nnz;
+  const void* x;
+  int64_t incx;
+  void* y;
+  int64_t incy;
+  double alpha;
+  double beta;
+  int form;  // SpmvForm
+};
+
+// Fused epilogue for one finished row.
+//   compat:   y = alpha*sum (+ beta*y when beta != 0)
+//   standard: y = y + alpha*sum
+// FP64 storage computes in FP64; FP32-accumulated forms compute in FP32, and
+// the compensated form finishes its (s, c) pair in FP64 (one multiply-add
+// per row: the only FP64 arithmetic of the reduced-precision path).
+template <typename V, typename Acc>
+__device__ __forceinline__ void finish_row(const KParams& p, int64_t row,
+                                           const Acc& acc) {
+  using A = typename acc_of<V>::type;
+  A* y = static_cast<A*>(p.y) + row * p.incy;
+  if constexpr (std::is_same<Acc, TwoSumAcc>::value) {
+    const double s = acc.total_f64();
+    double r;
+    if (p.form == 0) {
+      r = p.alpha * s;
+      if (p.beta != 0.0) r += p.beta * static_cast<double>(*y);
+    } else {
+      r = static_cast<double>(*y) + p.alpha * s;
+    }
+    *y = static_cast<float>(r);
+  } else {
+    const A alpha = static_cast<A>(p.alpha);
+    const A beta = static_cast<A>(p.beta);
+    const A s = acc.total();
+    if (p.form == 0) {
+      A r = alpha * s;
+      if (beta != A(0)) r += beta * *y;
+      *y = r;
+    } else {
+      *y = *y + alpha * s;
+    }
+  }
+}
+
+// Adds a carried partial (rows split across merge tiles) to a finished y.
+template <typename V, typename Acc>
+__device__ __forceinline__ void add_carry(const KParams& p, int64_t row,
+                                          const Acc& acc) {
+  using A = typename acc_of<V>::type;
+  A* y = static_cast<A*>(p.y) + row * p.incy;
+  if constexpr (std::is_same<Acc, TwoSumAcc>::value) {
+    *y = static_cast<float>(static_cast<double>(*y) + p.alpha * acc.total_f64());
+  } else {
+    *y = *y + static_cast<A>(p.alpha) * acc.total();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// vector<G>: G lanes per row.
+// ---------------------------------------------------------------------------
+template <int G, typename V, typename I, bool TWOSUM>
+__global__ __launch_bounds__(kBlock) void spmv_vector_kernel(KParams p) {
+  using A = typename acc_of<V>::type;
+  using Acc = typename AccSel<V, TWOSUM>::type;
+  const int64_t blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t row = blk * (kBlock / G) + threadIdx.x / G;
+  const int lane = threadIdx.x % G;
+  if (row >= p.rows) return;  // whole G-groups leave together
+  const V* __restrict__ val = static_cast<const V*>(p.values);
+  const I* __restrict__ col = static_cast<const I*>(p.indices);
+  const A* __restrict__ x = static_cast<const A*>(p.x);
+  const int64_t b = ld_stream(p.offsets + row);
+  const int64_t e = ld_stream(p.offsets + row + 1);
+  Acc acc = Acc::zero();
+  Acc acc2 = Acc::zero();
+  int64_t k = b + lane;
+  // Two independent chains keep two gathers in flight per lane.
+  for (; k + G < e; k += 2 * G) {
+    const int64_t c0 = load_index(col, k);
+    const int64_t c1 = load_index(col, k + G);
+    const A v0 = widen<A>(ld_stream(val + k));
+    const A v1 = widen<A>(ld_stream(val + k + G));
+    acc.add_product(v0, x[c0 * p.incx]);
+    acc2.add_product(v1, x[c1 * p.incx]);
+  }
+  if (k < e) {
+    const int64_t c0 = load_index(col, k);
+    acc.add_product(widen<A>(ld_stream(val + k)), x[c0 * p.incx]);
+  }
+  acc.merge_after(acc2);
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) acc.merge_after(acc.shfl_xor(off, G));
+  if (lane == 0) finish_row<V>(p, row, acc);
+}
+
+// ---------------------------------------------------------------------------
+// ordered: one lane per row, exact serial arithmetic (FP32 / FP64 only).
+// ---------------------------------------------------------------------------
+template <typename T, typename I>
+__global__ __launch_bounds__(kBlock) void spmv_ordered_kernel(KParams p) {
+#pragma clang fp contract(off)
+  const int64_t row =
+      xcd_remap(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (row >= p.rows) return;
+  const T* __restrict__ val = static_cast<const T*>(p.values);
+  const I* __restrict__ col = static_cast<const I*>(p.indices);
+  const T* __restrict__ x = static_cast<const T*>(p.x);
+  T* y = static_cast<T*>(p.y) + row * p.incy;
+  const int64_t b = p.offsets[row];
+  const int64_t e = p.offsets[row + 1];
+  const T alpha = static_cast<T>(p.alpha);
+  if (p.form == 0) {
+    // reference_blas.cc:22-38
+    T product = T(0);
+    for (int64_t k = b; k < e; ++k) {
+      const T term = val[k] * x[static_cast<int64_t>(col[k]) * p.incx];
+      product = product + term;
+    }
+    T result = alpha * product;
+    const T beta = static_cast<T>(p.beta);
+    if (beta != T(0)) {
+      const T scaled = beta * *y;
+      result = result + scaled;
+    }
+    *y = result;
+  } else {
+    // blas.h:1653-1665: y += (alpha*v) * x per stored entry, in order.
+    T acc = *y;
+    for (int64_t k = b; k < e; ++k) {
+      const T av = alpha * val[k];
+      const T term = av * x[static_cast<int64_t>(col[k]) * p.incx];
+      acc = acc + term;
+    }
+    *y = acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// merge-path
+// ---------------------------------------------------------------------------
+constexpr int kIpt = 8;                 // merge items per thread
+constexpr int kTile = kBlock * kIpt;    // merge items per tile
+
+// Tile start coordinates on the merge path: coords[2t] = row, [2t+1] = nz.
+__global__ __launch_bounds__(kBlock) void merge_search_kernel(
+    const int64_t* __restrict__ offsets, int64_t rows, int64_t nnz,
+    int64_t ntiles, int64_t* __restrict__ coords) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (t > ntiles) return;
+  const int64_t total = rows + nnz;
+  const int64_t d = t * kTile < total ? t * kTile : total;
+  int64_t lo = d - nnz > 0 ? d - nnz : 0;
+  int64_t hi = d < rows ? d : rows;
+  // Largest prefix of row-end items that precede nz item (d - lo - 1).
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (offsets[mid + 1] <= d - mid - 1) lo = mid + 1;
+    else hi = mid;
+  }
+  coords[2 * t] = lo;
+  coords[2 * t + 1] = d - lo;
+}
+
+template <typename V, typename I, bool TWOSUM>
+__global__ __launch_bounds__(kBlock) void spmv_merge_kernel(
+    KParams p, const int64_t* __restrict__ coords, int64_t ntiles,
+    int64_t* __restrict__ carry_rows, void* __restrict__ carry_vals) {
+  using A = typename acc_of<V>::type;
+  using Acc = typename AccSel<V, TWOSUM>::type;
+  __shared__ int32_t s_rowend[kTile + 1];
+  __shared__ A s_val[kTile];
+  __shared__ float s_err[TWOSUM ? kTile : 1];
+  __shared__ int32_t s_wkey[kBlock / kWave];
+  __shared__ Acc s_wacc[kBlock / kWave];
+
+  const int64_t tile = xcd_remap(blockIdx.x, gridDim.x);
+  const int tid = threadIdx.x;
+  const int64_t i0 = coords[2 * tile], j0 = coords[2 * tile + 1];
+  const int64_t i1 = coords[2 * tile + 2], j1 = coords[2 * tile + 3];
+  const int nrows = static_cast<int>(i1 - i0);  // rows finished in this tile
+  const int nnzt = static_cast<int>(j1 - j0);
+  const int items = nrows + nnzt;
+
+  const V* __restrict__ val = static_cast<const V*>(p.values);
+  const I* __restrict__ col = static_cast<const I*>(p.indices);
+  const A* __restrict__ x = static_cast<const A*>(p.x);
+
+  // Row ends relative to j0 for rows i0..i1 (row i1 stays open; clamp).
+  for (int r = tid; r <= nrows; r += kBlock) {
+    const int64_t gr = i0 + r;
+    int64_t rel = (gr < p.rows ? ld_stream(p.offsets + gr + 1) : p.nnz) - j0;
+    if (rel > kTile + 1) rel = kTile + 1;
+    s_rowend[r] = static_cast<int32_t>(rel);
+  }
+  // Products, coalesced: lane k handles nz j0 + k + u*kBlock.
+  {
+    int64_t cidx[kIpt];
+    V vv[kIpt];
+#pragma unroll
+    for (int u = 0; u < kIpt; ++u) {
+      const int k = tid + u * kBlock;
+      if (k < nnzt) {
+        cidx[u] = load_index(col, j0 + k);
+        vv[u] = ld_stream(val + j0 + k);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kIpt; ++u) {
+      const int k = tid + u * kBlock;
+      if (k < nnzt) {
+        const A xv = x[cidx[u] * p.incx];
+        const A v = widen<A>(vv[u]);
+        if constexpr (TWOSUM) {
+#pragma clang fp contract(off)
+          const float prod = v * xv;
+          s_val[k] = prod;
+          s_err[k] = __builtin_fmaf(v, xv, -prod);
+        } else {
+          s_val[k] = v * xv;
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  // Thread-level merge-path search inside the tile.
+  const int d = tid * kIpt < items ? tid * kIpt : items;
+  int lo = d - nnzt > 0 ? d - nnzt : 0;
+  int hi = d < nrows ? d : nrows;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (s_rowend[mid] <= d - mid - 1) lo = mid + 1;
+    else hi = mid;
+  }
+  int r = lo, j = d - lo;
+  const int dend = d + kIpt < items ? d + kIpt : items;
+  Acc acc = Acc::zero();
+  int first_done = -1;
+  // Row sums land after the products: slot nnzt + row (fits: rows+nnz <= tile).
+  for (int step = d; step < dend; ++step) {
+    if (j < s_rowend[r]) {
+      if constexpr (TWOSUM) acc.add_pair(s_val[j], s_err[j]);
+      else acc.add(s_val[j]);
+      ++j;
+    } else {
+      s_val[nnzt + r] = acc.s;
+      if constexpr (TWOSUM) s_err[nnzt + r] = acc.c;
+      if (first_done < 0) first_done = r;
+      acc = Acc::zero();
+      ++r;
+    }
+  }
+  // Segmented inclusive scan of carries across the block.  key = the row
+  // still open when this thread stopped; keys never decrease with tid, so
+  // a run of equal keys is contiguous.  A thread's first finished row is
+  // always its predecessor's open row, so the predecessor's inclusive scan
+  // value is exactly the prefix that row is missing.
+  const int lane = tid % kWave;
+  const int wave = tid / kWave;
+  const int key = r;
+  Acc run = acc;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const int k_up = __shfl_up(key, off);
+    const Acc a_up = run.shfl_up(off);
+    if (lane >= off && k_up == key) {
+      Acc t = a_up;
+      t.merge_after(run);
+      run = t;
+    }
+  }
+  int prev_key = __shfl_up(key, 1);
+  Acc prev = run.shfl_up(1);
+  if (lane == kWave - 1) {
+    s_wkey[wave] = key;
+    s_wacc[wave] = run;
+  }
+  __syncthreads();
+  // Inclusive scan value at the last lane of the previous wave.
+  int in_key = -1;
+  Acc in_acc = Acc::zero();
+  for (int w = 0; w < wave; ++w) {
+    if (s_wkey[w] == in_key) {
+      in_acc.merge_after(s_wacc[w]);
+    } else {
+      in_key = s_wkey[w];
+      in_acc = s_wacc[w];
+    }
+  }
+  bool has_prev = true;
+  if (lane == 0) {
+    has_prev = wave > 0;
+    prev_key = in_key;
+    prev = in_acc;
+  } else if (prev_key == in_key) {
+    Acc t = in_acc;
+    t.merge_after(prev);
+    prev = t;
+  }
+  if (first_done >= 0 && has_prev && prev_key == first_done) {
+    Acc t = prev;
+    Acc mine = Acc::zero();
+    mine.s = s_val[nnzt + first_done];
+    if constexpr (TWOSUM) mine.c = s_err[nnzt + first_done];
+    t.merge_after(mine);
+    s_val[nnzt + first_done] = t.s;
+    if constexpr (TWOSUM) s_err[nnzt + first_done] = t.c;
+  }
+  // Tile carry-out: the open row i1 when this tile holds some of its nz.
+  if (tid == kBlock - 1) {
+    Acc tail = run;
+    if (key == in_key) {
+      Acc t = in_acc;
+      t.merge_after(run);
+      tail = t;
+    }
+    const int64_t open_begin = i1 < p.rows ? p.offsets[i1] : p.nnz;
+    if (i1 < p.rows && j1 > open_begin && j1 > j0) {
+      carry_rows[tile] = i1;
+      static_cast<Acc*>(carry_vals)[tile] = tail;
+    } else {
+      carry_rows[tile] = -1;
+    }
+  }
+  __syncthreads();
+  // Coalesced epilogue for the rows finished in this tile.
+  for (int rr = tid; rr < nrows; rr += kBlock) {
+    Acc a = Acc::zero();
+    a.s = s_val[nnzt + rr];
+    if constexpr (TWOSUM) a.c = s_err[nnzt + rr];
+    finish_row<V>(p, i0 + rr, a);
+  }
+}
+
+// Adds tile carries to rows that crossed tile boundaries.  The first tile of
+// each run of equal carry rows sums the run in tile order and adds it once.
+template <typename V, bool TWOSUM>
+__global__ __launch_bounds__(kBlock) void spmv_merge_fixup_kernel(
+    KParams p, int64_t ntiles, const int64_t* __restrict__ carry_rows,
+    const void* __restrict__ carry_vals) {
+  using Acc = typename AccSel<V, TWOSUM>::type;
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (t >= ntiles) return;
+  const int64_t row = carry_rows[t];
+  if (row < 0) return;
+  if (t > 0 && carry_rows[t - 1] == row) return;
+  const Acc* vals = static_cast<const Acc*>(carry_vals);
+  Acc sum = vals[t];
+  for (int64_t u = t + 1; u < ntiles && carry_rows[u] == row; ++u)
+    sum.merge_after(vals[u]);
+  add_carry<V>(p, row, sum);
+}
+
+// ---------------------------------------------------------------------------
+// dispatch
+// ---------------------------------------------------------------------------
+int64_t merge_tiles(const SpmvArgs& a) {
+  const int64_t total = a.a.outer + a.a.nnz;
+  return (total + kTile - 1) / kTile;
+}
+
+template <typename V, bool TWOSUM>
+size_t carry_bytes() {
+  return sizeof(typename AccSel<V, TWOSUM>::type);
+}
+
+inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+template <typename V, typename I, bool TWOSUM>
+hipError_t launch_typed(const SpmvArgs& a, const KParams& kp, void* ws,
+                        hipStream_t s) {
+  const int64_t rows = a.a.outer;
+  auto blocks_for = [](int64_t n, int64_t per) {
+    return static_cast<unsigned>((n + per - 1) / per);
+  };
+  switch (a.alg) {
+    case SpmvAlg::Ordered: {
+      if constexpr (!TWOSUM && (std::is_same<V, double>::value ||
+                                std::is_same<V, float>::value)) {
+        spmv_ordered_kernel<V, I><<<blocks_for(rows, kBlock), kBlock, 0, s>>>(kp);
+        return hipGetLastError();
+      }
+      return hipErrorInvalidValue;
+    }
+    case SpmvAlg::Merge: {
+      const int64_t nt = merge_tiles(a);
+      char* w = static_cast<char*>(ws);
+      int64_t* coords = reinterpret_cast<int64_t*>(w);
+      w += align256(sizeof(int64_t) * 2 * (nt + 1));
+      int64_t* crow = reinterpret_cast<int64_t*>(w);
+      w += align256(sizeof(int64_t) * nt);
+      void* cval = w;
+      merge_search_kernel<<<blocks_for(nt + 1, kBlock), kBlock, 0, s>>>(
+          a.a.offsets, rows, a.a.nnz, nt, coords);
+      spmv_merge_kernel<V, I, TWOSUM><<<static_cast<unsigned>(nt), kBlock, 0, s>>>(
+          kp, coords, nt, crow, cval);
+      spmv_merge_fixup_kernel<V, TWOSUM><<<blocks_for(nt, kBlock), kBlock, 0, s>>>(
+          kp, nt, crow, cval);
+      return hipGetLastError();
+    }
+#define ASC_VEC_CASE(ALG, G)                                                  \
+  case SpmvAlg::ALG:                                                          \
+    spmv_vector_kernel<G, V, I, TWOSUM>                                       \
+        <<<blocks_for(rows, kBlock / G), kBlock, 0, s>>>(kp);                 \
+    return hipGetLastError();
+    ASC_VEC_CASE(Vec1, 1)
+    ASC_VEC_CASE(Vec2, 2)
+    ASC_VEC_CASE(Vec4, 4)
+    ASC_VEC_CASE(Vec8, 8)
+    ASC_VEC_CASE(Vec16, 16)
+    ASC_VEC_CASE(Vec32, 32)
+    ASC_VEC_CASE(Vec64, 64)
+#undef ASC_VEC_CASE
+  }
+  return hipErrorInvalidValue;
+}
+
+template <typename V, bool TWOSUM>
+hipError_t launch_by_index(const SpmvArgs& a, const KParams& kp, void* ws,
+                           hipStream_t s) {
+  if (a.a.index_bytes == 4) return launch_typed<V, int32_t, TWOSUM>(a, kp, ws, s);
+  return launch_typed<V, int64_t, TWOSUM>(a, kp, ws, s);
+}
+
+template <typename V>
+size_t ws_typed(const SpmvArgs& a) {
+  if (a.alg != SpmvAlg::Merge) return 0;
+  const int64_t nt = merge_tiles(a);
+  const size_t cb = a.twosum ? carry_bytes<V, true>() : carry_bytes<V, false>();
+  return align256(sizeof(int64_t) * 2 * (nt + 1)) +
+         align256(sizeof(int64_t) * nt) + align256(cb * nt);
+}
+
+}  // namespace
+
+size_t elem_bytes(Elem e) {
+  switch (e) {
+    case Elem::F64: return 8;
+    case Elem::F32: return 4;
+    default: return 2;
+  }
+}
+
+size_t spmv_workspace_bytes(const SpmvArgs& a) {
+  switch (a.a.elem) {
+    case Elem::F64: return ws_typed<double>(a);
+    case Elem::F32: return ws_typed<float>(a);
+    case Elem::BF16: return ws_typed<bf16_t>(a);
+    case Elem::F16: return ws_typed<f16_t>(a);
+  }
+  return 0;
+}
+
+hipError_t spmv_launch(const SpmvArgs& a, void* ws, hipStream_t s) {
+  if (a.a.outer == 0) return hipSuccess;
+  KParams kp;
+  kp.offsets = a.a.offsets;
+  kp.indices = a.a.indices;
+  kp.values = a.a.values;
+  kp.rows = a.a.outer;
+  kp.nnz = a.a.nnz;
+  kp.x = a.x;
+  kp.incx = a.incx;
+  kp.y = a.y;
+  kp.incy = a.incy;
+  kp.alpha = a.alpha;
+  kp.beta = a.beta;
+  kp.form = static_cast<int>(a.form);
+  switch (a.a.elem) {
+    case Elem::F64: return launch_by_index<double, false>(a, kp, ws, s);
+    case Elem::F32:
+      return a.twosum ? launch_by_index<float, true>(a, kp, ws, s)
+                      : launch_by_index<float, false>(a, kp, ws, s);
+    case Elem::BF16:
+      return a.twosum ? launch_by_index<bf16_t, true>(a, kp, ws, s)
+                      : launch_by_index<bf16_t, false>(a, kp, ws, s);
+    case Elem::F16:
+      return a.twosum ? launch_by_index<f16_t, true>(a, kp, ws, s)
+                      : launch_by_index<f16_t, false>(a, kp, ws, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace asc_hip

@@ -1,0 +1,376 @@
+/*
+ * asc_sparse_hip.h -- C ABI of ASC::sparse_hip, an MI355X (gfx950) Sparse
+ * BLAS provider for the asc::sparse CSR/CSC/COO storage types.
+ *
+ * This is the type-erased seam that the C++ facet
+ * (include/asc/sparse/providers/hip.h) and foreign callers (ctypes, cgo, JNI)
+ * bind.  It replaces the reference's exported type-erased provider functions
+ * (include/asc/sparse/providers/cuda.h:212-253) one for one where the
+ * reference has a counterpart, and adds the device conversions the reference
+ * provider lacks.
+ *
+ * Conventions
+ *  - Every function returns asc_hip_status.  `code` carries the numeric value
+ *    of asc::ErrorCode (include/asc/core/status.h:35-55); `native` carries the
+ *    hipError_t when `provider` is ASC_HIP_PROVIDER_HIP.  A human-readable
+ *    message for the calling thread's last failure is available from
+ *    asc_sparse_hip_last_message().  Nothing throws across this boundary.
+ *  - Every operand is validated before anything is enqueued; a failing call
+ *    leaves every output untouched (reference operations.cc:349-398).
+ *  - Operations are stream-ordered on the context's HIP stream and return
+ *    without waiting; record an event or synchronize the context to observe
+ *    completion (reference runtime.cc:467-482).
+ *  - Operations never allocate device memory.  Temporaries come from a
+ *    caller-owned workspace whose size is queried first (reference
+ *    providers/cuda.h:1758-1771, docs/modules/sparse.md:481-486).
+ *  - Device matrices must be "trusted": produced by asc_sparse_hip_clone_* or
+ *    a device conversion of this library (reference compressed.h:759-770,
+ *    operations.cc:163-166).
+ *  - One context must not be used from two threads at once; independent
+ *    contexts may run concurrently (reference docs/modules/sparse.md:440-442).
+ */
+#ifndef ASC_SPARSE_HIP_H_
+#define ASC_SPARSE_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status ------------------------------------------------------------ */
+
+/* asc::ErrorCode values (include/asc/core/status.h:35-55). */
+enum {
+  ASC_HIP_OK = 0,
+  ASC_HIP_INVALID_ARGUMENT = 1,
+  ASC_HIP_SHAPE = 2,
+  ASC_HIP_INDEX = 3,
+  ASC_HIP_OVERFLOW = 4,
+  ASC_HIP_INVALID_STATE = 5,
+  ASC_HIP_ALLOCATION = 6,
+  ASC_HIP_MEMORY_ACCESS = 7,
+  ASC_HIP_MEMORY_TRANSFER = 8,
+  ASC_HIP_UNSUPPORTED = 9,
+  ASC_HIP_UNAVAILABLE = 10,
+  ASC_HIP_PROVIDER = 11,
+  ASC_HIP_NUMERICAL = 12,
+  ASC_HIP_INTERNAL = 18
+};
+
+enum { ASC_HIP_PROVIDER_NONE = 0, ASC_HIP_PROVIDER_HIP = 1 };
+
+typedef struct asc_hip_status {
+  int32_t code;     /* asc::ErrorCode */
+  int32_t provider; /* ASC_HIP_PROVIDER_* */
+  int64_t native;   /* hipError_t when provider == ASC_HIP_PROVIDER_HIP */
+} asc_hip_status;
+
+/* ---- vocabulary (values mirror the reference enums) -------------------- */
+
+/* internal_sparse_cuda::ElementKind (providers/cuda.h:103-108), extended by
+ * the storage-only reduced precisions of this provider. */
+enum {
+  ASC_HIP_F32 = 0,
+  ASC_HIP_F64 = 1,
+  ASC_HIP_C64 = 2,  /* reserved: complex float */
+  ASC_HIP_C128 = 3, /* reserved: complex double */
+  ASC_HIP_BF16 = 4, /* storage only; arithmetic in FP32 */
+  ASC_HIP_F16 = 5   /* storage only; arithmetic in FP32 */
+};
+
+/* internal_sparse_cuda::FormatKind (providers/cuda.h:125-129). */
+enum { ASC_HIP_COO = 0, ASC_HIP_CSR = 1, ASC_HIP_CSC = 2 };
+
+/* asc::MemorySpace (include/asc/core/memory.h:30-35). */
+enum {
+  ASC_HIP_HOST = 0,
+  ASC_HIP_PINNED_HOST = 1,
+  ASC_HIP_DEVICE = 2,
+  ASC_HIP_MANAGED = 3
+};
+
+/* asc::SparseBlasTranspose (include/asc/sparse/blas.h:88-92). */
+enum { ASC_HIP_OP_N = 0, ASC_HIP_OP_T = 1, ASC_HIP_OP_H = 2 };
+
+/* asc::SparseBlasLayout (include/asc/sparse/blas.h:103-106). */
+enum { ASC_HIP_COLUMN_MAJOR = 0, ASC_HIP_ROW_MAJOR = 1 };
+
+/* asc::Determinism (include/asc/core/execution.h:122-125). */
+enum { ASC_HIP_DETERMINISTIC = 0, ASC_HIP_BACKEND_DEFAULT = 1 };
+
+/* asc::DuplicatePolicy / ExplicitZeroPolicy (coordinate.h:60-78). */
+enum { ASC_HIP_DUPLICATE_REJECT = 0, ASC_HIP_DUPLICATE_SUM = 1 };
+enum { ASC_HIP_ZERO_KEEP = 0, ASC_HIP_ZERO_DROP = 1 };
+
+/*
+ * Sparse operand, after internal_sparse_cuda::SparseDescriptor
+ * (providers/cuda.h:143-156).  CSR: outer = rows; CSC: outer = cols.
+ * COO: `outer_offsets` points at the interleaved (row, col) int64
+ * coordinates and `inner_indices` is unused.  Offsets are int64
+ * (offset_bytes = 8; 4 is accepted for offsets that fit int32); inner indices
+ * are int64 as in the reference or int32 after an explicit narrowing clone.
+ */
+typedef struct asc_hip_sparse_desc {
+  const void* outer_offsets;
+  const void* inner_indices;
+  const void* values;
+  int64_t rows;
+  int64_t cols;
+  int64_t nnz;
+  uint8_t format;       /* ASC_HIP_COO / CSR / CSC */
+  uint8_t element_kind; /* ASC_HIP_F32 ... */
+  uint8_t index_bytes;  /* 4 or 8 */
+  uint8_t offset_bytes; /* 4 or 8 */
+  uint8_t memory_space; /* ASC_HIP_HOST ... */
+  uint8_t trusted;      /* canonical structure vouched for by this library */
+  uint8_t reserved[2];
+} asc_hip_sparse_desc;
+
+/* Strided vector, after VectorDescriptor (providers/cuda.h:158-167).
+ * `data` addresses logical element 0; element i is data[i * stride]. */
+typedef struct asc_hip_vector_desc {
+  const void* data;
+  int64_t extent;
+  int64_t stride;
+  uint8_t element_kind;
+  uint8_t memory_space;
+  uint8_t writable;
+  uint8_t reserved[5];
+} asc_hip_vector_desc;
+
+/* Dense block, after MatrixDescriptor (providers/cuda.h:180-191). */
+typedef struct asc_hip_matrix_desc {
+  const void* data;
+  int64_t rows;
+  int64_t cols;
+  int64_t leading_dimension;
+  uint8_t layout; /* ASC_HIP_COLUMN_MAJOR / ROW_MAJOR */
+  uint8_t element_kind;
+  uint8_t memory_space;
+  uint8_t writable;
+  uint8_t reserved[4];
+} asc_hip_matrix_desc;
+
+/* ScalarValue (providers/cuda.h:119-122). */
+typedef struct asc_hip_scalar {
+  double real;
+  double imaginary;
+} asc_hip_scalar;
+
+/*
+ * SpMV/SpMM algorithm choice.  ASC_HIP_ALG_AUTO uses a tuned or learned
+ * choice when the context holds one for the matrix and the merge-path kernel
+ * otherwise.  ASC_HIP_ALG_ORDERED reproduces the serial reference's
+ * left-to-right per-row summation bit for bit (no FMA, no reassociation).
+ * The remaining values name one kernel variant each (the autotuner's space).
+ */
+enum {
+  ASC_HIP_ALG_AUTO = 0,
+  ASC_HIP_ALG_ORDERED = 1,
+  ASC_HIP_ALG_MERGE = 2,    /* merge-path: rows+nnz split evenly per tile */
+  ASC_HIP_ALG_VECTOR1 = 3,  /* 1 lane per row */
+  ASC_HIP_ALG_VECTOR2 = 4,  /* 2 lanes per row */
+  ASC_HIP_ALG_VECTOR4 = 5,
+  ASC_HIP_ALG_VECTOR8 = 6,
+  ASC_HIP_ALG_VECTOR16 = 7,
+  ASC_HIP_ALG_VECTOR32 = 8,
+  ASC_HIP_ALG_VECTOR64 = 9, /* one wave64 per row */
+  ASC_HIP_ALG_COUNT = 10
+};
+
+/* Accumulation for FP32/BF16/F16-stored values (FP64 always accumulates in
+ * FP64).  TWOSUM is compensated (Knuth TwoSum plus the FMA product error). */
+enum { ASC_HIP_ACC_PLAIN = 0, ASC_HIP_ACC_TWOSUM = 1 };
+
+typedef struct asc_hip_spmv_options {
+  uint8_t algorithm;  /* ASC_HIP_ALG_* */
+  uint8_t accumulate; /* ASC_HIP_ACC_* */
+  uint8_t reserved[6];
+} asc_hip_spmv_options;
+
+typedef struct asc_hip_context asc_hip_context;
+typedef struct asc_hip_event asc_hip_event;
+
+/* ---- context, memory, events (core_hip) -------------------------------- */
+
+/* Binds `device` and a stream.  `stream` is a hipStream_t to borrow, or NULL
+ * to create a non-blocking stream owned by the context.  Mirrors
+ * SparseCudaContext::Create (providers/cuda.h:514; context.cc:69-113) over
+ * CreateCudaExecutionContext (src/core/cuda/runtime.cc:434-465). */
+asc_hip_status asc_sparse_hip_context_create(int32_t device, void* stream,
+                                             uint8_t determinism,
+                                             asc_hip_context** out);
+asc_hip_status asc_sparse_hip_context_destroy(asc_hip_context* context);
+asc_hip_status asc_sparse_hip_context_stream(asc_hip_context* context,
+                                             void** out_stream);
+asc_hip_status asc_sparse_hip_context_synchronize(asc_hip_context* context);
+
+/* Message of the calling thread's most recent failed call ("" if none). */
+const char* asc_sparse_hip_last_message(void);
+
+/* Number of visible HIP devices (0 without a GPU; never fails). */
+int32_t asc_sparse_hip_device_count(void);
+
+/* Device / pinned-host allocation, after CudaMemoryResource
+ * (include/asc/core/providers/cuda.h:39-120).  Zero bytes yields NULL. */
+asc_hip_status asc_sparse_hip_allocate(asc_hip_context* context, size_t bytes,
+                                       uint8_t memory_space, void** out);
+asc_hip_status asc_sparse_hip_deallocate(asc_hip_context* context, void* ptr,
+                                         uint8_t memory_space);
+/* Stream-ordered copy between any host/device pointers (CopyBytes,
+ * src/core/cuda/runtime.cc:226-253). */
+asc_hip_status asc_sparse_hip_copy(asc_hip_context* context, void* dst,
+                                   const void* src, size_t bytes);
+/* Stream-ordered byte fill (a kernel, safe under graph capture). */
+asc_hip_status asc_sparse_hip_fill(asc_hip_context* context, void* dst,
+                                   uint8_t byte, size_t bytes);
+
+/* Completion events (CompletionEvent, include/asc/core/execution.h:246-369). */
+asc_hip_status asc_sparse_hip_event_record(asc_hip_context* context,
+                                           asc_hip_event** out);
+asc_hip_status asc_sparse_hip_event_wait(asc_hip_event* event);
+asc_hip_status asc_sparse_hip_event_elapsed_ms(asc_hip_event* start,
+                                               asc_hip_event* stop,
+                                               float* out_ms);
+asc_hip_status asc_sparse_hip_event_destroy(asc_hip_event* event);
+
+/* ---- staging ------------------------------------------------------------ */
+
+/*
+ * Copies a canonical host CSR or CSC matrix into caller-provided device
+ * buffers (rows+1 or cols+1 offsets, nnz indices, nnz values) and fills
+ * `out` with a trusted device descriptor.  `index_bytes` 8 keeps the
+ * reference's int64 indices; 4 is the explicit, validated narrowing clone
+ * (fails with ASC_HIP_OVERFLOW if an extent exceeds INT32_MAX).  Replaces
+ * CloneCsrErased (providers/cuda.h:212-214; operations.cc:696-800).
+ */
+asc_hip_status asc_sparse_hip_clone_compressed(
+    asc_hip_context* context, const asc_hip_sparse_desc* host_source,
+    uint8_t index_bytes, void* device_offsets, void* device_indices,
+    void* device_values, asc_hip_sparse_desc* out);
+
+/* Canonical host COO (interleaved int64 coordinates) into device buffers
+ * (2*nnz int64 coordinates, nnz values); `out` is a trusted device COO. */
+asc_hip_status asc_sparse_hip_clone_coordinate(
+    asc_hip_context* context, const asc_hip_sparse_desc* host_source,
+    void* device_coordinates, void* device_values, asc_hip_sparse_desc* out);
+
+/* Device-side narrowing of a trusted device matrix's int64 inner indices
+ * into `device_indices32` (nnz int32).  `out` shares offsets and values. */
+asc_hip_status asc_sparse_hip_narrow_indices(asc_hip_context* context,
+                                             const asc_hip_sparse_desc* source,
+                                             void* device_indices32,
+                                             asc_hip_sparse_desc* out);
+
+/* Device-side value-type change of a trusted matrix (FP64 -> FP32/BF16/F16,
+ * round to nearest even) into `device_values`; `out` shares the structure. */
+asc_hip_status asc_sparse_hip_convert_values(asc_hip_context* context,
+                                             const asc_hip_sparse_desc* source,
+                                             uint8_t element_kind,
+                                             void* device_values,
+                                             asc_hip_sparse_desc* out);
+
+/* ---- SpMV --------------------------------------------------------------- */
+
+/*
+ * Compat CSR SpMV  y = alpha*A*x + beta*y  (y is not read when beta == 0),
+ * the device form of asc::Spmv(ctx, alpha, CsrView, x, beta, y)
+ * (blas.h:2031-2081; serial algorithm reference_blas.cc:9-41).  Replaces
+ * CsrSpmvWorkspaceSizeErased / CsrSpmvErased (providers/cuda.h:220-227).
+ * Element kinds: A F64 with x,y F64; A F32/BF16/F16 with x,y F32.
+ * `options` may be NULL (ASC_HIP_ALG_AUTO, ASC_HIP_ACC_PLAIN).
+ */
+asc_hip_status asc_sparse_hip_csr_spmv_workspace_size(
+    asc_hip_context* context, const asc_hip_sparse_desc* matrix,
+    const asc_hip_vector_desc* x, const asc_hip_vector_desc* y,
+    const asc_hip_spmv_options* options, size_t* out_bytes);
+asc_hip_status asc_sparse_hip_csr_spmv(
+    asc_hip_context* context, double alpha, const asc_hip_sparse_desc* matrix,
+    const asc_hip_vector_desc* x, double beta, const asc_hip_vector_desc* y,
+    void* workspace, size_t workspace_bytes,
+    const asc_hip_spmv_options* options);
+
+/*
+ * Standardized SpMV  y += alpha*op(A)*x  for CSR or CSC, real F32/F64
+ * (blas.h:1607-1667).  alpha == 0 returns without reading anything.
+ * CSR with op N and CSC with op T/H run directly; the other two combinations
+ * transpose the structure into the workspace first.  Replaces
+ * StandardSpmvErased (providers/cuda.h:240-243).
+ */
+asc_hip_status asc_sparse_hip_standard_spmv_workspace_size(
+    asc_hip_context* context, uint8_t transpose,
+    const asc_hip_sparse_desc* matrix, const asc_hip_vector_desc* x,
+    const asc_hip_vector_desc* y, const asc_hip_spmv_options* options,
+    size_t* out_bytes);
+asc_hip_status asc_sparse_hip_standard_spmv(
+    asc_hip_context* context, uint8_t transpose, asc_hip_scalar alpha,
+    const asc_hip_sparse_desc* matrix, const asc_hip_vector_desc* x,
+    const asc_hip_vector_desc* y, void* workspace, size_t workspace_bytes,
+    const asc_hip_spmv_options* options);
+
+/* ---- SpMM --------------------------------------------------------------- */
+
+/*
+ * Standardized SpMM  C += alpha*op(A)*B  against a dense block in either
+ * layout (blas.h:1692-1750).  Every C(i,j) is accumulated in stored order
+ * without FMA, so the result equals the serial reference bit for bit.
+ * Replaces StandardSpmmErased (providers/cuda.h:245-248).
+ */
+asc_hip_status asc_sparse_hip_standard_spmm_workspace_size(
+    asc_hip_context* context, uint8_t transpose,
+    const asc_hip_sparse_desc* matrix, const asc_hip_matrix_desc* b,
+    const asc_hip_matrix_desc* c, size_t* out_bytes);
+asc_hip_status asc_sparse_hip_standard_spmm(
+    asc_hip_context* context, uint8_t transpose, asc_hip_scalar alpha,
+    const asc_hip_sparse_desc* matrix, const asc_hip_matrix_desc* b,
+    const asc_hip_matrix_desc* c, void* workspace, size_t workspace_bytes);
+
+/* ---- conversions --------------------------------------------------------- */
+
+/*
+ * Canonical device COO -> CSR (out_format ASC_HIP_CSR) or CSC
+ * (ASC_HIP_CSC), bit-exact with ConvertToCsr / ConvertToCsc
+ * (compressed.h:1675-1708 -> FromCoordinate :1082-1131).  Outputs are
+ * caller-provided device buffers (outer+1 offsets int64, nnz indices of
+ * `index_bytes`, nnz values); `out` receives a trusted descriptor.  The
+ * reference provider has no device conversion (docs/modules/sparse.md:592).
+ */
+asc_hip_status asc_sparse_hip_convert_coordinate_workspace_size(
+    asc_hip_context* context, const asc_hip_sparse_desc* source,
+    uint8_t out_format, uint8_t index_bytes, size_t* out_bytes);
+asc_hip_status asc_sparse_hip_convert_coordinate(
+    asc_hip_context* context, const asc_hip_sparse_desc* source,
+    uint8_t out_format, uint8_t index_bytes, void* out_offsets,
+    void* out_indices, void* out_values, asc_hip_sparse_desc* out,
+    void* workspace, size_t workspace_bytes);
+
+/*
+ * Trusted device CSR -> CSC or CSC -> CSR, bit-exact with
+ * ConvertToCsc(ctx, CsrView) / ConvertToCsr(ctx, CscView)
+ * (compressed.h:1727-1760 -> FromCompressed :1151-1221), by a stable device
+ * radix sort on the inner index.
+ */
+asc_hip_status asc_sparse_hip_convert_compressed_workspace_size(
+    asc_hip_context* context, const asc_hip_sparse_desc* source,
+    size_t* out_bytes);
+asc_hip_status asc_sparse_hip_convert_compressed(
+    asc_hip_context* context, const asc_hip_sparse_desc* source,
+    void* out_offsets, void* out_indices, void* out_values,
+    asc_hip_sparse_desc* out, void* workspace, size_t workspace_bytes);
+
+/* ---- host-forced fault injection (tests) -------------------------------- */
+
+/* Makes the `countdown`-th next allocation (0 = the very next) or event
+ * record fail as if HIP had failed; -1 disarms.  Mirrors the reference's
+ * TrackingMemoryResource::FailOnAllocation (tests/sparse/test_resources.h:
+ * 47-53) and its CUDA runtime test hooks (src/core/cuda/runtime.cc:53-70). */
+enum { ASC_HIP_INJECT_ALLOCATION = 0, ASC_HIP_INJECT_EVENT = 1 };
+asc_hip_status asc_sparse_hip_test_inject(uint32_t what, int32_t countdown);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ASC_SPARSE_HIP_H_ */

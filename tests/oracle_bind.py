@@ -1,0 +1,133 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so) for the tests.
+
+Only tests, the smoke check and the benchmark's CPU-baseline leg load the
+oracle, and only as a checker.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE = os.path.join(ROOT, "oracle", "liboracle.so")
+REF = os.path.join(ROOT, "oracle", "_ref", "libref_driver.so")
+
+_lib = None
+I64, F64P, F32P, VP = C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE):
+            raise ImportError("oracle/liboracle.so is missing: run __graft_entry__.build()")
+        L = C.CDLL(ORACLE)
+        L.oracle_validate_compressed.argtypes = [I64, I64, VP, VP, I64]
+        L.oracle_compat_spmv_f64.argtypes = [I64, VP, VP, VP, C.c_double, VP, I64, C.c_double, VP, I64]
+        L.oracle_compat_spmv_f32.argtypes = [I64, VP, VP, VP, C.c_float, VP, I64, C.c_float, VP, I64]
+        L.oracle_std_spmv_f64.argtypes = [C.c_int, C.c_int, I64, I64, VP, VP, VP, C.c_double, VP, I64, VP, I64]
+        L.oracle_std_spmv_f32.argtypes = [C.c_int, C.c_int, I64, I64, VP, VP, VP, C.c_float, VP, I64, VP, I64]
+        L.oracle_std_spmm_f64.argtypes = [C.c_int, C.c_int, I64, I64, VP, VP, VP, C.c_double, VP, C.c_int, I64, I64, VP, C.c_int, I64]
+        L.oracle_std_spmm_f32.argtypes = [C.c_int, C.c_int, I64, I64, VP, VP, VP, C.c_float, VP, C.c_int, I64, I64, VP, C.c_int, I64]
+        L.oracle_abs_spmv_f64.argtypes = [I64, VP, VP, VP, VP, VP]
+        L.oracle_abs_spmv_f64.restype = None
+        L.oracle_coo_to_compressed_faithful.argtypes = [C.c_int, I64, I64, I64, VP, VP, C.c_int, VP, VP, VP]
+        L.oracle_coo_to_compressed_fast.argtypes = [C.c_int, I64, I64, I64, VP, VP, C.c_int, VP, VP, VP]
+        L.oracle_transpose_faithful.argtypes = [C.c_int, I64, I64, VP, VP, VP, C.c_int, VP, VP, VP]
+        L.oracle_transpose_fast.argtypes = [C.c_int, I64, I64, VP, VP, VP, C.c_int, VP, VP, VP]
+        L.oracle_compressed_to_coo.argtypes = [C.c_int, I64, VP, VP, VP, C.c_int, VP, VP]
+        L.oracle_coo_finalize_f64.argtypes = [I64, VP, VP, C.c_int, C.c_int, VP, VP, VP]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data if a is not None and a.size else None
+
+
+def i64(a):
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
+def compat_spmv(offsets, indices, values, alpha, x, beta, y, incx=1, incy=1):
+    """y (modified in place) = alpha*A*x (+ beta*y when beta != 0)."""
+    offsets, indices = i64(offsets), i64(indices)
+    rows = offsets.shape[0] - 1
+    if values.dtype == np.float64:
+        lib().oracle_compat_spmv_f64(rows, _p(offsets), _p(indices), _p(values), alpha, _p(x), incx, beta, _p(y), incy)
+    else:
+        lib().oracle_compat_spmv_f32(rows, _p(offsets), _p(indices), _p(values), alpha, _p(x), incx, beta, _p(y), incy)
+    return y
+
+
+def std_spmv(fmt, op, rows, cols, offsets, indices, values, alpha, x, y, incx=1, incy=1):
+    offsets, indices = i64(offsets), i64(indices)
+    f = lib().oracle_std_spmv_f64 if values.dtype == np.float64 else lib().oracle_std_spmv_f32
+    f(fmt, op, rows, cols, _p(offsets), _p(indices), _p(values), alpha, _p(x), incx, _p(y), incy)
+    return y
+
+
+def std_spmm(fmt, op, rows, cols, offsets, indices, values, alpha, b, b_layout, ldb, k, c, c_layout, ldc):
+    offsets, indices = i64(offsets), i64(indices)
+    f = lib().oracle_std_spmm_f64 if values.dtype == np.float64 else lib().oracle_std_spmm_f32
+    f(fmt, op, rows, cols, _p(offsets), _p(indices), _p(values), alpha, _p(b), b_layout, ldb, k,
+      _p(c), c_layout, ldc)
+    return c
+
+
+def abs_spmv(offsets, indices, values, x):
+    offsets, indices = i64(offsets), i64(indices)
+    out = np.empty(offsets.shape[0] - 1)
+    lib().oracle_abs_spmv_f64(out.shape[0], _p(offsets), _p(indices), _p(values), _p(x), _p(out))
+    return out
+
+
+def coo_to_compressed(fmt, rows, cols, coords, values, faithful=False):
+    coords = i64(coords).reshape(-1)
+    nnz = values.shape[0]
+    outer = rows if fmt == 0 else cols
+    off = np.empty(outer + 1, dtype=np.int64)
+    idx = np.empty(nnz, dtype=np.int64)
+    val = np.empty_like(values)
+    f = lib().oracle_coo_to_compressed_faithful if faithful else lib().oracle_coo_to_compressed_fast
+    f(fmt, rows, cols, nnz, _p(coords), _p(values), values.itemsize, _p(off), _p(idx), _p(val))
+    return off, idx, val
+
+
+def transpose(src_fmt, rows, cols, offsets, indices, values, faithful=False):
+    offsets, indices = i64(offsets), i64(indices)
+    out_outer = cols if src_fmt == 0 else rows
+    nnz = values.shape[0]
+    off = np.empty(out_outer + 1, dtype=np.int64)
+    idx = np.empty(nnz, dtype=np.int64)
+    val = np.empty_like(values)
+    f = lib().oracle_transpose_faithful if faithful else lib().oracle_transpose_fast
+    f(src_fmt, rows, cols, _p(offsets), _p(indices), _p(values), values.itemsize, _p(off), _p(idx), _p(val))
+    return off, idx, val
+
+
+def compressed_to_coo(src_fmt, outer, offsets, indices, values):
+    offsets, indices = i64(offsets), i64(indices)
+    nnz = values.shape[0]
+    coords = np.empty(2 * nnz, dtype=np.int64)
+    val = np.empty_like(values)
+    lib().oracle_compressed_to_coo(src_fmt, outer, _p(offsets), _p(indices), _p(values), values.itemsize,
+                                   _p(coords), _p(val))
+    return coords, val
+
+
+def finalize(coords, values, duplicate_policy, zero_policy):
+    coords = i64(coords).reshape(-1)
+    values = np.ascontiguousarray(values, dtype=np.float64)
+    nnz = values.shape[0]
+    oc = np.empty(max(2 * nnz, 2), dtype=np.int64)
+    ov = np.empty(max(nnz, 1), dtype=np.float64)
+    n = C.c_int64()
+    rc = lib().oracle_coo_finalize_f64(nnz, _p(coords), _p(values), duplicate_policy, zero_policy,
+                                       _p(oc), _p(ov), C.byref(n))
+    return rc, oc[: 2 * n.value], ov[: n.value]
+
+
+def validate(outer, inner, offsets, indices, nnz):
+    offsets, indices = i64(offsets), i64(indices)
+    return lib().oracle_validate_compressed(outer, inner, _p(offsets), _p(indices), nnz)

@@ -1,0 +1,205 @@
+"""C ABI checks that need no GPU.
+
+The library loads and exports every function include/asc_sparse_hip.h
+declares; a validation-only context (device -1) runs the full rejection
+matrix -- the analogue of the reference's CPU-side provider rejection tests
+(tests/sparse/blas_test.cc:298-443, sparse_cuda_test.cc:1010-1093) -- and
+answers UNAVAILABLE only where work would be enqueued.
+"""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import asc_sparse_hip as H
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "asc_sparse_hip.h")).read()
+    return sorted(set(re.findall(r"\b(asc_sparse_hip_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_function():
+    lib = H.load()
+    names = header_functions()
+    assert len(names) >= 25
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(names) <= set(H.exported_symbols())
+
+
+def test_device_count_without_gpu_is_zero_or_more():
+    assert H.device_count() >= 0
+
+
+# ---- rejection matrix on a validation-only context ---------------------------------
+
+FAKE = 0x10000  # plausible, never dereferenced: validation-only contexts do not touch memory
+
+
+def dev_csr(rows=3, cols=4, nnz=5, kind=H.F64, trusted=1, space=H.DEVICE, fmt=H.CSR,
+            index_bytes=8, base=FAKE):
+    return H.SparseDesc(base, base + 0x1000, base + 0x2000, rows, cols, nnz, fmt, kind,
+                        index_bytes, 8, space, trusted)
+
+
+def dev_vec(n, kind=H.F64, base=0x100000, stride=1, writable=1):
+    return H.VectorDesc(base, n, stride, kind, H.DEVICE, writable)
+
+
+def spmv_code(vctx, a, x, y, alg=H.ALG_AUTO, acc=H.ACC_PLAIN, ws=None, ws_bytes=0):
+    opt = H.options(alg, acc)
+    return vctx.lib.asc_sparse_hip_csr_spmv(vctx.handle, 1.0, C.byref(a), C.byref(x), 0.0,
+                                            C.byref(y), ws, ws_bytes, C.byref(opt)).code
+
+
+def test_valid_call_reaches_enqueue(vctx):
+    assert spmv_code(vctx, dev_csr(), dev_vec(4), dev_vec(3, base=0x200000),
+                     alg=H.ALG_VECTOR[8]) == H.UNAVAILABLE
+
+
+def test_rejects_untrusted_structure(vctx):
+    assert spmv_code(vctx, dev_csr(trusted=0), dev_vec(4), dev_vec(3, base=0x200000)) == H.INVALID_ARGUMENT
+
+
+def test_rejects_host_matrix(vctx):
+    assert spmv_code(vctx, dev_csr(space=H.HOST), dev_vec(4), dev_vec(3, base=0x200000)) == H.MEMORY_ACCESS
+
+
+def test_rejects_shape_mismatch(vctx):
+    assert spmv_code(vctx, dev_csr(), dev_vec(5), dev_vec(3, base=0x200000)) == H.SHAPE
+    assert spmv_code(vctx, dev_csr(), dev_vec(4), dev_vec(2, base=0x200000)) == H.SHAPE
+
+
+def test_rejects_nnz_beyond_shape(vctx):
+    assert spmv_code(vctx, dev_csr(nnz=13), dev_vec(4), dev_vec(3, base=0x200000)) == H.SHAPE
+
+
+def test_rejects_output_overlap(vctx):
+    a = dev_csr()
+    # y placed over the values array
+    assert spmv_code(vctx, a, dev_vec(4), dev_vec(3, base=FAKE + 0x2000)) == H.INVALID_ARGUMENT
+    # y overlapping x
+    assert spmv_code(vctx, a, dev_vec(4), dev_vec(3, base=0x100008)) == H.INVALID_ARGUMENT
+
+
+def test_rejects_wrong_vector_kind_and_csc(vctx):
+    assert spmv_code(vctx, dev_csr(), dev_vec(4, kind=H.F32), dev_vec(3, base=0x200000)) == H.INVALID_ARGUMENT
+    assert spmv_code(vctx, dev_csr(fmt=H.CSC), dev_vec(4), dev_vec(3, base=0x200000)) == H.INVALID_ARGUMENT
+
+
+def test_rejects_readonly_output_and_zero_stride(vctx):
+    assert spmv_code(vctx, dev_csr(), dev_vec(4), dev_vec(3, base=0x200000, writable=0)) == H.INVALID_ARGUMENT
+    assert spmv_code(vctx, dev_csr(), dev_vec(4, stride=0), dev_vec(3, base=0x200000)) == H.INVALID_ARGUMENT
+
+
+def test_rejects_unknown_algorithm_and_bad_modes(vctx):
+    a, x, y = dev_csr(), dev_vec(4), dev_vec(3, base=0x200000)
+    assert spmv_code(vctx, a, x, y, alg=42) == H.INVALID_ARGUMENT
+    assert spmv_code(vctx, a, x, y, acc=H.ACC_TWOSUM) == H.UNSUPPORTED  # FP64 storage
+    b = dev_csr(kind=H.BF16)
+    xf, yf = dev_vec(4, kind=H.F32), dev_vec(3, kind=H.F32, base=0x200000)
+    assert spmv_code(vctx, b, xf, yf, alg=H.ALG_ORDERED) == H.UNSUPPORTED
+    assert spmv_code(vctx, b, xf, yf, alg=H.ALG_VECTOR[4], acc=H.ACC_TWOSUM) == H.UNAVAILABLE
+
+
+def test_merge_requires_workspace(vctx):
+    a, x, y = dev_csr(), dev_vec(4), dev_vec(3, base=0x200000)
+    need = C.c_size_t()
+    opt = H.options(H.ALG_MERGE)
+    assert vctx.lib.asc_sparse_hip_csr_spmv_workspace_size(
+        vctx.handle, C.byref(a), C.byref(x), C.byref(y), C.byref(opt), C.byref(need)).code == H.OK
+    assert need.value > 0
+    assert spmv_code(vctx, a, x, y, alg=H.ALG_MERGE) == H.INVALID_ARGUMENT
+    assert spmv_code(vctx, a, x, y, alg=H.ALG_MERGE, ws=0x400000, ws_bytes=need.value - 1) == H.INVALID_ARGUMENT
+    assert spmv_code(vctx, a, x, y, alg=H.ALG_MERGE, ws=0x400000 + 8, ws_bytes=need.value) == H.INVALID_ARGUMENT
+    assert spmv_code(vctx, a, x, y, alg=H.ALG_MERGE, ws=0x400000, ws_bytes=need.value) == H.UNAVAILABLE
+    # workspace overlapping the output
+    assert spmv_code(vctx, a, x, y, alg=H.ALG_MERGE, ws=0x200000, ws_bytes=need.value) == H.INVALID_ARGUMENT
+
+
+def test_int32_indices_cannot_address_large_extent(vctx):
+    a = dev_csr(rows=3, cols=(1 << 31) + 5, nnz=5, index_bytes=4)
+    assert spmv_code(vctx, a, dev_vec((1 << 31) + 5), dev_vec(3, base=0x200000)) == H.OVERFLOW
+
+
+def test_null_and_dead_contexts():
+    lib = H.load()
+    a, x, y = dev_csr(), dev_vec(4), dev_vec(3, base=0x200000)
+    st = lib.asc_sparse_hip_csr_spmv(None, 1.0, C.byref(a), C.byref(x), 0.0, C.byref(y), None, 0, None)
+    assert st.code == H.INVALID_ARGUMENT
+    c = H.Context(-1)
+    c.close()
+
+
+def test_standard_spmv_validation(vctx):
+    a = dev_csr()
+    x, y = dev_vec(3), dev_vec(4, base=0x200000)  # op T: x has rows, y has cols
+    opt = H.options(H.ALG_ORDERED)
+    need = C.c_size_t()
+    st = vctx.lib.asc_sparse_hip_standard_spmv_workspace_size(
+        vctx.handle, H.OP_T, C.byref(a), C.byref(x), C.byref(y), C.byref(opt), C.byref(need))
+    assert st.code == H.OK and need.value > 0  # CSR with op T transposes in the workspace
+    st = vctx.lib.asc_sparse_hip_standard_spmv(vctx.handle, 7, H.Scalar(1, 0), C.byref(a),
+                                               C.byref(x), C.byref(y), None, 0, C.byref(opt))
+    assert st.code == H.INVALID_ARGUMENT
+    st = vctx.lib.asc_sparse_hip_standard_spmv(vctx.handle, H.OP_N, H.Scalar(1, 0), C.byref(a),
+                                               C.byref(x), C.byref(y), None, 0, C.byref(opt))
+    assert st.code == H.SHAPE
+
+
+def test_clone_validates_host_structure(vctx):
+    off = np.array([0, 2, 4, 5], np.int64)
+    bad = np.array([3, 1, 0, 2, 1], np.int64)  # first row unsorted
+    good = np.array([1, 3, 0, 2, 1], np.int64)
+    vals = np.ones(5)
+    out = H.SparseDesc()
+    for idx, code in ((bad, H.INVALID_ARGUMENT), (good, H.UNAVAILABLE)):
+        src = H.host_sparse(H.CSR, 3, 4, off, idx, vals, H.F64)
+        st = vctx.lib.asc_sparse_hip_clone_compressed(vctx.handle, C.byref(src), 4, FAKE,
+                                                      FAKE + 0x100, FAKE + 0x200, C.byref(out))
+        assert st.code == code
+    oob = np.array([1, 4, 0, 2, 1], np.int64)
+    src = H.host_sparse(H.CSR, 3, 4, off, oob, vals, H.F64)
+    st = vctx.lib.asc_sparse_hip_clone_compressed(vctx.handle, C.byref(src), 8, FAKE,
+                                                  FAKE + 0x100, FAKE + 0x200, C.byref(out))
+    assert st.code == H.INDEX
+    # overlapping destinations
+    src = H.host_sparse(H.CSR, 3, 4, off, good, vals, H.F64)
+    st = vctx.lib.asc_sparse_hip_clone_compressed(vctx.handle, C.byref(src), 8, FAKE,
+                                                  FAKE + 8, FAKE + 0x200, C.byref(out))
+    assert st.code == H.INVALID_ARGUMENT
+
+
+def test_conversion_validation(vctx):
+    coo = H.SparseDesc(FAKE, None, FAKE + 0x1000, 3, 4, 5, H.COO, H.F64, 8, 8, H.DEVICE, 1)
+    need = C.c_size_t()
+    assert vctx.lib.asc_sparse_hip_convert_coordinate_workspace_size(
+        vctx.handle, C.byref(coo), H.CSC, 8, C.byref(need)).code == H.OK
+    assert vctx.lib.asc_sparse_hip_convert_coordinate_workspace_size(
+        vctx.handle, C.byref(coo), 9, 8, C.byref(need)).code == H.INVALID_ARGUMENT
+    csr = dev_csr()
+    assert vctx.lib.asc_sparse_hip_convert_compressed_workspace_size(
+        vctx.handle, C.byref(csr), C.byref(need)).code == H.OK
+    untrusted = dev_csr(trusted=0)
+    assert vctx.lib.asc_sparse_hip_convert_compressed_workspace_size(
+        vctx.handle, C.byref(untrusted), C.byref(need)).code == H.INVALID_ARGUMENT
+
+
+def test_fault_injection_arms_and_disarms(vctx):
+    H.test_inject(H.INJECT_ALLOCATION, 0)
+    with pytest.raises(H.AscHipError) as e:
+        vctx.alloc(64, H.HOST)
+    assert e.value.code == H.ALLOCATION
+    buf = vctx.alloc(64, H.HOST)  # disarmed after firing
+    buf.free()
+    H.test_inject(H.INJECT_ALLOCATION, 1)
+    b1 = vctx.alloc(64, H.HOST)
+    with pytest.raises(H.AscHipError):
+        vctx.alloc(64, H.HOST)
+    b1.free()
+    H.test_inject(H.INJECT_ALLOCATION, -1)

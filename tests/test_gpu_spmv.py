@@ -1,0 +1,185 @@
+"""GPU parity of the compat CSR SpMV (y = alpha*A*x + beta*y) through the C ABI.
+
+Checks: the reference's golden vectors for every algorithm; bit-exactness
+of the ordered mode against the oracle; the plain-mode bound
+|y - y_ref| <= 4*len_i*eps*|alpha|*(|A||x|)_i + 4*eps*|beta*y0_i|; edge cases
+(empty rows, beta == 0 with NaN in y, zero-column and zero-row matrices);
+int32 vs int64 index identity and run-to-run determinism.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import asc_sparse_hip as H
+from asc_sparse_hip import gen as G
+import oracle_bind as O
+
+pytestmark = pytest.mark.gpu
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_golden.json")))
+ALGS = [H.ALG_ORDERED, H.ALG_MERGE] + [H.ALG_VECTOR[g] for g in (1, 2, 4, 8, 16, 32, 64)]
+EPS = {np.float64: 2.0 ** -53, np.float32: 2.0 ** -24}
+KIND = {np.float64: H.F64, np.float32: H.F32}
+
+
+def arr(v, dtype=np.float64):
+    return np.array([float(t) if t != "nan" else np.nan for t in v], dtype=dtype)
+
+
+def run_compat(ctx, a, x_host, y_host, alpha, beta, alg, incx=1, incy=1, acc=H.ACC_PLAIN,
+               xkind=None):
+    kind = xkind if xkind is not None else a.desc.element_kind
+    if kind in (H.BF16, H.F16):
+        kind = H.F32
+    dt = np.float64 if kind == H.F64 else np.float32
+    xb = ctx.upload(np.asarray(x_host, dtype=dt))
+    yb = ctx.upload(np.asarray(y_host, dtype=dt))
+    nx = (len(x_host) + incx - 1) // incx if len(x_host) else 0
+    ny = (len(y_host) + incy - 1) // incy if len(y_host) else 0
+    xd = H.vector(xb if len(x_host) else 0, a.cols if nx else 0, kind, incx, writable=False)
+    yd = H.vector(yb if len(y_host) else 0, a.rows if ny else 0, kind, incy)
+    H.csr_spmv(ctx, alpha, a, xd, beta, yd, alg=alg, acc=acc)
+    return ctx.download(yb, dt, len(y_host))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("case", range(len(GOLD["compat_spmv"])))
+@pytest.mark.parametrize("alg", ALGS)
+def test_golden_vectors(ctx, case, dtype, alg):
+    g = GOLD["compat_spmv"][case]
+    print(f"golden case {case} dtype {dtype.__name__} alg {H.ALG_NAMES[alg]}", flush=True)
+    a = H.clone_compressed(ctx, H.CSR, g["rows"], g["cols"], np.array(g["offsets"]),
+                           np.array(g["indices"], dtype=np.int64), arr(g["values"], dtype),
+                           KIND[dtype])
+    x = arr(g.get("x_storage", g.get("x")), dtype)
+    y = arr(g["y_storage"], dtype)
+    out = run_compat(ctx, a, x, y, g["alpha"], g["beta"], alg, g["incx"], g["incy"])
+    if "expected_storage" in g:
+        np.testing.assert_array_equal(out, arr(g["expected_storage"], dtype))
+    else:
+        np.testing.assert_array_equal(out[:: g["incy"]][: g["rows"]], arr(g["expected_rows"], dtype))
+
+
+def test_order_f32_all_ordered(ctx):
+    g = GOLD["compat_order_f32"]
+    a = H.clone_compressed(ctx, H.CSR, 1, 3, np.array(g["offsets"]), np.array(g["indices"]),
+                           arr(g["values"], np.float32), H.F32)
+    out = run_compat(ctx, a, arr(g["x"], np.float32), arr(g["y0"], np.float32), 1.0, 0.0,
+                     H.ALG_ORDERED)
+    assert out[0] == np.float32(0.0)
+
+
+def bound_check(y, yref, lens, absax, alpha, beta, y0, eps):
+    bound = 4.0 * np.maximum(lens, 1) * eps * abs(alpha) * absax + 4.0 * eps * np.abs(beta * y0)
+    err = np.abs(y.astype(np.float64) - yref.astype(np.float64))
+    bad = np.nonzero(~(err <= bound))[0]
+    assert bad.size == 0, f"{bad.size} rows exceed the bound, first {bad[:5]} err {err[bad[:5]]} bound {bound[bad[:5]]}"
+
+
+def _matrices():
+    return {
+        "uniform16_4096": lambda: G.uniform16(4096),
+        "zipf_2e16": lambda: G.make(G.ZIPF, 1 << 16, 1 << 16, seed=17, values=G.VALUES_NORMAL),
+        "banded_3e4": lambda: G.make(G.BANDED, 30000, 30000, seed=5, values=G.VALUES_UNIFORM, band=40, fill=0.5),
+        "stencil3d_30": lambda: G.make(G.STENCIL3D, 27000, 27000, seed=1, values=G.VALUES_LAPLACIAN, grid=30),
+    }
+
+
+@pytest.mark.parametrize("name", list(_matrices()))
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_random_parity_all_algorithms(ctx, name, dtype):
+    m = _matrices()[name]()
+    vals = m.values.astype(dtype)
+    x = G.vector(m.cols, 99).astype(dtype)
+    y0 = G.vector(m.rows, 98).astype(dtype)
+    lens = np.diff(m.offsets)
+    absax = O.abs_spmv(m.offsets, m.indices, vals.astype(np.float64), x.astype(np.float64))
+    a = H.clone_compressed(ctx, H.CSR, m.rows, m.cols, m.offsets, m.indices, vals, KIND[dtype],
+                           index_bytes=4)
+    for alpha, beta in ((1.0, 0.0), (-1.5, 0.25)):
+        yref = y0.copy()
+        O.compat_spmv(m.offsets, m.indices, vals, alpha, x, beta, yref)
+        for alg in ALGS:
+            print(f"{name} {dtype.__name__} alpha {alpha} beta {beta} alg {H.ALG_NAMES[alg]}", flush=True)
+            y = run_compat(ctx, a, x, y0, alpha, beta, alg)
+            if alg == H.ALG_ORDERED:
+                np.testing.assert_array_equal(y.view(np.uint8), yref.view(np.uint8))
+            else:
+                bound_check(y, yref, lens, absax, alpha, beta, y0, EPS[dtype])
+
+
+def test_int32_and_int64_indices_identical(ctx):
+    m = G.make(G.ZIPF, 1 << 14, 1 << 14, seed=3, values=G.VALUES_NORMAL)
+    x = G.vector(m.cols, 1)
+    a32 = H.clone_compressed(ctx, H.CSR, m.rows, m.cols, m.offsets, m.indices, m.values, H.F64, 4)
+    a64 = H.clone_compressed(ctx, H.CSR, m.rows, m.cols, m.offsets, m.indices, m.values, H.F64, 8)
+    y0 = np.zeros(m.rows)
+    for alg in ALGS:
+        r32 = run_compat(ctx, a32, x, y0, 1.0, 0.0, alg)
+        r64 = run_compat(ctx, a64, x, y0, 1.0, 0.0, alg)
+        np.testing.assert_array_equal(r32.view(np.uint64), r64.view(np.uint64))
+
+
+def test_deterministic_across_runs(ctx):
+    m = G.zipf_large(1 << 15, seed=77)
+    x = G.vector(m.cols, 2)
+    a = H.clone_compressed(ctx, H.CSR, m.rows, m.cols, m.offsets, m.indices, m.values, H.F64, 4)
+    y0 = np.zeros(m.rows)
+    for alg in ALGS:
+        r1 = run_compat(ctx, a, x, y0, 1.0, 0.0, alg)
+        r2 = run_compat(ctx, a, x, y0, 1.0, 0.0, alg)
+        np.testing.assert_array_equal(r1.view(np.uint64), r2.view(np.uint64))
+
+
+def test_empty_rows_and_beta_zero_nan(ctx):
+    # Many empty rows (merge path consumes row-end items only) and NaN y.
+    rng = np.random.default_rng(4)
+    rows, cols = 5000, 300
+    lens = np.where(rng.random(rows) < 0.7, 0, rng.integers(1, 40, rows))
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    idx = np.concatenate([np.sort(rng.choice(cols, l, replace=False)) for l in lens]).astype(np.int64)
+    vals = rng.standard_normal(off[-1])
+    x = rng.standard_normal(cols)
+    a = H.clone_compressed(ctx, H.CSR, rows, cols, off, idx, vals, H.F64)
+    y0 = np.full(rows, np.nan)
+    yref = y0.copy()
+    O.compat_spmv(off, idx, vals, -2.0, x, 0.0, yref)
+    absax = O.abs_spmv(off, idx, vals, x)
+    for alg in ALGS:
+        y = run_compat(ctx, a, x, y0, -2.0, 0.0, alg)
+        assert not np.isnan(y).any()
+        if alg == H.ALG_ORDERED:
+            np.testing.assert_array_equal(y.view(np.uint64), yref.view(np.uint64))
+        else:
+            bound_check(y, yref, lens, absax, -2.0, 0.0, np.zeros(rows), EPS[np.float64])
+
+
+def test_long_rows_span_many_merge_tiles(ctx):
+    # Rows far longer than a merge tile (2048 items) plus short neighbours.
+    rng = np.random.default_rng(8)
+    cols = 100000
+    lens = np.array([1, 50000, 3, 0, 20000, 7000, 1, 1, 9000, 2])
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    idx = np.concatenate([np.sort(rng.choice(cols, l, replace=False)) for l in lens]).astype(np.int64)
+    vals = rng.standard_normal(off[-1])
+    x = rng.standard_normal(cols)
+    a = H.clone_compressed(ctx, H.CSR, len(lens), cols, off, idx, vals, H.F64)
+    y0 = rng.standard_normal(len(lens))
+    yref = y0.copy()
+    O.compat_spmv(off, idx, vals, 0.5, x, 3.0, yref)
+    absax = O.abs_spmv(off, idx, vals, x)
+    for alg in ALGS:
+        y = run_compat(ctx, a, x, y0, 0.5, 3.0, alg)
+        if alg == H.ALG_ORDERED:
+            np.testing.assert_array_equal(y.view(np.uint64), yref.view(np.uint64))
+        else:
+            bound_check(y, yref, lens, absax, 0.5, 3.0, y0, EPS[np.float64])
+
+
+def test_zero_rows_is_noop(ctx):
+    a = H.clone_compressed(ctx, H.CSR, 0, 4, np.zeros(1, np.int64), np.zeros(0, np.int64),
+                           np.zeros(0), H.F64)
+    out = run_compat(ctx, a, np.zeros(4), [], 1.0, 0.0, H.ALG_MERGE)
+    assert out.size == 0
