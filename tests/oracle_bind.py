@@ -131,3 +131,97 @@ def finalize(coords, values, duplicate_policy, zero_policy):
 def validate(outer, inner, offsets, indices, nnz):
     offsets, indices = i64(offsets), i64(indices)
     return lib().oracle_validate_compressed(outer, inner, _p(offsets), _p(indices), nnz)
+
+
+# ---- the reference compiled from its own sources (oracle/_ref) ---------------------
+
+_ref = None
+
+
+def ref_available():
+    return os.path.exists(REF)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        R = C.CDLL(REF)
+        R.ref_compat_spmv_f64.argtypes = [I64, I64, I64, VP, VP, VP, C.c_double, VP, I64, C.c_double, VP, I64]
+        R.ref_compat_spmv_f32.argtypes = [I64, I64, I64, VP, VP, VP, C.c_float, VP, I64, C.c_float, VP, I64]
+        R.ref_std_spmv_f64.argtypes = [C.c_int, C.c_int, I64, I64, I64, VP, VP, VP, C.c_double, VP, I64, VP, I64]
+        R.ref_std_spmm_f64.argtypes = [C.c_int, C.c_int, I64, I64, I64, VP, VP, VP, C.c_double, VP, C.c_int, I64, I64, VP, C.c_int, I64]
+        R.ref_coo_to_compressed_f64.argtypes = [C.c_int, I64, I64, I64, VP, VP, VP, VP, VP]
+        R.ref_transpose_f64.argtypes = [C.c_int, I64, I64, VP, VP, VP, VP, VP, VP]
+        R.ref_finalize_f64.argtypes = [I64, I64, I64, VP, VP, C.c_int, C.c_int, VP, VP, VP]
+        _ref = R
+    return _ref
+
+
+def ref_compat_spmv(offsets, indices, values, alpha, x, beta, y, incx=1, incy=1, cols=None):
+    offsets, indices = i64(offsets), i64(indices)
+    rows = offsets.shape[0] - 1
+    cols = (x.shape[0] + abs(incx) - 1) // abs(incx) if cols is None else cols
+    f = ref().ref_compat_spmv_f64 if values.dtype == np.float64 else ref().ref_compat_spmv_f32
+    rc = f(rows, cols, values.shape[0], _p(offsets), _p(indices), _p(values), alpha, _p(x), incx,
+           beta, _p(y), incy)
+    if rc != 0:
+        raise RuntimeError(f"reference Spmv failed with ErrorCode {rc}")
+    return y
+
+
+def ref_std_spmv(fmt, op, rows, cols, offsets, indices, values, alpha, x, y):
+    offsets, indices = i64(offsets), i64(indices)
+    rc = ref().ref_std_spmv_f64(fmt, op, rows, cols, values.shape[0], _p(offsets), _p(indices),
+                                _p(values), alpha, _p(x), 1, _p(y), 1)
+    if rc != 0:
+        raise RuntimeError(f"reference Spmv failed with ErrorCode {rc}")
+    return y
+
+
+def ref_std_spmm(fmt, op, rows, cols, offsets, indices, values, alpha, b, b_layout, ldb, k, c, c_layout, ldc):
+    offsets, indices = i64(offsets), i64(indices)
+    rc = ref().ref_std_spmm_f64(fmt, op, rows, cols, values.shape[0], _p(offsets), _p(indices),
+                                _p(values), alpha, _p(b), b_layout, ldb, k, _p(c), c_layout, ldc)
+    if rc != 0:
+        raise RuntimeError(f"reference Spmm failed with ErrorCode {rc}")
+    return c
+
+
+def ref_coo_to_compressed(fmt, rows, cols, coords, values):
+    coords = i64(coords).reshape(-1)
+    nnz = values.shape[0]
+    outer = rows if fmt == 0 else cols
+    off = np.empty(outer + 1, np.int64)
+    idx = np.empty(max(nnz, 1), np.int64)
+    val = np.empty(max(nnz, 1))
+    rc = ref().ref_coo_to_compressed_f64(fmt, rows, cols, nnz, _p(coords), _p(values), _p(off),
+                                         _p(idx), _p(val))
+    if rc != 0:
+        raise RuntimeError(f"reference conversion failed with ErrorCode {rc}")
+    return off, idx[:nnz], val[:nnz]
+
+
+def ref_transpose(src_fmt, rows, cols, offsets, indices, values):
+    offsets, indices = i64(offsets), i64(indices)
+    nnz = values.shape[0]
+    outer = cols if src_fmt == 0 else rows
+    off = np.empty(outer + 1, np.int64)
+    idx = np.empty(max(nnz, 1), np.int64)
+    val = np.empty(max(nnz, 1))
+    rc = ref().ref_transpose_f64(src_fmt, rows, cols, _p(offsets), _p(indices), _p(values),
+                                 _p(off), _p(idx), _p(val))
+    if rc != 0:
+        raise RuntimeError(f"reference conversion failed with ErrorCode {rc}")
+    return off, idx[:nnz], val[:nnz]
+
+
+def ref_finalize(rows, cols, coords, values, duplicate_policy, zero_policy):
+    coords = i64(coords).reshape(-1)
+    values = np.ascontiguousarray(values, dtype=np.float64)
+    nnz = values.shape[0]
+    oc = np.empty(max(2 * nnz, 2), np.int64)
+    ov = np.empty(max(nnz, 1))
+    n = C.c_int64()
+    rc = ref().ref_finalize_f64(rows, cols, nnz, _p(coords), _p(values), duplicate_policy,
+                                zero_policy, _p(oc), _p(ov), C.byref(n))
+    return rc, oc[: 2 * n.value], ov[: n.value]
