@@ -77,6 +77,22 @@ class SpmvOptions(C.Structure):
                 ("reserved", C.c_uint8 * 6)]
 
 
+class BatchEntry(C.Structure):
+    _fields_ = [("matrix", SparseDesc), ("x", VectorDesc), ("y", VectorDesc),
+                ("alpha", C.c_double), ("beta", C.c_double)]
+
+
+class BatchClasses(C.Structure):
+    _fields_ = [("lanes", C.c_uint8 * 8)]
+
+
+class BatchPlan(C.Structure):
+    _fields_ = [("matrix_table", C.c_void_p), ("item_table", C.c_void_p),
+                ("matrices", C.c_int64), ("items", C.c_int64), ("grid", C.c_int64),
+                ("device", C.c_int32), ("element_kind", C.c_uint8), ("index_bytes", C.c_uint8),
+                ("reserved", C.c_uint8 * 2)]
+
+
 class AscHipError(RuntimeError):
     def __init__(self, code: int, message: str, native: int = 0):
         super().__init__(f"asc_sparse_hip error {code}: {message}")
@@ -104,6 +120,7 @@ _SIGS = {
     "asc_sparse_hip_clone_coordinate": [_VP, P(SparseDesc), _VP, _VP, P(SparseDesc)],
     "asc_sparse_hip_narrow_indices": [_VP, P(SparseDesc), _VP, P(SparseDesc)],
     "asc_sparse_hip_convert_values": [_VP, P(SparseDesc), C.c_uint8, _VP, P(SparseDesc)],
+    "asc_sparse_hip_convert_values_scaled": [_VP, P(SparseDesc), C.c_uint8, C.c_double, _VP, P(SparseDesc)],
     "asc_sparse_hip_csr_spmv_workspace_size": [_VP, P(SparseDesc), P(VectorDesc), P(VectorDesc), P(SpmvOptions), P(C.c_size_t)],
     "asc_sparse_hip_csr_spmv": [_VP, C.c_double, P(SparseDesc), P(VectorDesc), C.c_double, P(VectorDesc), _VP, C.c_size_t, P(SpmvOptions)],
     "asc_sparse_hip_standard_spmv_workspace_size": [_VP, C.c_uint8, P(SparseDesc), P(VectorDesc), P(VectorDesc), P(SpmvOptions), P(C.c_size_t)],
@@ -115,6 +132,13 @@ _SIGS = {
     "asc_sparse_hip_convert_compressed_workspace_size": [_VP, P(SparseDesc), P(C.c_size_t)],
     "asc_sparse_hip_convert_compressed": [_VP, P(SparseDesc), _VP, _VP, _VP, P(SparseDesc), _VP, C.c_size_t],
     "asc_sparse_hip_test_inject": [C.c_uint32, C.c_int32],
+    "asc_sparse_hip_batch_plan_size": [_VP, C.c_int64, P(BatchEntry), P(C.c_size_t)],
+    "asc_sparse_hip_batch_plan_build": [_VP, C.c_int64, P(BatchEntry), P(BatchClasses), _VP, C.c_size_t, P(BatchPlan)],
+    "asc_sparse_hip_batch_spmv": [_VP, P(BatchPlan)],
+    "asc_sparse_hip_batch_graph_create": [_VP, P(BatchPlan), P(_VP)],
+    "asc_sparse_hip_graph_launch": [_VP, _VP],
+    "asc_sparse_hip_graph_destroy": [_VP],
+    "asc_sparse_hip_batch_tune": [_VP, C.c_int64, P(BatchEntry), _VP, C.c_size_t, C.c_int32, P(BatchClasses), P(C.c_float)],
 }
 
 _lib = None
@@ -344,11 +368,11 @@ def narrow_indices(ctx: Context, a: DeviceSparse) -> DeviceSparse:
     return DeviceSparse(out, a.buffers + [b])
 
 
-def convert_values(ctx: Context, a: DeviceSparse, kind: int) -> DeviceSparse:
+def convert_values(ctx: Context, a: DeviceSparse, kind: int, scale: float = 1.0) -> DeviceSparse:
     b = ctx.alloc(max(ELEM_BYTES[kind] * a.nnz, 1))
     out = SparseDesc()
-    check(ctx.lib.asc_sparse_hip_convert_values(ctx.handle, C.byref(a.desc), kind, b.ptr,
-                                                C.byref(out)))
+    check(ctx.lib.asc_sparse_hip_convert_values_scaled(ctx.handle, C.byref(a.desc), kind, scale,
+                                                       b.ptr, C.byref(out)))
     return DeviceSparse(out, a.buffers + [b])
 
 
@@ -469,3 +493,87 @@ def convert_compressed(ctx: Context, a: DeviceSparse, ws: Workspace | None = Non
 
 def test_inject(what: int, countdown: int):
     check(load().asc_sparse_hip_test_inject(what, countdown))
+
+
+# ---- batched SpMV -------------------------------------------------------------------
+
+class Batch:
+    """A batched-SpMV plan over device matrices and vectors (kept alive here)."""
+
+    def __init__(self, ctx: Context, entries: list, keep: list | None = None,
+                 lanes: list | None = None):
+        self.ctx = ctx
+        self.keep = keep or []
+        self.n = len(entries)
+        self.entries = (BatchEntry * max(self.n, 1))(*entries)
+        need = C.c_size_t()
+        check(ctx.lib.asc_sparse_hip_batch_plan_size(ctx.handle, self.n, self.entries, C.byref(need)))
+        self.mem = ctx.alloc(max(need.value, 256))
+        self.plan = BatchPlan()
+        self.graph = None
+        self.build(lanes)
+
+    def build(self, lanes=None):
+        cls = None
+        if lanes is not None:
+            cls = BatchClasses((C.c_uint8 * 8)(*lanes))
+        check(self.ctx.lib.asc_sparse_hip_batch_plan_build(
+            self.ctx.handle, self.n, self.entries, C.byref(cls) if cls else None, self.mem.ptr,
+            self.mem.nbytes, C.byref(self.plan)))
+        self.drop_graph()
+
+    def tune(self, reps: int = 3):
+        out = BatchClasses()
+        ms = (C.c_float * 56)()
+        check(self.ctx.lib.asc_sparse_hip_batch_tune(self.ctx.handle, self.n, self.entries, self.mem.ptr,
+                                                     self.mem.nbytes, reps, C.byref(out), ms))
+        lanes = list(out.lanes)
+        self.build(lanes)
+        return lanes, [list(ms[c * 7:(c + 1) * 7]) for c in range(8)]
+
+    def run(self):
+        check(self.ctx.lib.asc_sparse_hip_batch_spmv(self.ctx.handle, C.byref(self.plan)))
+
+    def capture(self):
+        self.drop_graph()
+        g = C.c_void_p()
+        check(self.ctx.lib.asc_sparse_hip_batch_graph_create(self.ctx.handle, C.byref(self.plan), C.byref(g)))
+        self.graph = g
+
+    def replay(self):
+        check(self.ctx.lib.asc_sparse_hip_graph_launch(self.ctx.handle, self.graph))
+
+    def drop_graph(self):
+        if self.graph:
+            check(self.ctx.lib.asc_sparse_hip_graph_destroy(self.graph))
+            self.graph = None
+
+    def __del__(self):
+        try:
+            self.drop_graph()
+        except Exception:
+            pass
+
+
+def upload_packed(ctx: Context, packed, x: np.ndarray, y: np.ndarray, kind: int,
+                  alpha: float = 1.0, beta: float = 0.0):
+    """Uploads a packed batch (see gen.pack) and returns (entries, buffers).
+
+    `x` and `y` are the concatenated per-matrix vectors (square matrices)."""
+    dt = np.float64 if kind == F64 else np.float32
+    bo = ctx.upload(packed.offsets)
+    bi = ctx.upload(packed.indices)
+    bv = ctx.upload(packed.values.astype(dt))
+    bx = ctx.upload(x.astype(dt))
+    by = ctx.upload(y.astype(dt))
+    ib = packed.indices.dtype.itemsize
+    es = ELEM_BYTES[kind]
+    entries = []
+    for i in range(packed.rows.shape[0]):
+        r, n = int(packed.rows[i]), int(packed.nnz[i])
+        a = SparseDesc(bo.ptr + 8 * int(packed.off_start[i]), bi.ptr + ib * int(packed.nz_start[i]),
+                       bv.ptr + es * int(packed.nz_start[i]), r, r, n, CSR, kind, ib, 8, DEVICE, 1)
+        vs = int(packed.vec_start[i])
+        entries.append(BatchEntry(a, VectorDesc(bx.ptr + es * vs, r, 1, kind, DEVICE, 0),
+                                  VectorDesc(by.ptr + es * vs, r, 1, kind, DEVICE, 1), alpha, beta))
+    return entries, [bo, bi, bv, bx, by]

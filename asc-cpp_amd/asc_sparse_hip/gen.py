@@ -111,3 +111,60 @@ def zipf_large(n: int = 1 << 22, seed: int = SEED_BASE + 1) -> Csr:
 def precision_matrix(n: int = 1 << 20, seed: int = SEED_BASE + 3) -> Csr:
     """Precision case: 32 nnz/row (16 near-diagonal + 16 uniform), lognormal(0,2)."""
     return make(PRECISION, n, n, seed=seed, values=VALUES_LOGNORMAL)
+
+
+def batched_specs(count: int = 10_000, seed: int = SEED_BASE + 2, lo: int = 64, hi: int = 65536):
+    """Batched case: `count` matrices, rows log-uniform in [lo, hi], families
+    cycling 2D 5-point Laplacian (k = round(sqrt(rows))), 3D 7-point
+    Laplacian (k = round(cbrt(rows))) and random banded (half-bandwidth in
+    U{8..64}, each in-band off-diagonal kept with p = 0.5, diagonal always
+    kept, values U[-1,1])."""
+    rng = np.random.default_rng(seed)
+    rows = np.exp(rng.uniform(np.log(lo), np.log(hi), count))
+    bands = rng.integers(8, 65, count)
+    specs = []
+    for i in range(count):
+        fam = i % 3
+        r = float(rows[i])
+        if fam == 0:
+            k = max(1, int(round(r ** 0.5)))
+            specs.append(dict(family=STENCIL2D, rows=k * k, grid=k, values=VALUES_LAPLACIAN))
+        elif fam == 1:
+            k = max(1, int(round(r ** (1.0 / 3.0))))
+            specs.append(dict(family=STENCIL3D, rows=k ** 3, grid=k, values=VALUES_LAPLACIAN))
+        else:
+            specs.append(dict(family=BANDED, rows=int(round(r)), band=int(bands[i]), fill=0.5,
+                              values=VALUES_UNIFORM))
+        specs[-1]["seed"] = (seed * 1_000_003 + i) & 0xFFFFFFFFFFFF
+    return specs
+
+
+def make_spec(spec: dict, index_dtype=np.int32) -> Csr:
+    s = dict(spec)
+    fam, rows = s.pop("family"), s.pop("rows")
+    return make(fam, rows, rows, index_dtype=index_dtype, **s)
+
+
+@dataclass
+class Packed:
+    """Many CSR matrices concatenated into shared arrays (host side)."""
+    offsets: np.ndarray      # concatenated per-matrix offsets (each starts at 0)
+    indices: np.ndarray
+    values: np.ndarray
+    rows: np.ndarray         # per-matrix rows
+    nnz: np.ndarray          # per-matrix nnz
+    off_start: np.ndarray    # element offset of each matrix's offsets
+    nz_start: np.ndarray     # element offset of each matrix's indices/values
+    vec_start: np.ndarray    # element offset of each matrix's x / y
+
+
+def pack(mats: list) -> Packed:
+    rows = np.array([m.rows for m in mats], dtype=np.int64)
+    nnz = np.array([m.nnz for m in mats], dtype=np.int64)
+    off_start = np.concatenate([[0], np.cumsum(rows + 1)[:-1]]).astype(np.int64)
+    nz_start = np.concatenate([[0], np.cumsum(nnz)[:-1]]).astype(np.int64)
+    vec_start = np.concatenate([[0], np.cumsum(rows)[:-1]]).astype(np.int64)
+    return Packed(np.concatenate([m.offsets for m in mats]),
+                  np.concatenate([m.indices for m in mats]),
+                  np.concatenate([m.values for m in mats]),
+                  rows, nnz, off_start, nz_start, vec_start)

@@ -390,6 +390,14 @@ asc_hip_status asc_sparse_hip_convert_values(asc_hip_context* ctx,
                                              const asc_hip_sparse_desc* src,
                                              uint8_t kind, void* d_val,
                                              asc_hip_sparse_desc* out) {
+  return asc_sparse_hip_convert_values_scaled(ctx, src, kind, 1.0, d_val, out);
+}
+
+asc_hip_status asc_sparse_hip_convert_values_scaled(asc_hip_context* ctx,
+                                                    const asc_hip_sparse_desc* src,
+                                                    uint8_t kind, double scale,
+                                                    void* d_val,
+                                                    asc_hip_sparse_desc* out) {
   asc_hip_status st = check_context(ctx);
   if (st.code) return st;
   if (out == nullptr) return fail(ASC_HIP_INVALID_ARGUMENT, "output descriptor is null");
@@ -397,6 +405,8 @@ asc_hip_status asc_sparse_hip_convert_values(asc_hip_context* ctx,
   if (src->element_kind != ASC_HIP_F64) return fail(ASC_HIP_UNSUPPORTED, "value conversion reads FP64 storage");
   if (kind != ASC_HIP_F32 && kind != ASC_HIP_BF16 && kind != ASC_HIP_F16)
     return fail(ASC_HIP_INVALID_ARGUMENT, "target must be FP32, BF16 or F16");
+  if (!(scale > 0.0) || scale > 1e300)
+    return fail(ASC_HIP_INVALID_ARGUMENT, "scale must be positive and finite");
   if (src->nnz > 0) {
     if ((st = check_device_pointer(ctx, d_val, "values destination")).code) return st;
     if (sparse_overlaps(*src, span_of(d_val, elem_size(kind) * src->nnz)))
@@ -406,7 +416,7 @@ asc_hip_status asc_sparse_hip_convert_values(asc_hip_context* ctx,
   DeviceGuard guard(ctx->device);
   st = finish_launch(ctx, asc_hip::convert_values_launch(
       static_cast<const double*>(src->values), d_val, static_cast<Elem>(kind), src->nnz,
-      ctx->stream), "convert values");
+      scale, ctx->stream), "convert values");
   if (st.code) return st;
   *out = *src;
   out->values = d_val;

@@ -65,6 +65,36 @@ struct SpmmArgs {
 };
 hipError_t spmm_launch(const SpmmArgs& args, hipStream_t stream);
 
+// ---- batched SpMV ----------------------------------------------------------
+// Per-matrix table entry and work item of the persistent batched kernel
+// (device-resident arrays built by the host plan).
+struct BatchMatrix {
+  const int64_t* offsets;
+  const void* indices;
+  const void* values;
+  const void* x;
+  void* y;
+  int64_t incx;
+  int64_t incy;
+  double alpha;
+  double beta;
+};
+struct BatchItem {
+  int32_t matrix;
+  int32_t lanes;  // lanes per row: 1..64, power of two
+  int64_t row_begin;
+  int64_t row_end;
+};
+struct BatchLaunch {
+  const BatchMatrix* mats;
+  const BatchItem* items;
+  int64_t nitems;
+  int64_t grid;  // persistent workgroups
+  Elem elem;     // F32 or F64
+  int index_bytes;
+};
+hipError_t batched_spmv_launch(const BatchLaunch& launch, hipStream_t stream);
+
 // ---- conversions ---------------------------------------------------------
 // Stable transpose of a compressed operand (CSR<->CSC): out_offsets has
 // inner+1 entries, out_indices nnz entries of out_index_bytes, out_values
@@ -90,7 +120,7 @@ hipError_t narrow_indices_launch(const int64_t* src, int32_t* dst, int64_t n,
                                  hipStream_t stream);
 // FP64 -> FP32/BF16/F16 with round-to-nearest-even.
 hipError_t convert_values_launch(const double* src, void* dst, Elem to,
-                                 int64_t n, hipStream_t stream);
+                                 int64_t n, double scale, hipStream_t stream);
 // Byte fill as a kernel (graph-capture safe, unlike hipMemsetAsync).
 hipError_t fill_launch(void* dst, uint8_t byte, size_t bytes,
                        hipStream_t stream);

@@ -61,11 +61,12 @@ __device__ __forceinline__ uint16_t to_f16_rne(double d) {
 }
 
 __global__ __launch_bounds__(kBlock) void convert_kernel(
-    const double* __restrict__ src, void* __restrict__ dst, int to, int64_t n) {
+    const double* __restrict__ src, void* __restrict__ dst, int to, int64_t n,
+    double scale) {
   const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
        i < n; i += stride) {
-    const double d = ld_stream(src + i);
+    const double d = ld_stream(src + i) * scale;
     if (to == static_cast<int>(Elem::F32)) {
       static_cast<float*>(dst)[i] = static_cast<float>(d);
     } else if (to == static_cast<int>(Elem::BF16)) {
@@ -100,9 +101,9 @@ hipError_t narrow_indices_launch(const int64_t* src, int32_t* dst, int64_t n,
 }
 
 hipError_t convert_values_launch(const double* src, void* dst, Elem to,
-                                 int64_t n, hipStream_t s) {
+                                 int64_t n, double scale, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  convert_kernel<<<grid_for(n), kBlock, 0, s>>>(src, dst, static_cast<int>(to), n);
+  convert_kernel<<<grid_for(n), kBlock, 0, s>>>(src, dst, static_cast<int>(to), n, scale);
   return hipGetLastError();
 }
 

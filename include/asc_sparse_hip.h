@@ -272,6 +272,14 @@ asc_hip_status asc_sparse_hip_convert_values(asc_hip_context* context,
                                              void* device_values,
                                              asc_hip_sparse_desc* out);
 
+/* The same with every value multiplied by `scale` before rounding; a power
+ * of two keeps the scaling exact, so FP16 storage can hold matrices whose
+ * magnitudes exceed 65504 (run the SpMV with alpha / scale). */
+asc_hip_status asc_sparse_hip_convert_values_scaled(
+    asc_hip_context* context, const asc_hip_sparse_desc* source,
+    uint8_t element_kind, double scale, void* device_values,
+    asc_hip_sparse_desc* out);
+
 /* ---- SpMV --------------------------------------------------------------- */
 
 /*
@@ -359,6 +367,77 @@ asc_hip_status asc_sparse_hip_convert_compressed(
     asc_hip_context* context, const asc_hip_sparse_desc* source,
     void* out_offsets, void* out_indices, void* out_values,
     asc_hip_sparse_desc* out, void* workspace, size_t workspace_bytes);
+
+/* ---- batched SpMV --------------------------------------------------------- */
+
+/*
+ * Thousands of independent compat SpMVs  y_i = alpha_i*A_i*x_i + beta_i*y_i
+ * in one persistent launch.  No reference counterpart: every entry follows
+ * the contract of asc::Spmv(ctx, alpha, CsrView, x, beta, y)
+ * (blas.h:2031-2081), entries share element kind (F32/F64) and index width,
+ * and no output may overlap any input or other output.
+ *
+ * A plan is a work list of (matrix, row range, lanes per row) items in
+ * caller-owned device memory.  Lanes per row come from `classes`: class c
+ * holds matrices whose average row length lies in [2^c, 2^(c+1)) (class 7
+ * is open-ended); NULL selects {1,2,4,8,16,32,64,64}.  The plan stores raw
+ * pointers: entries' buffers must outlive it.
+ */
+typedef struct asc_hip_batch_entry {
+  asc_hip_sparse_desc matrix;
+  asc_hip_vector_desc x;
+  asc_hip_vector_desc y;
+  double alpha;
+  double beta;
+} asc_hip_batch_entry;
+
+typedef struct asc_hip_batch_classes {
+  uint8_t lanes[8];
+} asc_hip_batch_classes;
+
+typedef struct asc_hip_batch_plan {
+  void* matrix_table; /* device: per-matrix descriptors */
+  void* item_table;   /* device: work items */
+  int64_t matrices;
+  int64_t items;
+  int64_t grid; /* persistent workgroups */
+  int32_t device;
+  uint8_t element_kind;
+  uint8_t index_bytes;
+  uint8_t reserved[2];
+} asc_hip_batch_plan;
+
+typedef struct asc_hip_graph asc_hip_graph;
+
+asc_hip_status asc_sparse_hip_batch_plan_size(asc_hip_context* context,
+                                              int64_t count,
+                                              const asc_hip_batch_entry* entries,
+                                              size_t* out_bytes);
+asc_hip_status asc_sparse_hip_batch_plan_build(
+    asc_hip_context* context, int64_t count, const asc_hip_batch_entry* entries,
+    const asc_hip_batch_classes* classes, void* plan_memory, size_t plan_bytes,
+    asc_hip_batch_plan* out);
+asc_hip_status asc_sparse_hip_batch_spmv(asc_hip_context* context,
+                                         const asc_hip_batch_plan* plan);
+
+/* Captures the plan's launch into a hipGraph (one kernel node, no memset
+ * nodes) for replay on the context stream. */
+asc_hip_status asc_sparse_hip_batch_graph_create(asc_hip_context* context,
+                                                 const asc_hip_batch_plan* plan,
+                                                 asc_hip_graph** out);
+asc_hip_status asc_sparse_hip_graph_launch(asc_hip_context* context,
+                                           asc_hip_graph* graph);
+asc_hip_status asc_sparse_hip_graph_destroy(asc_hip_graph* graph);
+
+/* Exhaustive per-class autotuner: times every lanes-per-row choice (1..64)
+ * on each class's matrices with `reps` launches and writes the fastest to
+ * `out`; `class_ms` (8 x 7 floats, may be NULL) receives the timings (-1
+ * for empty classes).  Uses `plan_memory` as scratch; outputs are written. */
+asc_hip_status asc_sparse_hip_batch_tune(asc_hip_context* context, int64_t count,
+                                         const asc_hip_batch_entry* entries,
+                                         void* plan_memory, size_t plan_bytes,
+                                         int32_t reps, asc_hip_batch_classes* out,
+                                         float* class_ms);
 
 /* ---- host-forced fault injection (tests) -------------------------------- */
 

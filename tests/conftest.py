@@ -12,19 +12,22 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs a HIP device (MI355X)")
 
 
-def _has_gpu():
+def _gpu_problem():
+    """None when a HIP device is usable, else a description of why not."""
     try:
         import asc_sparse_hip as H
-        return H.device_count() > 0
-    except Exception:
-        return False
+        n = H.device_count()
+        return None if n > 0 else "asc_sparse_hip_device_count() returned 0"
+    except Exception as e:  # report the real cause
+        return f"{type(e).__name__}: {e}"
 
 
 @pytest.fixture(scope="session")
 def ctx():
     import asc_sparse_hip as H
-    if not _has_gpu():
-        pytest.fail("no HIP device visible: GPU tests must run on a GPU box")
+    problem = _gpu_problem()
+    if problem:
+        pytest.fail(f"no usable HIP device ({problem}): GPU tests must run on a GPU box")
     c = H.Context(0)
     yield c
     c.close()
