@@ -24,7 +24,7 @@ class Params(C.Structure):
     _fields_ = [("family", C.c_int32), ("value_dist", C.c_int32), ("rows", C.c_int64),
                 ("cols", C.c_int64), ("per_row", C.c_int64), ("zipf_s", C.c_double),
                 ("zipf_max", C.c_int64), ("band", C.c_int64), ("fill", C.c_double),
-                ("grid", C.c_int64), ("seed", C.c_uint64)]
+                ("grid", C.c_int64), ("seed", C.c_uint64), ("window", C.c_int64)]
 
 
 _lib = None
@@ -68,10 +68,10 @@ class Csr:
 
 def make(family: int, rows: int, cols: int | None = None, *, seed: int, values: int,
          per_row: int = 0, zipf_s: float = 1.5, zipf_max: int = 10_000, band: int = 0,
-         fill: float = 0.5, grid: int = 0, index_dtype=np.int32) -> Csr:
+         fill: float = 0.5, grid: int = 0, window: int = 0, index_dtype=np.int32) -> Csr:
     cols = rows if cols is None else cols
     p = Params(family, values, rows, cols, per_row, zipf_s, zipf_max, band, fill, grid,
-               seed & 0xFFFFFFFFFFFFFFFF)
+               seed & 0xFFFFFFFFFFFFFFFF, window)
     lib = _load()
     off = np.empty(rows + 1, dtype=np.int64)
     nnz = lib.asc_gen_lengths(C.byref(p), off.ctypes.data, threads())

@@ -173,7 +173,15 @@ struct Gen {
       case ASC_GEN_UNIFORM_FIXED:
       case ASC_GEN_ZIPF:
       case ASC_GEN_UNIFORM_VAR:
-        sorted_distinct(s, 0, L, 0, p.cols, out);
+        if (p.window > 0 && p.window < p.cols) {
+          const int64_t w = std::max(p.window, L);
+          const int64_t centre = p.rows > 0 ? static_cast<int64_t>(
+              (static_cast<double>(row) + 0.5) * static_cast<double>(p.cols) / static_cast<double>(p.rows)) : 0;
+          const int64_t lo = std::min(std::max<int64_t>(0, centre - w / 2), p.cols - w);
+          sorted_distinct(s, 0, L, lo, w, out);
+        } else {
+          sorted_distinct(s, 0, L, 0, p.cols, out);
+        }
         return;
       case ASC_GEN_BIPARTITE: {
         const int64_t half = p.cols / 2;

@@ -48,6 +48,8 @@ typedef struct asc_gen_params {
   double fill;      /* BANDED keep probability */
   int64_t grid;     /* STENCIL2D/3D grid edge k */
   uint64_t seed;
+  int64_t window;   /* > 0: UNIFORM/ZIPF columns drawn from a window of this
+                       width centred on the row's diagonal position */
 } asc_gen_params;
 
 /* Writes offsets[0..rows] and returns nnz (or -1 on a bad parameter). */

@@ -270,10 +270,24 @@ __global__ __launch_bounds__(kBlock) void spmv_ordered_kernel(KParams p) {
 }
 
 // ---------------------------------------------------------------------------
-// merge-path
+// merge-path, wave-level and persistent
+//
+// The merge sequence (row ends interleaved with nonzeros, Merrill & Garland)
+// is cut into wave tiles of kWTile items.  Each persistent wave owns a
+// contiguous run of tiles and walks them in order:
+//   - the next tile's column indices, values and row ends are loaded into
+//     registers while the current tile is reduced (one-tile prefetch), and
+//     no block barrier stands in between, so those loads stay in flight;
+//   - products and row ends are staged in the wave's own LDS slice, each
+//     lane walks kWIpt consecutive merge items, and a wave64 segmented scan
+//     hands partial rows to the lane that finishes them;
+//   - the open row at a tile's end carries into the next tile in registers;
+//     only the carry left at a wave's end goes to the fix-up kernel.
 // ---------------------------------------------------------------------------
-constexpr int kIpt = 8;                 // merge items per thread
-constexpr int kTile = kBlock * kIpt;    // merge items per tile
+constexpr int kWIpt = 8;                         // merge items per lane
+constexpr int kWTile = kWave * kWIpt;            // merge items per wave tile
+constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr int64_t kMaxMergeWaves = 65536;        // carry slots in the workspace
 
 // Tile start coordinates on the merge path: coords[2t] = row, [2t+1] = nz.
 __global__ __launch_bounds__(kBlock) void merge_search_kernel(
@@ -282,7 +296,7 @@ __global__ __launch_bounds__(kBlock) void merge_search_kernel(
   const int64_t t = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (t > ntiles) return;
   const int64_t total = rows + nnz;
-  const int64_t d = t * kTile < total ? t * kTile : total;
+  const int64_t d = t * kWTile < total ? t * kWTile : total;
   int64_t lo = d - nnz > 0 ? d - nnz : 0;
   int64_t hi = d < rows ? d : rows;
   // Largest prefix of row-end items that precede nz item (d - lo - 1).
@@ -295,192 +309,235 @@ __global__ __launch_bounds__(kBlock) void merge_search_kernel(
   coords[2 * t + 1] = d - lo;
 }
 
+template <typename V, typename I>
+struct TileRegs {
+  I c[kWIpt];
+  V v[kWIpt];
+  int32_t e[kWIpt];
+  int32_t e_last;  // row end for tile row kWTile (only when nrows == kWTile)
+};
+
+// Row end of tile row r (0 <= r <= nrows) relative to the tile's first nz;
+// the open row's end is clamped (only "beyond the tile" matters).
+__device__ __forceinline__ int32_t rel_row_end(const KParams& p, int64_t i0,
+                                               int64_t j0, int r) {
+  const int64_t gr = i0 + r;
+  int64_t rel = (gr < p.rows ? ld_stream(p.offsets + gr + 1) : p.nnz) - j0;
+  return static_cast<int32_t>(rel > kWTile + 1 ? kWTile + 1 : rel);
+}
+
+template <typename V, typename I>
+__device__ __forceinline__ void load_tile(const KParams& p, TileRegs<V, I>& t,
+                                          int64_t i0, int64_t j0, int64_t i1,
+                                          int64_t j1, int lane) {
+  const V* __restrict__ val = static_cast<const V*>(p.values);
+  const I* __restrict__ col = static_cast<const I*>(p.indices);
+  const int nnzt = static_cast<int>(j1 - j0);
+  const int nrows = static_cast<int>(i1 - i0);
+#pragma unroll
+  for (int u = 0; u < kWIpt; ++u) {
+    const int k = lane + u * kWave;
+    if (k < nnzt) {
+      t.c[u] = ld_stream(col + j0 + k);
+      t.v[u] = ld_stream(val + j0 + k);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < kWIpt; ++u) {
+    const int r = lane + u * kWave;
+    if (r <= nrows) t.e[u] = rel_row_end(p, i0, j0, r);
+  }
+  if (lane == 0 && nrows == kWTile) t.e_last = rel_row_end(p, i0, j0, kWTile);
+}
+
 template <typename V, typename I, bool TWOSUM>
 __global__ __launch_bounds__(kBlock) void spmv_merge_kernel(
     KParams p, const int64_t* __restrict__ coords, int64_t ntiles,
-    int64_t* __restrict__ carry_rows, void* __restrict__ carry_vals) {
+    int64_t tiles_per_wave, int64_t* __restrict__ carry_rows,
+    void* __restrict__ carry_vals) {
   using A = typename acc_of<V>::type;
   using Acc = typename AccSel<V, TWOSUM>::type;
-  __shared__ int32_t s_rowend[kTile + 1];
-  __shared__ A s_val[kTile];
-  __shared__ float s_err[TWOSUM ? kTile : 1];
-  __shared__ int32_t s_wkey[kBlock / kWave];
-  __shared__ Acc s_wacc[kBlock / kWave];
-
-  const int64_t tile = xcd_remap(blockIdx.x, gridDim.x);
-  const int tid = threadIdx.x;
-  const int64_t i0 = coords[2 * tile], j0 = coords[2 * tile + 1];
-  const int64_t i1 = coords[2 * tile + 2], j1 = coords[2 * tile + 3];
-  const int nrows = static_cast<int>(i1 - i0);  // rows finished in this tile
-  const int nnzt = static_cast<int>(j1 - j0);
-  const int items = nrows + nnzt;
-
-  const V* __restrict__ val = static_cast<const V*>(p.values);
-  const I* __restrict__ col = static_cast<const I*>(p.indices);
+  __shared__ int32_t s_end_all[kWavesPerBlock][kWTile + 1];
+  __shared__ A s_val_all[kWavesPerBlock][kWTile];
+  __shared__ float s_err_all[kWavesPerBlock][TWOSUM ? kWTile : 1];
+  const int lane = threadIdx.x % kWave;
+  const int wib = threadIdx.x / kWave;
+  int32_t* s_end = s_end_all[wib];
+  A* s_val = s_val_all[wib];
+  float* s_err = s_err_all[wib];
+  const int64_t wave = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + wib;
+  const int64_t t_begin = wave * tiles_per_wave;
+  int64_t t_end = t_begin + tiles_per_wave;
+  if (t_end > ntiles) t_end = ntiles;
+  if (t_begin >= t_end) {
+    if (lane == 0) carry_rows[wave] = -1;
+    return;
+  }
   const A* __restrict__ x = static_cast<const A*>(p.x);
 
-  // Row ends relative to j0 for rows i0..i1 (row i1 stays open; clamp).
-  for (int r = tid; r <= nrows; r += kBlock) {
-    const int64_t gr = i0 + r;
-    int64_t rel = (gr < p.rows ? ld_stream(p.offsets + gr + 1) : p.nnz) - j0;
-    if (rel > kTile + 1) rel = kTile + 1;
-    s_rowend[r] = static_cast<int32_t>(rel);
-  }
-  // Products, coalesced: lane k handles nz j0 + k + u*kBlock.
-  {
-    int64_t cidx[kIpt];
-    V vv[kIpt];
+  int64_t i0 = coords[2 * t_begin], j0 = coords[2 * t_begin + 1];
+  int64_t i1 = coords[2 * t_begin + 2], j1 = coords[2 * t_begin + 3];
+  TileRegs<V, I> cur, nxt;
+  load_tile(p, cur, i0, j0, i1, j1, lane);
+  Acc carry = Acc::zero();  // partial of row i0 from earlier tiles of this wave
+
+  for (int64_t t = t_begin; t < t_end; ++t) {
+    const int nnzt = static_cast<int>(j1 - j0);
+    const int nrows = static_cast<int>(i1 - i0);
+    const int items = nrows + nnzt;
+    // Gathers for this tile.
+    A xv[kWIpt];
 #pragma unroll
-    for (int u = 0; u < kIpt; ++u) {
-      const int k = tid + u * kBlock;
-      if (k < nnzt) {
-        cidx[u] = load_index(col, j0 + k);
-        vv[u] = ld_stream(val + j0 + k);
-      }
+    for (int u = 0; u < kWIpt; ++u) {
+      const int k = lane + u * kWave;
+      if (k < nnzt) xv[u] = x[static_cast<int64_t>(cur.c[u]) * p.incx];
     }
+    // Prefetch the next tile while this one is reduced.
+    int64_t ni1 = i1, nj1 = j1;
+    if (t + 1 < t_end) {
+      ni1 = coords[2 * t + 4];
+      nj1 = coords[2 * t + 5];
+      load_tile(p, nxt, i1, j1, ni1, nj1, lane);
+    }
+    // Stage products and row ends in this wave's LDS slice.
 #pragma unroll
-    for (int u = 0; u < kIpt; ++u) {
-      const int k = tid + u * kBlock;
+    for (int u = 0; u < kWIpt; ++u) {
+      const int k = lane + u * kWave;
       if (k < nnzt) {
-        const A xv = x[cidx[u] * p.incx];
-        const A v = widen<A>(vv[u]);
+        const A v = widen<A>(cur.v[u]);
         if constexpr (TWOSUM) {
 #pragma clang fp contract(off)
-          const float prod = v * xv;
+          const float prod = v * xv[u];
           s_val[k] = prod;
-          s_err[k] = __builtin_fmaf(v, xv, -prod);
+          s_err[k] = __builtin_fmaf(v, xv[u], -prod);
         } else {
-          s_val[k] = v * xv;
+          s_val[k] = v * xv[u];
         }
       }
     }
-  }
-  __syncthreads();
-
-  // Thread-level merge-path search inside the tile.
-  const int d = tid * kIpt < items ? tid * kIpt : items;
-  int lo = d - nnzt > 0 ? d - nnzt : 0;
-  int hi = d < nrows ? d : nrows;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (s_rowend[mid] <= d - mid - 1) lo = mid + 1;
-    else hi = mid;
-  }
-  int r = lo, j = d - lo;
-  const int dend = d + kIpt < items ? d + kIpt : items;
-  Acc acc = Acc::zero();
-  int first_done = -1;
-  // Row sums land after the products: slot nnzt + row (fits: rows+nnz <= tile).
-  for (int step = d; step < dend; ++step) {
-    if (j < s_rowend[r]) {
-      if constexpr (TWOSUM) acc.add_pair(s_val[j], s_err[j]);
-      else acc.add(s_val[j]);
-      ++j;
-    } else {
-      s_val[nnzt + r] = acc.s;
-      if constexpr (TWOSUM) s_err[nnzt + r] = acc.c;
-      if (first_done < 0) first_done = r;
-      acc = Acc::zero();
-      ++r;
-    }
-  }
-  // Segmented inclusive scan of carries across the block.  key = the row
-  // still open when this thread stopped; keys never decrease with tid, so
-  // a run of equal keys is contiguous.  A thread's first finished row is
-  // always its predecessor's open row, so the predecessor's inclusive scan
-  // value is exactly the prefix that row is missing.
-  const int lane = tid % kWave;
-  const int wave = tid / kWave;
-  const int key = r;
-  Acc run = acc;
 #pragma unroll
-  for (int off = 1; off < kWave; off <<= 1) {
-    const int k_up = __shfl_up(key, off);
-    const Acc a_up = run.shfl_up(off);
-    if (lane >= off && k_up == key) {
-      Acc t = a_up;
-      t.merge_after(run);
-      run = t;
+    for (int u = 0; u < kWIpt; ++u) {
+      const int r = lane + u * kWave;
+      if (r <= nrows) s_end[r] = cur.e[u];
     }
-  }
-  int prev_key = __shfl_up(key, 1);
-  Acc prev = run.shfl_up(1);
-  if (lane == kWave - 1) {
-    s_wkey[wave] = key;
-    s_wacc[wave] = run;
-  }
-  __syncthreads();
-  // Inclusive scan value at the last lane of the previous wave.
-  int in_key = -1;
-  Acc in_acc = Acc::zero();
-  for (int w = 0; w < wave; ++w) {
-    if (s_wkey[w] == in_key) {
-      in_acc.merge_after(s_wacc[w]);
+    if (lane == 0 && nrows == kWTile) s_end[kWTile] = cur.e_last;
+    __builtin_amdgcn_wave_barrier();
+    // Lane-level merge walk of kWIpt items.
+    const int d = lane * kWIpt < items ? lane * kWIpt : items;
+    int lo = d - nnzt > 0 ? d - nnzt : 0;
+    int hi = d < nrows ? d : nrows;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (s_end[mid] <= d - mid - 1) lo = mid + 1;
+      else hi = mid;
+    }
+    int r = lo, j = d - lo;
+    const int dend = d + kWIpt < items ? d + kWIpt : items;
+    Acc acc = Acc::zero();
+    int first_done = -1;
+    for (int step = d; step < dend; ++step) {
+      if (j < s_end[r]) {
+        if constexpr (TWOSUM) acc.add_pair(s_val[j], s_err[j]);
+        else acc.add(s_val[j]);
+        ++j;
+      } else {
+        s_val[nnzt + r] = acc.s;
+        if constexpr (TWOSUM) s_err[nnzt + r] = acc.c;
+        if (first_done < 0) first_done = r;
+        acc = Acc::zero();
+        ++r;
+      }
+    }
+    // Segmented inclusive scan over lanes; keys (open rows) never decrease.
+    const int key = r;
+    Acc run = acc;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const int k_up = __shfl_up(key, off);
+      const Acc a_up = run.shfl_up(off);
+      if (lane >= off && k_up == key) {
+        Acc tt = a_up;
+        tt.merge_after(run);
+        run = tt;
+      }
+    }
+    // Prefix owed to this lane's first finished row: the predecessor's run,
+    // preceded by the carry from earlier tiles when that run is for row 0.
+    int prev_key = __shfl_up(key, 1);
+    Acc prev = run.shfl_up(1);
+    if (lane == 0) {
+      prev_key = 0;
+      prev = carry;
+    } else if (prev_key == 0) {
+      Acc tt = carry;
+      tt.merge_after(prev);
+      prev = tt;
+    }
+    if (first_done >= 0 && prev_key == first_done) {
+      Acc mine = Acc::zero();
+      mine.s = s_val[nnzt + first_done];
+      if constexpr (TWOSUM) mine.c = s_err[nnzt + first_done];
+      Acc tt = prev;
+      tt.merge_after(mine);
+      s_val[nnzt + first_done] = tt.s;
+      if constexpr (TWOSUM) s_err[nnzt + first_done] = tt.c;
+    }
+    // Open row at the tile end: the last lane's run (plus the old carry when
+    // no row finished in this tile).
+    Acc last;
+    last.s = __shfl(run.s, kWave - 1);
+    if constexpr (TWOSUM) last.c = __shfl(run.c, kWave - 1);
+    if (nrows == 0) {
+      Acc tt = carry;
+      tt.merge_after(last);
+      carry = tt;
     } else {
-      in_key = s_wkey[w];
-      in_acc = s_wacc[w];
+      carry = last;
     }
+    __builtin_amdgcn_wave_barrier();
+    // Coalesced epilogue for rows finished in this tile.
+#pragma unroll 1
+    for (int rr = lane; rr < nrows; rr += kWave) {
+      Acc a = Acc::zero();
+      a.s = s_val[nnzt + rr];
+      if constexpr (TWOSUM) a.c = s_err[nnzt + rr];
+      finish_row<V>(p, i0 + rr, a);
+    }
+    __builtin_amdgcn_wave_barrier();
+    // Advance.
+    i0 = i1;
+    j0 = j1;
+    i1 = ni1;
+    j1 = nj1;
+    cur = nxt;
   }
-  bool has_prev = true;
+  // Carry left at this wave's end: the open row, if this wave read any of it.
   if (lane == 0) {
-    has_prev = wave > 0;
-    prev_key = in_key;
-    prev = in_acc;
-  } else if (prev_key == in_key) {
-    Acc t = in_acc;
-    t.merge_after(prev);
-    prev = t;
-  }
-  if (first_done >= 0 && has_prev && prev_key == first_done) {
-    Acc t = prev;
-    Acc mine = Acc::zero();
-    mine.s = s_val[nnzt + first_done];
-    if constexpr (TWOSUM) mine.c = s_err[nnzt + first_done];
-    t.merge_after(mine);
-    s_val[nnzt + first_done] = t.s;
-    if constexpr (TWOSUM) s_err[nnzt + first_done] = t.c;
-  }
-  // Tile carry-out: the open row i1 when this tile holds some of its nz.
-  if (tid == kBlock - 1) {
-    Acc tail = run;
-    if (key == in_key) {
-      Acc t = in_acc;
-      t.merge_after(run);
-      tail = t;
-    }
-    const int64_t open_begin = i1 < p.rows ? p.offsets[i1] : p.nnz;
-    if (i1 < p.rows && j1 > open_begin && j1 > j0) {
-      carry_rows[tile] = i1;
-      static_cast<Acc*>(carry_vals)[tile] = tail;
+    const int64_t open_begin = i0 < p.rows ? p.offsets[i0] : p.nnz;
+    if (i0 < p.rows && j0 > open_begin) {
+      carry_rows[wave] = i0;
+      static_cast<Acc*>(carry_vals)[wave] = carry;
     } else {
-      carry_rows[tile] = -1;
+      carry_rows[wave] = -1;
     }
-  }
-  __syncthreads();
-  // Coalesced epilogue for the rows finished in this tile.
-  for (int rr = tid; rr < nrows; rr += kBlock) {
-    Acc a = Acc::zero();
-    a.s = s_val[nnzt + rr];
-    if constexpr (TWOSUM) a.c = s_err[nnzt + rr];
-    finish_row<V>(p, i0 + rr, a);
   }
 }
 
-// Adds tile carries to rows that crossed tile boundaries.  The first tile of
-// each run of equal carry rows sums the run in tile order and adds it once.
+// Adds wave carries to rows that crossed wave boundaries.  The first wave of
+// each run of equal carry rows sums the run in order and adds it once.
 template <typename V, bool TWOSUM>
 __global__ __launch_bounds__(kBlock) void spmv_merge_fixup_kernel(
-    KParams p, int64_t ntiles, const int64_t* __restrict__ carry_rows,
+    KParams p, int64_t nwaves, const int64_t* __restrict__ carry_rows,
     const void* __restrict__ carry_vals) {
   using Acc = typename AccSel<V, TWOSUM>::type;
   const int64_t t = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  if (t >= ntiles) return;
+  if (t >= nwaves) return;
   const int64_t row = carry_rows[t];
   if (row < 0) return;
   if (t > 0 && carry_rows[t - 1] == row) return;
   const Acc* vals = static_cast<const Acc*>(carry_vals);
   Acc sum = vals[t];
-  for (int64_t u = t + 1; u < ntiles && carry_rows[u] == row; ++u)
+  for (int64_t u = t + 1; u < nwaves && carry_rows[u] == row; ++u)
     sum.merge_after(vals[u]);
   add_carry<V>(p, row, sum);
 }
@@ -490,7 +547,37 @@ __global__ __launch_bounds__(kBlock) void spmv_merge_fixup_kernel(
 // ---------------------------------------------------------------------------
 int64_t merge_tiles(const SpmvArgs& a) {
   const int64_t total = a.a.outer + a.a.nnz;
-  return (total + kTile - 1) / kTile;
+  return (total + kWTile - 1) / kWTile;
+}
+
+int device_cus() {
+  static int cached[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (cached[dev] == 0) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
+      v = 256;
+    cached[dev] = v;
+  }
+  return cached[dev];
+}
+
+// Persistent grid for the merge kernel: as many resident blocks as the
+// kernel's occupancy allows, so every wave runs exactly one tile range.
+template <typename V, typename I, bool TWOSUM>
+int64_t merge_blocks() {
+  static int per_cu = 0;
+  if (per_cu == 0) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &nb, spmv_merge_kernel<V, I, TWOSUM>, kBlock, 0) != hipSuccess || nb <= 0)
+      nb = 4;
+    per_cu = nb;
+  }
+  int64_t blocks = static_cast<int64_t>(per_cu) * device_cus();
+  if (blocks * kWavesPerBlock > kMaxMergeWaves) blocks = kMaxMergeWaves / kWavesPerBlock;
+  return blocks;
 }
 
 template <typename V, bool TWOSUM>
@@ -522,14 +609,22 @@ hipError_t launch_typed(const SpmvArgs& a, const KParams& kp, void* ws,
       int64_t* coords = reinterpret_cast<int64_t*>(w);
       w += align256(sizeof(int64_t) * 2 * (nt + 1));
       int64_t* crow = reinterpret_cast<int64_t*>(w);
-      w += align256(sizeof(int64_t) * nt);
+      w += align256(sizeof(int64_t) * kMaxMergeWaves);
       void* cval = w;
+      // Waves: one per resident slot, each with a contiguous run of tiles.
+      int64_t blocks = merge_blocks<V, I, TWOSUM>();
+      int64_t waves = blocks * kWavesPerBlock;
+      if (waves > nt) waves = nt;
+      const int64_t per_wave = (nt + waves - 1) / waves;
+      waves = (nt + per_wave - 1) / per_wave;
+      blocks = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
+      const int64_t slots = blocks * kWavesPerBlock;
       merge_search_kernel<<<blocks_for(nt + 1, kBlock), kBlock, 0, s>>>(
           a.a.offsets, rows, a.a.nnz, nt, coords);
-      spmv_merge_kernel<V, I, TWOSUM><<<static_cast<unsigned>(nt), kBlock, 0, s>>>(
-          kp, coords, nt, crow, cval);
-      spmv_merge_fixup_kernel<V, TWOSUM><<<blocks_for(nt, kBlock), kBlock, 0, s>>>(
-          kp, nt, crow, cval);
+      spmv_merge_kernel<V, I, TWOSUM><<<static_cast<unsigned>(blocks), kBlock, 0, s>>>(
+          kp, coords, nt, per_wave, crow, cval);
+      spmv_merge_fixup_kernel<V, TWOSUM><<<blocks_for(slots, kBlock), kBlock, 0, s>>>(
+          kp, slots, crow, cval);
       return hipGetLastError();
     }
 #define ASC_VEC_CASE(ALG, G)                                                  \
@@ -562,7 +657,7 @@ size_t ws_typed(const SpmvArgs& a) {
   const int64_t nt = merge_tiles(a);
   const size_t cb = a.twosum ? carry_bytes<V, true>() : carry_bytes<V, false>();
   return align256(sizeof(int64_t) * 2 * (nt + 1)) +
-         align256(sizeof(int64_t) * nt) + align256(cb * nt);
+         align256(sizeof(int64_t) * kMaxMergeWaves) + align256(cb * kMaxMergeWaves);
 }
 
 }  // namespace
