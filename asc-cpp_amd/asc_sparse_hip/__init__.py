@@ -577,3 +577,80 @@ def upload_packed(ctx: Context, packed, x: np.ndarray, y: np.ndarray, kind: int,
         entries.append(BatchEntry(a, VectorDesc(bx.ptr + es * vs, r, 1, kind, DEVICE, 0),
                                   VectorDesc(by.ptr + es * vs, r, 1, kind, DEVICE, 1), alpha, beta))
     return entries, [bo, bi, bv, bx, by]
+
+
+# ---- variant selection: autotuner, plan cache, features, MLP --------------------------
+
+class Selector(C.Structure):
+    _fields_ = [("device_model", C.c_void_p), ("outputs", C.c_int32), ("margin", C.c_float),
+                ("clip", C.c_float), ("reserved", C.c_int32)]
+
+
+_SIGS2 = {
+    "asc_sparse_hip_plan_set": [_VP, P(SparseDesc), C.c_uint8],
+    "asc_sparse_hip_plan_get": [_VP, P(SparseDesc), P(C.c_uint8)],
+    "asc_sparse_hip_plan_clear": [_VP],
+    "asc_sparse_hip_csr_spmv_autotune": [_VP, P(SparseDesc), P(VectorDesc), P(VectorDesc), _VP, C.c_size_t,
+                                         C.c_int32, P(C.c_float), P(C.c_uint8)],
+    "asc_sparse_hip_csr_features_workspace_size": [_VP, P(C.c_size_t)],
+    "asc_sparse_hip_csr_features": [_VP, P(SparseDesc), _VP, _VP, C.c_size_t],
+    "asc_sparse_hip_mlp_sizes": [C.c_int32, C.c_int64, P(C.c_size_t), P(C.c_size_t)],
+    "asc_sparse_hip_mlp_forward": [_VP, C.c_int32, _VP, C.c_int64, _VP, _VP, _VP, C.c_size_t],
+    "asc_sparse_hip_mlp_gradient": [_VP, C.c_int32, _VP, C.c_int64, _VP, _VP, _VP, _VP, _VP, C.c_size_t],
+    "asc_sparse_hip_mlp_adam": [_VP, _VP, _VP, _VP, _VP, C.c_int64, C.c_float, C.c_float, C.c_float,
+                                C.c_float, C.c_int64],
+    "asc_sparse_hip_normalize_features": [_VP, _VP, _VP, _VP, C.c_int64, _VP],
+    "asc_sparse_hip_select_workspace_size": [_VP, P(Selector), P(C.c_size_t)],
+    "asc_sparse_hip_select": [_VP, P(SparseDesc), P(Selector), _VP, C.c_size_t, P(C.c_uint8), P(C.c_float)],
+}
+_SIGS.update(_SIGS2)
+if _lib is not None:  # signatures added after an early load()
+    for _n, _a in _SIGS2.items():
+        getattr(_lib, _n).argtypes = _a
+        getattr(_lib, _n).restype = Status
+
+
+def plan_set(ctx: Context, a: DeviceSparse, alg: int):
+    check(ctx.lib.asc_sparse_hip_plan_set(ctx.handle, C.byref(a.desc), alg))
+
+
+def plan_get(ctx: Context, a: DeviceSparse) -> int:
+    v = C.c_uint8()
+    check(ctx.lib.asc_sparse_hip_plan_get(ctx.handle, C.byref(a.desc), C.byref(v)))
+    return v.value
+
+
+def plan_clear(ctx: Context):
+    check(ctx.lib.asc_sparse_hip_plan_clear(ctx.handle))
+
+
+def autotune(ctx: Context, a: DeviceSparse, x: VectorDesc, y: VectorDesc, reps: int = 3,
+             ws: Workspace | None = None):
+    """Times every tunable variant; returns (best algorithm, {alg: ms})."""
+    opt = options(ALG_MERGE)
+    need = C.c_size_t()
+    check(ctx.lib.asc_sparse_hip_csr_spmv_workspace_size(ctx.handle, C.byref(a.desc), C.byref(x),
+                                                         C.byref(y), C.byref(opt), C.byref(need)))
+    ws = ws or Workspace(ctx)
+    p, n = ws.get(need.value)
+    times = (C.c_float * 10)(*([-1.0] * 10))
+    best = C.c_uint8()
+    check(ctx.lib.asc_sparse_hip_csr_spmv_autotune(ctx.handle, C.byref(a.desc), C.byref(x), C.byref(y),
+                                                   p, n, reps, times, C.byref(best)))
+    return best.value, {alg: float(times[alg]) for alg in TUNABLE_ALGS}
+
+
+def features(ctx: Context, a: DeviceSparse, ws: Workspace | None = None) -> np.ndarray:
+    need = C.c_size_t()
+    check(ctx.lib.asc_sparse_hip_csr_features_workspace_size(ctx.handle, C.byref(need)))
+    ws = ws or Workspace(ctx)
+    p, n = ws.get(need.value)
+    out = ctx.alloc(64)
+    check(ctx.lib.asc_sparse_hip_csr_features(ctx.handle, C.byref(a.desc), out.ptr, p, n))
+    return ctx.download(out, np.float32, 16)
+
+
+def mlp_sizes(k: int, batch: int):
+    pf, sb = C.c_size_t(), C.c_size_t()
+    check(load().asc_sparse_hip_mlp_sizes(k, batch, C.byref(pf), C.byref(sb)))
+    return pf.value, sb.value

@@ -95,6 +95,24 @@ struct BatchLaunch {
 };
 hipError_t batched_spmv_launch(const BatchLaunch& launch, hipStream_t stream);
 
+// ---- selector: features and MLP --------------------------------------------
+size_t features_workspace_bytes();
+hipError_t features_launch(const Compressed& a, float* out16_device, void* workspace,
+                           hipStream_t stream);
+size_t mlp_param_count(int k);
+size_t mlp_scratch_floats(int64_t batch, int k);
+hipError_t mlp_forward_launch(const float* params, int k, int64_t batch, const float* x,
+                              float* y, float* scratch, hipStream_t stream);
+hipError_t mlp_grad_launch(const float* params, int k, int64_t batch, const float* x,
+                           const float* target, float* grads, float* scratch,
+                           float* loss, hipStream_t stream);
+hipError_t mlp_adam_launch(float* params, const float* grads, float* m, float* v,
+                           int64_t n, float lr, float b1, float b2, float eps,
+                           int64_t step, hipStream_t stream);
+// y = (x - mean) / std over [n][16] rows, clamped to +-clip.
+hipError_t normalize_launch(const float* x, const float* mean, const float* stddev,
+                            int64_t n, float* out, hipStream_t stream);
+
 // ---- conversions ---------------------------------------------------------
 // Stable transpose of a compressed operand (CSR<->CSC): out_offsets has
 // inner+1 entries, out_indices nnz entries of out_index_bytes, out_values

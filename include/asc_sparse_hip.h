@@ -439,6 +439,90 @@ asc_hip_status asc_sparse_hip_batch_tune(asc_hip_context* context, int64_t count
                                          int32_t reps, asc_hip_batch_classes* out,
                                          float* class_ms);
 
+/* ---- variant selection: autotuner, plan cache, features, learned MLP ------ */
+
+/*
+ * No reference counterpart: the reference provider always runs one library
+ * algorithm (src/sparse/cuda/operations.cc:1004-1040).  The context keeps a
+ * small plan cache (matrix identity -> algorithm) consulted by
+ * ASC_HIP_ALG_AUTO; autotuning and selection fill it.
+ */
+asc_hip_status asc_sparse_hip_plan_set(asc_hip_context* context,
+                                       const asc_hip_sparse_desc* matrix,
+                                       uint8_t algorithm);
+asc_hip_status asc_sparse_hip_plan_get(asc_hip_context* context,
+                                       const asc_hip_sparse_desc* matrix,
+                                       uint8_t* out_algorithm);
+asc_hip_status asc_sparse_hip_plan_clear(asc_hip_context* context);
+
+/* Exhaustive autotuner: times every tunable variant (ASC_HIP_ALG_MERGE ..
+ * ASC_HIP_ALG_VECTOR64) as y = A*x with `reps` launches after a warm-up,
+ * writes per-algorithm milliseconds into times_ms[ASC_HIP_ALG_COUNT]
+ * (indexed by algorithm id; may be NULL), stores the fastest in the plan
+ * cache and returns it.  y is overwritten; the workspace must hold what
+ * the merge variant needs. */
+asc_hip_status asc_sparse_hip_csr_spmv_autotune(
+    asc_hip_context* context, const asc_hip_sparse_desc* matrix,
+    const asc_hip_vector_desc* x, const asc_hip_vector_desc* y, void* workspace,
+    size_t workspace_bytes, int32_t reps, float* times_ms, uint8_t* out_algorithm);
+
+/* Sixteen sparsity features of a trusted device CSR matrix (FP32, device
+ * output), computed in one deterministic pass. */
+asc_hip_status asc_sparse_hip_csr_features_workspace_size(asc_hip_context* context,
+                                                          size_t* out_bytes);
+asc_hip_status asc_sparse_hip_csr_features(asc_hip_context* context,
+                                           const asc_hip_sparse_desc* matrix,
+                                           float* device_features16, void* workspace,
+                                           size_t workspace_bytes);
+
+/* The selector MLP 16 -> 64 -> 64 -> k (ReLU), FP32, on device buffers:
+ * params (see asc_sparse_hip_mlp_sizes), x [batch][16], y / target
+ * [batch][k].  The gradient is of the mean squared error over batch*k
+ * outputs; `loss` (device float) receives that error. */
+asc_hip_status asc_sparse_hip_mlp_sizes(int32_t k, int64_t batch,
+                                        size_t* param_floats, size_t* scratch_bytes);
+asc_hip_status asc_sparse_hip_mlp_forward(asc_hip_context* context, int32_t k,
+                                          const float* params, int64_t batch,
+                                          const float* x, float* y, void* scratch,
+                                          size_t scratch_bytes);
+asc_hip_status asc_sparse_hip_mlp_gradient(asc_hip_context* context, int32_t k,
+                                           const float* params, int64_t batch,
+                                           const float* x, const float* target,
+                                           float* grads, float* loss, void* scratch,
+                                           size_t scratch_bytes);
+asc_hip_status asc_sparse_hip_mlp_adam(asc_hip_context* context, float* params,
+                                       const float* grads, float* m, float* v,
+                                       int64_t n, float lr, float beta1, float beta2,
+                                       float eps, int64_t step);
+/* out[i][f] = (x[i][f] - mean[f]) / stddev[f] for n rows of 16 features. */
+asc_hip_status asc_sparse_hip_normalize_features(asc_hip_context* context,
+                                                 const float* x, const float* mean,
+                                                 const float* stddev, int64_t n,
+                                                 float* out);
+
+/* A trained selector: device_model = [mean 16][stddev 16][MLP params].
+ * Output j predicts log(time / best time) of algorithm ASC_HIP_ALG_MERGE+j.
+ * Selection takes the smallest prediction unless unsure -- some normalized
+ * feature outside [-clip, clip], or the merge-path prediction within
+ * `margin` of the best -- in which case it falls back to merge-path.  The
+ * choice is stored in the plan cache. */
+typedef struct asc_hip_selector {
+  const float* device_model;
+  int32_t outputs;
+  float margin;
+  float clip;
+  int32_t reserved;
+} asc_hip_selector;
+
+asc_hip_status asc_sparse_hip_select_workspace_size(asc_hip_context* context,
+                                                    const asc_hip_selector* selector,
+                                                    size_t* out_bytes);
+asc_hip_status asc_sparse_hip_select(asc_hip_context* context,
+                                     const asc_hip_sparse_desc* matrix,
+                                     const asc_hip_selector* selector, void* workspace,
+                                     size_t workspace_bytes, uint8_t* out_algorithm,
+                                     float* out_scores);
+
 /* ---- host-forced fault injection (tests) -------------------------------- */
 
 /* Makes the `countdown`-th next allocation (0 = the very next) or event
