@@ -654,3 +654,27 @@ def mlp_sizes(k: int, batch: int):
     pf, sb = C.c_size_t(), C.c_size_t()
     check(load().asc_sparse_hip_mlp_sizes(k, batch, C.byref(pf), C.byref(sb)))
     return pf.value, sb.value
+
+_SIGS3 = {
+    "asc_sparse_hip_convert_to_coordinate_workspace_size": [_VP, P(SparseDesc), P(C.c_size_t)],
+    "asc_sparse_hip_convert_to_coordinate": [_VP, P(SparseDesc), _VP, _VP, P(SparseDesc), _VP, C.c_size_t],
+}
+_SIGS.update(_SIGS3)
+if _lib is not None:
+    for _n, _a in _SIGS3.items():
+        getattr(_lib, _n).argtypes = _a
+        getattr(_lib, _n).restype = Status
+
+
+def convert_to_coordinate(ctx: Context, a: DeviceSparse, ws: Workspace | None = None) -> DeviceSparse:
+    d = a.desc
+    need = C.c_size_t()
+    check(ctx.lib.asc_sparse_hip_convert_to_coordinate_workspace_size(ctx.handle, C.byref(d), C.byref(need)))
+    bc = ctx.alloc(max(16 * d.nnz, 1))
+    bv = ctx.alloc(max(ELEM_BYTES[d.element_kind] * d.nnz, 1))
+    ws = ws or Workspace(ctx)
+    p, n = ws.get(need.value)
+    out = SparseDesc()
+    check(ctx.lib.asc_sparse_hip_convert_to_coordinate(ctx.handle, C.byref(d), bc.ptr, bv.ptr,
+                                                       C.byref(out), p, n))
+    return DeviceSparse(out, [bc, bv])

@@ -714,4 +714,97 @@ asc_hip_status asc_sparse_hip_convert_compressed(asc_hip_context* ctx,
   return ok();
 }
 
+// ---- compressed -> canonical COO ------------------------------------------------
+
+namespace {
+
+// CSC sources go through a CSR transpose in the workspace first.
+struct ToCooPlan {
+  bool via_transpose = false;
+  size_t t_off = 0, t_idx = 0, t_val = 0, t_ws = 0, total = 0;
+};
+
+asc_hip_status prepare_to_coo(asc_hip_context* ctx, const asc_hip_sparse_desc* src,
+                              ToCooPlan* plan) {
+  asc_hip_status st = check_context(ctx);
+  if (st.code) return st;
+  if ((st = check_device_sparse(ctx, src, "source")).code) return st;
+  if (src->format == ASC_HIP_COO) return fail(ASC_HIP_INVALID_ARGUMENT, "source must be CSR or CSC");
+  if (src->format == ASC_HIP_CSC) {
+    if (src->nnz > kU32Max || src->rows > kU32Max || src->cols > kU32Max)
+      return fail(ASC_HIP_UNSUPPORTED, "device transpose supports extents and nnz below 2^32");
+    plan->via_transpose = true;
+    plan->t_off = align256(sizeof(int64_t) * (src->rows + 1));
+    plan->t_idx = align256(static_cast<size_t>(src->index_bytes) * src->nnz);
+    plan->t_val = align256(elem_size(src->element_kind) * src->nnz);
+    plan->t_ws = align256(asc_hip::transpose_workspace_bytes(as_compressed(*src, false)));
+    plan->total = plan->t_off + plan->t_idx + plan->t_val + plan->t_ws;
+  }
+  return ok();
+}
+
+}  // namespace
+
+asc_hip_status asc_sparse_hip_convert_to_coordinate_workspace_size(
+    asc_hip_context* ctx, const asc_hip_sparse_desc* src, size_t* out) {
+  if (out == nullptr) return fail(ASC_HIP_INVALID_ARGUMENT, "output pointer is null");
+  ToCooPlan plan;
+  asc_hip_status st = prepare_to_coo(ctx, src, &plan);
+  if (st.code) return st;
+  *out = plan.total;
+  return ok();
+}
+
+asc_hip_status asc_sparse_hip_convert_to_coordinate(asc_hip_context* ctx,
+                                                    const asc_hip_sparse_desc* src,
+                                                    void* d_coords, void* d_values,
+                                                    asc_hip_sparse_desc* out, void* ws,
+                                                    size_t ws_bytes) {
+  ToCooPlan plan;
+  asc_hip_status st = prepare_to_coo(ctx, src, &plan);
+  if (st.code) return st;
+  if (out == nullptr) return fail(ASC_HIP_INVALID_ARGUMENT, "output descriptor is null");
+  const size_t cb = 16 * static_cast<size_t>(src->nnz);
+  const size_t vb = elem_size(src->element_kind) * src->nnz;
+  if (src->nnz > 0) {
+    if ((st = check_device_pointer(ctx, d_coords, "coordinates destination")).code) return st;
+    if ((st = check_device_pointer(ctx, d_values, "values destination")).code) return st;
+    if (overlaps(span_of(d_coords, cb), span_of(d_values, vb)) ||
+        sparse_overlaps(*src, span_of(d_coords, cb)) || sparse_overlaps(*src, span_of(d_values, vb)))
+      return fail(ASC_HIP_INVALID_ARGUMENT, "conversion destinations overlap");
+  }
+  if ((st = check_workspace(ctx, ws, ws_bytes, plan.total,
+                            operand_spans(*src, {span_of(d_coords, cb), span_of(d_values, vb)}))).code)
+    return st;
+  if ((st = unavailable_if_validation_only(ctx)).code) return st;
+  DeviceGuard guard(ctx->device);
+  asc_hip::Compressed csr = as_compressed(*src, false);
+  hipError_t e = hipSuccess;
+  if (plan.via_transpose) {
+    char* w = static_cast<char*>(ws);
+    int64_t* off = reinterpret_cast<int64_t*>(w);
+    void* idx = w + plan.t_off;
+    void* val = w + plan.t_off + plan.t_idx;
+    e = asc_hip::transpose_launch(csr, off, idx, src->index_bytes, val,
+                                  w + plan.t_off + plan.t_idx + plan.t_val, ctx->stream);
+    csr.offsets = off;
+    csr.indices = idx;
+    csr.values = val;
+    csr.outer = src->rows;
+    csr.inner = src->cols;
+  }
+  if (e == hipSuccess)
+    e = asc_hip::csr_to_coo_launch(csr, static_cast<int64_t*>(d_coords), d_values, ctx->stream);
+  if ((st = finish_launch(ctx, e, "conversion to COO")).code) return st;
+  *out = *src;
+  out->format = ASC_HIP_COO;
+  out->outer_offsets = d_coords;
+  out->inner_indices = nullptr;
+  out->values = d_values;
+  out->index_bytes = 8;
+  out->offset_bytes = 8;
+  out->trusted = 1;
+  return ok();
+}
+
 }  // extern "C"

@@ -444,7 +444,63 @@ hipError_t transpose_typed(const Compressed& src, int64_t* out_offsets,
       ws, s);
 }
 
+// CSR -> canonical COO: coordinates (row, col) interleaved, values moved as
+// raw bits; G lanes walk each row (compressed.h:1781-1837).
+template <int G, typename I, typename VB>
+__global__ __launch_bounds__(kBlock) void csr_to_coo_kernel(
+    const int64_t* __restrict__ offsets, const I* __restrict__ idx,
+    const VB* __restrict__ vals, int64_t rows, int64_t* __restrict__ coords,
+    VB* __restrict__ out_vals) {
+  const int64_t r = xcd_remap(blockIdx.x, gridDim.x) * (kBlock / G) + threadIdx.x / G;
+  if (r >= rows) return;
+  const int64_t b = offsets[r], e = offsets[r + 1];
+  for (int64_t p = b + threadIdx.x % G; p < e; p += G) {
+    coords[2 * p] = r;
+    coords[2 * p + 1] = static_cast<int64_t>(ld_stream(idx + p));
+    out_vals[p] = ld_stream(vals + p);
+  }
+}
+
+template <typename I, typename VB>
+hipError_t csr_to_coo_typed(const int64_t* offsets, const I* idx, const VB* vals,
+                            int64_t rows, int64_t nnz, int64_t* coords, VB* out_vals,
+                            hipStream_t s) {
+  const int64_t avg = rows > 0 ? nnz / rows : 0;
+  if (avg >= 32)
+    csr_to_coo_kernel<64, I, VB><<<blocks_for(rows, kBlock / 64), kBlock, 0, s>>>(
+        offsets, idx, vals, rows, coords, out_vals);
+  else if (avg >= 4)
+    csr_to_coo_kernel<8, I, VB><<<blocks_for(rows, kBlock / 8), kBlock, 0, s>>>(
+        offsets, idx, vals, rows, coords, out_vals);
+  else
+    csr_to_coo_kernel<1, I, VB><<<blocks_for(rows, kBlock), kBlock, 0, s>>>(
+        offsets, idx, vals, rows, coords, out_vals);
+  return hipGetLastError();
+}
+
+template <typename VB>
+hipError_t csr_to_coo_by_index(const Compressed& a, int64_t* coords, void* out_vals,
+                               hipStream_t s) {
+  if (a.index_bytes == 4)
+    return csr_to_coo_typed<int32_t, VB>(a.offsets, static_cast<const int32_t*>(a.indices),
+                                         static_cast<const VB*>(a.values), a.outer, a.nnz,
+                                         coords, static_cast<VB*>(out_vals), s);
+  return csr_to_coo_typed<int64_t, VB>(a.offsets, static_cast<const int64_t*>(a.indices),
+                                       static_cast<const VB*>(a.values), a.outer, a.nnz,
+                                       coords, static_cast<VB*>(out_vals), s);
+}
+
 }  // namespace
+
+hipError_t csr_to_coo_launch(const Compressed& a, int64_t* coords, void* out_values,
+                             hipStream_t s) {
+  if (a.outer == 0 || a.nnz == 0) return hipSuccess;
+  switch (elem_bytes(a.elem)) {
+    case 8: return csr_to_coo_by_index<uint64_t>(a, coords, out_values, s);
+    case 4: return csr_to_coo_by_index<uint32_t>(a, coords, out_values, s);
+    default: return csr_to_coo_by_index<uint16_t>(a, coords, out_values, s);
+  }
+}
 
 size_t transpose_workspace_bytes(const Compressed& src) {
   const SortLayout L = sort_layout(src.nnz, src.inner, elem_bytes(src.elem));

@@ -133,6 +133,10 @@ hipError_t coo_convert_launch(const int64_t* coords, const void* values,
                               void* out_values, void* workspace,
                               hipStream_t stream);
 
+// CSR (outer = rows) -> canonical COO: interleaved int64 (row, col) and values.
+hipError_t csr_to_coo_launch(const Compressed& a, int64_t* coords, void* out_values,
+                             hipStream_t stream);
+
 // int64 -> int32 index narrowing (caller guarantees inner <= INT32_MAX).
 hipError_t narrow_indices_launch(const int64_t* src, int32_t* dst, int64_t n,
                                  hipStream_t stream);

@@ -1,19 +1,27 @@
-"""Benchmark: CSR SpMV GB/s and fraction of MI355X HBM peak.
+"""Benchmark: CSR SpMV GB/s and fraction of MI355X HBM peak, plus the
+reduced-precision, learned-selection and batched forms.
 
-Default workload (one MI355X): the single large matrix -- n = 2^22 rows and
+Headline workload (one MI355X): the single large matrix -- n = 2^22 rows and
 columns, row lengths Zipf(s=1.5) clipped to [1, 10^4] (about 6.4e8 nnz),
 sorted distinct uniform int32 columns, N(0,1) values, FP64, alpha=1, beta=0.
-One step = one compat SpMV y = alpha*A*x + beta*y over the whole matrix,
-inputs resident in HBM.  `value` = algorithmic bytes per SpMV / time:
+One step = one compat SpMV y = alpha*A*x + beta*y over the whole matrix with
+the variant the learned selector picks, inputs resident in HBM.  `value` is
+algorithmic bytes per SpMV / time:
 
   B = nnz*(s_v + s_i) + (rows+1)*s_o + cols*s_x + rows*s_y*(1 + [beta != 0])
 
-Prints one JSON line (see the repository's benchmark contract).
+Extra lines (same JSON object, "extra"): FP32 storage; SpMM with a 32-column
+row-major block; CSR->CSC and COO->CSR on the same matrix; the precision
+case (n = 2^20, 32 nnz/row, lognormal values: FP64 vs FP32/BF16/FP16
+storage, rows/s and error against the FP64 result); the batched case
+(10,000 matrices in one hipGraph-replayed persistent launch, SpMV/s).
+
+Prints one JSON line (the repository's benchmark contract).
 """
 import argparse
+import ctypes as C
 import json
 import os
-import platform
 import sys
 import time
 
@@ -24,6 +32,7 @@ sys.path.insert(0, os.path.join(ROOT, "asc-cpp_amd"))
 
 import asc_sparse_hip as H  # noqa: E402
 from asc_sparse_hip import gen as G  # noqa: E402
+from asc_sparse_hip import selector as S  # noqa: E402
 
 METRIC = "CSR SpMV GB/s & %HBM peak (FP64 plain vs BF16/learned-select); batched SpMV/s"
 HBM_PEAK_GBS = 8000.0
@@ -37,26 +46,25 @@ def spmv_bytes(rows, cols, nnz, sv, si, sx, sy, so=8, beta_nonzero=False):
     return nnz * (sv + si) + (rows + 1) * so + cols * sx + rows * sy * (2 if beta_nonzero else 1)
 
 
-def time_spmv(ctx, a, xd, yd, alg, steps, warmup, acc=H.ACC_PLAIN):
-    ws = H.Workspace(ctx)
+def time_ops(ctx, fn, steps, warmup, flush=None):
+    """Per-call HIP events on the context stream (median and mean, ms) and
+    the whole loop's events; `flush` (untimed) runs before every timed call."""
     for _ in range(warmup):
-        H.csr_spmv(ctx, 1.0, a, xd, 0.0, yd, alg=alg, acc=acc, ws=ws)
+        fn()
     ctx.synchronize()
-    # Whole-loop timing (enqueue + run), and per-call events on the same stream.
     starts, stops = [], []
-    t0 = time.perf_counter()
     first = ctx.record()
     for _ in range(steps):
+        if flush is not None:
+            flush()
         starts.append(ctx.record())
-        H.csr_spmv(ctx, 1.0, a, xd, 0.0, yd, alg=alg, acc=acc, ws=ws)
+        fn()
         stops.append(ctx.record())
     last = ctx.record()
     ctx.synchronize()
-    wall = time.perf_counter() - t0
-    per_call = [s.elapsed_ms(e) for s, e in zip(starts, stops)]
-    total_ms = first.elapsed_ms(last)
-    return {"ms_per_step": total_ms / steps, "kernel_ms_mean": float(np.mean(per_call)),
-            "kernel_ms_median": float(np.median(per_call)), "wall_s": wall}
+    per = [s.elapsed_ms(e) for s, e in zip(starts, stops)]
+    return {"loop_ms": first.elapsed_ms(last) / steps, "mean_ms": float(np.mean(per)),
+            "median_ms": float(np.median(per))}
 
 
 def measured_traffic(rows, nnz, alg):
@@ -85,10 +93,9 @@ def cpu_baseline(m, sample_rows, reps=3):
     y = np.zeros(r)
     kind = "port"
     fn = lambda: O.compat_spmv(off, idx, val, 1.0, x, 0.0, y)  # noqa: E731
-    ref = getattr(O, "ref_compat_spmv", None)
-    if ref is not None and O.ref_available():
+    if O.ref_available():
         kind = "reference"
-        fn = lambda: ref(off, idx, val, 1.0, x, 0.0, y)  # noqa: E731
+        fn = lambda: O.ref_compat_spmv(off, idx, val, 1.0, x, 0.0, y, cols=m.cols)  # noqa: E731
     fn()
     times = []
     for _ in range(reps):
@@ -97,10 +104,197 @@ def cpu_baseline(m, sample_rows, reps=3):
         times.append(time.perf_counter() - t0)
     t = float(np.median(times))
     b = spmv_bytes(r, m.cols, nnz, 8, 8, 8, 8)  # the reference's int64 indices
-    return {"value": b / t / 1e9, "unit": "GB/s", "cores": 1, "kind": kind,
+    return {"value": round(b / t / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": kind,
             "sample": f"rows [0, {r}) of the same matrix ({nnz} nnz, int64 indices as in the "
-                      f"reference), compat SpMV FP64, median of {reps} after 1 warmup, "
-                      f"{t * 1e3:.1f} ms per SpMV"}
+                      f"reference), compat SpMV FP64 on one host core, median of {reps} after "
+                      f"1 warm-up, {t * 1e3:.1f} ms per SpMV"}
+
+
+def learned_pick(ctx, a):
+    """Runs the committed learned selector on the device matrix."""
+    if not os.path.exists(S.DEFAULT_MODEL):
+        return H.ALG_MERGE, None
+    model = S.load_model()
+    sel, keep = S.device_selector(ctx, model)
+    need = C.c_size_t()
+    H.check(ctx.lib.asc_sparse_hip_select_workspace_size(ctx.handle, C.byref(sel), C.byref(need)))
+    ws = ctx.alloc(need.value)
+    alg = C.c_uint8()
+    scores = (C.c_float * model["k"])()
+    H.check(ctx.lib.asc_sparse_hip_select(ctx.handle, C.byref(a.desc), C.byref(sel), ws.ptr,
+                                          ws.nbytes, C.byref(alg), scores))
+    return alg.value, [round(float(s), 4) for s in scores]
+
+
+def bench_large(ctx, args, extra):
+    t0 = time.perf_counter()
+    m = G.zipf_large(args.rows)
+    log(f"generated n={m.rows} nnz={m.nnz} in {time.perf_counter() - t0:.1f}s")
+    a = H.clone_compressed(ctx, H.CSR, m.rows, m.cols, m.offsets, m.indices, m.values, H.F64, 4)
+    x = G.vector(m.cols, 12345)
+    xb, yb = ctx.upload(x), ctx.alloc(8 * m.rows)
+    xd, yd = H.vector(xb, m.cols, H.F64, writable=False), H.vector(yb, m.rows, H.F64)
+    ws = H.Workspace(ctx)
+    if args.calibrate:
+        # A plain copy of known size: HBM counters read it as ~1 GiB fetched
+        # and 1 GiB written, which scales the SpMV kernel's counters.
+        ca, cb = ctx.alloc(1 << 30), ctx.alloc(1 << 30)
+        ctx.fill(ca, 1)
+        ctx.copy(cb.ptr, ca.ptr, 1 << 30)
+        ctx.synchronize()
+        del ca, cb
+    if args.alg == "auto":
+        alg, scores = learned_pick(ctx, a)
+        extra["learned_select"] = {"algorithm": H.ALG_NAMES[alg], "scores": scores}
+    else:
+        alg = {"merge": H.ALG_MERGE, "ordered": H.ALG_ORDERED,
+               **{f"vec{g}": v for g, v in H.ALG_VECTOR.items()}}[args.alg]
+    name = H.ALG_NAMES[alg]
+    res = time_ops(ctx, lambda: H.csr_spmv(ctx, 1.0, a, xd, 0.0, yd, alg=alg, ws=ws),
+                   args.steps, args.warmup)
+    nbytes = spmv_bytes(m.rows, m.cols, m.nnz, 8, 4, 8, 8)
+    log(f"FP64 {name}: {res['loop_ms']:.3f} ms/step, {nbytes / res['loop_ms'] / 1e6:.0f} GB/s")
+    if args.sweep or not args.quick:
+        sweep = {}
+        for v in H.TUNABLE_ALGS:
+            r = time_ops(ctx, lambda: H.csr_spmv(ctx, 1.0, a, xd, 0.0, yd, alg=v, ws=ws), 3, 1)
+            sweep[H.ALG_NAMES[v]] = round(nbytes / r["median_ms"] / 1e6, 1)
+        extra["fp64_variant_gbs"] = sweep
+        log(f"variants: {sweep}")
+    if not args.quick:
+        a32 = H.convert_values(ctx, a, H.F32)
+        xb32, yb32 = ctx.upload(x.astype(np.float32)), ctx.alloc(4 * m.rows)
+        x32d, y32d = H.vector(xb32, m.cols, H.F32, writable=False), H.vector(yb32, m.rows, H.F32)
+        r32 = time_ops(ctx, lambda: H.csr_spmv(ctx, 1.0, a32, x32d, 0.0, y32d, alg=alg, ws=ws), 5, 2)
+        b32 = spmv_bytes(m.rows, m.cols, m.nnz, 4, 4, 4, 4)
+        extra["fp32_same_matrix"] = {"gbs": round(b32 / r32["median_ms"] / 1e6, 1),
+                                     "ms": round(r32["median_ms"], 3), "algorithm": name}
+        del a32, xb32, yb32
+        # SpMM: C += A*B, B row-major 2^22 x 32 (U[-1,1]).
+        k = 32
+        bmat = ctx.alloc(8 * m.cols * k)
+        bh = G.vector(m.cols * k, 777)
+        ctx.copy(bmat.ptr, bh.ctypes.data, bh.nbytes)
+        cmat = ctx.alloc(8 * m.rows * k)
+        ctx.fill(cmat, 0)
+        ctx.synchronize()
+        bd = H.dense(bmat, m.cols, k, k, H.ROW_MAJOR, H.F64, writable=False)
+        cd = H.dense(cmat, m.rows, k, k, H.ROW_MAJOR, H.F64)
+        rs = time_ops(ctx, lambda: H.standard_spmm(ctx, H.OP_N, 1.0, a, bd, cd), 2, 1)
+        bs = m.nnz * 12 + (m.rows + 1) * 8 + m.cols * k * 8 + 2 * m.rows * k * 8
+        extra["spmm_k32_row_major"] = {"gbs": round(bs / rs["median_ms"] / 1e6, 1),
+                                       "ms": round(rs["median_ms"], 3)}
+        log(f"SpMM k=32: {rs['median_ms']:.2f} ms")
+        del bmat, cmat, bh
+        # Conversions on the same matrix.
+        cws = H.Workspace(ctx)
+        holder = {}
+
+        def to_csc():
+            holder["csc"] = H.convert_compressed(ctx, a, cws)
+        rc = time_ops(ctx, to_csc, 2, 1)
+        bc = 2 * m.nnz * 12 + (m.rows + 1) * 8 + (m.cols + 1) * 8
+        extra["csr_to_csc"] = {"gbs": round(bc / rc["median_ms"] / 1e6, 1), "ms": round(rc["median_ms"], 3)}
+        holder.clear()
+        del cws
+        coo = H.convert_to_coordinate(ctx, a)
+
+        def to_csr():
+            holder["csr"] = H.convert_coordinate(ctx, coo, H.CSR, 4)
+        rr = time_ops(ctx, to_csr, 3, 1)
+        bcoo = m.nnz * (16 + 8) + m.nnz * (4 + 8) + (m.rows + 1) * 8
+        extra["coo_to_csr"] = {"gbs": round(bcoo / rr["median_ms"] / 1e6, 1), "ms": round(rr["median_ms"], 3)}
+        log(f"CSR->CSC {rc['median_ms']:.2f} ms, COO->CSR {rr['median_ms']:.2f} ms")
+        holder.clear()
+        del coo
+    return m, alg, res, nbytes
+
+
+def bench_precision(ctx, extra):
+    """n = 2^20, 32 nnz/row: FP64 vs FP32/BF16/FP16 storage (FP32 x, y, FP32
+    accumulation plain or compensated).  MALL is flushed before each timed
+    call (the BF16 working set, 222 MB, would otherwise stay cached)."""
+    m = G.precision_matrix(1 << 20)
+    n = m.rows
+    a64 = H.clone_compressed(ctx, H.CSR, n, n, m.offsets, m.indices, m.values, H.F64, 4)
+    aabs = H.clone_compressed(ctx, H.CSR, n, n, m.offsets, m.indices, np.abs(m.values), H.F64, 4)
+    x64 = G.vector(n, 4242)
+    ws = H.Workspace(ctx)
+    flush = ctx.alloc(1 << 30)
+    xb64, xabs = ctx.upload(x64), ctx.upload(np.abs(x64))
+    yref, yabs = ctx.alloc(8 * n), ctx.alloc(8 * n)
+    xd64 = H.vector(xb64, n, H.F64, writable=False)
+    # FP64 reference on the GPU in ordered mode (bit-exact with the serial reference).
+    H.csr_spmv(ctx, 1.0, a64, xd64, 0.0, H.vector(yref, n, H.F64), alg=H.ALG_ORDERED, ws=ws)
+    H.csr_spmv(ctx, 1.0, aabs, H.vector(xabs, n, H.F64, writable=False), 0.0,
+               H.vector(yabs, n, H.F64), alg=H.ALG_ORDERED, ws=ws)
+    y64 = ctx.download(yref, np.float64, n)
+    norm = float(np.max(ctx.download(yabs, np.float64, n)))
+    out = {}
+    yb64 = ctx.alloc(8 * n)
+    yd64 = H.vector(yb64, n, H.F64)
+    best64, _ = H.autotune(ctx, a64, xd64, yd64, 2, ws)
+    r = time_ops(ctx, lambda: H.csr_spmv(ctx, 1.0, a64, xd64, 0.0, yd64, alg=best64, ws=ws),
+                 7, 2, flush=lambda: ctx.fill(flush, 0))
+    fp64_rows = n / (r["median_ms"] * 1e-3)
+    out["fp64"] = {"rows_per_s": fp64_rows, "ms": round(r["median_ms"], 4),
+                   "algorithm": H.ALG_NAMES[best64]}
+    xb32 = ctx.upload(x64.astype(np.float32))
+    x32d = H.vector(xb32, n, H.F32, writable=False)
+    vmax = float(np.max(np.abs(m.values)))
+    for kind, kname in ((H.F32, "fp32"), (H.BF16, "bf16"), (H.F16, "fp16")):
+        scale = 1.0
+        if kind == H.F16:
+            while vmax * scale > 32768.0:
+                scale *= 0.5
+        ak = H.convert_values(ctx, a64, kind, scale)
+        yb = ctx.alloc(4 * n)
+        yd = H.vector(yb, n, H.F32)
+        best, _ = H.autotune(ctx, ak, x32d, yd, 2, ws)
+        for acc, aname in ((H.ACC_PLAIN, "plain"), (H.ACC_TWOSUM, "twosum")):
+            rr = time_ops(ctx, lambda: H.csr_spmv(ctx, 1.0 / scale, ak, x32d, 0.0, yd, alg=best,
+                                                  acc=acc, ws=ws), 7, 2,
+                          flush=lambda: ctx.fill(flush, 0))
+            y = ctx.download(yb, np.float32, n).astype(np.float64)
+            err = float(np.max(np.abs(y - y64)) / norm)
+            rows_s = n / (rr["median_ms"] * 1e-3)
+            out[f"{kname}_{aname}"] = {"rows_per_s": rows_s, "ms": round(rr["median_ms"], 4),
+                                       "speedup_vs_fp64": round(rows_s / fp64_rows, 3),
+                                       "rel_error_vs_fp64": err, "algorithm": H.ALG_NAMES[best],
+                                       "value_scale": scale}
+        del ak, yb
+    log("precision: " + ", ".join(f"{k} {v.get('speedup_vs_fp64', 1.0)}x err "
+                                  f"{v.get('rel_error_vs_fp64', 0):.2e}" for k, v in out.items()))
+    extra["precision_n2^20_32nnz"] = out
+
+
+def bench_batched(ctx, extra, count):
+    t0 = time.perf_counter()
+    specs = G.batched_specs(count)
+    mats = [G.make_spec(s) for s in specs]
+    p = G.pack(mats)
+    del mats
+    n = int(p.rows.sum())
+    nnz = int(p.nnz.sum())
+    log(f"batched: {count} matrices, {n} rows, {nnz} nnz generated in {time.perf_counter() - t0:.1f}s")
+    res = {}
+    for kind, kname in ((H.F64, "fp64"), (H.F32, "fp32")):
+        entries, bufs = H.upload_packed(ctx, p, G.vector(n, 1), np.zeros(n), kind, 1.0, 0.0)
+        b = H.Batch(ctx, entries, keep=bufs)
+        lanes, _ = b.tune(reps=2)
+        b.capture()
+        r = time_ops(ctx, b.replay, 10, 2)
+        es = 8 if kind == H.F64 else 4
+        by = nnz * (es + 4) + (n + count) * 8 + n * es * 2
+        res[kname] = {"spmv_per_s": count / (r["median_ms"] * 1e-3),
+                      "ms_per_batch": round(r["median_ms"], 4),
+                      "gbs": round(by / r["median_ms"] / 1e6, 1), "class_lanes": lanes,
+                      "work_items": int(b.plan.items)}
+        log(f"batched {kname}: {r['median_ms']:.3f} ms per launch of {count} SpMVs "
+            f"({res[kname]['spmv_per_s']:.3g} SpMV/s, {res[kname]['gbs']} GB/s)")
+        del b, entries, bufs
+    res["matrices"], res["rows"], res["nnz"] = count, n, nnz
+    extra["batched_10k"] = res
 
 
 def main():
@@ -109,10 +303,11 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--rows", type=int, default=1 << 22)
-    ap.add_argument("--alg", default="merge")
+    ap.add_argument("--alg", default="auto", help="auto (learned selector), merge, ordered, vecN")
     ap.add_argument("--cpu-sample-rows", type=int, default=1 << 19)
+    ap.add_argument("--batch-count", type=int, default=10_000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--quick", action="store_true", help="headline line only")
     ap.add_argument("--sweep", action="store_true", help="time every SpMV variant")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run one 1 GiB device copy (counter calibration)")
@@ -124,67 +319,23 @@ def main():
     if H.device_count() < 1:
         print(json.dumps({"error": "no HIP device visible"}))
         sys.exit(1)
-    algs = {"merge": H.ALG_MERGE, "ordered": H.ALG_ORDERED}
-    algs.update({f"vec{g}": v for g, v in H.ALG_VECTOR.items()})
-    alg = algs[args.alg]
-
-    t0 = time.perf_counter()
-    m = G.zipf_large(args.rows)
-    log(f"generated n={m.rows} nnz={m.nnz} in {time.perf_counter() - t0:.1f}s")
     ctx = H.Context(0)
-    t0 = time.perf_counter()
-    a = H.clone_compressed(ctx, H.CSR, m.rows, m.cols, m.offsets, m.indices, m.values, H.F64, 4)
-    x = G.vector(m.cols, 12345)
-    xb = ctx.upload(x)
-    yb = ctx.alloc(8 * m.rows)
-    xd = H.vector(xb, m.cols, H.F64, writable=False)
-    yd = H.vector(yb, m.rows, H.F64)
-    log(f"staged on device in {time.perf_counter() - t0:.1f}s")
-
-    if args.calibrate:
-        # A plain copy of known size: HBM counters read ~1 GiB fetched and
-        # ~1 GiB written for it, which scales the SpMV kernel's counters.
-        ca, cb = ctx.alloc(1 << 30), ctx.alloc(1 << 30)
-        ctx.fill(ca, 1)
-        ctx.copy(cb.ptr, ca.ptr, 1 << 30)
-        ctx.synchronize()
-        del ca, cb
-    res = time_spmv(ctx, a, xd, yd, alg, args.steps, args.warmup)
-    nbytes = spmv_bytes(m.rows, m.cols, m.nnz, 8, 4, 8, 8)
-    gbs = nbytes / (res["ms_per_step"] * 1e-3) / 1e9
-    kgbs = nbytes / (res["kernel_ms_mean"] * 1e-3) / 1e9
-    log(f"FP64 {args.alg}: {res['ms_per_step']:.3f} ms/step, {gbs:.0f} GB/s "
-        f"(per-call events {res['kernel_ms_mean']:.3f} ms, {kgbs:.0f} GB/s)")
-
     extra = {}
-    if args.sweep:
-        sweep = {}
-        for name, v in algs.items():
-            r = time_spmv(ctx, a, xd, yd, v, max(3, args.steps // 4), 2)
-            sweep[name] = round(nbytes / (r["kernel_ms_mean"] * 1e-3) / 1e9, 1)
-            log(f"  sweep {name}: {r['kernel_ms_mean']:.3f} ms  {sweep[name]} GB/s")
-        extra["fp64_variant_gbs"] = sweep
-    if not args.no_extra:
-        # FP32-stored values of the same matrix (FP32 x, y).
-        a32 = H.convert_values(ctx, a, H.F32)
-        xb32 = ctx.upload(x.astype(np.float32))
-        yb32 = ctx.alloc(4 * m.rows)
-        r32 = time_spmv(ctx, a32, H.vector(xb32, m.cols, H.F32, writable=False),
-                        H.vector(yb32, m.rows, H.F32), alg, args.steps, args.warmup)
-        b32 = spmv_bytes(m.rows, m.cols, m.nnz, 4, 4, 4, 4)
-        extra["fp32_gbs"] = round(b32 / (r32["kernel_ms_mean"] * 1e-3) / 1e9, 1)
-        extra["fp32_rows_per_s"] = m.rows / (r32["kernel_ms_mean"] * 1e-3)
-        extra["fp64_rows_per_s"] = m.rows / (res["kernel_ms_mean"] * 1e-3)
-        log(f"FP32 {args.alg}: {r32['kernel_ms_mean']:.3f} ms, {extra['fp32_gbs']} GB/s")
-        del a32, xb32, yb32
-
+    m, alg, res, nbytes = bench_large(ctx, args, extra)
+    name = H.ALG_NAMES[alg]
+    gbs = nbytes / (res["loop_ms"] * 1e-3) / 1e9
+    kgbs = nbytes / (res["mean_ms"] * 1e-3) / 1e9
     cpu = None
     if not args.no_cpu:
         t0 = time.perf_counter()
         cpu = cpu_baseline(m, args.cpu_sample_rows)
-        log(f"cpu baseline {cpu['value']:.2f} GB/s ({time.perf_counter() - t0:.1f}s)")
-
-    traffic, traffic_src = measured_traffic(m.rows, m.nnz, args.alg)
+        log(f"cpu baseline {cpu['value']} GB/s ({time.perf_counter() - t0:.1f}s)")
+    rows, nnz = m.rows, m.nnz
+    del m
+    if not args.quick:
+        bench_precision(ctx, extra)
+        bench_batched(ctx, extra, args.batch_count)
+    traffic, traffic_src = measured_traffic(rows, nnz, name)
     out = {
         "metric": METRIC,
         "value": round(gbs, 1),
@@ -192,7 +343,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(res["ms_per_step"], 4),
+        "ms_per_step": round(res["loop_ms"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -201,14 +352,14 @@ def main():
         "config": {"workload": "single large matrix: n=2^22 square, Zipf(1.5) row lengths "
                                "clipped to [1,1e4], int32 columns, int64 offsets, N(0,1) FP64 "
                                "values, x U[-1,1], alpha=1, beta=0",
-                   "rows": m.rows, "cols": m.cols, "nnz": m.nnz, "algorithm": args.alg,
+                   "rows": rows, "cols": rows, "nnz": nnz, "algorithm": name,
+                   "selection": "learned selector" if args.alg == "auto" else "forced",
                    "bytes_per_spmv": nbytes, "hbm_frac_of_peak": round(gbs / HBM_PEAK_GBS, 4)},
         "roofline": {"bound": "hbm", "achieved": round(kgbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(kgbs / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "traffic_source": traffic_src},
         "cpu_baseline": cpu,
         "extra": extra,
-        "host": platform.node(),
     }
     print(json.dumps(out), flush=True)
     ctx.close()

@@ -368,6 +368,19 @@ asc_hip_status asc_sparse_hip_convert_compressed(
     void* out_offsets, void* out_indices, void* out_values,
     asc_hip_sparse_desc* out, void* workspace, size_t workspace_bytes);
 
+/*
+ * Trusted device CSR or CSC -> canonical COO (row-major, interleaved int64
+ * coordinates; values moved bit for bit), the device form of
+ * ConvertToCoordinate (compressed.h:1781-1837).  CSC sources are transposed
+ * in the workspace first (zero workspace for CSR).
+ */
+asc_hip_status asc_sparse_hip_convert_to_coordinate_workspace_size(
+    asc_hip_context* context, const asc_hip_sparse_desc* source, size_t* out_bytes);
+asc_hip_status asc_sparse_hip_convert_to_coordinate(
+    asc_hip_context* context, const asc_hip_sparse_desc* source,
+    void* device_coordinates, void* device_values, asc_hip_sparse_desc* out,
+    void* workspace, size_t workspace_bytes);
+
 /* ---- batched SpMV --------------------------------------------------------- */
 
 /*

@@ -103,3 +103,36 @@ def test_transpose_many_digits(ctx):
     ro, ri, rv = O.transpose(0, rows, cols, m.offsets, m.indices, m.values)
     csr = H.clone_compressed(ctx, H.CSR, rows, cols, m.offsets, m.indices, m.values, H.F64, 4)
     check_compressed(ctx, H.convert_compressed(ctx, csr), ro, ri, rv)
+
+
+@pytest.mark.parametrize("fmt", [H.CSR, H.CSC])
+def test_compressed_to_coordinate(ctx, fmt):
+    g = GOLD["conversion"]
+    rows, cols = g["shape"]
+    off = g["csr_offsets"] if fmt == H.CSR else g["csc_offsets"]
+    idx = g["csr_indices"] if fmt == H.CSR else g["csc_indices"]
+    val = np.array(g["csr_values"] if fmt == H.CSR else g["csc_values"], float)
+    a = H.clone_compressed(ctx, fmt, rows, cols, np.array(off), np.array(idx), val, H.F64)
+    coo = H.convert_to_coordinate(ctx, a)
+    coords = ctx.download(H._PtrView(coo.desc.outer_offsets), np.int64, 2 * coo.nnz)
+    vals = ctx.download(H._PtrView(coo.desc.values), np.float64, coo.nnz)
+    assert coords.tolist() == g["coo"]
+    np.testing.assert_array_equal(vals, g["coo_values"])
+
+
+def test_compressed_to_coordinate_random_round_trip(ctx):
+    m = G.make(G.ZIPF, 30000, 20000, seed=12, values=G.VALUES_NORMAL, zipf_max=3000)
+    rc, rv = O.compressed_to_coo(0, m.rows, m.offsets, m.indices, m.values)
+    for fmt in (H.CSR, H.CSC):
+        if fmt == H.CSR:
+            a = H.clone_compressed(ctx, H.CSR, m.rows, m.cols, m.offsets, m.indices, m.values, H.F64, 4)
+        else:
+            co, ci, cv = O.transpose(0, m.rows, m.cols, m.offsets, m.indices, m.values)
+            a = H.clone_compressed(ctx, H.CSC, m.rows, m.cols, co, ci, cv, H.F64, 4)
+        coo = H.convert_to_coordinate(ctx, a)
+        coords = ctx.download(H._PtrView(coo.desc.outer_offsets), np.int64, 2 * coo.nnz)
+        vals = ctx.download(H._PtrView(coo.desc.values), np.float64, coo.nnz)
+        np.testing.assert_array_equal(coords, rc)
+        np.testing.assert_array_equal(vals.view(np.uint64), rv.view(np.uint64))
+        back = H.convert_coordinate(ctx, coo, H.CSR, 4)
+        check_compressed(ctx, back, m.offsets, m.indices, m.values)
