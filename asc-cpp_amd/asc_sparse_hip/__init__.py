@@ -32,8 +32,9 @@ DETERMINISTIC, BACKEND_DEFAULT = 0, 1
 
 ALG_AUTO, ALG_ORDERED, ALG_MERGE = 0, 1, 2
 ALG_VECTOR = {1: 3, 2: 4, 4: 5, 8: 6, 16: 7, 32: 8, 64: 9}
+ALG_PANEL = 10
 ALG_NAMES = {2: "merge", 3: "vec1", 4: "vec2", 5: "vec4", 6: "vec8", 7: "vec16",
-             8: "vec32", 9: "vec64", 1: "ordered"}
+             8: "vec32", 9: "vec64", 1: "ordered", 10: "panel"}
 TUNABLE_ALGS = (2, 3, 4, 5, 6, 7, 8, 9)
 ACC_PLAIN, ACC_TWOSUM = 0, 1
 INJECT_ALLOCATION, INJECT_EVENT = 0, 1
@@ -678,3 +679,49 @@ def convert_to_coordinate(ctx: Context, a: DeviceSparse, ws: Workspace | None = 
     check(ctx.lib.asc_sparse_hip_convert_to_coordinate(ctx.handle, C.byref(d), bc.ptr, bv.ptr,
                                                        C.byref(out), p, n))
     return DeviceSparse(out, [bc, bv])
+
+
+class PanelOptions(C.Structure):
+    _fields_ = [("panel_columns", C.c_int64), ("long_row_min", C.c_int64),
+                ("short_algorithm", C.c_uint8), ("long_algorithm", C.c_uint8),
+                ("reserved", C.c_uint8 * 6)]
+
+
+_SIGS4 = {
+    "asc_sparse_hip_csr_spmv_analysis_size": [_VP, P(SparseDesc), P(PanelOptions), P(C.c_size_t)],
+    "asc_sparse_hip_csr_spmv_analyze": [_VP, P(SparseDesc), P(PanelOptions), _VP, C.c_size_t],
+}
+_SIGS.update(_SIGS4)
+if _lib is not None:
+    for _n, _a in _SIGS4.items():
+        getattr(_lib, _n).argtypes = _a
+        getattr(_lib, _n).restype = Status
+
+
+def panel_options(panel_columns: int = 0, long_row_min: int = 0, short_alg: int = ALG_AUTO,
+                  long_alg: int = ALG_AUTO) -> PanelOptions:
+    return PanelOptions(panel_columns, long_row_min, short_alg, long_alg)
+
+
+def csr_spmv_analysis_size(ctx: Context, a: DeviceSparse, opts: PanelOptions | None = None) -> int:
+    need = C.c_size_t()
+    check(ctx.lib.asc_sparse_hip_csr_spmv_analysis_size(ctx.handle, C.byref(a.desc),
+                                                        C.byref(opts) if opts else None, C.byref(need)))
+    return need.value
+
+
+def csr_spmv_analyze(ctx: Context, a: DeviceSparse, opts: PanelOptions | None = None) -> Buffer:
+    """Column-panel analysis: registers ALG_PANEL for `a` in the context's plan
+    cache.  The analysis buffer is kept alive with the matrix (`a.analysis`),
+    since the plan reads it on every ALG_PANEL / ALG_AUTO call."""
+    need = csr_spmv_analysis_size(ctx, a, opts)
+    buffer = ctx.alloc(need)
+    check(ctx.lib.asc_sparse_hip_csr_spmv_analyze(ctx.handle, C.byref(a.desc),
+                                                  C.byref(opts) if opts else None, buffer.ptr,
+                                                  buffer.nbytes))
+    previous = getattr(a, "analysis", None)
+    a.analysis = buffer
+    if previous is not None:
+        ctx.synchronize()  # queued work may still read the replaced plan
+        previous.free()
+    return buffer

@@ -27,6 +27,7 @@ struct asc_hip_context {
     int64_t rows, cols, nnz;
     uint8_t element_kind;
     uint8_t algorithm;
+    asc_hip::PanelPlan panel;  // ASC_HIP_ALG_PANEL only
   };
   std::vector<Plan> plans;
 };

@@ -1,6 +1,7 @@
 // C ABI operations: staging, SpMV, SpMM and conversions.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <limits>
@@ -65,18 +66,90 @@ struct SpmvPlan {
   bool noop = false;
 };
 
+// The algorithm a call runs: the explicit choice, else the plan cache's entry
+// for the matrix, else merge-path.  `panel` receives the matrix's
+// column-panel analysis when the cache holds one.
 uint8_t resolve_alg(asc_hip_context* ctx, const asc_hip_sparse_desc& a,
-                    const asc_hip_spmv_options* opt) {
-  const uint8_t alg = opt ? opt->algorithm : ASC_HIP_ALG_AUTO;
-  if (alg != ASC_HIP_ALG_AUTO) return alg;
+                    const asc_hip_spmv_options* opt, asc_hip::PanelPlan* panel,
+                    bool* has_panel) {
+  const uint8_t req = opt ? opt->algorithm : ASC_HIP_ALG_AUTO;
+  *has_panel = false;
+  if (req != ASC_HIP_ALG_AUTO && req != ASC_HIP_ALG_PANEL) return req;
   std::lock_guard<std::mutex> lock(ctx->plan_mutex);
   for (const auto& p : ctx->plans) {
     if (p.offsets == a.outer_offsets && p.indices == a.inner_indices &&
         p.rows == a.rows && p.cols == a.cols && p.nnz == a.nnz &&
-        p.element_kind == a.element_kind)
-      return p.algorithm;
+        p.element_kind == a.element_kind) {
+      if (p.algorithm == ASC_HIP_ALG_PANEL) {
+        *panel = p.panel;
+        *has_panel = true;
+      }
+      return req == ASC_HIP_ALG_AUTO ? p.algorithm : req;
+    }
   }
-  return ASC_HIP_ALG_MERGE;
+  return req == ASC_HIP_ALG_AUTO ? ASC_HIP_ALG_MERGE : req;
+}
+
+asc_hip_status missing_panel_analysis() {
+  return fail(ASC_HIP_INVALID_ARGUMENT,
+              "ASC_HIP_ALG_PANEL needs a column-panel analysis of this matrix "
+              "(asc_sparse_hip_csr_spmv_analyze)");
+}
+
+// Default panel: 2 MiB of x, half an XCD's L2 (the other half keeps the
+// streamed matrix lines).  Rows this long or longer take the panel kernels.
+constexpr int64_t kDefaultPanelBytes = int64_t(2) << 20;
+constexpr int64_t kDefaultLongRowMin = 128;
+constexpr int64_t kMaxPanels = 1024;
+
+struct PanelShape {
+  int64_t panel_cols = 1;
+  int64_t panels = 8;
+  int64_t long_min = kDefaultLongRowMin;
+  size_t list_bytes = 0, split_bytes = 0, scratch_bytes = 0, bytes = 0;
+};
+
+asc_hip_status panel_shape(asc_hip_context* ctx, const asc_hip_sparse_desc* a,
+                           const asc_hip_panel_options* opt, PanelShape* out) {
+  asc_hip_status st = check_context(ctx);
+  if (st.code) return st;
+  if ((st = check_device_sparse(ctx, a, "matrix")).code) return st;
+  if (a->format != ASC_HIP_CSR)
+    return fail(ASC_HIP_INVALID_ARGUMENT, "column-panel analysis requires CSR format");
+  if (a->element_kind == ASC_HIP_C64 || a->element_kind == ASC_HIP_C128)
+    return fail(ASC_HIP_UNSUPPORTED, "compat SpMV is defined for real storage");
+  const int64_t want_cols = opt ? opt->panel_columns : 0;
+  const int64_t long_min = opt && opt->long_row_min ? opt->long_row_min : kDefaultLongRowMin;
+  if (want_cols < 0 || long_min < 0)
+    return fail(ASC_HIP_INVALID_ARGUMENT, "panel width and long-row length must be nonnegative");
+  if (a->rows > kU32Max || std::min(a->nnz, a->cols) > kU32Max)
+    return fail(ASC_HIP_UNSUPPORTED, "column-panel analysis stores 32-bit row ids and positions");
+  const int64_t xbytes = a->element_kind == ASC_HIP_F64 ? 8 : 4;
+  const int64_t w = want_cols > 0 ? want_cols : kDefaultPanelBytes / xbytes;
+  const int64_t p0 = a->cols > w ? (a->cols - 1) / w + 1 : 1;
+  const int64_t panels = (p0 + 7) / 8 * 8;
+  if (panels > kMaxPanels)
+    return fail(ASC_HIP_INVALID_ARGUMENT, "panel width gives more than 1024 panels");
+  int64_t split_entries = 0;
+  if (!checked_mul(panels - 1, a->rows, &split_entries) ||
+      split_entries > std::numeric_limits<int64_t>::max() / 8)
+    return fail(ASC_HIP_OVERFLOW, "panel split table size overflows");
+  out->panels = panels;
+  out->panel_cols = a->cols > 0 ? (a->cols - 1) / panels + 1 : 1;
+  out->long_min = long_min;
+  out->list_bytes = align256(sizeof(uint32_t) * static_cast<size_t>(a->rows));
+  out->split_bytes = align256(sizeof(uint32_t) * static_cast<size_t>(split_entries));
+  out->scratch_bytes = asc_hip::panel_analysis_scratch_bytes(a->rows);
+  out->bytes = out->list_bytes + out->split_bytes + out->scratch_bytes;
+  return ok();
+}
+
+// The row list and split table must survive the SpMV's own writes.
+bool panel_overlaps(const asc_hip::PanelPlan& pp, Span out) {
+  if (pp.nlong == 0) return false;
+  const size_t n = static_cast<size_t>(pp.nlong);
+  return overlaps(span_of(pp.long_rows, 4 * n), out) ||
+         (pp.panels > 1 && overlaps(span_of(pp.splits, 4 * n * static_cast<size_t>(pp.panels - 1)), out));
 }
 
 asc_hip_status check_options(const asc_hip_sparse_desc& a, const asc_hip_spmv_options* opt,
@@ -110,9 +183,15 @@ asc_hip_status prepare_compat_spmv(asc_hip_context* ctx, const asc_hip_sparse_de
   const Span ys = vector_span(*y);
   if (overlaps(vector_span(*x), ys) || sparse_overlaps(*a, ys))
     return fail(ASC_HIP_INVALID_ARGUMENT, "SpMV rejects output operand overlap");
-  const uint8_t alg = resolve_alg(ctx, *a, opt);
+  asc_hip::PanelPlan panel;
+  bool has_panel = false;
+  const uint8_t alg = resolve_alg(ctx, *a, opt, &panel, &has_panel);
   if ((st = check_options(*a, opt, alg)).code) return st;
+  if (alg == ASC_HIP_ALG_PANEL && !has_panel) return missing_panel_analysis();
+  if (alg == ASC_HIP_ALG_PANEL && panel_overlaps(panel, ys))
+    return fail(ASC_HIP_INVALID_ARGUMENT, "SpMV output overlaps the column-panel analysis");
   asc_hip::SpmvArgs& g = plan->args;
+  g.panel = panel;
   g.a = as_compressed(*a, false);
   g.x = x->data;
   g.incx = x->stride;
@@ -199,10 +278,22 @@ asc_hip_status prepare_standard_spmv(asc_hip_context* ctx, uint8_t transpose,
   const Span ys = vector_span(*y);
   if (overlaps(vector_span(*x), ys) || sparse_overlaps(*a, ys))
     return fail(ASC_HIP_INVALID_ARGUMENT, "Sparse matrix-vector input and output overlap");
-  const uint8_t alg = resolve_alg(ctx, *a, opt);
+  asc_hip::PanelPlan panel;
+  bool has_panel = false;
+  uint8_t alg = resolve_alg(ctx, *a, opt, &panel, &has_panel);
   if ((st = check_options(*a, opt, alg)).code) return st;
+  if (alg == ASC_HIP_ALG_PANEL) {
+    if (!has_panel) return missing_panel_analysis();
+    // The analysis describes the stored CSR rows: other row views run
+    // merge-path.
+    if (a->format != ASC_HIP_CSR || transpose != ASC_HIP_OP_N)
+      alg = ASC_HIP_ALG_MERGE;
+    else if (panel_overlaps(panel, ys))
+      return fail(ASC_HIP_INVALID_ARGUMENT, "SpMV output overlaps the column-panel analysis");
+  }
   if ((st = plan_row_view(*a, transpose, plan)).code) return st;
   asc_hip::SpmvArgs& g = plan->args;
+  g.panel = panel;
   g.x = x->data;
   g.incx = x->stride;
   g.y = const_cast<void*>(y->data);
@@ -453,6 +544,88 @@ asc_hip_status asc_sparse_hip_csr_spmv(asc_hip_context* ctx, double alpha,
   plan.args.alpha = alpha;
   plan.args.beta = beta;
   return run_spmv(ctx, plan, ws);
+}
+
+// ---- column-panel analysis -----------------------------------------------
+
+asc_hip_status asc_sparse_hip_csr_spmv_analysis_size(asc_hip_context* ctx,
+                                                     const asc_hip_sparse_desc* a,
+                                                     const asc_hip_panel_options* opt,
+                                                     size_t* out) {
+  if (out == nullptr) return fail(ASC_HIP_INVALID_ARGUMENT, "output pointer is null");
+  PanelShape shape;
+  asc_hip_status st = panel_shape(ctx, a, opt, &shape);
+  if (st.code) return st;
+  *out = shape.bytes;
+  return ok();
+}
+
+asc_hip_status asc_sparse_hip_csr_spmv_analyze(asc_hip_context* ctx,
+                                               const asc_hip_sparse_desc* a,
+                                               const asc_hip_panel_options* opt,
+                                               void* analysis, size_t analysis_bytes) {
+  PanelShape shape;
+  asc_hip_status st = panel_shape(ctx, a, opt, &shape);
+  if (st.code) return st;
+  uint8_t short_alg = opt ? opt->short_algorithm : ASC_HIP_ALG_AUTO;
+  uint8_t long_alg = opt ? opt->long_algorithm : ASC_HIP_ALG_AUTO;
+  for (uint8_t alg : {short_alg, long_alg})
+    if (alg != ASC_HIP_ALG_AUTO && (alg < ASC_HIP_ALG_VECTOR1 || alg > ASC_HIP_ALG_VECTOR64))
+      return fail(ASC_HIP_INVALID_ARGUMENT, "panel plans run ASC_HIP_ALG_VECTOR1..VECTOR64");
+  const SparseSpans sp = sparse_spans(*a);
+  if ((st = check_workspace(ctx, analysis, analysis_bytes, shape.bytes,
+                            {sp.first, sp.second, sp.values})).code)
+    return st;
+  if ((st = unavailable_if_validation_only(ctx)).code) return st;
+  char* w = static_cast<char*>(analysis);
+  asc_hip::PanelPlan pp;
+  pp.long_rows = reinterpret_cast<const uint32_t*>(w);
+  pp.splits = reinterpret_cast<const uint32_t*>(w + shape.list_bytes);
+  pp.panels = shape.panels;
+  pp.panel_cols = shape.panel_cols;
+  pp.long_min = shape.long_min;
+  {
+    DeviceGuard guard(ctx->device);
+    if (guard.error != hipSuccess) return fail_hip(guard.error, "hipSetDevice");
+    int64_t nlong = 0;
+    const hipError_t e = asc_hip::panel_analysis_launch(
+        as_compressed(*a, false), shape.panel_cols, shape.panels, shape.long_min,
+        reinterpret_cast<uint32_t*>(w), reinterpret_cast<uint32_t*>(w + shape.list_bytes),
+        w + shape.list_bytes + shape.split_bytes, &nlong, ctx->stream);
+    if ((st = finish_launch(ctx, e, "panel analysis")).code) return st;
+    pp.nlong = nlong;
+  }
+  // AUTO: for the short rows, the power of two at or below long_min / 8
+  // lanes; for the listed rows, the power of two at or below their mean
+  // entries per panel (every entry counted as a long row's: most are, in the
+  // skewed matrices this plan is for).  Both between 1 and 64.
+  if (short_alg == ASC_HIP_ALG_AUTO) {
+    short_alg = ASC_HIP_ALG_VECTOR1;
+    for (int64_t g = 2; g <= shape.long_min / 8 && short_alg < ASC_HIP_ALG_VECTOR64; g *= 2)
+      ++short_alg;
+  }
+  if (long_alg == ASC_HIP_ALG_AUTO) {
+    const double mean = pp.nlong > 0 ? static_cast<double>(a->nnz) /
+                                           static_cast<double>(pp.nlong) /
+                                           static_cast<double>(pp.panels)
+                                     : 1.0;
+    long_alg = ASC_HIP_ALG_VECTOR1;
+    for (double g = 2.0; g <= mean && long_alg < ASC_HIP_ALG_VECTOR64; g *= 2.0) ++long_alg;
+  }
+  pp.short_alg = static_cast<asc_hip::SpmvAlg>(short_alg);
+  pp.long_alg = static_cast<asc_hip::SpmvAlg>(long_alg);
+  std::lock_guard<std::mutex> lock(ctx->plan_mutex);
+  for (auto& p : ctx->plans) {
+    if (p.offsets == a->outer_offsets && p.indices == a->inner_indices && p.rows == a->rows &&
+        p.cols == a->cols && p.nnz == a->nnz && p.element_kind == a->element_kind) {
+      p.algorithm = ASC_HIP_ALG_PANEL;
+      p.panel = pp;
+      return ok();
+    }
+  }
+  ctx->plans.push_back({a->outer_offsets, a->inner_indices, a->rows, a->cols, a->nnz,
+                        a->element_kind, ASC_HIP_ALG_PANEL, pp});
+  return ok();
 }
 
 asc_hip_status asc_sparse_hip_standard_spmv_workspace_size(

@@ -53,16 +53,20 @@ asc_hip_status asc_sparse_hip_plan_set(asc_hip_context* ctx, const asc_hip_spars
   if (a == nullptr) return fail(ASC_HIP_INVALID_ARGUMENT, "matrix descriptor is null");
   if (algorithm < ASC_HIP_ALG_ORDERED || algorithm >= ASC_HIP_ALG_COUNT)
     return fail(ASC_HIP_INVALID_ARGUMENT, "unknown SpMV algorithm");
+  if (algorithm == ASC_HIP_ALG_PANEL)
+    return fail(ASC_HIP_INVALID_ARGUMENT,
+                "ASC_HIP_ALG_PANEL plans are made by asc_sparse_hip_csr_spmv_analyze");
   std::lock_guard<std::mutex> lock(ctx->plan_mutex);
   for (auto& p : ctx->plans) {
     if (p.offsets == a->outer_offsets && p.indices == a->inner_indices && p.rows == a->rows &&
         p.cols == a->cols && p.nnz == a->nnz && p.element_kind == a->element_kind) {
       p.algorithm = algorithm;
+      p.panel = asc_hip::PanelPlan{};
       return ok();
     }
   }
   ctx->plans.push_back({a->outer_offsets, a->inner_indices, a->rows, a->cols, a->nnz,
-                        a->element_kind, algorithm});
+                        a->element_kind, algorithm, asc_hip::PanelPlan{}});
   return ok();
 }
 
@@ -110,7 +114,7 @@ asc_hip_status asc_sparse_hip_csr_spmv_autotune(asc_hip_context* ctx,
   if (err != hipSuccess) return fail_hip(err, "hipEventCreate");
   float best = -1.0f;
   uint8_t best_alg = ASC_HIP_ALG_MERGE;
-  for (uint8_t alg = ASC_HIP_ALG_MERGE; alg < ASC_HIP_ALG_COUNT && st.code == 0; ++alg) {
+  for (uint8_t alg = ASC_HIP_ALG_MERGE; alg <= ASC_HIP_ALG_VECTOR64 && st.code == 0; ++alg) {
     opt.algorithm = alg;
     st = asc_sparse_hip_csr_spmv(ctx, 1.0, a, x, 0.0, y, ws, ws_bytes, &opt);  // warm-up
     if (st.code) break;
@@ -248,7 +252,7 @@ asc_hip_status asc_sparse_hip_select_workspace_size(asc_hip_context* ctx,
   asc_hip_status st = check_context(ctx);
   if (st.code) return st;
   if (sel == nullptr || out == nullptr) return fail(ASC_HIP_INVALID_ARGUMENT, "null argument");
-  if (sel->outputs < 1 || sel->outputs > ASC_HIP_ALG_COUNT - ASC_HIP_ALG_MERGE)
+  if (sel->outputs < 1 || sel->outputs > ASC_HIP_ALG_VECTOR64 - ASC_HIP_ALG_MERGE + 1)
     return fail(ASC_HIP_INVALID_ARGUMENT, "selector outputs must cover merge..vector64");
   *out = select_ws(sel->outputs);
   return ok();

@@ -492,6 +492,13 @@ hipError_t csr_to_coo_by_index(const Compressed& a, int64_t* coords, void* out_v
 
 }  // namespace
 
+size_t scan_u32_scratch_bytes(int64_t n) { return scan_workspace_bytes(n); }
+
+hipError_t scan_u32_exclusive(const uint32_t* in, uint32_t* out, int64_t n, void* scratch,
+                              hipStream_t s) {
+  return scan_exclusive(in, out, n, static_cast<uint32_t*>(scratch), s);
+}
+
 hipError_t csr_to_coo_launch(const Compressed& a, int64_t* coords, void* out_values,
                              hipStream_t s) {
   if (a.outer == 0 || a.nnz == 0) return hipSuccess;

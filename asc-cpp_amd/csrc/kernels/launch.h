@@ -26,12 +26,30 @@ struct Compressed {
 
 enum class SpmvAlg : uint8_t {
   Ordered = 1, Merge = 2,
-  Vec1 = 3, Vec2 = 4, Vec4 = 5, Vec8 = 6, Vec16 = 7, Vec32 = 8, Vec64 = 9
+  Vec1 = 3, Vec2 = 4, Vec4 = 5, Vec8 = 6, Vec16 = 7, Vec32 = 8, Vec64 = 9,
+  Panel = 10  // column-panel plan for long rows (see PanelPlan)
 };
 
 enum class SpmvForm : uint8_t {
   Compat = 0,    // y = alpha*sum + beta*y (y unread when beta == 0)
   Standard = 1,  // y += alpha*v*x per entry (ordered) / y += alpha*sum (plain)
+};
+
+// Column-panel plan (built by panel_analysis_launch): rows with at least
+// `long_min` entries are listed in `long_rows` (ascending); the columns are cut
+// into `panels` ranges of `panel_cols` (a multiple of 8 ranges, one per XCD at
+// a time); splits[(q-1)*nlong + i] (q = 1..panels-1) is the position, relative
+// to the start of listed row i, of its first entry in panel q.  Shorter rows
+// run through vector<short_alg> in one pass.
+struct PanelPlan {
+  const uint32_t* long_rows = nullptr;
+  const uint32_t* splits = nullptr;
+  int64_t nlong = 0;
+  int64_t panels = 8;
+  int64_t panel_cols = 0;
+  int64_t long_min = INT64_MAX;
+  SpmvAlg short_alg = SpmvAlg::Vec4;  // Vec1..Vec64
+  SpmvAlg long_alg = SpmvAlg::Vec32;  // Vec1..Vec64
 };
 
 struct SpmvArgs {
@@ -45,12 +63,29 @@ struct SpmvArgs {
   SpmvForm form;
   SpmvAlg alg;
   bool twosum;  // reduced-precision compensated accumulation
+  PanelPlan panel;  // used when alg == Panel
 };
 
 // Workspace bytes the algorithm needs for this operand (0 for most).
 size_t spmv_workspace_bytes(const SpmvArgs& args);
 hipError_t spmv_launch(const SpmvArgs& args, void* workspace,
                        hipStream_t stream);
+
+// Builds a PanelPlan's row list and split table for a canonical CSR operand
+// (sorted columns per row): `long_rows` holds `outer` and `splits`
+// (panels-1)*outer uint32 entries; `scratch` holds
+// panel_analysis_scratch_bytes(outer).  Waits for the stream to read back
+// the number of listed rows.
+size_t panel_analysis_scratch_bytes(int64_t outer);
+hipError_t panel_analysis_launch(const Compressed& a, int64_t panel_cols, int64_t panels,
+                                 int64_t long_min, uint32_t* long_rows, uint32_t* splits,
+                                 void* scratch, int64_t* out_nlong, hipStream_t stream);
+
+// Exclusive prefix sum of n uint32 values; `scratch` holds
+// scan_u32_scratch_bytes(n).
+size_t scan_u32_scratch_bytes(int64_t n);
+hipError_t scan_u32_exclusive(const uint32_t* in, uint32_t* out, int64_t n, void* scratch,
+                              hipStream_t stream);
 
 struct SpmmArgs {
   Compressed a;

@@ -11,6 +11,9 @@
 //   ordered    one lane per row, strict left-to-right unfused chain: the
 //              serial reference's arithmetic bit for bit
 //              (src/sparse/reference_blas.cc:13-39, blas.h:1653-1665).
+//   panel      column-panel plan: one vector<G> pass per column range of x,
+//              from a split table built once per matrix, so every pass
+//              gathers from an x panel resident in L2.
 //
 // Value storage V in {double, float, bf16, f16}; accumulation in double for
 // double and in float otherwise (plain, or compensated with TwoSum plus the
@@ -22,6 +25,9 @@ namespace asc_hip {
 namespace {
 
 constexpr int kBlock = 256;
+// Row blocks per panel in one panel-kernel launch (the grid stays well below
+// the 2^32 work-items a dispatch can address; blocks stride beyond it).
+constexpr int64_t kPanelRowBlockCap = int64_t(1) << 20;
 
 // ---- accumulators ---------------------------------------------------------
 template <typename A>
@@ -51,6 +57,11 @@ struct PlainAcc {
     return static_cast<double>(s);
   }
   __device__ __forceinline__ A total() const { return s; }
+  __device__ __forceinline__ static PlainAcc from_f64(double v) {
+    PlainAcc r;
+    r.s = static_cast<A>(v);
+    return r;
+  }
 };
 
 // Compensated FP32 accumulation: s + c carries the running sum; every
@@ -115,6 +126,13 @@ struct TwoSumAcc {
     return static_cast<double>(s) + static_cast<double>(c);
   }
   __device__ __forceinline__ float total() const { return s + c; }
+  // (s, c) with s + c == v to within FP32's rounding of the tail.
+  __device__ __forceinline__ static TwoSumAcc from_f64(double v) {
+    TwoSumAcc r;
+    r.s = static_cast<float>(v);
+    r.c = static_cast<float>(v - static_cast<double>(r.s));
+    return r;
+  }
 };
 
 template <typename V, bool TWOSUM>
@@ -192,23 +210,17 @@ __device__ __forceinline__ void add_carry(const KParams& p, int64_t row,
 // ---------------------------------------------------------------------------
 // vector<G>: G lanes per row.
 // ---------------------------------------------------------------------------
-template <int G, typename V, typename I, bool TWOSUM>
-__global__ __launch_bounds__(kBlock) void spmv_vector_kernel(KParams p) {
+// The G-lane sum of entries [b, e), reduced to every lane of the group.  Two
+// independent chains keep two gathers in flight per lane.
+template <int G, typename V, typename I, typename Acc>
+__device__ __forceinline__ Acc group_sum(const KParams& p, int64_t b, int64_t e, int lane) {
   using A = typename acc_of<V>::type;
-  using Acc = typename AccSel<V, TWOSUM>::type;
-  const int64_t blk = xcd_remap(blockIdx.x, gridDim.x);
-  const int64_t row = blk * (kBlock / G) + threadIdx.x / G;
-  const int lane = threadIdx.x % G;
-  if (row >= p.rows) return;  // whole G-groups leave together
   const V* __restrict__ val = static_cast<const V*>(p.values);
   const I* __restrict__ col = static_cast<const I*>(p.indices);
   const A* __restrict__ x = static_cast<const A*>(p.x);
-  const int64_t b = ld_stream(p.offsets + row);
-  const int64_t e = ld_stream(p.offsets + row + 1);
   Acc acc = Acc::zero();
   Acc acc2 = Acc::zero();
   int64_t k = b + lane;
-  // Two independent chains keep two gathers in flight per lane.
   for (; k + G < e; k += 2 * G) {
     const int64_t c0 = load_index(col, k);
     const int64_t c1 = load_index(col, k + G);
@@ -224,7 +236,117 @@ __global__ __launch_bounds__(kBlock) void spmv_vector_kernel(KParams p) {
   acc.merge_after(acc2);
 #pragma unroll
   for (int off = G / 2; off > 0; off >>= 1) acc.merge_after(acc.shfl_xor(off, G));
+  return acc;
+}
+
+// Rows of at least `skip_from` entries are left to the panel kernels
+// (INT64_MAX: none are).
+template <int G, typename V, typename I, bool TWOSUM>
+__global__ __launch_bounds__(kBlock) void spmv_vector_kernel(KParams p, int64_t skip_from) {
+  using Acc = typename AccSel<V, TWOSUM>::type;
+  const int64_t blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t row = blk * (kBlock / G) + threadIdx.x / G;
+  const int lane = threadIdx.x % G;
+  if (row >= p.rows) return;  // whole G-groups leave together
+  const int64_t b = ld_stream(p.offsets + row);
+  const int64_t e = ld_stream(p.offsets + row + 1);
+  if (e - b >= skip_from) return;
+  const Acc acc = group_sum<G, V, I, Acc>(p, b, e, lane);
   if (lane == 0) finish_row<V>(p, row, acc);
+}
+
+// ---------------------------------------------------------------------------
+// Column-panel plan for long rows.  Random columns of a large x miss in L2 on
+// nearly every gather.  The columns are cut into `panels` ranges (a multiple
+// of 8) and the rows with at least `long_min` entries are listed; block b
+// works on panel 8*group + b % 8 for one block of listed rows.  Work-groups
+// are dispatched to the 8 XCDs round robin, so each XCD gathers from one
+// panel of x at a time, which stays resident in its L2, and the 8 XCDs read
+// adjacent slices of the same rows at about the same time.  Each (panel, row)
+// partial sum goes to an FP64 buffer; the combine kernel adds them in panel
+// order and applies the row epilogue.  Shorter rows (a small share of the
+// entries of a skewed matrix) run through vector<G> in one pass.
+// ---------------------------------------------------------------------------
+template <int G, typename V, typename I, bool TWOSUM>
+__global__ __launch_bounds__(kBlock) void spmv_panel_kernel(
+    KParams p, const uint32_t* __restrict__ long_rows, const uint32_t* __restrict__ splits,
+    int64_t nlong, int64_t panels, int64_t row_blocks, int64_t rb_grid,
+    double* __restrict__ partial) {
+  using Acc = typename AccSel<V, TWOSUM>::type;
+  const int64_t slot = blockIdx.x % 8;
+  const int64_t rest = blockIdx.x / 8;
+  const int64_t q = (rest / rb_grid) * 8 + slot;
+  if (q >= panels) return;
+  const int lane = threadIdx.x % G;
+  // rb_grid blocks per panel stride over the row blocks (launch-size cap).
+  for (int64_t rbk = rest % rb_grid; rbk < row_blocks; rbk += rb_grid) {
+    const int64_t i = rbk * (kBlock / G) + threadIdx.x / G;
+    if (i < nlong) {  // uniform over the G-group
+      const int64_t row = long_rows[i];
+      const int64_t rb = p.offsets[row];
+      const int64_t b = q == 0 ? rb : rb + ld_stream(splits + (q - 1) * nlong + i);
+      const int64_t e = q + 1 == panels ? p.offsets[row + 1]
+                                        : rb + ld_stream(splits + q * nlong + i);
+      const Acc acc = group_sum<G, V, I, Acc>(p, b, e, lane);
+      if (lane == 0) partial[q * nlong + i] = acc.total_f64();
+    }
+  }
+}
+
+template <typename V, bool TWOSUM>
+__global__ __launch_bounds__(kBlock) void spmv_panel_combine_kernel(
+    KParams p, const uint32_t* __restrict__ long_rows, int64_t nlong, int64_t panels,
+    const double* __restrict__ partial) {
+  using Acc = typename AccSel<V, TWOSUM>::type;
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= nlong) return;
+  double s = 0.0;
+  for (int64_t q = 0; q < panels; ++q) s += partial[q * nlong + i];
+  finish_row<V>(p, long_rows[i], Acc::from_f64(s));
+}
+
+// ---- column-panel analysis ------------------------------------------------
+__global__ __launch_bounds__(kBlock) void panel_flag_kernel(
+    const int64_t* __restrict__ offsets, int64_t rows, int64_t long_min,
+    uint32_t* __restrict__ flags) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (r < rows) flags[r] = offsets[r + 1] - offsets[r] >= long_min ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(kBlock) void panel_list_kernel(
+    const uint32_t* __restrict__ flags, const uint32_t* __restrict__ pos, int64_t rows,
+    uint32_t* __restrict__ long_rows, uint32_t* __restrict__ count) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (r >= rows) return;
+  if (flags[r]) long_rows[pos[r]] = static_cast<uint32_t>(r);
+  if (r == rows - 1) *count = pos[r] + flags[r];
+}
+
+// Split table: one thread per listed row, an advancing binary search per
+// panel boundary over the row's sorted columns.
+template <typename I>
+__global__ __launch_bounds__(kBlock) void panel_split_kernel(
+    const int64_t* __restrict__ offsets, const I* __restrict__ col,
+    const uint32_t* __restrict__ long_rows, int64_t nlong, int64_t panel_cols,
+    int64_t panels, uint32_t* __restrict__ splits) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= nlong) return;
+  const int64_t row = long_rows[i];
+  const int64_t b = offsets[row];
+  const int64_t e = offsets[row + 1];
+  int64_t lo = b;
+  for (int64_t q = 1; q < panels; ++q) {
+    const int64_t bound = q * panel_cols;
+    int64_t hi = e;
+    while (lo < hi) {
+      const int64_t mid = lo + ((hi - lo) >> 1);
+      if (static_cast<int64_t>(col[mid]) < bound)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    splits[(q - 1) * nlong + i] = static_cast<uint32_t>(lo - b);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -630,7 +752,7 @@ hipError_t launch_typed(const SpmvArgs& a, const KParams& kp, void* ws,
 #define ASC_VEC_CASE(ALG, G)                                                  \
   case SpmvAlg::ALG:                                                          \
     spmv_vector_kernel<G, V, I, TWOSUM>                                       \
-        <<<blocks_for(rows, kBlock / G), kBlock, 0, s>>>(kp);                 \
+        <<<blocks_for(rows, kBlock / G), kBlock, 0, s>>>(kp, INT64_MAX);      \
     return hipGetLastError();
     ASC_VEC_CASE(Vec1, 1)
     ASC_VEC_CASE(Vec2, 2)
@@ -640,6 +762,62 @@ hipError_t launch_typed(const SpmvArgs& a, const KParams& kp, void* ws,
     ASC_VEC_CASE(Vec32, 32)
     ASC_VEC_CASE(Vec64, 64)
 #undef ASC_VEC_CASE
+    case SpmvAlg::Panel: {
+      const PanelPlan& pp = a.panel;
+      const int64_t skip = pp.nlong > 0 ? pp.long_min : INT64_MAX;
+      hipError_t e = hipErrorInvalidValue;
+      switch (pp.short_alg) {
+#define ASC_SHORT_CASE(ALG, G)                                                \
+  case SpmvAlg::ALG:                                                          \
+    spmv_vector_kernel<G, V, I, TWOSUM>                                       \
+        <<<blocks_for(rows, kBlock / G), kBlock, 0, s>>>(kp, skip);           \
+    e = hipGetLastError();                                                    \
+    break;
+        ASC_SHORT_CASE(Vec1, 1)
+        ASC_SHORT_CASE(Vec2, 2)
+        ASC_SHORT_CASE(Vec4, 4)
+        ASC_SHORT_CASE(Vec8, 8)
+        ASC_SHORT_CASE(Vec16, 16)
+        ASC_SHORT_CASE(Vec32, 32)
+        ASC_SHORT_CASE(Vec64, 64)
+#undef ASC_SHORT_CASE
+        default:
+          break;
+      }
+      if (e != hipSuccess || pp.nlong == 0) return e;
+      double* partial = static_cast<double*>(ws);
+      const int64_t groups = (pp.panels + 7) / 8;
+      switch (pp.long_alg) {
+#define ASC_LONG_CASE(ALG, G)                                                 \
+  case SpmvAlg::ALG: {                                                        \
+    const int64_t row_blocks = (pp.nlong + kBlock / G - 1) / (kBlock / G);    \
+    const int64_t rb_grid = row_blocks < kPanelRowBlockCap / groups           \
+                                ? row_blocks                                  \
+                                : kPanelRowBlockCap / groups;                 \
+    spmv_panel_kernel<G, V, I, TWOSUM>                                        \
+        <<<static_cast<unsigned>(8 * groups * rb_grid), kBlock, 0, s>>>(      \
+            kp, pp.long_rows, pp.splits, pp.nlong, pp.panels, row_blocks,     \
+            rb_grid, partial);                                                \
+    e = hipGetLastError();                                                    \
+    break;                                                                    \
+  }
+        ASC_LONG_CASE(Vec1, 1)
+        ASC_LONG_CASE(Vec2, 2)
+        ASC_LONG_CASE(Vec4, 4)
+        ASC_LONG_CASE(Vec8, 8)
+        ASC_LONG_CASE(Vec16, 16)
+        ASC_LONG_CASE(Vec32, 32)
+        ASC_LONG_CASE(Vec64, 64)
+#undef ASC_LONG_CASE
+        default:
+          e = hipErrorInvalidValue;
+          break;
+      }
+      if (e != hipSuccess) return e;
+      spmv_panel_combine_kernel<V, TWOSUM><<<blocks_for(pp.nlong, kBlock), kBlock, 0, s>>>(
+          kp, pp.long_rows, pp.nlong, pp.panels, partial);
+      return hipGetLastError();
+    }
   }
   return hipErrorInvalidValue;
 }
@@ -653,6 +831,9 @@ hipError_t launch_by_index(const SpmvArgs& a, const KParams& kp, void* ws,
 
 template <typename V>
 size_t ws_typed(const SpmvArgs& a) {
+  if (a.alg == SpmvAlg::Panel)  // FP64 partial sum per (panel, long row)
+    return align256(sizeof(double) * static_cast<size_t>(a.panel.panels) *
+                    static_cast<size_t>(a.panel.nlong));
   if (a.alg != SpmvAlg::Merge) return 0;
   const int64_t nt = merge_tiles(a);
   const size_t cb = a.twosum ? carry_bytes<V, true>() : carry_bytes<V, false>();
@@ -678,6 +859,53 @@ size_t spmv_workspace_bytes(const SpmvArgs& a) {
     case Elem::F16: return ws_typed<f16_t>(a);
   }
   return 0;
+}
+
+size_t panel_analysis_scratch_bytes(int64_t outer) {
+  const size_t n = static_cast<size_t>(outer > 0 ? outer : 1);
+  return 2 * align256(sizeof(uint32_t) * n) + align256(sizeof(uint32_t)) +
+         align256(scan_u32_scratch_bytes(outer));
+}
+
+hipError_t panel_analysis_launch(const Compressed& a, int64_t panel_cols, int64_t panels,
+                                 int64_t long_min, uint32_t* long_rows, uint32_t* splits,
+                                 void* scratch, int64_t* out_nlong, hipStream_t s) {
+  *out_nlong = 0;
+  const int64_t rows = a.outer;
+  if (rows == 0) return hipSuccess;
+  char* w = static_cast<char*>(scratch);
+  uint32_t* flags = reinterpret_cast<uint32_t*>(w);
+  w += align256(sizeof(uint32_t) * rows);
+  uint32_t* pos = reinterpret_cast<uint32_t*>(w);
+  w += align256(sizeof(uint32_t) * rows);
+  uint32_t* count = reinterpret_cast<uint32_t*>(w);
+  w += align256(sizeof(uint32_t));
+  const unsigned blocks = static_cast<unsigned>((rows + kBlock - 1) / kBlock);
+  panel_flag_kernel<<<blocks, kBlock, 0, s>>>(a.offsets, rows, long_min, flags);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = scan_u32_exclusive(flags, pos, rows, w, s);
+  if (e == hipSuccess) {
+    panel_list_kernel<<<blocks, kBlock, 0, s>>>(flags, pos, rows, long_rows, count);
+    e = hipGetLastError();
+  }
+  uint32_t host_count = 0;
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(&host_count, count, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return e;
+  const int64_t nlong = host_count;
+  *out_nlong = nlong;
+  if (nlong == 0 || panels <= 1) return hipSuccess;
+  const unsigned lblocks = static_cast<unsigned>((nlong + kBlock - 1) / kBlock);
+  if (a.index_bytes == 4)
+    panel_split_kernel<int32_t><<<lblocks, kBlock, 0, s>>>(
+        a.offsets, static_cast<const int32_t*>(a.indices), long_rows, nlong, panel_cols,
+        panels, splits);
+  else
+    panel_split_kernel<int64_t><<<lblocks, kBlock, 0, s>>>(
+        a.offsets, static_cast<const int64_t*>(a.indices), long_rows, nlong, panel_cols,
+        panels, splits);
+  return hipGetLastError();
 }
 
 hipError_t spmv_launch(const SpmvArgs& a, void* ws, hipStream_t s) {

@@ -4,9 +4,10 @@ reduced-precision, learned-selection and batched forms.
 Headline workload (one MI355X): the single large matrix -- n = 2^22 rows and
 columns, row lengths Zipf(s=1.5) clipped to [1, 10^4] (about 6.4e8 nnz),
 sorted distinct uniform int32 columns, N(0,1) values, FP64, alpha=1, beta=0.
-One step = one compat SpMV y = alpha*A*x + beta*y over the whole matrix with
-the variant the learned selector picks, inputs resident in HBM.  `value` is
-algorithmic bytes per SpMV / time:
+One step = one compat SpMV y = alpha*A*x + beta*y over the whole matrix
+through the column-panel plan (its analysis runs once, before timing; the
+learned selector's direct-kernel pick is reported beside it), inputs resident
+in HBM.  `value` is algorithmic bytes per SpMV / time:
 
   B = nnz*(s_v + s_i) + (rows+1)*s_o + cols*s_x + rows*s_y*(1 + [beta != 0])
 
@@ -143,16 +144,33 @@ def bench_large(ctx, args, extra):
         ctx.copy(cb.ptr, ca.ptr, 1 << 30)
         ctx.synchronize()
         del ca, cb
-    if args.alg == "auto":
-        alg, scores = learned_pick(ctx, a)
-        extra["learned_select"] = {"algorithm": H.ALG_NAMES[alg], "scores": scores}
+    nbytes = spmv_bytes(m.rows, m.cols, m.nnz, 8, 4, 8, 8)
+    lens = np.diff(m.offsets)
+    if args.alg in ("auto", "panel"):
+        # The direct kernel the learned selector picks (no analysis step) ...
+        direct, scores = learned_pick(ctx, a)
+        rd = time_ops(ctx, lambda: H.csr_spmv(ctx, 1.0, a, xd, 0.0, yd, alg=direct, ws=ws), 5, 2)
+        extra["learned_select"] = {"algorithm": H.ALG_NAMES[direct], "scores": scores,
+                                   "gbs": round(nbytes / rd["median_ms"] / 1e6, 1),
+                                   "ms": round(rd["median_ms"], 3)}
+        log(f"learned pick {H.ALG_NAMES[direct]}: {rd['median_ms']:.3f} ms")
+        # ... and the column-panel plan with its default options, which the
+        # timed steps run (analysis once, outside the timed region).
+        t0 = time.perf_counter()
+        H.csr_spmv_analyze(ctx, a)
+        ctx.synchronize()
+        extra["panel_analysis"] = {"ms": round((time.perf_counter() - t0) * 1e3, 2),
+                                   "panel_columns": 1 << 18, "long_row_min": 128,
+                                   "long_rows": int(np.sum(lens >= 128)),
+                                   "long_row_nnz_share": round(float(lens[lens >= 128].sum()) / m.nnz, 4),
+                                   "analysis_bytes": int(a.analysis.nbytes)}
+        alg = H.ALG_PANEL
     else:
         alg = {"merge": H.ALG_MERGE, "ordered": H.ALG_ORDERED,
                **{f"vec{g}": v for g, v in H.ALG_VECTOR.items()}}[args.alg]
     name = H.ALG_NAMES[alg]
     res = time_ops(ctx, lambda: H.csr_spmv(ctx, 1.0, a, xd, 0.0, yd, alg=alg, ws=ws),
                    args.steps, args.warmup)
-    nbytes = spmv_bytes(m.rows, m.cols, m.nnz, 8, 4, 8, 8)
     log(f"FP64 {name}: {res['loop_ms']:.3f} ms/step, {nbytes / res['loop_ms'] / 1e6:.0f} GB/s")
     if args.sweep or not args.quick:
         sweep = {}
@@ -165,10 +183,13 @@ def bench_large(ctx, args, extra):
         a32 = H.convert_values(ctx, a, H.F32)
         xb32, yb32 = ctx.upload(x.astype(np.float32)), ctx.alloc(4 * m.rows)
         x32d, y32d = H.vector(xb32, m.cols, H.F32, writable=False), H.vector(yb32, m.rows, H.F32)
+        if alg == H.ALG_PANEL:
+            H.csr_spmv_analyze(ctx, a32)
         r32 = time_ops(ctx, lambda: H.csr_spmv(ctx, 1.0, a32, x32d, 0.0, y32d, alg=alg, ws=ws), 5, 2)
         b32 = spmv_bytes(m.rows, m.cols, m.nnz, 4, 4, 4, 4)
         extra["fp32_same_matrix"] = {"gbs": round(b32 / r32["median_ms"] / 1e6, 1),
                                      "ms": round(r32["median_ms"], 3), "algorithm": name}
+        log(f"FP32 {name}: {r32['median_ms']:.3f} ms")
         del a32, xb32, yb32
         # SpMM: C += A*B, B row-major 2^22 x 32 (U[-1,1]).
         k = 32
@@ -303,7 +324,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--rows", type=int, default=1 << 22)
-    ap.add_argument("--alg", default="auto", help="auto (learned selector), merge, ordered, vecN")
+    ap.add_argument("--alg", default="auto",
+                    help="auto (column-panel plan, learned pick reported), merge, ordered, vecN")
     ap.add_argument("--cpu-sample-rows", type=int, default=1 << 19)
     ap.add_argument("--batch-count", type=int, default=10_000)
     ap.add_argument("--no-cpu", action="store_true")
@@ -353,7 +375,8 @@ def main():
                                "clipped to [1,1e4], int32 columns, int64 offsets, N(0,1) FP64 "
                                "values, x U[-1,1], alpha=1, beta=0",
                    "rows": rows, "cols": rows, "nnz": nnz, "algorithm": name,
-                   "selection": "learned selector" if args.alg == "auto" else "forced",
+                   "selection": ("column-panel analysis, default options" if alg == H.ALG_PANEL
+                                 else "forced"),
                    "bytes_per_spmv": nbytes, "hbm_frac_of_peak": round(gbs / HBM_PEAK_GBS, 4)},
         "roofline": {"bound": "hbm", "achieved": round(kgbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(kgbs / HBM_PEAK_GBS, 4),

@@ -177,7 +177,8 @@ enum {
   ASC_HIP_ALG_VECTOR16 = 7,
   ASC_HIP_ALG_VECTOR32 = 8,
   ASC_HIP_ALG_VECTOR64 = 9, /* one wave64 per row */
-  ASC_HIP_ALG_COUNT = 10
+  ASC_HIP_ALG_PANEL = 10,   /* column panels; needs asc_sparse_hip_csr_spmv_analyze */
+  ASC_HIP_ALG_COUNT = 11
 };
 
 /* Accumulation for FP32/BF16/F16-stored values (FP64 always accumulates in
@@ -299,6 +300,38 @@ asc_hip_status asc_sparse_hip_csr_spmv(
     const asc_hip_vector_desc* x, double beta, const asc_hip_vector_desc* y,
     void* workspace, size_t workspace_bytes,
     const asc_hip_spmv_options* options);
+
+/*
+ * Column-panel analysis for a trusted device CSR matrix.  No reference
+ * counterpart: it is the analysis step of this library's ASC_HIP_ALG_PANEL
+ * SpMV, which reads the caller's CSR arrays in place.  Rows with at least
+ * `long_row_min` entries are listed; the columns are cut into a multiple of 8
+ * panels of about `panel_columns` columns, and the listed rows are summed one
+ * panel per XCD at a time, so that their x gathers hit an L2-resident panel
+ * instead of missing to memory.  Shorter rows run in one vector pass.  The
+ * analysis buffer (caller-owned device memory, its size from the query)
+ * receives the row list and the per-panel split positions; the call waits for
+ * its stream.  On success the context's plan cache maps the matrix to
+ * ASC_HIP_ALG_PANEL with this buffer: it must stay allocated and unmodified,
+ * and the matrix's structure unchanged, until asc_sparse_hip_plan_clear or
+ * another plan for the matrix replaces it.  Requires rows and min(nnz, cols)
+ * <= UINT32_MAX.  Panel partial sums are added in FP64, so this is a
+ * plain-mode (any association) algorithm.
+ */
+typedef struct asc_hip_panel_options {
+  int64_t panel_columns;   /* 0: 2 MiB of x per panel (2^18 FP64 / 2^19 FP32 columns) */
+  int64_t long_row_min;    /* 0: 128 */
+  uint8_t short_algorithm; /* ASC_HIP_ALG_VECTOR1..VECTOR64, or AUTO */
+  uint8_t long_algorithm;  /* ASC_HIP_ALG_VECTOR1..VECTOR64, or AUTO */
+  uint8_t reserved[6];
+} asc_hip_panel_options;
+
+asc_hip_status asc_sparse_hip_csr_spmv_analysis_size(
+    asc_hip_context* context, const asc_hip_sparse_desc* matrix,
+    const asc_hip_panel_options* options, size_t* out_bytes);
+asc_hip_status asc_sparse_hip_csr_spmv_analyze(
+    asc_hip_context* context, const asc_hip_sparse_desc* matrix,
+    const asc_hip_panel_options* options, void* analysis, size_t analysis_bytes);
 
 /*
  * Standardized SpMV  y += alpha*op(A)*x  for CSR or CSC, real F32/F64
@@ -470,7 +503,7 @@ asc_hip_status asc_sparse_hip_plan_clear(asc_hip_context* context);
 
 /* Exhaustive autotuner: times every tunable variant (ASC_HIP_ALG_MERGE ..
  * ASC_HIP_ALG_VECTOR64) as y = A*x with `reps` launches after a warm-up,
- * writes per-algorithm milliseconds into times_ms[ASC_HIP_ALG_COUNT]
+ * writes per-algorithm milliseconds into times_ms[ASC_HIP_ALG_VECTOR64 + 1]
  * (indexed by algorithm id; may be NULL), stores the fastest in the plan
  * cache and returns it.  y is overwritten; the workspace must hold what
  * the merge variant needs. */

@@ -42,6 +42,8 @@ def lib():
 
 
 def _p(a):
+    if a is not None and a.size and not a.flags.c_contiguous:
+        raise ValueError("oracle arrays must be contiguous (pass strides as incx/incy)")
     return a.ctypes.data if a is not None and a.size else None
 
 

@@ -203,3 +203,60 @@ def test_fault_injection_arms_and_disarms(vctx):
         vctx.alloc(64, H.HOST)
     b1.free()
     H.test_inject(H.INJECT_ALLOCATION, -1)
+
+
+def test_panel_analysis_validation(vctx):
+    lib, h = vctx.lib, vctx.handle
+    need = C.c_size_t()
+
+    def size(a, opts=None):
+        st = lib.asc_sparse_hip_csr_spmv_analysis_size(h, C.byref(a), C.byref(opts) if opts else None,
+                                                       C.byref(need))
+        return st.code
+
+    def aligned(n):
+        return (n + 255) // 256 * 256
+
+    def scratch(rows):  # flags + positions + count + scan partials (tiles of 4096)
+        return 2 * aligned(4 * rows) + 256 + aligned(4 * max(1, (rows + 4095) // 4096))
+
+    big = dev_csr(rows=1000, cols=10_000_000, nnz=5000)
+    # default width 2^18 FP64 columns -> 39 panels, rounded up to 40 (one per XCD at a time)
+    assert size(big) == H.OK
+    assert need.value == aligned(4 * 1000) + aligned(39 * 1000 * 4) + scratch(1000)
+    f32 = dev_csr(rows=1000, cols=10_000_000, nnz=5000, kind=H.F32)
+    assert size(f32) == H.OK  # 2^19 FP32 columns -> 20 panels -> 24
+    assert need.value == aligned(4 * 1000) + aligned(23 * 1000 * 4) + scratch(1000)
+    assert size(big, H.panel_options(4_000_000)) == H.OK  # 3 panels -> 8
+    assert need.value == aligned(4 * 1000) + aligned(7 * 1000 * 4) + scratch(1000)
+    assert size(dev_csr()) == H.OK  # 4 columns: still 8 (tiny) panels
+    assert size(big, H.panel_options(-1)) == H.INVALID_ARGUMENT
+    assert size(big, H.panel_options(0, -5)) == H.INVALID_ARGUMENT
+    assert size(big, H.panel_options(1)) == H.INVALID_ARGUMENT  # > 1024 panels
+    assert size(dev_csr(fmt=H.CSC)) == H.INVALID_ARGUMENT
+    assert size(dev_csr(trusted=0)) == H.INVALID_ARGUMENT
+    assert size(dev_csr(kind=H.C128)) == H.UNSUPPORTED
+    ws = 0x4000000
+    assert size(big) == H.OK
+    n = need.value
+
+    def analyze(opts, ptr, nbytes):
+        return lib.asc_sparse_hip_csr_spmv_analyze(h, C.byref(big), C.byref(opts) if opts else None,
+                                                   ptr, nbytes).code
+
+    # bad kernels, missing/short/overlapping buffer, then the enqueue point
+    assert analyze(H.panel_options(0, 0, H.ALG_MERGE), ws, n) == H.INVALID_ARGUMENT
+    assert analyze(H.panel_options(0, 0, H.ALG_AUTO, H.ALG_PANEL), ws, n) == H.INVALID_ARGUMENT
+    assert analyze(None, None, 0) == H.INVALID_ARGUMENT
+    assert analyze(None, ws, n - 4) == H.INVALID_ARGUMENT
+    assert analyze(None, FAKE, n) == H.INVALID_ARGUMENT
+    assert analyze(H.panel_options(0, 16, H.ALG_VECTOR[2], H.ALG_VECTOR[32]), ws, n) == H.UNAVAILABLE
+
+
+def test_panel_algorithm_requires_analysis(vctx):
+    a = dev_csr()
+    # no analysis registered: explicit ALG_PANEL is rejected before enqueue
+    assert spmv_code(vctx, a, dev_vec(4), dev_vec(3, base=0x200000), alg=H.ALG_PANEL) == H.INVALID_ARGUMENT
+    # plan_set cannot register a panel plan without its split table
+    assert vctx.lib.asc_sparse_hip_plan_set(vctx.handle, C.byref(a), H.ALG_PANEL).code == H.INVALID_ARGUMENT
+    assert vctx.lib.asc_sparse_hip_plan_set(vctx.handle, C.byref(a), 11).code == H.INVALID_ARGUMENT

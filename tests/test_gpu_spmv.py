@@ -183,3 +183,102 @@ def test_zero_rows_is_noop(ctx):
                            np.zeros(0), H.F64)
     out = run_compat(ctx, a, np.zeros(4), [], 1.0, 0.0, H.ALG_MERGE)
     assert out.size == 0
+
+
+# ---- column-panel plan ------------------------------------------------------------
+
+def _panel_cases():
+    # (matrix, [(panel columns, long-row minimum)])
+    return {
+        "zipf_2e16": (lambda: G.make(G.ZIPF, 1 << 16, 1 << 16, seed=23, values=G.VALUES_NORMAL),
+                      [(0, 0), (4099, 8), (1 << 13, 1), (300, 64)]),
+        "uniform16_4096": (lambda: G.uniform16(4096), [(0, 0), (300, 16), (1000, 17)]),
+        "banded_3e4": (lambda: G.make(G.BANDED, 30000, 30000, seed=6, values=G.VALUES_UNIFORM,
+                                      band=40, fill=0.5), [(2048, 4), (37, 30)]),
+    }
+
+
+PANEL_KERNELS = [(H.ALG_AUTO, H.ALG_AUTO), (H.ALG_VECTOR[1], H.ALG_VECTOR[64]),
+                 (H.ALG_VECTOR[8], H.ALG_VECTOR[1]), (H.ALG_VECTOR[64], H.ALG_VECTOR[8])]
+
+
+@pytest.mark.parametrize("name", list(_panel_cases()))
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_panel_plan_parity(ctx, name, dtype):
+    make, shapes = _panel_cases()[name]
+    m = make()
+    vals = m.values.astype(dtype)
+    x = G.vector(m.cols, 31).astype(dtype)
+    y0 = G.vector(m.rows, 32).astype(dtype)
+    y0[::97] = np.nan  # beta == 0 must not read y
+    lens = np.diff(m.offsets)
+    absax = O.abs_spmv(m.offsets, m.indices, vals.astype(np.float64), x.astype(np.float64))
+    a = H.clone_compressed(ctx, H.CSR, m.rows, m.cols, m.offsets, m.indices, vals, KIND[dtype],
+                           index_bytes=4)
+    for width, long_min in shapes:
+        for short_alg, long_alg in PANEL_KERNELS:
+            H.csr_spmv_analyze(ctx, a, H.panel_options(width, long_min, short_alg, long_alg))
+            assert H.plan_get(ctx, a) == H.ALG_PANEL
+            for alpha, beta in ((1.0, 0.0), (-1.5, 0.25)):
+                y_in = y0 if beta == 0.0 else np.nan_to_num(y0)
+                yref = y_in.copy()
+                O.compat_spmv(m.offsets, m.indices, vals, alpha, x, beta, yref)
+                print(f"panel {name} {dtype.__name__} width {width} long>={long_min} "
+                      f"kernels {short_alg},{long_alg} alpha {alpha} beta {beta}", flush=True)
+                for alg in (H.ALG_PANEL, H.ALG_AUTO):
+                    y = run_compat(ctx, a, x, y_in, alpha, beta, alg)
+                    bound_check(y, yref, lens, absax, alpha, beta, np.nan_to_num(y_in), EPS[dtype])
+    H.plan_clear(ctx)
+    with pytest.raises(H.AscHipError):
+        run_compat(ctx, a, x, y0, 1.0, 0.0, H.ALG_PANEL)
+
+
+def test_panel_plan_strides_and_empty_rows(ctx):
+    rng = np.random.default_rng(9)
+    rows, cols = 7000, 5000
+    lens = np.where(rng.random(rows) < 0.5, 0, rng.integers(1, 60, rows))
+    offsets = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    idx = np.concatenate([np.sort(rng.choice(cols, n, replace=False)) for n in lens]).astype(np.int64)
+    vals = rng.standard_normal(offsets[-1])
+    a = H.clone_compressed(ctx, H.CSR, rows, cols, offsets, idx, vals, H.F64, 8)
+    H.csr_spmv_analyze(ctx, a, H.panel_options(333, 20, H.ALG_VECTOR[2], H.ALG_VECTOR[4]))
+    x = rng.uniform(-1, 1, cols * 3)
+    y0 = rng.uniform(-1, 1, rows * 2)
+    xs = np.ascontiguousarray(x[::3])
+    yref = y0[::2].copy()
+    O.compat_spmv(offsets, idx, vals, -2.0, xs, 0.5, yref)
+    absax = O.abs_spmv(offsets, idx, vals, xs)
+    y = run_compat(ctx, a, x, y0, -2.0, 0.5, H.ALG_PANEL, incx=3, incy=2)
+    bound_check(y[::2], yref, lens, absax, -2.0, 0.5, y0[::2], EPS[np.float64])
+    np.testing.assert_array_equal(y[1::2], y0[1::2])  # holes untouched
+    # empty rows with alpha < 0, beta == 0: y = alpha * (+0.0) = -0.0
+    y = run_compat(ctx, a, xs, np.full(rows, np.nan), -1.0, 0.0, H.ALG_PANEL)
+    empty = lens == 0
+    assert np.all(np.signbit(y[empty])) and np.all(y[empty] == 0.0)
+    H.plan_clear(ctx)
+
+
+def test_panel_plan_standard_spmv_and_precision(ctx):
+    m = G.make(G.ZIPF, 1 << 14, 1 << 14, seed=41, values=G.VALUES_NORMAL)
+    a = H.clone_compressed(ctx, H.CSR, m.rows, m.cols, m.offsets, m.indices, m.values, H.F64, 4)
+    H.csr_spmv_analyze(ctx, a, H.panel_options(1500, 32))
+    x = G.vector(m.cols, 5)
+    y0 = G.vector(m.rows, 6)
+    yref = y0.copy()
+    O.compat_spmv(m.offsets, m.indices, m.values, 0.75, x, 1.0, yref)  # y + 0.75*A*x
+    xb, yb = ctx.upload(x), ctx.upload(y0)
+    H.standard_spmv(ctx, H.OP_N, 0.75, a, H.vector(xb, m.cols, H.F64, writable=False),
+                    H.vector(yb, m.rows, H.F64), alg=H.ALG_PANEL)
+    y = ctx.download(yb, np.float64, m.rows)
+    absax = O.abs_spmv(m.offsets, m.indices, m.values, x)
+    bound_check(y, yref, np.diff(m.offsets), absax, 0.75, 1.0, y0, EPS[np.float64])
+    # BF16 storage, FP32 accumulation (plain and compensated) through the plan
+    bf = H.convert_values(ctx, a, H.BF16)
+    H.csr_spmv_analyze(ctx, bf, H.panel_options(1500, 32))
+    y64 = np.zeros(m.rows)
+    O.compat_spmv(m.offsets, m.indices, m.values, 1.0, x, 0.0, y64)
+    for acc in (H.ACC_PLAIN, H.ACC_TWOSUM):
+        y = run_compat(ctx, bf, x, np.zeros(m.rows), 1.0, 0.0, H.ALG_PANEL, acc=acc)
+        err = np.max(np.abs(y - y64)) / np.max(absax)
+        assert err <= 1e-2, err
+    H.plan_clear(ctx)
