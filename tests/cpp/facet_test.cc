@@ -1,0 +1,410 @@
+// Tests of the C++20 facet (include/asc_sparse_hip/hip.hpp), written the way
+// the reference tests its CUDA facet (tests/sparse_cuda/sparse_cuda_test.cc):
+// golden vectors restated from the reference's tests, element types float and
+// double, and the rejection matrix.
+//
+//   facet_test validate   no GPU needed: host-view validation and every
+//                         rejection on a validation-only context
+//   facet_test gpu        the golden vectors on device 0
+//
+// Exit status 0 when every check passes; each failing check prints a line.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "asc_sparse_hip/hip.hpp"
+
+namespace {
+
+int g_failures = 0;
+int g_checks = 0;
+
+#define CHECK(cond)                                                              \
+  do {                                                                           \
+    ++g_checks;                                                                  \
+    if (!(cond)) {                                                               \
+      ++g_failures;                                                              \
+      std::fprintf(stderr, "FAILED %s:%d: %s\n", __FILE__, __LINE__, #cond);     \
+    }                                                                            \
+  } while (0)
+
+#define CHECK_CODE(status_expr, expected)                                        \
+  do {                                                                           \
+    const asc::Status _s = (status_expr);                                        \
+    ++g_checks;                                                                  \
+    if (_s.code() != (expected)) {                                               \
+      ++g_failures;                                                              \
+      std::fprintf(stderr, "FAILED %s:%d: code %d (%s), expected %d\n", __FILE__, \
+                   __LINE__, static_cast<int>(_s.code()), _s.message().c_str(),  \
+                   static_cast<int>(expected));                                  \
+    }                                                                            \
+  } while (0)
+
+void Case(const char* name) {
+  std::printf("case %s\n", name);
+  std::fflush(stdout);
+}
+
+using asc::ErrorCode;
+using asc::MemorySpace;
+
+// tests/sparse/blas_test.cc:123-125 and sparse_cuda_test.cc:240-341.
+const std::vector<asc::nnz_t> kOffsets3x4{0, 2, 4, 5};
+const std::vector<asc::index_t> kIndices3x4{1, 3, 0, 2, 1};
+const std::vector<double> kValues3x4{2, -1, 4, 5, 3};
+
+const std::vector<asc::nnz_t> kOffsets4x5{0, 2, 3, 6, 8};
+const std::vector<asc::index_t> kIndices4x5{0, 3, 1, 0, 2, 4, 1, 4};
+const std::vector<double> kValues4x5{2, -1, 4, 7, 3, 5, -2, 6};
+
+template <typename E>
+std::vector<E> Cast(const std::vector<double>& v) {
+  return std::vector<E>(v.begin(), v.end());
+}
+
+// ---- validation (no GPU) ---------------------------------------------------------
+
+void TestHostViewValidation() {
+  Case("host view validation");
+  const std::array<asc::extent_t, 2> shape{3, 4};
+  auto ok = asc::CsrView<const double>::Create(kOffsets3x4.data(), kIndices3x4.data(),
+                                              kValues3x4.data(), shape, 5, MemorySpace::kHost);
+  CHECK(ok.ok());
+  // compressed.h:711-746: unsorted and out-of-range indices, wrong final offset
+  const std::vector<asc::index_t> unsorted{3, 1, 0, 2, 1};
+  CHECK_CODE(asc::CsrView<const double>::Create(kOffsets3x4.data(), unsorted.data(),
+                                                kValues3x4.data(), shape, 5, MemorySpace::kHost)
+                 .status(),
+             ErrorCode::kInvalidArgument);
+  const std::vector<asc::index_t> out_of_range{1, 4, 0, 2, 1};
+  CHECK_CODE(asc::CsrView<const double>::Create(kOffsets3x4.data(), out_of_range.data(),
+                                                kValues3x4.data(), shape, 5, MemorySpace::kHost)
+                 .status(),
+             ErrorCode::kIndex);
+  CHECK_CODE(asc::CsrView<const double>::Create(kOffsets3x4.data(), kIndices3x4.data(),
+                                                kValues3x4.data(), shape, 4, MemorySpace::kHost)
+                 .status(),
+             ErrorCode::kInvalidArgument);
+  const std::array<asc::extent_t, 2> tiny{1, 1};
+  CHECK_CODE(asc::CsrView<const double>::Create(kOffsets3x4.data(), kIndices3x4.data(),
+                                                kValues3x4.data(), tiny, 5, MemorySpace::kHost)
+                 .status(),
+             ErrorCode::kShape);
+  CHECK_CODE(asc::HipStridedVectorView<double>::Create(nullptr, 4, 0).status(),
+             ErrorCode::kInvalidArgument);
+  CHECK_CODE(asc::HipStridedVectorView<double>::Create(nullptr, -1, 1).status(),
+             ErrorCode::kInvalidArgument);
+}
+
+void TestRejectionMatrix() {
+  Case("rejection matrix on a validation-only context");
+  auto ctx = asc::SparseHipContext::CreateValidationOnly();
+  CHECK(ctx.ok());
+  if (!ctx.ok()) return;
+  // A device view made by Create is untrusted (compressed.h:759-770).
+  const std::array<asc::extent_t, 2> shape{3, 4};
+  auto* fake = reinterpret_cast<double*>(std::uintptr_t{0x10000});
+  auto untrusted = asc::CsrView<const double>::Create(
+      reinterpret_cast<const asc::nnz_t*>(0x20000), reinterpret_cast<const asc::index_t*>(0x30000),
+      fake, shape, 5, MemorySpace::kDevice);
+  CHECK(untrusted.ok());
+  auto x = asc::HipStridedVectorView<const double>::Create(reinterpret_cast<const double*>(0x40000), 4, 1);
+  auto y = asc::HipStridedVectorView<double>::Create(reinterpret_cast<double*>(0x50000), 3, 1);
+  CHECK(x.ok() && y.ok());
+  CHECK_CODE(asc::HipCsrSpmvWorkspaceSize(*ctx, *untrusted, *x, *y).status(),
+             ErrorCode::kInvalidArgument);
+  // A trusted device view reaches the enqueue point: kUnavailable.
+  const auto trusted = asc::HipProviderAccess::TrustedDevice<const double, asc::SparseCompressedFormat::kCsr>(
+      reinterpret_cast<const void*>(0x20000), reinterpret_cast<const void*>(0x30000), fake, 3, 4, 5, 8);
+  auto ws = asc::HipCsrSpmvWorkspaceSize(*ctx, trusted, *x, *y);
+  CHECK(ws.ok());
+  CHECK_CODE(asc::HipCsrSpmv(*ctx, 1.0, trusted, *x, 0.0, *y, {}, asc::HipSpmvAlgorithm::kVector8).status(),
+             ErrorCode::kUnavailable);
+  // Merge-path needs its workspace.
+  CHECK_CODE(asc::HipCsrSpmv(*ctx, 1.0, trusted, *x, 0.0, *y, {}, asc::HipSpmvAlgorithm::kMergePath).status(),
+             ErrorCode::kInvalidArgument);
+  // Shape mismatches (blas.h:140-205).
+  auto x5 = asc::HipStridedVectorView<const double>::Create(reinterpret_cast<const double*>(0x40000), 5, 1);
+  CHECK_CODE(asc::HipCsrSpmv(*ctx, 1.0, trusted, *x5, 0.0, *y, {}).status(), ErrorCode::kShape);
+  // Output overlapping the matrix values.
+  auto y_on_values = asc::HipStridedVectorView<double>::Create(fake, 3, 1);
+  CHECK_CODE(asc::HipCsrSpmv(*ctx, 1.0, trusted, *x, 0.0, *y_on_values, {}).status(),
+             ErrorCode::kInvalidArgument);
+  // Host matrices are not device operands.
+  auto host = asc::CsrView<const double>::Create(kOffsets3x4.data(), kIndices3x4.data(),
+                                                kValues3x4.data(), shape, 5, MemorySpace::kHost);
+  CHECK_CODE(asc::HipCsrSpmv(*ctx, 1.0, *host, *x, 0.0, *y, {}).status(), ErrorCode::kMemoryAccess);
+  // Column-panel algorithm without an analysis.
+  CHECK_CODE(asc::HipCsrSpmv(*ctx, 1.0, trusted, *x, 0.0, *y, {}, asc::HipSpmvAlgorithm::kColumnPanel)
+                 .status(),
+             ErrorCode::kInvalidArgument);
+  // Standardized SpMV with a wrong output extent.
+  auto vx = asc::SparseBlasVectorView<const double>::Create(
+      reinterpret_cast<const double*>(0x40000), 4, 1, {reinterpret_cast<const void*>(0x40000), 32, MemorySpace::kDevice});
+  auto vy = asc::SparseBlasVectorView<double>::Create(
+      reinterpret_cast<double*>(0x50000), 4, 1, {reinterpret_cast<const void*>(0x50000), 32, MemorySpace::kDevice});
+  CHECK_CODE(asc::HipSpmv(*ctx, asc::SparseBlasTranspose::kNone, 1.0, trusted, *vx, *vy).status(),
+             ErrorCode::kShape);
+}
+
+// ---- GPU ---------------------------------------------------------------------------
+
+struct Gpu {
+  asc::SparseHipContext ctx;
+  asc::HipMemoryResource device;
+};
+
+bool Copy(Gpu& g, void* dst, const void* src, std::size_t bytes) {
+  auto e = asc::HipCopy(g.ctx, dst, src, bytes);
+  return e.ok() && e->Wait().ok();
+}
+
+template <typename E>
+asc::HipBuffer Upload(Gpu& g, const std::vector<E>& host) {
+  auto b = asc::HipBuffer::Allocate(g.device, sizeof(E) * host.size() + 8);
+  CHECK(b.ok());
+  if (!host.empty()) {
+    auto e = asc::HipCopy(g.ctx, b->data(), host.data(), sizeof(E) * host.size());
+    CHECK(e.ok() && e->Wait().ok());
+  }
+  return *b;
+}
+
+template <typename E>
+std::vector<E> Download(Gpu& g, const asc::HipBuffer& b, std::size_t n) {
+  std::vector<E> host(n);
+  if (n) {
+    auto e = asc::HipCopy(g.ctx, host.data(), b.data(), sizeof(E) * n);
+    CHECK(e.ok() && e->Wait().ok());
+  }
+  return host;
+}
+
+template <typename E>
+asc::HipCsrArray<E> CloneCsr(Gpu& g, const std::vector<asc::nnz_t>& off, const std::vector<asc::index_t>& idx,
+                             const std::vector<E>& val, asc::extent_t rows, asc::extent_t cols,
+                             asc::HipIndexWidth width = asc::HipIndexWidth::k64) {
+  const std::array<asc::extent_t, 2> shape{rows, cols};
+  auto host = asc::CsrView<const E>::Create(off.data(), idx.data(), val.data(), shape,
+                                            static_cast<asc::nnz_t>(idx.size()), MemorySpace::kHost);
+  CHECK(host.ok());
+  auto clone = asc::HipCloneCsr(g.ctx, *host, g.device, width);
+  CHECK(clone.ok());
+  CHECK(clone->completion.Wait().ok());
+  return std::move(clone->array);
+}
+
+const asc::HipSpmvAlgorithm kAlgorithms[] = {
+    asc::HipSpmvAlgorithm::kOrdered,  asc::HipSpmvAlgorithm::kMergePath,
+    asc::HipSpmvAlgorithm::kVector1,  asc::HipSpmvAlgorithm::kVector4,
+    asc::HipSpmvAlgorithm::kVector64, asc::HipSpmvAlgorithm::kAuto};
+
+// sparse_cuda_test.cc:240-341: strided device golden, alpha=2, beta=-0.5 and
+// alpha=1, beta=0 with NaN in y.
+template <typename E>
+void TestCsrSpmvGolden(Gpu& g) {
+  Case(sizeof(E) == 8 ? "compat SpMV golden (double)" : "compat SpMV golden (float)");
+  for (auto width : {asc::HipIndexWidth::k64, asc::HipIndexWidth::k32}) {
+    auto a = CloneCsr<E>(g, kOffsets4x5, kIndices4x5, Cast<E>(kValues4x5), 4, 5, width);
+    auto view = a.view();
+    CHECK(view.ok() && view->canonical_structure_trusted());
+    const asc::CsrView<const E> cv = *view;
+    auto xb = Upload<E>(g, {1, -2, 3, 4, -1});
+    for (auto alg : kAlgorithms) {
+      auto yb = Upload<E>(g, {10, 20, 30, 40});
+      auto x = asc::HipStridedVectorView<const E>::Create(static_cast<const E*>(xb.data()), 5, 1);
+      auto y = asc::HipStridedVectorView<E>::Create(static_cast<E*>(yb.data()), 4, 1);
+      auto need = asc::HipCsrSpmvWorkspaceSize(g.ctx, cv, *x, *y, alg);
+      CHECK(need.ok());
+      auto ws = asc::HipBuffer::Allocate(g.device, *need + 256);
+      auto done = asc::HipCsrSpmv(g.ctx, E(2), cv, *x, E(-0.5), *y, ws->mutable_view(), alg);
+      CHECK(done.ok() && done->Wait().ok());
+      const auto out = Download<E>(g, yb, 4);
+      CHECK(out[0] == E(-9) && out[1] == E(-26) && out[2] == E(7) && out[3] == E(-24));
+      auto nanb = Upload<E>(g, {NAN, NAN, NAN, NAN});
+      auto yn = asc::HipStridedVectorView<E>::Create(static_cast<E*>(nanb.data()), 4, 1);
+      done = asc::HipCsrSpmv(g.ctx, E(1), cv, *x, E(0), *yn, ws->mutable_view(), alg);
+      CHECK(done.ok() && done->Wait().ok());
+      const auto out2 = Download<E>(g, nanb, 4);
+      CHECK(out2[0] == E(-2) && out2[1] == E(-8) && out2[2] == E(11) && out2[3] == E(-2));
+    }
+  }
+}
+
+// blas_test.cc:221-242: [1e20, 1, -1e20] . [1, 1, 1] is exactly 0 in float
+// under the reference's left-to-right order.
+void TestSummationOrder(Gpu& g) {
+  Case("summation order (float, ordered)");
+  auto a = CloneCsr<float>(g, {0, 3}, {0, 1, 2}, {1e20f, 1.0f, -1e20f}, 1, 3);
+  const asc::CsrView<const float> cv = *a.view();
+  auto xb = Upload<float>(g, {1, 1, 1});
+  auto yb = Upload<float>(g, {-1});
+  auto x = asc::HipStridedVectorView<const float>::Create(static_cast<const float*>(xb.data()), 3, 1);
+  auto y = asc::HipStridedVectorView<float>::Create(static_cast<float*>(yb.data()), 1, 1);
+  auto done = asc::HipCsrSpmv(g.ctx, 1.0f, cv, *x, 0.0f, *y, {}, asc::HipSpmvAlgorithm::kOrdered);
+  CHECK(done.ok() && done->Wait().ok());
+  CHECK(Download<float>(g, yb, 1)[0] == 0.0f);
+}
+
+// standard_blas_test.cc:176-334 (real parts): 2x2 A = [[2,0],[1,3]].
+template <typename E>
+void TestStandard(Gpu& g) {
+  Case(sizeof(E) == 8 ? "standardized SpMV/SpMM golden (double)" : "standardized SpMV/SpMM golden (float)");
+  auto a = CloneCsr<E>(g, {0, 1, 3}, {0, 0, 1}, {2, 1, 3}, 2, 2);
+  const asc::CsrView<const E> cv = *a.view();
+  auto xb = Upload<E>(g, {1, -1});
+  const asc::ConstMemoryView xs{xb.data(), 2 * sizeof(E), MemorySpace::kDevice};
+  auto x = asc::SparseBlasVectorView<const E>::Create(static_cast<const E*>(xb.data()), 2, 1, xs);
+  for (auto [op, expect] : {std::pair{asc::SparseBlasTranspose::kNone, std::array<E, 2>{2, -2}},
+                            std::pair{asc::SparseBlasTranspose::kTranspose, std::array<E, 2>{1, -3}}}) {
+    auto yb = Upload<E>(g, {0, 0});
+    const asc::ConstMemoryView ys{yb.data(), 2 * sizeof(E), MemorySpace::kDevice};
+    auto y = asc::SparseBlasVectorView<E>::Create(static_cast<E*>(yb.data()), 2, 1, ys);
+    auto need = asc::HipSpmvWorkspaceSize(g.ctx, op, cv, *x, *y);
+    CHECK(need.ok());
+    auto ws = asc::HipBuffer::Allocate(g.device, *need + 256);
+    auto done = asc::HipSpmv(g.ctx, op, E(1), cv, *x, *y, ws->mutable_view());
+    CHECK(done.ok() && done->Wait().ok());
+    const auto out = Download<E>(g, yb, 2);
+    CHECK(out[0] == expect[0] && out[1] == expect[1]);
+  }
+  // Row-major SpMM: C += A*B with B = [[1,2],[3,4]] gives [[2,4],[10,14]].
+  auto bb = Upload<E>(g, {1, 2, 3, 4});
+  auto cb = Upload<E>(g, {0, 0, 0, 0});
+  const asc::ConstMemoryView bs{bb.data(), 4 * sizeof(E), MemorySpace::kDevice};
+  const asc::ConstMemoryView cs{cb.data(), 4 * sizeof(E), MemorySpace::kDevice};
+  auto b = asc::SparseBlasMatrixView<const E>::Create(static_cast<const E*>(bb.data()), 2, 2,
+                                                      asc::SparseBlasLayout::kRowMajor, 2, bs);
+  auto c = asc::SparseBlasMatrixView<E>::Create(static_cast<E*>(cb.data()), 2, 2,
+                                                asc::SparseBlasLayout::kRowMajor, 2, cs);
+  CHECK(b.ok() && c.ok());
+  auto done = asc::HipSpmm(g.ctx, asc::SparseBlasTranspose::kNone, E(1), cv, *b, *c);
+  CHECK(done.ok() && done->Wait().ok());
+  const auto out = Download<E>(g, cb, 4);
+  CHECK(out[0] == 2 && out[1] == 4 && out[2] == 10 && out[3] == 14);
+  // alpha == 0 is a no-op: C keeps its values (standard_blas_test.cc:317-334).
+  done = asc::HipSpmm(g.ctx, asc::SparseBlasTranspose::kNone, E(0), cv, *b, *c);
+  CHECK(done.ok() && done->Wait().ok());
+  CHECK(Download<E>(g, cb, 4) == out);
+}
+
+// compressed_conversion_test.cc:26-34,305-400: CSR <-> CSC <-> COO, bit-exact.
+template <typename E>
+void TestConversions(Gpu& g) {
+  Case(sizeof(E) == 8 ? "conversions golden (double)" : "conversions golden (float)");
+  auto a = CloneCsr<E>(g, kOffsets3x4, kIndices3x4, Cast<E>(kValues3x4), 3, 4);
+  const asc::CsrView<const E> cv = *a.view();
+  auto need = asc::HipTransposeWorkspaceSize(g.ctx, cv);
+  CHECK(need.ok());
+  auto ws = asc::HipBuffer::Allocate(g.device, *need + 256);
+  auto csc = asc::HipConvertToCsc(g.ctx, cv, g.device, ws->mutable_view());
+  CHECK(csc.ok() && csc->completion.Wait().ok());
+  {
+    auto view = csc->array.view();
+    CHECK(view.ok());
+    std::vector<asc::nnz_t> o(5);
+    std::vector<asc::index_t> i(5);
+    std::vector<E> v(5);
+    CHECK(Copy(g, o.data(), view->outer_offsets(), 5 * 8));
+    CHECK(Copy(g, i.data(), view->inner_indices(), 5 * 8));
+    CHECK(Copy(g, v.data(), view->values(), 5 * sizeof(E)));
+    CHECK((o == std::vector<asc::nnz_t>{0, 1, 3, 4, 5}));
+    CHECK((i == std::vector<asc::index_t>{1, 0, 2, 1, 0}));
+    CHECK((v == std::vector<E>{4, 2, 3, 5, -1}));
+    // CSC -> CSR returns the original arrays.
+    const asc::CscView<const E> ccv = *view;
+    auto back = asc::HipConvertToCsr(g.ctx, ccv, g.device, ws->mutable_view());
+    CHECK(back.ok() && back->completion.Wait().ok());
+    auto bv = back->array.view();
+    std::vector<asc::nnz_t> bo(4);
+    std::vector<asc::index_t> bi(5);
+    std::vector<E> bvals(5);
+    CHECK(Copy(g, bo.data(), bv->outer_offsets(), 4 * 8));
+    CHECK(Copy(g, bi.data(), bv->inner_indices(), 5 * 8));
+    CHECK(Copy(g, bvals.data(), bv->values(), 5 * sizeof(E)));
+    CHECK(bo == kOffsets3x4);
+    CHECK(bi == kIndices3x4);
+    CHECK(bvals == Cast<E>(kValues3x4));
+  }
+  // CSR -> COO -> CSR.
+  auto coo = asc::HipConvertToCoordinate(g.ctx, cv, g.device);
+  CHECK(coo.ok() && coo->completion.Wait().ok());
+  auto coov = coo->array.view();
+  CHECK(coov.ok());
+  std::vector<asc::index_t> pairs(10);
+  CHECK(Copy(g, pairs.data(), coov->coordinates(), 10 * 8));
+  CHECK((pairs == std::vector<asc::index_t>{0, 1, 0, 3, 1, 0, 1, 2, 2, 1}));
+  auto cneed = asc::HipConvertCoordinateWorkspaceSize(g.ctx, *coov, asc::SparseCompressedFormat::kCsr);
+  CHECK(cneed.ok());
+  auto cws = asc::HipBuffer::Allocate(g.device, *cneed + 256);
+  auto csr = asc::HipConvertToCsr(g.ctx, *coov, g.device, cws->mutable_view());
+  CHECK(csr.ok() && csr->completion.Wait().ok());
+  auto rv = csr->array.view();
+  std::vector<asc::nnz_t> ro(4);
+  std::vector<asc::index_t> ri(5);
+  CHECK(Copy(g, ro.data(), rv->outer_offsets(), 4 * 8));
+  CHECK(Copy(g, ri.data(), rv->inner_indices(), 5 * 8));
+  CHECK(ro == kOffsets3x4);
+  CHECK(ri == kIndices3x4);
+}
+
+// Column-panel analysis on the golden matrix: panels of 1 column, every row
+// listed (long_row_min 1); the result still meets the golden vector.
+void TestColumnPanel(Gpu& g) {
+  Case("column-panel plan (double)");
+  auto a = CloneCsr<double>(g, kOffsets4x5, kIndices4x5, kValues4x5, 4, 5);
+  const asc::CsrView<const double> cv = *a.view();
+  asc_hip_panel_options opt{1, 1, ASC_HIP_ALG_AUTO, ASC_HIP_ALG_AUTO, {}};
+  auto need = asc::HipCsrSpmvAnalysisSize(g.ctx, cv, &opt);
+  CHECK(need.ok());
+  auto analysis = asc::HipBuffer::Allocate(g.device, *need);
+  CHECK(asc::HipCsrSpmvAnalyze(g.ctx, cv, analysis->mutable_view(), &opt).ok());
+  auto xb = Upload<double>(g, {1, -2, 3, 4, -1});
+  auto yb = Upload<double>(g, {10, 20, 30, 40});
+  auto x = asc::HipStridedVectorView<const double>::Create(static_cast<const double*>(xb.data()), 5, 1);
+  auto y = asc::HipStridedVectorView<double>::Create(static_cast<double*>(yb.data()), 4, 1);
+  auto wneed = asc::HipCsrSpmvWorkspaceSize(g.ctx, cv, *x, *y, asc::HipSpmvAlgorithm::kColumnPanel);
+  CHECK(wneed.ok());
+  auto ws = asc::HipBuffer::Allocate(g.device, *wneed + 256);
+  auto done = asc::HipCsrSpmv(g.ctx, 2.0, cv, *x, -0.5, *y, ws->mutable_view(),
+                              asc::HipSpmvAlgorithm::kColumnPanel);
+  CHECK(done.ok() && done->Wait().ok());
+  const auto out = Download<double>(g, yb, 4);
+  CHECK(out[0] == -9 && out[1] == -26 && out[2] == 7 && out[3] == -24);
+  CHECK(asc_sparse_hip_plan_clear(g.ctx.native()).code == ASC_HIP_OK);
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  const std::string mode = argc > 1 ? argv[1] : "validate";
+  TestHostViewValidation();
+  TestRejectionMatrix();
+  if (mode == "gpu") {
+    auto count = asc::HipDeviceCount();
+    if (!count.ok() || *count < 1) {
+      std::fprintf(stderr, "FAILED: no HIP device\n");
+      return 2;
+    }
+    auto exec = asc::CreateHipExecutionContext(0);
+    CHECK(exec.ok() && exec->backend() == asc::Backend::kHip);
+    auto ctx = asc::SparseHipContext::Create(*exec);
+    CHECK(ctx.ok());
+    if (!ctx.ok()) return 3;
+    auto res = asc::HipMemoryResource::Create(*ctx, MemorySpace::kDevice);
+    CHECK(res.ok());
+    Gpu g{std::move(*ctx), *res};
+    TestCsrSpmvGolden<double>(g);
+    TestCsrSpmvGolden<float>(g);
+    TestSummationOrder(g);
+    TestStandard<double>(g);
+    TestStandard<float>(g);
+    TestConversions<double>(g);
+    TestConversions<float>(g);
+    TestColumnPanel(g);
+  }
+  std::printf("%d checks, %d failures\n", g_checks, g_failures);
+  return g_failures == 0 ? 0 : 1;
+}
