@@ -725,3 +725,47 @@ def csr_spmv_analyze(ctx: Context, a: DeviceSparse, opts: PanelOptions | None = 
         ctx.synchronize()  # queued work may still read the replaced plan
         previous.free()
     return buffer
+
+
+DUPLICATE_REJECT, DUPLICATE_SUM = 0, 1
+ZERO_KEEP, ZERO_DROP = 0, 1
+
+_SIGS5 = {
+    "asc_sparse_hip_coo_finalize_workspace_size": [_VP, P(SparseDesc), P(C.c_size_t)],
+    "asc_sparse_hip_coo_finalize": [_VP, P(SparseDesc), C.c_uint8, C.c_uint8, _VP, _VP, P(SparseDesc),
+                                    _VP, C.c_size_t],
+}
+_SIGS.update(_SIGS5)
+if _lib is not None:
+    for _n, _a in _SIGS5.items():
+        getattr(_lib, _n).argtypes = _a
+        getattr(_lib, _n).restype = Status
+
+
+def triplets(ctx: Context, rows: int, cols: int, coords: np.ndarray, values: np.ndarray,
+             kind: int) -> DeviceSparse:
+    """Raw (row, col, value) triplets in insertion order, uploaded as an
+    untrusted device COO descriptor for coo_finalize."""
+    coords = np.ascontiguousarray(coords, dtype=np.int64).reshape(-1)
+    vals = np.ascontiguousarray(values, dtype=NP_STORAGE[kind])
+    nnz = vals.shape[0]
+    bc = ctx.upload(coords) if nnz else ctx.alloc(16)
+    bv = ctx.upload(vals) if nnz else ctx.alloc(8)
+    d = SparseDesc(bc.ptr, None, bv.ptr, rows, cols, nnz, COO, kind, 8, 8, DEVICE, 0)
+    return DeviceSparse(d, [bc, bv])
+
+
+def coo_finalize(ctx: Context, t: DeviceSparse, duplicate_policy: int, zero_policy: int,
+                 ws: Workspace | None = None) -> DeviceSparse:
+    """Device CoordinateBuilder::Finalize: canonical COO from raw triplets."""
+    d = t.desc
+    need = C.c_size_t()
+    check(ctx.lib.asc_sparse_hip_coo_finalize_workspace_size(ctx.handle, C.byref(d), C.byref(need)))
+    bc = ctx.alloc(max(16 * d.nnz, 16))
+    bv = ctx.alloc(max(ELEM_BYTES[d.element_kind] * d.nnz, 8))
+    ws = ws or Workspace(ctx)
+    p, n = ws.get(need.value)
+    out = SparseDesc()
+    check(ctx.lib.asc_sparse_hip_coo_finalize(ctx.handle, C.byref(d), duplicate_policy, zero_policy,
+                                              bc.ptr, bv.ptr, C.byref(out), p, n))
+    return DeviceSparse(out, [bc, bv])

@@ -839,6 +839,95 @@ asc_hip_status asc_sparse_hip_convert_coordinate(
   return ok();
 }
 
+// ---- COO finalize ------------------------------------------------------------
+
+namespace {
+
+asc_hip_status check_triplets(asc_hip_context* ctx, const asc_hip_sparse_desc* t) {
+  asc_hip_status st = check_context(ctx);
+  if (st.code) return st;
+  if (t == nullptr) return fail(ASC_HIP_INVALID_ARGUMENT, "triplet descriptor is null");
+  if (t->format != ASC_HIP_COO) return fail(ASC_HIP_INVALID_ARGUMENT, "finalize takes COO triplets");
+  if (t->element_kind != ASC_HIP_F32 && t->element_kind != ASC_HIP_F64)
+    return fail(ASC_HIP_UNSUPPORTED, "finalize supports FP32/FP64 values");
+  if (t->rows < 0 || t->cols < 0 || t->nnz < 0)
+    return fail(ASC_HIP_SHAPE, "triplets: negative extent or count");
+  if (t->rows > kU32Max || t->cols > kU32Max || t->nnz > kU32Max)
+    return fail(ASC_HIP_UNSUPPORTED, "device finalize supports extents and counts below 2^32");
+  if (t->memory_space != ASC_HIP_DEVICE && t->memory_space != ASC_HIP_MANAGED)
+    return fail(ASC_HIP_MEMORY_ACCESS, "triplets: operand is not device memory");
+  if (t->nnz > 0) {
+    if (t->outer_offsets == nullptr || t->values == nullptr)
+      return fail(ASC_HIP_INVALID_ARGUMENT, "triplets: arrays are null");
+    if ((st = check_device_pointer(ctx, t->outer_offsets, "triplets")).code) return st;
+    if ((st = check_device_pointer(ctx, t->values, "triplets")).code) return st;
+  }
+  return ok();
+}
+
+}  // namespace
+
+asc_hip_status asc_sparse_hip_coo_finalize_workspace_size(asc_hip_context* ctx,
+                                                          const asc_hip_sparse_desc* t,
+                                                          size_t* out) {
+  if (out == nullptr) return fail(ASC_HIP_INVALID_ARGUMENT, "output pointer is null");
+  asc_hip_status st = check_triplets(ctx, t);
+  if (st.code) return st;
+  *out = asc_hip::coo_finalize_workspace_bytes(t->rows, t->cols, t->nnz,
+                                               static_cast<Elem>(t->element_kind));
+  return ok();
+}
+
+asc_hip_status asc_sparse_hip_coo_finalize(asc_hip_context* ctx, const asc_hip_sparse_desc* t,
+                                           uint8_t duplicate_policy, uint8_t zero_policy,
+                                           void* out_coords, void* out_vals,
+                                           asc_hip_sparse_desc* out, void* ws, size_t ws_bytes) {
+  size_t need = 0;
+  asc_hip_status st = asc_sparse_hip_coo_finalize_workspace_size(ctx, t, &need);
+  if (st.code) return st;
+  if (duplicate_policy > ASC_HIP_DUPLICATE_SUM)
+    return fail(ASC_HIP_INVALID_ARGUMENT, "Unknown sparse duplicate policy");
+  if (zero_policy > ASC_HIP_ZERO_DROP)
+    return fail(ASC_HIP_INVALID_ARGUMENT, "Unknown sparse explicit-zero policy");
+  if (out == nullptr) return fail(ASC_HIP_INVALID_ARGUMENT, "output descriptor is null");
+  const size_t n = static_cast<size_t>(t->nnz);
+  const size_t esz = elem_size(t->element_kind);
+  if (n > 0) {
+    if (out_coords == nullptr || out_vals == nullptr)
+      return fail(ASC_HIP_INVALID_ARGUMENT, "output arrays are null");
+    if ((st = check_device_pointer(ctx, out_coords, "output")).code) return st;
+    if ((st = check_device_pointer(ctx, out_vals, "output")).code) return st;
+  }
+  const Span oc = span_of(out_coords, 16 * n), ov = span_of(out_vals, esz * n);
+  const Span ic = span_of(t->outer_offsets, 16 * n), iv = span_of(t->values, esz * n);
+  if (overlaps(oc, ov) || overlaps(oc, ic) || overlaps(oc, iv) || overlaps(ov, ic) || overlaps(ov, iv))
+    return fail(ASC_HIP_INVALID_ARGUMENT, "finalize outputs overlap each other or the triplets");
+  if ((st = check_workspace(ctx, ws, ws_bytes, need, {oc, ov, ic, iv})).code) return st;
+  if ((st = unavailable_if_validation_only(ctx)).code) return st;
+  DeviceGuard guard(ctx->device);
+  if (guard.error != hipSuccess) return fail_hip(guard.error, "hipSetDevice");
+  int64_t count = 0;
+  int failure = 0;
+  const hipError_t e = asc_hip::coo_finalize_launch(
+      static_cast<const int64_t*>(t->outer_offsets), t->values,
+      static_cast<Elem>(t->element_kind), t->rows, t->cols, t->nnz,
+      duplicate_policy == ASC_HIP_DUPLICATE_SUM, zero_policy == ASC_HIP_ZERO_DROP,
+      static_cast<int64_t*>(out_coords), out_vals, &count, &failure, ws, ctx->stream);
+  if ((st = finish_launch(ctx, e, "COO finalize")).code) return st;
+  if (failure == 1) return fail(ASC_HIP_INDEX, "Sparse coordinate is outside the shape");
+  if (failure == 2) return fail(ASC_HIP_INVALID_ARGUMENT, "Duplicate sparse coordinates were rejected");
+  *out = *t;
+  out->outer_offsets = out_coords;
+  out->inner_indices = nullptr;
+  out->values = out_vals;
+  out->nnz = count;
+  out->index_bytes = 8;
+  out->offset_bytes = 8;
+  out->memory_space = ASC_HIP_DEVICE;
+  out->trusted = 1;
+  return ok();
+}
+
 asc_hip_status asc_sparse_hip_convert_compressed_workspace_size(
     asc_hip_context* ctx, const asc_hip_sparse_desc* src, size_t* out) {
   if (out == nullptr) return fail(ASC_HIP_INVALID_ARGUMENT, "output pointer is null");

@@ -413,10 +413,179 @@ hipError_t stable_sort(const K0* key0, int64_t key_stride, const R0* row0,
 #undef ASC_SCATTER
     if ((err = hipGetLastError()) != hipSuccess) return err;
   }
+  if (out_offsets == nullptr) return hipSuccess;  // sorted keys stay in the workspace
   const uint32_t* sorted_keys = kbuf[(passes - 1) & 1];
   boundary_kernel<<<grid_stride_blocks(n + 1), kBlock, 0, s>>>(
       sorted_keys, n, outer, out_offsets);
   return hipGetLastError();
+}
+
+// Where stable_sort leaves the sorted keys (uint32) inside its workspace.
+const uint32_t* sorted_keys_in(const char* ws, int64_t n, int64_t outer, size_t vbytes) {
+  const SortLayout L = sort_layout(n, outer, vbytes);
+  return reinterpret_cast<const uint32_t*>(ws + L.keys[(passes_for(outer) - 1) & 1]);
+}
+
+// ---- COO finalize (CoordinateBuilder::Finalize) ----------------------------
+// Raw triplets are checked against the shape, stably sorted by column and
+// then by row (two LSD sorts: equal coordinates keep insertion order), and
+// compacted: each run of equal coordinates is summed left to right in
+// insertion order, as the reference's Compact does (coordinate.h:1462-1497).
+
+__global__ __launch_bounds__(kBlock) void finalize_bounds_kernel(
+    const int64_t* __restrict__ coords, int64_t n, int64_t rows, int64_t cols,
+    uint32_t* __restrict__ bad) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) {
+    const int64_t r = coords[2 * i], c = coords[2 * i + 1];
+    if (r < 0 || r >= rows || c < 0 || c >= cols) *bad = 1u;  // any writer: same value
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void finalize_head_kernel(
+    const uint32_t* __restrict__ rows_sorted, const int64_t* __restrict__ cols_sorted,
+    int64_t n, uint32_t* __restrict__ head, uint32_t* __restrict__ duplicate) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) {
+    const bool h = i == 0 || rows_sorted[i] != rows_sorted[i - 1] ||
+                   cols_sorted[i] != cols_sorted[i - 1];
+    head[i] = h ? 1u : 0u;
+    if (!h) *duplicate = 1u;
+  }
+}
+
+// One thread per run head: the run's sum in insertion order (no FMA), and
+// whether it survives the zero policy (exact zeros, -0.0 included, are
+// dropped; NaN is kept).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void finalize_sum_kernel(
+    const uint32_t* __restrict__ head, const T* __restrict__ vals, int64_t n, int drop_zeros,
+    T* __restrict__ sums, uint32_t* __restrict__ keep) {
+#pragma clang fp contract(off)
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) {
+    if (!head[i]) {
+      keep[i] = 0u;
+      continue;
+    }
+    T v = vals[i];
+    for (int64_t j = i + 1; j < n && !head[j]; ++j) v = v + vals[j];
+    sums[i] = v;
+    keep[i] = (drop_zeros && v == T(0)) ? 0u : 1u;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void finalize_write_kernel(
+    const uint32_t* __restrict__ keep, const uint32_t* __restrict__ pos,
+    const uint32_t* __restrict__ rows_sorted, const int64_t* __restrict__ cols_sorted,
+    const T* __restrict__ sums, int64_t n, int64_t* __restrict__ out_coords,
+    T* __restrict__ out_vals, uint32_t* __restrict__ count) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) {
+    if (keep[i]) {
+      const int64_t d = pos[i];
+      out_coords[2 * d] = rows_sorted[i];
+      out_coords[2 * d + 1] = cols_sorted[i];
+      out_vals[d] = sums[i];
+    }
+    if (i == n - 1) *count = pos[i] + keep[i];
+  }
+}
+
+struct FinalizeLayout {
+  size_t sort_a, sort_b, rows_a, vals_a, cols_b, vals_b, head, keep, pos, sums, scan, flags, total;
+};
+
+FinalizeLayout finalize_layout(int64_t n, int64_t rows, int64_t cols, size_t vbytes) {
+  FinalizeLayout L{};
+  const size_t un = static_cast<size_t>(n > 0 ? n : 1);
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = off;
+    off += align256(bytes);
+    return o;
+  };
+  L.sort_a = take(sort_layout(n, cols, vbytes).total);
+  L.sort_b = take(sort_layout(n, rows, vbytes).total);
+  L.rows_a = take(8 * un);
+  L.vals_a = take(vbytes * un);
+  L.cols_b = take(8 * un);
+  L.vals_b = take(vbytes * un);
+  L.head = take(4 * un);
+  L.keep = take(4 * un);
+  L.pos = take(4 * un);
+  L.sums = take(vbytes * un);
+  L.scan = take(scan_workspace_bytes(n));
+  L.flags = take(16);
+  L.total = off;
+  return L;
+}
+
+template <typename T>
+hipError_t finalize_typed(const int64_t* coords, const T* vals, int64_t rows, int64_t cols,
+                          int64_t n, bool sum_duplicates, bool drop_zeros,
+                          int64_t* out_coords, T* out_vals, int64_t* out_nnz,
+                          int* out_failure, char* ws, hipStream_t s) {
+  using VB = T;
+  const FinalizeLayout L = finalize_layout(n, rows, cols, sizeof(T));
+  uint32_t* flags = reinterpret_cast<uint32_t*>(ws + L.flags);  // [0] bounds, [1] dup, [2] count
+  uint32_t host_flags[3] = {0u, 0u, 0u};
+  *out_nnz = 0;
+  *out_failure = 0;
+  hipError_t e = fill_launch(flags, 0, 16, s);
+  if (e != hipSuccess || n == 0) return e;
+  const unsigned g = grid_stride_blocks(n);
+  finalize_bounds_kernel<<<g, kBlock, 0, s>>>(coords, n, rows, cols, flags);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = hipMemcpyAsync(host_flags, flags, 4, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+  if (host_flags[0]) {
+    *out_failure = 1;  // coordinate out of range
+    return hipSuccess;
+  }
+  // Sort by column, then (stably) by row.
+  char* wa = ws + L.sort_a;
+  char* wb = ws + L.sort_b;
+  int64_t* rows_a = reinterpret_cast<int64_t*>(ws + L.rows_a);
+  VB* vals_a = reinterpret_cast<VB*>(ws + L.vals_a);
+  int64_t* cols_b = reinterpret_cast<int64_t*>(ws + L.cols_b);
+  VB* vals_b = reinterpret_cast<VB*>(ws + L.vals_b);
+  if ((e = stable_sort<int64_t, int64_t, VB>(coords + 1, 2, coords, 2, vals, n, cols, rows_a, 8,
+                                             vals_a, nullptr, wa, s)) != hipSuccess)
+    return e;
+  const uint32_t* cols_a = sorted_keys_in(wa, n, cols, sizeof(VB));
+  if ((e = stable_sort<int64_t, uint32_t, VB>(rows_a, 1, cols_a, 1, vals_a, n, rows, cols_b, 8,
+                                              vals_b, nullptr, wb, s)) != hipSuccess)
+    return e;
+  const uint32_t* rows_b = sorted_keys_in(wb, n, rows, sizeof(VB));
+  uint32_t* head = reinterpret_cast<uint32_t*>(ws + L.head);
+  uint32_t* keep = reinterpret_cast<uint32_t*>(ws + L.keep);
+  uint32_t* pos = reinterpret_cast<uint32_t*>(ws + L.pos);
+  T* sums = reinterpret_cast<T*>(ws + L.sums);
+  finalize_head_kernel<<<g, kBlock, 0, s>>>(rows_b, cols_b, n, head, flags + 1);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if (!sum_duplicates) {
+    if ((e = hipMemcpyAsync(host_flags + 1, flags + 1, 4, hipMemcpyDeviceToHost, s)) != hipSuccess)
+      return e;
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+    if (host_flags[1]) {
+      *out_failure = 2;  // duplicate coordinates under the reject policy
+      return hipSuccess;
+    }
+  }
+  finalize_sum_kernel<T><<<g, kBlock, 0, s>>>(head, vals_b, n, drop_zeros ? 1 : 0, sums, keep);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = scan_exclusive(keep, pos, n, reinterpret_cast<uint32_t*>(ws + L.scan), s)) != hipSuccess)
+    return e;
+  finalize_write_kernel<T><<<g, kBlock, 0, s>>>(keep, pos, rows_b, cols_b, sums, n, out_coords,
+                                                out_vals, flags + 2);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = hipMemcpyAsync(host_flags + 2, flags + 2, 4, hipMemcpyDeviceToHost, s)) != hipSuccess)
+    return e;
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+  *out_nnz = host_flags[2];
+  return hipSuccess;
 }
 
 template <typename VB>
@@ -571,6 +740,25 @@ hipError_t coo_convert_launch(const int64_t* coords, const void* values,
           cols, out_indices, out_index_bytes, static_cast<uint16_t*>(out_values),
           out_offsets, ws, s);
   }
+}
+
+size_t coo_finalize_workspace_bytes(int64_t rows, int64_t cols, int64_t nnz, Elem elem) {
+  return finalize_layout(nnz, rows, cols, elem_bytes(elem)).total;
+}
+
+hipError_t coo_finalize_launch(const int64_t* coords, const void* values, Elem elem,
+                               int64_t rows, int64_t cols, int64_t nnz, bool sum_duplicates,
+                               bool drop_zeros, int64_t* out_coords, void* out_values,
+                               int64_t* out_nnz, int* out_failure, void* workspace,
+                               hipStream_t s) {
+  char* ws = static_cast<char*>(workspace);
+  if (elem == Elem::F64)
+    return finalize_typed<double>(coords, static_cast<const double*>(values), rows, cols, nnz,
+                                  sum_duplicates, drop_zeros, out_coords,
+                                  static_cast<double*>(out_values), out_nnz, out_failure, ws, s);
+  return finalize_typed<float>(coords, static_cast<const float*>(values), rows, cols, nnz,
+                               sum_duplicates, drop_zeros, out_coords,
+                               static_cast<float*>(out_values), out_nnz, out_failure, ws, s);
 }
 
 }  // namespace asc_hip

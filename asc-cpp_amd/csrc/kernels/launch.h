@@ -168,6 +168,20 @@ hipError_t coo_convert_launch(const int64_t* coords, const void* values,
                               void* out_values, void* workspace,
                               hipStream_t stream);
 
+// Raw device triplets (interleaved int64 (row, col), F32/F64 values) ->
+// canonical COO, as CoordinateBuilder::Finalize: stable (row, col) order,
+// runs of equal coordinates summed in insertion order (or rejected), exact
+// zeros dropped on request.  Synchronous: *out_nnz is the finalized count;
+// *out_failure is 1 for a coordinate outside the shape, 2 for a duplicate
+// under the reject policy (outputs untouched), else 0.  rows, cols and nnz
+// must not exceed UINT32_MAX.
+size_t coo_finalize_workspace_bytes(int64_t rows, int64_t cols, int64_t nnz, Elem elem);
+hipError_t coo_finalize_launch(const int64_t* coords, const void* values, Elem elem,
+                               int64_t rows, int64_t cols, int64_t nnz, bool sum_duplicates,
+                               bool drop_zeros, int64_t* out_coords, void* out_values,
+                               int64_t* out_nnz, int* out_failure, void* workspace,
+                               hipStream_t stream);
+
 // CSR (outer = rows) -> canonical COO: interleaved int64 (row, col) and values.
 hipError_t csr_to_coo_launch(const Compressed& a, int64_t* coords, void* out_values,
                              hipStream_t stream);

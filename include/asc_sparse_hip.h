@@ -414,6 +414,31 @@ asc_hip_status asc_sparse_hip_convert_to_coordinate(
     void* device_coordinates, void* device_values, asc_hip_sparse_desc* out,
     void* workspace, size_t workspace_bytes);
 
+/*
+ * Device CoordinateBuilder::Finalize (coordinate.h:1300-1497): raw device
+ * triplets (a COO descriptor of interleaved int64 (row, col) coordinates and
+ * F32/F64 values, in insertion order; `trusted` not required, nnz may exceed
+ * rows*cols through duplicates) -> canonical COO in caller-provided buffers
+ * sized for `nnz` entries (2*nnz int64 coordinates, nnz values).
+ *  - Coordinates outside the shape fail with ASC_HIP_INDEX.
+ *  - ASC_HIP_DUPLICATE_REJECT fails with ASC_HIP_INVALID_ARGUMENT if any
+ *    coordinate repeats; ASC_HIP_DUPLICATE_SUM sums each run of equal
+ *    coordinates left to right in insertion order (no FMA).
+ *  - ASC_HIP_ZERO_DROP then drops entries equal to zero (-0.0 included,
+ *    NaN kept); ASC_HIP_ZERO_KEEP keeps them.
+ * On failure the outputs are untouched.  The call waits for its stream and
+ * returns the finalized descriptor (its nnz is the entry count) in `out`.
+ * rows, cols and nnz must not exceed UINT32_MAX.  The reference builder is
+ * host-only; this is its device form.
+ */
+asc_hip_status asc_sparse_hip_coo_finalize_workspace_size(
+    asc_hip_context* context, const asc_hip_sparse_desc* triplets, size_t* out_bytes);
+asc_hip_status asc_sparse_hip_coo_finalize(
+    asc_hip_context* context, const asc_hip_sparse_desc* triplets,
+    uint8_t duplicate_policy, uint8_t zero_policy, void* out_coordinates,
+    void* out_values, asc_hip_sparse_desc* out, void* workspace,
+    size_t workspace_bytes);
+
 /* ---- batched SpMV --------------------------------------------------------- */
 
 /*

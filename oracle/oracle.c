@@ -361,49 +361,53 @@ static void merge_sort(int64_t* perm, int64_t* tmp, int64_t n) {
   }
 }
 
-int oracle_coo_finalize_f64(int64_t nnz, const int64_t* coords,
-                            const double* values, int duplicate_policy,
-                            int zero_policy, int64_t* out_coords,
-                            double* out_values, int64_t* out_nnz) {
-  int64_t* perm = (int64_t*)malloc((size_t)(nnz > 0 ? nnz : 1) * sizeof(int64_t));
-  int64_t* tmp = (int64_t*)malloc((size_t)(nnz > 0 ? nnz : 1) * sizeof(int64_t));
-  if (!perm || !tmp) {
-    free(perm);
-    free(tmp);
-    return ORACLE_SHAPE;
+/* One body for both element types: T is the element, NAME the symbol. */
+#define ORACLE_FINALIZE_BODY(NAME, T)                                            \
+  int NAME(int64_t nnz, const int64_t* coords, const T* values,                 \
+           int duplicate_policy, int zero_policy, int64_t* out_coords,          \
+           T* out_values, int64_t* out_nnz) {                                   \
+    int64_t* perm = (int64_t*)malloc((size_t)(nnz > 0 ? nnz : 1) * sizeof(int64_t)); \
+    int64_t* tmp = (int64_t*)malloc((size_t)(nnz > 0 ? nnz : 1) * sizeof(int64_t));  \
+    if (!perm || !tmp) {                                                         \
+      free(perm);                                                                \
+      free(tmp);                                                                 \
+      return ORACLE_SHAPE;                                                       \
+    }                                                                            \
+    for (int64_t i = 0; i < nnz; ++i) perm[i] = i;                               \
+    g_sort_coords = coords;                                                      \
+    merge_sort(perm, tmp, nnz);                                                  \
+    /* Preflight: any duplicate under kReject fails before output is written. */ \
+    if (duplicate_policy == 0) {                                                 \
+      for (int64_t i = 1; i < nnz; ++i) {                                        \
+        const int64_t a = perm[i - 1], b = perm[i];                              \
+        if (coords[2 * a] == coords[2 * b] && coords[2 * a + 1] == coords[2 * b + 1]) { \
+          free(perm);                                                            \
+          free(tmp);                                                             \
+          *out_nnz = 0;                                                          \
+          return ORACLE_INVALID_ARGUMENT;                                        \
+        }                                                                        \
+      }                                                                          \
+    }                                                                            \
+    int64_t output = 0, input = 0;                                               \
+    while (input < nnz) {                                                        \
+      const int64_t head = perm[input++];                                        \
+      T value = values[head];                                                    \
+      while (input < nnz && coords[2 * perm[input]] == coords[2 * head] &&       \
+             coords[2 * perm[input] + 1] == coords[2 * head + 1]) {              \
+        value += values[perm[input]];                                            \
+        ++input;                                                                 \
+      }                                                                          \
+      if (zero_policy == 1 && value == (T)0) continue;                           \
+      out_coords[2 * output] = coords[2 * head];                                 \
+      out_coords[2 * output + 1] = coords[2 * head + 1];                         \
+      out_values[output] = value;                                                \
+      ++output;                                                                  \
+    }                                                                            \
+    *out_nnz = output;                                                           \
+    free(perm);                                                                  \
+    free(tmp);                                                                   \
+    return ORACLE_OK;                                                            \
   }
-  for (int64_t i = 0; i < nnz; ++i) perm[i] = i;
-  g_sort_coords = coords;
-  merge_sort(perm, tmp, nnz);
-  /* Preflight: any duplicate under kReject fails before output is written. */
-  if (duplicate_policy == 0) {
-    for (int64_t i = 1; i < nnz; ++i) {
-      const int64_t a = perm[i - 1], b = perm[i];
-      if (coords[2 * a] == coords[2 * b] && coords[2 * a + 1] == coords[2 * b + 1]) {
-        free(perm);
-        free(tmp);
-        *out_nnz = 0;
-        return ORACLE_INVALID_ARGUMENT;
-      }
-    }
-  }
-  int64_t output = 0, input = 0;
-  while (input < nnz) {
-    const int64_t head = perm[input++];
-    double value = values[head];
-    while (input < nnz && coords[2 * perm[input]] == coords[2 * head] &&
-           coords[2 * perm[input] + 1] == coords[2 * head + 1]) {
-      value += values[perm[input]];
-      ++input;
-    }
-    if (zero_policy == 1 && value == 0.0) continue;
-    out_coords[2 * output] = coords[2 * head];
-    out_coords[2 * output + 1] = coords[2 * head + 1];
-    out_values[output] = value;
-    ++output;
-  }
-  *out_nnz = output;
-  free(perm);
-  free(tmp);
-  return ORACLE_OK;
-}
+
+ORACLE_FINALIZE_BODY(oracle_coo_finalize_f64, double)
+ORACLE_FINALIZE_BODY(oracle_coo_finalize_f32, float)

@@ -29,7 +29,7 @@
  *   oracle_*_fast           O(nnz) counting-sort restatement; equal to the
  *                           faithful form on random inputs (the canonical
  *                           order is unique) and to oracle/_ref.
- *   oracle_coo_finalize_f64 golden vectors of coordinate_test.cc and a
+ *   oracle_coo_finalize_*   golden vectors of coordinate_test.cc and a
  *                           Python sorted()-based brute force.
  */
 #ifndef ASC_SPARSE_HIP_ORACLE_H_
@@ -154,6 +154,10 @@ int oracle_coo_finalize_f64(int64_t nnz, const int64_t* coords,
                             const double* values, int duplicate_policy,
                             int zero_policy, int64_t* out_coords,
                             double* out_values, int64_t* out_nnz);
+int oracle_coo_finalize_f32(int64_t nnz, const int64_t* coords,
+                            const float* values, int duplicate_policy,
+                            int zero_policy, int64_t* out_coords,
+                            float* out_values, int64_t* out_nnz);
 
 #ifdef __cplusplus
 }

@@ -270,13 +270,18 @@ int ref_transpose_f64(int src_fmt, int64_t rows, int64_t cols, const int64_t* of
   return copy_out(*out, rows, out_off, out_idx, out_val);
 }
 
-int ref_finalize_f64(int64_t rows, int64_t cols, int64_t nnz, const int64_t* coords,
-                     const double* values, int duplicate_policy, int zero_policy,
-                     int64_t* out_coords, double* out_values, int64_t* out_nnz) {
+}  // extern "C"
+
+namespace {
+
+template <typename T>
+int RefFinalize(int64_t rows, int64_t cols, int64_t nnz, const int64_t* coords,
+                const T* values, int duplicate_policy, int zero_policy,
+                int64_t* out_coords, T* out_values, int64_t* out_nnz) {
   asc::HostMemoryResource resource;
   auto extents = Dyn2::Create(rows, cols);
   if (!extents.ok()) return static_cast<int>(extents.status().code());
-  auto builder = asc::CoordinateBuilder<double, Dyn2>::Create(resource, *extents, nnz);
+  auto builder = asc::CoordinateBuilder<T, Dyn2>::Create(resource, *extents, nnz);
   if (!builder.ok()) return static_cast<int>(builder.status().code());
   for (int64_t p = 0; p < nnz; ++p) {
     const std::array<asc::index_t, 2> c{coords[2 * p], coords[2 * p + 1]};
@@ -291,8 +296,26 @@ int ref_finalize_f64(int64_t rows, int64_t cols, int64_t nnz, const int64_t* coo
   if (!view.ok()) return static_cast<int>(view.status().code());
   *out_nnz = view->nnz();
   std::memcpy(out_coords, view->coordinates(), sizeof(int64_t) * 2 * view->nnz());
-  std::memcpy(out_values, view->values(), sizeof(double) * view->nnz());
+  std::memcpy(out_values, view->values(), sizeof(T) * view->nnz());
   return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ref_finalize_f64(int64_t rows, int64_t cols, int64_t nnz, const int64_t* coords,
+                     const double* values, int duplicate_policy, int zero_policy,
+                     int64_t* out_coords, double* out_values, int64_t* out_nnz) {
+  return RefFinalize<double>(rows, cols, nnz, coords, values, duplicate_policy, zero_policy,
+                             out_coords, out_values, out_nnz);
+}
+
+int ref_finalize_f32(int64_t rows, int64_t cols, int64_t nnz, const int64_t* coords,
+                     const float* values, int duplicate_policy, int zero_policy,
+                     int64_t* out_coords, float* out_values, int64_t* out_nnz) {
+  return RefFinalize<float>(rows, cols, nnz, coords, values, duplicate_policy, zero_policy,
+                            out_coords, out_values, out_nnz);
 }
 
 }  // extern "C"

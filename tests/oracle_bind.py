@@ -37,6 +37,7 @@ def lib():
         L.oracle_transpose_fast.argtypes = [C.c_int, I64, I64, VP, VP, VP, C.c_int, VP, VP, VP]
         L.oracle_compressed_to_coo.argtypes = [C.c_int, I64, VP, VP, VP, C.c_int, VP, VP]
         L.oracle_coo_finalize_f64.argtypes = [I64, VP, VP, C.c_int, C.c_int, VP, VP, VP]
+        L.oracle_coo_finalize_f32.argtypes = [I64, VP, VP, C.c_int, C.c_int, VP, VP, VP]
         _lib = L
     return _lib
 
@@ -119,14 +120,16 @@ def compressed_to_coo(src_fmt, outer, offsets, indices, values):
 
 
 def finalize(coords, values, duplicate_policy, zero_policy):
+    """FP64 or FP32 by the dtype of `values` (anything else is taken as FP64)."""
     coords = i64(coords).reshape(-1)
-    values = np.ascontiguousarray(values, dtype=np.float64)
+    dt = np.float32 if np.asarray(values).dtype == np.float32 else np.float64
+    values = np.ascontiguousarray(values, dtype=dt)
     nnz = values.shape[0]
     oc = np.empty(max(2 * nnz, 2), dtype=np.int64)
-    ov = np.empty(max(nnz, 1), dtype=np.float64)
+    ov = np.empty(max(nnz, 1), dtype=dt)
     n = C.c_int64()
-    rc = lib().oracle_coo_finalize_f64(nnz, _p(coords), _p(values), duplicate_policy, zero_policy,
-                                       _p(oc), _p(ov), C.byref(n))
+    fn = lib().oracle_coo_finalize_f32 if dt == np.float32 else lib().oracle_coo_finalize_f64
+    rc = fn(nnz, _p(coords), _p(values), duplicate_policy, zero_policy, _p(oc), _p(ov), C.byref(n))
     return rc, oc[: 2 * n.value], ov[: n.value]
 
 
@@ -155,6 +158,7 @@ def ref():
         R.ref_coo_to_compressed_f64.argtypes = [C.c_int, I64, I64, I64, VP, VP, VP, VP, VP]
         R.ref_transpose_f64.argtypes = [C.c_int, I64, I64, VP, VP, VP, VP, VP, VP]
         R.ref_finalize_f64.argtypes = [I64, I64, I64, VP, VP, C.c_int, C.c_int, VP, VP, VP]
+        R.ref_finalize_f32.argtypes = [I64, I64, I64, VP, VP, C.c_int, C.c_int, VP, VP, VP]
         _ref = R
     return _ref
 
@@ -219,11 +223,13 @@ def ref_transpose(src_fmt, rows, cols, offsets, indices, values):
 
 def ref_finalize(rows, cols, coords, values, duplicate_policy, zero_policy):
     coords = i64(coords).reshape(-1)
-    values = np.ascontiguousarray(values, dtype=np.float64)
+    dt = np.float32 if np.asarray(values).dtype == np.float32 else np.float64
+    values = np.ascontiguousarray(values, dtype=dt)
     nnz = values.shape[0]
     oc = np.empty(max(2 * nnz, 2), np.int64)
-    ov = np.empty(max(nnz, 1))
+    ov = np.empty(max(nnz, 1), dt)
     n = C.c_int64()
-    rc = ref().ref_finalize_f64(rows, cols, nnz, _p(coords), _p(values), duplicate_policy,
-                                zero_policy, _p(oc), _p(ov), C.byref(n))
+    fn = ref().ref_finalize_f32 if dt == np.float32 else ref().ref_finalize_f64
+    rc = fn(rows, cols, nnz, _p(coords), _p(values), duplicate_policy, zero_policy, _p(oc),
+            _p(ov), C.byref(n))
     return rc, oc[: 2 * n.value], ov[: n.value]

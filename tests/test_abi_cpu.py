@@ -260,3 +260,38 @@ def test_panel_algorithm_requires_analysis(vctx):
     # plan_set cannot register a panel plan without its split table
     assert vctx.lib.asc_sparse_hip_plan_set(vctx.handle, C.byref(a), H.ALG_PANEL).code == H.INVALID_ARGUMENT
     assert vctx.lib.asc_sparse_hip_plan_set(vctx.handle, C.byref(a), 11).code == H.INVALID_ARGUMENT
+
+
+def test_finalize_validation(vctx):
+    lib, h = vctx.lib, vctx.handle
+    need = C.c_size_t()
+
+    def trip(rows=3, cols=4, nnz=5, kind=H.F64, space=H.DEVICE, fmt=H.COO, base=FAKE):
+        return H.SparseDesc(base, None, base + 0x1000, rows, cols, nnz, fmt, kind, 8, 8, space, 0)
+
+    def size(t):
+        return lib.asc_sparse_hip_coo_finalize_workspace_size(h, C.byref(t), C.byref(need)).code
+
+    assert size(trip()) == H.OK  # untrusted raw triplets are the input
+    assert size(trip(nnz=50)) == H.OK  # duplicates may exceed rows*cols
+    assert size(trip(kind=H.BF16)) == H.UNSUPPORTED
+    assert size(trip(fmt=H.CSR)) == H.INVALID_ARGUMENT
+    assert size(trip(space=H.HOST)) == H.MEMORY_ACCESS
+    assert size(trip(rows=-1)) == H.SHAPE
+    assert size(trip(cols=1 << 33)) == H.UNSUPPORTED
+    assert size(trip()) == H.OK
+    n = need.value
+    out = H.SparseDesc()
+    ws, oc, ov = 0x4000000, 0x8000000, 0x9000000
+
+    def fin(t, dup, zero, c=oc, v=ov, w=ws, nb=None):
+        return lib.asc_sparse_hip_coo_finalize(h, C.byref(t), dup, zero, c, v, C.byref(out), w,
+                                               n if nb is None else nb).code
+
+    assert fin(trip(), 2, 0) == H.INVALID_ARGUMENT  # unknown duplicate policy
+    assert fin(trip(), 1, 2) == H.INVALID_ARGUMENT  # unknown zero policy
+    assert fin(trip(), 1, 0, c=None) == H.INVALID_ARGUMENT
+    assert fin(trip(), 1, 0, c=FAKE) == H.INVALID_ARGUMENT  # output over the triplets
+    assert fin(trip(), 1, 0, v=oc + 8) == H.INVALID_ARGUMENT  # outputs overlap
+    assert fin(trip(), 1, 0, nb=n - 8) == H.INVALID_ARGUMENT  # short workspace
+    assert fin(trip(), 1, 1) == H.UNAVAILABLE  # valid: reaches the enqueue point
