@@ -194,7 +194,10 @@ __device__ __forceinline__ T ld_plain(const T* p) { return *p; }
 
 // One stable scatter round.  Entry i of the tile is handled by wave
 // (i / kWaveItems), chunk ((i % kWaveItems) / 64), lane (i % 64), so chunk
-// order and lane order are source order.
+// order and lane order are source order.  The tile is first reordered by
+// digit in LDS (stable: digit, then wave, then rank within the wave), then
+// written out by consecutive threads, so each digit's run of the tile leaves
+// as contiguous stores instead of one scattered store per entry.
 template <typename K, typename R, typename VB, typename RO>
 __global__ __launch_bounds__(kBlock) void radix_scatter_kernel(
     const K* __restrict__ keys, int64_t key_stride, const R* __restrict__ rows,
@@ -203,13 +206,21 @@ __global__ __launch_bounds__(kBlock) void radix_scatter_kernel(
     uint32_t* __restrict__ keys_out, RO* __restrict__ rows_out,
     VB* __restrict__ vals_out) {
   __shared__ uint32_t s_count[kWaves][kRadix];
+  __shared__ uint32_t s_wave_off[kWaves][kRadix];  // earlier waves' counts per digit
+  __shared__ uint32_t s_local[kRadix];              // tile-local start of each digit
+  __shared__ uint32_t s_global[kRadix];             // output start of each digit
+  __shared__ uint32_t s_tmp[kWaves];
+  __shared__ uint32_t s_key[kTile];
+  __shared__ uint32_t s_row[kTile];
+  __shared__ VB s_val[kTile];
   const int lane = threadIdx.x % kWave;
   const int wave = threadIdx.x / kWave;
   for (int d = threadIdx.x; d < kWaves * kRadix; d += kBlock)
     (&s_count[0][0])[d] = 0;
   __syncthreads();
   const int64_t tile = blockIdx.x;
-  const int64_t base = tile * kTile + static_cast<int64_t>(wave) * kWaveItems;
+  const int64_t tile_base = tile * kTile;
+  const int64_t base = tile_base + static_cast<int64_t>(wave) * kWaveItems;
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
   uint32_t key[kIpt];
@@ -236,27 +247,44 @@ __global__ __launch_bounds__(kBlock) void radix_scatter_kernel(
     if (valid && lower == total - 1) s_count[wave][d] += total;
   }
   __syncthreads();
-  // Per-digit base of this wave: global tile offset + earlier waves' counts.
-  __shared__ uint32_t s_base[kWaves][kRadix];
   {
     const int d = threadIdx.x;  // kBlock == kRadix
-    uint32_t run = offsets[static_cast<int64_t>(d) * ntiles + tile];
+    uint32_t run = 0;
 #pragma unroll
     for (int w = 0; w < kWaves; ++w) {
-      s_base[w][d] = run;
+      s_wave_off[w][d] = run;
       run += s_count[w][d];
     }
+    uint32_t all;
+    s_local[d] = block_exclusive_scan(run, s_tmp, &all);
+    s_global[d] = offsets[static_cast<int64_t>(d) * ntiles + tile];
   }
   __syncthreads();
+  // Stage the tile in digit order.
 #pragma unroll
   for (int u = 0; u < kIpt; ++u) {
     const int64_t i = base + u * kWave + lane;
     if (i < n) {
       const uint32_t d = (key[u] >> shift) & (kRadix - 1);
-      const uint32_t dst = s_base[wave][d] + rank[u];
-      keys_out[dst] = key[u];
-      rows_out[dst] = static_cast<RO>(ld_stream(rows + i * row_stride));
-      vals_out[dst] = ld_stream(vals + i);
+      const uint32_t pos = s_local[d] + s_wave_off[wave][d] + rank[u];
+      s_key[pos] = key[u];
+      s_row[pos] = static_cast<uint32_t>(ld_stream(rows + i * row_stride));
+      s_val[pos] = ld_stream(vals + i);
+    }
+  }
+  __syncthreads();
+  // Consecutive threads write consecutive entries of each digit's run.
+  const int64_t valid_in_tile = n - tile_base < kTile ? n - tile_base : kTile;
+#pragma unroll
+  for (int u = 0; u < kIpt; ++u) {
+    const int t = u * kBlock + threadIdx.x;
+    if (t < valid_in_tile) {
+      const uint32_t k = s_key[t];
+      const uint32_t d = (k >> shift) & (kRadix - 1);
+      const uint32_t dst = s_global[d] + (static_cast<uint32_t>(t) - s_local[d]);
+      keys_out[dst] = k;
+      rows_out[dst] = static_cast<RO>(s_row[t]);
+      vals_out[dst] = s_val[t];
     }
   }
 }

@@ -16,6 +16,7 @@ namespace asc_hip {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kUnroll = 8;
 
 struct MParams {
   const int64_t* offsets;
@@ -49,7 +50,24 @@ __global__ __launch_bounds__(kBlock) void spmm_kernel(MParams p) {
   for (int64_t j = lane; j < p.k; j += G) {
     const int64_t cpos = p.c_row_major ? row * p.ldc + j : j * p.ldc + row;
     T acc = c[cpos];
-    for (int64_t e = begin; e < end; ++e) {
+    int64_t e = begin;
+    // kUnroll B gathers in flight per lane; the additions stay in stored
+    // order, so the result is unchanged.
+    for (; e + kUnroll <= end; e += kUnroll) {
+      T av[kUnroll], bv[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        const int64_t src = static_cast<int64_t>(col[e + u]);
+        av[u] = alpha * val[e + u];
+        bv[u] = p.b_row_major ? b[src * p.ldb + j] : b[j * p.ldb + src];
+      }
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        const T term = av[u] * bv[u];
+        acc = acc + term;
+      }
+    }
+    for (; e < end; ++e) {
       const int64_t src = static_cast<int64_t>(col[e]);
       const T av = alpha * val[e];
       const T bv = p.b_row_major ? b[src * p.ldb + j] : b[j * p.ldb + src];

@@ -289,6 +289,38 @@ def bench_precision(ctx, extra):
     extra["precision_n2^20_32nnz"] = out
 
 
+def bench_finalize(ctx, extra, n=1 << 26, dim=1 << 22):
+    """COO finalize (CoordinateBuilder::Finalize on the device): n raw
+    triplets over a dim x dim shape in random order, 1/8 of them repeating an
+    earlier coordinate, FP64, duplicates summed, zeros kept.  Algorithmic
+    bytes: triplets read (16 + 8 per entry) and the canonical result written
+    (16 + 8 per surviving entry)."""
+    rng = np.random.default_rng(0x5EEDA5C5)
+    r = rng.integers(0, dim, n)
+    c = rng.integers(0, dim, n)
+    rep = rng.random(n) < 0.125
+    src = rng.integers(0, n, n)
+    r = np.where(rep, r[src], r)
+    c = np.where(rep, c[src], c)
+    coords = np.stack([r, c], 1).reshape(-1)
+    vals = rng.standard_normal(n)
+    del r, c, rep, src
+    t = H.triplets(ctx, dim, dim, coords, vals, H.F64)
+    del coords, vals
+    ws = H.Workspace(ctx)
+    holder = {}
+
+    def fin():
+        holder["out"] = H.coo_finalize(ctx, t, H.DUPLICATE_SUM, H.ZERO_KEEP, ws)
+    res = time_ops(ctx, fin, 3, 1)
+    kept = int(holder["out"].desc.nnz)
+    by = n * 24 + kept * 24
+    extra["coo_finalize"] = {"entries": n, "unique": kept, "ms": round(res["median_ms"], 3),
+                             "gbs": round(by / res["median_ms"] / 1e6, 1),
+                             "entries_per_s": round(n / (res["median_ms"] * 1e-3))}
+    log(f"COO finalize of {n} triplets ({kept} unique): {res['median_ms']:.2f} ms")
+
+
 def bench_batched(ctx, extra, count):
     t0 = time.perf_counter()
     specs = G.batched_specs(count)
@@ -356,6 +388,7 @@ def main():
     del m
     if not args.quick:
         bench_precision(ctx, extra)
+        bench_finalize(ctx, extra)
         bench_batched(ctx, extra, args.batch_count)
     traffic, traffic_src = measured_traffic(rows, nnz, name)
     out = {

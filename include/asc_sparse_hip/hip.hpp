@@ -451,6 +451,8 @@ class HipCoordinateArray {
   }
   [[nodiscard]] void* coords_data() const noexcept { return coords_.data(); }
   [[nodiscard]] void* values_data() const noexcept { return values_.data(); }
+  // The entry count after a finalize that merged or dropped entries.
+  void set_count(nnz_t nnz) noexcept { nnz_ = nnz; }
 
  private:
   HipBuffer coords_, values_;
@@ -797,6 +799,45 @@ template <typename Element, SparseCompressedFormat Format>
   auto done = context.Record();
   if (!done.ok()) return done.status();
   return HipCoordinateClone<Element>{std::move(*out), *done};
+}
+
+// Device CoordinateBuilder::Finalize (coordinate.h:1300-1497): raw device
+// triplets (interleaved int64 (row, col) in insertion order, values) to a
+// canonical coordinate array.  DuplicatePolicy / ExplicitZeroPolicy values
+// follow coordinate.h:60-78 (ASC_HIP_DUPLICATE_*, ASC_HIP_ZERO_*).  Waits for
+// the device; the array's view() has the finalized count.
+template <typename Element>
+  requires internal_sparse_hip::kSupportedElement<Element>
+[[nodiscard]] Result<HipCoordinateArray<Element>> HipFinalizeCoordinates(
+    SparseHipContext& context, const index_t* device_coordinates, const Element* device_values,
+    extent_t rows, extent_t cols, nnz_t count, std::uint8_t duplicate_policy,
+    std::uint8_t zero_policy, const HipMemoryResource& resource) {
+  asc_hip_sparse_desc t{};
+  t.outer_offsets = device_coordinates;
+  t.values = device_values;
+  t.rows = rows;
+  t.cols = cols;
+  t.nnz = count;
+  t.format = ASC_HIP_COO;
+  t.element_kind = internal_sparse_hip::ElementKind<Element>();
+  t.index_bytes = 8;
+  t.offset_bytes = 8;
+  t.memory_space = ASC_HIP_DEVICE;
+  std::size_t bytes = 0;
+  Status s = internal_sparse_hip::ToStatus(
+      asc_sparse_hip_coo_finalize_workspace_size(context.native(), &t, &bytes));
+  if (!s.ok()) return s;
+  auto ws = HipBuffer::Allocate(resource, bytes);
+  if (!ws.ok()) return ws.status();
+  auto out = HipCoordinateArray<Element>::Allocate(resource, rows, cols, count);
+  if (!out.ok()) return out.status();
+  asc_hip_sparse_desc d{};
+  s = internal_sparse_hip::ToStatus(asc_sparse_hip_coo_finalize(
+      context.native(), &t, duplicate_policy, zero_policy, out->coords_data(), out->values_data(),
+      &d, ws->data(), ws->size()));
+  if (!s.ok()) return s;
+  out->set_count(d.nnz);
+  return std::move(*out);
 }
 
 // Stream-ordered copies between host and device memory (CopyBytes,

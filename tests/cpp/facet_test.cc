@@ -376,6 +376,42 @@ void TestColumnPanel(Gpu& g) {
   CHECK(asc_sparse_hip_plan_clear(g.ctx.native()).code == ASC_HIP_OK);
 }
 
+// coordinate_test.cc:96-142: triplets (2,1)=5, (0,3)=2, (2,1)=-1, (1,0)=0,
+// (0,1)=7 finalize (kSum, kKeep) to (0,1)=7, (0,3)=2, (1,0)=0, (2,1)=4;
+// kReject refuses them; (kSum, kDrop) keeps 3 entries.
+void TestFinalize(Gpu& g) {
+  Case("coordinate finalize golden (double)");
+  auto cb = Upload<asc::index_t>(g, {2, 1, 0, 3, 2, 1, 1, 0, 0, 1});
+  auto vb = Upload<double>(g, {5, 2, -1, 0, 7});
+  const auto* c = static_cast<const asc::index_t*>(cb.data());
+  const auto* v = static_cast<const double*>(vb.data());
+  auto sum = asc::HipFinalizeCoordinates(g.ctx, c, v, 3, 4, 5, ASC_HIP_DUPLICATE_SUM,
+                                         ASC_HIP_ZERO_KEEP, g.device);
+  CHECK(sum.ok());
+  if (sum.ok()) {
+    auto view = sum->view();
+    CHECK(view.ok() && view->nnz() == 4);
+    std::vector<asc::index_t> pairs(8);
+    std::vector<double> vals(4);
+    CHECK(Copy(g, pairs.data(), view->coordinates(), 8 * 8));
+    CHECK(Copy(g, vals.data(), view->values(), 4 * 8));
+    CHECK((pairs == std::vector<asc::index_t>{0, 1, 0, 3, 1, 0, 2, 1}));
+    CHECK((vals == std::vector<double>{7, 2, 0, 4}));
+  }
+  CHECK_CODE(asc::HipFinalizeCoordinates(g.ctx, c, v, 3, 4, 5, ASC_HIP_DUPLICATE_REJECT,
+                                         ASC_HIP_ZERO_KEEP, g.device)
+                 .status(),
+             ErrorCode::kInvalidArgument);
+  auto drop = asc::HipFinalizeCoordinates(g.ctx, c, v, 3, 4, 5, ASC_HIP_DUPLICATE_SUM,
+                                          ASC_HIP_ZERO_DROP, g.device);
+  CHECK(drop.ok() && drop->view()->nnz() == 3);
+  // A coordinate outside the shape.
+  CHECK_CODE(asc::HipFinalizeCoordinates(g.ctx, c, v, 2, 4, 5, ASC_HIP_DUPLICATE_SUM,
+                                         ASC_HIP_ZERO_KEEP, g.device)
+                 .status(),
+             ErrorCode::kIndex);
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -404,6 +440,7 @@ int main(int argc, char** argv) {
     TestConversions<double>(g);
     TestConversions<float>(g);
     TestColumnPanel(g);
+    TestFinalize(g);
   }
   std::printf("%d checks, %d failures\n", g_checks, g_failures);
   return g_failures == 0 ? 0 : 1;
