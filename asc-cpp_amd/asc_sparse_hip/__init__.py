@@ -769,3 +769,16 @@ def coo_finalize(ctx: Context, t: DeviceSparse, duplicate_policy: int, zero_poli
     check(ctx.lib.asc_sparse_hip_coo_finalize(ctx.handle, C.byref(d), duplicate_policy, zero_policy,
                                               bc.ptr, bv.ptr, C.byref(out), p, n))
     return DeviceSparse(out, [bc, bv])
+
+
+_SIGS6 = {"asc_sparse_hip_context_set_spmm_panel_bytes": [_VP, C.c_int64]}
+_SIGS.update(_SIGS6)
+if _lib is not None:
+    for _n, _a in _SIGS6.items():
+        getattr(_lib, _n).argtypes = _a
+        getattr(_lib, _n).restype = Status
+
+
+def set_spmm_panel_bytes(ctx: Context, nbytes: int):
+    """Bytes of B per SpMM column pass (0: automatic, < 0: one pass)."""
+    check(ctx.lib.asc_sparse_hip_context_set_spmm_panel_bytes(ctx.handle, nbytes))

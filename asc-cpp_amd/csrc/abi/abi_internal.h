@@ -20,6 +20,8 @@ struct asc_hip_context {
   hipStream_t stream = nullptr;
   bool owns_stream = false;
   uint8_t determinism = ASC_HIP_DETERMINISTIC;
+  // Bytes of B per SpMM column pass; <= 0: one pass.
+  int64_t spmm_panel_bytes = 0;
   std::mutex plan_mutex;  // guards the tuned-plan cache below
   struct Plan {
     const void* offsets;

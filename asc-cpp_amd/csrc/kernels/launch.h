@@ -97,8 +97,13 @@ struct SpmmArgs {
   bool c_row_major;
   int64_t k;  // right-hand sides
   double alpha;
+  // Columns of op(A) (rows of B) per pass; 0 or >= inner: one pass.  Each
+  // pass gathers only from its slice of B, which stays memory-side cached.
+  int64_t panel_cols = 0;
 };
-hipError_t spmm_launch(const SpmmArgs& args, hipStream_t stream);
+// Workspace: the passes' split table (0 bytes for one pass).
+size_t spmm_workspace_bytes(const SpmmArgs& args);
+hipError_t spmm_launch(const SpmmArgs& args, void* workspace, hipStream_t stream);
 
 // ---- batched SpMV ----------------------------------------------------------
 // Per-matrix table entry and work item of the persistent batched kernel

@@ -31,7 +31,45 @@ struct MParams {
   int c_row_major;
   int64_t k;
   double alpha;
+  // Column passes: splits[(q-1)*rows + r] is the position, relative to the
+  // start of row r, of its first entry in pass q's column range.
+  const uint32_t* splits;
+  int64_t passes;
+  int64_t pass;
 };
+
+// The row's entries in this pass; with one pass, the whole row.
+__device__ __forceinline__ void pass_range(const MParams& p, int64_t row, int64_t* b,
+                                           int64_t* e) {
+  const int64_t rb = p.offsets[row];
+  *b = p.pass == 0 ? rb : rb + p.splits[(p.pass - 1) * p.rows + row];
+  *e = p.pass + 1 == p.passes ? p.offsets[row + 1] : rb + p.splits[p.pass * p.rows + row];
+}
+
+// Split table of the column passes: one thread per row, an advancing binary
+// search per pass boundary over the row's sorted columns.
+template <typename I>
+__global__ __launch_bounds__(kBlock) void spmm_split_kernel(
+    const int64_t* __restrict__ offsets, const I* __restrict__ col, int64_t rows,
+    int64_t width, int64_t passes, uint32_t* __restrict__ splits) {
+  const int64_t row = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (row >= rows) return;
+  const int64_t b = offsets[row];
+  const int64_t e = offsets[row + 1];
+  int64_t lo = b;
+  for (int64_t q = 1; q < passes; ++q) {
+    const int64_t bound = q * width;
+    int64_t hi = e;
+    while (lo < hi) {
+      const int64_t mid = lo + ((hi - lo) >> 1);
+      if (static_cast<int64_t>(col[mid]) < bound)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    splits[(q - 1) * rows + row] = static_cast<uint32_t>(lo - b);
+  }
+}
 
 template <int G, typename T, typename I>
 __global__ __launch_bounds__(kBlock) void spmm_kernel(MParams p) {
@@ -45,8 +83,11 @@ __global__ __launch_bounds__(kBlock) void spmm_kernel(MParams p) {
   const T* __restrict__ b = static_cast<const T*>(p.b);
   T* c = static_cast<T*>(p.c);
   const T alpha = static_cast<T>(p.alpha);
-  const int64_t begin = p.offsets[row];
-  const int64_t end = p.offsets[row + 1];
+  int64_t begin, end;
+  pass_range(p, row, &begin, &end);
+  // Rows with no entry in this pass leave C untouched.  Passes run in column
+  // order, so every C(i,j) still receives its terms in stored order.
+  if (p.passes > 1 && begin == end) return;  // uniform over the G-group
   for (int64_t j = lane; j < p.k; j += G) {
     const int64_t cpos = p.c_row_major ? row * p.ldc + j : j * p.ldc + row;
     T acc = c[cpos];
@@ -78,8 +119,86 @@ __global__ __launch_bounds__(kBlock) void spmm_kernel(MParams p) {
   }
 }
 
+// k <= 32: one wave per output row.  Lanes 0-31 hold C(i, j = lane) and take
+// the row's even-numbered entries, lanes 32-63 the odd ones, so one wave
+// instruction gathers two B rows.  Each batch's column indices and values
+// are loaded while the previous batch's B rows are in flight.  The odd
+// lanes' products move to the even lanes by a cross-lane shuffle and every
+// C(i,j) adds its terms strictly in stored order: the result is the serial
+// reference's, bit for bit.
+template <typename T, typename I>
+__global__ __launch_bounds__(kBlock) void spmm_pair_kernel(MParams p) {
+#pragma clang fp contract(off)
+  constexpr int kRowsPerBlock = kBlock / kWave;
+  constexpr int U = kUnroll;  // entries per lane per batch (2U per wave)
+  const int64_t blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t row = blk * kRowsPerBlock + threadIdx.x / kWave;
+  if (row >= p.rows) return;  // whole wave
+  const int lane = threadIdx.x % kWave;
+  const int slot = lane >> 5;
+  const int64_t j = lane & 31;
+  const bool jvalid = j < p.k;
+  const T* __restrict__ val = static_cast<const T*>(p.values);
+  const I* __restrict__ col = static_cast<const I*>(p.indices);
+  const T* __restrict__ b = static_cast<const T*>(p.b);
+  T* c = static_cast<T*>(p.c);
+  const T alpha = static_cast<T>(p.alpha);
+  int64_t begin, end;
+  pass_range(p, row, &begin, &end);
+  if (p.passes > 1 && begin == end) return;  // uniform over the wave
+  const int64_t cpos = p.c_row_major ? row * p.ldc + j : j * p.ldc + row;
+  T acc = (slot == 0 && jvalid) ? c[cpos] : T(0);
+  int64_t cur_c[U];
+  T cur_v[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int64_t e = begin + 2 * u + slot;
+    cur_c[u] = e < end ? static_cast<int64_t>(col[e]) : 0;
+    cur_v[u] = e < end ? alpha * val[e] : T(0);
+  }
+  for (int64_t e0 = begin; e0 < end; e0 += 2 * U) {
+    T bv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t e = e0 + 2 * u + slot;
+      bv[u] = (e < end && jvalid)
+                  ? (p.b_row_major ? b[cur_c[u] * p.ldb + j] : b[j * p.ldb + cur_c[u]])
+                  : T(0);
+    }
+    int64_t nxt_c[U];
+    T nxt_v[U];
+    const int64_t e1 = e0 + 2 * U;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t e = e1 + 2 * u + slot;
+      nxt_c[u] = e < end ? static_cast<int64_t>(col[e]) : 0;
+      nxt_v[u] = e < end ? alpha * val[e] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const T term = cur_v[u] * bv[u];
+      const T odd = __shfl(term, static_cast<int>(j) + 32);
+      if (e0 + 2 * u < end) acc = acc + term;      // uniform over the wave
+      if (e0 + 2 * u + 1 < end) acc = acc + odd;  // uniform over the wave
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      cur_c[u] = nxt_c[u];
+      cur_v[u] = nxt_v[u];
+    }
+  }
+  if (slot == 0 && jvalid) c[cpos] = acc;
+}
+
 template <typename T, typename I>
 hipError_t launch_typed(const MParams& p, hipStream_t s) {
+  if (p.k <= 32) {
+    constexpr int rows_per_block = kBlock / kWave;
+    const unsigned grid =
+        static_cast<unsigned>((p.rows + rows_per_block - 1) / rows_per_block);
+    spmm_pair_kernel<T, I><<<grid, kBlock, 0, s>>>(p);
+    return hipGetLastError();
+  }
   int g = 1;
   while (g < p.k && g < 64) g <<= 1;
   auto grid = [&](int G) {
@@ -99,7 +218,31 @@ hipError_t launch_typed(const MParams& p, hipStream_t s) {
 
 }  // namespace
 
-hipError_t spmm_launch(const SpmmArgs& a, hipStream_t s) {
+namespace {
+
+// Column passes and the bytes of their split table (0 for one pass).
+int64_t pass_width(const SpmmArgs& a) {
+  const int64_t inner = a.a.inner;
+  if (a.panel_cols <= 0 || a.panel_cols >= inner) return inner;
+  if (a.a.nnz > static_cast<int64_t>(UINT32_MAX)) return inner;  // 32-bit split positions
+  return a.panel_cols;
+}
+
+int64_t pass_count(const SpmmArgs& a) {
+  const int64_t w = pass_width(a);
+  return w > 0 && w < a.a.inner ? (a.a.inner - 1) / w + 1 : 1;
+}
+
+}  // namespace
+
+size_t spmm_workspace_bytes(const SpmmArgs& a) {
+  const int64_t passes = pass_count(a);
+  if (passes <= 1) return 0;
+  return (sizeof(uint32_t) * static_cast<size_t>(passes - 1) * static_cast<size_t>(a.a.outer) +
+          255) & ~size_t(255);
+}
+
+hipError_t spmm_launch(const SpmmArgs& a, void* workspace, hipStream_t s) {
   if (a.a.outer == 0 || a.k == 0) return hipSuccess;
   MParams p;
   p.offsets = a.a.offsets;
@@ -114,12 +257,31 @@ hipError_t spmm_launch(const SpmmArgs& a, hipStream_t s) {
   p.c_row_major = a.c_row_major;
   p.k = a.k;
   p.alpha = a.alpha;
+  p.passes = pass_count(a);
+  p.splits = static_cast<const uint32_t*>(workspace);
   const bool i32 = a.a.index_bytes == 4;
-  if (a.a.elem == Elem::F64)
-    return i32 ? launch_typed<double, int32_t>(p, s) : launch_typed<double, int64_t>(p, s);
-  if (a.a.elem == Elem::F32)
-    return i32 ? launch_typed<float, int32_t>(p, s) : launch_typed<float, int64_t>(p, s);
-  return hipErrorInvalidValue;
+  if (p.passes > 1) {
+    const unsigned blocks = static_cast<unsigned>((p.rows + kBlock - 1) / kBlock);
+    uint32_t* splits = static_cast<uint32_t*>(workspace);
+    if (i32)
+      spmm_split_kernel<int32_t><<<blocks, kBlock, 0, s>>>(
+          p.offsets, static_cast<const int32_t*>(p.indices), p.rows, pass_width(a), p.passes, splits);
+    else
+      spmm_split_kernel<int64_t><<<blocks, kBlock, 0, s>>>(
+          p.offsets, static_cast<const int64_t*>(p.indices), p.rows, pass_width(a), p.passes, splits);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  for (int64_t q = 0; q < p.passes; ++q) {
+    p.pass = q;
+    hipError_t e = hipErrorInvalidValue;
+    if (a.a.elem == Elem::F64)
+      e = i32 ? launch_typed<double, int32_t>(p, s) : launch_typed<double, int64_t>(p, s);
+    else if (a.a.elem == Elem::F32)
+      e = i32 ? launch_typed<float, int32_t>(p, s) : launch_typed<float, int64_t>(p, s);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 }  // namespace asc_hip

@@ -201,7 +201,8 @@ def bench_large(ctx, args, extra):
         ctx.synchronize()
         bd = H.dense(bmat, m.cols, k, k, H.ROW_MAJOR, H.F64, writable=False)
         cd = H.dense(cmat, m.rows, k, k, H.ROW_MAJOR, H.F64)
-        rs = time_ops(ctx, lambda: H.standard_spmm(ctx, H.OP_N, 1.0, a, bd, cd), 2, 1)
+        sws = H.Workspace(ctx)
+        rs = time_ops(ctx, lambda: H.standard_spmm(ctx, H.OP_N, 1.0, a, bd, cd, sws), 2, 1)
         bs = m.nnz * 12 + (m.rows + 1) * 8 + m.cols * k * 8 + 2 * m.rows * k * 8
         extra["spmm_k32_row_major"] = {"gbs": round(bs / rs["median_ms"] / 1e6, 1),
                                        "ms": round(rs["median_ms"], 3)}

@@ -368,6 +368,16 @@ asc_hip_status asc_sparse_hip_standard_spmm(
     const asc_hip_sparse_desc* matrix, const asc_hip_matrix_desc* b,
     const asc_hip_matrix_desc* c, void* workspace, size_t workspace_bytes);
 
+/* SpMM column passes (no reference counterpart): B is gathered one slice of
+ * `bytes` at a time (one pass over A per slice, in column order, so every
+ * C(i,j) still receives its terms in stored order and stays bit-exact), which
+ * keeps the slice in the memory-side cache.  0 (the default) or a negative
+ * value runs one pass.  The passes need a split table of (passes-1)*rows uint32
+ * in the SpMM workspace (asc_sparse_hip_standard_spmm_workspace_size counts
+ * it); a call given less workspace than that runs in one pass. */
+asc_hip_status asc_sparse_hip_context_set_spmm_panel_bytes(asc_hip_context* context,
+                                                           int64_t bytes);
+
 /* ---- conversions --------------------------------------------------------- */
 
 /*

@@ -4,6 +4,7 @@
 Ordered SpMV and every SpMM are bit-exact with the oracle; plain SpMV stays
 within 4*len*eps*|alpha|*(|A||x|)_i.  Golden vectors: standard_blas_test.cc.
 """
+import ctypes as C
 import json
 import os
 
@@ -143,4 +144,57 @@ def test_spmm_k32_row_major_large(ctx):
     got, (bb, cc, ldb, ldc) = run_spmm(ctx, H.OP_N, 1.0, a, b, c0, H.ROW_MAJOR)
     ref = O.std_spmm(0, 0, m.rows, m.cols, m.offsets, m.indices, m.values, 1.0, bb.copy(), 1, ldb, k,
                      cc.copy(), 1, ldc).reshape(m.rows, k)
+    np.testing.assert_array_equal(got.view(np.uint64), ref.view(np.uint64))
+
+
+@pytest.mark.parametrize("panel_bytes", [-1, 8 * 32 * 7, 8 * 32 * 1000, 1])
+@pytest.mark.parametrize("op", [H.OP_N, H.OP_T])
+def test_spmm_column_passes_bit_exact(ctx, panel_bytes, op):
+    """SpMM in column passes (slices of B) keeps every C(i,j)'s terms in
+    stored order: bit-exact with the oracle for any slice width, including
+    one column per pass and the transposed (workspace) form."""
+    m = G.make(G.ZIPF, 4000, 3000, seed=43, values=G.VALUES_NORMAL, zipf_max=900)
+    k = 32
+    nin, nout = (m.cols, m.rows) if op == H.OP_N else (m.rows, m.cols)
+    rng = np.random.default_rng(3)
+    b = rng.uniform(-1, 1, (nin, k))
+    c0 = rng.uniform(-1, 1, (nout, k))
+    a = device_matrix(ctx, H.CSR, m.rows, m.cols, m.offsets, m.indices, m.values, 4)
+    print(f"spmm passes panel_bytes {panel_bytes} op {op}", flush=True)
+    H.set_spmm_panel_bytes(ctx, panel_bytes)
+    try:
+        got, (bb, cc, ldb, ldc) = run_spmm(ctx, op, -0.75, a, b, c0, H.ROW_MAJOR)
+    finally:
+        H.set_spmm_panel_bytes(ctx, 0)
+    ref = O.std_spmm(0, op, m.rows, m.cols, m.offsets, m.indices.astype(np.int64), m.values, -0.75,
+                     bb.copy(), 1, ldb, k, cc.copy(), 1, ldc).reshape(nout, k)
+    np.testing.assert_array_equal(got.view(np.uint64), ref.view(np.uint64))
+
+
+def test_spmm_column_passes_without_workspace_run_one_pass(ctx):
+    """Column passes need a split table in the workspace; a call given none
+    runs in one pass, with the same (bit-exact) result."""
+    m = G.make(G.ZIPF, 2000, 1500, seed=44, values=G.VALUES_NORMAL, zipf_max=600)
+    k = 32
+    rng = np.random.default_rng(4)
+    b = rng.uniform(-1, 1, (m.cols, k))
+    c0 = rng.uniform(-1, 1, (m.rows, k))
+    a = device_matrix(ctx, H.CSR, m.rows, m.cols, m.offsets, m.indices, m.values, 4)
+    bb, cb = ctx.upload(b), ctx.upload(c0)
+    bd = H.dense(bb, m.cols, k, k, H.ROW_MAJOR, H.F64, writable=False)
+    cd = H.dense(cb, m.rows, k, k, H.ROW_MAJOR, H.F64)
+    H.set_spmm_panel_bytes(ctx, 8 * k * 100)
+    try:
+        need = C.c_size_t()
+        H.check(ctx.lib.asc_sparse_hip_standard_spmm_workspace_size(
+            ctx.handle, H.OP_N, C.byref(a.desc), C.byref(bd), C.byref(cd), C.byref(need)))
+        assert need.value > 0  # the passes' split table
+        H.check(ctx.lib.asc_sparse_hip_standard_spmm(ctx.handle, H.OP_N, H.Scalar(1.25, 0.0),
+                                                     C.byref(a.desc), C.byref(bd), C.byref(cd),
+                                                     None, 0))
+    finally:
+        H.set_spmm_panel_bytes(ctx, 0)
+    got = ctx.download(cb, np.float64, m.rows * k).reshape(m.rows, k)
+    ref = O.std_spmm(0, 0, m.rows, m.cols, m.offsets, m.indices.astype(np.int64), m.values, 1.25,
+                     b.copy(), 1, k, k, c0.copy(), 1, k).reshape(m.rows, k)
     np.testing.assert_array_equal(got.view(np.uint64), ref.view(np.uint64))
