@@ -17,6 +17,7 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kUnroll = 8;
+constexpr int kWideBatches = 4;
 
 struct MParams {
   const int64_t* offsets;
@@ -190,8 +191,120 @@ __global__ __launch_bounds__(kBlock) void spmm_pair_kernel(MParams p) {
   if (slot == 0 && jvalid) c[cpos] = acc;
 }
 
+// Row-major B, k <= 32 and a multiple of 4, 4-element aligned rows: one wave
+// per output row, eight entries per instruction.  Lane l takes entry slot
+// l / 8 and columns 4*(l % 8) .. +3, loading them as one 4-wide vector, so a
+// wave instruction gathers eight B rows.  kWideBatches batches of eight are
+// in flight, with the next batches' indices and values loaded meanwhile.
+// Each batch's products go through LDS to the slot-0 lanes, which own
+// C(i, 4q..4q+3) and add the terms in stored order: bit-exact with the
+// serial reference.  Waves of a block run different rows, so they
+// synchronise only within the wave.
+template <typename T, typename I>
+__global__ __launch_bounds__(kBlock) void spmm_wide_kernel(MParams p) {
+#pragma clang fp contract(off)
+  typedef T v4 __attribute__((ext_vector_type(4)));
+  constexpr int kRowsPerBlock = kBlock / kWave;
+  constexpr int U = kWideBatches;
+  __shared__ T s_terms[kRowsPerBlock][8][32];
+  const int w = threadIdx.x / kWave;
+  const int64_t blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int64_t row = blk * kRowsPerBlock + w;
+  if (row >= p.rows) return;  // whole wave
+  const int lane = threadIdx.x % kWave;
+  const int g = lane >> 3;
+  const int64_t jq = 4 * (lane & 7);
+  const bool qvalid = jq < p.k;
+  const T* __restrict__ val = static_cast<const T*>(p.values);
+  const I* __restrict__ col = static_cast<const I*>(p.indices);
+  const T* __restrict__ b = static_cast<const T*>(p.b);
+  T* c = static_cast<T*>(p.c);
+  const T alpha = static_cast<T>(p.alpha);
+  int64_t begin, end;
+  pass_range(p, row, &begin, &end);
+  if (p.passes > 1 && begin == end) return;  // uniform over the wave
+  const bool owner = g == 0 && qvalid;
+  auto cpos = [&](int64_t j) { return p.c_row_major ? row * p.ldc + j : j * p.ldc + row; };
+  T acc[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) acc[i] = owner ? c[cpos(jq + i)] : T(0);
+  int64_t cur_c[U];
+  T cur_v[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int64_t e = begin + 8 * u + g;
+    cur_c[u] = e < end ? static_cast<int64_t>(col[e]) : 0;
+    cur_v[u] = e < end ? alpha * val[e] : T(0);
+  }
+  for (int64_t e0 = begin; e0 < end; e0 += 8 * U) {
+    v4 bv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t e = e0 + 8 * u + g;
+      if (e < end && qvalid)
+        bv[u] = *reinterpret_cast<const v4*>(b + cur_c[u] * p.ldb + jq);
+      else
+        bv[u] = v4{T(0), T(0), T(0), T(0)};
+    }
+    int64_t nxt_c[U];
+    T nxt_v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t e = e0 + 8 * U + 8 * u + g;
+      nxt_c[u] = e < end ? static_cast<int64_t>(col[e]) : 0;
+      nxt_v[u] = e < end ? alpha * val[e] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const T t0 = cur_v[u] * bv[u].x;
+      const T t1 = cur_v[u] * bv[u].y;
+      const T t2 = cur_v[u] * bv[u].z;
+      const T t3 = cur_v[u] * bv[u].w;
+      T* slot = &s_terms[w][g][jq];
+      slot[0] = t0;
+      slot[1] = t1;
+      slot[2] = t2;
+      slot[3] = t3;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (owner) {
+        const int64_t eb = e0 + 8 * u;
+#pragma unroll
+        for (int sl = 0; sl < 8; ++sl) {
+          if (eb + sl < end) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) acc[i] = acc[i] + s_terms[w][sl][jq + i];
+          }
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      cur_c[u] = nxt_c[u];
+      cur_v[u] = nxt_v[u];
+    }
+  }
+  if (owner) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) c[cpos(jq + i)] = acc[i];
+  }
+}
+
 template <typename T, typename I>
 hipError_t launch_typed(const MParams& p, hipStream_t s) {
+  const uintptr_t vec_bytes = 4 * sizeof(T);
+  if (p.k <= 32 && p.k % 4 == 0 && p.b_row_major && p.ldb % 4 == 0 &&
+      reinterpret_cast<uintptr_t>(p.b) % vec_bytes == 0) {
+    constexpr int rows_per_block = kBlock / kWave;
+    const unsigned grid =
+        static_cast<unsigned>((p.rows + rows_per_block - 1) / rows_per_block);
+    spmm_wide_kernel<T, I><<<grid, kBlock, 0, s>>>(p);
+    return hipGetLastError();
+  }
   if (p.k <= 32) {
     constexpr int rows_per_block = kBlock / kWave;
     const unsigned grid =
