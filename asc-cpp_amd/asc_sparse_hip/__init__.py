@@ -782,3 +782,53 @@ if _lib is not None:
 def set_spmm_panel_bytes(ctx: Context, nbytes: int):
     """Bytes of B per SpMM column pass (0: automatic, < 0: one pass)."""
     check(ctx.lib.asc_sparse_hip_context_set_spmm_panel_bytes(ctx.handle, nbytes))
+
+
+LEVEL1_DOT, LEVEL1_AXPY, LEVEL1_GATHER, LEVEL1_GATHER_ZERO, LEVEL1_SCATTER = 0, 1, 2, 3, 4
+UNCONJUGATED, CONJUGATED = 0, 1
+
+
+class IndexedDesc(C.Structure):
+    _fields_ = [("indices", C.c_void_p), ("values", C.c_void_p), ("nnz", C.c_int64),
+                ("dense_extent", C.c_int64), ("element_kind", C.c_uint8), ("memory_space", C.c_uint8),
+                ("writable", C.c_uint8), ("trusted", C.c_uint8), ("reserved", C.c_uint8 * 4)]
+
+
+_SIGS7 = {
+    "asc_sparse_hip_clone_indexed": [_VP, P(IndexedDesc), _VP, _VP, P(IndexedDesc)],
+    "asc_sparse_hip_standard_level1": [_VP, C.c_uint8, C.c_uint8, Scalar, P(IndexedDesc),
+                                       P(VectorDesc), P(VectorDesc), P(SpmvOptions)],
+}
+_SIGS.update(_SIGS7)
+if _lib is not None:
+    for _n, _a in _SIGS7.items():
+        getattr(_lib, _n).argtypes = _a
+        getattr(_lib, _n).restype = Status
+
+
+class DeviceIndexed:
+    """A trusted device indexed vector and the buffers that own it."""
+
+    def __init__(self, desc: IndexedDesc, buffers: list):
+        self.desc, self.buffers = desc, buffers
+
+
+def clone_indexed(ctx: Context, indices: np.ndarray, values: np.ndarray, dense_extent: int,
+                  kind: int) -> DeviceIndexed:
+    idx = np.ascontiguousarray(indices, dtype=np.int64)
+    vals = np.ascontiguousarray(values, dtype=NP_STORAGE[kind])
+    src = IndexedDesc(idx.ctypes.data, vals.ctypes.data, idx.shape[0], dense_extent, kind, HOST, 0, 0)
+    bi = ctx.alloc(max(8 * idx.shape[0], 8))
+    bv = ctx.alloc(max(vals.nbytes, 8))
+    out = IndexedDesc()
+    check(ctx.lib.asc_sparse_hip_clone_indexed(ctx.handle, C.byref(src), bi.ptr, bv.ptr, C.byref(out)))
+    return DeviceIndexed(out, [bi, bv])
+
+
+def level1(ctx: Context, op: int, sparse: DeviceIndexed, dense: VectorDesc, alpha: float = 1.0,
+           result: VectorDesc | None = None, alg: int = ALG_AUTO, conjugation: int = UNCONJUGATED):
+    opt = options(alg)
+    check(ctx.lib.asc_sparse_hip_standard_level1(ctx.handle, op, conjugation, Scalar(alpha, 0.0),
+                                                 C.byref(sparse.desc), C.byref(dense),
+                                                 C.byref(result) if result is not None else None,
+                                                 C.byref(opt)))

@@ -201,6 +201,14 @@ hipError_t convert_values_launch(const double* src, void* dst, Elem to,
 hipError_t fill_launch(void* dst, uint8_t byte, size_t bytes,
                        hipStream_t stream);
 
+// Sparse Level-1 over a canonical indexed vector: op 0 dot (into `result`,
+// one element; `ordered` = serial left-to-right), 1 axpy (x += alpha*v at the
+// indices), 2 gather (v = x at the indices), 3 gather-zero (and zero x
+// there), 4 scatter (x = v at the indices).  Dense element i is dense[i*inc].
+hipError_t level1_launch(int op, bool ordered, Elem elem, double alpha, const int64_t* indices,
+                         void* values, void* dense, int64_t inc, int64_t nnz, void* result,
+                         hipStream_t stream);
+
 size_t elem_bytes(Elem e);
 
 }  // namespace asc_hip

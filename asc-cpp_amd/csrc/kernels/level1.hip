@@ -1,0 +1,105 @@
+// Sparse Level-1 kernels for gfx950 (blas.h:1330-1582) over a canonical
+// indexed vector (strictly increasing indices, so every dense element is
+// touched at most once) and a strided dense vector (element i at x[i*inc]).
+//
+//   dot (plain)    one 1024-thread block: thread t sums entries t, t+1024, ...
+//                  in order, then a fixed-shape tree: deterministic run to run.
+//   dot (ordered)  one lane, left to right from +0 without FMA: the serial
+//                  reference's arithmetic bit for bit.
+//   axpy / gather / gather-zero / scatter   one thread per entry; each dense
+//                  element is updated once, so these are exact.
+#include "device_common.h"
+#include "launch.h"
+
+namespace asc_hip {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kDotBlock = 1024;
+
+template <typename T>
+__global__ __launch_bounds__(kDotBlock) void dot_plain_kernel(
+    const int64_t* __restrict__ idx, const T* __restrict__ v, const T* __restrict__ x,
+    int64_t inc, int64_t nnz, T* __restrict__ out) {
+  __shared__ T s_part[kDotBlock];
+  T sum = T(0);
+  for (int64_t p = threadIdx.x; p < nnz; p += kDotBlock) sum += v[p] * x[idx[p] * inc];
+  s_part[threadIdx.x] = sum;
+  __syncthreads();
+  for (int half = kDotBlock / 2; half > 0; half >>= 1) {
+    if (static_cast<int>(threadIdx.x) < half) s_part[threadIdx.x] += s_part[threadIdx.x + half];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = s_part[0];
+}
+
+template <typename T>
+__global__ void dot_ordered_kernel(const int64_t* __restrict__ idx, const T* __restrict__ v,
+                                   const T* __restrict__ x, int64_t inc, int64_t nnz,
+                                   T* __restrict__ out) {
+#pragma clang fp contract(off)
+  if (threadIdx.x != 0) return;
+  T r = T(0);
+  for (int64_t p = 0; p < nnz; ++p) {
+    const T term = v[p] * x[idx[p] * inc];
+    r = r + term;
+  }
+  *out = r;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void level1_kernel(int op, T alpha,
+                                                       const int64_t* __restrict__ idx,
+                                                       T* __restrict__ v, T* __restrict__ x,
+                                                       int64_t inc, int64_t nnz) {
+#pragma clang fp contract(off)
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
+  for (int64_t p = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; p < nnz; p += stride) {
+    T* xp = x + idx[p] * inc;
+    switch (op) {
+      case 1: {  // axpy
+        const T scaled = alpha * v[p];
+        *xp = *xp + scaled;
+        break;
+      }
+      case 2: v[p] = *xp; break;  // gather
+      case 3:                     // gather-zero
+        v[p] = *xp;
+        *xp = T(0);
+        break;
+      default: *xp = v[p]; break;  // scatter
+    }
+  }
+}
+
+template <typename T>
+hipError_t level1_typed(int op, bool ordered, double alpha, const int64_t* idx, void* v, void* x,
+                        int64_t inc, int64_t nnz, void* result, hipStream_t s) {
+  T* tv = static_cast<T*>(v);
+  T* tx = static_cast<T*>(x);
+  if (op == 0) {
+    if (ordered)
+      dot_ordered_kernel<T><<<1, 64, 0, s>>>(idx, tv, tx, inc, nnz, static_cast<T*>(result));
+    else
+      dot_plain_kernel<T><<<1, kDotBlock, 0, s>>>(idx, tv, tx, inc, nnz, static_cast<T*>(result));
+    return hipGetLastError();
+  }
+  if (nnz == 0) return hipSuccess;
+  int64_t blocks = (nnz + kBlock - 1) / kBlock;
+  if (blocks > 65536) blocks = 65536;
+  level1_kernel<T><<<static_cast<unsigned>(blocks), kBlock, 0, s>>>(op, static_cast<T>(alpha), idx,
+                                                                     tv, tx, inc, nnz);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t level1_launch(int op, bool ordered, Elem elem, double alpha, const int64_t* indices,
+                         void* values, void* dense, int64_t inc, int64_t nnz, void* result,
+                         hipStream_t s) {
+  if (elem == Elem::F64)
+    return level1_typed<double>(op, ordered, alpha, indices, values, dense, inc, nnz, result, s);
+  return level1_typed<float>(op, ordered, alpha, indices, values, dense, inc, nnz, result, s);
+}
+
+}  // namespace asc_hip

@@ -351,6 +351,65 @@ asc_hip_status asc_sparse_hip_standard_spmv(
     const asc_hip_vector_desc* y, void* workspace, size_t workspace_bytes,
     const asc_hip_spmv_options* options);
 
+/* ---- sparse Level-1 ------------------------------------------------------ */
+
+/* Indexed (sparse) vector, after IndexedVectorDescriptor
+ * (providers/cuda.h:175-184) and SparseBlasIndexedVectorView
+ * (blas.h:526-789): `nnz` int64 indices, strictly increasing in
+ * [0, dense_extent), and `nnz` values. */
+typedef struct asc_hip_indexed_desc {
+  const int64_t* indices;
+  const void* values;
+  int64_t nnz;
+  int64_t dense_extent;
+  uint8_t element_kind; /* ASC_HIP_F32 / F64 */
+  uint8_t memory_space;
+  uint8_t writable;
+  uint8_t trusted; /* canonical structure vouched for by this library */
+  uint8_t reserved[4];
+} asc_hip_indexed_desc;
+
+/* internal_sparse_cuda::StandardOperation (providers/cuda.h:111-117). */
+enum {
+  ASC_HIP_LEVEL1_DOT = 0,
+  ASC_HIP_LEVEL1_AXPY = 1,
+  ASC_HIP_LEVEL1_GATHER = 2,
+  ASC_HIP_LEVEL1_GATHER_ZERO = 3,
+  ASC_HIP_LEVEL1_SCATTER = 4
+};
+/* asc::SparseBlasConjugation (blas.h:74-77); the identity for real data. */
+enum { ASC_HIP_UNCONJUGATED = 0, ASC_HIP_CONJUGATED = 1 };
+
+/* Copies a host indexed vector, checked canonical (blas.h:606-620:
+ * ASC_HIP_INDEX otherwise), into device buffers (nnz int64 indices, nnz
+ * values); `out` is trusted.  Replaces CloneIndexedErased
+ * (providers/cuda.h:216-218). */
+asc_hip_status asc_sparse_hip_clone_indexed(asc_hip_context* context,
+                                            const asc_hip_indexed_desc* host_source,
+                                            void* device_indices, void* device_values,
+                                            asc_hip_indexed_desc* out);
+
+/*
+ * SparseDot / SparseAxpy / SparseGather / SparseGatherZero / SparseScatter
+ * (blas.h:1330-1582) on a trusted device indexed vector and a device dense
+ * vector of extent dense_extent.  Replaces StandardLevel1Erased
+ * (providers/cuda.h:234-238).
+ *  - DOT writes sum(v[p] * x[idx[p]]) to `result` (one element, device).
+ *    `options->algorithm` ASC_HIP_ALG_ORDERED sums left to right from +0
+ *    without FMA (bit-exact with the serial reference); AUTO (or NULL
+ *    options) uses a deterministic parallel tree.
+ *  - AXPY: x[idx[p]] += alpha * v[p]; alpha == 0 returns without reading.
+ *  - GATHER: v[p] = x[idx[p]]; GATHER_ZERO also sets x[idx[p]] = 0.
+ *  - SCATTER: x[idx[p]] = v[p].
+ * Writable operands must not overlap the others.
+ */
+asc_hip_status asc_sparse_hip_standard_level1(asc_hip_context* context, uint8_t operation,
+                                              uint8_t conjugation, asc_hip_scalar alpha,
+                                              const asc_hip_indexed_desc* sparse,
+                                              const asc_hip_vector_desc* dense,
+                                              const asc_hip_vector_desc* result,
+                                              const asc_hip_spmv_options* options);
+
 /* ---- SpMM --------------------------------------------------------------- */
 
 /*

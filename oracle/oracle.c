@@ -411,3 +411,45 @@ static void merge_sort(int64_t* perm, int64_t* tmp, int64_t n) {
 
 ORACLE_FINALIZE_BODY(oracle_coo_finalize_f64, double)
 ORACLE_FINALIZE_BODY(oracle_coo_finalize_f32, float)
+
+/* ------------------------------------------------------------------------ */
+/* Sparse Level-1 (blas.h:1330-1582) over a canonical indexed vector        */
+/* (strictly increasing indices, blas.h:606-620) and a strided dense vector */
+/* (element i at dense[i*inc], inc > 0 here).  Real elements: conjugation  */
+/* is the identity.                                                         */
+/*   dot:         r = +0; r += v[p]*x[idx[p]] for p in order (:1360-1366)   */
+/*   axpy:        alpha == 0 returns; x[idx[p]] += alpha*v[p] (:1391-1398)  */
+/*   gather:      v[p] = x[idx[p]] (:1427-1430)                             */
+/*   gather-zero: v[p] = x[idx[p]]; x[idx[p]] = 0 (:1460-1465)              */
+/*   scatter:     x[idx[p]] = v[p] (:1494-1497)                             */
+/* ------------------------------------------------------------------------ */
+#define ORACLE_LEVEL1(SUFFIX, T)                                                 \
+  T oracle_sparse_dot_##SUFFIX(int64_t nnz, const int64_t* idx, const T* v,     \
+                               const T* x, int64_t inc) {                       \
+    T r = (T)0;                                                                  \
+    for (int64_t p = 0; p < nnz; ++p) r += v[p] * x[idx[p] * inc];               \
+    return r;                                                                    \
+  }                                                                              \
+  void oracle_sparse_axpy_##SUFFIX(int64_t nnz, T alpha, const int64_t* idx,    \
+                                   const T* v, T* x, int64_t inc) {             \
+    if (alpha == (T)0) return;                                                   \
+    for (int64_t p = 0; p < nnz; ++p) x[idx[p] * inc] += alpha * v[p];           \
+  }                                                                              \
+  void oracle_sparse_gather_##SUFFIX(int64_t nnz, const int64_t* idx, T* v,     \
+                                     const T* x, int64_t inc) {                 \
+    for (int64_t p = 0; p < nnz; ++p) v[p] = x[idx[p] * inc];                    \
+  }                                                                              \
+  void oracle_sparse_gather_zero_##SUFFIX(int64_t nnz, const int64_t* idx,      \
+                                          T* v, T* x, int64_t inc) {            \
+    for (int64_t p = 0; p < nnz; ++p) {                                          \
+      v[p] = x[idx[p] * inc];                                                    \
+      x[idx[p] * inc] = (T)0;                                                    \
+    }                                                                            \
+  }                                                                              \
+  void oracle_sparse_scatter_##SUFFIX(int64_t nnz, const int64_t* idx,          \
+                                      const T* v, T* x, int64_t inc) {          \
+    for (int64_t p = 0; p < nnz; ++p) x[idx[p] * inc] = v[p];                    \
+  }
+
+ORACLE_LEVEL1(f64, double)
+ORACLE_LEVEL1(f32, float)

@@ -29,6 +29,8 @@
  *   oracle_*_fast           O(nnz) counting-sort restatement; equal to the
  *                           faithful form on random inputs (the canonical
  *                           order is unique) and to oracle/_ref.
+ *   oracle_sparse_*         golden vectors of standard_blas_test.cc:69-174,
+ *                           exact integer brute force, and oracle/_ref.
  *   oracle_coo_finalize_*   golden vectors of coordinate_test.cc and a
  *                           Python sorted()-based brute force.
  */
@@ -158,6 +160,28 @@ int oracle_coo_finalize_f32(int64_t nnz, const int64_t* coords,
                             const float* values, int duplicate_policy,
                             int zero_policy, int64_t* out_coords,
                             float* out_values, int64_t* out_nnz);
+
+/* Sparse Level-1 over a canonical indexed vector (blas.h:1330-1582). */
+double oracle_sparse_dot_f64(int64_t nnz, const int64_t* idx, const double* v,
+                             const double* x, int64_t inc);
+float oracle_sparse_dot_f32(int64_t nnz, const int64_t* idx, const float* v,
+                            const float* x, int64_t inc);
+void oracle_sparse_axpy_f64(int64_t nnz, double alpha, const int64_t* idx,
+                            const double* v, double* x, int64_t inc);
+void oracle_sparse_axpy_f32(int64_t nnz, float alpha, const int64_t* idx,
+                            const float* v, float* x, int64_t inc);
+void oracle_sparse_gather_f64(int64_t nnz, const int64_t* idx, double* v,
+                              const double* x, int64_t inc);
+void oracle_sparse_gather_f32(int64_t nnz, const int64_t* idx, float* v,
+                              const float* x, int64_t inc);
+void oracle_sparse_gather_zero_f64(int64_t nnz, const int64_t* idx, double* v,
+                                   double* x, int64_t inc);
+void oracle_sparse_gather_zero_f32(int64_t nnz, const int64_t* idx, float* v,
+                                   float* x, int64_t inc);
+void oracle_sparse_scatter_f64(int64_t nnz, const int64_t* idx, const double* v,
+                               double* x, int64_t inc);
+void oracle_sparse_scatter_f32(int64_t nnz, const int64_t* idx, const float* v,
+                               float* x, int64_t inc);
 
 #ifdef __cplusplus
 }

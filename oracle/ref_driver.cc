@@ -319,3 +319,60 @@ int ref_finalize_f32(int64_t rows, int64_t cols, int64_t nnz, const int64_t* coo
 }
 
 }  // extern "C"
+
+// ---- sparse Level-1: the reference's own SparseDot / SparseAxpy /
+// SparseGather / SparseGatherZero / SparseScatter (blas.h:1330-1582) on host
+// arrays (dense stride `inc`).  op: 0 dot, 1 axpy, 2 gather, 3 gather-zero,
+// 4 scatter.  `v` is read (dot, axpy, scatter) or written (gathers); `x` is
+// read (dot, gather) or written (axpy, gather-zero, scatter); the dot result
+// goes to *dot.
+namespace {
+
+template <typename T>
+int RefLevel1(int op, int64_t nnz, int64_t extent, const int64_t* idx, T* v, T* x, int64_t inc,
+              T alpha, T* dot) {
+  const asc::ExecutionContext ctx = asc::ExecutionContext::Serial();
+  const asc::ConstMemoryView istore{idx, sizeof(int64_t) * static_cast<std::size_t>(nnz),
+                                    asc::MemorySpace::kHost};
+  const asc::ConstMemoryView vstore{v, sizeof(T) * static_cast<std::size_t>(nnz),
+                                    asc::MemorySpace::kHost};
+  const std::size_t xbytes =
+      extent > 0 ? sizeof(T) * static_cast<std::size_t>((extent - 1) * inc + 1) : 0;
+  const asc::ConstMemoryView xstore{x, xbytes, asc::MemorySpace::kHost};
+  auto sparse = asc::SparseBlasIndexedVectorView<T>::Create(idx, v, nnz, extent, istore, vstore);
+  auto dense = asc::SparseBlasVectorView<T>::Create(x, extent, inc, xstore);
+  if (!sparse.ok()) return static_cast<int>(sparse.status().code());
+  if (!dense.ok()) return static_cast<int>(dense.status().code());
+  const asc::SparseBlasIndexedVectorView<const T> csparse(*sparse);
+  const asc::SparseBlasVectorView<const T> cdense(*dense);
+  asc::Status st;
+  switch (op) {
+    case 0: {
+      auto r = asc::SparseDot(ctx, asc::SparseBlasConjugation::kUnconjugated, csparse, cdense);
+      if (!r.ok()) return static_cast<int>(r.status().code());
+      *dot = *r;
+      return 0;
+    }
+    case 1: st = asc::SparseAxpy(ctx, alpha, csparse, *dense); break;
+    case 2: st = asc::SparseGather(ctx, cdense, *sparse); break;
+    case 3: st = asc::SparseGatherZero(ctx, *dense, *sparse); break;
+    default: st = asc::SparseScatter(ctx, csparse, *dense); break;
+  }
+  return static_cast<int>(st.code());
+}
+
+}  // namespace
+
+extern "C" {
+
+int ref_level1_f64(int op, int64_t nnz, int64_t extent, const int64_t* idx, double* v, double* x,
+                   int64_t inc, double alpha, double* dot) {
+  return RefLevel1<double>(op, nnz, extent, idx, v, x, inc, alpha, dot);
+}
+
+int ref_level1_f32(int op, int64_t nnz, int64_t extent, const int64_t* idx, float* v, float* x,
+                   int64_t inc, float alpha, float* dot) {
+  return RefLevel1<float>(op, nnz, extent, idx, v, x, inc, alpha, dot);
+}
+
+}  // extern "C"

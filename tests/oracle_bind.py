@@ -38,6 +38,12 @@ def lib():
         L.oracle_compressed_to_coo.argtypes = [C.c_int, I64, VP, VP, VP, C.c_int, VP, VP]
         L.oracle_coo_finalize_f64.argtypes = [I64, VP, VP, C.c_int, C.c_int, VP, VP, VP]
         L.oracle_coo_finalize_f32.argtypes = [I64, VP, VP, C.c_int, C.c_int, VP, VP, VP]
+        for sfx, ct in (("f64", C.c_double), ("f32", C.c_float)):
+            getattr(L, f"oracle_sparse_dot_{sfx}").argtypes = [I64, VP, VP, VP, I64]
+            getattr(L, f"oracle_sparse_dot_{sfx}").restype = ct
+            getattr(L, f"oracle_sparse_axpy_{sfx}").argtypes = [I64, ct, VP, VP, VP, I64]
+            for op in ("gather", "gather_zero", "scatter"):
+                getattr(L, f"oracle_sparse_{op}_{sfx}").argtypes = [I64, VP, VP, VP, I64]
         _lib = L
     return _lib
 
@@ -159,6 +165,8 @@ def ref():
         R.ref_transpose_f64.argtypes = [C.c_int, I64, I64, VP, VP, VP, VP, VP, VP]
         R.ref_finalize_f64.argtypes = [I64, I64, I64, VP, VP, C.c_int, C.c_int, VP, VP, VP]
         R.ref_finalize_f32.argtypes = [I64, I64, I64, VP, VP, C.c_int, C.c_int, VP, VP, VP]
+        R.ref_level1_f64.argtypes = [C.c_int, I64, I64, VP, VP, VP, I64, C.c_double, VP]
+        R.ref_level1_f32.argtypes = [C.c_int, I64, I64, VP, VP, VP, I64, C.c_float, VP]
         _ref = R
     return _ref
 
@@ -233,3 +241,36 @@ def ref_finalize(rows, cols, coords, values, duplicate_policy, zero_policy):
     rc = fn(rows, cols, nnz, _p(coords), _p(values), duplicate_policy, zero_policy, _p(oc),
             _p(ov), C.byref(n))
     return rc, oc[: 2 * n.value], ov[: n.value]
+
+
+# ---- sparse Level-1 -------------------------------------------------------------
+LEVEL1_OPS = ("dot", "axpy", "gather", "gather_zero", "scatter")
+
+
+def level1(op, idx, v, x, inc=1, alpha=1.0):
+    """Oracle Level-1 op on copies; returns (dot or None, v, x) after the op."""
+    idx = i64(idx)
+    dt = np.float32 if np.asarray(v).dtype == np.float32 else np.float64
+    sfx = "f32" if dt == np.float32 else "f64"
+    v = np.ascontiguousarray(v, dtype=dt).copy()
+    x = np.ascontiguousarray(x, dtype=dt).copy()
+    n = idx.shape[0]
+    L = lib()
+    if op == "dot":
+        return dt(getattr(L, f"oracle_sparse_dot_{sfx}")(n, _p(idx), _p(v), _p(x), inc)), v, x
+    if op == "axpy":
+        getattr(L, f"oracle_sparse_axpy_{sfx}")(n, alpha, _p(idx), _p(v), _p(x), inc)
+    else:
+        getattr(L, f"oracle_sparse_{op}_{sfx}")(n, _p(idx), _p(v), _p(x), inc)
+    return None, v, x
+
+
+def ref_level1(op, idx, v, x, extent, inc=1, alpha=1.0):
+    idx = i64(idx)
+    dt = np.float32 if np.asarray(v).dtype == np.float32 else np.float64
+    v = np.ascontiguousarray(v, dtype=dt).copy()
+    x = np.ascontiguousarray(x, dtype=dt).copy()
+    dot = np.zeros(1, dt)
+    fn = ref().ref_level1_f32 if dt == np.float32 else ref().ref_level1_f64
+    rc = fn(LEVEL1_OPS.index(op), idx.shape[0], extent, _p(idx), _p(v), _p(x), inc, alpha, _p(dot))
+    return rc, (dot[0] if op == "dot" else None), v, x

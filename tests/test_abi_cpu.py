@@ -295,3 +295,50 @@ def test_finalize_validation(vctx):
     assert fin(trip(), 1, 0, v=oc + 8) == H.INVALID_ARGUMENT  # outputs overlap
     assert fin(trip(), 1, 0, nb=n - 8) == H.INVALID_ARGUMENT  # short workspace
     assert fin(trip(), 1, 1) == H.UNAVAILABLE  # valid: reaches the enqueue point
+
+
+def test_level1_validation(vctx):
+    lib, h = vctx.lib, vctx.handle
+
+    def ix(nnz=2, extent=3, kind=H.F64, space=H.DEVICE, trusted=1, writable=1, base=FAKE):
+        return H.IndexedDesc(base, base + 0x1000, nnz, extent, kind, space, writable, trusted)
+
+    def call(op, sp, dense, result=None, alg=H.ALG_AUTO, conj=0):
+        opt = H.options(alg)
+        return lib.asc_sparse_hip_standard_level1(h, op, conj, H.Scalar(1.0, 0.0), C.byref(sp),
+                                                  C.byref(dense),
+                                                  C.byref(result) if result is not None else None,
+                                                  C.byref(opt)).code
+
+    d3 = dev_vec(3, base=0x200000)
+    r1 = dev_vec(1, base=0x300000)
+    assert call(H.LEVEL1_DOT, ix(), d3, r1) == H.UNAVAILABLE
+    assert call(H.LEVEL1_DOT, ix(), d3, r1, alg=H.ALG_ORDERED) == H.UNAVAILABLE
+    assert call(H.LEVEL1_DOT, ix(), d3, r1, alg=H.ALG_MERGE) == H.INVALID_ARGUMENT
+    assert call(H.LEVEL1_DOT, ix(), d3, None) == H.INVALID_ARGUMENT  # no result vector
+    assert call(H.LEVEL1_DOT, ix(), d3, dev_vec(2, base=0x300000)) == H.SHAPE
+    assert call(H.LEVEL1_DOT, ix(), d3, dev_vec(1, base=0x200000)) == H.INVALID_ARGUMENT  # overlap
+    assert call(5, ix(), d3) == H.INVALID_ARGUMENT
+    assert call(H.LEVEL1_AXPY, ix(), d3, conj=2) == H.INVALID_ARGUMENT
+    assert call(H.LEVEL1_AXPY, ix(trusted=0), d3) == H.INVALID_ARGUMENT
+    assert call(H.LEVEL1_AXPY, ix(space=H.HOST), d3) == H.MEMORY_ACCESS
+    assert call(H.LEVEL1_AXPY, ix(kind=H.C128), d3) == H.UNSUPPORTED
+    assert call(H.LEVEL1_AXPY, ix(nnz=4), d3) == H.INVALID_ARGUMENT  # nnz > dense extent
+    assert call(H.LEVEL1_AXPY, ix(), dev_vec(4, base=0x200000)) == H.SHAPE
+    assert call(H.LEVEL1_AXPY, ix(), dev_vec(3, base=0x200000, writable=0)) == H.INVALID_ARGUMENT
+    assert call(H.LEVEL1_AXPY, ix(), dev_vec(3, base=FAKE + 0x1000)) == H.INVALID_ARGUMENT  # overlap
+    assert call(H.LEVEL1_GATHER, ix(writable=0), d3) == H.INVALID_ARGUMENT
+    for op in (H.LEVEL1_AXPY, H.LEVEL1_GATHER, H.LEVEL1_GATHER_ZERO, H.LEVEL1_SCATTER):
+        assert call(op, ix(), d3) == H.UNAVAILABLE
+    # host clone validation: unsorted / out-of-range indices
+    idx = np.array([0, 2, 1], np.int64)
+    vals = np.zeros(3)
+    out = H.IndexedDesc()
+    src = H.IndexedDesc(idx.ctypes.data, vals.ctypes.data, 3, 5, H.F64, H.HOST, 0, 0)
+    assert lib.asc_sparse_hip_clone_indexed(h, C.byref(src), FAKE, FAKE + 0x100, C.byref(out)).code == H.INDEX
+    idx2 = np.array([0, 2, 5], np.int64)
+    src = H.IndexedDesc(idx2.ctypes.data, vals.ctypes.data, 3, 5, H.F64, H.HOST, 0, 0)
+    assert lib.asc_sparse_hip_clone_indexed(h, C.byref(src), FAKE, FAKE + 0x100, C.byref(out)).code == H.INDEX
+    idx3 = np.array([0, 2, 4], np.int64)
+    src = H.IndexedDesc(idx3.ctypes.data, vals.ctypes.data, 3, 5, H.F64, H.HOST, 0, 0)
+    assert lib.asc_sparse_hip_clone_indexed(h, C.byref(src), FAKE, FAKE + 0x100, C.byref(out)).code == H.UNAVAILABLE

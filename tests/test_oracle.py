@@ -310,3 +310,28 @@ def test_validate_rejects_noncanonical():
     assert O.validate(3, 4, [0, 2, 4, 5], [3, 1, 0, 2, 1], 5) != 0  # unsorted
     assert O.validate(3, 4, [0, 2, 4, 5], [1, 3, 0, 2, 4], 5) != 0  # out of range
     assert O.validate(3, 4, [1, 2, 4, 5], [1, 3, 0, 2, 1], 5) != 0  # bad start
+
+
+def test_level1_golden_vectors():
+    g = GOLD["level1"]
+    idx, v, x = np.array(g["indices"]), np.array(g["values"]), np.array(g["dense"])
+    assert O.level1("dot", idx, v, x)[0] == g["dot"]
+    np.testing.assert_array_equal(O.level1("axpy", idx, v, x, alpha=g["axpy_alpha"])[2], g["axpy"])
+    np.testing.assert_array_equal(O.level1("gather", idx, np.zeros(2), x)[1], g["gather"])
+    _, gv, gx = O.level1("gather_zero", idx, np.zeros(2), x)
+    np.testing.assert_array_equal(gv, g["gather_zero_values"])
+    np.testing.assert_array_equal(gx, g["gather_zero_dense"])
+    np.testing.assert_array_equal(O.level1("scatter", idx, v, np.zeros(3))[2], g["scatter_into_zeros"])
+    # alpha == 0 leaves the dense vector untouched even with NaN values
+    out = O.level1("axpy", idx, np.full(2, np.nan), x, alpha=0.0)[2]
+    np.testing.assert_array_equal(out, x)
+
+
+def test_level1_dot_exact_integers():
+    # integer data: every partial sum is exact, so the dot equals the integer
+    # dot product whatever the association
+    rng = np.random.default_rng(9)
+    idx = np.sort(rng.choice(5000, 700, replace=False))
+    v = rng.integers(-50, 50, 700).astype(np.float64)
+    x = rng.integers(-50, 50, 5000).astype(np.float64)
+    assert O.level1("dot", idx, v, x)[0] == float(np.dot(v.astype(np.int64), x[idx].astype(np.int64)))

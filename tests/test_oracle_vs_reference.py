@@ -121,3 +121,24 @@ def test_finalize_matches_reference(seed, dup, zero, dtype):
         np.testing.assert_array_equal(a[1], b[1])
         np.testing.assert_array_equal(np.isnan(a[2]), np.isnan(b[2]))
         np.testing.assert_array_equal(np.nan_to_num(a[2]), np.nan_to_num(b[2]))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("op", ["dot", "axpy", "gather", "gather_zero", "scatter"])
+@pytest.mark.parametrize("seed", range(3))
+def test_level1_matches_reference(op, dtype, seed):
+    rng = np.random.default_rng(300 + seed)
+    extent = int(rng.integers(1, 400))
+    inc = int(rng.integers(1, 4))
+    nnz = int(rng.integers(0, extent + 1))
+    idx = np.sort(rng.choice(extent, nnz, replace=False)).astype(np.int64)
+    v = rng.standard_normal(nnz).astype(dtype)
+    x = rng.standard_normal((extent - 1) * inc + 1).astype(dtype)
+    alpha = [0.0, 1.5, -0.25][seed]
+    a = O.level1(op, idx, v, x, inc, alpha)
+    rc, dot, rv, rx = O.ref_level1(op, idx, v, x, extent, inc, alpha)
+    assert rc == 0
+    if op == "dot":
+        assert np.asarray([a[0]], dtype).tobytes() == np.asarray([dot], dtype).tobytes()
+    np.testing.assert_array_equal(a[1].view(np.uint8), rv.view(np.uint8))
+    np.testing.assert_array_equal(a[2].view(np.uint8), rx.view(np.uint8))
