@@ -254,6 +254,31 @@ class HipProviderAccess {
     return d;
   }
   template <typename E>
+  static SparseBlasIndexedVectorView<E> TrustedDeviceIndexed(const void* indices, E* values,
+                                                             nnz_t nnz, extent_t extent) {
+    SparseBlasIndexedVectorView<E> v;
+    v.indices_ = static_cast<const index_t*>(indices);
+    v.values_ = values;
+    v.nonzeros_ = nnz;
+    v.dense_extent_ = extent;
+    v.memory_space_ = MemorySpace::kDevice;
+    v.trusted_ = true;
+    return v;
+  }
+  template <typename E>
+  static asc_hip_indexed_desc Describe(const SparseBlasIndexedVectorView<E>& v) {
+    asc_hip_indexed_desc d{};
+    d.indices = v.indices();
+    d.values = v.values();
+    d.nnz = v.nnz();
+    d.dense_extent = v.dense_extent();
+    d.element_kind = internal_sparse_hip::ElementKind<E>();
+    d.memory_space = static_cast<std::uint8_t>(v.memory_space());
+    d.writable = std::is_const_v<E> ? 0 : 1;
+    d.trusted = v.canonical_structure_trusted() ? 1 : 0;
+    return d;
+  }
+  template <typename E>
   static asc_hip_sparse_desc Describe(const CoordinateView<E, 2>& v) {
     asc_hip_sparse_desc d{};
     d.outer_offsets = v.coordinates();
@@ -838,6 +863,150 @@ template <typename Element>
   if (!s.ok()) return s;
   out->set_count(d.nnz);
   return std::move(*out);
+}
+
+// ---- sparse Level-1 (blas.h:1330-1582; CudaSparseDot ... CudaSparseScatter,
+// providers/cuda.h:1464-1614) ----------------------------------------------
+
+// Owner of a device indexed vector (CudaIndexedVectorArray,
+// providers/cuda.h:1066-1220); view() is trusted.
+template <typename Element>
+  requires internal_sparse_hip::kSupportedElement<Element>
+class HipIndexedVectorArray {
+ public:
+  [[nodiscard]] bool valid() const noexcept { return indices_.data() != nullptr; }
+  [[nodiscard]] Result<SparseBlasIndexedVectorView<Element>> view() {
+    if (!valid()) return Status(ErrorCode::kInvalidState, "HIP indexed vector is empty");
+    return HipProviderAccess::TrustedDeviceIndexed<Element>(
+        indices_.data(), static_cast<Element*>(values_.data()), nnz_, extent_);
+  }
+  [[nodiscard]] Result<SparseBlasIndexedVectorView<const Element>> view() const {
+    if (!valid()) return Status(ErrorCode::kInvalidState, "HIP indexed vector is empty");
+    return HipProviderAccess::TrustedDeviceIndexed<const Element>(
+        indices_.data(), static_cast<const Element*>(values_.data()), nnz_, extent_);
+  }
+  static Result<HipIndexedVectorArray> Allocate(const HipMemoryResource& resource, nnz_t nnz,
+                                                extent_t extent) {
+    HipIndexedVectorArray a;
+    auto i = HipBuffer::Allocate(resource, 8 * static_cast<std::size_t>(nnz) + 8);
+    if (!i.ok()) return i.status();
+    auto v = HipBuffer::Allocate(resource, sizeof(Element) * static_cast<std::size_t>(nnz) + 8);
+    if (!v.ok()) return v.status();
+    a.indices_ = *i;
+    a.values_ = *v;
+    a.nnz_ = nnz;
+    a.extent_ = extent;
+    return a;
+  }
+  [[nodiscard]] void* indices_data() const noexcept { return indices_.data(); }
+  [[nodiscard]] void* values_data() const noexcept { return values_.data(); }
+
+ private:
+  HipBuffer indices_, values_;
+  nnz_t nnz_ = 0;
+  extent_t extent_ = 0;
+};
+
+template <typename Element>
+struct HipIndexedVectorClone {
+  HipIndexedVectorArray<Element> array;
+  CompletionEvent completion;
+};
+
+// Host canonical indexed vector -> device (CudaCloneIndexedVector,
+// providers/cuda.h:1247-1272).
+template <typename SourceElement>
+  requires internal_sparse_hip::kSupportedElement<SourceElement>
+[[nodiscard]] Result<HipIndexedVectorClone<std::remove_const_t<SourceElement>>>
+HipCloneIndexedVector(SparseHipContext& context, SparseBlasIndexedVectorView<SourceElement> source,
+                      const HipMemoryResource& resource) {
+  using E = std::remove_const_t<SourceElement>;
+  auto array = HipIndexedVectorArray<E>::Allocate(resource, source.nnz(), source.dense_extent());
+  if (!array.ok()) return array.status();
+  const asc_hip_indexed_desc src = HipProviderAccess::Describe(source);
+  asc_hip_indexed_desc out{};
+  const Status s = internal_sparse_hip::ToStatus(asc_sparse_hip_clone_indexed(
+      context.native(), &src, array->indices_data(), array->values_data(), &out));
+  if (!s.ok()) return s;
+  auto done = context.Record();
+  if (!done.ok()) return done.status();
+  return HipIndexedVectorClone<E>{std::move(*array), *done};
+}
+
+namespace internal_sparse_hip {
+
+template <typename Element>
+Result<CompletionEvent> Level1(SparseHipContext& context, std::uint8_t op,
+                               SparseBlasConjugation conj, Element alpha,
+                               const asc_hip_indexed_desc& sparse, const asc_hip_vector_desc& dense,
+                               const asc_hip_vector_desc* result, HipSpmvAlgorithm algorithm) {
+  const asc_hip_spmv_options opt{static_cast<std::uint8_t>(algorithm), ASC_HIP_ACC_PLAIN, {}};
+  const asc_hip_scalar a{static_cast<double>(alpha), 0.0};
+  return Enqueued(context, asc_sparse_hip_standard_level1(
+                               context.native(), op,
+                               conj == SparseBlasConjugation::kConjugated ? ASC_HIP_CONJUGATED
+                                                                          : ASC_HIP_UNCONJUGATED,
+                               a, &sparse, &dense, result, &opt));
+}
+
+}  // namespace internal_sparse_hip
+
+// Writes the dot product into `result` (one element).  kOrdered: the serial
+// reference's left-to-right sum, bit for bit; kAuto: a deterministic tree.
+template <typename Element>
+  requires internal_sparse_hip::kSupportedElement<Element>
+[[nodiscard]] Result<CompletionEvent> HipSparseDot(
+    SparseHipContext& context, SparseBlasConjugation conjugation,
+    SparseBlasIndexedVectorView<const Element> sparse, SparseBlasVectorView<const Element> dense,
+    SparseBlasVectorView<Element> result, HipSpmvAlgorithm algorithm = HipSpmvAlgorithm::kAuto) {
+  const auto r = internal_sparse_hip::Describe(result);
+  return internal_sparse_hip::Level1<Element>(context, ASC_HIP_LEVEL1_DOT, conjugation, Element(1),
+                                              HipProviderAccess::Describe(sparse),
+                                              internal_sparse_hip::Describe(dense), &r, algorithm);
+}
+
+template <typename Element>
+  requires internal_sparse_hip::kSupportedElement<Element>
+[[nodiscard]] Result<CompletionEvent> HipSparseAxpy(SparseHipContext& context, Element alpha,
+                                                    SparseBlasIndexedVectorView<const Element> sparse,
+                                                    SparseBlasVectorView<Element> dense) {
+  return internal_sparse_hip::Level1<Element>(
+      context, ASC_HIP_LEVEL1_AXPY, SparseBlasConjugation::kUnconjugated, alpha,
+      HipProviderAccess::Describe(sparse), internal_sparse_hip::Describe(dense), nullptr,
+      HipSpmvAlgorithm::kAuto);
+}
+
+template <typename Element>
+  requires internal_sparse_hip::kSupportedElement<Element>
+[[nodiscard]] Result<CompletionEvent> HipSparseGather(SparseHipContext& context,
+                                                      SparseBlasVectorView<const Element> dense,
+                                                      SparseBlasIndexedVectorView<Element> sparse) {
+  return internal_sparse_hip::Level1<Element>(
+      context, ASC_HIP_LEVEL1_GATHER, SparseBlasConjugation::kUnconjugated, Element(1),
+      HipProviderAccess::Describe(sparse), internal_sparse_hip::Describe(dense), nullptr,
+      HipSpmvAlgorithm::kAuto);
+}
+
+template <typename Element>
+  requires internal_sparse_hip::kSupportedElement<Element>
+[[nodiscard]] Result<CompletionEvent> HipSparseGatherZero(SparseHipContext& context,
+                                                          SparseBlasVectorView<Element> dense,
+                                                          SparseBlasIndexedVectorView<Element> sparse) {
+  return internal_sparse_hip::Level1<Element>(
+      context, ASC_HIP_LEVEL1_GATHER_ZERO, SparseBlasConjugation::kUnconjugated, Element(1),
+      HipProviderAccess::Describe(sparse), internal_sparse_hip::Describe(dense), nullptr,
+      HipSpmvAlgorithm::kAuto);
+}
+
+template <typename Element>
+  requires internal_sparse_hip::kSupportedElement<Element>
+[[nodiscard]] Result<CompletionEvent> HipSparseScatter(SparseHipContext& context,
+                                                       SparseBlasIndexedVectorView<const Element> sparse,
+                                                       SparseBlasVectorView<Element> dense) {
+  return internal_sparse_hip::Level1<Element>(
+      context, ASC_HIP_LEVEL1_SCATTER, SparseBlasConjugation::kUnconjugated, Element(1),
+      HipProviderAccess::Describe(sparse), internal_sparse_hip::Describe(dense), nullptr,
+      HipSpmvAlgorithm::kAuto);
 }
 
 // Stream-ordered copies between host and device memory (CopyBytes,

@@ -301,6 +301,68 @@ class SparseBlasVectorView {
   MemorySpace memory_space_ = MemorySpace::kHost;
 };
 
+// include/asc/sparse/blas.h:74-77
+enum class SparseBlasConjugation : std::uint8_t { kUnconjugated, kConjugated };
+
+// Canonical indexed (sparse) vector (blas.h:526-789): nnz strictly
+// increasing indices in [0, dense_extent) and their values.  Host views are
+// checked by Create; device views are trusted only when a provider made them.
+template <typename Element>
+class SparseBlasIndexedVectorView {
+ public:
+  static Result<SparseBlasIndexedVectorView> Create(const index_t* indices, Element* values,
+                                                    nnz_t nonzeros, extent_t dense_extent,
+                                                    ConstMemoryView index_storage,
+                                                    ConstMemoryView value_storage) {
+    if (nonzeros < 0 || dense_extent < 0 || nonzeros > dense_extent ||
+        index_storage.memory_space != value_storage.memory_space)
+      return Status(ErrorCode::kInvalidArgument, "A sparse indexed-vector descriptor is invalid");
+    SparseBlasIndexedVectorView v;
+    v.indices_ = indices;
+    v.values_ = values;
+    v.nonzeros_ = nonzeros;
+    v.dense_extent_ = dense_extent;
+    v.memory_space_ = index_storage.memory_space;
+    if (v.memory_space_ == MemorySpace::kHost || v.memory_space_ == MemorySpace::kPinnedHost) {
+      index_t previous = -1;
+      for (nnz_t p = 0; p < nonzeros; ++p) {
+        if (indices[p] <= previous || indices[p] < 0 || indices[p] >= dense_extent)
+          return Status(ErrorCode::kIndex,
+                        "Sparse indexed-vector indices must be sorted, unique, zero-based, and in range");
+        previous = indices[p];
+      }
+    }
+    return v;
+  }
+  [[nodiscard]] constexpr const index_t* indices() const noexcept { return indices_; }
+  [[nodiscard]] constexpr Element* values() const noexcept { return values_; }
+  [[nodiscard]] constexpr nnz_t nnz() const noexcept { return nonzeros_; }
+  [[nodiscard]] constexpr extent_t dense_extent() const noexcept { return dense_extent_; }
+  [[nodiscard]] constexpr MemorySpace memory_space() const noexcept { return memory_space_; }
+  [[nodiscard]] constexpr bool canonical_structure_trusted() const noexcept { return trusted_; }
+  operator SparseBlasIndexedVectorView<const Element>() const noexcept {  // NOLINT
+    SparseBlasIndexedVectorView<const Element> v;
+    v.indices_ = indices_;
+    v.values_ = values_;
+    v.nonzeros_ = nonzeros_;
+    v.dense_extent_ = dense_extent_;
+    v.memory_space_ = memory_space_;
+    v.trusted_ = trusted_;
+    return v;
+  }
+
+ private:
+  template <typename>
+  friend class SparseBlasIndexedVectorView;
+  friend class HipProviderAccess;
+  const index_t* indices_ = nullptr;
+  Element* values_ = nullptr;
+  nnz_t nonzeros_ = 0;
+  extent_t dense_extent_ = 0;
+  MemorySpace memory_space_ = MemorySpace::kHost;
+  bool trusted_ = false;
+};
+
 // Dense block, row- or column-major with a leading dimension (blas.h:816-1063).
 template <typename Element>
 class SparseBlasMatrixView {

@@ -412,6 +412,55 @@ void TestFinalize(Gpu& g) {
              ErrorCode::kIndex);
 }
 
+// standard_blas_test.cc:69-174 (real parts): indices {0, 2}, values
+// {1, -2}, dense {3, 9, 2}.
+template <typename E>
+void TestLevelOne(Gpu& g) {
+  Case(sizeof(E) == 8 ? "Level-1 golden (double)" : "Level-1 golden (float)");
+  const std::vector<asc::index_t> idx{0, 2};
+  const std::vector<E> vals{1, -2};
+  auto host = asc::SparseBlasIndexedVectorView<const E>::Create(
+      idx.data(), vals.data(), 2, 3, {idx.data(), 16, MemorySpace::kHost},
+      {vals.data(), 2 * sizeof(E), MemorySpace::kHost});
+  CHECK(host.ok());
+  auto clone = asc::HipCloneIndexedVector(g.ctx, *host, g.device);
+  CHECK(clone.ok() && clone->completion.Wait().ok());
+  auto sp = clone->array.view();
+  const asc::SparseBlasIndexedVectorView<const E> csp = *sp;
+  auto xb = Upload<E>(g, {3, 9, 2});
+  auto rb = Upload<E>(g, {7});
+  const asc::ConstMemoryView xs{xb.data(), 3 * sizeof(E), MemorySpace::kDevice};
+  const asc::ConstMemoryView rs{rb.data(), sizeof(E), MemorySpace::kDevice};
+  auto x = asc::SparseBlasVectorView<E>::Create(static_cast<E*>(xb.data()), 3, 1, xs);
+  auto cx = asc::SparseBlasVectorView<const E>::Create(static_cast<const E*>(xb.data()), 3, 1, xs);
+  auto r = asc::SparseBlasVectorView<E>::Create(static_cast<E*>(rb.data()), 1, 1, rs);
+  for (auto alg : {asc::HipSpmvAlgorithm::kAuto, asc::HipSpmvAlgorithm::kOrdered}) {
+    auto d = asc::HipSparseDot(g.ctx, asc::SparseBlasConjugation::kConjugated, csp, *cx, *r, alg);
+    CHECK(d.ok() && d->Wait().ok());
+    CHECK(Download<E>(g, rb, 1)[0] == E(-1));
+  }
+  auto a = asc::HipSparseAxpy(g.ctx, E(2), csp, *x);
+  CHECK(a.ok() && a->Wait().ok());
+  CHECK((Download<E>(g, xb, 3) == std::vector<E>{5, 9, -2}));
+  auto gz = asc::HipSparseGatherZero(g.ctx, *x, *sp);
+  CHECK(gz.ok() && gz->Wait().ok());
+  CHECK((Download<E>(g, xb, 3) == std::vector<E>{0, 9, 0}));
+  std::vector<E> gathered(2);
+  CHECK(Copy(g, gathered.data(), sp->values(), 2 * sizeof(E)));
+  CHECK((gathered == std::vector<E>{5, -2}));
+  auto sc = asc::HipSparseScatter(g.ctx, csp, *x);
+  CHECK(sc.ok() && sc->Wait().ok());
+  CHECK((Download<E>(g, xb, 3) == std::vector<E>{5, 9, -2}));
+  auto ga = asc::HipSparseGather(g.ctx, *cx, *sp);
+  CHECK(ga.ok() && ga->Wait().ok());
+  // An untrusted device view is refused (blas.h:1231-1239 for the host form).
+  auto untrusted = asc::SparseBlasIndexedVectorView<const E>::Create(
+      sp->indices(), sp->values(), 2, 3, {sp->indices(), 16, MemorySpace::kDevice},
+      {sp->values(), 2 * sizeof(E), MemorySpace::kDevice});
+  CHECK(untrusted.ok());
+  CHECK_CODE(asc::HipSparseAxpy(g.ctx, E(1), *untrusted, *x).status(), ErrorCode::kInvalidArgument);
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -441,6 +490,8 @@ int main(int argc, char** argv) {
     TestConversions<float>(g);
     TestColumnPanel(g);
     TestFinalize(g);
+    TestLevelOne<double>(g);
+    TestLevelOne<float>(g);
   }
   std::printf("%d checks, %d failures\n", g_checks, g_failures);
   return g_failures == 0 ? 0 : 1;
