@@ -832,3 +832,55 @@ def level1(ctx: Context, op: int, sparse: DeviceIndexed, dense: VectorDesc, alph
                                                  C.byref(sparse.desc), C.byref(dense),
                                                  C.byref(result) if result is not None else None,
                                                  C.byref(opt)))
+
+
+UPPER, LOWER = 0, 1
+NON_UNIT, UNIT = 0, 1
+
+
+class TriangularInfo(C.Structure):
+    _fields_ = [("rows", C.c_int64), ("nnz", C.c_int64), ("levels", C.c_int64),
+                ("max_level_rows", C.c_int64), ("offsets", C.c_void_p), ("indices", C.c_void_p),
+                ("values", C.c_void_p), ("transpose", C.c_uint8), ("triangle", C.c_uint8),
+                ("diagonal", C.c_uint8), ("format", C.c_uint8), ("reserved", C.c_uint8 * 4)]
+
+
+_SIGS8 = {
+    "asc_sparse_hip_triangular_analysis_size": [_VP, C.c_uint8, P(SparseDesc), P(C.c_size_t)],
+    "asc_sparse_hip_triangular_analyze": [_VP, C.c_uint8, C.c_uint8, C.c_uint8, P(SparseDesc), _VP,
+                                          C.c_size_t, P(TriangularInfo)],
+    "asc_sparse_hip_triangular_solve": [_VP, C.c_uint8, Scalar, P(SparseDesc), P(MatrixDesc), _VP,
+                                        C.c_size_t, P(TriangularInfo)],
+}
+_SIGS.update(_SIGS8)
+if _lib is not None:
+    for _n, _a in _SIGS8.items():
+        getattr(_lib, _n).argtypes = _a
+        getattr(_lib, _n).restype = Status
+
+
+class TriangularPlan:
+    """The analysis buffer and its info for one matrix and operation."""
+
+    def __init__(self, buffer: Buffer, info: TriangularInfo):
+        self.buffer, self.info = buffer, info
+
+
+def triangular_analyze(ctx: Context, transpose: int, triangle: int, diagonal: int,
+                       a: DeviceSparse) -> TriangularPlan:
+    need = C.c_size_t()
+    check(ctx.lib.asc_sparse_hip_triangular_analysis_size(ctx.handle, transpose, C.byref(a.desc),
+                                                          C.byref(need)))
+    buf = ctx.alloc(need.value)
+    info = TriangularInfo()
+    check(ctx.lib.asc_sparse_hip_triangular_analyze(ctx.handle, transpose, triangle, diagonal,
+                                                    C.byref(a.desc), buf.ptr, buf.nbytes,
+                                                    C.byref(info)))
+    return TriangularPlan(buf, info)
+
+
+def triangular_solve(ctx: Context, transpose: int, alpha: float, a: DeviceSparse, rhs: MatrixDesc,
+                     plan: TriangularPlan):
+    check(ctx.lib.asc_sparse_hip_triangular_solve(ctx.handle, transpose, Scalar(alpha, 0.0),
+                                                  C.byref(a.desc), C.byref(rhs), plan.buffer.ptr,
+                                                  plan.buffer.nbytes, C.byref(plan.info)))

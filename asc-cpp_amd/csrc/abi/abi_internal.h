@@ -74,6 +74,9 @@ asc_hip_status check_device_sparse(const asc_hip_context* ctx,
 asc_hip_status check_vector(const asc_hip_context* ctx, const asc_hip_vector_desc* v,
                             uint8_t kind, int64_t extent, bool writable,
                             const char* what);
+asc_hip_status check_matrix(const asc_hip_context* ctx, const asc_hip_matrix_desc* m,
+                            uint8_t kind, int64_t rows, int64_t cols, bool writable,
+                            const char* what);
 asc_hip_status check_workspace(const asc_hip_context* ctx, void* ws, size_t ws_bytes,
                                size_t needed, const std::vector<Span>& operands);
 asc_hip_status unavailable_if_validation_only(const asc_hip_context* ctx);

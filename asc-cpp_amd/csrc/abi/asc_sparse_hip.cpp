@@ -258,6 +258,24 @@ asc_hip_status finish_launch(asc_hip_context* ctx, hipError_t e, const char* wha
   return fail_hip(e, what);
 }
 
+asc_hip_status check_matrix(const asc_hip_context* ctx, const asc_hip_matrix_desc* m,
+                            uint8_t kind, int64_t rows, int64_t cols, bool writable,
+                            const char* what) {
+  if (m == nullptr) return fail(ASC_HIP_INVALID_ARGUMENT, std::string(what) + " descriptor is null");
+  if (m->element_kind != kind) return fail(ASC_HIP_INVALID_ARGUMENT, std::string(what) + ": element kind does not match");
+  if (m->layout > ASC_HIP_ROW_MAJOR) return fail(ASC_HIP_INVALID_ARGUMENT, std::string(what) + ": unknown layout");
+  if (m->rows != rows || (cols >= 0 && m->cols != cols))
+    return fail(ASC_HIP_SHAPE, "Sparse matrix-matrix operand extents do not match");
+  const int64_t minor = m->layout == ASC_HIP_ROW_MAJOR ? m->cols : m->rows;
+  if (m->leading_dimension < (minor > 1 ? minor : 1))
+    return fail(ASC_HIP_INVALID_ARGUMENT, std::string(what) + ": leading dimension is too small");
+  if (writable && !m->writable) return fail(ASC_HIP_INVALID_ARGUMENT, std::string(what) + ": output is not writable");
+  if (m->memory_space != ASC_HIP_DEVICE && m->memory_space != ASC_HIP_MANAGED)
+    return fail(ASC_HIP_MEMORY_ACCESS, std::string(what) + ": block is not device memory");
+  if (m->rows > 0 && m->cols > 0) return check_device_pointer(ctx, m->data, what);
+  return ok();
+}
+
 asc_hip::Compressed as_compressed(const asc_hip_sparse_desc& d, bool row_view_of_csc) {
   asc_hip::Compressed c;
   c.offsets = static_cast<const int64_t*>(d.outer_offsets);

@@ -770,6 +770,36 @@ hipError_t coo_convert_launch(const int64_t* coords, const void* values,
   }
 }
 
+namespace {
+
+__global__ __launch_bounds__(kBlock) void iota_kernel(uint32_t* __restrict__ out, int64_t n) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride)
+    out[i] = static_cast<uint32_t>(i);
+}
+
+}  // namespace
+
+size_t sort_by_key_workspace_bytes(int64_t n, int64_t key_bound) {
+  return align256(sizeof(uint32_t) * static_cast<size_t>(n > 0 ? n : 1)) * 2 +
+         sort_layout(n, key_bound, sizeof(uint32_t)).total;
+}
+
+hipError_t sort_by_key_launch(const uint32_t* keys, int64_t n, int64_t key_bound,
+                              uint32_t* out_index, int64_t* out_key_offsets, void* workspace,
+                              hipStream_t s) {
+  if (n == 0) return fill_launch(out_key_offsets, 0, sizeof(int64_t) * (key_bound + 1), s);
+  char* ws = static_cast<char*>(workspace);
+  uint32_t* iota = reinterpret_cast<uint32_t*>(ws);
+  uint32_t* dummy = reinterpret_cast<uint32_t*>(ws + align256(sizeof(uint32_t) * n));
+  char* sws = ws + 2 * align256(sizeof(uint32_t) * n);
+  iota_kernel<<<grid_stride_blocks(n), kBlock, 0, s>>>(iota, n);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return stable_sort<uint32_t, uint32_t, uint32_t>(keys, 1, iota, 1, iota, n, key_bound, out_index,
+                                                   4, dummy, out_key_offsets, sws, s);
+}
+
 size_t coo_finalize_workspace_bytes(int64_t rows, int64_t cols, int64_t nnz, Elem elem) {
   return finalize_layout(nnz, rows, cols, elem_bytes(elem)).total;
 }

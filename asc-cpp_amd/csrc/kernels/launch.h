@@ -81,6 +81,40 @@ hipError_t panel_analysis_launch(const Compressed& a, int64_t panel_cols, int64_
                                  int64_t long_min, uint32_t* long_rows, uint32_t* splits,
                                  void* scratch, int64_t* out_nlong, hipStream_t stream);
 
+// Stable sort of the indices 0..n-1 by uint32 key (< key_bound): out_index
+// gets the sorted order, out_key_offsets[k] (k = 0..key_bound) the first
+// sorted position with key >= k.  `workspace` holds
+// sort_by_key_workspace_bytes(n, key_bound).
+size_t sort_by_key_workspace_bytes(int64_t n, int64_t key_bound);
+hipError_t sort_by_key_launch(const uint32_t* keys, int64_t n, int64_t key_bound,
+                              uint32_t* out_index, int64_t* out_key_offsets, void* workspace,
+                              hipStream_t stream);
+
+// ---- triangular solve (level-scheduled) ------------------------------------
+// The row view `rows` is op(A) in CSR form (outer = n rows of op(A)).
+// Analysis: validates the stored matrix `stored` against its declared
+// triangle (stored_inner_le_outer: every inner index <= its outer index) and,
+// when !unit, a stored nonzero diagonal (returns *failure 1 / 2), then levels
+// the rows of the row view (`lower`: the solve runs forward) and sorts them
+// by level.  Synchronous.
+struct TrsvLevels {
+  int64_t levels = 0;
+  int64_t max_level_rows = 0;
+};
+size_t trsv_analysis_scratch_bytes(int64_t n);
+hipError_t trsv_analysis_launch(const Compressed& stored, bool stored_inner_le_outer, bool unit,
+                                const Compressed& rows, bool lower, int64_t* level_ptr,
+                                uint32_t* row_order, void* scratch, TrsvLevels* out,
+                                int* failure, hipStream_t stream);
+// Solve in place on X (n x k, element (r, j) at x[r*rs + j*cs]); alpha == 0
+// is handled by the caller.
+hipError_t trsv_solve_launch(const Compressed& rows, bool lower, bool unit, double alpha,
+                             const int64_t* level_ptr, const uint32_t* row_order,
+                             const TrsvLevels& levels, void* x, int64_t rs, int64_t cs,
+                             int64_t k, hipStream_t stream);
+hipError_t dense_zero_launch(Elem elem, void* x, int64_t rows, int64_t cols, int64_t rs,
+                             int64_t cs, hipStream_t stream);
+
 // Exclusive prefix sum of n uint32 values; `scratch` holds
 // scan_u32_scratch_bytes(n).
 size_t scan_u32_scratch_bytes(int64_t n);

@@ -410,6 +410,62 @@ asc_hip_status asc_sparse_hip_standard_level1(asc_hip_context* context, uint8_t 
                                               const asc_hip_vector_desc* result,
                                               const asc_hip_spmv_options* options);
 
+/* ---- triangular solve ---------------------------------------------------- */
+
+/* asc::SparseBlasTriangle / SparseBlasDiagonal (blas.h:117-134). */
+enum { ASC_HIP_UPPER = 0, ASC_HIP_LOWER = 1 };
+enum { ASC_HIP_NON_UNIT = 0, ASC_HIP_UNIT = 1 };
+
+/* What an analysis describes; the solve checks it against its arguments. */
+typedef struct asc_hip_triangular_info {
+  int64_t rows;
+  int64_t nnz;
+  int64_t levels;          /* solve launches: rows of one level run in parallel */
+  int64_t max_level_rows;
+  const void* offsets;     /* the analysed matrix's arrays */
+  const void* indices;
+  const void* values;
+  uint8_t transpose;
+  uint8_t triangle;
+  uint8_t diagonal;
+  uint8_t format;
+  uint8_t reserved[4];
+} asc_hip_triangular_info;
+
+/*
+ * Level-scheduled sparse triangular solve, the device form of
+ * SparseTriangularSolve / SparseTriangularSolveMultiple (blas.h:1760-1880);
+ * replaces StandardTriangularSolveErased (providers/cuda.h:250-253).  No
+ * reference counterpart for the split into analysis and solve.
+ *  - analyze (synchronous) checks what SparseBlasTriangularView::Create
+ *    checks (blas.h:1090-1160: square, every entry inside the declared
+ *    triangle, a stored nonzero diagonal when non-unit; ASC_HIP_SHAPE /
+ *    ASC_HIP_INVALID_ARGUMENT otherwise), then levels the rows of op(A) into
+ *    the caller's analysis buffer (with a transposed copy of A for CSR with
+ *    op T/H and CSC with op N).  The buffer must stay unmodified while it is
+ *    used.
+ *  - solve overwrites the n x k right-hand sides with
+ *    alpha * op(A)^-1 * B: one launch per level, each (row, column) a serial
+ *    chain over the row's stored entries in column order without FMA.  That
+ *    equals the reference's dense loop bit for bit while the solution stays
+ *    finite.  alpha == 0 zeroes the right-hand sides.
+ */
+asc_hip_status asc_sparse_hip_triangular_analysis_size(asc_hip_context* context,
+                                                       uint8_t transpose,
+                                                       const asc_hip_sparse_desc* matrix,
+                                                       size_t* out_bytes);
+asc_hip_status asc_sparse_hip_triangular_analyze(asc_hip_context* context, uint8_t transpose,
+                                                 uint8_t triangle, uint8_t diagonal,
+                                                 const asc_hip_sparse_desc* matrix,
+                                                 void* analysis, size_t analysis_bytes,
+                                                 asc_hip_triangular_info* out_info);
+asc_hip_status asc_sparse_hip_triangular_solve(asc_hip_context* context, uint8_t transpose,
+                                               asc_hip_scalar alpha,
+                                               const asc_hip_sparse_desc* matrix,
+                                               const asc_hip_matrix_desc* right_hand_sides,
+                                               const void* analysis, size_t analysis_bytes,
+                                               const asc_hip_triangular_info* info);
+
 /* ---- SpMM --------------------------------------------------------------- */
 
 /*

@@ -453,3 +453,56 @@ ORACLE_FINALIZE_BODY(oracle_coo_finalize_f32, float)
 
 ORACLE_LEVEL1(f64, double)
 ORACLE_LEVEL1(f32, float)
+
+/* ------------------------------------------------------------------------ */
+/* SparseTriangularSolve(Multiple), blas.h:1760-1880: in place on k          */
+/* right-hand sides (element (r, j) at b[r*rs + j*cs]).  alpha == 0 zeroes  */
+/* the right-hand sides.  lower = transposed ? triangle == upper :          */
+/* triangle == lower; rows run forward (lower) or backward (upper); each    */
+/* row subtracts op(A)(row, col)*x(col) for every col strictly before it in */
+/* the solve order, in increasing col order, as the reference's dense loop  */
+/* does (unstored entries contribute 0*x), then divides by the diagonal     */
+/* when non-unit.  op(A)(r, c) is looked up in the stored CSR/CSC arrays.   */
+/* ------------------------------------------------------------------------ */
+#define ORACLE_TRSV(SUFFIX, T)                                                   \
+  static T op_value_##SUFFIX(int fmt, int op, const int64_t* off,               \
+                             const int64_t* idx, const T* val, int64_t r,       \
+                             int64_t c) {                                       \
+    /* op(A)(r, c) = A(r, c) for N, A(c, r) for T/H (real: no conjugation) */   \
+    const int64_t ar = op == ORACLE_OP_N ? r : c;                                \
+    const int64_t ac = op == ORACLE_OP_N ? c : r;                                \
+    const int64_t outer = fmt == ORACLE_CSR ? ar : ac;                           \
+    const int64_t inner = fmt == ORACLE_CSR ? ac : ar;                           \
+    for (int64_t p = off[outer]; p < off[outer + 1]; ++p)                        \
+      if (idx[p] == inner) return val[p];                                        \
+    return (T)0;                                                                 \
+  }                                                                              \
+  void oracle_trsv_##SUFFIX(int fmt, int op, int lower_triangle, int unit,      \
+                            int64_t n, const int64_t* off, const int64_t* idx,  \
+                            const T* val, T alpha, T* b, int64_t rs, int64_t cs, \
+                            int64_t k) {                                        \
+    if (alpha == (T)0) {                                                         \
+      for (int64_t j = 0; j < k; ++j)                                            \
+        for (int64_t r = 0; r < n; ++r) b[r * rs + j * cs] = (T)0;               \
+      return;                                                                    \
+    }                                                                            \
+    const int transposed = op != ORACLE_OP_N;                                    \
+    const int lower = transposed ? !lower_triangle : lower_triangle;             \
+    for (int64_t j = 0; j < k; ++j) {                                            \
+      for (int64_t step = 0; step < n; ++step) {                                 \
+        const int64_t row = lower ? step : n - 1 - step;                         \
+        T value = alpha * b[row * rs + j * cs];                                  \
+        for (int64_t col = 0; col < n; ++col) {                                  \
+          if ((lower && col >= row) || (!lower && col <= row)) continue;         \
+          const T prod = op_value_##SUFFIX(fmt, op, off, idx, val, row, col) *   \
+                         b[col * rs + j * cs];                                   \
+          value -= prod;                                                         \
+        }                                                                        \
+        if (!unit) value /= op_value_##SUFFIX(fmt, op, off, idx, val, row, row); \
+        b[row * rs + j * cs] = value;                                            \
+      }                                                                          \
+    }                                                                            \
+  }
+
+ORACLE_TRSV(f64, double)
+ORACLE_TRSV(f32, float)

@@ -31,6 +31,8 @@
  *                           order is unique) and to oracle/_ref.
  *   oracle_sparse_*         golden vectors of standard_blas_test.cc:69-174,
  *                           exact integer brute force, and oracle/_ref.
+ *   oracle_trsv_*           golden vectors of standard_blas_test.cc:335-430
+ *                           and oracle/_ref.
  *   oracle_coo_finalize_*   golden vectors of coordinate_test.cc and a
  *                           Python sorted()-based brute force.
  */
@@ -182,6 +184,16 @@ void oracle_sparse_scatter_f64(int64_t nnz, const int64_t* idx, const double* v,
                                double* x, int64_t inc);
 void oracle_sparse_scatter_f32(int64_t nnz, const int64_t* idx, const float* v,
                                float* x, int64_t inc);
+
+/* SparseTriangularSolve(Multiple), blas.h:1760-1880, in place on k
+ * right-hand sides (element (r, j) at b[r*rs + j*cs]).  lower_triangle: the
+ * declared triangle is lower; unit: implicit unit diagonal. */
+void oracle_trsv_f64(int fmt, int op, int lower_triangle, int unit, int64_t n,
+                     const int64_t* off, const int64_t* idx, const double* val,
+                     double alpha, double* b, int64_t rs, int64_t cs, int64_t k);
+void oracle_trsv_f32(int fmt, int op, int lower_triangle, int unit, int64_t n,
+                     const int64_t* off, const int64_t* idx, const float* val,
+                     float alpha, float* b, int64_t rs, int64_t cs, int64_t k);
 
 #ifdef __cplusplus
 }

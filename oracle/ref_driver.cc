@@ -376,3 +376,59 @@ int ref_level1_f32(int op, int64_t nnz, int64_t extent, const int64_t* idx, floa
 }
 
 }  // extern "C"
+
+// ---- triangular solve: the reference's SparseTriangularSolveMultiple
+// (blas.h:1823-1880) on host arrays; b is n x k, row-major (ld = k) when
+// row_major, else column-major (ld = n).
+namespace {
+
+template <typename T, asc::SparseCompressedFormat F>
+int RefTrsv(int op, int lower, int unit, int64_t n, int64_t nnz, const int64_t* off,
+            const int64_t* idx, const T* val, T alpha, T* b, int64_t k, int row_major) {
+  const std::array<asc::extent_t, 2> shape{n, n};
+  auto m = asc::CompressedSparseView<const T, F>::Create(off, idx, val, shape, nnz,
+                                                         asc::MemorySpace::kHost);
+  if (!m.ok()) return static_cast<int>(m.status().code());
+  auto tri = asc::SparseBlasTriangularView<T, F>::Create(
+      *m, lower ? asc::SparseBlasTriangle::kLower : asc::SparseBlasTriangle::kUpper,
+      unit ? asc::SparseBlasDiagonal::kUnit : asc::SparseBlasDiagonal::kNonUnit);
+  if (!tri.ok()) return static_cast<int>(tri.status().code());
+  const asc::MutableMemoryView storage{b, sizeof(T) * static_cast<std::size_t>(n * k),
+                                       asc::MemorySpace::kHost};
+  auto rhs = asc::SparseBlasMatrixView<T>::Create(
+      b, n, k, row_major ? asc::SparseBlasLayout::kRowMajor : asc::SparseBlasLayout::kColumnMajor,
+      row_major ? (k > 0 ? k : 1) : (n > 0 ? n : 1), storage);
+  if (!rhs.ok()) return static_cast<int>(rhs.status().code());
+  const auto t = op == 0 ? asc::SparseBlasTranspose::kNone
+                         : (op == 1 ? asc::SparseBlasTranspose::kTranspose
+                                    : asc::SparseBlasTranspose::kConjugateTranspose);
+  return static_cast<int>(
+      asc::SparseTriangularSolveMultiple(asc::ExecutionContext::Serial(), t, alpha, *tri, *rhs)
+          .code());
+}
+
+}  // namespace
+
+extern "C" {
+
+int ref_trsv_f64(int fmt, int op, int lower, int unit, int64_t n, int64_t nnz, const int64_t* off,
+                 const int64_t* idx, const double* val, double alpha, double* b, int64_t k,
+                 int row_major) {
+  if (fmt == 0)
+    return RefTrsv<double, asc::SparseCompressedFormat::kCsr>(op, lower, unit, n, nnz, off, idx,
+                                                              val, alpha, b, k, row_major);
+  return RefTrsv<double, asc::SparseCompressedFormat::kCsc>(op, lower, unit, n, nnz, off, idx, val,
+                                                            alpha, b, k, row_major);
+}
+
+int ref_trsv_f32(int fmt, int op, int lower, int unit, int64_t n, int64_t nnz, const int64_t* off,
+                 const int64_t* idx, const float* val, float alpha, float* b, int64_t k,
+                 int row_major) {
+  if (fmt == 0)
+    return RefTrsv<float, asc::SparseCompressedFormat::kCsr>(op, lower, unit, n, nnz, off, idx, val,
+                                                             alpha, b, k, row_major);
+  return RefTrsv<float, asc::SparseCompressedFormat::kCsc>(op, lower, unit, n, nnz, off, idx, val,
+                                                           alpha, b, k, row_major);
+}
+
+}  // extern "C"

@@ -44,6 +44,8 @@ def lib():
             getattr(L, f"oracle_sparse_axpy_{sfx}").argtypes = [I64, ct, VP, VP, VP, I64]
             for op in ("gather", "gather_zero", "scatter"):
                 getattr(L, f"oracle_sparse_{op}_{sfx}").argtypes = [I64, VP, VP, VP, I64]
+            getattr(L, f"oracle_trsv_{sfx}").argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, I64, VP,
+                                                         VP, VP, ct, VP, I64, I64, I64]
         _lib = L
     return _lib
 
@@ -167,6 +169,8 @@ def ref():
         R.ref_finalize_f32.argtypes = [I64, I64, I64, VP, VP, C.c_int, C.c_int, VP, VP, VP]
         R.ref_level1_f64.argtypes = [C.c_int, I64, I64, VP, VP, VP, I64, C.c_double, VP]
         R.ref_level1_f32.argtypes = [C.c_int, I64, I64, VP, VP, VP, I64, C.c_float, VP]
+        R.ref_trsv_f64.argtypes = [C.c_int] * 4 + [I64, I64, VP, VP, VP, C.c_double, VP, I64, C.c_int]
+        R.ref_trsv_f32.argtypes = [C.c_int] * 4 + [I64, I64, VP, VP, VP, C.c_float, VP, I64, C.c_int]
         _ref = R
     return _ref
 
@@ -274,3 +278,30 @@ def ref_level1(op, idx, v, x, extent, inc=1, alpha=1.0):
     fn = ref().ref_level1_f32 if dt == np.float32 else ref().ref_level1_f64
     rc = fn(LEVEL1_OPS.index(op), idx.shape[0], extent, _p(idx), _p(v), _p(x), inc, alpha, _p(dot))
     return rc, (dot[0] if op == "dot" else None), v, x
+
+
+# ---- triangular solve ------------------------------------------------------------
+def trsv(fmt, op, lower, unit, n, offsets, indices, values, alpha, b):
+    """Oracle triangular solve on a copy of b (n x k row-major, or a vector);
+    fmt 0 CSR / 1 CSC, op 0 N / 1 T / 2 H."""
+    offsets, indices = i64(offsets), i64(indices)
+    dt = np.float32 if values.dtype == np.float32 else np.float64
+    values = np.ascontiguousarray(values, dtype=dt)
+    out = np.array(b, dtype=dt, copy=True, order="C")
+    k = 1 if out.ndim == 1 else out.shape[1]
+    fn = lib().oracle_trsv_f32 if dt == np.float32 else lib().oracle_trsv_f64
+    fn(fmt, op, 1 if lower else 0, 1 if unit else 0, n, _p(offsets), _p(indices), _p(values), alpha,
+       _p(out), k, 1, k)
+    return out
+
+
+def ref_trsv(fmt, op, lower, unit, n, offsets, indices, values, alpha, b):
+    offsets, indices = i64(offsets), i64(indices)
+    dt = np.float32 if values.dtype == np.float32 else np.float64
+    values = np.ascontiguousarray(values, dtype=dt)
+    out = np.array(b, dtype=dt, copy=True, order="C")
+    k = 1 if out.ndim == 1 else out.shape[1]
+    fn = ref().ref_trsv_f32 if dt == np.float32 else ref().ref_trsv_f64
+    rc = fn(fmt, op, 1 if lower else 0, 1 if unit else 0, n, indices.shape[0], _p(offsets),
+            _p(indices), _p(values), alpha, _p(out), k, 1)
+    return rc, out

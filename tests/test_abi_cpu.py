@@ -342,3 +342,40 @@ def test_level1_validation(vctx):
     idx3 = np.array([0, 2, 4], np.int64)
     src = H.IndexedDesc(idx3.ctypes.data, vals.ctypes.data, 3, 5, H.F64, H.HOST, 0, 0)
     assert lib.asc_sparse_hip_clone_indexed(h, C.byref(src), FAKE, FAKE + 0x100, C.byref(out)).code == H.UNAVAILABLE
+
+
+def test_triangular_validation(vctx):
+    lib, h = vctx.lib, vctx.handle
+    need = C.c_size_t()
+    sq = dev_csr(rows=4, cols=4, nnz=6)
+    assert lib.asc_sparse_hip_triangular_analysis_size(h, H.OP_N, C.byref(sq), C.byref(need)).code == H.OK
+    n_direct = need.value
+    assert lib.asc_sparse_hip_triangular_analysis_size(h, H.OP_T, C.byref(sq), C.byref(need)).code == H.OK
+    assert need.value > n_direct  # op T on CSR keeps a transposed copy
+    assert lib.asc_sparse_hip_triangular_analysis_size(h, 3, C.byref(sq), C.byref(need)).code == H.INVALID_ARGUMENT
+    assert lib.asc_sparse_hip_triangular_analysis_size(h, H.OP_N, C.byref(dev_csr()), C.byref(need)).code == H.SHAPE
+    assert lib.asc_sparse_hip_triangular_analysis_size(
+        h, H.OP_N, C.byref(dev_csr(rows=4, cols=4, nnz=6, kind=H.BF16)), C.byref(need)).code == H.UNSUPPORTED
+    info = H.TriangularInfo()
+    ws = 0x4000000
+    lib.asc_sparse_hip_triangular_analysis_size(h, H.OP_N, C.byref(sq), C.byref(need))
+    assert lib.asc_sparse_hip_triangular_analyze(h, H.OP_N, 2, 0, C.byref(sq), ws, need.value,
+                                                 C.byref(info)).code == H.INVALID_ARGUMENT
+    assert lib.asc_sparse_hip_triangular_analyze(h, H.OP_N, 1, 2, C.byref(sq), ws, need.value,
+                                                 C.byref(info)).code == H.INVALID_ARGUMENT
+    assert lib.asc_sparse_hip_triangular_analyze(h, H.OP_N, 1, 0, C.byref(sq), ws, need.value - 8,
+                                                 C.byref(info)).code == H.INVALID_ARGUMENT
+    assert lib.asc_sparse_hip_triangular_analyze(h, H.OP_N, 1, 0, C.byref(sq), ws, need.value,
+                                                 C.byref(info)).code == H.UNAVAILABLE
+    # a solve with an info that does not describe this matrix
+    rhs = H.MatrixDesc(0x6000000, 4, 1, 1, H.ROW_MAJOR, H.F64, H.DEVICE, 1)
+    bogus = H.TriangularInfo(4, 6, 1, 4)
+    assert lib.asc_sparse_hip_triangular_solve(h, H.OP_N, H.Scalar(1.0, 0.0), C.byref(sq), C.byref(rhs),
+                                               ws, need.value, C.byref(bogus)).code == H.INVALID_ARGUMENT
+    good = H.TriangularInfo(4, 6, 1, 4, sq.outer_offsets, sq.inner_indices, sq.values, H.OP_N, 1, 0, H.CSR)
+    assert lib.asc_sparse_hip_triangular_solve(h, H.OP_N, H.Scalar(1.0, 0.0), C.byref(sq), C.byref(rhs),
+                                               ws, need.value, C.byref(good)).code == H.UNAVAILABLE
+    wrong_rows = H.MatrixDesc(0x6000000, 3, 1, 1, H.ROW_MAJOR, H.F64, H.DEVICE, 1)
+    assert lib.asc_sparse_hip_triangular_solve(h, H.OP_N, H.Scalar(1.0, 0.0), C.byref(sq),
+                                               C.byref(wrong_rows), ws, need.value,
+                                               C.byref(good)).code == H.SHAPE

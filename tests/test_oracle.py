@@ -335,3 +335,37 @@ def test_level1_dot_exact_integers():
     v = rng.integers(-50, 50, 700).astype(np.float64)
     x = rng.integers(-50, 50, 5000).astype(np.float64)
     assert O.level1("dot", idx, v, x)[0] == float(np.dot(v.astype(np.int64), x[idx].astype(np.int64)))
+
+
+def test_triangular_golden_vectors():
+    g = GOLD["triangular"]
+    vals = np.array(g["values"])
+    x = O.trsv(0, 0, True, False, 2, g["csr_offsets"], g["csr_indices"], vals, 1.0, g["lower_n_rhs"])
+    np.testing.assert_array_equal(x, g["lower_n_solution"])
+    # CSC storage of the same lower matrix, conjugate transpose (values are the
+    # CSC order of the same entries: (0,0)=2, (1,0)=1, (1,1)=3)
+    x = O.trsv(1, 2, True, False, 2, g["csc_offsets"], g["csc_indices"], vals, 1.0, g["csc_lower_h_rhs"])
+    np.testing.assert_array_equal(x, g["csc_lower_h_solution"])
+    b = np.array(g["multiple_rhs_row_major"]).reshape(2, 2)
+    x = O.trsv(0, 0, True, False, 2, g["csr_offsets"], g["csr_indices"], vals, 1.0, b)
+    np.testing.assert_array_equal(x.reshape(-1), g["multiple_solution"])
+    zv = np.array([float(t) for t in g["alpha_zero_values"]])
+    x = O.trsv(0, 0, True, False, 2, g["csr_offsets"], g["csr_indices"], zv, 0.0, g["alpha_zero_rhs"])
+    np.testing.assert_array_equal(x, g["alpha_zero_solution"])
+
+
+def test_triangular_solves_recover_known_solutions():
+    # build L with a dominant diagonal, pick x, form b = alpha^-1 * op(L) x in
+    # exact integer arithmetic, and solve back
+    rng = np.random.default_rng(2)
+    n = 60
+    dense = np.tril(rng.integers(-3, 4, (n, n)) * (rng.random((n, n)) < 0.2), -1).astype(np.float64)
+    dense[np.arange(n), np.arange(n)] = 1.0  # unit-valued diagonal keeps every step exact
+    off = np.concatenate([[0], np.cumsum((dense != 0).sum(1))])
+    idx = np.concatenate([np.nonzero(r)[0] for r in dense])
+    val = dense[dense != 0]
+    x = rng.integers(-5, 6, n).astype(np.float64)
+    for op, mat in ((0, dense), (1, dense.T)):
+        b = mat @ x
+        np.testing.assert_array_equal(O.trsv(0, op, True, False, n, off, idx, val, 1.0, b), x)
+        np.testing.assert_array_equal(O.trsv(0, op, True, True, n, off, idx, val, 1.0, b), x)

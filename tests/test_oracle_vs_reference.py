@@ -142,3 +142,26 @@ def test_level1_matches_reference(op, dtype, seed):
         assert np.asarray([a[0]], dtype).tobytes() == np.asarray([dot], dtype).tobytes()
     np.testing.assert_array_equal(a[1].view(np.uint8), rv.view(np.uint8))
     np.testing.assert_array_equal(a[2].view(np.uint8), rx.view(np.uint8))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("fmt", [0, 1])
+@pytest.mark.parametrize("op", [0, 1, 2])
+@pytest.mark.parametrize("lower,unit", [(True, False), (False, False), (True, True), (False, True)])
+def test_triangular_matches_reference(dtype, fmt, op, lower, unit):
+    rng = np.random.default_rng(17 + fmt * 7 + op * 3 + lower * 2 + unit)
+    n, k = 40, 3
+    dense = rng.standard_normal((n, n)) * (rng.random((n, n)) < 0.15)
+    dense = np.tril(dense, -1) if lower else np.triu(dense, 1)
+    dense[np.arange(n), np.arange(n)] = rng.uniform(1.0, 2.0, n) * rng.choice([-1, 1], n)
+    dense = dense.astype(dtype)
+    src = dense if fmt == 0 else dense.T  # CSC of A = CSR of A^T
+    off = np.concatenate([[0], np.cumsum((src != 0).sum(1))])
+    idx = np.concatenate([np.nonzero(r)[0] for r in src])
+    val = src[src != 0].astype(dtype)
+    b = rng.standard_normal((n, k)).astype(dtype)
+    for alpha in (1.0, -0.5, 0.0):
+        rc, ref = O.ref_trsv(fmt, op, lower, unit, n, off, idx, val, alpha, b)
+        assert rc == 0
+        got = O.trsv(fmt, op, lower, unit, n, off, idx, val, alpha, b)
+        np.testing.assert_array_equal(got.view(np.uint8), ref.view(np.uint8))
