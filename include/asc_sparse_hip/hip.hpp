@@ -1009,6 +1009,77 @@ template <typename Element>
       HipSpmvAlgorithm::kAuto);
 }
 
+// ---- triangular solve (blas.h:1760-1880; CudaSparseTriangularSolve,
+// providers/cuda.h:1680-1756) -------------------------------------------------
+
+// A triangular matrix with its level schedule for one operation: the device
+// form of SparseBlasTriangularView (blas.h:1074-1160) plus the analysis.
+// Analyze checks what the view's Create checks; the matrix's arrays and the
+// analysis must stay unmodified while the plan is used.
+template <typename Element, SparseCompressedFormat Format>
+  requires internal_sparse_hip::kSupportedElement<Element>
+class HipTriangularPlan {
+ public:
+  static Result<HipTriangularPlan> Analyze(SparseHipContext& context,
+                                           CompressedSparseView<const Element, Format> matrix,
+                                           SparseBlasTriangle triangle, SparseBlasDiagonal diagonal,
+                                           SparseBlasTranspose transpose,
+                                           const HipMemoryResource& resource) {
+    HipTriangularPlan p;
+    p.desc_ = HipProviderAccess::Describe(matrix);
+    p.transpose_ = internal_sparse_hip::Op(transpose);
+    std::size_t bytes = 0;
+    Status s = internal_sparse_hip::ToStatus(asc_sparse_hip_triangular_analysis_size(
+        context.native(), p.transpose_, &p.desc_, &bytes));
+    if (!s.ok()) return s;
+    auto buf = HipBuffer::Allocate(resource, bytes);
+    if (!buf.ok()) return buf.status();
+    p.analysis_ = *buf;
+    s = internal_sparse_hip::ToStatus(asc_sparse_hip_triangular_analyze(
+        context.native(), p.transpose_,
+        triangle == SparseBlasTriangle::kLower ? ASC_HIP_LOWER : ASC_HIP_UPPER,
+        diagonal == SparseBlasDiagonal::kUnit ? ASC_HIP_UNIT : ASC_HIP_NON_UNIT, &p.desc_,
+        p.analysis_.data(), p.analysis_.size(), &p.info_));
+    if (!s.ok()) return s;
+    return p;
+  }
+  [[nodiscard]] std::int64_t levels() const noexcept { return info_.levels; }
+
+  // right_hand_sides := alpha * op(A)^-1 * right_hand_sides
+  // (SparseTriangularSolveMultiple; a vector is the one-column case).
+  [[nodiscard]] Result<CompletionEvent> Solve(SparseHipContext& context, Element alpha,
+                                              SparseBlasMatrixView<Element> right_hand_sides) const {
+    const asc_hip_matrix_desc b = internal_sparse_hip::Describe(right_hand_sides);
+    const asc_hip_scalar a{static_cast<double>(alpha), 0.0};
+    return internal_sparse_hip::Enqueued(
+        context, asc_sparse_hip_triangular_solve(context.native(), transpose_, a, &desc_, &b,
+                                                 analysis_.data(), analysis_.size(), &info_));
+  }
+  [[nodiscard]] Result<CompletionEvent> Solve(SparseHipContext& context, Element alpha,
+                                              SparseBlasVectorView<Element> right_hand_side) const {
+    if (right_hand_side.increment() <= 0)
+      return Status(ErrorCode::kUnsupported, "the device solve takes a positive increment");
+    auto m = SparseBlasMatrixView<Element>::Create(
+        right_hand_side.data(), right_hand_side.size(), 1, SparseBlasLayout::kRowMajor,
+        right_hand_side.increment(),
+        {right_hand_side.data(),
+         sizeof(Element) * static_cast<std::size_t>(right_hand_side.size() > 0
+                                                        ? (right_hand_side.size() - 1) *
+                                                                  right_hand_side.increment() +
+                                                              1
+                                                        : 0),
+         right_hand_side.memory_space()});
+    if (!m.ok()) return m.status();
+    return Solve(context, alpha, *m);
+  }
+
+ private:
+  asc_hip_sparse_desc desc_{};
+  asc_hip_triangular_info info_{};
+  std::uint8_t transpose_ = ASC_HIP_OP_N;
+  HipBuffer analysis_;
+};
+
 // Stream-ordered copies between host and device memory (CopyBytes,
 // src/core/cuda/runtime.cc:226-253).
 [[nodiscard]] inline Result<CompletionEvent> HipCopy(SparseHipContext& context, void* dst,

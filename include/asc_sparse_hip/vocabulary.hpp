@@ -301,8 +301,10 @@ class SparseBlasVectorView {
   MemorySpace memory_space_ = MemorySpace::kHost;
 };
 
-// include/asc/sparse/blas.h:74-77
+// include/asc/sparse/blas.h:74-77, 117-134
 enum class SparseBlasConjugation : std::uint8_t { kUnconjugated, kConjugated };
+enum class SparseBlasTriangle : std::uint8_t { kUpper, kLower };
+enum class SparseBlasDiagonal : std::uint8_t { kNonUnit, kUnit };
 
 // Canonical indexed (sparse) vector (blas.h:526-789): nnz strictly
 // increasing indices in [0, dense_extent) and their values.  Host views are

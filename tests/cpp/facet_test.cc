@@ -461,6 +461,42 @@ void TestLevelOne(Gpu& g) {
   CHECK_CODE(asc::HipSparseAxpy(g.ctx, E(1), *untrusted, *x).status(), ErrorCode::kInvalidArgument);
 }
 
+// standard_blas_test.cc:335-398 (real parts): A = [[2, 0], [1, 3]] lower.
+template <typename E>
+void TestTriangular(Gpu& g) {
+  Case(sizeof(E) == 8 ? "triangular solve golden (double)" : "triangular solve golden (float)");
+  auto a = CloneCsr<E>(g, {0, 1, 3}, {0, 0, 1}, {2, 1, 3}, 2, 2);
+  const asc::CsrView<const E> cv = *a.view();
+  auto plan = asc::HipTriangularPlan<E, asc::SparseCompressedFormat::kCsr>::Analyze(
+      g.ctx, cv, asc::SparseBlasTriangle::kLower, asc::SparseBlasDiagonal::kNonUnit,
+      asc::SparseBlasTranspose::kNone, g.device);
+  CHECK(plan.ok());
+  if (!plan.ok()) return;
+  CHECK(plan->levels() == 2);
+  auto xb = Upload<E>(g, {4, -1});
+  const asc::ConstMemoryView xs{xb.data(), 2 * sizeof(E), MemorySpace::kDevice};
+  auto x = asc::SparseBlasVectorView<E>::Create(static_cast<E*>(xb.data()), 2, 1, xs);
+  auto done = plan->Solve(g.ctx, E(1), *x);
+  CHECK(done.ok() && done->Wait().ok());
+  CHECK((Download<E>(g, xb, 2) == std::vector<E>{2, -1}));
+  // Several right-hand sides, row-major: [[4, 2], [-1, 10]] -> [[2, 1], [-1, 3]].
+  auto bb = Upload<E>(g, {4, 2, -1, 10});
+  const asc::ConstMemoryView bs{bb.data(), 4 * sizeof(E), MemorySpace::kDevice};
+  auto bm = asc::SparseBlasMatrixView<E>::Create(static_cast<E*>(bb.data()), 2, 2,
+                                                 asc::SparseBlasLayout::kRowMajor, 2, bs);
+  done = plan->Solve(g.ctx, E(1), *bm);
+  CHECK(done.ok() && done->Wait().ok());
+  CHECK((Download<E>(g, bb, 4) == std::vector<E>{2, 1, -1, 3}));
+  // A singular non-unit diagonal is refused.
+  auto s = CloneCsr<E>(g, {0, 1, 2}, {0, 1}, {1, 0}, 2, 2);
+  const asc::CsrView<const E> sv = *s.view();
+  CHECK_CODE((asc::HipTriangularPlan<E, asc::SparseCompressedFormat::kCsr>::Analyze(
+                  g.ctx, sv, asc::SparseBlasTriangle::kLower, asc::SparseBlasDiagonal::kNonUnit,
+                  asc::SparseBlasTranspose::kNone, g.device)
+                  .status()),
+             ErrorCode::kInvalidArgument);
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -492,6 +528,8 @@ int main(int argc, char** argv) {
     TestFinalize(g);
     TestLevelOne<double>(g);
     TestLevelOne<float>(g);
+    TestTriangular<double>(g);
+    TestTriangular<float>(g);
   }
   std::printf("%d checks, %d failures\n", g_checks, g_failures);
   return g_failures == 0 ? 0 : 1;

@@ -39,7 +39,7 @@ def solve(ctx, fmt, op, lower, unit, n, off, idx, val, alpha, b, layout=H.ROW_MA
     H.triangular_solve(ctx, OPS[op], alpha, a, rhs, plan)
     out = ctx.download(buf, dtype, n * k)
     out = out.reshape(n, k) if layout == H.ROW_MAJOR else out.reshape(k, n).T
-    return out.reshape(np.shape(b)), plan
+    return np.ascontiguousarray(out.reshape(np.shape(b))), plan
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
@@ -90,7 +90,7 @@ def test_random_against_oracle(ctx, dtype, fmt, op, lower, unit):
         for alpha in (1.0, -0.75):
             ref = O.trsv(fmt, op, lower, unit, n, off, idx, val, alpha, b)
             got, plan = solve(ctx, fmt, op, lower, unit, n, off, idx, val, alpha, b, layout)
-            np.testing.assert_array_equal(got.view(np.uint8), ref.view(np.uint8))
+            np.testing.assert_array_equal(got.view(np.uint8), np.ascontiguousarray(ref).view(np.uint8))
             assert 1 <= plan.info.levels <= n
 
 
