@@ -119,6 +119,7 @@ _SIGS = {
     "asc_sparse_hip_event_destroy": [_VP],
     "asc_sparse_hip_clone_compressed": [_VP, P(SparseDesc), C.c_uint8, _VP, _VP, _VP, P(SparseDesc)],
     "asc_sparse_hip_clone_coordinate": [_VP, P(SparseDesc), _VP, _VP, P(SparseDesc)],
+    "asc_sparse_hip_rebind_values": [P(SparseDesc), _VP, P(SparseDesc)],
     "asc_sparse_hip_narrow_indices": [_VP, P(SparseDesc), _VP, P(SparseDesc)],
     "asc_sparse_hip_convert_values": [_VP, P(SparseDesc), C.c_uint8, _VP, P(SparseDesc)],
     "asc_sparse_hip_convert_values_scaled": [_VP, P(SparseDesc), C.c_uint8, C.c_double, _VP, P(SparseDesc)],
@@ -345,6 +346,18 @@ def clone_compressed(ctx: Context, fmt: int, rows: int, cols: int, offsets: np.n
     check(ctx.lib.asc_sparse_hip_clone_compressed(ctx.handle, C.byref(src), index_bytes,
                                                   bo.ptr, bi.ptr, bv.ptr, C.byref(out)))
     return DeviceSparse(out, [bo, bi, bv])
+
+
+def rebind_values(a: DeviceSparse, values) -> DeviceSparse:
+    """The same trusted structure over other device values (RebindValues,
+    compressed.h:472-516); `values` is a Buffer, kept alive by the result."""
+    out = SparseDesc()
+    ptr = values.ptr if hasattr(values, "ptr") else values
+    check(load().asc_sparse_hip_rebind_values(C.byref(a.desc), ptr, C.byref(out)))
+    r = DeviceSparse(out, [b for b in a.buffers] + ([values] if hasattr(values, "ptr") else []))
+    if getattr(a, "analysis", None) is not None:
+        r.analysis = a.analysis
+    return r
 
 
 def clone_coordinate(ctx: Context, rows: int, cols: int, coords: np.ndarray,

@@ -66,8 +66,9 @@ bool checked_mul(int64_t a, int64_t b, int64_t* out);
 size_t elem_size(uint8_t kind);
 
 asc_hip_status check_context(const asc_hip_context* ctx);
+// With bytes > 0, [p, p + bytes) must also lie inside one device allocation.
 asc_hip_status check_device_pointer(const asc_hip_context* ctx, const void* p,
-                                    const char* what);
+                                    const char* what, size_t bytes = 0);
 asc_hip_status check_sparse_meta(const asc_hip_sparse_desc* d, const char* what);
 asc_hip_status check_device_sparse(const asc_hip_context* ctx,
                                    const asc_hip_sparse_desc* d, const char* what);

@@ -142,9 +142,11 @@ bool sparse_overlaps(const asc_hip_sparse_desc& d, Span out) {
   return overlaps(s.first, out) || overlaps(s.second, out) || overlaps(s.values, out);
 }
 
-// The pointer must be device (or managed) memory of the context's device.
+// The pointer must be device (or managed) memory of the context's device;
+// for device memory, bytes > 0 must also fit the allocation holding p
+// (placement checked before enqueue: sparse.md:119-125, 455-466).
 asc_hip_status check_device_pointer(const asc_hip_context* ctx, const void* p,
-                                    const char* what) {
+                                    const char* what, size_t bytes) {
   if (ctx->device < 0) return ok();  // validation-only context
   if (p == nullptr) return fail(ASC_HIP_INVALID_ARGUMENT, std::string(what) + " is null");
   hipPointerAttribute_t attr;
@@ -162,6 +164,21 @@ asc_hip_status check_device_pointer(const asc_hip_context* ctx, const void* p,
   if (attr.type == hipMemoryTypeDevice && attr.device != ctx->device) {
     return fail(ASC_HIP_MEMORY_ACCESS,
                 std::string(what) + " belongs to another device");
+  }
+  if (attr.type == hipMemoryTypeDevice && bytes > 0) {
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    const hipError_t r = hipMemGetAddressRange(&base, &size, const_cast<void*>(p));
+    if (r != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(ASC_HIP_MEMORY_ACCESS,
+                  std::string(what) + ": its allocation cannot be determined");
+    }
+    const uintptr_t b = reinterpret_cast<uintptr_t>(base);
+    const uintptr_t q = reinterpret_cast<uintptr_t>(p);
+    if (q < b || bytes > size - (q - b))
+      return fail(ASC_HIP_MEMORY_ACCESS,
+                  std::string(what) + " extends past the end of its allocation");
   }
   return ok();
 }
@@ -206,10 +223,15 @@ asc_hip_status check_device_sparse(const asc_hip_context* ctx, const asc_hip_spa
     return fail(ASC_HIP_MEMORY_ACCESS, std::string(what) + ": operand is not device memory");
   if (!d->trusted)
     return fail(ASC_HIP_INVALID_ARGUMENT, std::string(what) + ": operations require trusted canonical structure");
-  if ((st = check_device_pointer(ctx, d->outer_offsets, what)).code) return st;
+  const SparseSpans s = sparse_spans(*d);
+  if ((st = check_device_pointer(ctx, d->outer_offsets, what, s.first.end - s.first.begin)).code)
+    return st;
   if (d->nnz > 0) {
-    if (d->format != ASC_HIP_COO && (st = check_device_pointer(ctx, d->inner_indices, what)).code) return st;
-    if ((st = check_device_pointer(ctx, d->values, what)).code) return st;
+    if (d->format != ASC_HIP_COO &&
+        (st = check_device_pointer(ctx, d->inner_indices, what, s.second.end - s.second.begin)).code)
+      return st;
+    if ((st = check_device_pointer(ctx, d->values, what, s.values.end - s.values.begin)).code)
+      return st;
   }
   return ok();
 }
@@ -226,6 +248,9 @@ asc_hip_status check_vector(const asc_hip_context* ctx, const asc_hip_vector_des
   if (extent > 0) {
     asc_hip_status st = check_device_pointer(ctx, v->data, what);
     if (st.code) return st;
+    const Span s = vector_span(*v);
+    st = check_device_pointer(ctx, reinterpret_cast<const void*>(s.begin), what, s.end - s.begin);
+    if (st.code) return st;
   }
   return ok();
 }
@@ -241,7 +266,7 @@ asc_hip_status check_workspace(const asc_hip_context* ctx, void* ws, size_t ws_b
   const Span w = span_of(ws, needed);
   for (const Span& s : operands)
     if (overlaps(w, s)) return fail(ASC_HIP_INVALID_ARGUMENT, "workspace overlaps an operand");
-  return check_device_pointer(ctx, ws, "workspace");
+  return check_device_pointer(ctx, ws, "workspace", needed);
 }
 
 asc_hip_status unavailable_if_validation_only(const asc_hip_context* ctx) {
@@ -272,7 +297,10 @@ asc_hip_status check_matrix(const asc_hip_context* ctx, const asc_hip_matrix_des
   if (writable && !m->writable) return fail(ASC_HIP_INVALID_ARGUMENT, std::string(what) + ": output is not writable");
   if (m->memory_space != ASC_HIP_DEVICE && m->memory_space != ASC_HIP_MANAGED)
     return fail(ASC_HIP_MEMORY_ACCESS, std::string(what) + ": block is not device memory");
-  if (m->rows > 0 && m->cols > 0) return check_device_pointer(ctx, m->data, what);
+  if (m->rows > 0 && m->cols > 0) {
+    const Span s = matrix_span(*m);
+    return check_device_pointer(ctx, m->data, what, s.end - s.begin);
+  }
   return ok();
 }
 

@@ -461,6 +461,40 @@ asc_hip_status asc_sparse_hip_clone_coordinate(asc_hip_context* ctx,
   return ok();
 }
 
+// compressed.h:472-516, coordinate.h:533-569: the new values must form a
+// non-wrapping span, must not overlap the structure, and may only overlap the
+// current values by being exactly them.
+asc_hip_status asc_sparse_hip_rebind_values(const asc_hip_sparse_desc* source,
+                                            const void* values, asc_hip_sparse_desc* out) {
+  asc_hip_status st = check_sparse_meta(source, "rebound matrix");
+  if (st.code) return st;
+  if (out == nullptr) return fail(ASC_HIP_INVALID_ARGUMENT, "output descriptor is null");
+  const size_t esz = elem_size(source->element_kind);
+  if (source->nnz > 0) {
+    if (static_cast<uint64_t>(source->nnz) > SIZE_MAX / esz)
+      return fail(ASC_HIP_OVERFLOW, "value byte count overflows");
+    const size_t bytes = static_cast<size_t>(source->nnz) * esz;
+    if (values == nullptr)
+      return fail(ASC_HIP_INVALID_ARGUMENT, "Nonempty sparse storage cannot be null");
+    if (bytes > UINTPTR_MAX - reinterpret_cast<uintptr_t>(values))
+      return fail(ASC_HIP_OVERFLOW, "Sparse storage address range overflows uintptr_t");
+    const SparseSpans s = sparse_spans(*source);
+    const Span v = span_of(values, bytes);
+    if (overlaps(s.first, v) || overlaps(s.second, v))
+      return fail(ASC_HIP_INVALID_ARGUMENT,
+                  source->format == ASC_HIP_COO
+                      ? "Rebound sparse values cannot overlap coordinates"
+                      : "Rebound sparse values cannot overlap compressed structure");
+    if (values != source->values && overlaps(s.values, v))
+      return fail(ASC_HIP_INVALID_ARGUMENT,
+                  "Rebound sparse values cannot partially overlap existing values");
+  }
+  asc_hip_sparse_desc r = *source;
+  r.values = values;
+  *out = r;
+  return ok();
+}
+
 asc_hip_status asc_sparse_hip_narrow_indices(asc_hip_context* ctx,
                                              const asc_hip_sparse_desc* src,
                                              void* d_idx32, asc_hip_sparse_desc* out) {

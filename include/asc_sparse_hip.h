@@ -258,6 +258,22 @@ asc_hip_status asc_sparse_hip_clone_coordinate(
     asc_hip_context* context, const asc_hip_sparse_desc* host_source,
     void* device_coordinates, void* device_values, asc_hip_sparse_desc* out);
 
+/* A view of `source` with its value pointer replaced by `values`: same
+ * structure pointers, shape, NNZ, memory space and trust.  Host-only; it
+ * neither allocates nor copies nor enqueues.  Rejects, as the reference does,
+ * a null `values` when nnz > 0 (ASC_HIP_INVALID_ARGUMENT), an address range
+ * that wraps (ASC_HIP_OVERFLOW), values overlapping the offsets, indices or
+ * coordinates, and values that partly overlap the current values (the same
+ * pointer is allowed).  Operations then check, before enqueue, that each
+ * operand's whole span lies in one allocation of the context's device, as
+ * the runtime reports the allocation's extent (hipMemGetAddressRange, which
+ * rounds small requests up to its allocation granularity). 
+ * Replaces CompressedSparseView::RebindValues (compressed.h:472-516) and
+ * CoordinateView::RebindValues (coordinate.h:533-569). */
+asc_hip_status asc_sparse_hip_rebind_values(const asc_hip_sparse_desc* source,
+                                            const void* values,
+                                            asc_hip_sparse_desc* out);
+
 /* Device-side narrowing of a trusted device matrix's int64 inner indices
  * into `device_indices32` (nnz int32).  `out` shares offsets and values. */
 asc_hip_status asc_sparse_hip_narrow_indices(asc_hip_context* context,

@@ -8,11 +8,13 @@
 #pragma once
 
 #include <array>
+#include <concepts>
 #include <cstddef>
 #include <cstdint>
 #include <optional>
 #include <span>
 #include <string>
+#include <type_traits>
 #include <utility>
 
 namespace asc {
@@ -139,6 +141,34 @@ inline Status ValidateCompressed(const nnz_t* offsets, const index_t* indices, e
   return Status();
 }
 
+// Value-pointer replacement checks shared by the compressed and coordinate
+// views (compressed.h:472-516, coordinate.h:533-569): a non-wrapping span,
+// no overlap with the structure spans, and no partial overlap with the
+// current values.
+inline bool SpansOverlap(const void* a, std::size_t an, const void* b, std::size_t bn) noexcept {
+  if (an == 0 || bn == 0) return false;
+  const auto x = reinterpret_cast<std::uintptr_t>(a), y = reinterpret_cast<std::uintptr_t>(b);
+  if (an > UINTPTR_MAX - x || bn > UINTPTR_MAX - y) return true;
+  return x < y + bn && y < x + an;
+}
+
+inline Status CheckRebind(const void* values, const void* current, std::size_t value_bytes,
+                          const void* first, std::size_t first_bytes, const void* second,
+                          std::size_t second_bytes, const char* structure_message) {
+  if (value_bytes == 0) return Status();
+  if (values == nullptr)
+    return Status(ErrorCode::kInvalidArgument, "Nonempty sparse storage cannot be null");
+  if (value_bytes > UINTPTR_MAX - reinterpret_cast<std::uintptr_t>(values))
+    return Status(ErrorCode::kOverflow, "Sparse storage address range overflows uintptr_t");
+  if (SpansOverlap(first, first_bytes, values, value_bytes) ||
+      SpansOverlap(second, second_bytes, values, value_bytes))
+    return Status(ErrorCode::kInvalidArgument, structure_message);
+  if (values != current && SpansOverlap(current, value_bytes, values, value_bytes))
+    return Status(ErrorCode::kInvalidArgument,
+                  "Rebound sparse values cannot partially overlap existing values");
+  return Status();
+}
+
 }  // namespace internal_hip_vocabulary
 
 class HipProviderAccess;
@@ -204,6 +234,33 @@ class CompressedSparseView {
   // Inner-index width of the stored indices (8, or 4 after a narrowing clone).
   [[nodiscard]] constexpr int index_bytes() const noexcept { return index_bytes_; }
 
+  // The same structure, shape, NNZ, memory space and trust over other values
+  // (compressed.h:472-516); neither allocates nor copies.
+  template <typename ReboundElement>
+    requires std::same_as<std::remove_const_t<ReboundElement>, std::remove_const_t<Element>>
+  [[nodiscard]] Result<CompressedSparseView<ReboundElement, Format>> RebindValues(
+      ReboundElement* values) const {
+    const std::size_t n = static_cast<std::size_t>(nonzeros_);
+    if (n > SIZE_MAX / sizeof(Element))
+      return Status(ErrorCode::kOverflow, "Sparse value byte count overflows");
+    Status s = internal_hip_vocabulary::CheckRebind(
+        values, values_, n * sizeof(Element), offsets_,
+        static_cast<std::size_t>(outer() + 1) * sizeof(nnz_t), indices_,
+        n * static_cast<std::size_t>(index_bytes_),
+        "Rebound sparse values cannot overlap compressed structure");
+    if (!s.ok()) return s;
+    CompressedSparseView<ReboundElement, Format> v;
+    v.offsets_ = offsets_;
+    v.indices_ = indices_;
+    v.values_ = values;
+    v.shape_ = shape_;
+    v.nonzeros_ = nonzeros_;
+    v.memory_space_ = memory_space_;
+    v.trusted_ = trusted_;
+    v.index_bytes_ = index_bytes_;
+    return v;
+  }
+
  private:
   template <typename, SparseCompressedFormat>
   friend class CompressedSparseView;
@@ -263,7 +320,32 @@ class CoordinateView {
   [[nodiscard]] constexpr MemorySpace memory_space() const noexcept { return memory_space_; }
   [[nodiscard]] constexpr bool canonical_structure_trusted() const noexcept { return trusted_; }
 
+  // The same coordinates, extents, NNZ, memory space and trust over other
+  // values (coordinate.h:533-569).
+  template <typename ReboundElement>
+    requires std::same_as<std::remove_const_t<ReboundElement>, std::remove_const_t<Element>>
+  [[nodiscard]] Result<CoordinateView<ReboundElement, Rank>> RebindValues(
+      ReboundElement* values) const {
+    const std::size_t n = static_cast<std::size_t>(nonzeros_);
+    if (n > SIZE_MAX / (Rank * sizeof(index_t)))
+      return Status(ErrorCode::kOverflow, "Sparse coordinate byte count overflows");
+    Status s = internal_hip_vocabulary::CheckRebind(
+        values, values_, n * sizeof(Element), coordinates_, n * Rank * sizeof(index_t), nullptr,
+        0, "Rebound sparse values cannot overlap coordinates");
+    if (!s.ok()) return s;
+    CoordinateView<ReboundElement, Rank> v;
+    v.coordinates_ = coordinates_;
+    v.values_ = values;
+    v.extents_ = extents_;
+    v.nonzeros_ = nonzeros_;
+    v.memory_space_ = memory_space_;
+    v.trusted_ = trusted_;
+    return v;
+  }
+
  private:
+  template <typename, std::size_t>
+  friend class CoordinateView;
   friend class HipProviderAccess;
   const index_t* coordinates_ = nullptr;
   Element* values_ = nullptr;

@@ -92,6 +92,31 @@ void TestHostViewValidation() {
                                                 kValues3x4.data(), tiny, 5, MemorySpace::kHost)
                  .status(),
              ErrorCode::kShape);
+  // RebindValues (compressed.h:472-516; sparse_cuda_test.cc:549-598): the
+  // same pointer and a distinct buffer rebind; structure and partial overlap
+  // are refused.
+  CHECK(ok->RebindValues(kValues3x4.data()).ok());
+  const std::vector<double> other{9, 8, 7, 6, 5, 4};
+  auto rebound = ok->RebindValues(other.data());
+  CHECK(rebound.ok() && rebound->values() == other.data() &&
+        rebound->outer_offsets() == kOffsets3x4.data() && rebound->nnz() == 5 &&
+        rebound->canonical_structure_trusted() == ok->canonical_structure_trusted());
+  CHECK_CODE(ok->RebindValues(reinterpret_cast<const double*>(kOffsets3x4.data())).status(),
+             ErrorCode::kInvalidArgument);
+  CHECK_CODE(ok->RebindValues(reinterpret_cast<const double*>(kIndices3x4.data())).status(),
+             ErrorCode::kInvalidArgument);
+  CHECK_CODE(rebound->RebindValues(other.data() + 1).status(), ErrorCode::kInvalidArgument);
+  CHECK_CODE(ok->RebindValues(static_cast<const double*>(nullptr)).status(),
+             ErrorCode::kInvalidArgument);
+  const std::vector<asc::index_t> coords{0, 1, 2, 3};
+  const std::array<asc::extent_t, 2> cshape{3, 4};
+  auto coo = asc::CoordinateView<const double, 2>::Create(coords.data(), other.data(), cshape, 2,
+                                                          MemorySpace::kHost);
+  CHECK(coo.ok());
+  CHECK(coo->RebindValues(other.data() + 2).ok());
+  CHECK_CODE(coo->RebindValues(other.data() + 1).status(), ErrorCode::kInvalidArgument);
+  CHECK_CODE(coo->RebindValues(reinterpret_cast<const double*>(coords.data() + 1)).status(),
+             ErrorCode::kInvalidArgument);
   CHECK_CODE(asc::HipStridedVectorView<double>::Create(nullptr, 4, 0).status(),
              ErrorCode::kInvalidArgument);
   CHECK_CODE(asc::HipStridedVectorView<double>::Create(nullptr, -1, 1).status(),
@@ -376,6 +401,58 @@ void TestColumnPanel(Gpu& g) {
   CHECK(asc_sparse_hip_plan_clear(g.ctx.native()).code == ASC_HIP_OK);
 }
 
+// Values-only refresh on trusted device structure (sparse.md:455-466): a
+// rebound clone runs every path, keeps its column-panel plan, and a values
+// span that runs past its allocation is refused before enqueue.
+template <typename E>
+void TestRebind(Gpu& g) {
+  Case(sizeof(E) == 8 ? "rebound values (double)" : "rebound values (float)");
+  auto a = CloneCsr<E>(g, kOffsets4x5, kIndices4x5, Cast<E>(kValues4x5), 4, 5);
+  const asc::CsrView<const E> cv = *a.view();
+  std::vector<E> doubled;
+  for (double v : kValues4x5) doubled.push_back(E(2 * v));
+  auto vb = Upload<E>(g, doubled);
+  auto rebound = cv.RebindValues(static_cast<const E*>(vb.data()));
+  CHECK(rebound.ok() && rebound->canonical_structure_trusted());
+  if (!rebound.ok()) return;
+  asc_hip_panel_options opt{1, 1, ASC_HIP_ALG_AUTO, ASC_HIP_ALG_AUTO, {}};
+  auto need = asc::HipCsrSpmvAnalysisSize(g.ctx, cv, &opt);
+  CHECK(need.ok());
+  auto analysis = asc::HipBuffer::Allocate(g.device, *need);
+  CHECK(asc::HipCsrSpmvAnalyze(g.ctx, cv, analysis->mutable_view(), &opt).ok());
+  auto xb = Upload<E>(g, {1, -2, 3, 4, -1});
+  auto x = asc::HipStridedVectorView<const E>::Create(static_cast<const E*>(xb.data()), 5, 1);
+  for (auto alg : {asc::HipSpmvAlgorithm::kOrdered, asc::HipSpmvAlgorithm::kMergePath,
+                   asc::HipSpmvAlgorithm::kColumnPanel}) {
+    auto yb = Upload<E>(g, {NAN, NAN, NAN, NAN});
+    auto y = asc::HipStridedVectorView<E>::Create(static_cast<E*>(yb.data()), 4, 1);
+    auto wneed = asc::HipCsrSpmvWorkspaceSize(g.ctx, *rebound, *x, *y, alg);
+    CHECK(wneed.ok());
+    auto ws = asc::HipBuffer::Allocate(g.device, *wneed + 256);
+    auto done = asc::HipCsrSpmv(g.ctx, E(1), *rebound, *x, E(0), *y, ws->mutable_view(), alg);
+    CHECK(done.ok() && done->Wait().ok());
+    CHECK((Download<E>(g, yb, 4) == std::vector<E>{-4, -16, 22, -4}));
+  }
+  // Values whose eighth element lies one element past the end of a 4 MiB
+  // allocation (a size the runtime keeps as requested): the host rebind
+  // cannot see the allocation, the operation refuses it and leaves y as it was.
+  constexpr std::size_t kBlock = std::size_t{1} << 22;
+  auto block = asc::HipBuffer::Allocate(g.device, kBlock);
+  CHECK(block.ok());
+  if (!block.ok()) return;
+  const E* tail = reinterpret_cast<const E*>(static_cast<const char*>(block->data()) + kBlock) - 7;
+  auto too_short = cv.RebindValues(tail);
+  CHECK(too_short.ok());
+  auto yb = Upload<E>(g, {1, 2, 3, 4});
+  auto y = asc::HipStridedVectorView<E>::Create(static_cast<E*>(yb.data()), 4, 1);
+  CHECK_CODE(asc::HipCsrSpmv(g.ctx, E(1), *too_short, *x, E(0), *y, asc::MutableMemoryView{},
+                             asc::HipSpmvAlgorithm::kOrdered)
+                 .status(),
+             ErrorCode::kMemoryAccess);
+  CHECK((Download<E>(g, yb, 4) == std::vector<E>{1, 2, 3, 4}));
+  CHECK(asc_sparse_hip_plan_clear(g.ctx.native()).code == ASC_HIP_OK);
+}
+
 // coordinate_test.cc:96-142: triplets (2,1)=5, (0,3)=2, (2,1)=-1, (1,0)=0,
 // (0,1)=7 finalize (kSum, kKeep) to (0,1)=7, (0,3)=2, (1,0)=0, (2,1)=4;
 // kReject refuses them; (kSum, kDrop) keeps 3 entries.
@@ -530,6 +607,8 @@ int main(int argc, char** argv) {
     TestLevelOne<float>(g);
     TestTriangular<double>(g);
     TestTriangular<float>(g);
+    TestRebind<double>(g);
+    TestRebind<float>(g);
   }
   std::printf("%d checks, %d failures\n", g_checks, g_failures);
   return g_failures == 0 ? 0 : 1;
