@@ -160,7 +160,7 @@ asc_hip_status build(asc_hip_context* ctx, int64_t n, const asc_hip_batch_entry*
     return fail(ASC_HIP_INVALID_ARGUMENT, "plan memory is smaller than the queried size");
   if (reinterpret_cast<uintptr_t>(mem) % 256 != 0)
     return fail(ASC_HIP_INVALID_ARGUMENT, "plan memory must be 256-byte aligned");
-  asc_hip_status st = check_device_pointer(ctx, mem, "plan memory");
+  asc_hip_status st = check_device_pointer(ctx, mem, "plan memory", need);
   if (st.code) return st;
   if ((st = unavailable_if_validation_only(ctx)).code) return st;
   DeviceGuard guard(ctx->device);

@@ -60,8 +60,8 @@ asc_hip_status asc_sparse_hip_clone_indexed(asc_hip_context* ctx, const asc_hip_
   const size_t ib = 8 * static_cast<size_t>(src->nnz);
   const size_t vb = elem_size(src->element_kind) * static_cast<size_t>(src->nnz);
   if (src->nnz > 0) {
-    if ((st = check_device_pointer(ctx, d_idx, "indices destination")).code) return st;
-    if ((st = check_device_pointer(ctx, d_val, "values destination")).code) return st;
+    if ((st = check_device_pointer(ctx, d_idx, "indices destination", ib)).code) return st;
+    if ((st = check_device_pointer(ctx, d_val, "values destination", vb)).code) return st;
     if (overlaps(span_of(d_idx, ib), span_of(d_val, vb)))
       return fail(ASC_HIP_INVALID_ARGUMENT, "clone destinations overlap");
   }
@@ -100,8 +100,11 @@ asc_hip_status asc_sparse_hip_standard_level1(asc_hip_context* ctx, uint8_t oper
   if (!sparse->trusted)
     return fail(ASC_HIP_INVALID_ARGUMENT, "operations require a trusted canonical indexed vector");
   if (sparse->nnz > 0) {
-    if ((st = check_device_pointer(ctx, sparse->indices, "indexed vector")).code) return st;
-    if ((st = check_device_pointer(ctx, sparse->values, "indexed vector")).code) return st;
+    const size_t n = static_cast<size_t>(sparse->nnz);
+    if ((st = check_device_pointer(ctx, sparse->indices, "indexed vector", 8 * n)).code) return st;
+    if ((st = check_device_pointer(ctx, sparse->values, "indexed vector",
+                                   elem_size(sparse->element_kind) * n)).code)
+      return st;
   }
   const bool dot = operation == ASC_HIP_LEVEL1_DOT;
   const bool writes_dense = operation == ASC_HIP_LEVEL1_AXPY ||

@@ -365,10 +365,10 @@ asc_hip_status asc_sparse_hip_clone_compressed(asc_hip_context* ctx,
   if ((st = validate_host_compressed(*src)).code) return st;
   const size_t esz = elem_size(src->element_kind);
   const size_t off_b = sizeof(int64_t) * (outer + 1), idx_b = index_bytes * n, val_b = esz * n;
-  if ((st = check_device_pointer(ctx, d_off, "offsets destination")).code) return st;
+  if ((st = check_device_pointer(ctx, d_off, "offsets destination", off_b)).code) return st;
   if (n > 0) {
-    if ((st = check_device_pointer(ctx, d_idx, "indices destination")).code) return st;
-    if ((st = check_device_pointer(ctx, d_val, "values destination")).code) return st;
+    if ((st = check_device_pointer(ctx, d_idx, "indices destination", idx_b)).code) return st;
+    if ((st = check_device_pointer(ctx, d_val, "values destination", val_b)).code) return st;
   }
   if (overlaps(span_of(d_off, off_b), span_of(d_idx, idx_b)) ||
       overlaps(span_of(d_off, off_b), span_of(d_val, val_b)) ||
@@ -437,8 +437,8 @@ asc_hip_status asc_sparse_hip_clone_coordinate(asc_hip_context* ctx,
   }
   const size_t cb = 16 * static_cast<size_t>(src->nnz), vb = elem_size(src->element_kind) * src->nnz;
   if (src->nnz > 0) {
-    if ((st = check_device_pointer(ctx, d_coords, "coordinates destination")).code) return st;
-    if ((st = check_device_pointer(ctx, d_val, "values destination")).code) return st;
+    if ((st = check_device_pointer(ctx, d_coords, "coordinates destination", cb)).code) return st;
+    if ((st = check_device_pointer(ctx, d_val, "values destination", vb)).code) return st;
     if (overlaps(span_of(d_coords, cb), span_of(d_val, vb)))
       return fail(ASC_HIP_INVALID_ARGUMENT, "clone destinations overlap");
   }
@@ -507,7 +507,9 @@ asc_hip_status asc_sparse_hip_narrow_indices(asc_hip_context* ctx,
   if (inner_of(*src) > std::numeric_limits<int32_t>::max())
     return fail(ASC_HIP_OVERFLOW, "inner extent exceeds INT32_MAX");
   if (src->nnz > 0) {
-    if ((st = check_device_pointer(ctx, d_idx32, "indices destination")).code) return st;
+    if ((st = check_device_pointer(ctx, d_idx32, "indices destination",
+                                   sizeof(int32_t) * static_cast<size_t>(src->nnz))).code)
+      return st;
     if (sparse_overlaps(*src, span_of(d_idx32, 4 * static_cast<size_t>(src->nnz))))
       return fail(ASC_HIP_INVALID_ARGUMENT, "destination overlaps the source");
   }
@@ -545,7 +547,9 @@ asc_hip_status asc_sparse_hip_convert_values_scaled(asc_hip_context* ctx,
   if (!(scale > 0.0) || scale > 1e300)
     return fail(ASC_HIP_INVALID_ARGUMENT, "scale must be positive and finite");
   if (src->nnz > 0) {
-    if ((st = check_device_pointer(ctx, d_val, "values destination")).code) return st;
+    if ((st = check_device_pointer(ctx, d_val, "values destination",
+                                   elem_size(kind) * static_cast<size_t>(src->nnz))).code)
+      return st;
     if (sparse_overlaps(*src, span_of(d_val, elem_size(kind) * src->nnz)))
       return fail(ASC_HIP_INVALID_ARGUMENT, "destination overlaps the source");
   }
@@ -835,10 +839,10 @@ asc_hip_status check_outputs(const asc_hip_context* ctx, const asc_hip_sparse_de
   const size_t ob = sizeof(int64_t) * (out_outer + 1);
   const size_t ib = static_cast<size_t>(index_bytes) * src.nnz;
   const size_t vb = elem_size(src.element_kind) * src.nnz;
-  if ((st = check_device_pointer(ctx, off, "offsets destination")).code) return st;
+  if ((st = check_device_pointer(ctx, off, "offsets destination", ob)).code) return st;
   if (src.nnz > 0) {
-    if ((st = check_device_pointer(ctx, idx, "indices destination")).code) return st;
-    if ((st = check_device_pointer(ctx, val, "values destination")).code) return st;
+    if ((st = check_device_pointer(ctx, idx, "indices destination", ib)).code) return st;
+    if ((st = check_device_pointer(ctx, val, "values destination", vb)).code) return st;
   }
   const Span so = span_of(off, ob), si = span_of(idx, ib), sv = span_of(val, vb);
   if (overlaps(so, si) || overlaps(so, sv) || overlaps(si, sv))
@@ -912,8 +916,10 @@ asc_hip_status check_triplets(asc_hip_context* ctx, const asc_hip_sparse_desc* t
   if (t->nnz > 0) {
     if (t->outer_offsets == nullptr || t->values == nullptr)
       return fail(ASC_HIP_INVALID_ARGUMENT, "triplets: arrays are null");
-    if ((st = check_device_pointer(ctx, t->outer_offsets, "triplets")).code) return st;
-    if ((st = check_device_pointer(ctx, t->values, "triplets")).code) return st;
+    const size_t n = static_cast<size_t>(t->nnz);
+    if ((st = check_device_pointer(ctx, t->outer_offsets, "triplets", 16 * n)).code) return st;
+    if ((st = check_device_pointer(ctx, t->values, "triplets", elem_size(t->element_kind) * n)).code)
+      return st;
   }
   return ok();
 }
@@ -948,8 +954,8 @@ asc_hip_status asc_sparse_hip_coo_finalize(asc_hip_context* ctx, const asc_hip_s
   if (n > 0) {
     if (out_coords == nullptr || out_vals == nullptr)
       return fail(ASC_HIP_INVALID_ARGUMENT, "output arrays are null");
-    if ((st = check_device_pointer(ctx, out_coords, "output")).code) return st;
-    if ((st = check_device_pointer(ctx, out_vals, "output")).code) return st;
+    if ((st = check_device_pointer(ctx, out_coords, "output", 16 * n)).code) return st;
+    if ((st = check_device_pointer(ctx, out_vals, "output", esz * n)).code) return st;
   }
   const Span oc = span_of(out_coords, 16 * n), ov = span_of(out_vals, esz * n);
   const Span ic = span_of(t->outer_offsets, 16 * n), iv = span_of(t->values, esz * n);
@@ -1082,8 +1088,8 @@ asc_hip_status asc_sparse_hip_convert_to_coordinate(asc_hip_context* ctx,
   const size_t cb = 16 * static_cast<size_t>(src->nnz);
   const size_t vb = elem_size(src->element_kind) * src->nnz;
   if (src->nnz > 0) {
-    if ((st = check_device_pointer(ctx, d_coords, "coordinates destination")).code) return st;
-    if ((st = check_device_pointer(ctx, d_values, "values destination")).code) return st;
+    if ((st = check_device_pointer(ctx, d_coords, "coordinates destination", cb)).code) return st;
+    if ((st = check_device_pointer(ctx, d_values, "values destination", vb)).code) return st;
     if (overlaps(span_of(d_coords, cb), span_of(d_values, vb)) ||
         sparse_overlaps(*src, span_of(d_coords, cb)) || sparse_overlaps(*src, span_of(d_values, vb)))
       return fail(ASC_HIP_INVALID_ARGUMENT, "conversion destinations overlap");

@@ -166,7 +166,7 @@ asc_hip_status asc_sparse_hip_triangular_solve(asc_hip_context* ctx, uint8_t tra
   const TrsvLayout L = trsv_layout(*a, transpose);
   if (analysis == nullptr || analysis_bytes < L.total)
     return fail(ASC_HIP_INVALID_ARGUMENT, "analysis buffer is smaller than the analysis size");
-  if ((st = check_device_pointer(ctx, analysis, "analysis")).code) return st;
+  if ((st = check_device_pointer(ctx, analysis, "analysis", L.total)).code) return st;
   if (overlaps(span_of(analysis, L.total), rs))
     return fail(ASC_HIP_INVALID_ARGUMENT, "right-hand sides overlap the analysis");
   if ((st = unavailable_if_validation_only(ctx)).code) return st;

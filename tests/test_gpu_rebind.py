@@ -128,3 +128,22 @@ def test_span_past_allocation_is_refused(ctx):
     ref = np.zeros(m.rows)
     O.compat_spmv(m.offsets, m.indices, np.ones(m.nnz), 1.0, x, 0.0, ref)
     np.testing.assert_array_equal(ctx.download(yb, np.float64, m.rows), ref)
+
+
+def test_rebind_needs_new_triangular_analysis(ctx):
+    # lower [[2, 0], [1, 3]]: the analysis checked these values, so a rebound
+    # matrix is refused until analysed again
+    off, idx = np.array([0, 1, 3]), np.array([0, 0, 1])
+    a = H.clone_compressed(ctx, H.CSR, 2, 2, off, idx, np.array([2.0, 1.0, 3.0]), H.F64)
+    plan = H.triangular_analyze(ctx, H.OP_N, H.LOWER, H.NON_UNIT, a)
+    new_vals = np.array([4.0, -1.0, 0.5])
+    b = H.rebind_values(a, ctx.upload(new_vals))
+    rhs = ctx.upload(np.array([8.0, 1.0]))
+    with pytest.raises(H.AscHipError) as e:
+        H.triangular_solve(ctx, H.OP_N, 1.0, b, H.dense(rhs, 2, 1, 1, H.ROW_MAJOR, H.F64), plan)
+    assert e.value.code == H.INVALID_ARGUMENT
+    plan_b = H.triangular_analyze(ctx, H.OP_N, H.LOWER, H.NON_UNIT, b)
+    H.triangular_solve(ctx, H.OP_N, 1.0, b, H.dense(rhs, 2, 1, 1, H.ROW_MAJOR, H.F64), plan_b)
+    ref = O.trsv(0, 0, True, False, 2, off, idx, new_vals, 1.0, np.array([8.0, 1.0]))
+    np.testing.assert_array_equal(ctx.download(rhs, np.float64, 2), ref)
+    assert list(ref) == [2.0, 6.0]  # 4*2 = 8; -1*2 + 0.5*6 = 1
