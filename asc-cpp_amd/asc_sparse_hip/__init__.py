@@ -694,10 +694,13 @@ def convert_to_coordinate(ctx: Context, a: DeviceSparse, ws: Workspace | None = 
     return DeviceSparse(out, [bc, bv])
 
 
+STAGING_AUTO, STAGING_L2, STAGING_LDS = 0, 1, 2
+
+
 class PanelOptions(C.Structure):
     _fields_ = [("panel_columns", C.c_int64), ("long_row_min", C.c_int64),
                 ("short_algorithm", C.c_uint8), ("long_algorithm", C.c_uint8),
-                ("reserved", C.c_uint8 * 6)]
+                ("staging", C.c_uint8), ("reserved", C.c_uint8 * 5)]
 
 
 _SIGS4 = {
@@ -712,8 +715,8 @@ if _lib is not None:
 
 
 def panel_options(panel_columns: int = 0, long_row_min: int = 0, short_alg: int = ALG_AUTO,
-                  long_alg: int = ALG_AUTO) -> PanelOptions:
-    return PanelOptions(panel_columns, long_row_min, short_alg, long_alg)
+                  long_alg: int = ALG_AUTO, staging: int = STAGING_AUTO) -> PanelOptions:
+    return PanelOptions(panel_columns, long_row_min, short_alg, long_alg, staging)
 
 
 def csr_spmv_analysis_size(ctx: Context, a: DeviceSparse, opts: PanelOptions | None = None) -> int:

@@ -113,12 +113,14 @@ asc_hip_status missing_panel_analysis() {
 constexpr int64_t kDefaultPanelBytes = int64_t(2) << 20;
 constexpr int64_t kDefaultLongRowMin = 128;
 constexpr int64_t kMaxPanels = 1024;
+constexpr double kLdsMinSegment = 48.0;
 
 struct PanelShape {
   int64_t panel_cols = 1;
   int64_t panels = 8;
   int64_t long_min = kDefaultLongRowMin;
-  size_t list_bytes = 0, split_bytes = 0, scratch_bytes = 0, bytes = 0;
+  bool lds = false;
+  size_t list_bytes = 0, begin_bytes = 0, split_bytes = 0, scratch_bytes = 0, bytes = 0;
 };
 
 asc_hip_status panel_shape(asc_hip_context* ctx, const asc_hip_sparse_desc* a,
@@ -136,23 +138,55 @@ asc_hip_status panel_shape(asc_hip_context* ctx, const asc_hip_sparse_desc* a,
     return fail(ASC_HIP_INVALID_ARGUMENT, "panel width and long-row length must be nonnegative");
   if (a->rows > kU32Max || std::min(a->nnz, a->cols) > kU32Max)
     return fail(ASC_HIP_UNSUPPORTED, "column-panel analysis stores 32-bit row ids and positions");
+  const uint8_t staging = opt ? opt->staging : ASC_HIP_STAGING_AUTO;
+  if (staging > ASC_HIP_STAGING_LDS)
+    return fail(ASC_HIP_INVALID_ARGUMENT, "unknown panel staging");
   const int64_t xbytes = a->element_kind == ASC_HIP_F64 ? 8 : 4;
-  const int64_t w = want_cols > 0 ? want_cols : kDefaultPanelBytes / xbytes;
-  const int64_t p0 = a->cols > w ? (a->cols - 1) / w + 1 : 1;
-  const int64_t panels = (p0 + 7) / 8 * 8;
+  // Panels of a multiple of 8 covering the columns with at most `w` each.
+  auto cut = [&](int64_t w, int64_t* panels, int64_t* cols) {
+    const int64_t p0 = a->cols > w ? (a->cols - 1) / w + 1 : 1;
+    *panels = (p0 + 7) / 8 * 8;
+    *cols = a->cols > 0 ? (a->cols - 1) / *panels + 1 : 1;
+  };
+  const int64_t lds_cols = asc_hip::kLdsPanelBytes / xbytes;
+  int64_t panels = 0, panel_cols = 0;
+  cut(want_cols > 0 ? want_cols : (staging == ASC_HIP_STAGING_LDS ? lds_cols
+                                                                   : kDefaultPanelBytes / xbytes),
+      &panels, &panel_cols);
+  bool lds = staging == ASC_HIP_STAGING_LDS;
+  if (lds && panel_cols > lds_cols)
+    return fail(ASC_HIP_INVALID_ARGUMENT, "an LDS-staged panel holds at most 160 KiB of x");
+  if (staging == ASC_HIP_STAGING_AUTO) {
+    // LDS staging pays when a row's entries in one LDS-sized panel form
+    // segments long enough to stream (measured crossover near 48 entries);
+    // shorter segments stream better from fewer, L2-sized panels.
+    int64_t lp = panels, lc = panel_cols;
+    if (want_cols == 0) cut(lds_cols, &lp, &lc);
+    const double per_segment = a->rows > 0 ? static_cast<double>(a->nnz) /
+                                                 static_cast<double>(a->rows) /
+                                                 static_cast<double>(lp)
+                                           : 0.0;
+    if (lc <= lds_cols && per_segment >= kLdsMinSegment) {
+      lds = true;
+      panels = lp;
+      panel_cols = lc;
+    }
+  }
   if (panels > kMaxPanels)
     return fail(ASC_HIP_INVALID_ARGUMENT, "panel width gives more than 1024 panels");
   int64_t split_entries = 0;
-  if (!checked_mul(panels - 1, a->rows, &split_entries) ||
+  if (!checked_mul(panels, a->rows, &split_entries) ||
       split_entries > std::numeric_limits<int64_t>::max() / 8)
     return fail(ASC_HIP_OVERFLOW, "panel split table size overflows");
   out->panels = panels;
-  out->panel_cols = a->cols > 0 ? (a->cols - 1) / panels + 1 : 1;
+  out->panel_cols = panel_cols;
+  out->lds = lds;
   out->long_min = long_min;
   out->list_bytes = align256(sizeof(uint32_t) * static_cast<size_t>(a->rows));
+  out->begin_bytes = align256(sizeof(int64_t) * static_cast<size_t>(a->rows));
   out->split_bytes = align256(sizeof(uint32_t) * static_cast<size_t>(split_entries));
   out->scratch_bytes = asc_hip::panel_analysis_scratch_bytes(a->rows);
-  out->bytes = out->list_bytes + out->split_bytes + out->scratch_bytes;
+  out->bytes = out->list_bytes + out->begin_bytes + out->split_bytes + out->scratch_bytes;
   return ok();
 }
 
@@ -161,7 +195,8 @@ bool panel_overlaps(const asc_hip::PanelPlan& pp, Span out) {
   if (pp.nlong == 0) return false;
   const size_t n = static_cast<size_t>(pp.nlong);
   return overlaps(span_of(pp.long_rows, 4 * n), out) ||
-         (pp.panels > 1 && overlaps(span_of(pp.splits, 4 * n * static_cast<size_t>(pp.panels - 1)), out));
+         overlaps(span_of(pp.row_begin, 8 * n), out) ||
+         overlaps(span_of(pp.splits, 4 * n * static_cast<size_t>(pp.panels)), out);
 }
 
 asc_hip_status check_options(const asc_hip_sparse_desc& a, const asc_hip_spmv_options* opt,
@@ -628,22 +663,38 @@ asc_hip_status asc_sparse_hip_csr_spmv_analyze(asc_hip_context* ctx,
     return st;
   if ((st = unavailable_if_validation_only(ctx)).code) return st;
   char* w = static_cast<char*>(analysis);
+  uint32_t* long_rows = reinterpret_cast<uint32_t*>(w);
+  int64_t* row_begin = reinterpret_cast<int64_t*>(w + shape.list_bytes);
+  uint32_t* splits = reinterpret_cast<uint32_t*>(w + shape.list_bytes + shape.begin_bytes);
+  void* scratch = w + shape.list_bytes + shape.begin_bytes + shape.split_bytes;
   asc_hip::PanelPlan pp;
-  pp.long_rows = reinterpret_cast<const uint32_t*>(w);
-  pp.splits = reinterpret_cast<const uint32_t*>(w + shape.list_bytes);
+  pp.long_rows = long_rows;
+  pp.row_begin = row_begin;
+  pp.splits = splits;
   pp.panels = shape.panels;
   pp.panel_cols = shape.panel_cols;
   pp.long_min = shape.long_min;
+  pp.lds = shape.lds;
   {
     DeviceGuard guard(ctx->device);
     if (guard.error != hipSuccess) return fail_hip(guard.error, "hipSetDevice");
     int64_t nlong = 0;
     const hipError_t e = asc_hip::panel_analysis_launch(
-        as_compressed(*a, false), shape.panel_cols, shape.panels, shape.long_min,
-        reinterpret_cast<uint32_t*>(w), reinterpret_cast<uint32_t*>(w + shape.list_bytes),
-        w + shape.list_bytes + shape.split_bytes, &nlong, ctx->stream);
+        as_compressed(*a, false), shape.panel_cols, shape.panels, shape.long_min, long_rows,
+        row_begin, splits, scratch, &nlong, ctx->stream);
     if ((st = finish_launch(ctx, e, "panel analysis")).code) return st;
     pp.nlong = nlong;
+  }
+  // LDS form: row chunks so that each work-group gathers about eight entries
+  // per staged x element (the panel copy stays a small share of its reads),
+  // with at least one work-group per CU.
+  if (pp.lds && pp.nlong > 0) {
+    const int64_t per_chunk = std::max<int64_t>(1, 8 * a->cols);
+    int64_t chunks = std::max<int64_t>(1, a->nnz / per_chunk);
+    chunks = std::max<int64_t>(chunks, (256 + pp.panels - 1) / pp.panels);
+    chunks = std::min<int64_t>(chunks, pp.nlong);
+    chunks = std::min<int64_t>(chunks, std::numeric_limits<uint32_t>::max() / pp.panels);
+    pp.chunks = chunks;
   }
   // AUTO: for the short rows, the power of two at or below long_min / 8
   // lanes; for the listed rows, the power of two at or below their mean
@@ -655,9 +706,12 @@ asc_hip_status asc_sparse_hip_csr_spmv_analyze(asc_hip_context* ctx,
       ++short_alg;
   }
   if (long_alg == ASC_HIP_ALG_AUTO) {
+    // LDS staging: about six entries per lane (a pass's metadata loads are
+    // then a small share of its work); L2 staging: one entry per lane.
     const double mean = pp.nlong > 0 ? static_cast<double>(a->nnz) /
                                            static_cast<double>(pp.nlong) /
-                                           static_cast<double>(pp.panels)
+                                           static_cast<double>(pp.panels) /
+                                           (pp.lds ? 6.0 : 1.0)
                                      : 1.0;
     long_alg = ASC_HIP_ALG_VECTOR1;
     for (double g = 2.0; g <= mean && long_alg < ASC_HIP_ALG_VECTOR64; g *= 2.0) ++long_alg;

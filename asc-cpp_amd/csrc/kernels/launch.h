@@ -38,11 +38,13 @@ enum class SpmvForm : uint8_t {
 // Column-panel plan (built by panel_analysis_launch): rows with at least
 // `long_min` entries are listed in `long_rows` (ascending); the columns are cut
 // into `panels` ranges of `panel_cols` (a multiple of 8 ranges, one per XCD at
-// a time); splits[(q-1)*nlong + i] (q = 1..panels-1) is the position, relative
-// to the start of listed row i, of its first entry in panel q.  Shorter rows
-// run through vector<short_alg> in one pass.
+// a time).  For listed row i, row_begin[i] is its first entry and
+// splits[q*nlong + i] (q = 0..panels-1) the end of its panel-q entries
+// relative to row_begin[i] (the last panel's end is the row length).  Shorter
+// rows run through vector<short_alg> in one pass.
 struct PanelPlan {
   const uint32_t* long_rows = nullptr;
+  const int64_t* row_begin = nullptr;
   const uint32_t* splits = nullptr;
   int64_t nlong = 0;
   int64_t panels = 8;
@@ -50,7 +52,12 @@ struct PanelPlan {
   int64_t long_min = INT64_MAX;
   SpmvAlg short_alg = SpmvAlg::Vec4;  // Vec1..Vec64
   SpmvAlg long_alg = SpmvAlg::Vec32;  // Vec1..Vec64
+  bool lds = false;     // stage each x panel in LDS (panel_cols * x bytes <= kLdsPanelBytes)
+  int64_t chunks = 1;   // LDS form: row chunks per panel (work-groups = panels * chunks)
 };
+
+// LDS bytes one work-group stages an x panel in (gfx950: 160 KiB per CU).
+constexpr int64_t kLdsPanelBytes = int64_t(160) << 10;
 
 struct SpmvArgs {
   Compressed a;
@@ -71,15 +78,16 @@ size_t spmv_workspace_bytes(const SpmvArgs& args);
 hipError_t spmv_launch(const SpmvArgs& args, void* workspace,
                        hipStream_t stream);
 
-// Builds a PanelPlan's row list and split table for a canonical CSR operand
-// (sorted columns per row): `long_rows` holds `outer` and `splits`
-// (panels-1)*outer uint32 entries; `scratch` holds
-// panel_analysis_scratch_bytes(outer).  Waits for the stream to read back
-// the number of listed rows.
+// Builds a PanelPlan's row list, row starts and split table for a canonical
+// CSR operand (sorted columns per row): `long_rows` holds `outer` uint32,
+// `row_begin` `outer` int64 and `splits` panels*outer uint32 entries;
+// `scratch` holds panel_analysis_scratch_bytes(outer).  Waits for the stream
+// to read back the number of listed rows.
 size_t panel_analysis_scratch_bytes(int64_t outer);
 hipError_t panel_analysis_launch(const Compressed& a, int64_t panel_cols, int64_t panels,
-                                 int64_t long_min, uint32_t* long_rows, uint32_t* splits,
-                                 void* scratch, int64_t* out_nlong, hipStream_t stream);
+                                 int64_t long_min, uint32_t* long_rows, int64_t* row_begin,
+                                 uint32_t* splits, void* scratch, int64_t* out_nlong,
+                                 hipStream_t stream);
 
 // Stable sort of the indices 0..n-1 by uint32 key (< key_bound): out_index
 // gets the sorted order, out_key_offsets[k] (k = 0..key_bound) the first

@@ -258,18 +258,26 @@ __global__ __launch_bounds__(kBlock) void spmv_vector_kernel(KParams p, int64_t 
 // ---------------------------------------------------------------------------
 // Column-panel plan for long rows.  Random columns of a large x miss in L2 on
 // nearly every gather.  The columns are cut into `panels` ranges (a multiple
-// of 8) and the rows with at least `long_min` entries are listed; block b
-// works on panel 8*group + b % 8 for one block of listed rows.  Work-groups
-// are dispatched to the 8 XCDs round robin, so each XCD gathers from one
-// panel of x at a time, which stays resident in its L2, and the 8 XCDs read
-// adjacent slices of the same rows at about the same time.  Each (panel, row)
-// partial sum goes to an FP64 buffer; the combine kernel adds them in panel
-// order and applies the row epilogue.  Shorter rows (a small share of the
-// entries of a skewed matrix) run through vector<G> in one pass.
+// of 8) and the rows with at least `long_min` entries are listed; each listed
+// row's entries in one panel form a segment, found from the split table.
+// Each (panel, row) partial sum goes to an FP64 buffer; the combine kernel
+// adds them in panel order and applies the row epilogue.  Shorter rows (a
+// small share of the entries of a skewed matrix) run through vector<G> in one
+// pass.  Two forms walk the segments:
+//   L2:  block b works on panel 8*group + b % 8 for one block of listed rows.
+//        Work-groups are dispatched to the 8 XCDs round robin, so each XCD
+//        gathers from one panel of x at a time, which stays in its L2, and
+//        the 8 XCDs read adjacent slices of the same rows at about the same
+//        time.
+//   LDS: a work-group stages its panel of x (at most kLdsPanelBytes) in LDS
+//        and walks one chunk of listed rows, gathering from LDS.  Work-groups
+//        are ordered chunk-major, so every panel of a chunk runs at about the
+//        same time (the chunk's matrix lines are fetched from HBM about once)
+//        and XCD k stages the panels congruent to k mod 8, which its L2 keeps.
 // ---------------------------------------------------------------------------
 template <int G, typename V, typename I, bool TWOSUM>
 __global__ __launch_bounds__(kBlock) void spmv_panel_kernel(
-    KParams p, const uint32_t* __restrict__ long_rows, const uint32_t* __restrict__ splits,
+    KParams p, const int64_t* __restrict__ row_begin, const uint32_t* __restrict__ splits,
     int64_t nlong, int64_t panels, int64_t row_blocks, int64_t rb_grid,
     double* __restrict__ partial) {
   using Acc = typename AccSel<V, TWOSUM>::type;
@@ -282,14 +290,85 @@ __global__ __launch_bounds__(kBlock) void spmv_panel_kernel(
   for (int64_t rbk = rest % rb_grid; rbk < row_blocks; rbk += rb_grid) {
     const int64_t i = rbk * (kBlock / G) + threadIdx.x / G;
     if (i < nlong) {  // uniform over the G-group
-      const int64_t row = long_rows[i];
-      const int64_t rb = p.offsets[row];
+      const int64_t rb = row_begin[i];
       const int64_t b = q == 0 ? rb : rb + ld_stream(splits + (q - 1) * nlong + i);
-      const int64_t e = q + 1 == panels ? p.offsets[row + 1]
-                                        : rb + ld_stream(splits + q * nlong + i);
+      const int64_t e = rb + ld_stream(splits + q * nlong + i);
       const Acc acc = group_sum<G, V, I, Acc>(p, b, e, lane);
       if (lane == 0) partial[q * nlong + i] = acc.total_f64();
     }
+  }
+}
+
+constexpr int kLdsBlock = 1024;
+
+// The G-lane sum of entries [b, e) of one panel segment, x read from the LDS
+// copy `sx` of columns [col0, col0 + panel width).
+template <int G, typename V, typename I, typename Acc, typename A>
+__device__ __forceinline__ Acc group_sum_lds(const V* __restrict__ val, const I* __restrict__ col,
+                                             const A* sx, int64_t col0, int64_t b, int64_t e,
+                                             int lane) {
+  Acc acc = Acc::zero();
+  Acc acc2 = Acc::zero();
+  int64_t k = b + lane;
+  for (; k + G < e; k += 2 * G) {
+    const int c0 = static_cast<int>(load_index(col, k) - col0);
+    const int c1 = static_cast<int>(load_index(col, k + G) - col0);
+    const A v0 = widen<A>(ld_stream(val + k));
+    const A v1 = widen<A>(ld_stream(val + k + G));
+    acc.add_product(v0, sx[c0]);
+    acc2.add_product(v1, sx[c1]);
+  }
+  if (k < e) {
+    const int c0 = static_cast<int>(load_index(col, k) - col0);
+    acc.add_product(widen<A>(ld_stream(val + k)), sx[c0]);
+  }
+  acc.merge_after(acc2);
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) acc.merge_after(acc.shfl_xor(off, G));
+  return acc;
+}
+
+template <int G, typename V, typename I, bool TWOSUM>
+__global__ __launch_bounds__(kLdsBlock) void spmv_panel_lds_kernel(
+    KParams p, int64_t cols, const int64_t* __restrict__ row_begin,
+    const uint32_t* __restrict__ splits, int64_t nlong, int64_t panels, int64_t panel_cols,
+    int64_t chunk_rows, double* __restrict__ partial) {
+  using A = typename acc_of<V>::type;
+  using Acc = typename AccSel<V, TWOSUM>::type;
+  __shared__ A sx[kLdsPanelBytes / sizeof(A)];
+  const int64_t q = blockIdx.x % panels;
+  const int64_t chunk = blockIdx.x / panels;
+  const int64_t col0 = q * panel_cols;
+  const int64_t width = cols - col0 < panel_cols ? cols - col0 : panel_cols;
+  const A* __restrict__ x = static_cast<const A*>(p.x);
+  for (int64_t k = threadIdx.x; k < width; k += kLdsBlock) sx[k] = x[(col0 + k) * p.incx];
+  __syncthreads();
+  const V* __restrict__ val = static_cast<const V*>(p.values);
+  const I* __restrict__ col = static_cast<const I*>(p.indices);
+  const int64_t r0 = chunk * chunk_rows;
+  const int64_t r1 = nlong - r0 < chunk_rows ? nlong : r0 + chunk_rows;
+  const int lane = threadIdx.x % G;
+  constexpr int kRowsPerPass = kLdsBlock / G;
+  int64_t i = r0 + threadIdx.x / G;
+  // The next pass's segment bounds load while this pass runs.
+  int64_t b = 0, e = 0;
+  if (i < r1) {
+    const int64_t rb = row_begin[i];
+    b = q == 0 ? rb : rb + ld_stream(splits + (q - 1) * nlong + i);
+    e = rb + ld_stream(splits + q * nlong + i);
+  }
+  for (; i < r1; i += kRowsPerPass) {  // uniform over the G-group
+    const int64_t inext = i + kRowsPerPass;
+    int64_t bn = 0, en = 0;
+    if (inext < r1) {
+      const int64_t rb = row_begin[inext];
+      bn = q == 0 ? rb : rb + ld_stream(splits + (q - 1) * nlong + inext);
+      en = rb + ld_stream(splits + q * nlong + inext);
+    }
+    const Acc acc = group_sum_lds<G, V, I, Acc, A>(val, col, sx, col0, b, e, lane);
+    if (lane == 0) partial[q * nlong + i] = acc.total_f64();
+    b = bn;
+    e = en;
   }
 }
 
@@ -322,18 +401,19 @@ __global__ __launch_bounds__(kBlock) void panel_list_kernel(
   if (r == rows - 1) *count = pos[r] + flags[r];
 }
 
-// Split table: one thread per listed row, an advancing binary search per
-// panel boundary over the row's sorted columns.
+// Row starts and split table: one thread per listed row, an advancing binary
+// search per panel boundary over the row's sorted columns.
 template <typename I>
 __global__ __launch_bounds__(kBlock) void panel_split_kernel(
     const int64_t* __restrict__ offsets, const I* __restrict__ col,
     const uint32_t* __restrict__ long_rows, int64_t nlong, int64_t panel_cols,
-    int64_t panels, uint32_t* __restrict__ splits) {
+    int64_t panels, int64_t* __restrict__ row_begin, uint32_t* __restrict__ splits) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (i >= nlong) return;
   const int64_t row = long_rows[i];
   const int64_t b = offsets[row];
   const int64_t e = offsets[row + 1];
+  row_begin[i] = b;
   int64_t lo = b;
   for (int64_t q = 1; q < panels; ++q) {
     const int64_t bound = q * panel_cols;
@@ -347,6 +427,7 @@ __global__ __launch_bounds__(kBlock) void panel_split_kernel(
     }
     splits[(q - 1) * nlong + i] = static_cast<uint32_t>(lo - b);
   }
+  splits[(panels - 1) * nlong + i] = static_cast<uint32_t>(e - b);
 }
 
 // ---------------------------------------------------------------------------
@@ -787,16 +868,25 @@ hipError_t launch_typed(const SpmvArgs& a, const KParams& kp, void* ws,
       if (e != hipSuccess || pp.nlong == 0) return e;
       double* partial = static_cast<double*>(ws);
       const int64_t groups = (pp.panels + 7) / 8;
+      const int64_t chunk_rows = (pp.nlong + pp.chunks - 1) / pp.chunks;
       switch (pp.long_alg) {
 #define ASC_LONG_CASE(ALG, G)                                                 \
   case SpmvAlg::ALG: {                                                        \
+    if (pp.lds) {                                                             \
+      spmv_panel_lds_kernel<G, V, I, TWOSUM>                                  \
+          <<<static_cast<unsigned>(pp.panels * pp.chunks), kLdsBlock, 0, s>>>( \
+              kp, a.a.inner, pp.row_begin, pp.splits, pp.nlong, pp.panels,    \
+              pp.panel_cols, chunk_rows, partial);                            \
+      e = hipGetLastError();                                                  \
+      break;                                                                  \
+    }                                                                         \
     const int64_t row_blocks = (pp.nlong + kBlock / G - 1) / (kBlock / G);    \
     const int64_t rb_grid = row_blocks < kPanelRowBlockCap / groups           \
                                 ? row_blocks                                  \
                                 : kPanelRowBlockCap / groups;                 \
     spmv_panel_kernel<G, V, I, TWOSUM>                                        \
         <<<static_cast<unsigned>(8 * groups * rb_grid), kBlock, 0, s>>>(      \
-            kp, pp.long_rows, pp.splits, pp.nlong, pp.panels, row_blocks,     \
+            kp, pp.row_begin, pp.splits, pp.nlong, pp.panels, row_blocks,     \
             rb_grid, partial);                                                \
     e = hipGetLastError();                                                    \
     break;                                                                    \
@@ -868,8 +958,9 @@ size_t panel_analysis_scratch_bytes(int64_t outer) {
 }
 
 hipError_t panel_analysis_launch(const Compressed& a, int64_t panel_cols, int64_t panels,
-                                 int64_t long_min, uint32_t* long_rows, uint32_t* splits,
-                                 void* scratch, int64_t* out_nlong, hipStream_t s) {
+                                 int64_t long_min, uint32_t* long_rows, int64_t* row_begin,
+                                 uint32_t* splits, void* scratch, int64_t* out_nlong,
+                                 hipStream_t s) {
   *out_nlong = 0;
   const int64_t rows = a.outer;
   if (rows == 0) return hipSuccess;
@@ -895,16 +986,16 @@ hipError_t panel_analysis_launch(const Compressed& a, int64_t panel_cols, int64_
   if (e != hipSuccess) return e;
   const int64_t nlong = host_count;
   *out_nlong = nlong;
-  if (nlong == 0 || panels <= 1) return hipSuccess;
+  if (nlong == 0) return hipSuccess;
   const unsigned lblocks = static_cast<unsigned>((nlong + kBlock - 1) / kBlock);
   if (a.index_bytes == 4)
     panel_split_kernel<int32_t><<<lblocks, kBlock, 0, s>>>(
         a.offsets, static_cast<const int32_t*>(a.indices), long_rows, nlong, panel_cols,
-        panels, splits);
+        panels, row_begin, splits);
   else
     panel_split_kernel<int64_t><<<lblocks, kBlock, 0, s>>>(
         a.offsets, static_cast<const int64_t*>(a.indices), long_rows, nlong, panel_cols,
-        panels, splits);
+        panels, row_begin, splits);
   return hipGetLastError();
 }
 

@@ -322,9 +322,12 @@ asc_hip_status asc_sparse_hip_csr_spmv(
  * counterpart: it is the analysis step of this library's ASC_HIP_ALG_PANEL
  * SpMV, which reads the caller's CSR arrays in place.  Rows with at least
  * `long_row_min` entries are listed; the columns are cut into a multiple of 8
- * panels of about `panel_columns` columns, and the listed rows are summed one
- * panel per XCD at a time, so that their x gathers hit an L2-resident panel
- * instead of missing to memory.  Shorter rows run in one vector pass.  The
+ * panels of about `panel_columns` columns, and the listed rows are summed
+ * panel by panel, so that their x gathers hit a resident panel instead of
+ * missing to memory: with L2 staging one panel per XCD at a time stays in
+ * that XCD's L2; with LDS staging each work-group copies its panel (at most
+ * 160 KiB of x) into LDS and walks a chunk of the listed rows.  Shorter rows
+ * run in one vector pass.  The
  * analysis buffer (caller-owned device memory, its size from the query)
  * receives the row list and the per-panel split positions; the call waits for
  * its stream.  On success the context's plan cache maps the matrix to
@@ -334,12 +337,19 @@ asc_hip_status asc_sparse_hip_csr_spmv(
  * <= UINT32_MAX.  Panel partial sums are added in FP64, so this is a
  * plain-mode (any association) algorithm.
  */
+enum {
+  ASC_HIP_STAGING_AUTO = 0, /* LDS when a panel fits in 160 KiB of LDS, else L2 */
+  ASC_HIP_STAGING_L2 = 1,
+  ASC_HIP_STAGING_LDS = 2   /* ASC_HIP_INVALID_ARGUMENT when a panel does not fit */
+};
+
 typedef struct asc_hip_panel_options {
   int64_t panel_columns;   /* 0: 2 MiB of x per panel (2^18 FP64 / 2^19 FP32 columns) */
   int64_t long_row_min;    /* 0: 128 */
   uint8_t short_algorithm; /* ASC_HIP_ALG_VECTOR1..VECTOR64, or AUTO */
   uint8_t long_algorithm;  /* ASC_HIP_ALG_VECTOR1..VECTOR64, or AUTO */
-  uint8_t reserved[6];
+  uint8_t staging;         /* ASC_HIP_STAGING_* */
+  uint8_t reserved[5];
 } asc_hip_panel_options;
 
 asc_hip_status asc_sparse_hip_csr_spmv_analysis_size(

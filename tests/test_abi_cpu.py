@@ -220,15 +220,33 @@ def test_panel_analysis_validation(vctx):
     def scratch(rows):  # flags + positions + count + scan partials (tiles of 4096)
         return 2 * aligned(4 * rows) + 256 + aligned(4 * max(1, (rows + 4095) // 4096))
 
+    def layout(rows, panels):  # row list, row starts, split table, scratch
+        return aligned(4 * rows) + aligned(8 * rows) + aligned(panels * rows * 4) + scratch(rows)
+
     big = dev_csr(rows=1000, cols=10_000_000, nnz=5000)
     # default width 2^18 FP64 columns -> 39 panels, rounded up to 40 (one per XCD at a time)
     assert size(big) == H.OK
-    assert need.value == aligned(4 * 1000) + aligned(39 * 1000 * 4) + scratch(1000)
+    assert need.value == layout(1000, 40)
     f32 = dev_csr(rows=1000, cols=10_000_000, nnz=5000, kind=H.F32)
     assert size(f32) == H.OK  # 2^19 FP32 columns -> 20 panels -> 24
-    assert need.value == aligned(4 * 1000) + aligned(23 * 1000 * 4) + scratch(1000)
+    assert need.value == layout(1000, 24)
     assert size(big, H.panel_options(4_000_000)) == H.OK  # 3 panels -> 8
-    assert need.value == aligned(4 * 1000) + aligned(7 * 1000 * 4) + scratch(1000)
+    assert need.value == layout(1000, 8)
+    # LDS staging: a panel must fit in 160 KiB of x (20480 FP64 / 40960 FP32 columns)
+    # (the width is rounded to a multiple of 8 panels: 20480 -> 496 panels of 20162 columns)
+    assert size(big, H.panel_options(20480, staging=H.STAGING_LDS)) == H.OK
+    assert need.value == layout(1000, 496)
+    assert size(big, H.panel_options(30000, staging=H.STAGING_LDS)) == H.INVALID_ARGUMENT
+    assert size(big, H.panel_options(0, staging=H.STAGING_LDS)) == H.OK  # LDS-sized default
+    assert need.value == layout(1000, 496)
+    # AUTO takes LDS staging once a row averages 48 entries per LDS-sized panel
+    dense_rows = dev_csr(rows=1000, cols=10_000_000, nnz=1000 * 48 * 496)
+    assert size(dense_rows) == H.OK and need.value == layout(1000, 496)
+    sparser = dev_csr(rows=1000, cols=10_000_000, nnz=1000 * 47 * 496)
+    assert size(sparser) == H.OK and need.value == layout(1000, 40)
+    assert size(f32, H.panel_options(40960, staging=H.STAGING_LDS)) == H.OK
+    assert size(f32, H.panel_options(50000, staging=H.STAGING_LDS)) == H.INVALID_ARGUMENT
+    assert size(big, H.panel_options(0, staging=3)) == H.INVALID_ARGUMENT
     assert size(dev_csr()) == H.OK  # 4 columns: still 8 (tiny) panels
     assert size(big, H.panel_options(-1)) == H.INVALID_ARGUMENT
     assert size(big, H.panel_options(0, -5)) == H.INVALID_ARGUMENT

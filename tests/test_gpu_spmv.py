@@ -215,16 +215,21 @@ def test_panel_plan_parity(ctx, name, dtype):
     absax = O.abs_spmv(m.offsets, m.indices, vals.astype(np.float64), x.astype(np.float64))
     a = H.clone_compressed(ctx, H.CSR, m.rows, m.cols, m.offsets, m.indices, vals, KIND[dtype],
                            index_bytes=4)
+    xbytes = 8 if dtype == np.float64 else 4
     for width, long_min in shapes:
-        for short_alg, long_alg in PANEL_KERNELS:
-            H.csr_spmv_analyze(ctx, a, H.panel_options(width, long_min, short_alg, long_alg))
+        # both stagings where a panel fits in LDS (160 KiB of x)
+        fits = 0 < width and width * xbytes <= 160 << 10
+        stagings = (H.STAGING_L2, H.STAGING_LDS) if fits else (H.STAGING_AUTO,)
+        for (short_alg, long_alg), staging in [(k, st) for k in PANEL_KERNELS for st in stagings]:
+            H.csr_spmv_analyze(ctx, a, H.panel_options(width, long_min, short_alg, long_alg, staging))
             assert H.plan_get(ctx, a) == H.ALG_PANEL
             for alpha, beta in ((1.0, 0.0), (-1.5, 0.25)):
                 y_in = y0 if beta == 0.0 else np.nan_to_num(y0)
                 yref = y_in.copy()
                 O.compat_spmv(m.offsets, m.indices, vals, alpha, x, beta, yref)
                 print(f"panel {name} {dtype.__name__} width {width} long>={long_min} "
-                      f"kernels {short_alg},{long_alg} alpha {alpha} beta {beta}", flush=True)
+                      f"kernels {short_alg},{long_alg} staging {staging} alpha {alpha} beta {beta}",
+                      flush=True)
                 for alg in (H.ALG_PANEL, H.ALG_AUTO):
                     y = run_compat(ctx, a, x, y_in, alpha, beta, alg)
                     bound_check(y, yref, lens, absax, alpha, beta, np.nan_to_num(y_in), EPS[dtype])
