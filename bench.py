@@ -322,6 +322,36 @@ def bench_finalize(ctx, extra, n=1 << 26, dim=1 << 22):
     log(f"COO finalize of {n} triplets ({kept} unique): {res['median_ms']:.2f} ms")
 
 
+def bench_dense_rows(ctx, extra, rows=1 << 14, cols=1 << 22, per_row=20_000):
+    """Column-panel SpMV on dense rows: rows x cols with per_row uniformly
+    random columns each (FP64, int32 columns), where the default analysis
+    stages 160 KiB x panels in LDS; the direct vec64 kernel and the L2-staged
+    plan are timed beside it.  Same algorithmic bytes as the headline."""
+    m = G.make(G.UNIFORM_FIXED, rows, cols, seed=G.SEED_BASE + 9, values=G.VALUES_NORMAL,
+               per_row=per_row)
+    a = H.clone_compressed(ctx, H.CSR, m.rows, m.cols, m.offsets, m.indices, m.values, H.F64, 4)
+    nnz = m.nnz
+    del m
+    xb, yb = ctx.upload(G.vector(cols, 5)), ctx.alloc(8 * rows)
+    xd, yd = H.vector(xb, cols, H.F64, writable=False), H.vector(yb, rows, H.F64)
+    ws = H.Workspace(ctx)
+    by = spmv_bytes(rows, cols, nnz, 8, 4, 8, 8)
+    out = {"rows": rows, "cols": cols, "nnz": nnz}
+    r = time_ops(ctx, lambda: H.csr_spmv(ctx, 1.0, a, xd, 0.0, yd, alg=H.ALG_VECTOR[64], ws=ws), 5, 2)
+    out["direct_vec64_gbs"] = round(by / r["median_ms"] / 1e6, 1)
+    for tag, opts in (("panel_l2_gbs", H.panel_options(staging=H.STAGING_L2)),
+                      ("panel_auto_lds_gbs", None)):
+        H.csr_spmv_analyze(ctx, a, opts)
+        r = time_ops(ctx, lambda: H.csr_spmv(ctx, 1.0, a, xd, 0.0, yd, alg=H.ALG_PANEL, ws=ws), 10, 2)
+        out[tag] = round(by / r["median_ms"] / 1e6, 1)
+        if opts is None:
+            out["panel_auto_lds_ms"] = round(r["median_ms"], 3)
+    H.plan_clear(ctx)
+    extra["dense_rows_2^14x2^22_20k"] = out
+    log(f"dense rows: direct {out['direct_vec64_gbs']} GB/s, L2 panels {out['panel_l2_gbs']}, "
+        f"LDS panels {out['panel_auto_lds_gbs']} GB/s")
+
+
 def bench_batched(ctx, extra, count):
     t0 = time.perf_counter()
     specs = G.batched_specs(count)
@@ -389,6 +419,7 @@ def main():
     del m
     if not args.quick:
         bench_precision(ctx, extra)
+        bench_dense_rows(ctx, extra)
         bench_finalize(ctx, extra)
         bench_batched(ctx, extra, args.batch_count)
     traffic, traffic_src = measured_traffic(rows, nnz, name)
