@@ -212,6 +212,18 @@ asc_hip_status check_sparse_meta(const asc_hip_sparse_desc* d, const char* what)
   }
   if (d->nnz > 0 && (d->values == nullptr || (d->format != ASC_HIP_COO && d->inner_indices == nullptr)))
     return fail(ASC_HIP_INVALID_ARGUMENT, std::string(what) + ": arrays are null");
+  // Every array's byte count must be representable (CheckedByteCount,
+  // compressed.h:477-497): the spans checked against allocations use them.
+  const int64_t outer = d->format == ASC_HIP_CSC ? d->cols : d->rows;
+  int64_t bytes = 0;
+  const bool sized =
+      d->format == ASC_HIP_COO
+          ? checked_mul(d->nnz, 16, &bytes)
+          : outer < std::numeric_limits<int64_t>::max() &&
+                checked_mul(outer + 1, d->offset_bytes, &bytes) &&
+                checked_mul(d->nnz, d->index_bytes, &bytes);
+  if (!sized || !checked_mul(d->nnz, static_cast<int64_t>(elem_size(d->element_kind)), &bytes))
+    return fail(ASC_HIP_OVERFLOW, std::string(what) + ": array byte count overflows");
   return ok();
 }
 
@@ -242,6 +254,17 @@ asc_hip_status check_vector(const asc_hip_context* ctx, const asc_hip_vector_des
   if (v->element_kind != kind) return fail(ASC_HIP_INVALID_ARGUMENT, std::string(what) + ": element kind does not match the operation");
   if (v->extent != extent) return fail(ASC_HIP_SHAPE, std::string(what) + ": length does not match the matrix");
   if (v->stride == 0 && extent > 1) return fail(ASC_HIP_INVALID_ARGUMENT, std::string(what) + ": stride must be nonzero");
+  {
+    // The span (extent - 1) * |stride| + 1 elements must have a byte count.
+    const int64_t mag = v->stride == std::numeric_limits<int64_t>::min()
+                            ? -1
+                            : (v->stride < 0 ? -v->stride : v->stride);
+    int64_t reach = 0, bytes = 0;
+    if (extent > 1 && (mag < 0 || !checked_mul(extent - 1, mag, &reach) ||
+                       reach == std::numeric_limits<int64_t>::max() ||
+                       !checked_mul(reach + 1, static_cast<int64_t>(elem_size(kind)), &bytes)))
+      return fail(ASC_HIP_OVERFLOW, std::string(what) + ": strided span overflows");
+  }
   if (writable && !v->writable) return fail(ASC_HIP_INVALID_ARGUMENT, std::string(what) + ": output is not writable");
   if (v->memory_space != ASC_HIP_DEVICE && v->memory_space != ASC_HIP_MANAGED)
     return fail(ASC_HIP_MEMORY_ACCESS, std::string(what) + ": vector is not device memory");
@@ -289,11 +312,20 @@ asc_hip_status check_matrix(const asc_hip_context* ctx, const asc_hip_matrix_des
   if (m == nullptr) return fail(ASC_HIP_INVALID_ARGUMENT, std::string(what) + " descriptor is null");
   if (m->element_kind != kind) return fail(ASC_HIP_INVALID_ARGUMENT, std::string(what) + ": element kind does not match");
   if (m->layout > ASC_HIP_ROW_MAJOR) return fail(ASC_HIP_INVALID_ARGUMENT, std::string(what) + ": unknown layout");
-  if (m->rows != rows || (cols >= 0 && m->cols != cols))
+  if (m->rows != rows || m->cols < 0 || (cols >= 0 && m->cols != cols))
     return fail(ASC_HIP_SHAPE, "Sparse matrix-matrix operand extents do not match");
   const int64_t minor = m->layout == ASC_HIP_ROW_MAJOR ? m->cols : m->rows;
   if (m->leading_dimension < (minor > 1 ? minor : 1))
     return fail(ASC_HIP_INVALID_ARGUMENT, std::string(what) + ": leading dimension is too small");
+  if (m->rows > 0 && m->cols > 0) {
+    // ((major - 1) * ld + minor) elements must have a byte count.
+    const int64_t major = m->layout == ASC_HIP_ROW_MAJOR ? m->rows : m->cols;
+    int64_t reach = 0, bytes = 0;
+    if (!checked_mul(major - 1, m->leading_dimension, &reach) ||
+        reach > std::numeric_limits<int64_t>::max() - minor ||
+        !checked_mul(reach + minor, static_cast<int64_t>(elem_size(kind)), &bytes))
+      return fail(ASC_HIP_OVERFLOW, std::string(what) + ": block span overflows");
+  }
   if (writable && !m->writable) return fail(ASC_HIP_INVALID_ARGUMENT, std::string(what) + ": output is not writable");
   if (m->memory_space != ASC_HIP_DEVICE && m->memory_space != ASC_HIP_MANAGED)
     return fail(ASC_HIP_MEMORY_ACCESS, std::string(what) + ": block is not device memory");

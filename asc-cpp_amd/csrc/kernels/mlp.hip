@@ -155,7 +155,8 @@ hipError_t mlp_grad_launch(const float* params, int k, int64_t batch, const floa
   float* dh1 = dh2 + batch * 64;
   float* partial = dh1 + batch * 64;
   MlpPtrs g = split(grads, k);
-  mlp_forward_launch(params, k, batch, x, y, scratch, s);
+  const hipError_t fe = mlp_forward_launch(params, k, batch, x, y, scratch, s);
+  if (fe != hipSuccess) return fe;
   const int64_t n = batch * k;
   mse_grad_kernel<<<blocks(n), kBlock, 0, s>>>(y, target, n, dy, partial);
   loss_final_kernel<<<1, 64, 0, s>>>(partial, static_cast<int>(blocks(n)), n, loss);

@@ -397,3 +397,29 @@ def test_triangular_validation(vctx):
     assert lib.asc_sparse_hip_triangular_solve(h, H.OP_N, H.Scalar(1.0, 0.0), C.byref(sq),
                                                C.byref(wrong_rows), ws, need.value,
                                                C.byref(good)).code == H.SHAPE
+
+
+def test_span_byte_counts_that_overflow_are_refused(vctx):
+    """Array, strided-vector and dense-block byte counts must be representable
+    (CheckedByteCount, compressed.h:477-497): an operand whose span would wrap
+    is refused before its allocation is looked at."""
+    big = 1 << 61
+    # (rows + 1) int64 offsets overflow: rows = 2^61, one column, no entries
+    a = dev_csr(rows=big, cols=1, nnz=0)
+    x, y = dev_vec(1), dev_vec(big)
+    assert spmv_code(vctx, a, x, y, alg=H.ALG_ORDERED) == H.OVERFLOW
+    # a stride whose span wraps
+    a = dev_csr()
+    assert spmv_code(vctx, a, dev_vec(4, stride=1 << 62), dev_vec(3), alg=H.ALG_ORDERED) == H.OVERFLOW
+    assert spmv_code(vctx, a, dev_vec(4), dev_vec(3, stride=-(1 << 62)), alg=H.ALG_ORDERED) == H.OVERFLOW
+    assert spmv_code(vctx, a, dev_vec(4), dev_vec(3, stride=-(2 ** 63)), alg=H.ALG_ORDERED) == H.OVERFLOW
+    # large strides that still fit pass validation (a validation-only context
+    # then refuses to enqueue)
+    assert spmv_code(vctx, a, dev_vec(4, stride=1 << 40), dev_vec(3, stride=-(1 << 40)),
+                     alg=H.ALG_ORDERED) == H.UNAVAILABLE
+    # a dense block whose leading dimension makes the span wrap
+    b = H.MatrixDesc(0x200000, 4, 2, 1 << 62, H.COLUMN_MAJOR, H.F64, H.DEVICE, 0)
+    c = H.MatrixDesc(0x300000, 3, 2, 3, H.COLUMN_MAJOR, H.F64, H.DEVICE, 1)
+    st = vctx.lib.asc_sparse_hip_standard_spmm(vctx.handle, H.OP_N, H.Scalar(1.0, 0.0), C.byref(a),
+                                               C.byref(b), C.byref(c), None, 0)
+    assert st.code == H.OVERFLOW
