@@ -192,12 +192,31 @@ __global__ __launch_bounds__(kBlock) void radix_hist_kernel(
 template <typename T>
 __device__ __forceinline__ T ld_plain(const T* p) { return *p; }
 
+// Value bits through one 8-byte LDS slot.
+template <typename VB>
+__device__ __forceinline__ uint2 pack_bits(VB v) {
+  static_assert(sizeof(VB) <= 8, "value payloads are at most 8 bytes");
+  uint2 r{0u, 0u};
+  __builtin_memcpy(&r, &v, sizeof(VB));
+  return r;
+}
+template <typename VB>
+__device__ __forceinline__ VB unpack_bits(uint2 r) {
+  VB v;
+  __builtin_memcpy(&v, &r, sizeof(VB));
+  return v;
+}
+
 // One stable scatter round.  Entry i of the tile is handled by wave
 // (i / kWaveItems), chunk ((i % kWaveItems) / 64), lane (i % 64), so chunk
-// order and lane order are source order.  The tile is first reordered by
-// digit in LDS (stable: digit, then wave, then rank within the wave), then
-// written out by consecutive threads, so each digit's run of the tile leaves
-// as contiguous stores instead of one scattered store per entry.
+// order and lane order are source order.  Keys, rows and values are all
+// loaded first, so their latency overlaps the ranking.  The tile is then
+// reordered by digit in LDS (stable: digit, then wave, then rank within the
+// wave) and written out by consecutive threads, so each digit's run of the
+// tile leaves as contiguous stores instead of one scattered store per entry.
+// The reorder goes through one 8-byte-per-entry LDS buffer in two rounds
+// (key and row, then value), which keeps the block at 42 KiB of LDS
+// (three blocks per CU).
 template <typename K, typename R, typename VB, typename RO>
 __global__ __launch_bounds__(kBlock) void radix_scatter_kernel(
     const K* __restrict__ keys, int64_t key_stride, const R* __restrict__ rows,
@@ -210,26 +229,39 @@ __global__ __launch_bounds__(kBlock) void radix_scatter_kernel(
   __shared__ uint32_t s_local[kRadix];              // tile-local start of each digit
   __shared__ uint32_t s_global[kRadix];             // output start of each digit
   __shared__ uint32_t s_tmp[kWaves];
-  __shared__ uint32_t s_key[kTile];
-  __shared__ uint32_t s_row[kTile];
-  __shared__ VB s_val[kTile];
+  __shared__ uint2 s_stage[kTile];
   const int lane = threadIdx.x % kWave;
   const int wave = threadIdx.x / kWave;
   for (int d = threadIdx.x; d < kWaves * kRadix; d += kBlock)
     (&s_count[0][0])[d] = 0;
-  __syncthreads();
   const int64_t tile = blockIdx.x;
   const int64_t tile_base = tile * kTile;
   const int64_t base = tile_base + static_cast<int64_t>(wave) * kWaveItems;
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
   uint32_t key[kIpt];
-  uint32_t rank[kIpt];
+  uint32_t row[kIpt];
+  VB val[kIpt];
+#pragma unroll
+  for (int u = 0; u < kIpt; ++u) {
+    const int64_t i = base + u * kWave + lane;
+    if (i < n) {
+      key[u] = read_key(keys, key_stride, i);
+      row[u] = static_cast<uint32_t>(ld_stream(rows + i * row_stride));
+      val[u] = ld_stream(vals + i);
+    } else {
+      key[u] = 0u;
+      row[u] = 0u;
+      val[u] = VB{};
+    }
+  }
+  __syncthreads();  // s_count cleared
+
+  uint32_t pos[kIpt];
 #pragma unroll
   for (int u = 0; u < kIpt; ++u) {
     const int64_t i = base + u * kWave + lane;
     const bool valid = i < n;
-    key[u] = valid ? read_key(keys, key_stride, i) : 0u;
     const uint32_t d = (key[u] >> shift) & (kRadix - 1);
     uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -241,7 +273,7 @@ __global__ __launch_bounds__(kBlock) void radix_scatter_kernel(
     const uint32_t total = __popcll(peers);
     uint32_t r = 0;
     if (valid) r = s_count[wave][d] + lower;
-    rank[u] = r;
+    pos[u] = r;
     // The highest peer publishes the new running count after every peer
     // has read the old one (LDS operations of one wave retire in order).
     if (valid && lower == total - 1) s_count[wave][d] += total;
@@ -260,32 +292,44 @@ __global__ __launch_bounds__(kBlock) void radix_scatter_kernel(
     s_global[d] = offsets[static_cast<int64_t>(d) * ntiles + tile];
   }
   __syncthreads();
-  // Stage the tile in digit order.
+  // Round 1: keys and rows in digit order.
 #pragma unroll
   for (int u = 0; u < kIpt; ++u) {
     const int64_t i = base + u * kWave + lane;
     if (i < n) {
       const uint32_t d = (key[u] >> shift) & (kRadix - 1);
-      const uint32_t pos = s_local[d] + s_wave_off[wave][d] + rank[u];
-      s_key[pos] = key[u];
-      s_row[pos] = static_cast<uint32_t>(ld_stream(rows + i * row_stride));
-      s_val[pos] = ld_stream(vals + i);
+      pos[u] += s_local[d] + s_wave_off[wave][d];
+      s_stage[pos[u]] = make_uint2(key[u], row[u]);
     }
   }
   __syncthreads();
-  // Consecutive threads write consecutive entries of each digit's run.
+  // Consecutive threads write consecutive entries of each digit's run; the
+  // destinations are kept for the value round (`key` is reused for them).
   const int64_t valid_in_tile = n - tile_base < kTile ? n - tile_base : kTile;
 #pragma unroll
   for (int u = 0; u < kIpt; ++u) {
     const int t = u * kBlock + threadIdx.x;
     if (t < valid_in_tile) {
-      const uint32_t k = s_key[t];
-      const uint32_t d = (k >> shift) & (kRadix - 1);
+      const uint2 kr = s_stage[t];
+      const uint32_t d = (kr.x >> shift) & (kRadix - 1);
       const uint32_t dst = s_global[d] + (static_cast<uint32_t>(t) - s_local[d]);
-      keys_out[dst] = k;
-      rows_out[dst] = static_cast<RO>(s_row[t]);
-      vals_out[dst] = s_val[t];
+      keys_out[dst] = kr.x;
+      rows_out[dst] = static_cast<RO>(kr.y);
+      key[u] = dst;
     }
+  }
+  __syncthreads();
+  // Round 2: values in digit order.
+#pragma unroll
+  for (int u = 0; u < kIpt; ++u) {
+    const int64_t i = base + u * kWave + lane;
+    if (i < n) s_stage[pos[u]] = pack_bits(val[u]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < kIpt; ++u) {
+    const int t = u * kBlock + threadIdx.x;
+    if (t < valid_in_tile) vals_out[key[u]] = unpack_bits<VB>(s_stage[t]);
   }
 }
 
