@@ -22,11 +22,16 @@ namespace asc_hip {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kIpt = 16;
-constexpr int kTile = kBlock * kIpt;  // entries per radix tile
 constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;
 constexpr int kWaves = kBlock / kWave;
+// Radix scatter blocks: 16 waves of 8 entries per lane, so a digit's run of
+// one tile averages 32 entries (longer contiguous stores) and one block per
+// CU holds the LDS reorder buffer.
+constexpr int kSortBlock = 1024;
+constexpr int kIpt = 16;
+constexpr int kTile = kSortBlock * kIpt;  // entries per radix tile
+constexpr int kSortWaves = kSortBlock / kWave;
 constexpr int kWaveItems = kWave * kIpt;
 
 inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
@@ -177,8 +182,8 @@ __global__ __launch_bounds__(kBlock) void radix_hist_kernel(
   __syncthreads();
   const int64_t tile = blockIdx.x;
   const int64_t base = tile * kTile;
-#pragma unroll
-  for (int u = 0; u < kIpt; ++u) {
+#pragma unroll 8
+  for (int u = 0; u < kTile / kBlock; ++u) {
     const int64_t i = base + u * kBlock + threadIdx.x;
     if (i < n) {
       const uint32_t d = (read_key(keys, key_stride, i) >> shift) & (kRadix - 1);
@@ -218,21 +223,22 @@ __device__ __forceinline__ VB unpack_bits(uint2 r) {
 // (key and row, then value), which keeps the block at 42 KiB of LDS
 // (three blocks per CU).
 template <typename K, typename R, typename VB, typename RO>
-__global__ __launch_bounds__(kBlock) void radix_scatter_kernel(
+__global__ __launch_bounds__(kSortBlock) void radix_scatter_kernel(
     const K* __restrict__ keys, int64_t key_stride, const R* __restrict__ rows,
     int64_t row_stride, const VB* __restrict__ vals, int64_t n, int shift,
     const uint32_t* __restrict__ offsets, int64_t ntiles,
     uint32_t* __restrict__ keys_out, RO* __restrict__ rows_out,
     VB* __restrict__ vals_out) {
-  __shared__ uint32_t s_count[kWaves][kRadix];
-  __shared__ uint32_t s_wave_off[kWaves][kRadix];  // earlier waves' counts per digit
-  __shared__ uint32_t s_local[kRadix];              // tile-local start of each digit
-  __shared__ uint32_t s_global[kRadix];             // output start of each digit
-  __shared__ uint32_t s_tmp[kWaves];
+  // s_count holds each wave's digit counts, then (in place) the counts of
+  // the earlier waves per digit.
+  __shared__ uint32_t s_count[kSortWaves][kRadix];
+  __shared__ uint32_t s_local[kRadix];   // tile-local start of each digit
+  __shared__ uint32_t s_global[kRadix];  // output start of each digit
+  __shared__ uint32_t s_tmp[kRadix / kWave];
   __shared__ uint2 s_stage[kTile];
   const int lane = threadIdx.x % kWave;
   const int wave = threadIdx.x / kWave;
-  for (int d = threadIdx.x; d < kWaves * kRadix; d += kBlock)
+  for (int d = threadIdx.x; d < kSortWaves * kRadix; d += kSortBlock)
     (&s_count[0][0])[d] = 0;
   const int64_t tile = blockIdx.x;
   const int64_t tile_base = tile * kTile;
@@ -279,17 +285,31 @@ __global__ __launch_bounds__(kBlock) void radix_scatter_kernel(
     if (valid && lower == total - 1) s_count[wave][d] += total;
   }
   __syncthreads();
-  {
-    const int d = threadIdx.x;  // kBlock == kRadix
-    uint32_t run = 0;
+  // Digit scan by the first kRadix threads: per digit, the earlier waves'
+  // counts (in place) and the digit's tile-local start.
+  uint32_t run = 0, incl = 0;
+  if (threadIdx.x < kRadix) {
+    const int d = threadIdx.x;
 #pragma unroll
-    for (int w = 0; w < kWaves; ++w) {
-      s_wave_off[w][d] = run;
-      run += s_count[w][d];
+    for (int w = 0; w < kSortWaves; ++w) {
+      const uint32_t c = s_count[w][d];
+      s_count[w][d] = run;
+      run += c;
     }
-    uint32_t all;
-    s_local[d] = block_exclusive_scan(run, s_tmp, &all);
+    incl = run;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const uint32_t y = __shfl_up(incl, off);
+      if (lane >= off) incl += y;
+    }
+    if (lane == kWave - 1) s_tmp[wave] = incl;
     s_global[d] = offsets[static_cast<int64_t>(d) * ntiles + tile];
+  }
+  __syncthreads();
+  if (threadIdx.x < kRadix) {
+    uint32_t before = 0;
+    for (int w = 0; w < wave; ++w) before += s_tmp[w];
+    s_local[threadIdx.x] = before + incl - run;
   }
   __syncthreads();
   // Round 1: keys and rows in digit order.
@@ -298,7 +318,7 @@ __global__ __launch_bounds__(kBlock) void radix_scatter_kernel(
     const int64_t i = base + u * kWave + lane;
     if (i < n) {
       const uint32_t d = (key[u] >> shift) & (kRadix - 1);
-      pos[u] += s_local[d] + s_wave_off[wave][d];
+      pos[u] += s_local[d] + s_count[wave][d];
       s_stage[pos[u]] = make_uint2(key[u], row[u]);
     }
   }
@@ -308,7 +328,7 @@ __global__ __launch_bounds__(kBlock) void radix_scatter_kernel(
   const int64_t valid_in_tile = n - tile_base < kTile ? n - tile_base : kTile;
 #pragma unroll
   for (int u = 0; u < kIpt; ++u) {
-    const int t = u * kBlock + threadIdx.x;
+    const int t = u * kSortBlock + threadIdx.x;
     if (t < valid_in_tile) {
       const uint2 kr = s_stage[t];
       const uint32_t d = (kr.x >> shift) & (kRadix - 1);
@@ -328,7 +348,7 @@ __global__ __launch_bounds__(kBlock) void radix_scatter_kernel(
   __syncthreads();
 #pragma unroll
   for (int u = 0; u < kIpt; ++u) {
-    const int t = u * kBlock + threadIdx.x;
+    const int t = u * kSortBlock + threadIdx.x;
     if (t < valid_in_tile) vals_out[key[u]] = unpack_bits<VB>(s_stage[t]);
   }
 }
@@ -454,7 +474,7 @@ hipError_t stable_sort(const K0* key0, int64_t key_stride, const R0* row0,
       return err;
     const unsigned g = static_cast<unsigned>(ntiles);
 #define ASC_SCATTER(KT, KP, KS, RT, RP, RS, VP, ROT, ROP)                        \
-  radix_scatter_kernel<KT, RT, VB, ROT><<<g, kBlock, 0, s>>>(                    \
+  radix_scatter_kernel<KT, RT, VB, ROT><<<g, kSortBlock, 0, s>>>(                \
       KP, KS, RP, RS, VP, n, shift, scanned, ntiles, kbuf[out], ROP,             \
       last ? out_vals : vbuf[out])
     if (pass == 0) {
