@@ -263,10 +263,12 @@ def test_panel_plan_strides_and_empty_rows(ctx):
     H.plan_clear(ctx)
 
 
-def test_panel_plan_standard_spmv_and_precision(ctx):
+@pytest.mark.parametrize("staging", [H.STAGING_L2, H.STAGING_LDS])
+def test_panel_plan_standard_spmv_and_precision(ctx, staging):
     m = G.make(G.ZIPF, 1 << 14, 1 << 14, seed=41, values=G.VALUES_NORMAL)
     a = H.clone_compressed(ctx, H.CSR, m.rows, m.cols, m.offsets, m.indices, m.values, H.F64, 4)
-    H.csr_spmv_analyze(ctx, a, H.panel_options(1500, 32))
+    H.csr_spmv_analyze(ctx, a, H.panel_options(1500, 32, staging=staging))
+    print(f"panel standard/precision staging {staging}", flush=True)
     x = G.vector(m.cols, 5)
     y0 = G.vector(m.rows, 6)
     yref = y0.copy()
@@ -278,12 +280,39 @@ def test_panel_plan_standard_spmv_and_precision(ctx):
     absax = O.abs_spmv(m.offsets, m.indices, m.values, x)
     bound_check(y, yref, np.diff(m.offsets), absax, 0.75, 1.0, y0, EPS[np.float64])
     # BF16 storage, FP32 accumulation (plain and compensated) through the plan
-    bf = H.convert_values(ctx, a, H.BF16)
-    H.csr_spmv_analyze(ctx, bf, H.panel_options(1500, 32))
     y64 = np.zeros(m.rows)
     O.compat_spmv(m.offsets, m.indices, m.values, 1.0, x, 0.0, y64)
-    for acc in (H.ACC_PLAIN, H.ACC_TWOSUM):
-        y = run_compat(ctx, bf, x, np.zeros(m.rows), 1.0, 0.0, H.ALG_PANEL, acc=acc)
-        err = np.max(np.abs(y - y64)) / np.max(absax)
-        assert err <= 1e-2, err
+    for kind, tol in ((H.BF16, 1e-2), (H.F16, 1e-2), (H.F32, 1e-6)):
+        low = H.convert_values(ctx, a, kind)
+        H.csr_spmv_analyze(ctx, low, H.panel_options(1500, 32, staging=staging))
+        for acc in (H.ACC_PLAIN, H.ACC_TWOSUM):
+            y = run_compat(ctx, low, x, np.zeros(m.rows), 1.0, 0.0, H.ALG_PANEL, acc=acc)
+            err = np.max(np.abs(y - y64)) / np.max(absax)
+            assert err <= tol, (kind, acc, err)
+    H.plan_clear(ctx)
+
+
+def test_lds_panels_negative_stride_and_dense_rows(ctx):
+    """LDS staging with x read through a negative stride, on rows dense enough
+    that AUTO staging picks LDS by itself (48+ entries per LDS-sized panel)."""
+    rows, cols, per_row = 64, 1 << 20, 4000  # 56 LDS panels of 18725 columns: 71 per segment
+    m = G.make(G.UNIFORM_FIXED, rows, cols, seed=44, values=G.VALUES_NORMAL, per_row=per_row)
+    a = H.clone_compressed(ctx, H.CSR, m.rows, m.cols, m.offsets, m.indices, m.values, H.F64, 4)
+    need_auto = H.csr_spmv_analysis_size(ctx, a)
+    need_l2 = H.csr_spmv_analysis_size(ctx, a, H.panel_options(staging=H.STAGING_L2))
+    assert need_auto > need_l2  # AUTO took the LDS-sized panels (more of them)
+    H.csr_spmv_analyze(ctx, a)
+    x = G.vector(cols * 2, 8)
+    xs = np.ascontiguousarray(x[::-2])  # logical x for stride -2 from the far end
+    y0 = G.vector(rows, 9)
+    yref = y0.copy()
+    O.compat_spmv(m.offsets, m.indices, m.values, -1.25, xs, 0.5, yref)
+    absax = O.abs_spmv(m.offsets, m.indices, m.values, xs)
+    xb, yb = ctx.upload(x), ctx.upload(y0)
+    last = xb.ptr + 8 * (2 * cols - 1)  # element 0 of a stride -2 view sits at the far end
+    print("LDS panels, negative stride x", flush=True)
+    H.csr_spmv(ctx, -1.25, a, H.vector(last, cols, H.F64, stride=-2, writable=False), 0.5,
+               H.vector(yb, rows, H.F64), alg=H.ALG_PANEL)
+    y = ctx.download(yb, np.float64, rows)
+    bound_check(y, yref, np.diff(m.offsets), absax, -1.25, 0.5, y0, EPS[np.float64])
     H.plan_clear(ctx)
