@@ -32,6 +32,14 @@ struct asc_hip_context {
     asc_hip::PanelPlan panel;  // ASC_HIP_ALG_PANEL only
   };
   std::vector<Plan> plans;
+  // Level bounds of this context's triangular analyses, by analysis buffer
+  // (they only shape the solve's launches; the device copy is the truth).
+  struct TrsvSchedule {
+    const void* analysis;
+    int64_t levels;
+    std::vector<int64_t> level_ptr;
+  };
+  std::vector<TrsvSchedule> trsv_schedules;  // guarded by plan_mutex
 };
 
 namespace asc_hip_abi {

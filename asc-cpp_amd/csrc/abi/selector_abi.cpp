@@ -89,6 +89,7 @@ asc_hip_status asc_sparse_hip_plan_clear(asc_hip_context* ctx) {
   if (st.code) return st;
   std::lock_guard<std::mutex> lock(ctx->plan_mutex);
   ctx->plans.clear();
+  ctx->trsv_schedules.clear();
   return ok();
 }
 

@@ -14,6 +14,8 @@ using namespace asc_hip_abi;
 namespace {
 
 constexpr int64_t kU32Max = std::numeric_limits<uint32_t>::max();
+// Triangular schedules a context keeps (the oldest goes first).
+constexpr size_t kMaxSchedules = 64;
 
 inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
@@ -142,6 +144,17 @@ asc_hip_status asc_sparse_hip_triangular_analyze(asc_hip_context* ctx, uint8_t t
   info->triangle = triangle;
   info->diagonal = diagonal;
   info->format = a->format;
+  {
+    std::lock_guard<std::mutex> lock(ctx->plan_mutex);
+    auto& list = ctx->trsv_schedules;
+    for (size_t i = 0; i < list.size(); ++i)
+      if (list[i].analysis == analysis) {
+        list.erase(list.begin() + static_cast<std::ptrdiff_t>(i));
+        break;
+      }
+    if (list.size() >= kMaxSchedules) list.erase(list.begin());  // oldest first
+    list.push_back({analysis, lv.levels, std::move(lv.host_level_ptr)});
+  }
   return ok();
 }
 
@@ -194,6 +207,14 @@ asc_hip_status asc_sparse_hip_triangular_solve(asc_hip_context* ctx, uint8_t tra
   asc_hip::TrsvLevels lv;
   lv.levels = info->levels;
   lv.max_level_rows = info->max_level_rows;
+  {
+    std::lock_guard<std::mutex> lock(ctx->plan_mutex);
+    for (const auto& sch : ctx->trsv_schedules)
+      if (sch.analysis == analysis && sch.levels == info->levels) {
+        lv.host_level_ptr = sch.level_ptr;
+        break;
+      }
+  }
   const hipError_t e = asc_hip::trsv_solve_launch(
       rows, solve_lower, info->diagonal == ASC_HIP_UNIT, alpha.real,
       reinterpret_cast<const int64_t*>(w + L.level_ptr),

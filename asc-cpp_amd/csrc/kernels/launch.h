@@ -7,6 +7,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <vector>
+
 namespace asc_hip {
 
 enum class Elem : uint8_t { F32 = 0, F64 = 1, BF16 = 4, F16 = 5 };
@@ -108,6 +110,9 @@ hipError_t sort_by_key_launch(const uint32_t* keys, int64_t n, int64_t key_bound
 struct TrsvLevels {
   int64_t levels = 0;
   int64_t max_level_rows = 0;
+  // level_ptr on the host (levels + 1 entries) when known: the solve then
+  // runs each stretch of small levels in one work-group launch.
+  std::vector<int64_t> host_level_ptr;
 };
 size_t trsv_analysis_scratch_bytes(int64_t n);
 hipError_t trsv_analysis_launch(const Compressed& stored, bool stored_inner_le_outer, bool unit,

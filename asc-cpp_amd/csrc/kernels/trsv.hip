@@ -16,6 +16,7 @@
 // entry contributes an exact zero product there).
 #include <algorithm>
 #include <cstdlib>
+#include <vector>
 
 #include "device_common.h"
 #include "launch.h"
@@ -83,18 +84,13 @@ __global__ __launch_bounds__(kBlock) void trsv_level_round_kernel(
   }
 }
 
+// One (row, right-hand side) of a level: the row's serial chain.
 template <typename T, typename I>
-__global__ __launch_bounds__(kBlock) void trsv_level_solve_kernel(
-    const int64_t* __restrict__ off, const I* __restrict__ idx, const T* __restrict__ val,
-    const int64_t* __restrict__ level_ptr, const uint32_t* __restrict__ row_order, int64_t level,
-    int unit, T alpha, T* __restrict__ x, int64_t rs, int64_t cs, int64_t k) {
+__device__ __forceinline__ void solve_row(const int64_t* __restrict__ off,
+                                          const I* __restrict__ idx, const T* __restrict__ val,
+                                          int64_t row, int64_t j, int unit, T alpha,
+                                          T* __restrict__ x, int64_t rs, int64_t cs) {
 #pragma clang fp contract(off)
-  const int64_t begin = level_ptr[level];
-  const int64_t count = level_ptr[level + 1] - begin;
-  const int64_t t = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  if (t >= count * k) return;
-  const int64_t row = row_order[begin + t % count];
-  const int64_t j = t / count;
   T value = alpha * x[row * rs + j * cs];
   T diag = T(0);
   for (int64_t p = off[row]; p < off[row + 1]; ++p) {
@@ -108,6 +104,42 @@ __global__ __launch_bounds__(kBlock) void trsv_level_solve_kernel(
   }
   if (!unit) value = value / diag;
   x[row * rs + j * cs] = value;
+}
+
+template <typename T, typename I>
+__global__ __launch_bounds__(kBlock) void trsv_level_solve_kernel(
+    const int64_t* __restrict__ off, const I* __restrict__ idx, const T* __restrict__ val,
+    const int64_t* __restrict__ level_ptr, const uint32_t* __restrict__ row_order, int64_t level,
+    int unit, T alpha, T* __restrict__ x, int64_t rs, int64_t cs, int64_t k) {
+  const int64_t begin = level_ptr[level];
+  const int64_t count = level_ptr[level + 1] - begin;
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (t >= count * k) return;
+  solve_row<T, I>(off, idx, val, row_order[begin + t % count], t / count, unit, alpha, x, rs, cs);
+}
+
+// Consecutive small levels [l0, l1) in one work-group: a barrier between
+// levels orders each level's writes of x before the next level's reads (the
+// work-group's waves share one CU and its L1), so a run of small levels
+// costs one launch instead of one per level.
+constexpr int kGroupBlock = 1024;
+// Row solves of one level (times right-hand sides) that a grouped launch
+// takes on: two passes of its threads.
+constexpr int64_t kGroupMax = 2 * kGroupBlock;
+
+template <typename T, typename I>
+__global__ __launch_bounds__(kGroupBlock) void trsv_group_solve_kernel(
+    const int64_t* __restrict__ off, const I* __restrict__ idx, const T* __restrict__ val,
+    const int64_t* __restrict__ level_ptr, const uint32_t* __restrict__ row_order, int64_t l0,
+    int64_t l1, int unit, T alpha, T* __restrict__ x, int64_t rs, int64_t cs, int64_t k) {
+  for (int64_t l = l0; l < l1; ++l) {
+    const int64_t begin = level_ptr[l];
+    const int64_t count = level_ptr[l + 1] - begin;
+    for (int64_t t = threadIdx.x; t < count * k; t += kGroupBlock)
+      solve_row<T, I>(off, idx, val, row_order[begin + t % count], t / count, unit, alpha, x, rs,
+                      cs);
+    __syncthreads();
+  }
 }
 
 template <typename T>
@@ -130,13 +162,98 @@ hipError_t check_typed(const Compressed& stored, bool inner_le_outer, bool unit,
   return hipGetLastError();
 }
 
+// Levels of deep schedules in one work-group, in solve order: chunks of
+// kDpBlock rows, each row's level from its dependencies outside the chunk
+// (already final), then a Jacobi fixed point over the chunk in LDS:
+//   level[i] = max(outside bound, 1 + level[j] for dependencies j in the chunk).
+// Starting from the outside bounds the values only grow, never past the
+// true levels, and a pass without change ends the chunk; a triangular
+// chunk's longest internal chain has at most kDpBlock - 1 edges, so that
+// happens within kDpBlock + 1 passes.  out[0] gets the number of levels.
+constexpr int kDpBlock = 1024;
+
 template <typename I>
-hipError_t level_typed(const Compressed& rows, int32_t* level, uint32_t* flags, uint32_t* host,
-                       int64_t* levels, hipStream_t s) {
+__global__ __launch_bounds__(kDpBlock) void trsv_level_dp_kernel(
+    const int64_t* __restrict__ off, const I* __restrict__ idx, int64_t n, int lower,
+    int32_t* __restrict__ level, uint32_t* __restrict__ out) {
+  __shared__ int32_t s_a[kDpBlock];
+  __shared__ int32_t s_b[kDpBlock];
+  __shared__ int32_t s_max[kDpBlock / kWave];
+  __shared__ int s_changed;
+  int32_t top = -1;
+  const int64_t chunks = (n + kDpBlock - 1) / kDpBlock;
+  for (int64_t c = 0; c < chunks; ++c) {
+    const int64_t base = lower ? c * kDpBlock
+                               : (n - (c + 1) * kDpBlock > 0 ? n - (c + 1) * kDpBlock : 0);
+    const int64_t end = lower ? (base + kDpBlock < n ? base + kDpBlock : n) : n - c * kDpBlock;
+    const int64_t i = base + threadIdx.x;
+    const bool active = i < end;
+    int32_t bound = 0;
+    if (active)
+      for (int64_t p = off[i]; p < off[i + 1]; ++p) {
+        const int64_t j = static_cast<int64_t>(idx[p]);
+        if (j == i || (j >= base && j < end)) continue;
+        bound = level[j] + 1 > bound ? level[j] + 1 : bound;
+      }
+    s_a[threadIdx.x] = bound;
+    int32_t* cur = s_a;
+    int32_t* nxt = s_b;
+    for (int pass = 0; pass <= kDpBlock; ++pass) {
+      if (threadIdx.x == 0) s_changed = 0;
+      __syncthreads();
+      int32_t v = bound;
+      if (active)
+        for (int64_t p = off[i]; p < off[i + 1]; ++p) {
+          const int64_t j = static_cast<int64_t>(idx[p]);
+          if (j != i && j >= base && j < end) {
+            const int32_t lj = cur[j - base] + 1;
+            v = lj > v ? lj : v;
+          }
+        }
+      nxt[threadIdx.x] = v;
+      if (active && v != cur[threadIdx.x]) s_changed = 1;
+      __syncthreads();
+      const int changed = s_changed;
+      int32_t* t = cur;
+      cur = nxt;
+      nxt = t;
+      if (!changed) break;  // uniform: every thread read s_changed after the barrier
+      __syncthreads();      // all reads of s_changed precede its next reset
+    }
+    if (active) {
+      level[i] = cur[threadIdx.x];
+      top = cur[threadIdx.x] > top ? cur[threadIdx.x] : top;
+    }
+    __syncthreads();  // this chunk's levels are visible to the next chunk's rows
+  }
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    const int32_t other = __shfl_xor(top, o);
+    top = other > top ? other : top;
+  }
+  if (threadIdx.x % kWave == 0) s_max[threadIdx.x / kWave] = top;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int32_t m = -1;
+    for (int w = 0; w < kDpBlock / kWave; ++w) m = s_max[w] > m ? s_max[w] : m;
+    out[0] = static_cast<uint32_t>(m + 1);
+  }
+}
+
+// Shallow schedules level in a few synchronized multi-block rounds; a
+// schedule still unfinished after kRounds rounds is levelled from scratch
+// by the one-work-group pass above (no per-level host synchronization).
+constexpr int32_t kRounds = 8;
+
+template <typename I>
+hipError_t level_typed(const Compressed& rows, bool lower, int32_t* level, uint32_t* flags,
+                       uint32_t* host, int64_t* levels, hipStream_t s) {
   const int64_t n = rows.outer;
+  *levels = 0;
+  if (n == 0) return hipSuccess;
   trsv_level_init_kernel<<<grid_for(n), kBlock, 0, s>>>(level, n);
   hipError_t e = hipGetLastError();
-  for (int32_t round = 0; e == hipSuccess && n > 0; ++round) {
+  for (int32_t round = 0; e == hipSuccess && round < kRounds; ++round) {
     e = fill_launch(flags, 0, 8, s);
     if (e != hipSuccess) break;
     trsv_level_round_kernel<I><<<grid_for(n), kBlock, 0, s>>>(
@@ -150,8 +267,14 @@ hipError_t level_typed(const Compressed& rows, int32_t* level, uint32_t* flags, 
     }
     if (host[0] == 0u) return hipErrorInvalidValue;  // no progress: not triangular
   }
-  if (n == 0) *levels = 0;
-  return e;
+  if (e != hipSuccess) return e;
+  trsv_level_dp_kernel<I><<<1, kDpBlock, 0, s>>>(rows.offsets, static_cast<const I*>(rows.indices),
+                                                 n, lower ? 1 : 0, level, flags);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = hipMemcpyAsync(host, flags, 4, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+  *levels = static_cast<int64_t>(host[0]);
+  return hipSuccess;
 }
 
 }  // namespace
@@ -165,7 +288,6 @@ hipError_t trsv_analysis_launch(const Compressed& stored, bool stored_inner_le_o
                                 const Compressed& rows, bool lower, int64_t* level_ptr,
                                 uint32_t* row_order, void* scratch, TrsvLevels* out,
                                 int* failure, hipStream_t s) {
-  (void)lower;  // the dependency test (j != i) needs no direction: A is triangular
   *failure = 0;
   out->levels = 0;
   out->max_level_rows = 0;
@@ -196,23 +318,24 @@ hipError_t trsv_analysis_launch(const Compressed& stored, bool stored_inner_le_o
     return hipSuccess;
   }
   int64_t levels = 0;
-  e = rows.index_bytes == 4 ? level_typed<int32_t>(rows, level, flags, host, &levels, s)
-                            : level_typed<int64_t>(rows, level, flags, host, &levels, s);
+  e = rows.index_bytes == 4 ? level_typed<int32_t>(rows, lower, level, flags, host, &levels, s)
+                            : level_typed<int64_t>(rows, lower, level, flags, host, &levels, s);
   if (e != hipSuccess) return e;
   // Rows in level order (stable: ascending within a level); level_ptr gets
   // levels + 1 entries.
   e = sort_by_key_launch(reinterpret_cast<const uint32_t*>(level), n, levels, row_order, level_ptr,
                          sort_ws, s);
   if (e != hipSuccess) return e;
-  // The widest level sizes the solve's grid.
-  int64_t* host_ptr = static_cast<int64_t*>(malloc(sizeof(int64_t) * (levels + 1)));
-  if (host_ptr == nullptr) return hipErrorOutOfMemory;
-  e = hipMemcpyAsync(host_ptr, level_ptr, sizeof(int64_t) * (levels + 1), hipMemcpyDeviceToHost, s);
+  // The level bounds come back to the host: the widest level sizes the
+  // solve's grid, and the solve groups runs of small levels.
+  std::vector<int64_t>& host_ptr = out->host_level_ptr;
+  host_ptr.assign(static_cast<size_t>(levels + 1), 0);
+  e = hipMemcpyAsync(host_ptr.data(), level_ptr, sizeof(int64_t) * (levels + 1),
+                     hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   int64_t widest = 0;
   for (int64_t l = 0; e == hipSuccess && l < levels; ++l)
     widest = std::max(widest, host_ptr[l + 1] - host_ptr[l]);
-  free(host_ptr);
   out->levels = levels;
   out->max_level_rows = widest;
   return e;
@@ -227,7 +350,32 @@ hipError_t trsv_solve_launch(const Compressed& rows, bool lower, bool unit, doub
   const int64_t threads = lv.max_level_rows * k;
   const unsigned grid = static_cast<unsigned>((threads + kBlock - 1) / kBlock);
   const bool i32 = rows.index_bytes == 4;
+  const bool grouped = static_cast<int64_t>(lv.host_level_ptr.size()) == lv.levels + 1;
   for (int64_t l = 0; l < lv.levels; ++l) {
+    if (grouped) {
+      int64_t g = l;
+      while (g < lv.levels && (lv.host_level_ptr[g + 1] - lv.host_level_ptr[g]) * k <= kGroupMax)
+        ++g;
+      if (g > l) {
+#define ASC_TRSV_GROUP(T, I)                                                               \
+  trsv_group_solve_kernel<T, I><<<1, kGroupBlock, 0, s>>>(                                 \
+      rows.offsets, static_cast<const I*>(rows.indices), static_cast<const T*>(rows.values), \
+      level_ptr, row_order, l, g, unit ? 1 : 0, static_cast<T>(alpha), static_cast<T*>(x),   \
+      rs, cs, k)
+        if (rows.elem == Elem::F64) {
+          if (i32) ASC_TRSV_GROUP(double, int32_t);
+          else ASC_TRSV_GROUP(double, int64_t);
+        } else {
+          if (i32) ASC_TRSV_GROUP(float, int32_t);
+          else ASC_TRSV_GROUP(float, int64_t);
+        }
+#undef ASC_TRSV_GROUP
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        l = g - 1;
+        continue;
+      }
+    }
     if (rows.elem == Elem::F64) {
       if (i32)
         trsv_level_solve_kernel<double, int32_t><<<grid, kBlock, 0, s>>>(
