@@ -352,6 +352,46 @@ def bench_dense_rows(ctx, extra, rows=1 << 14, cols=1 << 22, per_row=20_000):
         f"LDS panels {out['panel_auto_lds_gbs']} GB/s")
 
 
+def bench_triangular(ctx, extra, n=1 << 18, per_row=8, band=2000):
+    """Level-scheduled triangular solve (FP64, one right-hand side, bit-exact
+    with the serial solver): a lower-triangular n x n matrix whose row i has
+    a nonzero diagonal and up to per_row entries at distinct offsets spread
+    over the band below it (offsets k*band/per_row + U[1, band/per_row]).
+    Reports the analysis and the solve."""
+    rng = np.random.default_rng(G.SEED_BASE + 11)
+    step = band // per_row
+    d = np.arange(per_row) * step + rng.integers(1, step + 1, (n, per_row))
+    i = np.arange(n)[:, None]
+    cols = i - d[:, ::-1]  # ascending columns
+    keep = cols >= 0
+    counts = keep.sum(1) + 1
+    offsets = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    full = np.concatenate([np.where(keep, cols, -1), np.arange(n)[:, None]], 1)
+    indices = full[full >= 0].astype(np.int64)
+    vals = np.where(full >= 0, np.concatenate([rng.uniform(-0.5, 0.5, (n, per_row)),
+                                               rng.uniform(1.0, 2.0, (n, 1))], 1), np.nan)
+    values = vals[full >= 0]
+    a = H.clone_compressed(ctx, H.CSR, n, n, offsets, indices, values, H.F64)
+    t0 = time.perf_counter()
+    plan = H.triangular_analyze(ctx, H.OP_N, H.LOWER, H.NON_UNIT, a)
+    analysis_ms = (time.perf_counter() - t0) * 1e3
+    b = G.vector(n, 12)
+    bb = ctx.upload(b)
+    rhs = H.dense(bb, n, 1, 1, H.ROW_MAJOR, H.F64)
+
+    def solve():
+        ctx.copy(bb.ptr, b.ctypes.data, b.nbytes)
+        H.triangular_solve(ctx, H.OP_N, 1.0, a, rhs, plan)
+    r = time_ops(ctx, solve, 5, 1)
+    extra["triangular_lower_band"] = {
+        "rows": n, "nnz": int(offsets[-1]), "levels": int(plan.info.levels),
+        "widest_level": int(plan.info.max_level_rows), "analysis_ms": round(analysis_ms, 2),
+        "solve_ms_incl_rhs_copy": round(r["median_ms"], 3),
+        "rows_per_s": round(n / (r["median_ms"] * 1e-3))}
+    log(f"triangular: {n} rows, {plan.info.levels} levels, analysis {analysis_ms:.1f} ms, "
+        f"solve {r['median_ms']:.3f} ms")
+
+
 def bench_batched(ctx, extra, count):
     t0 = time.perf_counter()
     specs = G.batched_specs(count)
@@ -420,6 +460,7 @@ def main():
     if not args.quick:
         bench_precision(ctx, extra)
         bench_dense_rows(ctx, extra)
+        bench_triangular(ctx, extra)
         bench_finalize(ctx, extra)
         bench_batched(ctx, extra, args.batch_count)
     traffic, traffic_src = measured_traffic(rows, nnz, name)
