@@ -538,12 +538,12 @@ class Batch:
 
     def tune(self, reps: int = 3):
         out = BatchClasses()
-        ms = (C.c_float * 56)()
+        ms = (C.c_float * 64)()
         check(self.ctx.lib.asc_sparse_hip_batch_tune(self.ctx.handle, self.n, self.entries, self.mem.ptr,
                                                      self.mem.nbytes, reps, C.byref(out), ms))
         lanes = list(out.lanes)
         self.build(lanes)
-        return lanes, [list(ms[c * 7:(c + 1) * 7]) for c in range(8)]
+        return lanes, [list(ms[c * 8:(c + 1) * 8]) for c in range(8)]
 
     def run(self):
         check(self.ctx.lib.asc_sparse_hip_batch_spmv(self.ctx.handle, C.byref(self.plan)))

@@ -30,6 +30,7 @@ namespace {
 constexpr int64_t kItemNnz = 8192;   // target nonzeros per work item
 constexpr int kClasses = 8;          // average-row-length classes [2^c, 2^(c+1))
 constexpr int64_t kWorkgroupsPerCu = 8;
+constexpr int kChoices = 8;  // tuner candidates: lanes 1..64, then CSR-stream
 
 inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
@@ -40,15 +41,22 @@ int row_class(const asc_hip_sparse_desc& a) {
   return c;
 }
 
-int64_t rows_per_item(const asc_hip_sparse_desc& a) {
+// Rows per work item: about kItemNnz entries for the lane path, and about
+// three quarters of the LDS window for a CSR-stream item (lanes == 0), so
+// that rows longer than average rarely overflow it.
+int64_t rows_per_item(const asc_hip_sparse_desc& a, int lanes) {
   const int64_t avg = std::max<int64_t>(1, a.rows > 0 ? a.nnz / a.rows : 1);
-  return std::max<int64_t>(1, kItemNnz / avg);
+  const int64_t target = lanes == 0 ? asc_hip::kStreamEntries * 3 / 4 : kItemNnz;
+  return std::max<int64_t>(1, target / avg);
 }
 
+// Items of the finest cut any class may take (CSR-stream), so plan memory of
+// this size serves every choice of lanes.
 int64_t count_items(int64_t n, const asc_hip_batch_entry* e) {
   int64_t items = 0;
   for (int64_t i = 0; i < n; ++i) {
-    const int64_t rows = e[i].matrix.rows, per = rows_per_item(e[i].matrix);
+    const int64_t rows = e[i].matrix.rows;
+    const int64_t per = std::min(rows_per_item(e[i].matrix, 0), rows_per_item(e[i].matrix, 1));
     items += (rows + per - 1) / per;
   }
   return items;
@@ -107,9 +115,13 @@ asc_hip_status validate_entries(asc_hip_context* ctx, int64_t n, const asc_hip_b
   return ok();
 }
 
+// Largest persistent grid a plan's range table holds.
+constexpr int64_t kMaxGrid = 16384;
+
 size_t plan_bytes(int64_t n, int64_t items) {
   return align256(sizeof(asc_hip::BatchMatrix) * std::max<int64_t>(n, 1)) +
-         align256(sizeof(asc_hip::BatchItem) * std::max<int64_t>(items, 1));
+         align256(sizeof(asc_hip::BatchItem) * std::max<int64_t>(items, 1)) +
+         align256(sizeof(int64_t) * (kMaxGrid + 1));
 }
 
 int64_t persistent_grid(const asc_hip_context* ctx, int64_t items) {
@@ -119,7 +131,7 @@ int64_t persistent_grid(const asc_hip_context* ctx, int64_t items) {
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, ctx->device) == hipSuccess && v > 0)
       cus = v;
   }
-  return std::max<int64_t>(1, std::min<int64_t>(items, cus * kWorkgroupsPerCu));
+  return std::max<int64_t>(1, std::min<int64_t>({items, cus * kWorkgroupsPerCu, kMaxGrid}));
 }
 
 // Builds host tables for the entries selected by `pick` (all when null) and
@@ -129,6 +141,7 @@ asc_hip_status build(asc_hip_context* ctx, int64_t n, const asc_hip_batch_entry*
                      size_t bytes, uint8_t elem, uint8_t ib, asc_hip_batch_plan* plan) {
   std::vector<asc_hip::BatchMatrix> mats;
   std::vector<asc_hip::BatchItem> items;
+  std::vector<int64_t> weight;  // estimated entries + rows per item
   const int64_t count = pick ? static_cast<int64_t>(pick->size()) : n;
   mats.reserve(count);
   for (int64_t j = 0; j < count; ++j) {
@@ -144,8 +157,9 @@ asc_hip_status build(asc_hip_context* ctx, int64_t n, const asc_hip_batch_entry*
     m.alpha = b.alpha;
     m.beta = b.beta;
     mats.push_back(m);
-    const int64_t per = rows_per_item(b.matrix);
     const int g = lanes[row_class(b.matrix)];
+    const int64_t per = rows_per_item(b.matrix, g);
+    const int64_t avg = b.matrix.rows > 0 ? b.matrix.nnz / b.matrix.rows : 0;
     for (int64_t r = 0; r < b.matrix.rows; r += per) {
       asc_hip::BatchItem it;
       it.matrix = static_cast<int32_t>(j);
@@ -153,7 +167,23 @@ asc_hip_status build(asc_hip_context* ctx, int64_t n, const asc_hip_batch_entry*
       it.row_begin = r;
       it.row_end = std::min(b.matrix.rows, r + per);
       items.push_back(it);
+      weight.push_back((it.row_end - it.row_begin) * (avg + 1));
     }
+  }
+  // Contiguous item ranges of about equal weight, one per work-group.
+  const int64_t grid = persistent_grid(ctx, static_cast<int64_t>(items.size()));
+  std::vector<int64_t> ranges(static_cast<size_t>(grid) + 1, 0);
+  {
+    int64_t total = 0;
+    for (int64_t w : weight) total += w;
+    int64_t acc = 0;
+    size_t i = 0;
+    for (int64_t w = 1; w < grid; ++w) {
+      const int64_t goal = (total / grid) * w + (total % grid) * w / grid;
+      while (i < items.size() && acc + weight[i] / 2 < goal) acc += weight[i++];
+      ranges[static_cast<size_t>(w)] = static_cast<int64_t>(i);
+    }
+    ranges[static_cast<size_t>(grid)] = static_cast<int64_t>(items.size());
   }
   const size_t need = plan_bytes(count, static_cast<int64_t>(items.size()));
   if (mem == nullptr || bytes < need)
@@ -166,12 +196,17 @@ asc_hip_status build(asc_hip_context* ctx, int64_t n, const asc_hip_batch_entry*
   DeviceGuard guard(ctx->device);
   char* base = static_cast<char*>(mem);
   const size_t items_off = align256(sizeof(asc_hip::BatchMatrix) * std::max<int64_t>(count, 1));
+  const size_t ranges_off =
+      items_off + align256(sizeof(asc_hip::BatchItem) * std::max<size_t>(items.size(), 1));
   hipError_t err = hipSuccess;
   if (!mats.empty())
     err = hipMemcpyAsync(base, mats.data(), sizeof(asc_hip::BatchMatrix) * mats.size(),
                          hipMemcpyHostToDevice, ctx->stream);
   if (err == hipSuccess && !items.empty())
     err = hipMemcpyAsync(base + items_off, items.data(), sizeof(asc_hip::BatchItem) * items.size(),
+                         hipMemcpyHostToDevice, ctx->stream);
+  if (err == hipSuccess)
+    err = hipMemcpyAsync(base + ranges_off, ranges.data(), sizeof(int64_t) * ranges.size(),
                          hipMemcpyHostToDevice, ctx->stream);
   const hipError_t sync = hipStreamSynchronize(ctx->stream);
   if (err == hipSuccess) err = sync;
@@ -181,7 +216,7 @@ asc_hip_status build(asc_hip_context* ctx, int64_t n, const asc_hip_batch_entry*
   plan->item_table = base + items_off;
   plan->matrices = count;
   plan->items = static_cast<int64_t>(items.size());
-  plan->grid = persistent_grid(ctx, plan->items);
+  plan->grid = grid;
   plan->element_kind = elem;
   plan->index_bytes = ib;
   plan->device = ctx->device;
@@ -192,6 +227,9 @@ asc_hip::BatchLaunch as_launch(const asc_hip_batch_plan& p) {
   asc_hip::BatchLaunch L;
   L.mats = static_cast<const asc_hip::BatchMatrix*>(p.matrix_table);
   L.items = static_cast<const asc_hip::BatchItem*>(p.item_table);
+  L.ranges = reinterpret_cast<const int64_t*>(
+      static_cast<const char*>(p.item_table) +
+      align256(sizeof(asc_hip::BatchItem) * std::max<int64_t>(p.items, 1)));
   L.nitems = p.items;
   L.grid = p.grid;
   L.elem = static_cast<Elem>(p.element_kind);
@@ -238,8 +276,9 @@ asc_hip_status asc_sparse_hip_batch_plan_build(asc_hip_context* ctx, int64_t cou
   const uint8_t* lanes = classes ? classes->lanes : kDefaultLanes;
   for (int c = 0; c < kClasses; ++c) {
     const uint8_t g = lanes[c];
-    if (g == 0 || g > 64 || (g & (g - 1)) != 0)
-      return fail(ASC_HIP_INVALID_ARGUMENT, "class lanes must be powers of two in [1, 64]");
+    if (g > 64 || (g & (g - 1)) != 0)
+      return fail(ASC_HIP_INVALID_ARGUMENT,
+                  "class lanes must be powers of two in [1, 64], or 0 for CSR-stream");
   }
   return build(ctx, count, entries, lanes, nullptr, plan_memory, plan_bytes_given, elem, ib, out);
 }
@@ -331,9 +370,9 @@ asc_hip_status asc_sparse_hip_batch_tune(asc_hip_context* ctx, int64_t count,
   asc_hip_status result = ok();
   for (int c = 0; c < kClasses && result.code == 0; ++c) {
     float best = -1.0f;
-    for (int li = 0; li < 7; ++li) {
-      const uint8_t g = static_cast<uint8_t>(1 << li);
-      if (class_ms) class_ms[c * 7 + li] = -1.0f;
+    for (int li = 0; li < kChoices; ++li) {
+      const uint8_t g = li < 7 ? static_cast<uint8_t>(1 << li) : 0;  // last: CSR-stream
+      if (class_ms) class_ms[c * kChoices + li] = -1.0f;
       if (members[c].empty()) continue;
       uint8_t lanes[kClasses];
       std::memcpy(lanes, kDefaultLanes, kClasses);
@@ -354,7 +393,7 @@ asc_hip_status asc_sparse_hip_batch_tune(asc_hip_context* ctx, int64_t count,
         break;
       }
       ms /= static_cast<float>(reps);
-      if (class_ms) class_ms[c * 7 + li] = ms;
+      if (class_ms) class_ms[c * kChoices + li] = ms;
       if (best < 0.0f || ms < best) {
         best = ms;
         out->lanes[c] = g;

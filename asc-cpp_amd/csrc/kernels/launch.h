@@ -168,13 +168,16 @@ struct BatchMatrix {
 };
 struct BatchItem {
   int32_t matrix;
-  int32_t lanes;  // lanes per row: 1..64, power of two
+  int32_t lanes;  // lanes per row: 1..64, power of two; 0: CSR-stream through LDS
   int64_t row_begin;
   int64_t row_end;
 };
+// Entries a CSR-stream item stages in LDS (larger items take one lane per row).
+constexpr int64_t kStreamEntries = 2048;
 struct BatchLaunch {
   const BatchMatrix* mats;
   const BatchItem* items;
+  const int64_t* ranges;  // work-group w walks items [ranges[w], ranges[w+1])
   int64_t nitems;
   int64_t grid;  // persistent workgroups
   Elem elem;     // F32 or F64

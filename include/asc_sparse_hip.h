@@ -602,7 +602,10 @@ asc_hip_status asc_sparse_hip_coo_finalize(
  * A plan is a work list of (matrix, row range, lanes per row) items in
  * caller-owned device memory.  Lanes per row come from `classes`: class c
  * holds matrices whose average row length lies in [2^c, 2^(c+1)) (class 7
- * is open-ended); NULL selects {1,2,4,8,16,32,64,64}.  The plan stores raw
+ * is open-ended); NULL selects {1,2,4,8,16,32,64,64}.  Lanes 0 selects
+ * CSR-stream items: a work-group stages an item's products (up to 2048
+ * entries) in LDS with coalesced loads and sums each row from there; an
+ * item with more entries runs one lane per row.  The plan stores raw
  * pointers: entries' buffers must outlive it.
  */
 typedef struct asc_hip_batch_entry {
@@ -651,10 +654,11 @@ asc_hip_status asc_sparse_hip_graph_launch(asc_hip_context* context,
                                            asc_hip_graph* graph);
 asc_hip_status asc_sparse_hip_graph_destroy(asc_hip_graph* graph);
 
-/* Exhaustive per-class autotuner: times every lanes-per-row choice (1..64)
- * on each class's matrices with `reps` launches and writes the fastest to
- * `out`; `class_ms` (8 x 7 floats, may be NULL) receives the timings (-1
- * for empty classes).  Uses `plan_memory` as scratch; outputs are written. */
+/* Exhaustive per-class autotuner: times every lanes-per-row choice (1..64,
+ * then 0 for CSR-stream) on each class's matrices with `reps` launches and
+ * writes the fastest to `out`; `class_ms` (8 x 8 floats, may be NULL)
+ * receives the timings in that order (-1 for empty classes).  Uses
+ * `plan_memory` as scratch; outputs are written. */
 asc_hip_status asc_sparse_hip_batch_tune(asc_hip_context* context, int64_t count,
                                          const asc_hip_batch_entry* entries,
                                          void* plan_memory, size_t plan_bytes,
