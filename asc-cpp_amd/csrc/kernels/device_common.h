@@ -53,6 +53,22 @@ __device__ __forceinline__ f16_t ld_stream(const f16_t* p) {
   return r;
 }
 
+// Row-walking kernels with few lanes per row leave a cache line of the
+// matrix arrays shared by the row groups of neighbouring waves; a
+// non-temporal load does not keep it for the second reader, so they read
+// the matrix with ordinary cached loads (measured on banded and stencil
+// matrices: up to 2x).  From 16 lanes per row a wave reads whole lines, and
+// the non-temporal loads keep gathered x in L2.
+template <bool NT, typename T>
+__device__ __forceinline__ T ld_matrix(const T* p) {
+  if constexpr (NT) return ld_stream(p);
+  else return *p;
+}
+template <bool NT, typename I>
+__device__ __forceinline__ int64_t load_index_as(const I* p, int64_t k) {
+  return static_cast<int64_t>(ld_matrix<NT>(p + k));
+}
+
 // ---- wave reductions -----------------------------------------------------
 // Butterfly sum over aligned groups of G lanes (G a power of two <= 64).
 template <int G, typename T>

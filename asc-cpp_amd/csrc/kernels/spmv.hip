@@ -218,20 +218,21 @@ __device__ __forceinline__ Acc group_sum(const KParams& p, int64_t b, int64_t e,
   const V* __restrict__ val = static_cast<const V*>(p.values);
   const I* __restrict__ col = static_cast<const I*>(p.indices);
   const A* __restrict__ x = static_cast<const A*>(p.x);
+  constexpr bool kNt = G >= 16;
   Acc acc = Acc::zero();
   Acc acc2 = Acc::zero();
   int64_t k = b + lane;
   for (; k + G < e; k += 2 * G) {
-    const int64_t c0 = load_index(col, k);
-    const int64_t c1 = load_index(col, k + G);
-    const A v0 = widen<A>(ld_stream(val + k));
-    const A v1 = widen<A>(ld_stream(val + k + G));
+    const int64_t c0 = load_index_as<kNt>(col, k);
+    const int64_t c1 = load_index_as<kNt>(col, k + G);
+    const A v0 = widen<A>(ld_matrix<kNt>(val + k));
+    const A v1 = widen<A>(ld_matrix<kNt>(val + k + G));
     acc.add_product(v0, x[c0 * p.incx]);
     acc2.add_product(v1, x[c1 * p.incx]);
   }
   if (k < e) {
-    const int64_t c0 = load_index(col, k);
-    acc.add_product(widen<A>(ld_stream(val + k)), x[c0 * p.incx]);
+    const int64_t c0 = load_index_as<kNt>(col, k);
+    acc.add_product(widen<A>(ld_matrix<kNt>(val + k)), x[c0 * p.incx]);
   }
   acc.merge_after(acc2);
 #pragma unroll
@@ -248,8 +249,8 @@ __global__ __launch_bounds__(kBlock) void spmv_vector_kernel(KParams p, int64_t 
   const int64_t row = blk * (kBlock / G) + threadIdx.x / G;
   const int lane = threadIdx.x % G;
   if (row >= p.rows) return;  // whole G-groups leave together
-  const int64_t b = ld_stream(p.offsets + row);
-  const int64_t e = ld_stream(p.offsets + row + 1);
+  const int64_t b = ld_matrix<(G >= 16)>(p.offsets + row);
+  const int64_t e = ld_matrix<(G >= 16)>(p.offsets + row + 1);
   if (e - b >= skip_from) return;
   const Acc acc = group_sum<G, V, I, Acc>(p, b, e, lane);
   if (lane == 0) finish_row<V>(p, row, acc);

@@ -255,7 +255,19 @@ def bench_precision(ctx, extra):
     out = {}
     yb64 = ctx.alloc(8 * n)
     yd64 = H.vector(yb64, n, H.F64)
-    best64, _ = H.autotune(ctx, a64, xd64, yd64, 2, ws)
+
+    def cold_pick(a, xd, yd, scale=1.0):
+        """The direct variant fastest under the timed conditions (MALL
+        flushed before each call): a warm back-to-back autotune can favour a
+        variant that loses once the matrix comes from HBM."""
+        best, best_ms = None, None
+        for alg in H.TUNABLE_ALGS:
+            t = time_ops(ctx, lambda: H.csr_spmv(ctx, 1.0 / scale, a, xd, 0.0, yd, alg=alg, ws=ws),
+                         3, 1, flush=lambda: ctx.fill(flush, 0))["median_ms"]
+            if best_ms is None or t < best_ms:
+                best, best_ms = alg, t
+        return best
+    best64 = cold_pick(a64, xd64, yd64)
     r = time_ops(ctx, lambda: H.csr_spmv(ctx, 1.0, a64, xd64, 0.0, yd64, alg=best64, ws=ws),
                  7, 2, flush=lambda: ctx.fill(flush, 0))
     fp64_rows = n / (r["median_ms"] * 1e-3)
@@ -272,7 +284,7 @@ def bench_precision(ctx, extra):
         ak = H.convert_values(ctx, a64, kind, scale)
         yb = ctx.alloc(4 * n)
         yd = H.vector(yb, n, H.F32)
-        best, _ = H.autotune(ctx, ak, x32d, yd, 2, ws)
+        best = cold_pick(ak, x32d, yd, scale)
         for acc, aname in ((H.ACC_PLAIN, "plain"), (H.ACC_TWOSUM, "twosum")):
             rr = time_ops(ctx, lambda: H.csr_spmv(ctx, 1.0 / scale, ak, x32d, 0.0, yd, alg=best,
                                                   acc=acc, ws=ws), 7, 2,
