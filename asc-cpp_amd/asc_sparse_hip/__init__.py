@@ -714,6 +714,24 @@ if _lib is not None:
         getattr(_lib, _n).restype = Status
 
 
+GATHER_PROBE_BLOCKS = 2048
+_SIGS_PROBE = {
+    "asc_sparse_hip_gather_probe": [_VP, _VP, _VP, C.c_int64, _VP, C.c_int64, _VP, C.c_size_t],
+}
+_SIGS.update(_SIGS_PROBE)
+if _lib is not None:
+    for _n, _a in _SIGS_PROBE.items():
+        getattr(_lib, _n).argtypes = _a
+        getattr(_lib, _n).restype = Status
+
+
+def gather_probe(ctx: Context, indices, values, count: int, x, x_extent: int, partials):
+    """Enqueues the gather-rate probe over device buffers (see the C header)."""
+    ptr = lambda b: b.ptr if hasattr(b, "ptr") else b
+    check(ctx.lib.asc_sparse_hip_gather_probe(ctx.handle, ptr(indices), ptr(values), count, ptr(x),
+                                              x_extent, ptr(partials), GATHER_PROBE_BLOCKS))
+
+
 def panel_options(panel_columns: int = 0, long_row_min: int = 0, short_alg: int = ALG_AUTO,
                   long_alg: int = ALG_AUTO, staging: int = STAGING_AUTO) -> PanelOptions:
     return PanelOptions(panel_columns, long_row_min, short_alg, long_alg, staging)

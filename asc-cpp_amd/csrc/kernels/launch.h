@@ -185,6 +185,13 @@ struct BatchLaunch {
 };
 hipError_t batched_spmv_launch(const BatchLaunch& launch, hipStream_t stream);
 
+// ---- gather-rate probe -------------------------------------------------------
+// Work-groups of the probe launch (one FP64 partial each).
+constexpr int64_t kGatherProbeBlocks = 2048;
+hipError_t gather_probe_launch(const int32_t* indices, const double* values, int64_t n,
+                               const double* x, int64_t x_extent, double* partials,
+                               hipStream_t stream);
+
 // ---- selector: features and MLP --------------------------------------------
 size_t features_workspace_bytes();
 hipError_t features_launch(const Compressed& a, float* out16_device, void* workspace,

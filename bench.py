@@ -334,6 +334,28 @@ def bench_finalize(ctx, extra, n=1 << 26, dim=1 << 22):
     log(f"COO finalize of {n} triplets ({kept} unique): {res['median_ms']:.2f} ms")
 
 
+def bench_gather_ceiling(ctx, extra, bytes_per_entry, n=1 << 26):
+    """The random-gather ceiling behind the SpMV roofline: n streamed
+    (int32 index, FP64 value) pairs, each gathering x at a uniformly random
+    position, for x of 256 KiB (L2-resident), 2 MiB (one column panel) and
+    32 MiB (the headline's x, MALL-resident).  `spmv_ceiling_gbs` is that
+    rate times the headline's algorithmic bytes per nonzero."""
+    rng = np.random.default_rng(G.SEED_BASE + 13)
+    bv = ctx.upload(np.ones(n))
+    out = ctx.alloc(8 * H.GATHER_PROBE_BLOCKS)
+    res = {}
+    for xn, tag in ((1 << 15, "x_256KiB_l2"), (1 << 18, "x_2MiB_panel"), (1 << 22, "x_32MiB_mall")):
+        bi = ctx.upload(rng.integers(0, xn, n).astype(np.int32))
+        bx = ctx.upload(G.vector(xn, 3))
+        r = time_ops(ctx, lambda: H.gather_probe(ctx, bi, bv, n, bx, xn, out), 5, 2)
+        rate = n / (r["median_ms"] * 1e-3)
+        res[tag] = {"ggathers_per_s": round(rate / 1e9, 1),
+                    "spmv_ceiling_gbs": round(rate * bytes_per_entry / 1e9, 1)}
+        del bi, bx
+    extra["gather_ceiling"] = res
+    log("gather ceiling: " + ", ".join(f"{k} {v['ggathers_per_s']} G/s" for k, v in res.items()))
+
+
 def bench_dense_rows(ctx, extra, rows=1 << 14, cols=1 << 22, per_row=20_000):
     """Column-panel SpMV on dense rows: rows x cols with per_row uniformly
     random columns each (FP64, int32 columns), where the default analysis
@@ -470,6 +492,7 @@ def main():
     rows, nnz = m.rows, m.nnz
     del m
     if not args.quick:
+        bench_gather_ceiling(ctx, extra, nbytes / nnz)
         bench_precision(ctx, extra)
         bench_dense_rows(ctx, extra)
         bench_triangular(ctx, extra)

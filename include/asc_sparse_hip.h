@@ -665,6 +665,26 @@ asc_hip_status asc_sparse_hip_batch_tune(asc_hip_context* context, int64_t count
                                          int32_t reps, asc_hip_batch_classes* out,
                                          float* class_ms);
 
+/* ---- gather-rate probe ---------------------------------------------------- */
+
+/* Work-groups of the probe launch: the number of partials it writes. */
+#define ASC_HIP_GATHER_PROBE_BLOCKS 2048
+
+/*
+ * Measurement probe, no reference counterpart: the ceiling behind the SpMV
+ * rooflines.  For p in [0, count) it streams indices[p] (int32, clamped on
+ * the device to [0, x_extent)) and values[p], gathers x[indices[p]] through
+ * the caches as the SpMV kernels do, and sums values[p] * x[indices[p]]; one
+ * FP64 partial per work-group goes to `partials` (device, at least
+ * ASC_HIP_GATHER_PROBE_BLOCKS values, overlapping no input).  Timing it over
+ * random indices with x of a given size gives the gather rate a
+ * random-column SpMV can reach when x is resident at that cache level.
+ */
+asc_hip_status asc_sparse_hip_gather_probe(asc_hip_context* context, const int32_t* indices,
+                                           const double* values, int64_t count, const double* x,
+                                           int64_t x_extent, double* partials,
+                                           size_t partials_count);
+
 /* ---- variant selection: autotuner, plan cache, features, learned MLP ------ */
 
 /*
