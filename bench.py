@@ -463,7 +463,7 @@ def main():
     ap.add_argument("--rows", type=int, default=1 << 22)
     ap.add_argument("--alg", default="auto",
                     help="auto (column-panel plan, learned pick reported), merge, ordered, vecN")
-    ap.add_argument("--cpu-sample-rows", type=int, default=1 << 19)
+    ap.add_argument("--cpu-sample-rows", type=int, default=1 << 20)
     ap.add_argument("--batch-count", type=int, default=10_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline line only")
