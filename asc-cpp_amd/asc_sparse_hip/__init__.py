@@ -694,7 +694,7 @@ def convert_to_coordinate(ctx: Context, a: DeviceSparse, ws: Workspace | None = 
     return DeviceSparse(out, [bc, bv])
 
 
-STAGING_AUTO, STAGING_L2, STAGING_LDS = 0, 1, 2
+STAGING_AUTO, STAGING_L2, STAGING_LDS, STAGING_STREAM = 0, 1, 2, 3
 
 
 class PanelOptions(C.Structure):

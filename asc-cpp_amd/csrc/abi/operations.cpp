@@ -82,6 +82,10 @@ uint8_t resolve_alg(asc_hip_context* ctx, const asc_hip_sparse_desc& a,
         p.rows == a.rows && p.cols == a.cols && p.nnz == a.nnz &&
         p.element_kind == a.element_kind) {
       if (p.algorithm == ASC_HIP_ALG_PANEL) {
+        // A stream-form analysis copied other values: AUTO runs merge-path
+        // (an explicit ASC_HIP_ALG_PANEL is refused by the caller).
+        if (req == ASC_HIP_ALG_AUTO && p.panel.stream && p.panel.values_src != a.values)
+          return ASC_HIP_ALG_MERGE;
         *panel = p.panel;
         *has_panel = true;
       }
@@ -120,8 +124,38 @@ struct PanelShape {
   int64_t panels = 8;
   int64_t long_min = kDefaultLongRowMin;
   bool lds = false;
+  bool stream = false;
+  asc_hip::StreamLayout layout;  // stream staging only
   size_t list_bytes = 0, begin_bytes = 0, split_bytes = 0, scratch_bytes = 0, bytes = 0;
 };
+
+// Rows with at least long_min entries and their entries, counted on the host
+// from the matrix's offsets (read back after the context's stream drains).
+// A validation-only context has no device to read: every row counts.
+asc_hip_status count_listed_rows(asc_hip_context* ctx, const asc_hip_sparse_desc* a,
+                                 int64_t long_min, int64_t* nlong, int64_t* nnz_long) {
+  *nlong = a->rows;
+  *nnz_long = a->nnz;
+  if (ctx->device < 0 || a->rows == 0) return ok();
+  std::vector<int64_t> off(static_cast<size_t>(a->rows) + 1);
+  DeviceGuard guard(ctx->device);
+  if (guard.error != hipSuccess) return fail_hip(guard.error, "hipSetDevice");
+  hipError_t e = hipMemcpyAsync(off.data(), a->outer_offsets, sizeof(int64_t) * off.size(),
+                                hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return fail_hip(e, "reading the matrix offsets");
+  int64_t n = 0, entries = 0;
+  for (int64_t r = 0; r < a->rows; ++r) {
+    const int64_t len = off[r + 1] - off[r];
+    if (len >= long_min) {
+      ++n;
+      entries += len;
+    }
+  }
+  *nlong = n;
+  *nnz_long = entries;
+  return ok();
+}
 
 asc_hip_status panel_shape(asc_hip_context* ctx, const asc_hip_sparse_desc* a,
                            const asc_hip_panel_options* opt, PanelShape* out) {
@@ -139,9 +173,37 @@ asc_hip_status panel_shape(asc_hip_context* ctx, const asc_hip_sparse_desc* a,
   if (a->rows > kU32Max || std::min(a->nnz, a->cols) > kU32Max)
     return fail(ASC_HIP_UNSUPPORTED, "column-panel analysis stores 32-bit row ids and positions");
   const uint8_t staging = opt ? opt->staging : ASC_HIP_STAGING_AUTO;
-  if (staging > ASC_HIP_STAGING_LDS)
+  if (staging > ASC_HIP_STAGING_STREAM)
     return fail(ASC_HIP_INVALID_ARGUMENT, "unknown panel staging");
   const int64_t xbytes = a->element_kind == ASC_HIP_F64 ? 8 : 4;
+  if (staging == ASC_HIP_STAGING_STREAM) {
+    // LDS-sized panels, at most 32768 columns (15-bit column field).
+    const int64_t max_cols = std::min<int64_t>(asc_hip::kLdsPanelBytes / xbytes,
+                                               asc_hip::kStreamMaxPanelCols);
+    if (want_cols > max_cols)
+      return fail(ASC_HIP_INVALID_ARGUMENT,
+                  "a stream panel holds at most 160 KiB of x and 32768 columns");
+    const int64_t w = want_cols > 0 ? want_cols : max_cols;
+    int64_t panels = a->cols > w ? (a->cols - 1) / w + 1 : 1;
+    panels = (panels + 7) / 8 * 8;
+    if (panels > kMaxPanels)
+      return fail(ASC_HIP_INVALID_ARGUMENT, "panel width gives more than 1024 panels");
+    int64_t nlong = 0, nnz_long = 0;
+    if ((st = count_listed_rows(ctx, a, long_min, &nlong, &nnz_long)).code) return st;
+    int64_t pairs = 0;
+    if (!checked_mul(panels, nlong, &pairs) || pairs > std::numeric_limits<int32_t>::max() ||
+        nnz_long + panels * asc_hip::kStreamTile > static_cast<int64_t>(kU32Max))
+      return fail(ASC_HIP_UNSUPPORTED,
+                  "the stream form numbers its entries and (panel, row) pairs in 32 bits");
+    out->panels = panels;
+    out->panel_cols = a->cols > 0 ? (a->cols - 1) / panels + 1 : 1;
+    out->long_min = long_min;
+    out->stream = true;
+    out->layout = asc_hip::stream_layout(a->rows, nlong, nnz_long, panels,
+                                         elem_size(a->element_kind));
+    out->bytes = out->layout.total;
+    return ok();
+  }
   // Panels of a multiple of 8 covering the columns with at most `w` each.
   auto cut = [&](int64_t w, int64_t* panels, int64_t* cols) {
     const int64_t p0 = a->cols > w ? (a->cols - 1) / w + 1 : 1;
@@ -192,11 +254,26 @@ asc_hip_status panel_shape(asc_hip_context* ctx, const asc_hip_sparse_desc* a,
 
 // The row list and split table must survive the SpMV's own writes.
 bool panel_overlaps(const asc_hip::PanelPlan& pp, Span out) {
+  if (pp.stream) return overlaps(span_of(pp.stream_base, pp.stream_bytes), out);
   if (pp.nlong == 0) return false;
   const size_t n = static_cast<size_t>(pp.nlong);
   return overlaps(span_of(pp.long_rows, 4 * n), out) ||
          overlaps(span_of(pp.row_begin, 8 * n), out) ||
          overlaps(span_of(pp.splits, 4 * n * static_cast<size_t>(pp.panels)), out);
+}
+
+// The stream form runs on its copy of the analysed values, with plain
+// accumulation.
+asc_hip_status check_stream_plan(const asc_hip_sparse_desc& a, const asc_hip::PanelPlan& pp,
+                                 const asc_hip_spmv_options* opt) {
+  if (!pp.stream) return ok();
+  if (a.values != pp.values_src)
+    return fail(ASC_HIP_INVALID_ARGUMENT,
+                "the stream-staged panel analysis holds a copy of the analysed values; "
+                "analyze the matrix again after rebinding its values");
+  if (opt && opt->accumulate == ASC_HIP_ACC_TWOSUM)
+    return fail(ASC_HIP_UNSUPPORTED, "the stream-staged panel plan accumulates plainly");
+  return ok();
 }
 
 asc_hip_status check_options(const asc_hip_sparse_desc& a, const asc_hip_spmv_options* opt,
@@ -237,6 +314,7 @@ asc_hip_status prepare_compat_spmv(asc_hip_context* ctx, const asc_hip_sparse_de
   if (alg == ASC_HIP_ALG_PANEL && !has_panel) return missing_panel_analysis();
   if (alg == ASC_HIP_ALG_PANEL && panel_overlaps(panel, ys))
     return fail(ASC_HIP_INVALID_ARGUMENT, "SpMV output overlaps the column-panel analysis");
+  if (alg == ASC_HIP_ALG_PANEL && (st = check_stream_plan(*a, panel, opt)).code) return st;
   asc_hip::SpmvArgs& g = plan->args;
   g.panel = panel;
   g.a = as_compressed(*a, false);
@@ -337,6 +415,8 @@ asc_hip_status prepare_standard_spmv(asc_hip_context* ctx, uint8_t transpose,
       alg = ASC_HIP_ALG_MERGE;
     else if (panel_overlaps(panel, ys))
       return fail(ASC_HIP_INVALID_ARGUMENT, "SpMV output overlaps the column-panel analysis");
+    else if ((st = check_stream_plan(*a, panel, opt)).code)
+      return st;
   }
   if ((st = plan_row_view(*a, transpose, plan)).code) return st;
   asc_hip::SpmvArgs& g = plan->args;
@@ -361,6 +441,31 @@ std::vector<Span> operand_spans(const asc_hip_sparse_desc& a, std::initializer_l
   std::vector<Span> v{s.first, s.second, s.values};
   v.insert(v.end(), more.begin(), more.end());
   return v;
+}
+
+// AUTO lanes for a panel plan's short rows: the power of two at or below
+// long_min / 8, between 1 and 64.
+uint8_t auto_short_alg(int64_t long_min) {
+  uint8_t alg = ASC_HIP_ALG_VECTOR1;
+  for (int64_t g = 2; g <= long_min / 8 && alg < ASC_HIP_ALG_VECTOR64; g *= 2) ++alg;
+  return alg;
+}
+
+// Maps the matrix to ASC_HIP_ALG_PANEL with `pp` in the context's plan cache.
+asc_hip_status record_panel_plan(asc_hip_context* ctx, const asc_hip_sparse_desc* a,
+                                 const asc_hip::PanelPlan& pp) {
+  std::lock_guard<std::mutex> lock(ctx->plan_mutex);
+  for (auto& p : ctx->plans) {
+    if (p.offsets == a->outer_offsets && p.indices == a->inner_indices && p.rows == a->rows &&
+        p.cols == a->cols && p.nnz == a->nnz && p.element_kind == a->element_kind) {
+      p.algorithm = ASC_HIP_ALG_PANEL;
+      p.panel = pp;
+      return ok();
+    }
+  }
+  ctx->plans.push_back({a->outer_offsets, a->inner_indices, a->rows, a->cols, a->nnz,
+                        a->element_kind, ASC_HIP_ALG_PANEL, pp});
+  return ok();
 }
 
 asc_hip_status run_spmv(asc_hip_context* ctx, SpmvPlan& plan, void* ws) {
@@ -662,6 +767,23 @@ asc_hip_status asc_sparse_hip_csr_spmv_analyze(asc_hip_context* ctx,
                             {sp.first, sp.second, sp.values})).code)
     return st;
   if ((st = unavailable_if_validation_only(ctx)).code) return st;
+  if (shape.stream) {
+    asc_hip::PanelPlan pp;
+    pp.panels = shape.panels;
+    pp.panel_cols = shape.panel_cols;
+    pp.long_min = shape.long_min;
+    {
+      DeviceGuard guard(ctx->device);
+      if (guard.error != hipSuccess) return fail_hip(guard.error, "hipSetDevice");
+      const hipError_t e = asc_hip::panel_stream_analysis_launch(
+          as_compressed(*a, false), shape.layout, analysis, &pp, ctx->stream);
+      if ((st = finish_launch(ctx, e, "stream panel analysis")).code) return st;
+    }
+    if (short_alg == ASC_HIP_ALG_AUTO) short_alg = auto_short_alg(shape.long_min);
+    pp.short_alg = static_cast<asc_hip::SpmvAlg>(short_alg);
+    pp.long_alg = asc_hip::SpmvAlg::Vec64;  // unused by the stream kernels
+    return record_panel_plan(ctx, a, pp);
+  }
   char* w = static_cast<char*>(analysis);
   uint32_t* long_rows = reinterpret_cast<uint32_t*>(w);
   int64_t* row_begin = reinterpret_cast<int64_t*>(w + shape.list_bytes);
@@ -700,11 +822,7 @@ asc_hip_status asc_sparse_hip_csr_spmv_analyze(asc_hip_context* ctx,
   // lanes; for the listed rows, the power of two at or below their mean
   // entries per panel (every entry counted as a long row's: most are, in the
   // skewed matrices this plan is for).  Both between 1 and 64.
-  if (short_alg == ASC_HIP_ALG_AUTO) {
-    short_alg = ASC_HIP_ALG_VECTOR1;
-    for (int64_t g = 2; g <= shape.long_min / 8 && short_alg < ASC_HIP_ALG_VECTOR64; g *= 2)
-      ++short_alg;
-  }
+  if (short_alg == ASC_HIP_ALG_AUTO) short_alg = auto_short_alg(shape.long_min);
   if (long_alg == ASC_HIP_ALG_AUTO) {
     // LDS staging: about six entries per lane (a pass's metadata loads are
     // then a small share of its work); L2 staging: one entry per lane.
@@ -718,18 +836,7 @@ asc_hip_status asc_sparse_hip_csr_spmv_analyze(asc_hip_context* ctx,
   }
   pp.short_alg = static_cast<asc_hip::SpmvAlg>(short_alg);
   pp.long_alg = static_cast<asc_hip::SpmvAlg>(long_alg);
-  std::lock_guard<std::mutex> lock(ctx->plan_mutex);
-  for (auto& p : ctx->plans) {
-    if (p.offsets == a->outer_offsets && p.indices == a->inner_indices && p.rows == a->rows &&
-        p.cols == a->cols && p.nnz == a->nnz && p.element_kind == a->element_kind) {
-      p.algorithm = ASC_HIP_ALG_PANEL;
-      p.panel = pp;
-      return ok();
-    }
-  }
-  ctx->plans.push_back({a->outer_offsets, a->inner_indices, a->rows, a->cols, a->nnz,
-                        a->element_kind, ASC_HIP_ALG_PANEL, pp});
-  return ok();
+  return record_panel_plan(ctx, a, pp);
 }
 
 asc_hip_status asc_sparse_hip_standard_spmv_workspace_size(

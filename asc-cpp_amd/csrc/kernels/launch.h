@@ -56,10 +56,45 @@ struct PanelPlan {
   SpmvAlg long_alg = SpmvAlg::Vec32;  // Vec1..Vec64
   bool lds = false;     // stage each x panel in LDS (panel_cols * x bytes <= kLdsPanelBytes)
   int64_t chunks = 1;   // LDS form: row chunks per panel (work-groups = panels * chunks)
+  // Stream form (built by panel_stream_analysis_launch): the listed rows'
+  // entries copied panel-major, so the SpMV streams them contiguously and
+  // gathers x from an LDS copy of the panel.  Panel q's entries occupy
+  // [slice starts of q ...), each panel padded to a multiple of kStreamTile
+  // entries.  scol holds the column within the panel, with kStreamHead set on
+  // the first entry of every (panel, row) segment; sval the value.  Nonempty
+  // segments are numbered panel-major; seg_row maps a segment to its listed
+  // row.  A slice (kStreamSlice entries at most, one wave) is described by
+  // slice_info[4k..4k+3] = {start, end, segment of the first entry,
+  // panel | kStreamClosed when its last segment ends inside it}; wg_table[2w..]
+  // = {first slice, slice count} of work-group w (one panel, XCD-ordered).
+  // spartial[q*nlong + i] holds (panel q, row i) partial sums; the analysis
+  // zeroes it and the SpMV rewrites only nonempty segments.
+  bool stream = false;
+  const uint16_t* scol = nullptr;
+  const void* sval = nullptr;
+  const uint32_t* seg_row = nullptr;
+  const uint32_t* slice_info = nullptr;
+  const uint32_t* wg_table = nullptr;
+  double* spartial = nullptr;
+  int64_t slices = 0;
+  int64_t wgs = 0;
+  const void* values_src = nullptr;  // the values the copy was taken from
+  size_t stream_bytes = 0;           // bytes of the stream arrays (overlap checks)
+  const void* stream_base = nullptr;
 };
 
 // LDS bytes one work-group stages an x panel in (gfx950: 160 KiB per CU).
 constexpr int64_t kLdsPanelBytes = int64_t(160) << 10;
+// Stream form geometry: entries per lane per step, entries per wave step,
+// entries per slice (one wave's share), slices per work-group.
+constexpr int kStreamEpl = 4;
+constexpr int kStreamTile = 64 * kStreamEpl;
+constexpr int kStreamSlice = 32 * kStreamTile;
+constexpr int kStreamWaves = 16;
+constexpr uint16_t kStreamHead = 0x8000;
+constexpr uint32_t kStreamClosed = 0x80000000u;
+// Widest stream panel: the column field keeps 15 bits beside the head flag.
+constexpr int64_t kStreamMaxPanelCols = 32768;
 
 struct SpmvArgs {
   Compressed a;
@@ -90,6 +125,24 @@ hipError_t panel_analysis_launch(const Compressed& a, int64_t panel_cols, int64_
                                  int64_t long_min, uint32_t* long_rows, int64_t* row_begin,
                                  uint32_t* splits, void* scratch, int64_t* out_nlong,
                                  hipStream_t stream);
+
+// Stream-form layout for a matrix whose listed rows (nlong of them, nnz_long
+// entries) are known: the byte offsets of every array in the analysis buffer.
+struct StreamLayout {
+  int64_t nlong = 0, nnz_long = 0, panels = 0, pairs = 0;
+  int64_t entries_cap = 0, slices_cap = 0, segs_cap = 0;
+  size_t long_rows = 0, partial = 0, scol = 0, sval = 0, seg_row = 0, slice_info = 0,
+         wg_table = 0;  // kept
+  size_t row_begin = 0, splits = 0, lens = 0, begins = 0, ords = 0, small = 0,
+         slice_stage = 0, scan = 0, list_scratch = 0;  // analysis only
+  size_t kept_bytes = 0, total = 0;
+};
+StreamLayout stream_layout(int64_t rows, int64_t nlong, int64_t nnz_long, int64_t panels,
+                           size_t value_bytes);
+// Builds the stream form into `base` (laid out by `L`); fills the stream
+// fields of `pp` (whose panels/panel_cols/long_min are set).  Synchronous.
+hipError_t panel_stream_analysis_launch(const Compressed& a, const StreamLayout& L, void* base,
+                                        PanelPlan* pp, hipStream_t stream);
 
 // Stable sort of the indices 0..n-1 by uint32 key (< key_bound): out_index
 // gets the sorted order, out_key_offsets[k] (k = 0..key_bound) the first

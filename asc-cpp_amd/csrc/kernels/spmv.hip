@@ -18,6 +18,8 @@
 // Value storage V in {double, float, bf16, f16}; accumulation in double for
 // double and in float otherwise (plain, or compensated with TwoSum plus the
 // FMA product error).
+#include <vector>
+
 #include "device_common.h"
 #include "launch.h"
 
@@ -429,6 +431,286 @@ __global__ __launch_bounds__(kBlock) void panel_split_kernel(
     splits[(q - 1) * nlong + i] = static_cast<uint32_t>(lo - b);
   }
   splits[(panels - 1) * nlong + i] = static_cast<uint32_t>(e - b);
+}
+
+// ---------------------------------------------------------------------------
+// Stream form of the column-panel plan.  The analysis copies the listed rows'
+// entries panel-major (every panel's entries contiguous, rows ascending inside
+// it), each as a 16-bit column within the panel (bit 15 flags the first entry
+// of a (panel, row) segment) and the value.  A work-group stages its panel of
+// x in LDS and each of its waves walks one slice of the panel's stream in
+// steps of kStreamTile entries, kStreamEpl consecutive entries per lane:
+// fully coalesced loads, x gathered from LDS, a wave64 segmented scan over
+// the head flags, and every finished segment's sum written to its (panel,
+// row) slot.  A segment still open at a slice's end leaves its part in
+// carry[slice]; the fix-up kernel adds such parts to the slot, and the
+// combine kernel adds the slots of a row in panel order.
+// ---------------------------------------------------------------------------
+typedef double f64x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ void stream_values(const double* p, double (&v)[kStreamEpl]) {
+  const f64x2_t a = __builtin_nontemporal_load(reinterpret_cast<const f64x2_t*>(p));
+  const f64x2_t b = __builtin_nontemporal_load(reinterpret_cast<const f64x2_t*>(p) + 1);
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = b.x;
+  v[3] = b.y;
+}
+__device__ __forceinline__ void stream_values(const float* p, float (&v)[kStreamEpl]) {
+  const f32x4_t a = __builtin_nontemporal_load(reinterpret_cast<const f32x4_t*>(p));
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = a.z;
+  v[3] = a.w;
+}
+__device__ __forceinline__ void stream_values(const bf16_t* p, float (&v)[kStreamEpl]) {
+  const u32x2_t a = __builtin_nontemporal_load(reinterpret_cast<const u32x2_t*>(p));
+  v[0] = __uint_as_float(a.x << 16);
+  v[1] = __uint_as_float(a.x & 0xFFFF0000u);
+  v[2] = __uint_as_float(a.y << 16);
+  v[3] = __uint_as_float(a.y & 0xFFFF0000u);
+}
+__device__ __forceinline__ void stream_values(const f16_t* p, float (&v)[kStreamEpl]) {
+  const u32x2_t a = __builtin_nontemporal_load(reinterpret_cast<const u32x2_t*>(p));
+  f16_t h;
+  h.bits = static_cast<uint16_t>(a.x & 0xFFFFu);
+  v[0] = to_float(h);
+  h.bits = static_cast<uint16_t>(a.x >> 16);
+  v[1] = to_float(h);
+  h.bits = static_cast<uint16_t>(a.y & 0xFFFFu);
+  v[2] = to_float(h);
+  h.bits = static_cast<uint16_t>(a.y >> 16);
+  v[3] = to_float(h);
+}
+
+template <typename V>
+struct StreamStep {
+  u32x2_t c;  // four 16-bit columns
+  typename acc_of<V>::type v[kStreamEpl];
+};
+
+template <typename V>
+__device__ __forceinline__ void stream_load(const uint16_t* __restrict__ scol,
+                                            const V* __restrict__ sval, int64_t e,
+                                            StreamStep<V>& s) {
+  s.c = __builtin_nontemporal_load(reinterpret_cast<const u32x2_t*>(scol + e));
+  stream_values(sval + e, s.v);
+}
+
+template <typename V>
+__global__ __launch_bounds__(kLdsBlock) void spmv_stream_kernel(
+    const typename acc_of<V>::type* __restrict__ x, int64_t incx, int64_t cols,
+    int64_t panel_cols, const uint16_t* __restrict__ scol, const V* __restrict__ sval,
+    const uint32_t* __restrict__ seg_row, const uint32_t* __restrict__ slice_info,
+    const uint32_t* __restrict__ wg_table, int64_t nlong, double* __restrict__ partial,
+    double* __restrict__ carry) {
+  using A = typename acc_of<V>::type;
+  __shared__ A sx[kLdsPanelBytes / sizeof(A)];
+  const uint32_t first = wg_table[2 * blockIdx.x];
+  const uint32_t count = wg_table[2 * blockIdx.x + 1];
+  const int64_t q = slice_info[4 * static_cast<int64_t>(first) + 3] & ~kStreamClosed;
+  const int64_t col0 = q * panel_cols;
+  const int64_t width = cols - col0 < panel_cols ? cols - col0 : panel_cols;
+  for (int64_t k = threadIdx.x; k < width; k += kLdsBlock) sx[k] = x[(col0 + k) * incx];
+  __syncthreads();
+  const int wave = threadIdx.x / kWave;
+  if (wave >= static_cast<int>(count)) return;  // whole waves leave together
+  const int lane = threadIdx.x % kWave;
+  const int64_t slice = static_cast<int64_t>(first) + wave;
+  const uint32_t* si = slice_info + 4 * slice;
+  const int64_t s0 = si[0], s1 = si[1];
+  const bool closed = (si[3] & kStreamClosed) != 0;
+  double* __restrict__ prow = partial + q * nlong;
+  // The open segment: its ordinal (segments are numbered panel-major; the
+  // value before the slice's first head may wrap below zero), and its sum
+  // over this slice's entries so far.
+  uint32_t open_ord = si[2];
+  A open = A(0);
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (kWave - lane));
+  StreamStep<V> cur, nxt;
+  stream_load(scol, sval, s0 + kStreamEpl * lane, cur);
+  for (int64_t t = s0; t < s1; t += kStreamTile) {
+    const int64_t e = t + kStreamEpl * lane;
+    // Slices end inside their panel's padded range, so the next step's loads
+    // stay in the allocation even past the slice's end.
+    if (t + kStreamTile < s1) stream_load(scol, sval, e + kStreamTile, nxt);
+    bool h[kStreamEpl];
+    A p[kStreamEpl];
+#pragma unroll
+    for (int j = 0; j < kStreamEpl; ++j) {
+      const uint32_t c = ((j < 2 ? cur.c.x : cur.c.y) >> (16 * (j & 1))) & 0xFFFFu;
+      const bool valid = e + j < s1;
+      h[j] = valid && (c & kStreamHead) != 0;
+      const A prod = cur.v[j] * sx[c & 0x7FFFu];
+      p[j] = valid ? prod : A(0);
+    }
+    // A head on the step's first entry ends the segment open before it.
+    if (lane == 0 && h[0] && t > s0) prow[seg_row[open_ord]] = static_cast<double>(open);
+    // The lane's trailing run, then a segmented inclusive scan over lanes.
+    A u = A(0);
+    bool f = false;
+#pragma unroll
+    for (int j = 0; j < kStreamEpl; ++j) {
+      u = h[j] ? p[j] : u + p[j];
+      f = f || h[j];
+    }
+    A s = u;
+    int fl = f ? 1 : 0;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const A su = __shfl_up(s, off);
+      const int fu = __shfl_up(fl, off);
+      if (lane >= off) {
+        if (!fl) s += su;
+        fl |= fu;
+      }
+    }
+    if (!fl) s += open;
+    A run = __shfl_up(s, 1);
+    if (lane == 0) run = open;
+    uint64_t heads[kStreamEpl];
+    int before = 0, total = 0;
+#pragma unroll
+    for (int j = 0; j < kStreamEpl; ++j) {
+      heads[j] = __ballot(h[j]);
+      before += __popcll(heads[j] & below);
+      total += __popcll(heads[j]);
+    }
+    // An entry ends its segment when the next entry is a head; the next
+    // lane's first entry for a lane's last, the next step for lane 63's.
+    const int next_head = __shfl_down(h[0] ? 1 : 0, 1);
+    bool ends[kStreamEpl];
+#pragma unroll
+    for (int j = 0; j + 1 < kStreamEpl; ++j) ends[j] = h[j + 1];
+    ends[kStreamEpl - 1] = lane + 1 < kWave && next_head != 0;
+    uint32_t ord = open_ord + static_cast<uint32_t>(before);
+#pragma unroll
+    for (int j = 0; j < kStreamEpl; ++j) {
+      run = h[j] ? p[j] : run + p[j];
+      ord += h[j] ? 1u : 0u;
+      if (ends[j]) prow[seg_row[ord]] = static_cast<double>(run);
+    }
+    open = __shfl(s, kWave - 1);
+    open_ord += static_cast<uint32_t>(total);
+    cur = nxt;
+  }
+  if (lane == 0) {
+    if (closed) prow[seg_row[open_ord]] = static_cast<double>(open);
+    else carry[slice] = static_cast<double>(open);
+  }
+}
+
+// Adds the parts of segments that cross slice ends: the first slice of each
+// run of carries into one segment sums the run and adds it to the slot.
+__global__ __launch_bounds__(kBlock) void spmv_stream_fixup_kernel(
+    const uint32_t* __restrict__ slice_info, int64_t slices,
+    const uint32_t* __restrict__ seg_row, int64_t nlong, const double* __restrict__ carry,
+    double* __restrict__ partial) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (k >= slices) return;
+  const uint32_t w = slice_info[4 * k + 3];
+  if (w & kStreamClosed) return;  // an open slice always has a successor in its panel
+  const uint32_t target = slice_info[4 * (k + 1) + 2];
+  if (k > 0 && !(slice_info[4 * (k - 1) + 3] & kStreamClosed) &&
+      slice_info[4 * k + 2] == target)
+    return;  // not the run's first slice
+  double sum = carry[k];
+  for (int64_t j = k + 1;
+       !(slice_info[4 * j + 3] & kStreamClosed) && slice_info[4 * (j + 1) + 2] == target; ++j)
+    sum += carry[j];
+  const int64_t q = w & ~kStreamClosed;
+  partial[q * nlong + seg_row[target]] += sum;
+}
+
+// ---- stream-form analysis -------------------------------------------------
+// Segment lengths from the split table: lens[q*nlong + i] = entries of listed
+// row i in panel q.
+__global__ __launch_bounds__(kBlock) void stream_len_kernel(
+    const uint32_t* __restrict__ splits, int64_t nlong, int64_t pairs,
+    uint32_t* __restrict__ lens) {
+  const int64_t idx = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (idx >= pairs) return;
+  lens[idx] = splits[idx] - (idx >= nlong ? splits[idx - nlong] : 0u);
+}
+
+__global__ __launch_bounds__(kBlock) void stream_nonempty_kernel(uint32_t* __restrict__ lens,
+                                                                 int64_t pairs) {
+  const int64_t idx = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (idx < pairs) lens[idx] = lens[idx] > 0 ? 1u : 0u;
+}
+
+// small[q] = first raw stream position of panel q (q < panels),
+// small[panels] = total entries, small[panels + 1] = nonempty segments.
+__global__ __launch_bounds__(kBlock) void stream_totals_kernel(
+    const uint32_t* __restrict__ splits, const uint32_t* __restrict__ begins,
+    const uint32_t* __restrict__ ords, int64_t nlong, int64_t panels,
+    uint32_t* __restrict__ small) {
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  const int64_t pairs = nlong * panels;
+  if (q < panels) small[q] = begins[q * nlong];
+  if (q == panels) {
+    const int64_t last = pairs - 1;
+    const uint32_t len = splits[last] - (last >= nlong ? splits[last - nlong] : 0u);
+    small[panels] = begins[last] + len;
+    small[panels + 1] = ords[last] + (len > 0 ? 1u : 0u);
+  }
+}
+
+// One thread per (panel, listed row) segment: copies its entries into the
+// panel's stream, flags the first, and records the segment's row.
+template <typename I, typename VB>
+__global__ __launch_bounds__(kBlock) void stream_scatter_kernel(
+    const I* __restrict__ col, const VB* __restrict__ val, const int64_t* __restrict__ row_begin,
+    const uint32_t* __restrict__ splits, const uint32_t* __restrict__ begins,
+    const uint32_t* __restrict__ ords, const uint32_t* __restrict__ shift, int64_t nlong,
+    int64_t pairs, int64_t panel_cols, uint16_t* __restrict__ scol, VB* __restrict__ sval,
+    uint32_t* __restrict__ seg_row) {
+  const int64_t idx = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (idx >= pairs) return;
+  const int64_t q = idx / nlong;
+  const int64_t i = idx - q * nlong;
+  const uint32_t sb = q > 0 ? splits[idx - nlong] : 0u;
+  const uint32_t len = splits[idx] - sb;
+  if (len == 0) return;
+  seg_row[ords[idx]] = static_cast<uint32_t>(i);
+  const int64_t src = row_begin[i] + sb;
+  const int64_t dst = static_cast<int64_t>(begins[idx]) + shift[q];
+  const int64_t col0 = q * panel_cols;
+  for (uint32_t k = 0; k < len; ++k) {
+    const uint32_t c = static_cast<uint32_t>(static_cast<int64_t>(col[src + k]) - col0);
+    scol[dst + k] = static_cast<uint16_t>(c | (k == 0 ? kStreamHead : 0u));
+    sval[dst + k] = val[src + k];
+  }
+}
+
+// Completes slice_info from the host's staging table {start, end, raw start,
+// panel | kStreamClosed on the panel's last slice}: the segment open before
+// the slice's first entry, and whether the slice's last segment ends in it.
+__global__ __launch_bounds__(kBlock) void stream_slice_kernel(
+    const uint32_t* __restrict__ stage, int64_t slices, const uint32_t* __restrict__ begins,
+    const uint32_t* __restrict__ ords, int64_t nlong, const uint16_t* __restrict__ scol,
+    uint32_t* __restrict__ slice_info) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (k >= slices) return;
+  const uint32_t start = stage[4 * k], end = stage[4 * k + 1], raw = stage[4 * k + 2];
+  const uint32_t w = stage[4 * k + 3];
+  const int64_t q = w & ~kStreamClosed;
+  const uint32_t* b = begins + q * nlong;
+  // Largest i with b[i] <= raw: that segment is nonempty and holds entry raw.
+  int64_t lo = 0, hi = nlong - 1;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi + 1) >> 1;
+    if (b[mid] <= raw) lo = mid;
+    else hi = mid - 1;
+  }
+  const uint32_t ord = ords[q * nlong + lo] - (b[lo] == raw ? 1u : 0u);
+  const bool closed = (w & kStreamClosed) != 0 || (scol[end] & kStreamHead) != 0;
+  slice_info[4 * k] = start;
+  slice_info[4 * k + 1] = end;
+  slice_info[4 * k + 2] = ord;
+  slice_info[4 * k + 3] = static_cast<uint32_t>(q) | (closed ? kStreamClosed : 0u);
 }
 
 // ---------------------------------------------------------------------------
@@ -867,6 +1149,26 @@ hipError_t launch_typed(const SpmvArgs& a, const KParams& kp, void* ws,
           break;
       }
       if (e != hipSuccess || pp.nlong == 0) return e;
+      if (pp.stream) {
+        if constexpr (TWOSUM) {
+          return hipErrorInvalidValue;  // the stream form accumulates plainly
+        } else {
+          double* carry = static_cast<double*>(ws);
+          spmv_stream_kernel<V><<<static_cast<unsigned>(pp.wgs), kLdsBlock, 0, s>>>(
+              static_cast<const typename acc_of<V>::type*>(kp.x), kp.incx, a.a.inner,
+              pp.panel_cols, pp.scol, static_cast<const V*>(pp.sval), pp.seg_row,
+              pp.slice_info, pp.wg_table, pp.nlong, pp.spartial, carry);
+          e = hipGetLastError();
+          if (e != hipSuccess) return e;
+          spmv_stream_fixup_kernel<<<blocks_for(pp.slices, kBlock), kBlock, 0, s>>>(
+              pp.slice_info, pp.slices, pp.seg_row, pp.nlong, carry, pp.spartial);
+          e = hipGetLastError();
+          if (e != hipSuccess) return e;
+          spmv_panel_combine_kernel<V, TWOSUM><<<blocks_for(pp.nlong, kBlock), kBlock, 0, s>>>(
+              kp, pp.long_rows, pp.nlong, pp.panels, pp.spartial);
+          return hipGetLastError();
+        }
+      }
       double* partial = static_cast<double*>(ws);
       const int64_t groups = (pp.panels + 7) / 8;
       const int64_t chunk_rows = (pp.nlong + pp.chunks - 1) / pp.chunks;
@@ -922,6 +1224,8 @@ hipError_t launch_by_index(const SpmvArgs& a, const KParams& kp, void* ws,
 
 template <typename V>
 size_t ws_typed(const SpmvArgs& a) {
+  if (a.alg == SpmvAlg::Panel && a.panel.stream)  // one carry per slice
+    return align256(sizeof(double) * static_cast<size_t>(a.panel.slices > 0 ? a.panel.slices : 1));
   if (a.alg == SpmvAlg::Panel)  // FP64 partial sum per (panel, long row)
     return align256(sizeof(double) * static_cast<size_t>(a.panel.panels) *
                     static_cast<size_t>(a.panel.nlong));
@@ -998,6 +1302,200 @@ hipError_t panel_analysis_launch(const Compressed& a, int64_t panel_cols, int64_
         a.offsets, static_cast<const int64_t*>(a.indices), long_rows, nlong, panel_cols,
         panels, row_begin, splits);
   return hipGetLastError();
+}
+
+StreamLayout stream_layout(int64_t rows, int64_t nlong, int64_t nnz_long, int64_t panels,
+                           size_t value_bytes) {
+  StreamLayout L;
+  L.nlong = nlong;
+  L.nnz_long = nnz_long;
+  L.panels = panels;
+  L.pairs = nlong * panels;
+  L.entries_cap = nnz_long + panels * kStreamTile;
+  L.slices_cap = L.entries_cap / kStreamSlice + panels + 1;
+  L.segs_cap = nnz_long < L.pairs ? nnz_long : L.pairs;
+  const size_t pairs = static_cast<size_t>(L.pairs > 0 ? L.pairs : 1);
+  const size_t n = static_cast<size_t>(nlong > 0 ? nlong : 1);
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off += align256(bytes > 0 ? bytes : 1);
+    return at;
+  };
+  L.long_rows = take(sizeof(uint32_t) * static_cast<size_t>(rows > 0 ? rows : 1));
+  L.partial = take(sizeof(double) * pairs);
+  L.scol = take(sizeof(uint16_t) * static_cast<size_t>(L.entries_cap));
+  L.sval = take(value_bytes * static_cast<size_t>(L.entries_cap));
+  L.seg_row = take(sizeof(uint32_t) * static_cast<size_t>(L.segs_cap > 0 ? L.segs_cap : 1));
+  L.slice_info = take(4 * sizeof(uint32_t) * static_cast<size_t>(L.slices_cap));
+  L.wg_table = take(2 * sizeof(uint32_t) * static_cast<size_t>(L.slices_cap));
+  L.kept_bytes = off;
+  L.row_begin = take(sizeof(int64_t) * static_cast<size_t>(rows > 0 ? rows : 1));
+  L.splits = take(sizeof(uint32_t) * pairs);
+  L.lens = take(sizeof(uint32_t) * pairs);
+  L.begins = take(sizeof(uint32_t) * pairs);
+  L.ords = take(sizeof(uint32_t) * pairs);
+  L.small = take(sizeof(uint32_t) * static_cast<size_t>(2 * panels + 4));
+  L.slice_stage = take(4 * sizeof(uint32_t) * static_cast<size_t>(L.slices_cap));
+  L.scan = take(scan_u32_scratch_bytes(L.pairs));
+  L.list_scratch = take(panel_analysis_scratch_bytes(rows));
+  L.total = off;
+  (void)n;
+  return L;
+}
+
+hipError_t panel_stream_analysis_launch(const Compressed& a, const StreamLayout& L, void* base,
+                                        PanelPlan* pp, hipStream_t s) {
+  char* w = static_cast<char*>(base);
+  uint32_t* long_rows = reinterpret_cast<uint32_t*>(w + L.long_rows);
+  double* partial = reinterpret_cast<double*>(w + L.partial);
+  uint16_t* scol = reinterpret_cast<uint16_t*>(w + L.scol);
+  void* sval = w + L.sval;
+  uint32_t* seg_row = reinterpret_cast<uint32_t*>(w + L.seg_row);
+  uint32_t* slice_info = reinterpret_cast<uint32_t*>(w + L.slice_info);
+  uint32_t* wg_table = reinterpret_cast<uint32_t*>(w + L.wg_table);
+  int64_t* row_begin = reinterpret_cast<int64_t*>(w + L.row_begin);
+  uint32_t* splits = reinterpret_cast<uint32_t*>(w + L.splits);
+  uint32_t* lens = reinterpret_cast<uint32_t*>(w + L.lens);
+  uint32_t* begins = reinterpret_cast<uint32_t*>(w + L.begins);
+  uint32_t* ords = reinterpret_cast<uint32_t*>(w + L.ords);
+  uint32_t* small = reinterpret_cast<uint32_t*>(w + L.small);
+  uint32_t* stage = reinterpret_cast<uint32_t*>(w + L.slice_stage);
+  const int64_t panels = pp->panels;
+  const size_t vb = elem_bytes(a.elem);
+  pp->stream = true;
+  pp->long_rows = long_rows;
+  pp->row_begin = nullptr;
+  pp->splits = nullptr;
+  pp->scol = scol;
+  pp->sval = sval;
+  pp->seg_row = seg_row;
+  pp->slice_info = slice_info;
+  pp->wg_table = wg_table;
+  pp->spartial = partial;
+  pp->values_src = a.values;
+  pp->stream_base = base;
+  pp->stream_bytes = L.kept_bytes;
+  pp->slices = 0;
+  pp->wgs = 0;
+  int64_t nlong = 0;
+  hipError_t e = panel_analysis_launch(a, pp->panel_cols, panels, pp->long_min, long_rows,
+                                       row_begin, splits, w + L.list_scratch, &nlong, s);
+  if (e != hipSuccess) return e;
+  if (nlong != L.nlong) return hipErrorInvalidValue;  // the structure changed since sizing
+  pp->nlong = nlong;
+  if (nlong == 0) return hipSuccess;
+  const int64_t pairs = L.pairs;
+  const unsigned pb = static_cast<unsigned>((pairs + kBlock - 1) / kBlock);
+  // Segment lengths, their stream positions, nonempty-segment ordinals.
+  stream_len_kernel<<<pb, kBlock, 0, s>>>(splits, nlong, pairs, lens);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = scan_u32_exclusive(lens, begins, pairs, w + L.scan, s)) != hipSuccess) return e;
+  stream_nonempty_kernel<<<pb, kBlock, 0, s>>>(lens, pairs);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = scan_u32_exclusive(lens, ords, pairs, w + L.scan, s)) != hipSuccess) return e;
+  stream_totals_kernel<<<static_cast<unsigned>((panels + 1 + kBlock - 1) / kBlock), kBlock, 0,
+                         s>>>(splits, begins, ords, nlong, panels, small);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  std::vector<uint32_t> hs(static_cast<size_t>(panels + 2));
+  e = hipMemcpyAsync(hs.data(), small, sizeof(uint32_t) * hs.size(), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return e;
+  if (hs[panels] != static_cast<uint32_t>(L.nnz_long) ||
+      static_cast<int64_t>(hs[panels + 1]) > L.segs_cap)
+    return hipErrorInvalidValue;
+  // Padded panel starts, slices and work-groups (host: panels are few).
+  std::vector<uint32_t> shift(static_cast<size_t>(panels));
+  std::vector<uint32_t> hstage;
+  std::vector<std::vector<uint32_t>> xcd_wgs(kXcds);
+  int64_t padded = 0, slices = 0;
+  for (int64_t q = 0; q < panels; ++q) {
+    const int64_t raw = hs[q];
+    const int64_t len = static_cast<int64_t>(q + 1 < panels ? hs[q + 1] : hs[panels]) - raw;
+    shift[q] = static_cast<uint32_t>(padded - raw);
+    const int64_t ns = (len + kStreamSlice - 1) / kStreamSlice;
+    for (int64_t j = 0; j < ns; ++j) {
+      const int64_t b = j * kStreamSlice;
+      const int64_t e2 = b + kStreamSlice < len ? b + kStreamSlice : len;
+      hstage.push_back(static_cast<uint32_t>(padded + b));
+      hstage.push_back(static_cast<uint32_t>(padded + e2));
+      hstage.push_back(static_cast<uint32_t>(raw + b));
+      hstage.push_back(static_cast<uint32_t>(q) | (j + 1 == ns ? kStreamClosed : 0u));
+    }
+    for (int64_t j = 0; j < ns; j += kStreamWaves) {
+      const int64_t cnt = ns - j < kStreamWaves ? ns - j : kStreamWaves;
+      xcd_wgs[q % kXcds].push_back(static_cast<uint32_t>(slices + j));
+      xcd_wgs[q % kXcds].push_back(static_cast<uint32_t>(cnt));
+    }
+    slices += ns;
+    padded += (len + kStreamTile - 1) / kStreamTile * kStreamTile;
+  }
+  if (slices > L.slices_cap || padded > L.entries_cap) return hipErrorInvalidValue;
+  // Work-group b runs on XCD b % 8: give it a work-group of a panel that XCD
+  // stages (q % 8), so each XCD's panels stay in its L2; when an XCD's list
+  // runs out, the longest remaining list fills in.
+  std::vector<uint32_t> hwg;
+  std::vector<size_t> pos(kXcds, 0);
+  int64_t left = 0;
+  for (const auto& v : xcd_wgs) left += static_cast<int64_t>(v.size() / 2);
+  for (int64_t b = 0; left > 0; ++b) {
+    int x = static_cast<int>(b % kXcds);
+    if (pos[x] >= xcd_wgs[x].size()) {
+      size_t best = 0;
+      for (int y = 0; y < kXcds; ++y) {
+        const size_t rem = xcd_wgs[y].size() - pos[y];
+        if (rem > best) {
+          best = rem;
+          x = y;
+        }
+      }
+    }
+    hwg.push_back(xcd_wgs[x][pos[x]]);
+    hwg.push_back(xcd_wgs[x][pos[x] + 1]);
+    pos[x] += 2;
+    --left;
+  }
+  uint32_t* dshift = small + panels + 2;
+  e = hipMemcpyAsync(dshift, shift.data(), sizeof(uint32_t) * shift.size(),
+                     hipMemcpyHostToDevice, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(stage, hstage.data(), sizeof(uint32_t) * hstage.size(),
+                       hipMemcpyHostToDevice, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(wg_table, hwg.data(), sizeof(uint32_t) * hwg.size(),
+                       hipMemcpyHostToDevice, s);
+  // Zeroed stream (padding reads as column 0, value 0) and partial slots
+  // (empty segments are never written and stay zero).
+  if (e == hipSuccess) e = fill_launch(scol, 0, sizeof(uint16_t) * static_cast<size_t>(padded), s);
+  if (e == hipSuccess) e = fill_launch(sval, 0, vb * static_cast<size_t>(padded), s);
+  if (e == hipSuccess) e = fill_launch(partial, 0, sizeof(double) * static_cast<size_t>(pairs), s);
+  if (e != hipSuccess) return e;
+#define ASC_SCATTER(I, VB)                                                                   \
+  stream_scatter_kernel<I, VB><<<pb, kBlock, 0, s>>>(                                      \
+      static_cast<const I*>(a.indices), static_cast<const VB*>(a.values), row_begin, splits, \
+      begins, ords, dshift, nlong, pairs, pp->panel_cols, scol, static_cast<VB*>(sval),     \
+      seg_row)
+  if (a.index_bytes == 4) {
+    if (vb == 8) ASC_SCATTER(int32_t, uint64_t);
+    else if (vb == 4) ASC_SCATTER(int32_t, uint32_t);
+    else ASC_SCATTER(int32_t, uint16_t);
+  } else {
+    if (vb == 8) ASC_SCATTER(int64_t, uint64_t);
+    else if (vb == 4) ASC_SCATTER(int64_t, uint32_t);
+    else ASC_SCATTER(int64_t, uint16_t);
+  }
+#undef ASC_SCATTER
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if (slices > 0) {
+    stream_slice_kernel<<<static_cast<unsigned>((slices + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+        stage, slices, begins, ords, nlong, scol, slice_info);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  // The host tables above must outlive their copies.
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+  pp->slices = slices;
+  pp->wgs = static_cast<int64_t>(hwg.size() / 2);
+  return hipSuccess;
 }
 
 hipError_t spmv_launch(const SpmvArgs& a, void* ws, hipStream_t s) {

@@ -36,6 +36,9 @@ from asc_sparse_hip import gen as G  # noqa: E402
 from asc_sparse_hip import selector as S  # noqa: E402
 
 METRIC = "CSR SpMV GB/s & %HBM peak (FP64 plain vs BF16/learned-select); batched SpMV/s"
+STAGINGS = {"auto": H.STAGING_AUTO, "l2": H.STAGING_L2, "lds": H.STAGING_LDS,
+            "stream": H.STAGING_STREAM}
+STAGING_NAMES = {v: k for k, v in STAGINGS.items()}
 HBM_PEAK_GBS = 8000.0
 
 
@@ -154,13 +157,23 @@ def bench_large(ctx, args, extra):
                                    "gbs": round(nbytes / rd["median_ms"] / 1e6, 1),
                                    "ms": round(rd["median_ms"], 3)}
         log(f"learned pick {H.ALG_NAMES[direct]}: {rd['median_ms']:.3f} ms")
-        # ... and the column-panel plan with its default options, which the
+        # ... the column-panel plan with the other staging, for comparison ...
+        staging = STAGINGS[args.staging]
+        other = H.STAGING_STREAM if staging != H.STAGING_STREAM else H.STAGING_AUTO
+        H.csr_spmv_analyze(ctx, a, H.panel_options(staging=other))
+        ro = time_ops(ctx, lambda: H.csr_spmv(ctx, 1.0, a, xd, 0.0, yd, alg=H.ALG_PANEL, ws=ws), 5, 2)
+        extra["panel_other_staging"] = {"staging": STAGING_NAMES[other],
+                                        "gbs": round(nbytes / ro["median_ms"] / 1e6, 1),
+                                        "ms": round(ro["median_ms"], 3)}
+        log(f"panel plan, {STAGING_NAMES[other]} staging: {ro['median_ms']:.3f} ms")
+        # ... and the column-panel plan with the chosen staging, which the
         # timed steps run (analysis once, outside the timed region).
         t0 = time.perf_counter()
-        H.csr_spmv_analyze(ctx, a)
+        H.csr_spmv_analyze(ctx, a, H.panel_options(staging=staging))
         ctx.synchronize()
         extra["panel_analysis"] = {"ms": round((time.perf_counter() - t0) * 1e3, 2),
-                                   "panel_columns": 1 << 18, "long_row_min": 128,
+                                   "staging": STAGING_NAMES[staging],
+                                   "long_row_min": 128,
                                    "long_rows": int(np.sum(lens >= 128)),
                                    "long_row_nnz_share": round(float(lens[lens >= 128].sum()) / m.nnz, 4),
                                    "analysis_bytes": int(a.analysis.nbytes)}
@@ -184,7 +197,7 @@ def bench_large(ctx, args, extra):
         xb32, yb32 = ctx.upload(x.astype(np.float32)), ctx.alloc(4 * m.rows)
         x32d, y32d = H.vector(xb32, m.cols, H.F32, writable=False), H.vector(yb32, m.rows, H.F32)
         if alg == H.ALG_PANEL:
-            H.csr_spmv_analyze(ctx, a32)
+            H.csr_spmv_analyze(ctx, a32, H.panel_options(staging=STAGINGS[args.staging]))
         r32 = time_ops(ctx, lambda: H.csr_spmv(ctx, 1.0, a32, x32d, 0.0, y32d, alg=alg, ws=ws), 5, 2)
         b32 = spmv_bytes(m.rows, m.cols, m.nnz, 4, 4, 4, 4)
         extra["fp32_same_matrix"] = {"gbs": round(b32 / r32["median_ms"] / 1e6, 1),
@@ -463,6 +476,8 @@ def main():
     ap.add_argument("--rows", type=int, default=1 << 22)
     ap.add_argument("--alg", default="auto",
                     help="auto (column-panel plan, learned pick reported), merge, ordered, vecN")
+    ap.add_argument("--staging", default="stream", choices=sorted(STAGINGS),
+                    help="x-panel staging of the column-panel plan the timed steps run")
     ap.add_argument("--cpu-sample-rows", type=int, default=1 << 20)
     ap.add_argument("--batch-count", type=int, default=10_000)
     ap.add_argument("--no-cpu", action="store_true")
@@ -498,7 +513,8 @@ def main():
         bench_triangular(ctx, extra)
         bench_finalize(ctx, extra)
         bench_batched(ctx, extra, args.batch_count)
-    traffic, traffic_src = measured_traffic(rows, nnz, name)
+    plan_tag = name if alg != H.ALG_PANEL else ("stream" if args.staging == "stream" else "panel")
+    traffic, traffic_src = measured_traffic(rows, nnz, plan_tag)
     out = {
         "metric": METRIC,
         "value": round(gbs, 1),
@@ -516,8 +532,8 @@ def main():
                                "clipped to [1,1e4], int32 columns, int64 offsets, N(0,1) FP64 "
                                "values, x U[-1,1], alpha=1, beta=0",
                    "rows": rows, "cols": rows, "nnz": nnz, "algorithm": name,
-                   "selection": ("column-panel analysis, default options" if alg == H.ALG_PANEL
-                                 else "forced"),
+                   "selection": (f"column-panel analysis, {args.staging} staging"
+                                 if alg == H.ALG_PANEL else "forced"),
                    "bytes_per_spmv": nbytes, "hbm_frac_of_peak": round(gbs / HBM_PEAK_GBS, 4)},
         "roofline": {"bound": "hbm", "achieved": round(kgbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(kgbs / HBM_PEAK_GBS, 4),

@@ -340,7 +340,22 @@ asc_hip_status asc_sparse_hip_csr_spmv(
 enum {
   ASC_HIP_STAGING_AUTO = 0, /* LDS when a panel fits in 160 KiB of LDS, else L2 */
   ASC_HIP_STAGING_L2 = 1,
-  ASC_HIP_STAGING_LDS = 2   /* ASC_HIP_INVALID_ARGUMENT when a panel does not fit */
+  ASC_HIP_STAGING_LDS = 2,  /* ASC_HIP_INVALID_ARGUMENT when a panel does not fit */
+  /* Stream: the analysis copies the listed rows' entries panel-major (a
+   * 16-bit column within the panel and the value per entry) into the
+   * analysis buffer, and the SpMV streams that copy contiguously, gathering x
+   * from an LDS copy of each panel (at most 160 KiB of x and 32768 columns
+   * per panel).  The copy holds A's values as they were at analysis: after
+   * the values change (in place, or through asc_sparse_hip_rebind_values) the
+   * matrix needs a new analysis; a SpMV whose values pointer differs from the
+   * analysed one fails with ASC_HIP_INVALID_ARGUMENT.  The SpMV keeps its
+   * (panel, row) partial sums in the analysis buffer, so SpMVs through one
+   * analysis must not overlap (one context is never used concurrently).
+   * Plain accumulation only (ASC_HIP_ACC_TWOSUM is ASC_HIP_UNSUPPORTED).
+   * The size query counts the listed rows, so it reads the matrix's offsets
+   * and waits for the context's stream (a validation-only context returns
+   * the bound for every row listed). */
+  ASC_HIP_STAGING_STREAM = 3
 };
 
 typedef struct asc_hip_panel_options {

@@ -316,3 +316,167 @@ def test_lds_panels_negative_stride_and_dense_rows(ctx):
     y = ctx.download(yb, np.float64, rows)
     bound_check(y, yref, np.diff(m.offsets), absax, -1.25, 0.5, y0, EPS[np.float64])
     H.plan_clear(ctx)
+
+
+# ---- column-panel plan, stream staging -----------------------------------------------
+
+def _explicit(rows, cols, lens, seed):
+    """Sorted distinct random columns per row, N(0,1) values."""
+    rng = np.random.default_rng(seed)
+    offsets = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    idx = np.concatenate([np.sort(rng.choice(cols, int(n), replace=False)) if n < cols
+                          else np.arange(cols) for n in lens]).astype(np.int64)
+    return offsets, idx, rng.standard_normal(int(offsets[-1]))
+
+
+def _stream_cases():
+    # (offsets, indices, values, rows, cols) makers and [(panel columns, long-row minimum)]
+    def gm(make):
+        def f():
+            m = make()
+            return m.offsets, m.indices, m.values, m.rows, m.cols
+        return f
+
+    def dense_rows():
+        # whole dense rows: every (panel, row) segment spans several slices
+        rows, cols = 6, 1 << 18
+        lens = np.array([cols, 5, 0, cols - 17, 300, cols // 2])
+        o, i, v = _explicit(rows, cols, lens, 61)
+        return o, i, v, rows, cols
+
+    def half_empty():
+        rng = np.random.default_rng(62)
+        rows, cols = 20000, 90000
+        lens = np.where(rng.random(rows) < 0.6, 0, rng.integers(1, 400, rows))
+        o, i, v = _explicit(rows, cols, lens, 63)
+        return o, i, v, rows, cols
+
+    return {
+        "zipf_2e16": (gm(lambda: G.make(G.ZIPF, 1 << 16, 1 << 16, seed=23, values=G.VALUES_NORMAL)),
+                      [(0, 0), (4099, 8), (1 << 13, 1), (300, 64)]),
+        "uniform16_4096": (gm(lambda: G.uniform16(4096)), [(0, 0), (300, 16), (1000, 17)]),
+        "banded_3e4": (gm(lambda: G.make(G.BANDED, 30000, 30000, seed=6, values=G.VALUES_UNIFORM,
+                                         band=40, fill=0.5)), [(2048, 4), (37, 30)]),
+        "dense_rows": (dense_rows, [(0, 0), (0, 1), (5000, 100)]),
+        "half_empty": (half_empty, [(0, 1), (1 << 12, 100), (777, 1)]),
+    }
+
+
+@pytest.mark.parametrize("name", list(_stream_cases()))
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_stream_plan_parity(ctx, name, dtype):
+    make, shapes = _stream_cases()[name]
+    offsets, idx, values, rows, cols = make()
+    vals = values.astype(dtype)
+    x = G.vector(cols, 71).astype(dtype)
+    y0 = G.vector(rows, 72).astype(dtype)
+    y0[::89] = np.nan  # beta == 0 must not read y
+    lens = np.diff(offsets)
+    absax = O.abs_spmv(offsets, idx, vals.astype(np.float64), x.astype(np.float64))
+    a = H.clone_compressed(ctx, H.CSR, rows, cols, offsets, idx, vals, KIND[dtype], index_bytes=4)
+    for width, long_min in shapes:
+        opts = H.panel_options(width, long_min, staging=H.STAGING_STREAM)
+        H.csr_spmv_analyze(ctx, a, opts)
+        assert H.plan_get(ctx, a) == H.ALG_PANEL
+        for alpha, beta in ((1.0, 0.0), (-1.5, 0.25)):
+            y_in = y0 if beta == 0.0 else np.nan_to_num(y0)
+            yref = y_in.copy()
+            O.compat_spmv(offsets, idx, vals, alpha, x, beta, yref)
+            print(f"stream {name} {dtype.__name__} width {width} long>={long_min} "
+                  f"alpha {alpha} beta {beta}", flush=True)
+            first = None
+            for alg in (H.ALG_PANEL, H.ALG_AUTO, H.ALG_PANEL):
+                y = run_compat(ctx, a, x, y_in, alpha, beta, alg)
+                bound_check(y, yref, lens, absax, alpha, beta, np.nan_to_num(y_in), EPS[dtype])
+                if first is None:
+                    first = y
+                else:  # fixed decomposition: identical bits every call
+                    np.testing.assert_array_equal(y.view(np.uint8), first.view(np.uint8))
+    H.plan_clear(ctx)
+
+
+def test_stream_plan_strides_standard_and_empty_rows(ctx):
+    rng = np.random.default_rng(19)
+    rows, cols = 7000, 50000
+    lens = np.where(rng.random(rows) < 0.5, 0, rng.integers(1, 600, rows))
+    offsets, idx, vals = _explicit(rows, cols, lens, 20)
+    a = H.clone_compressed(ctx, H.CSR, rows, cols, offsets, idx, vals, H.F64, 8)
+    H.csr_spmv_analyze(ctx, a, H.panel_options(3333, 20, staging=H.STAGING_STREAM))
+    x = rng.uniform(-1, 1, cols * 3)
+    y0 = rng.uniform(-1, 1, rows * 2)
+    xs = np.ascontiguousarray(x[::3])
+    yref = y0[::2].copy()
+    O.compat_spmv(offsets, idx, vals, -2.0, xs, 0.5, yref)
+    absax = O.abs_spmv(offsets, idx, vals, xs)
+    print("stream plan, strided x and y", flush=True)
+    y = run_compat(ctx, a, x, y0, -2.0, 0.5, H.ALG_PANEL, incx=3, incy=2)
+    bound_check(y[::2], yref, lens, absax, -2.0, 0.5, y0[::2], EPS[np.float64])
+    np.testing.assert_array_equal(y[1::2], y0[1::2])  # holes untouched
+    # empty rows with alpha < 0, beta == 0: y = alpha * (+0.0) = -0.0
+    y = run_compat(ctx, a, xs, np.full(rows, np.nan), -1.0, 0.0, H.ALG_PANEL)
+    empty = lens == 0
+    assert np.all(np.signbit(y[empty])) and np.all(y[empty] == 0.0)
+    # negative stride x: element 0 at the far end
+    xb, yb = ctx.upload(x), ctx.upload(np.zeros(rows))
+    xneg = np.ascontiguousarray(x[::-3][:cols])
+    last = xb.ptr + 8 * (3 * cols - 1)
+    print("stream plan, negative stride x", flush=True)
+    H.csr_spmv(ctx, 1.0, a, H.vector(last, cols, H.F64, stride=-3, writable=False), 0.0,
+               H.vector(yb, rows, H.F64), alg=H.ALG_PANEL)
+    yref = np.zeros(rows)
+    O.compat_spmv(offsets, idx, vals, 1.0, xneg, 0.0, yref)
+    bound_check(ctx.download(yb, np.float64, rows), yref, lens,
+                O.abs_spmv(offsets, idx, vals, xneg), 1.0, 0.0, np.zeros(rows), EPS[np.float64])
+    # standardized SpMV through the same plan: y + 0.75*A*x
+    y1 = rng.uniform(-1, 1, rows)
+    yref = y1.copy()
+    O.compat_spmv(offsets, idx, vals, 0.75, xs, 1.0, yref)
+    xb, yb = ctx.upload(xs), ctx.upload(y1)
+    print("stream plan, standardized SpMV", flush=True)
+    H.standard_spmv(ctx, H.OP_N, 0.75, a, H.vector(xb, cols, H.F64, writable=False),
+                    H.vector(yb, rows, H.F64), alg=H.ALG_PANEL)
+    bound_check(ctx.download(yb, np.float64, rows), yref, lens, absax, 0.75, 1.0, y1,
+                EPS[np.float64])
+    H.plan_clear(ctx)
+
+
+def test_stream_plan_reduced_precision_and_refusals(ctx):
+    m = G.make(G.ZIPF, 1 << 14, 1 << 14, seed=43, values=G.VALUES_NORMAL)
+    a = H.clone_compressed(ctx, H.CSR, m.rows, m.cols, m.offsets, m.indices, m.values, H.F64, 4)
+    x = G.vector(m.cols, 5)
+    y64 = np.zeros(m.rows)
+    O.compat_spmv(m.offsets, m.indices, m.values, 1.0, x, 0.0, y64)
+    absax = O.abs_spmv(m.offsets, m.indices, m.values, x)
+    for kind, tol in ((H.BF16, 1e-2), (H.F16, 1e-2), (H.F32, 1e-6)):
+        low = H.convert_values(ctx, a, kind)
+        H.csr_spmv_analyze(ctx, low, H.panel_options(1500, 32, staging=H.STAGING_STREAM))
+        print(f"stream plan, storage kind {kind}", flush=True)
+        y = run_compat(ctx, low, x, np.zeros(m.rows), 1.0, 0.0, H.ALG_PANEL)
+        err = np.max(np.abs(y - y64)) / np.max(absax)
+        assert err <= tol, (kind, err)
+        with pytest.raises(H.AscHipError):  # plain accumulation only
+            run_compat(ctx, low, x, np.zeros(m.rows), 1.0, 0.0, H.ALG_PANEL, acc=H.ACC_TWOSUM)
+    # values rebound after the analysis: explicit PANEL is refused, AUTO runs
+    # merge-path on the new values, a new analysis serves them
+    H.csr_spmv_analyze(ctx, a, H.panel_options(1500, 32, staging=H.STAGING_STREAM))
+    new_vals = np.random.default_rng(4).standard_normal(m.nnz)
+    b = H.rebind_values(a, ctx.upload(new_vals))
+    yref = np.zeros(m.rows)
+    O.compat_spmv(m.offsets, m.indices, new_vals, 1.0, x, 0.0, yref)
+    absnew = O.abs_spmv(m.offsets, m.indices, new_vals, x)
+    lens = np.diff(m.offsets)
+    with pytest.raises(H.AscHipError):
+        run_compat(ctx, b, x, np.zeros(m.rows), 1.0, 0.0, H.ALG_PANEL)
+    print("stream plan, rebound values through AUTO", flush=True)
+    y = run_compat(ctx, b, x, np.zeros(m.rows), 1.0, 0.0, H.ALG_AUTO)
+    bound_check(y, yref, lens, absnew, 1.0, 0.0, np.zeros(m.rows), EPS[np.float64])
+    H.csr_spmv_analyze(ctx, b, H.panel_options(1500, 32, staging=H.STAGING_STREAM))
+    y = run_compat(ctx, b, x, np.zeros(m.rows), 1.0, 0.0, H.ALG_PANEL)
+    bound_check(y, yref, lens, absnew, 1.0, 0.0, np.zeros(m.rows), EPS[np.float64])
+    # the size query counts the listed rows: far below the every-row bound
+    need = H.csr_spmv_analysis_size(ctx, a, H.panel_options(0, 64, staging=H.STAGING_STREAM))
+    vctx = H.Context(-1)
+    bound = H.csr_spmv_analysis_size(vctx, a, H.panel_options(0, 64, staging=H.STAGING_STREAM))
+    vctx.close()
+    assert need < bound
+    H.plan_clear(ctx)
