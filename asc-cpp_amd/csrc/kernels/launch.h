@@ -87,9 +87,9 @@ struct PanelPlan {
 constexpr int64_t kLdsPanelBytes = int64_t(160) << 10;
 // Stream form geometry: entries per lane per step, entries per wave step,
 // entries per slice (one wave's share), slices per work-group.
-constexpr int kStreamEpl = 4;
+constexpr int kStreamEpl = 8;
 constexpr int kStreamTile = 64 * kStreamEpl;
-constexpr int kStreamSlice = 32 * kStreamTile;
+constexpr int kStreamSlice = 16 * kStreamTile;
 constexpr int kStreamWaves = 16;
 constexpr uint16_t kStreamHead = 0x8000;
 constexpr uint32_t kStreamClosed = 0x80000000u;

@@ -448,55 +448,157 @@ __global__ __launch_bounds__(kBlock) void panel_split_kernel(
 // ---------------------------------------------------------------------------
 typedef double f64x2_t __attribute__((ext_vector_type(2)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
-typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ void stream_values(const double* p, double (&v)[kStreamEpl]) {
-  const f64x2_t a = __builtin_nontemporal_load(reinterpret_cast<const f64x2_t*>(p));
-  const f64x2_t b = __builtin_nontemporal_load(reinterpret_cast<const f64x2_t*>(p) + 1);
-  v[0] = a.x;
-  v[1] = a.y;
-  v[2] = b.x;
-  v[3] = b.y;
+static_assert(kStreamEpl == 8, "the stream loads below move 8 entries per lane");
+
+// Values are stored lane-interleaved inside each step of kStreamTile entries:
+// with w = 16 / sizeof(V) values per 16-byte word, entry e = 8*l + w*k + r of
+// a step sits at k*64*w + l*w + r, so the k-th 16-byte load of a wave reads
+// one contiguous kiB and still hands lane l its own eight consecutive entries.
+template <typename V>
+__host__ __device__ __forceinline__ int64_t stream_value_pos(int64_t pos) {
+  constexpr int64_t w = 16 / static_cast<int64_t>(sizeof(V));
+  const int64_t base = pos & ~static_cast<int64_t>(kStreamTile - 1);
+  const int64_t e = pos & (kStreamTile - 1);
+  const int64_t l = e / kStreamEpl, j = e % kStreamEpl;
+  return base + (j / w) * kWave * w + l * w + j % w;
 }
-__device__ __forceinline__ void stream_values(const float* p, float (&v)[kStreamEpl]) {
-  const f32x4_t a = __builtin_nontemporal_load(reinterpret_cast<const f32x4_t*>(p));
-  v[0] = a.x;
-  v[1] = a.y;
-  v[2] = a.z;
-  v[3] = a.w;
+
+__device__ __forceinline__ void stream_values(const double* p, int lane, double (&v)[kStreamEpl]) {
+  const f64x2_t* q = reinterpret_cast<const f64x2_t*>(p) + lane;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const f64x2_t a = __builtin_nontemporal_load(q + k * kWave);
+    v[2 * k] = a.x;
+    v[2 * k + 1] = a.y;
+  }
 }
-__device__ __forceinline__ void stream_values(const bf16_t* p, float (&v)[kStreamEpl]) {
-  const u32x2_t a = __builtin_nontemporal_load(reinterpret_cast<const u32x2_t*>(p));
-  v[0] = __uint_as_float(a.x << 16);
-  v[1] = __uint_as_float(a.x & 0xFFFF0000u);
-  v[2] = __uint_as_float(a.y << 16);
-  v[3] = __uint_as_float(a.y & 0xFFFF0000u);
+__device__ __forceinline__ void stream_values(const float* p, int lane, float (&v)[kStreamEpl]) {
+  const f32x4_t* q = reinterpret_cast<const f32x4_t*>(p) + lane;
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const f32x4_t a = __builtin_nontemporal_load(q + k * kWave);
+    v[4 * k] = a.x;
+    v[4 * k + 1] = a.y;
+    v[4 * k + 2] = a.z;
+    v[4 * k + 3] = a.w;
+  }
 }
-__device__ __forceinline__ void stream_values(const f16_t* p, float (&v)[kStreamEpl]) {
-  const u32x2_t a = __builtin_nontemporal_load(reinterpret_cast<const u32x2_t*>(p));
-  f16_t h;
-  h.bits = static_cast<uint16_t>(a.x & 0xFFFFu);
-  v[0] = to_float(h);
-  h.bits = static_cast<uint16_t>(a.x >> 16);
-  v[1] = to_float(h);
-  h.bits = static_cast<uint16_t>(a.y & 0xFFFFu);
-  v[2] = to_float(h);
-  h.bits = static_cast<uint16_t>(a.y >> 16);
-  v[3] = to_float(h);
+// 16-bit storage: eight values in one 16-byte load, two per 32-bit word.
+__device__ __forceinline__ u32x4_t stream_words(const void* p) {
+  return __builtin_nontemporal_load(reinterpret_cast<const u32x4_t*>(p));
+}
+__device__ __forceinline__ uint32_t word_of(const u32x4_t& w, int k) {
+  return k == 0 ? w.x : k == 1 ? w.y : k == 2 ? w.z : w.w;
+}
+__device__ __forceinline__ void stream_values(const bf16_t* p, int lane, float (&v)[kStreamEpl]) {
+  const u32x4_t w = stream_words(p + kStreamEpl * lane);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    v[2 * k] = __uint_as_float(word_of(w, k) << 16);
+    v[2 * k + 1] = __uint_as_float(word_of(w, k) & 0xFFFF0000u);
+  }
+}
+__device__ __forceinline__ void stream_values(const f16_t* p, int lane, float (&v)[kStreamEpl]) {
+  const u32x4_t w = stream_words(p + kStreamEpl * lane);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    f16_t h;
+    h.bits = static_cast<uint16_t>(word_of(w, k) & 0xFFFFu);
+    v[2 * k] = to_float(h);
+    h.bits = static_cast<uint16_t>(word_of(w, k) >> 16);
+    v[2 * k + 1] = to_float(h);
+  }
 }
 
 template <typename V>
 struct StreamStep {
-  u32x2_t c;  // four 16-bit columns
+  u32x4_t c;  // eight 16-bit columns
   typename acc_of<V>::type v[kStreamEpl];
 };
 
+// The step starting at entry t (a multiple of kStreamTile from the panel's
+// start): lane's eight columns and values.
 template <typename V>
 __device__ __forceinline__ void stream_load(const uint16_t* __restrict__ scol,
-                                            const V* __restrict__ sval, int64_t e,
+                                            const V* __restrict__ sval, int64_t t, int lane,
                                             StreamStep<V>& s) {
-  s.c = __builtin_nontemporal_load(reinterpret_cast<const u32x2_t*>(scol + e));
-  stream_values(sval + e, s.v);
+  s.c = __builtin_nontemporal_load(reinterpret_cast<const u32x4_t*>(scol + t + kStreamEpl * lane));
+  stream_values(sval + t, lane, s.v);
+}
+
+// The running state of one wave's slice: the open segment's ordinal
+// (segments are numbered panel-major; before the slice's first head it may
+// wrap below zero) and its sum over this slice's entries so far.
+template <typename A>
+struct StreamOpen {
+  uint32_t ord;
+  A sum;
+};
+
+// One step: kStreamTile entries from t, kStreamEpl consecutive per lane.
+template <typename V, typename A>
+__device__ __forceinline__ void stream_step(const StreamStep<V>& cur, const A* sx, int64_t t,
+                                            int64_t s0, int64_t s1, int lane, uint64_t below,
+                                            const uint32_t* __restrict__ seg_row,
+                                            double* __restrict__ prow, StreamOpen<A>& st) {
+  const int64_t e = t + kStreamEpl * lane;
+  bool h[kStreamEpl];
+  A p[kStreamEpl];
+#pragma unroll
+  for (int j = 0; j < kStreamEpl; ++j) {
+    const uint32_t c = (word_of(cur.c, j / 2) >> (16 * (j & 1))) & 0xFFFFu;
+    const bool valid = e + j < s1;
+    h[j] = valid && (c & kStreamHead) != 0;
+    const A prod = cur.v[j] * sx[c & 0x7FFFu];
+    p[j] = valid ? prod : A(0);
+  }
+  // A head on the step's first entry ends the segment open before it.
+  if (lane == 0 && h[0] && t > s0) prow[seg_row[st.ord]] = static_cast<double>(st.sum);
+  // The lane's trailing run, then a segmented inclusive scan over lanes.
+  A u = A(0);
+  bool f = false;
+#pragma unroll
+  for (int j = 0; j < kStreamEpl; ++j) {
+    u = h[j] ? p[j] : u + p[j];
+    f = f || h[j];
+  }
+  A s = u;
+  int fl = f ? 1 : 0;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const A su = __shfl_up(s, off);
+    const int fu = __shfl_up(fl, off);
+    if (lane >= off) {
+      if (!fl) s += su;
+      fl |= fu;
+    }
+  }
+  if (!fl) s += st.sum;
+  A run = __shfl_up(s, 1);
+  if (lane == 0) run = st.sum;
+  int before = 0, total = 0;
+#pragma unroll
+  for (int j = 0; j < kStreamEpl; ++j) {
+    const uint64_t heads = __ballot(h[j]);
+    before += __popcll(heads & below);
+    total += __popcll(heads);
+  }
+  // An entry ends its segment when the next entry is a head: the next lane's
+  // first entry for a lane's last, the next step for lane 63's.
+  const int next_head = __shfl_down(h[0] ? 1 : 0, 1);
+  uint32_t ord = st.ord + static_cast<uint32_t>(before);
+#pragma unroll
+  for (int j = 0; j < kStreamEpl; ++j) {
+    run = h[j] ? p[j] : run + p[j];
+    ord += h[j] ? 1u : 0u;
+    const bool ends = j + 1 < kStreamEpl ? h[(j + 1) % kStreamEpl]
+                                         : (lane + 1 < kWave && next_head != 0);
+    if (ends) prow[seg_row[ord]] = static_cast<double>(run);
+  }
+  st.sum = __shfl(s, kWave - 1);
+  st.ord += static_cast<uint32_t>(total);
 }
 
 template <typename V>
@@ -511,94 +613,75 @@ __global__ __launch_bounds__(kLdsBlock) void spmv_stream_kernel(
   const uint32_t first = wg_table[2 * blockIdx.x];
   const uint32_t count = wg_table[2 * blockIdx.x + 1];
   const int64_t q = slice_info[4 * static_cast<int64_t>(first) + 3] & ~kStreamClosed;
-  const int64_t col0 = q * panel_cols;
-  const int64_t width = cols - col0 < panel_cols ? cols - col0 : panel_cols;
-  for (int64_t k = threadIdx.x; k < width; k += kLdsBlock) sx[k] = x[(col0 + k) * incx];
-  __syncthreads();
   const int wave = threadIdx.x / kWave;
-  if (wave >= static_cast<int>(count)) return;  // whole waves leave together
   const int lane = threadIdx.x % kWave;
-  const int64_t slice = static_cast<int64_t>(first) + wave;
+  const bool active = wave < static_cast<int>(count);
+  const int64_t slice = static_cast<int64_t>(first) + (active ? wave : 0);
   const uint32_t* si = slice_info + 4 * slice;
   const int64_t s0 = si[0], s1 = si[1];
+  // The slice's first two steps load while the panel of x is staged.
+  // Slices end inside their panel's padded range, so a step's loads stay in
+  // the allocation even past the slice's end.
+  StreamStep<V> b0, b1;
+  StreamStep<V> b2;  // the rotation's third buffer (storage narrower than FP64)
+  if (active) {
+    stream_load(scol, sval, s0, lane, b0);
+    if (s0 + kStreamTile < s1) stream_load(scol, sval, s0 + kStreamTile, lane, b1);
+  }
+  const int64_t col0 = q * panel_cols;
+  const int64_t width = cols - col0 < panel_cols ? cols - col0 : panel_cols;
+  const A* __restrict__ xp = x + col0 * incx;
+  int64_t k = threadIdx.x;
+  for (; k + 3 * kLdsBlock < width; k += 4 * kLdsBlock) {
+    const A x0 = xp[k * incx];
+    const A x1 = xp[(k + kLdsBlock) * incx];
+    const A x2 = xp[(k + 2 * kLdsBlock) * incx];
+    const A x3 = xp[(k + 3 * kLdsBlock) * incx];
+    sx[k] = x0;
+    sx[k + kLdsBlock] = x1;
+    sx[k + 2 * kLdsBlock] = x2;
+    sx[k + 3 * kLdsBlock] = x3;
+  }
+  for (; k < width; k += kLdsBlock) sx[k] = xp[k * incx];
+  __syncthreads();
+  if (!active) return;  // whole waves leave together
   const bool closed = (si[3] & kStreamClosed) != 0;
   double* __restrict__ prow = partial + q * nlong;
-  // The open segment: its ordinal (segments are numbered panel-major; the
-  // value before the slice's first head may wrap below zero), and its sum
-  // over this slice's entries so far.
-  uint32_t open_ord = si[2];
-  A open = A(0);
   const uint64_t below = lane == 0 ? 0ull : (~0ull >> (kWave - lane));
-  StreamStep<V> cur, nxt;
-  stream_load(scol, sval, s0 + kStreamEpl * lane, cur);
-  for (int64_t t = s0; t < s1; t += kStreamTile) {
-    const int64_t e = t + kStreamEpl * lane;
-    // Slices end inside their panel's padded range, so the next step's loads
-    // stay in the allocation even past the slice's end.
-    if (t + kStreamTile < s1) stream_load(scol, sval, e + kStreamTile, nxt);
-    bool h[kStreamEpl];
-    A p[kStreamEpl];
-#pragma unroll
-    for (int j = 0; j < kStreamEpl; ++j) {
-      const uint32_t c = ((j < 2 ? cur.c.x : cur.c.y) >> (16 * (j & 1))) & 0xFFFFu;
-      const bool valid = e + j < s1;
-      h[j] = valid && (c & kStreamHead) != 0;
-      const A prod = cur.v[j] * sx[c & 0x7FFFu];
-      p[j] = valid ? prod : A(0);
+  StreamOpen<A> st;
+  st.ord = si[2];
+  st.sum = A(0);
+  int64_t t = s0;
+  if constexpr (sizeof(V) < 8) {
+    // Two steps in flight: the three buffers rotate roles.
+    constexpr int64_t ahead = 2 * kStreamTile;
+    while (true) {
+      if (t + 2 * kStreamTile < s1) stream_load(scol, sval, t + ahead, lane, b2);
+      stream_step<V, A>(b0, sx, t, s0, s1, lane, below, seg_row, prow, st);
+      if ((t += kStreamTile) >= s1) break;
+      if (t + 2 * kStreamTile < s1) stream_load(scol, sval, t + ahead, lane, b0);
+      stream_step<V, A>(b1, sx, t, s0, s1, lane, below, seg_row, prow, st);
+      if ((t += kStreamTile) >= s1) break;
+      if (t + 2 * kStreamTile < s1) stream_load(scol, sval, t + ahead, lane, b1);
+      stream_step<V, A>(b2, sx, t, s0, s1, lane, below, seg_row, prow, st);
+      if ((t += kStreamTile) >= s1) break;
     }
-    // A head on the step's first entry ends the segment open before it.
-    if (lane == 0 && h[0] && t > s0) prow[seg_row[open_ord]] = static_cast<double>(open);
-    // The lane's trailing run, then a segmented inclusive scan over lanes.
-    A u = A(0);
-    bool f = false;
-#pragma unroll
-    for (int j = 0; j < kStreamEpl; ++j) {
-      u = h[j] ? p[j] : u + p[j];
-      f = f || h[j];
+  } else {
+    // FP64 steps hold twice the registers: one step in flight (b1 was loaded
+    // before the barrier), two buffers alternating.
+    constexpr int64_t ahead = kStreamTile;
+    while (true) {
+      stream_step<V, A>(b0, sx, t, s0, s1, lane, below, seg_row, prow, st);
+      if ((t += kStreamTile) >= s1) break;
+      if (t + kStreamTile < s1) stream_load(scol, sval, t + ahead, lane, b0);
+      stream_step<V, A>(b1, sx, t, s0, s1, lane, below, seg_row, prow, st);
+      if ((t += kStreamTile) >= s1) break;
+      if (t + kStreamTile < s1) stream_load(scol, sval, t + ahead, lane, b1);
     }
-    A s = u;
-    int fl = f ? 1 : 0;
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-      const A su = __shfl_up(s, off);
-      const int fu = __shfl_up(fl, off);
-      if (lane >= off) {
-        if (!fl) s += su;
-        fl |= fu;
-      }
-    }
-    if (!fl) s += open;
-    A run = __shfl_up(s, 1);
-    if (lane == 0) run = open;
-    uint64_t heads[kStreamEpl];
-    int before = 0, total = 0;
-#pragma unroll
-    for (int j = 0; j < kStreamEpl; ++j) {
-      heads[j] = __ballot(h[j]);
-      before += __popcll(heads[j] & below);
-      total += __popcll(heads[j]);
-    }
-    // An entry ends its segment when the next entry is a head; the next
-    // lane's first entry for a lane's last, the next step for lane 63's.
-    const int next_head = __shfl_down(h[0] ? 1 : 0, 1);
-    bool ends[kStreamEpl];
-#pragma unroll
-    for (int j = 0; j + 1 < kStreamEpl; ++j) ends[j] = h[j + 1];
-    ends[kStreamEpl - 1] = lane + 1 < kWave && next_head != 0;
-    uint32_t ord = open_ord + static_cast<uint32_t>(before);
-#pragma unroll
-    for (int j = 0; j < kStreamEpl; ++j) {
-      run = h[j] ? p[j] : run + p[j];
-      ord += h[j] ? 1u : 0u;
-      if (ends[j]) prow[seg_row[ord]] = static_cast<double>(run);
-    }
-    open = __shfl(s, kWave - 1);
-    open_ord += static_cast<uint32_t>(total);
-    cur = nxt;
   }
   if (lane == 0) {
-    if (closed) prow[seg_row[open_ord]] = static_cast<double>(open);
-    else carry[slice] = static_cast<double>(open);
+    if (closed) prow[seg_row[st.ord]] = static_cast<double>(st.sum);
+    else carry[slice] = static_cast<double>(st.sum);
   }
 }
 
@@ -681,7 +764,7 @@ __global__ __launch_bounds__(kBlock) void stream_scatter_kernel(
   for (uint32_t k = 0; k < len; ++k) {
     const uint32_t c = static_cast<uint32_t>(static_cast<int64_t>(col[src + k]) - col0);
     scol[dst + k] = static_cast<uint16_t>(c | (k == 0 ? kStreamHead : 0u));
-    sval[dst + k] = val[src + k];
+    sval[stream_value_pos<VB>(dst + k)] = val[src + k];
   }
 }
 
