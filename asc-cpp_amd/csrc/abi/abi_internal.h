@@ -19,6 +19,9 @@ struct asc_hip_context {
   int device = -1;  // -1: validation-only context
   hipStream_t stream = nullptr;
   bool owns_stream = false;
+  // A second stream (default priority, library-owned) and fork/join events:
+  // work of one call that can run beside the call's main kernels goes there.
+  asc_hip::SideStream side{};
   uint8_t determinism = ASC_HIP_DETERMINISTIC;
   // Bytes of B per SpMM column pass; <= 0: one pass.
   int64_t spmm_panel_bytes = 0;

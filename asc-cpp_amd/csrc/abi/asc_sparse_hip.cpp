@@ -302,6 +302,7 @@ asc_hip_status unavailable_if_validation_only(const asc_hip_context* ctx) {
 asc_hip_status finish_launch(asc_hip_context* ctx, hipError_t e, const char* what) {
   if (e == hipSuccess) return ok();
   (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->side.stream) (void)hipStreamSynchronize(ctx->side.stream);
   (void)hipGetLastError();
   return fail_hip(e, what);
 }
@@ -431,6 +432,17 @@ asc_hip_status asc_sparse_hip_context_create(int32_t device, void* stream,
     }
     ctx->owns_stream = true;
   }
+  e = hipStreamCreateWithFlags(&ctx->side.stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->side.fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->side.join, hipEventDisableTiming);
+  if (e != hipSuccess) {
+    if (ctx->side.join) (void)hipEventDestroy(ctx->side.join);
+    if (ctx->side.fork) (void)hipEventDestroy(ctx->side.fork);
+    if (ctx->side.stream) (void)hipStreamDestroy(ctx->side.stream);
+    if (ctx->owns_stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return fail_hip(e, "creating the side stream");
+  }
   *out = ctx;
   return ok();
 }
@@ -444,6 +456,12 @@ asc_hip_status asc_sparse_hip_context_destroy(asc_hip_context* ctx) {
       (void)hipStreamSynchronize(ctx->stream);
       (void)hipStreamDestroy(ctx->stream);
     }
+    if (ctx->side.stream) {
+      (void)hipStreamSynchronize(ctx->side.stream);
+      (void)hipStreamDestroy(ctx->side.stream);
+    }
+    if (ctx->side.fork) (void)hipEventDestroy(ctx->side.fork);
+    if (ctx->side.join) (void)hipEventDestroy(ctx->side.join);
   }
   ctx->magic = 0;
   delete ctx;

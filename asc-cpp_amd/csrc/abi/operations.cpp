@@ -477,7 +477,7 @@ asc_hip_status run_spmv(asc_hip_context* ctx, SpmvPlan& plan, void* ws) {
     if (e != hipSuccess) return finish_launch(ctx, e, "transpose");
   }
   const hipError_t e =
-      asc_hip::spmv_launch(plan.args, w + transpose_prefix(plan), ctx->stream);
+      asc_hip::spmv_launch(plan.args, w + transpose_prefix(plan), ctx->stream, &ctx->side);
   return finish_launch(ctx, e, "SpMV launch");
 }
 

@@ -62,17 +62,16 @@ struct PanelPlan {
   // [slice starts of q ...), each panel padded to a multiple of kStreamTile
   // entries.  scol holds the column within the panel, with kStreamHead set on
   // the first entry of every (panel, row) segment; sval the value.  Nonempty
-  // segments are numbered panel-major; seg_row maps a segment to its listed
-  // row.  A slice (kStreamSlice entries at most, one wave) is described by
+  // segments are numbered panel-major; seg_map[q*nlong + i] is the ordinal
+  // of the (panel q, listed row i) segment, or kStreamNoSegment when empty.  A slice (kStreamSlice entries at most, one wave) is described by
   // slice_info[4k..4k+3] = {start, end, segment of the first entry,
   // panel | kStreamClosed when its last segment ends inside it}; wg_table[2w..]
   // = {first slice, slice count} of work-group w (one panel, XCD-ordered).
-  // spartial[q*nlong + i] holds (panel q, row i) partial sums; the analysis
-  // zeroes it and the SpMV rewrites only nonempty segments.
+  // spartial[ordinal] holds each segment's sum, written by the SpMV.
   bool stream = false;
   const uint16_t* scol = nullptr;
   const void* sval = nullptr;
-  const uint32_t* seg_row = nullptr;
+  const uint32_t* seg_map = nullptr;
   const uint32_t* slice_info = nullptr;
   const uint32_t* wg_table = nullptr;
   double* spartial = nullptr;
@@ -93,6 +92,7 @@ constexpr int kStreamSlice = 16 * kStreamTile;
 constexpr int kStreamWaves = 16;
 constexpr uint16_t kStreamHead = 0x8000;
 constexpr uint32_t kStreamClosed = 0x80000000u;
+constexpr uint32_t kStreamNoSegment = 0xFFFFFFFFu;
 // Widest stream panel: the column field keeps 15 bits beside the head flag.
 constexpr int64_t kStreamMaxPanelCols = 32768;
 
@@ -110,10 +110,21 @@ struct SpmvArgs {
   PanelPlan panel;  // used when alg == Panel
 };
 
+// A stream beside the main one and two events: `fork` recorded on the main
+// stream before side work starts, `join` recorded on the side stream after it.
+struct SideStream {
+  hipStream_t stream = nullptr;
+  hipEvent_t fork = nullptr;
+  hipEvent_t join = nullptr;
+};
+
 // Workspace bytes the algorithm needs for this operand (0 for most).
 size_t spmv_workspace_bytes(const SpmvArgs& args);
+// With `side`, the stream-form panel plan runs its short rows on the side
+// stream, beside the panel kernels; the main stream waits for them before it
+// returns.
 hipError_t spmv_launch(const SpmvArgs& args, void* workspace,
-                       hipStream_t stream);
+                       hipStream_t stream, const SideStream* side = nullptr);
 
 // Builds a PanelPlan's row list, row starts and split table for a canonical
 // CSR operand (sorted columns per row): `long_rows` holds `outer` uint32,
@@ -131,9 +142,9 @@ hipError_t panel_analysis_launch(const Compressed& a, int64_t panel_cols, int64_
 struct StreamLayout {
   int64_t nlong = 0, nnz_long = 0, panels = 0, pairs = 0;
   int64_t entries_cap = 0, slices_cap = 0, segs_cap = 0;
-  size_t long_rows = 0, partial = 0, scol = 0, sval = 0, seg_row = 0, slice_info = 0,
-         wg_table = 0;  // kept
-  size_t row_begin = 0, splits = 0, lens = 0, begins = 0, ords = 0, small = 0,
+  size_t long_rows = 0, partial = 0, scol = 0, sval = 0, ords = 0, slice_info = 0,
+         wg_table = 0;  // kept (ords: the segment map)
+  size_t row_begin = 0, splits = 0, lens = 0, begins = 0, small = 0,
          slice_stage = 0, scan = 0, list_scratch = 0;  // analysis only
   size_t kept_bytes = 0, total = 0;
 };

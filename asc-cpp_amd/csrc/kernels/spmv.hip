@@ -541,8 +541,7 @@ struct StreamOpen {
 template <typename V, typename A>
 __device__ __forceinline__ void stream_step(const StreamStep<V>& cur, const A* sx, int64_t t,
                                             int64_t s0, int64_t s1, int lane, uint64_t below,
-                                            const uint32_t* __restrict__ seg_row,
-                                            double* __restrict__ prow, StreamOpen<A>& st) {
+                                            double* __restrict__ partial, StreamOpen<A>& st) {
   const int64_t e = t + kStreamEpl * lane;
   bool h[kStreamEpl];
   A p[kStreamEpl];
@@ -555,7 +554,7 @@ __device__ __forceinline__ void stream_step(const StreamStep<V>& cur, const A* s
     p[j] = valid ? prod : A(0);
   }
   // A head on the step's first entry ends the segment open before it.
-  if (lane == 0 && h[0] && t > s0) prow[seg_row[st.ord]] = static_cast<double>(st.sum);
+  if (lane == 0 && h[0] && t > s0) partial[st.ord] = static_cast<double>(st.sum);
   // The lane's trailing run, then a segmented inclusive scan over lanes.
   A u = A(0);
   bool f = false;
@@ -595,7 +594,7 @@ __device__ __forceinline__ void stream_step(const StreamStep<V>& cur, const A* s
     ord += h[j] ? 1u : 0u;
     const bool ends = j + 1 < kStreamEpl ? h[(j + 1) % kStreamEpl]
                                          : (lane + 1 < kWave && next_head != 0);
-    if (ends) prow[seg_row[ord]] = static_cast<double>(run);
+    if (ends) partial[ord] = static_cast<double>(run);
   }
   st.sum = __shfl(s, kWave - 1);
   st.ord += static_cast<uint32_t>(total);
@@ -605,9 +604,8 @@ template <typename V>
 __global__ __launch_bounds__(kLdsBlock) void spmv_stream_kernel(
     const typename acc_of<V>::type* __restrict__ x, int64_t incx, int64_t cols,
     int64_t panel_cols, const uint16_t* __restrict__ scol, const V* __restrict__ sval,
-    const uint32_t* __restrict__ seg_row, const uint32_t* __restrict__ slice_info,
-    const uint32_t* __restrict__ wg_table, int64_t nlong, double* __restrict__ partial,
-    double* __restrict__ carry) {
+    const uint32_t* __restrict__ slice_info, const uint32_t* __restrict__ wg_table,
+    double* __restrict__ partial, double* __restrict__ carry) {
   using A = typename acc_of<V>::type;
   __shared__ A sx[kLdsPanelBytes / sizeof(A)];
   const uint32_t first = wg_table[2 * blockIdx.x];
@@ -646,7 +644,6 @@ __global__ __launch_bounds__(kLdsBlock) void spmv_stream_kernel(
   __syncthreads();
   if (!active) return;  // whole waves leave together
   const bool closed = (si[3] & kStreamClosed) != 0;
-  double* __restrict__ prow = partial + q * nlong;
   const uint64_t below = lane == 0 ? 0ull : (~0ull >> (kWave - lane));
   StreamOpen<A> st;
   st.ord = si[2];
@@ -657,13 +654,13 @@ __global__ __launch_bounds__(kLdsBlock) void spmv_stream_kernel(
     constexpr int64_t ahead = 2 * kStreamTile;
     while (true) {
       if (t + 2 * kStreamTile < s1) stream_load(scol, sval, t + ahead, lane, b2);
-      stream_step<V, A>(b0, sx, t, s0, s1, lane, below, seg_row, prow, st);
+      stream_step<V, A>(b0, sx, t, s0, s1, lane, below, partial, st);
       if ((t += kStreamTile) >= s1) break;
       if (t + 2 * kStreamTile < s1) stream_load(scol, sval, t + ahead, lane, b0);
-      stream_step<V, A>(b1, sx, t, s0, s1, lane, below, seg_row, prow, st);
+      stream_step<V, A>(b1, sx, t, s0, s1, lane, below, partial, st);
       if ((t += kStreamTile) >= s1) break;
       if (t + 2 * kStreamTile < s1) stream_load(scol, sval, t + ahead, lane, b1);
-      stream_step<V, A>(b2, sx, t, s0, s1, lane, below, seg_row, prow, st);
+      stream_step<V, A>(b2, sx, t, s0, s1, lane, below, partial, st);
       if ((t += kStreamTile) >= s1) break;
     }
   } else {
@@ -671,30 +668,28 @@ __global__ __launch_bounds__(kLdsBlock) void spmv_stream_kernel(
     // before the barrier), two buffers alternating.
     constexpr int64_t ahead = kStreamTile;
     while (true) {
-      stream_step<V, A>(b0, sx, t, s0, s1, lane, below, seg_row, prow, st);
+      stream_step<V, A>(b0, sx, t, s0, s1, lane, below, partial, st);
       if ((t += kStreamTile) >= s1) break;
       if (t + kStreamTile < s1) stream_load(scol, sval, t + ahead, lane, b0);
-      stream_step<V, A>(b1, sx, t, s0, s1, lane, below, seg_row, prow, st);
+      stream_step<V, A>(b1, sx, t, s0, s1, lane, below, partial, st);
       if ((t += kStreamTile) >= s1) break;
       if (t + kStreamTile < s1) stream_load(scol, sval, t + ahead, lane, b1);
     }
   }
   if (lane == 0) {
-    if (closed) prow[seg_row[st.ord]] = static_cast<double>(st.sum);
+    if (closed) partial[st.ord] = static_cast<double>(st.sum);
     else carry[slice] = static_cast<double>(st.sum);
   }
 }
 
 // Adds the parts of segments that cross slice ends: the first slice of each
-// run of carries into one segment sums the run and adds it to the slot.
+// run of carries into one segment sums the run and adds it to the segment.
 __global__ __launch_bounds__(kBlock) void spmv_stream_fixup_kernel(
-    const uint32_t* __restrict__ slice_info, int64_t slices,
-    const uint32_t* __restrict__ seg_row, int64_t nlong, const double* __restrict__ carry,
+    const uint32_t* __restrict__ slice_info, int64_t slices, const double* __restrict__ carry,
     double* __restrict__ partial) {
   const int64_t k = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (k >= slices) return;
-  const uint32_t w = slice_info[4 * k + 3];
-  if (w & kStreamClosed) return;  // an open slice always has a successor in its panel
+  if (slice_info[4 * k + 3] & kStreamClosed) return;  // an open slice has a successor in its panel
   const uint32_t target = slice_info[4 * (k + 1) + 2];
   if (k > 0 && !(slice_info[4 * (k - 1) + 3] & kStreamClosed) &&
       slice_info[4 * k + 2] == target)
@@ -703,8 +698,25 @@ __global__ __launch_bounds__(kBlock) void spmv_stream_fixup_kernel(
   for (int64_t j = k + 1;
        !(slice_info[4 * j + 3] & kStreamClosed) && slice_info[4 * (j + 1) + 2] == target; ++j)
     sum += carry[j];
-  const int64_t q = w & ~kStreamClosed;
-  partial[q * nlong + seg_row[target]] += sum;
+  partial[target] += sum;
+}
+
+// Row epilogue of the stream form: listed row i adds its segments' sums in
+// panel order (seg_map[q*nlong + i]: the (panel q, row i) segment, or
+// kStreamNoSegment).
+template <typename V>
+__global__ __launch_bounds__(kBlock) void spmv_stream_combine_kernel(
+    KParams p, const uint32_t* __restrict__ long_rows, int64_t nlong, int64_t panels,
+    const uint32_t* __restrict__ seg_map, const double* __restrict__ partial) {
+  using Acc = typename AccSel<V, false>::type;
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= nlong) return;
+  double s = 0.0;
+  for (int64_t q = 0; q < panels; ++q) {
+    const uint32_t o = seg_map[q * nlong + i];
+    if (o != kStreamNoSegment) s += partial[o];
+  }
+  finish_row<V>(p, long_rows[i], Acc::from_f64(s));
 }
 
 // ---- stream-form analysis -------------------------------------------------
@@ -742,14 +754,13 @@ __global__ __launch_bounds__(kBlock) void stream_totals_kernel(
 }
 
 // One thread per (panel, listed row) segment: copies its entries into the
-// panel's stream, flags the first, and records the segment's row.
+// panel's stream and flags the first.
 template <typename I, typename VB>
 __global__ __launch_bounds__(kBlock) void stream_scatter_kernel(
     const I* __restrict__ col, const VB* __restrict__ val, const int64_t* __restrict__ row_begin,
     const uint32_t* __restrict__ splits, const uint32_t* __restrict__ begins,
-    const uint32_t* __restrict__ ords, const uint32_t* __restrict__ shift, int64_t nlong,
-    int64_t pairs, int64_t panel_cols, uint16_t* __restrict__ scol, VB* __restrict__ sval,
-    uint32_t* __restrict__ seg_row) {
+    const uint32_t* __restrict__ shift, int64_t nlong, int64_t pairs, int64_t panel_cols,
+    uint16_t* __restrict__ scol, VB* __restrict__ sval) {
   const int64_t idx = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (idx >= pairs) return;
   const int64_t q = idx / nlong;
@@ -757,7 +768,6 @@ __global__ __launch_bounds__(kBlock) void stream_scatter_kernel(
   const uint32_t sb = q > 0 ? splits[idx - nlong] : 0u;
   const uint32_t len = splits[idx] - sb;
   if (len == 0) return;
-  seg_row[ords[idx]] = static_cast<uint32_t>(i);
   const int64_t src = row_begin[i] + sb;
   const int64_t dst = static_cast<int64_t>(begins[idx]) + shift[q];
   const int64_t col0 = q * panel_cols;
@@ -794,6 +804,15 @@ __global__ __launch_bounds__(kBlock) void stream_slice_kernel(
   slice_info[4 * k + 1] = end;
   slice_info[4 * k + 2] = ord;
   slice_info[4 * k + 3] = static_cast<uint32_t>(q) | (closed ? kStreamClosed : 0u);
+}
+
+// Segment map: the ordinal of each nonempty (panel, row) segment, else
+// kStreamNoSegment (`flags` holds the nonempty flags).
+__global__ __launch_bounds__(kBlock) void stream_map_kernel(const uint32_t* __restrict__ flags,
+                                                            int64_t pairs,
+                                                            uint32_t* __restrict__ ords) {
+  const int64_t idx = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (idx < pairs && flags[idx] == 0) ords[idx] = kStreamNoSegment;
 }
 
 // ---------------------------------------------------------------------------
@@ -1156,9 +1175,67 @@ size_t carry_bytes() {
 
 inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
+// Short rows of a skewed matrix are bound by x gathers from the memory-side
+// cache, the stream-form panel kernels by HBM streaming: the short rows run
+// on the side stream while the panel kernels run on the main stream.
+template <typename V, typename I>
+hipError_t launch_short_rows(const SpmvArgs& a, const KParams& kp, int64_t skip, hipStream_t s) {
+  const int64_t rows = a.a.outer;
+  switch (a.panel.short_alg) {
+#define ASC_SIDE_CASE(ALG, G)                                                           \
+  case SpmvAlg::ALG:                                                                    \
+    spmv_vector_kernel<G, V, I, false>                                                  \
+        <<<static_cast<unsigned>((rows + kBlock / G - 1) / (kBlock / G)), kBlock, 0, s>>>( \
+            kp, skip);                                                                  \
+    return hipGetLastError();
+    ASC_SIDE_CASE(Vec1, 1)
+    ASC_SIDE_CASE(Vec2, 2)
+    ASC_SIDE_CASE(Vec4, 4)
+    ASC_SIDE_CASE(Vec8, 8)
+    ASC_SIDE_CASE(Vec16, 16)
+    ASC_SIDE_CASE(Vec32, 32)
+    ASC_SIDE_CASE(Vec64, 64)
+#undef ASC_SIDE_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+template <typename V, typename I>
+hipError_t launch_stream_forked(const SpmvArgs& a, const KParams& kp, void* ws, hipStream_t s,
+                                const SideStream& side) {
+  const PanelPlan& pp = a.panel;
+  double* carry = static_cast<double*>(ws);
+  hipError_t e = hipEventRecord(side.fork, s);
+  if (e != hipSuccess) return e;
+  spmv_stream_kernel<V><<<static_cast<unsigned>(pp.wgs), kLdsBlock, 0, s>>>(
+      static_cast<const typename acc_of<V>::type*>(kp.x), kp.incx, a.a.inner, pp.panel_cols,
+      pp.scol, static_cast<const V*>(pp.sval), pp.slice_info, pp.wg_table, pp.spartial, carry);
+  e = hipGetLastError();
+  // The side stream's work is joined below whatever happens after the fork.
+  hipError_t es = hipStreamWaitEvent(side.stream, side.fork, 0);
+  if (es == hipSuccess) es = launch_short_rows<V, I>(a, kp, pp.long_min, side.stream);
+  const hipError_t ej = hipEventRecord(side.join, side.stream);
+  if (e == hipSuccess) {
+    spmv_stream_fixup_kernel<<<static_cast<unsigned>((pp.slices + kBlock - 1) / kBlock), kBlock, 0,
+                               s>>>(pp.slice_info, pp.slices, carry, pp.spartial);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) {
+    spmv_stream_combine_kernel<V>
+        <<<static_cast<unsigned>((pp.nlong + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+            kp, pp.long_rows, pp.nlong, pp.panels, pp.seg_map, pp.spartial);
+    e = hipGetLastError();
+  }
+  const hipError_t ew = ej == hipSuccess ? hipStreamWaitEvent(s, side.join, 0) : ej;
+  if (e != hipSuccess) return e;
+  if (es != hipSuccess) return es;
+  return ew;
+}
+
 template <typename V, typename I, bool TWOSUM>
 hipError_t launch_typed(const SpmvArgs& a, const KParams& kp, void* ws,
-                        hipStream_t s) {
+                        hipStream_t s, const SideStream* side) {
   const int64_t rows = a.a.outer;
   auto blocks_for = [](int64_t n, int64_t per) {
     return static_cast<unsigned>((n + per - 1) / per);
@@ -1213,6 +1290,10 @@ hipError_t launch_typed(const SpmvArgs& a, const KParams& kp, void* ws,
       const PanelPlan& pp = a.panel;
       const int64_t skip = pp.nlong > 0 ? pp.long_min : INT64_MAX;
       hipError_t e = hipErrorInvalidValue;
+      if constexpr (!TWOSUM) {
+        if (pp.stream && pp.nlong > 0 && side != nullptr && side->stream != nullptr)
+          return launch_stream_forked<V, I>(a, kp, ws, s, *side);
+      }
       switch (pp.short_alg) {
 #define ASC_SHORT_CASE(ALG, G)                                                \
   case SpmvAlg::ALG:                                                          \
@@ -1239,16 +1320,16 @@ hipError_t launch_typed(const SpmvArgs& a, const KParams& kp, void* ws,
           double* carry = static_cast<double*>(ws);
           spmv_stream_kernel<V><<<static_cast<unsigned>(pp.wgs), kLdsBlock, 0, s>>>(
               static_cast<const typename acc_of<V>::type*>(kp.x), kp.incx, a.a.inner,
-              pp.panel_cols, pp.scol, static_cast<const V*>(pp.sval), pp.seg_row,
-              pp.slice_info, pp.wg_table, pp.nlong, pp.spartial, carry);
+              pp.panel_cols, pp.scol, static_cast<const V*>(pp.sval), pp.slice_info,
+              pp.wg_table, pp.spartial, carry);
           e = hipGetLastError();
           if (e != hipSuccess) return e;
           spmv_stream_fixup_kernel<<<blocks_for(pp.slices, kBlock), kBlock, 0, s>>>(
-              pp.slice_info, pp.slices, pp.seg_row, pp.nlong, carry, pp.spartial);
+              pp.slice_info, pp.slices, carry, pp.spartial);
           e = hipGetLastError();
           if (e != hipSuccess) return e;
-          spmv_panel_combine_kernel<V, TWOSUM><<<blocks_for(pp.nlong, kBlock), kBlock, 0, s>>>(
-              kp, pp.long_rows, pp.nlong, pp.panels, pp.spartial);
+          spmv_stream_combine_kernel<V><<<blocks_for(pp.nlong, kBlock), kBlock, 0, s>>>(
+              kp, pp.long_rows, pp.nlong, pp.panels, pp.seg_map, pp.spartial);
           return hipGetLastError();
         }
       }
@@ -1300,9 +1381,9 @@ hipError_t launch_typed(const SpmvArgs& a, const KParams& kp, void* ws,
 
 template <typename V, bool TWOSUM>
 hipError_t launch_by_index(const SpmvArgs& a, const KParams& kp, void* ws,
-                           hipStream_t s) {
-  if (a.a.index_bytes == 4) return launch_typed<V, int32_t, TWOSUM>(a, kp, ws, s);
-  return launch_typed<V, int64_t, TWOSUM>(a, kp, ws, s);
+                           hipStream_t s, const SideStream* side) {
+  if (a.a.index_bytes == 4) return launch_typed<V, int32_t, TWOSUM>(a, kp, ws, s, side);
+  return launch_typed<V, int64_t, TWOSUM>(a, kp, ws, s, side);
 }
 
 template <typename V>
@@ -1406,10 +1487,10 @@ StreamLayout stream_layout(int64_t rows, int64_t nlong, int64_t nnz_long, int64_
     return at;
   };
   L.long_rows = take(sizeof(uint32_t) * static_cast<size_t>(rows > 0 ? rows : 1));
-  L.partial = take(sizeof(double) * pairs);
+  L.partial = take(sizeof(double) * static_cast<size_t>(L.segs_cap > 0 ? L.segs_cap : 1));
   L.scol = take(sizeof(uint16_t) * static_cast<size_t>(L.entries_cap));
   L.sval = take(value_bytes * static_cast<size_t>(L.entries_cap));
-  L.seg_row = take(sizeof(uint32_t) * static_cast<size_t>(L.segs_cap > 0 ? L.segs_cap : 1));
+  L.ords = take(sizeof(uint32_t) * pairs);  // the segment map
   L.slice_info = take(4 * sizeof(uint32_t) * static_cast<size_t>(L.slices_cap));
   L.wg_table = take(2 * sizeof(uint32_t) * static_cast<size_t>(L.slices_cap));
   L.kept_bytes = off;
@@ -1417,7 +1498,6 @@ StreamLayout stream_layout(int64_t rows, int64_t nlong, int64_t nnz_long, int64_
   L.splits = take(sizeof(uint32_t) * pairs);
   L.lens = take(sizeof(uint32_t) * pairs);
   L.begins = take(sizeof(uint32_t) * pairs);
-  L.ords = take(sizeof(uint32_t) * pairs);
   L.small = take(sizeof(uint32_t) * static_cast<size_t>(2 * panels + 4));
   L.slice_stage = take(4 * sizeof(uint32_t) * static_cast<size_t>(L.slices_cap));
   L.scan = take(scan_u32_scratch_bytes(L.pairs));
@@ -1434,7 +1514,6 @@ hipError_t panel_stream_analysis_launch(const Compressed& a, const StreamLayout&
   double* partial = reinterpret_cast<double*>(w + L.partial);
   uint16_t* scol = reinterpret_cast<uint16_t*>(w + L.scol);
   void* sval = w + L.sval;
-  uint32_t* seg_row = reinterpret_cast<uint32_t*>(w + L.seg_row);
   uint32_t* slice_info = reinterpret_cast<uint32_t*>(w + L.slice_info);
   uint32_t* wg_table = reinterpret_cast<uint32_t*>(w + L.wg_table);
   int64_t* row_begin = reinterpret_cast<int64_t*>(w + L.row_begin);
@@ -1452,7 +1531,7 @@ hipError_t panel_stream_analysis_launch(const Compressed& a, const StreamLayout&
   pp->splits = nullptr;
   pp->scol = scol;
   pp->sval = sval;
-  pp->seg_row = seg_row;
+  pp->seg_map = ords;
   pp->slice_info = slice_info;
   pp->wg_table = wg_table;
   pp->spartial = partial;
@@ -1551,13 +1630,11 @@ hipError_t panel_stream_analysis_launch(const Compressed& a, const StreamLayout&
   // (empty segments are never written and stay zero).
   if (e == hipSuccess) e = fill_launch(scol, 0, sizeof(uint16_t) * static_cast<size_t>(padded), s);
   if (e == hipSuccess) e = fill_launch(sval, 0, vb * static_cast<size_t>(padded), s);
-  if (e == hipSuccess) e = fill_launch(partial, 0, sizeof(double) * static_cast<size_t>(pairs), s);
   if (e != hipSuccess) return e;
 #define ASC_SCATTER(I, VB)                                                                   \
   stream_scatter_kernel<I, VB><<<pb, kBlock, 0, s>>>(                                      \
       static_cast<const I*>(a.indices), static_cast<const VB*>(a.values), row_begin, splits, \
-      begins, ords, dshift, nlong, pairs, pp->panel_cols, scol, static_cast<VB*>(sval),     \
-      seg_row)
+      begins, dshift, nlong, pairs, pp->panel_cols, scol, static_cast<VB*>(sval))
   if (a.index_bytes == 4) {
     if (vb == 8) ASC_SCATTER(int32_t, uint64_t);
     else if (vb == 4) ASC_SCATTER(int32_t, uint32_t);
@@ -1574,6 +1651,8 @@ hipError_t panel_stream_analysis_launch(const Compressed& a, const StreamLayout&
         stage, slices, begins, ords, nlong, scol, slice_info);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
+  stream_map_kernel<<<pb, kBlock, 0, s>>>(lens, pairs, ords);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
   // The host tables above must outlive their copies.
   if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
   pp->slices = slices;
@@ -1581,7 +1660,7 @@ hipError_t panel_stream_analysis_launch(const Compressed& a, const StreamLayout&
   return hipSuccess;
 }
 
-hipError_t spmv_launch(const SpmvArgs& a, void* ws, hipStream_t s) {
+hipError_t spmv_launch(const SpmvArgs& a, void* ws, hipStream_t s, const SideStream* side) {
   if (a.a.outer == 0) return hipSuccess;
   KParams kp;
   kp.offsets = a.a.offsets;
@@ -1597,16 +1676,16 @@ hipError_t spmv_launch(const SpmvArgs& a, void* ws, hipStream_t s) {
   kp.beta = a.beta;
   kp.form = static_cast<int>(a.form);
   switch (a.a.elem) {
-    case Elem::F64: return launch_by_index<double, false>(a, kp, ws, s);
+    case Elem::F64: return launch_by_index<double, false>(a, kp, ws, s, side);
     case Elem::F32:
-      return a.twosum ? launch_by_index<float, true>(a, kp, ws, s)
-                      : launch_by_index<float, false>(a, kp, ws, s);
+      return a.twosum ? launch_by_index<float, true>(a, kp, ws, s, side)
+                      : launch_by_index<float, false>(a, kp, ws, s, side);
     case Elem::BF16:
-      return a.twosum ? launch_by_index<bf16_t, true>(a, kp, ws, s)
-                      : launch_by_index<bf16_t, false>(a, kp, ws, s);
+      return a.twosum ? launch_by_index<bf16_t, true>(a, kp, ws, s, side)
+                      : launch_by_index<bf16_t, false>(a, kp, ws, s, side);
     case Elem::F16:
-      return a.twosum ? launch_by_index<f16_t, true>(a, kp, ws, s)
-                      : launch_by_index<f16_t, false>(a, kp, ws, s);
+      return a.twosum ? launch_by_index<f16_t, true>(a, kp, ws, s, side)
+                      : launch_by_index<f16_t, false>(a, kp, ws, s, side);
   }
   return hipErrorInvalidValue;
 }
