@@ -177,12 +177,13 @@ asc_hip_status panel_shape(asc_hip_context* ctx, const asc_hip_sparse_desc* a,
     return fail(ASC_HIP_INVALID_ARGUMENT, "unknown panel staging");
   const int64_t xbytes = a->element_kind == ASC_HIP_F64 ? 8 : 4;
   if (staging == ASC_HIP_STAGING_STREAM) {
-    // LDS-sized panels, at most 32768 columns (15-bit column field).
-    const int64_t max_cols = std::min<int64_t>(asc_hip::kLdsPanelBytes / xbytes,
+    // Panels of at most 128 KiB of x (the rest of the 160 KiB LDS holds the
+    // waves' segment rings) and 32768 columns (15-bit column field).
+    const int64_t max_cols = std::min<int64_t>(asc_hip::kStreamPanelBytes / xbytes,
                                                asc_hip::kStreamMaxPanelCols);
     if (want_cols > max_cols)
       return fail(ASC_HIP_INVALID_ARGUMENT,
-                  "a stream panel holds at most 160 KiB of x and 32768 columns");
+                  "a stream panel holds at most 128 KiB of x and 32768 columns");
     const int64_t w = want_cols > 0 ? want_cols : max_cols;
     int64_t panels = a->cols > w ? (a->cols - 1) / w + 1 : 1;
     panels = (panels + 7) / 8 * 8;

@@ -90,6 +90,11 @@ constexpr int kStreamEpl = 8;
 constexpr int kStreamTile = 64 * kStreamEpl;
 constexpr int kStreamSlice = 16 * kStreamTile;
 constexpr int kStreamWaves = 16;
+// Finished segment sums a wave keeps in LDS before writing them out, and the
+// LDS left for the x panel beside those rings.
+constexpr int kStreamRing = 256;
+constexpr int64_t kStreamPanelBytes =
+    kLdsPanelBytes - int64_t(kStreamWaves) * kStreamRing * int64_t(sizeof(double));
 constexpr uint16_t kStreamHead = 0x8000;
 constexpr uint32_t kStreamClosed = 0x80000000u;
 constexpr uint32_t kStreamNoSegment = 0xFFFFFFFFu;

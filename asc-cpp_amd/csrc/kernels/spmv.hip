@@ -520,12 +520,16 @@ struct StreamStep {
 
 // The step starting at entry t (a multiple of kStreamTile from the panel's
 // start): lane's eight columns and values.
+// The empty asm with a memory clobber keeps the compiler from sinking the
+// loads down to their first use (which would leave nothing in flight while
+// the previous step is reduced); it emits no instruction.
 template <typename V>
 __device__ __forceinline__ void stream_load(const uint16_t* __restrict__ scol,
                                             const V* __restrict__ sval, int64_t t, int lane,
                                             StreamStep<V>& s) {
   s.c = __builtin_nontemporal_load(reinterpret_cast<const u32x4_t*>(scol + t + kStreamEpl * lane));
   stream_values(sval + t, lane, s.v);
+  asm volatile("" ::: "memory");
 }
 
 // The running state of one wave's slice: the open segment's ordinal
@@ -537,10 +541,111 @@ struct StreamOpen {
   A sum;
 };
 
+// DPP lane moves (VALU, no LDS traffic): the value of the source lane the
+// control selects, or `old` where the source lies outside the row / wave or
+// the row is masked off.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_move(int v, int old) {
+  return __builtin_amdgcn_update_dpp(old, v, CTRL, ROWS, 0xF, false);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_move(float v, float old) {
+  return __int_as_float(dpp_move<CTRL, ROWS>(__float_as_int(v), __float_as_int(old)));
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_move(double v, double old) {
+  const int64_t b = __double_as_longlong(v), o = __double_as_longlong(old);
+  const int lo = dpp_move<CTRL, ROWS>(static_cast<int>(b), static_cast<int>(o));
+  const int hi = dpp_move<CTRL, ROWS>(static_cast<int>(b >> 32), static_cast<int>(o >> 32));
+  return __longlong_as_double((static_cast<int64_t>(hi) << 32) |
+                              static_cast<int64_t>(static_cast<uint32_t>(lo)));
+}
+// gfx9 DPP controls: row_shr:n, row_bcast:15, row_bcast:31, wave_shr:1, wave_shl:1.
+constexpr int kDppRowShr = 0x110;
+constexpr int kDppRowBcast15 = 0x142;
+constexpr int kDppRowBcast31 = 0x143;
+constexpr int kDppWaveShr1 = 0x138;
+constexpr int kDppWaveShl1 = 0x130;
+
+// One step of the segmented inclusive scan: the lane combines the (sum,
+// flag) pair the control brings from lower lanes, (0, 0) where none.
+template <int CTRL, int ROWS, typename A>
+__device__ __forceinline__ void seg_scan_step(A& s, int& f) {
+  const A ls = dpp_move<CTRL, ROWS>(s, A(0));
+  const int lf = dpp_move<CTRL, ROWS>(f, 0);
+  if (!f) s = ls + s;
+  f |= lf;
+}
+
+// Segmented inclusive scan over the 64 lanes: within rows of 16 (shifts 1, 2,
+// 4, 8), then across rows (lane 15 of each row into rows 1 and 3, lane 31
+// into rows 2 and 3).
+template <typename A>
+__device__ __forceinline__ void seg_scan_wave(A& s, int& f) {
+  seg_scan_step<kDppRowShr | 1, 0xF>(s, f);
+  seg_scan_step<kDppRowShr | 2, 0xF>(s, f);
+  seg_scan_step<kDppRowShr | 4, 0xF>(s, f);
+  seg_scan_step<kDppRowShr | 8, 0xF>(s, f);
+  seg_scan_step<kDppRowBcast15, 0xA>(s, f);
+  seg_scan_step<kDppRowBcast31, 0xC>(s, f);
+}
+
+__device__ __forceinline__ double read_lane(double v, int lane) {
+  const int64_t b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), lane);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), lane);
+  return __longlong_as_double((static_cast<int64_t>(hi) << 32) |
+                              static_cast<int64_t>(static_cast<uint32_t>(lo)));
+}
+__device__ __forceinline__ float read_lane(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// Waits for a step's loads to land.  Stores and loads share one counter on
+// this target, so a wait issued after the next step's loads would also wait
+// for those: the current step's data is claimed first, then the next step's
+// loads go out and stay in flight while the current step is reduced.
+template <typename V>
+__device__ __forceinline__ void stream_claim(const StreamStep<V>& s) {
+  asm volatile("" ::"v"(s.c.x), "v"(s.c.y), "v"(s.c.z), "v"(s.c.w));
+#pragma unroll
+  for (int j = 0; j < kStreamEpl; ++j) asm volatile("" ::"v"(s.v[j]));
+}
+
+// Finished segment sums wait in a per-wave LDS ring and leave for HBM as
+// contiguous bursts.  A store between a step's loads and their use would make
+// the next wait cover the store's round trip (loads and stores share one
+// counter here), so stores are batched: one burst per ring of segments
+// instead of a few stores every step.  The ring holds ordinals [base, base +
+// kStreamRing); [lo, open) are finished and not yet written.
+struct StreamRing {
+  uint32_t base;
+  uint32_t lo;
+};
+// An LDS pointer the compiler keeps in the LDS address space (ds_* accesses,
+// counted apart from the vector-memory stores).
+typedef __attribute__((address_space(3))) double lds_f64;
+
+// Writes the finished ordinals [lo, upto) and restarts the ring at lo.  Before
+// a slice's first finished segment `upto` may still lie below lo (a slice
+// opening on a head never writes the ordinal before it).
+__device__ __forceinline__ void stream_flush(lds_f64* ring, StreamRing& rg, uint32_t upto,
+                                             int lane, double* __restrict__ partial) {
+  const int32_t n = static_cast<int32_t>(upto - rg.lo);
+  if (n > 0) {
+    const uint32_t off = rg.lo - rg.base;
+    for (int32_t k = lane; k < n; k += kWave)
+      partial[static_cast<int64_t>(rg.lo) + k] = ring[off + k];
+    rg.lo = upto;
+  }
+  rg.base = rg.lo;
+}
+
 // One step: kStreamTile entries from t, kStreamEpl consecutive per lane.
 template <typename V, typename A>
 __device__ __forceinline__ void stream_step(const StreamStep<V>& cur, const A* sx, int64_t t,
                                             int64_t s0, int64_t s1, int lane, uint64_t below,
+                                            lds_f64* ring, StreamRing& rg,
                                             double* __restrict__ partial, StreamOpen<A>& st) {
   const int64_t e = t + kStreamEpl * lane;
   bool h[kStreamEpl];
@@ -553,51 +658,63 @@ __device__ __forceinline__ void stream_step(const StreamStep<V>& cur, const A* s
     const A prod = cur.v[j] * sx[c & 0x7FFFu];
     p[j] = valid ? prod : A(0);
   }
-  // A head on the step's first entry ends the segment open before it.
-  if (lane == 0 && h[0] && t > s0) partial[st.ord] = static_cast<double>(st.sum);
   // The lane's trailing run, then a segmented inclusive scan over lanes.
-  A u = A(0);
+  A s = A(0);
   bool f = false;
 #pragma unroll
   for (int j = 0; j < kStreamEpl; ++j) {
-    u = h[j] ? p[j] : u + p[j];
+    s = h[j] ? p[j] : s + p[j];
     f = f || h[j];
   }
-  A s = u;
   int fl = f ? 1 : 0;
-#pragma unroll
-  for (int off = 1; off < kWave; off <<= 1) {
-    const A su = __shfl_up(s, off);
-    const int fu = __shfl_up(fl, off);
-    if (lane >= off) {
-      if (!fl) s += su;
-      fl |= fu;
-    }
-  }
+  seg_scan_wave(s, fl);
   if (!fl) s += st.sum;
-  A run = __shfl_up(s, 1);
-  if (lane == 0) run = st.sum;
+  A run = dpp_move<kDppWaveShr1, 0xF>(s, st.sum);  // lane 0: the open segment
   int before = 0, total = 0;
+  uint64_t first_heads = 0;
 #pragma unroll
   for (int j = 0; j < kStreamEpl; ++j) {
     const uint64_t heads = __ballot(h[j]);
+    if (j == 0) first_heads = heads;
     before += __popcll(heads & below);
     total += __popcll(heads);
   }
+  // A slice that opens on a head leaves the ordinal before it to the
+  // previous slice.
+  if (t == s0) rg.lo = rg.base + static_cast<uint32_t>(first_heads & 1ull);
+  // This step finishes ordinals in [open, open + total): room in the ring,
+  // or (a step of more than a ring of segments) straight to HBM.
+  const uint32_t end = st.ord + static_cast<uint32_t>(total);
+  bool direct = false;
+  if (end - rg.base > static_cast<uint32_t>(kStreamRing)) {
+    stream_flush(ring, rg, st.ord, lane, partial);
+    direct = total > kStreamRing;
+  }
+  // A head on the step's first entry ends the segment open before it.
+  if (lane == 0 && h[0] && t > s0) {
+    if (direct) partial[st.ord] = static_cast<double>(st.sum);
+    else ring[st.ord - rg.base] = static_cast<double>(st.sum);
+  }
   // An entry ends its segment when the next entry is a head: the next lane's
   // first entry for a lane's last, the next step for lane 63's.
-  const int next_head = __shfl_down(h[0] ? 1 : 0, 1);
+  const int next_head = dpp_move<kDppWaveShl1, 0xF>(h[0] ? 1 : 0, 0);
   uint32_t ord = st.ord + static_cast<uint32_t>(before);
 #pragma unroll
   for (int j = 0; j < kStreamEpl; ++j) {
     run = h[j] ? p[j] : run + p[j];
     ord += h[j] ? 1u : 0u;
-    const bool ends = j + 1 < kStreamEpl ? h[(j + 1) % kStreamEpl]
-                                         : (lane + 1 < kWave && next_head != 0);
-    if (ends) partial[ord] = static_cast<double>(run);
+    const bool ends = j + 1 < kStreamEpl ? h[(j + 1) % kStreamEpl] : next_head != 0;
+    if (ends) {
+      if (direct) partial[ord] = static_cast<double>(run);
+      else ring[ord - rg.base] = static_cast<double>(run);
+    }
   }
-  st.sum = __shfl(s, kWave - 1);
-  st.ord += static_cast<uint32_t>(total);
+  st.sum = read_lane(s, kWave - 1);
+  st.ord = end;
+  if (direct) {
+    rg.base = end;
+    rg.lo = end;
+  }
 }
 
 template <typename V>
@@ -607,7 +724,8 @@ __global__ __launch_bounds__(kLdsBlock) void spmv_stream_kernel(
     const uint32_t* __restrict__ slice_info, const uint32_t* __restrict__ wg_table,
     double* __restrict__ partial, double* __restrict__ carry) {
   using A = typename acc_of<V>::type;
-  __shared__ A sx[kLdsPanelBytes / sizeof(A)];
+  __shared__ A sx[kStreamPanelBytes / sizeof(A)];
+  __shared__ double sring[kStreamWaves * kStreamRing];
   const uint32_t first = wg_table[2 * blockIdx.x];
   const uint32_t count = wg_table[2 * blockIdx.x + 1];
   const int64_t q = slice_info[4 * static_cast<int64_t>(first) + 3] & ~kStreamClosed;
@@ -617,15 +735,11 @@ __global__ __launch_bounds__(kLdsBlock) void spmv_stream_kernel(
   const int64_t slice = static_cast<int64_t>(first) + (active ? wave : 0);
   const uint32_t* si = slice_info + 4 * slice;
   const int64_t s0 = si[0], s1 = si[1];
-  // The slice's first two steps load while the panel of x is staged.
-  // Slices end inside their panel's padded range, so a step's loads stay in
-  // the allocation even past the slice's end.
+  // The slice's first step loads while the panel of x is staged.  Slices end
+  // inside their panel's padded range, so a step's loads stay in the
+  // allocation even past the slice's end.
   StreamStep<V> b0, b1;
-  StreamStep<V> b2;  // the rotation's third buffer (storage narrower than FP64)
-  if (active) {
-    stream_load(scol, sval, s0, lane, b0);
-    if (s0 + kStreamTile < s1) stream_load(scol, sval, s0 + kStreamTile, lane, b1);
-  }
+  if (active) stream_load(scol, sval, s0, lane, b0);
   const int64_t col0 = q * panel_cols;
   const int64_t width = cols - col0 < panel_cols ? cols - col0 : panel_cols;
   const A* __restrict__ xp = x + col0 * incx;
@@ -645,37 +759,27 @@ __global__ __launch_bounds__(kLdsBlock) void spmv_stream_kernel(
   if (!active) return;  // whole waves leave together
   const bool closed = (si[3] & kStreamClosed) != 0;
   const uint64_t below = lane == 0 ? 0ull : (~0ull >> (kWave - lane));
+  lds_f64* ring = (lds_f64*)(sring + wave * kStreamRing);  // address-space cast
   StreamOpen<A> st;
   st.ord = si[2];
   st.sum = A(0);
+  StreamRing rg;
+  rg.base = si[2];
+  rg.lo = si[2];
+  // One step in flight while the current one is reduced; the two buffers
+  // alternate.
   int64_t t = s0;
-  if constexpr (sizeof(V) < 8) {
-    // Two steps in flight: the three buffers rotate roles.
-    constexpr int64_t ahead = 2 * kStreamTile;
-    while (true) {
-      if (t + 2 * kStreamTile < s1) stream_load(scol, sval, t + ahead, lane, b2);
-      stream_step<V, A>(b0, sx, t, s0, s1, lane, below, partial, st);
-      if ((t += kStreamTile) >= s1) break;
-      if (t + 2 * kStreamTile < s1) stream_load(scol, sval, t + ahead, lane, b0);
-      stream_step<V, A>(b1, sx, t, s0, s1, lane, below, partial, st);
-      if ((t += kStreamTile) >= s1) break;
-      if (t + 2 * kStreamTile < s1) stream_load(scol, sval, t + ahead, lane, b1);
-      stream_step<V, A>(b2, sx, t, s0, s1, lane, below, partial, st);
-      if ((t += kStreamTile) >= s1) break;
-    }
-  } else {
-    // FP64 steps hold twice the registers: one step in flight (b1 was loaded
-    // before the barrier), two buffers alternating.
-    constexpr int64_t ahead = kStreamTile;
-    while (true) {
-      stream_step<V, A>(b0, sx, t, s0, s1, lane, below, partial, st);
-      if ((t += kStreamTile) >= s1) break;
-      if (t + kStreamTile < s1) stream_load(scol, sval, t + ahead, lane, b0);
-      stream_step<V, A>(b1, sx, t, s0, s1, lane, below, partial, st);
-      if ((t += kStreamTile) >= s1) break;
-      if (t + kStreamTile < s1) stream_load(scol, sval, t + ahead, lane, b1);
-    }
+  while (true) {
+    stream_claim(b0);
+    if (t + kStreamTile < s1) stream_load(scol, sval, t + kStreamTile, lane, b1);
+    stream_step<V, A>(b0, sx, t, s0, s1, lane, below, ring, rg, partial, st);
+    if ((t += kStreamTile) >= s1) break;
+    stream_claim(b1);
+    if (t + kStreamTile < s1) stream_load(scol, sval, t + kStreamTile, lane, b0);
+    stream_step<V, A>(b1, sx, t, s0, s1, lane, below, ring, rg, partial, st);
+    if ((t += kStreamTile) >= s1) break;
   }
+  stream_flush(ring, rg, st.ord, lane, partial);
   if (lane == 0) {
     if (closed) partial[st.ord] = static_cast<double>(st.sum);
     else carry[slice] = static_cast<double>(st.sum);

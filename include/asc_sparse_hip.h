@@ -344,7 +344,7 @@ enum {
   /* Stream: the analysis copies the listed rows' entries panel-major (a
    * 16-bit column within the panel and the value per entry) into the
    * analysis buffer, and the SpMV streams that copy contiguously, gathering x
-   * from an LDS copy of each panel (at most 160 KiB of x and 32768 columns
+   * from an LDS copy of each panel (at most 128 KiB of x and 32768 columns
    * per panel).  The copy holds A's values as they were at analysis: after
    * the values change (in place, or through asc_sparse_hip_rebind_values) the
    * matrix needs a new analysis; a SpMV whose values pointer differs from the

@@ -354,7 +354,9 @@ def _stream_cases():
     return {
         "zipf_2e16": (gm(lambda: G.make(G.ZIPF, 1 << 16, 1 << 16, seed=23, values=G.VALUES_NORMAL)),
                       [(0, 0), (4099, 8), (1 << 13, 1), (300, 64)]),
-        "uniform16_4096": (gm(lambda: G.uniform16(4096)), [(0, 0), (300, 16), (1000, 17)]),
+        # width 4: 1024 panels, one-entry segments (a step finishes more
+        # segments than a wave's ring holds and writes them directly)
+        "uniform16_4096": (gm(lambda: G.uniform16(4096)), [(0, 0), (300, 16), (1000, 17), (4, 1)]),
         "banded_3e4": (gm(lambda: G.make(G.BANDED, 30000, 30000, seed=6, values=G.VALUES_UNIFORM,
                                          band=40, fill=0.5)), [(2048, 4), (37, 30)]),
         "dense_rows": (dense_rows, [(0, 0), (0, 1), (5000, 100)]),
