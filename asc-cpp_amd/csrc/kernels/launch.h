@@ -93,6 +93,7 @@ constexpr int kStreamWaves = 16;
 // Finished segment sums a wave keeps in LDS before writing them out, and the
 // LDS left for the x panel beside those rings.
 constexpr int kStreamRing = 256;
+constexpr int kStreamRingUse = kStreamRing - 64;  // the rest: one spare slot per lane
 constexpr int64_t kStreamPanelBytes =
     kLdsPanelBytes - int64_t(kStreamWaves) * kStreamRing * int64_t(sizeof(double));
 constexpr uint16_t kStreamHead = 0x8000;

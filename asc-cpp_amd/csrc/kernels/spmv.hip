@@ -641,80 +641,108 @@ __device__ __forceinline__ void stream_flush(lds_f64* ring, StreamRing& rg, uint
   rg.base = rg.lo;
 }
 
-// One step: kStreamTile entries from t, kStreamEpl consecutive per lane.
-template <typename V, typename A>
+// Inclusive scan of the lanes' head counts (DPP, VALU only).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ void count_step(int& n) {
+  n += dpp_move<CTRL, ROWS>(n, 0);
+}
+__device__ __forceinline__ void count_wave(int& n) {
+  count_step<kDppRowShr | 1, 0xF>(n);
+  count_step<kDppRowShr | 2, 0xF>(n);
+  count_step<kDppRowShr | 4, 0xF>(n);
+  count_step<kDppRowShr | 8, 0xF>(n);
+  count_step<kDppRowBcast15, 0xA>(n);
+  count_step<kDppRowBcast31, 0xC>(n);
+}
+
+// One step: kStreamTile entries from t, kStreamEpl consecutive per lane
+// (FULL: all of them inside the slice).  Segment ordinals come first, from
+// a scan of the lanes' head counts; one pass over the lane's entries then
+// finishes every segment that opens and closes inside the lane, and after a
+// segmented scan of the lanes' trailing runs each lane finishes at most two
+// more: the run before its first head (plus what the lanes to its left
+// carry) and its trailing run when the next lane opens on a head.  Finished
+// sums go to the wave's ring without branches: a lane with nothing to write
+// writes its spare slot past the ring's kStreamRingUse live slots.
+template <bool FULL, typename V, typename A>
 __device__ __forceinline__ void stream_step(const StreamStep<V>& cur, const A* sx, int64_t t,
-                                            int64_t s0, int64_t s1, int lane, uint64_t below,
+                                            int64_t s0, int64_t s1, int lane,
                                             lds_f64* ring, StreamRing& rg,
                                             double* __restrict__ partial, StreamOpen<A>& st) {
   const int64_t e = t + kStreamEpl * lane;
-  bool h[kStreamEpl];
-  A p[kStreamEpl];
+  uint32_t hmask = 0;  // bit j: entry j opens a segment
 #pragma unroll
   for (int j = 0; j < kStreamEpl; ++j) {
     const uint32_t c = (word_of(cur.c, j / 2) >> (16 * (j & 1))) & 0xFFFFu;
-    const bool valid = e + j < s1;
-    h[j] = valid && (c & kStreamHead) != 0;
-    const A prod = cur.v[j] * sx[c & 0x7FFFu];
-    p[j] = valid ? prod : A(0);
+    bool hj = (c & kStreamHead) != 0;
+    if constexpr (!FULL) hj = hj && e + j < s1;
+    hmask |= hj ? 1u << j : 0u;
   }
-  // The lane's trailing run, then a segmented inclusive scan over lanes.
-  A s = A(0);
-  bool f = false;
-#pragma unroll
-  for (int j = 0; j < kStreamEpl; ++j) {
-    s = h[j] ? p[j] : s + p[j];
-    f = f || h[j];
-  }
-  int fl = f ? 1 : 0;
-  seg_scan_wave(s, fl);
-  if (!fl) s += st.sum;
-  A run = dpp_move<kDppWaveShr1, 0xF>(s, st.sum);  // lane 0: the open segment
-  int before = 0, total = 0;
-  uint64_t first_heads = 0;
-#pragma unroll
-  for (int j = 0; j < kStreamEpl; ++j) {
-    const uint64_t heads = __ballot(h[j]);
-    if (j == 0) first_heads = heads;
-    before += __popcll(heads & below);
-    total += __popcll(heads);
-  }
+  const int own = __builtin_popcount(hmask);
+  int incl = own;
+  count_wave(incl);
+  const uint32_t total = static_cast<uint32_t>(__builtin_amdgcn_readlane(incl, kWave - 1));
+  const uint32_t o0 = st.ord + static_cast<uint32_t>(incl - own);  // open at the lane's start
+  const bool h0 = (hmask & 1u) != 0;
   // A slice that opens on a head leaves the ordinal before it to the
   // previous slice.
-  if (t == s0) rg.lo = rg.base + static_cast<uint32_t>(first_heads & 1ull);
+  if (t == s0) rg.lo = rg.base + static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(h0 ? 1 : 0));
   // This step finishes ordinals in [open, open + total): room in the ring,
-  // or (a step of more than a ring of segments) straight to HBM.
-  const uint32_t end = st.ord + static_cast<uint32_t>(total);
+  // or (a step of more segments than the ring holds) straight to HBM.
+  const uint32_t end = st.ord + total;
   bool direct = false;
-  if (end - rg.base > static_cast<uint32_t>(kStreamRing)) {
+  if (end - rg.base > static_cast<uint32_t>(kStreamRingUse)) {
     stream_flush(ring, rg, st.ord, lane, partial);
-    direct = total > kStreamRing;
+    direct = total > static_cast<uint32_t>(kStreamRingUse);
   }
-  // A head on the step's first entry ends the segment open before it.
-  if (lane == 0 && h[0] && t > s0) {
-    if (direct) partial[st.ord] = static_cast<double>(st.sum);
-    else ring[st.ord - rg.base] = static_cast<double>(st.sum);
-  }
-  // An entry ends its segment when the next entry is a head: the next lane's
-  // first entry for a lane's last, the next step for lane 63's.
-  const int next_head = dpp_move<kDppWaveShl1, 0xF>(h[0] ? 1 : 0, 0);
-  uint32_t ord = st.ord + static_cast<uint32_t>(before);
+  const uint32_t spare = kStreamRingUse + lane;
+  // The lane's pass: run s, the run before the first head (lead), segments
+  // closed inside the lane.
+  A s = A(0), lead = A(0);
+  int seen = 0;
 #pragma unroll
   for (int j = 0; j < kStreamEpl; ++j) {
-    run = h[j] ? p[j] : run + p[j];
-    ord += h[j] ? 1u : 0u;
-    const bool ends = j + 1 < kStreamEpl ? h[(j + 1) % kStreamEpl] : next_head != 0;
-    if (ends) {
-      if (direct) partial[ord] = static_cast<double>(run);
-      else ring[ord - rg.base] = static_cast<double>(run);
+    const uint32_t c = (word_of(cur.c, j / 2) >> (16 * (j & 1))) & 0x7FFFu;
+    A prod = cur.v[j] * sx[c];
+    if constexpr (!FULL) prod = e + j < s1 ? prod : A(0);
+    const bool hj = (hmask >> j) & 1u;
+    const bool closes = hj && seen > 0;
+    if (!direct) {
+      ring[closes ? o0 + seen - rg.base : spare] = static_cast<double>(s);
+    } else if (closes) {
+      partial[o0 + seen] = static_cast<double>(s);
     }
+    lead = hj && seen == 0 ? s : lead;
+    s = hj ? prod : s + prod;
+    seen += hj ? 1 : 0;
   }
-  st.sum = read_lane(s, kWave - 1);
-  st.ord = end;
-  if (direct) {
+  if (own == 0) lead = s;
+  // Segmented scan of the trailing runs: what reaches each lane from the left.
+  A sc = s;
+  int fl = own > 0 ? 1 : 0;
+  seg_scan_wave(sc, fl);
+  if (!fl) sc += st.sum;
+  const A from_left = dpp_move<kDppWaveShr1, 0xF>(sc, st.sum);  // lane 0: the open segment
+  const int next_h0 = dpp_move<kDppWaveShl1, 0xF>(h0 ? 1 : 0, 0);
+  // The run before the first head closes there (a head on entry 0 closes the
+  // segment the lane to the left carried, written by that lane, or for lane 0
+  // the open segment of the previous step).
+  const bool lead_closes = own > 0 && (!h0 || (lane == 0 && t > s0));
+  const A lead_sum = from_left + lead;
+  // The trailing run closes when the next lane opens on a head.
+  const bool tail_closes = next_h0 != 0;
+  const A tail_sum = own > 0 ? s : lead_sum;
+  if (!direct) {
+    ring[lead_closes ? o0 - rg.base : spare] = static_cast<double>(lead_sum);
+    ring[tail_closes ? o0 + own - rg.base : spare] = static_cast<double>(tail_sum);
+  } else {
+    if (lead_closes) partial[o0] = static_cast<double>(lead_sum);
+    if (tail_closes) partial[o0 + own] = static_cast<double>(tail_sum);
     rg.base = end;
     rg.lo = end;
   }
+  st.sum = read_lane(sc, kWave - 1);
+  st.ord = end;
 }
 
 template <typename V>
@@ -738,8 +766,12 @@ __global__ __launch_bounds__(kLdsBlock) void spmv_stream_kernel(
   // The slice's first step loads while the panel of x is staged.  Slices end
   // inside their panel's padded range, so a step's loads stay in the
   // allocation even past the slice's end.
-  StreamStep<V> b0, b1;
-  if (active) stream_load(scol, sval, s0, lane, b0);
+  StreamStep<V> b0, b1, b2;
+  if (active) {
+    stream_load(scol, sval, s0, lane, b0);
+    if (sizeof(V) < 8 && s0 + kStreamTile < s1)
+      stream_load(scol, sval, s0 + kStreamTile, lane, b1);
+  }
   const int64_t col0 = q * panel_cols;
   const int64_t width = cols - col0 < panel_cols ? cols - col0 : panel_cols;
   const A* __restrict__ xp = x + col0 * incx;
@@ -758,7 +790,6 @@ __global__ __launch_bounds__(kLdsBlock) void spmv_stream_kernel(
   __syncthreads();
   if (!active) return;  // whole waves leave together
   const bool closed = (si[3] & kStreamClosed) != 0;
-  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (kWave - lane));
   lds_f64* ring = (lds_f64*)(sring + wave * kStreamRing);  // address-space cast
   StreamOpen<A> st;
   st.ord = si[2];
@@ -766,19 +797,52 @@ __global__ __launch_bounds__(kLdsBlock) void spmv_stream_kernel(
   StreamRing rg;
   rg.base = si[2];
   rg.lo = si[2];
-  // One step in flight while the current one is reduced; the two buffers
-  // alternate.
+  // Full steps with the next one (or two) in flight; the last, partial step
+  // of the slice after the loop.
   int64_t t = s0;
-  while (true) {
-    stream_claim(b0);
-    if (t + kStreamTile < s1) stream_load(scol, sval, t + kStreamTile, lane, b1);
-    stream_step<V, A>(b0, sx, t, s0, s1, lane, below, ring, rg, partial, st);
-    if ((t += kStreamTile) >= s1) break;
-    stream_claim(b1);
-    if (t + kStreamTile < s1) stream_load(scol, sval, t + kStreamTile, lane, b0);
-    stream_step<V, A>(b1, sx, t, s0, s1, lane, below, ring, rg, partial, st);
-    if ((t += kStreamTile) >= s1) break;
+  if constexpr (sizeof(V) == 8) {
+    // FP64 steps hold twice the registers: one step in flight, two buffers.
+    while (t + kStreamTile < s1) {
+      stream_claim(b0);
+      stream_load(scol, sval, t + kStreamTile, lane, b1);
+      stream_step<true, V, A>(b0, sx, t, s0, s1, lane, ring, rg, partial, st);
+      t += kStreamTile;
+      if (t + kStreamTile >= s1) {
+        b0 = b1;
+        break;
+      }
+      stream_claim(b1);
+      stream_load(scol, sval, t + kStreamTile, lane, b0);
+      stream_step<true, V, A>(b1, sx, t, s0, s1, lane, ring, rg, partial, st);
+      t += kStreamTile;
+    }
+  } else {
+    // Two steps in flight; the three buffers rotate roles.
+    while (t + kStreamTile < s1) {
+      stream_claim(b0);
+      if (t + 2 * kStreamTile < s1) stream_load(scol, sval, t + 2 * kStreamTile, lane, b2);
+      stream_step<true, V, A>(b0, sx, t, s0, s1, lane, ring, rg, partial, st);
+      t += kStreamTile;
+      if (t + kStreamTile >= s1) {
+        b0 = b1;
+        break;
+      }
+      stream_claim(b1);
+      if (t + 2 * kStreamTile < s1) stream_load(scol, sval, t + 2 * kStreamTile, lane, b0);
+      stream_step<true, V, A>(b1, sx, t, s0, s1, lane, ring, rg, partial, st);
+      t += kStreamTile;
+      if (t + kStreamTile >= s1) {
+        b0 = b2;
+        break;
+      }
+      stream_claim(b2);
+      if (t + 2 * kStreamTile < s1) stream_load(scol, sval, t + 2 * kStreamTile, lane, b1);
+      stream_step<true, V, A>(b2, sx, t, s0, s1, lane, ring, rg, partial, st);
+      t += kStreamTile;
+    }
   }
+  stream_claim(b0);
+  stream_step<false, V, A>(b0, sx, t, s0, s1, lane, ring, rg, partial, st);
   stream_flush(ring, rg, st.ord, lane, partial);
   if (lane == 0) {
     if (closed) partial[st.ord] = static_cast<double>(st.sum);
@@ -815,10 +879,23 @@ __global__ __launch_bounds__(kBlock) void spmv_stream_combine_kernel(
   using Acc = typename AccSel<V, false>::type;
   const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (i >= nlong) return;
+  // Panels in blocks of 8 (panel counts are multiples of 8): a block's sums
+  // load together while the next block's map entries are in flight.
+  constexpr int kB = 8;
+  uint32_t m[kB];
+#pragma unroll
+  for (int k = 0; k < kB; ++k) m[k] = seg_map[k * nlong + i];
   double s = 0.0;
-  for (int64_t q = 0; q < panels; ++q) {
-    const uint32_t o = seg_map[q * nlong + i];
-    if (o != kStreamNoSegment) s += partial[o];
+  for (int64_t q0 = 0; q0 < panels; q0 += kB) {
+    double v[kB];
+#pragma unroll
+    for (int k = 0; k < kB; ++k) v[k] = m[k] != kStreamNoSegment ? partial[m[k]] : 0.0;
+    if (q0 + kB < panels) {
+#pragma unroll
+      for (int k = 0; k < kB; ++k) m[k] = seg_map[(q0 + kB + k) * nlong + i];
+    }
+#pragma unroll
+    for (int k = 0; k < kB; ++k) s += v[k];
   }
   finish_row<V>(p, long_rows[i], Acc::from_f64(s));
 }
