@@ -90,12 +90,13 @@ constexpr int kStreamEpl = 8;
 constexpr int kStreamTile = 64 * kStreamEpl;
 constexpr int kStreamSlice = 16 * kStreamTile;
 constexpr int kStreamWaves = 16;
-// Finished segment sums a wave keeps in LDS before writing them out, and the
-// LDS left for the x panel beside those rings.
-constexpr int kStreamRing = 256;
-constexpr int kStreamRingUse = kStreamRing - 64;  // the rest: one spare slot per lane
-constexpr int64_t kStreamPanelBytes =
-    kLdsPanelBytes - int64_t(kStreamWaves) * kStreamRing * int64_t(sizeof(double));
+// Segment sums a wave keeps in LDS before writing them out (a power of two),
+// one spare slot per lane beside them, and the LDS left for the x panel.
+constexpr int kStreamRing = 128;
+constexpr int64_t kStreamPanelBytes = int64_t(128) << 10;
+static_assert(kStreamPanelBytes + int64_t(kStreamWaves) * (kStreamRing + 64) * 8 <=
+                  kLdsPanelBytes,
+              "x panel and segment rings share one work-group's LDS");
 constexpr uint16_t kStreamHead = 0x8000;
 constexpr uint32_t kStreamClosed = 0x80000000u;
 constexpr uint32_t kStreamNoSegment = 0xFFFFFFFFu;

@@ -532,15 +532,6 @@ __device__ __forceinline__ void stream_load(const uint16_t* __restrict__ scol,
   asm volatile("" ::: "memory");
 }
 
-// The running state of one wave's slice: the open segment's ordinal
-// (segments are numbered panel-major; before the slice's first head it may
-// wrap below zero) and its sum over this slice's entries so far.
-template <typename A>
-struct StreamOpen {
-  uint32_t ord;
-  A sum;
-};
-
 // DPP lane moves (VALU, no LDS traffic): the value of the source lane the
 // control selects, or `old` where the source lies outside the row / wave or
 // the row is masked off.
@@ -567,27 +558,65 @@ constexpr int kDppRowBcast31 = 0x143;
 constexpr int kDppWaveShr1 = 0x138;
 constexpr int kDppWaveShl1 = 0x130;
 
-// One step of the segmented inclusive scan: the lane combines the (sum,
-// flag) pair the control brings from lower lanes, (0, 0) where none.
-template <int CTRL, int ROWS, typename A>
-__device__ __forceinline__ void seg_scan_step(A& s, int& f) {
-  const A ls = dpp_move<CTRL, ROWS>(s, A(0));
-  const int lf = dpp_move<CTRL, ROWS>(f, 0);
-  if (!f) s = ls + s;
-  f |= lf;
+// Branch-free segmented scan.  Each lane carries its sum s and a pass-through
+// factor m (1 while no head is at or before it in the scanned range, else 0),
+// kept as a value whose high word is the only nonzero part (1.0 or 0.0).  A
+// step folds in the left neighbour's pair: s += m * s_left, m = m * m_left.
+// Row shifts read zeros past the row start (bound control); the row
+// broadcasts leave masked-off rows with the identity (0, 1).
+constexpr uint32_t kOneHi64 = 0x3FF00000u;  // high word of 1.0 (FP64)
+constexpr uint32_t kOne32 = 0x3F800000u;    // 1.0f
+
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_f64_or_zero(double v) {
+  const int64_t b = __double_as_longlong(v);
+  int lo, hi;
+  if constexpr (ROWS == 0xF) {
+    lo = __builtin_amdgcn_mov_dpp(static_cast<int>(b), CTRL, 0xF, 0xF, true);
+    hi = __builtin_amdgcn_mov_dpp(static_cast<int>(b >> 32), CTRL, 0xF, 0xF, true);
+  } else {
+    lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(b), CTRL, ROWS, 0xF, false);
+    hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), CTRL, ROWS, 0xF, false);
+  }
+  return __longlong_as_double((static_cast<int64_t>(hi) << 32) |
+                              static_cast<int64_t>(static_cast<uint32_t>(lo)));
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_f64_or_zero(float v) {  // FP32 sums
+  if constexpr (ROWS == 0xF)
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
+  else
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xF, false));
 }
 
-// Segmented inclusive scan over the 64 lanes: within rows of 16 (shifts 1, 2,
-// 4, 8), then across rows (lane 15 of each row into rows 1 and 3, lane 31
-// into rows 2 and 3).
+template <typename A> struct PassMask;
+template <> struct PassMask<double> {
+  static constexpr uint32_t kOne = kOneHi64;
+  __device__ static double value(uint32_t w) {
+    return __longlong_as_double(static_cast<int64_t>(w) << 32);
+  }
+};
+template <> struct PassMask<float> {
+  static constexpr uint32_t kOne = kOne32;
+  __device__ static float value(uint32_t w) { return __uint_as_float(w); }
+};
+
+template <int CTRL, int ROWS, typename A>
+__device__ __forceinline__ void pass_scan_step(A& s, uint32_t& m) {
+  const A ls = dpp_f64_or_zero<CTRL, ROWS>(s);
+  const uint32_t lm = static_cast<uint32_t>(__builtin_amdgcn_update_dpp(
+      static_cast<int>(PassMask<A>::kOne), static_cast<int>(m), CTRL, ROWS, 0xF, false));
+  s = __builtin_fma(ls, PassMask<A>::value(m), s);
+  m &= lm;
+}
 template <typename A>
-__device__ __forceinline__ void seg_scan_wave(A& s, int& f) {
-  seg_scan_step<kDppRowShr | 1, 0xF>(s, f);
-  seg_scan_step<kDppRowShr | 2, 0xF>(s, f);
-  seg_scan_step<kDppRowShr | 4, 0xF>(s, f);
-  seg_scan_step<kDppRowShr | 8, 0xF>(s, f);
-  seg_scan_step<kDppRowBcast15, 0xA>(s, f);
-  seg_scan_step<kDppRowBcast31, 0xC>(s, f);
+__device__ __forceinline__ void pass_scan_wave(A& s, uint32_t& m) {
+  pass_scan_step<kDppRowShr | 1, 0xF>(s, m);
+  pass_scan_step<kDppRowShr | 2, 0xF>(s, m);
+  pass_scan_step<kDppRowShr | 4, 0xF>(s, m);
+  pass_scan_step<kDppRowShr | 8, 0xF>(s, m);
+  pass_scan_step<kDppRowBcast15, 0xA>(s, m);
+  pass_scan_step<kDppRowBcast31, 0xC>(s, m);
 }
 
 __device__ __forceinline__ double read_lane(double v, int lane) {
@@ -612,35 +641,6 @@ __device__ __forceinline__ void stream_claim(const StreamStep<V>& s) {
   for (int j = 0; j < kStreamEpl; ++j) asm volatile("" ::"v"(s.v[j]));
 }
 
-// Finished segment sums wait in a per-wave LDS ring and leave for HBM as
-// contiguous bursts.  A store between a step's loads and their use would make
-// the next wait cover the store's round trip (loads and stores share one
-// counter here), so stores are batched: one burst per ring of segments
-// instead of a few stores every step.  The ring holds ordinals [base, base +
-// kStreamRing); [lo, open) are finished and not yet written.
-struct StreamRing {
-  uint32_t base;
-  uint32_t lo;
-};
-// An LDS pointer the compiler keeps in the LDS address space (ds_* accesses,
-// counted apart from the vector-memory stores).
-typedef __attribute__((address_space(3))) double lds_f64;
-
-// Writes the finished ordinals [lo, upto) and restarts the ring at lo.  Before
-// a slice's first finished segment `upto` may still lie below lo (a slice
-// opening on a head never writes the ordinal before it).
-__device__ __forceinline__ void stream_flush(lds_f64* ring, StreamRing& rg, uint32_t upto,
-                                             int lane, double* __restrict__ partial) {
-  const int32_t n = static_cast<int32_t>(upto - rg.lo);
-  if (n > 0) {
-    const uint32_t off = rg.lo - rg.base;
-    for (int32_t k = lane; k < n; k += kWave)
-      partial[static_cast<int64_t>(rg.lo) + k] = ring[off + k];
-    rg.lo = upto;
-  }
-  rg.base = rg.lo;
-}
-
 // Inclusive scan of the lanes' head counts (DPP, VALU only).
 template <int CTRL, int ROWS>
 __device__ __forceinline__ void count_step(int& n) {
@@ -655,20 +655,60 @@ __device__ __forceinline__ void count_wave(int& n) {
   count_step<kDppRowBcast31, 0xC>(n);
 }
 
+// Segment sums accumulate in a per-wave circular LDS ring of kStreamRing
+// slots (slot = ordinal mod kStreamRing) and leave for HBM as contiguous
+// bursts, so the stream's loads never wait behind scattered stores.  Ordinals
+// [lo, open) are finished and not yet written; `open` is still accumulating.
+typedef __attribute__((address_space(3))) double lds_f64;
+
+__device__ __forceinline__ uint32_t ring_slot(uint32_t ordinal) {
+  return ordinal & (kStreamRing - 1u);
+}
+
+// Writes the finished ordinals [lo, upto) and advances lo.  Before a slice's
+// first finished segment `upto` may still lie below lo (a slice opening on a
+// head never writes the ordinal before it).
+__device__ __forceinline__ void stream_flush(lds_f64* ring, uint32_t& lo, uint32_t upto,
+                                             int lane, double* __restrict__ partial) {
+  const int32_t n = static_cast<int32_t>(upto - lo);
+  if (n > 0) {
+    for (int32_t k = lane; k < n; k += kWave)
+      __builtin_nontemporal_store(static_cast<double>(ring[ring_slot(lo + k)]),
+                                  partial + static_cast<int64_t>(lo) + k);
+    lo = upto;
+  }
+}
+
 // One step: kStreamTile entries from t, kStreamEpl consecutive per lane
-// (FULL: all of them inside the slice).  Segment ordinals come first, from
-// a scan of the lanes' head counts; one pass over the lane's entries then
-// finishes every segment that opens and closes inside the lane, and after a
-// segmented scan of the lanes' trailing runs each lane finishes at most two
-// more: the run before its first head (plus what the lanes to its left
-// carry) and its trailing run when the next lane opens on a head.  Finished
-// sums go to the wave's ring without branches: a lane with nothing to write
-// writes its spare slot past the ring's kStreamRingUse live slots.
-template <bool FULL, typename V, typename A>
-__device__ __forceinline__ void stream_step(const StreamStep<V>& cur, const A* sx, int64_t t,
-                                            int64_t s0, int64_t s1, int lane,
-                                            lds_f64* ring, StreamRing& rg,
-                                            double* __restrict__ partial, StreamOpen<A>& st) {
+// (FULL: all of them inside the slice).  Ordinals come from a scan of the
+// lanes' head counts: o0 is the segment open at the lane's start, and the
+// lane's k-th head opens o0 + k.  Each lane then walks its entries once:
+//   - a segment that opens and closes inside the lane is written to its slot;
+//   - the run after the lane's last head is written to that segment's slot
+//     (its first contribution);
+//   - the run before the first head (or the whole lane, without heads) is
+//     added to the slot of o0 with an LDS atomic, after all writes.
+// The open segment keeps accumulating in its slot from step to step.  A step
+// that finishes at least a ring of segments (about 4 entries per segment or
+// fewer) writes them to HBM directly instead, with a segmented scan of the
+// lanes' runs.
+struct StepHeads {
+  uint32_t hmask;  // bit j: entry j opens a segment
+  int own;         // heads in the lane
+  uint32_t o0;     // segment open at the lane's start
+  uint32_t end;    // segment open after the step
+  bool direct;     // the step writes its segments straight to HBM
+};
+
+// The step's heads and ordinals, and room in the ring (flushing finished
+// segments before the next step's loads are issued, so the stores drain
+// while those loads are in flight).
+template <bool FULL, typename V>
+__device__ __forceinline__ StepHeads stream_heads(const StreamStep<V>& cur, int64_t t,
+                                                  int64_t s0, int64_t s1, int lane,
+                                                  lds_f64* ring, uint32_t& lo,
+                                                  double* __restrict__ partial,
+                                                  uint32_t open) {
   const int64_t e = t + kStreamEpl * lane;
   uint32_t hmask = 0;  // bit j: entry j opens a segment
 #pragma unroll
@@ -682,67 +722,89 @@ __device__ __forceinline__ void stream_step(const StreamStep<V>& cur, const A* s
   int incl = own;
   count_wave(incl);
   const uint32_t total = static_cast<uint32_t>(__builtin_amdgcn_readlane(incl, kWave - 1));
-  const uint32_t o0 = st.ord + static_cast<uint32_t>(incl - own);  // open at the lane's start
+  const uint32_t o0 = open + static_cast<uint32_t>(incl - own);
   const bool h0 = (hmask & 1u) != 0;
   // A slice that opens on a head leaves the ordinal before it to the
   // previous slice.
-  if (t == s0) rg.lo = rg.base + static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(h0 ? 1 : 0));
-  // This step finishes ordinals in [open, open + total): room in the ring,
-  // or (a step of more segments than the ring holds) straight to HBM.
-  const uint32_t end = st.ord + total;
+  if (t == s0) lo = open + static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(h0 ? 1 : 0));
+  // Live ordinals [lo, end] share the ring: flush the finished ones when the
+  // step would wrap onto them.
+  const uint32_t end = open + total;
   bool direct = false;
-  if (end - rg.base > static_cast<uint32_t>(kStreamRingUse)) {
-    stream_flush(ring, rg, st.ord, lane, partial);
-    direct = total > static_cast<uint32_t>(kStreamRingUse);
+  if (end - lo >= static_cast<uint32_t>(kStreamRing)) {
+    stream_flush(ring, lo, open, lane, partial);
+    direct = end - lo >= static_cast<uint32_t>(kStreamRing);
   }
-  const uint32_t spare = kStreamRingUse + lane;
-  // The lane's pass: run s, the run before the first head (lead), segments
-  // closed inside the lane.
+  StepHeads hd;
+  hd.hmask = hmask;
+  hd.own = own;
+  hd.o0 = o0;
+  hd.end = end;
+  hd.direct = __builtin_amdgcn_readfirstlane(direct ? 1 : 0) != 0;  // wave-uniform
+  return hd;
+}
+
+template <bool FULL, typename V, typename A>
+__device__ __forceinline__ void stream_values(const StreamStep<V>& cur, const StepHeads& hd,
+                                              const A* sx, int64_t t, int64_t s0, int64_t s1,
+                                              int lane, lds_f64* ring, uint32_t& lo,
+                                              double* __restrict__ partial, uint32_t& open) {
+  const int64_t e = t + kStreamEpl * lane;
+  const uint32_t hmask = hd.hmask;
+  const int own = hd.own;
+  const uint32_t o0 = hd.o0;
+  const uint32_t end = hd.end;
+  const bool h0 = (hmask & 1u) != 0;
+  const uint32_t spare = kStreamRing + lane;
   A s = A(0), lead = A(0);
-  int seen = 0;
+  uint32_t seen = 0;
+  if (!hd.direct) {
 #pragma unroll
-  for (int j = 0; j < kStreamEpl; ++j) {
-    const uint32_t c = (word_of(cur.c, j / 2) >> (16 * (j & 1))) & 0x7FFFu;
-    A prod = cur.v[j] * sx[c];
-    if constexpr (!FULL) prod = e + j < s1 ? prod : A(0);
-    const bool hj = (hmask >> j) & 1u;
-    const bool closes = hj && seen > 0;
-    if (!direct) {
-      ring[closes ? o0 + seen - rg.base : spare] = static_cast<double>(s);
-    } else if (closes) {
-      partial[o0 + seen] = static_cast<double>(s);
+    for (int j = 0; j < kStreamEpl; ++j) {
+      const uint32_t c = (word_of(cur.c, j / 2) >> (16 * (j & 1))) & 0x7FFFu;
+      A prod = cur.v[j] * sx[c];
+      if constexpr (!FULL) prod = e + j < s1 ? prod : A(0);
+      const bool hj = (hmask >> j) & 1u;
+      ring[hj && seen > 0 ? ring_slot(o0 + seen) : spare] = static_cast<double>(s);
+      lead = hj && seen == 0 ? s : lead;
+      s = __builtin_fma(s, PassMask<A>::value(hj ? 0u : PassMask<A>::kOne), prod);
+      seen += hj ? 1u : 0u;
     }
-    lead = hj && seen == 0 ? s : lead;
-    s = hj ? prod : s + prod;
-    seen += hj ? 1 : 0;
-  }
-  if (own == 0) lead = s;
-  // Segmented scan of the trailing runs: what reaches each lane from the left.
-  A sc = s;
-  int fl = own > 0 ? 1 : 0;
-  seg_scan_wave(sc, fl);
-  if (!fl) sc += st.sum;
-  const A from_left = dpp_move<kDppWaveShr1, 0xF>(sc, st.sum);  // lane 0: the open segment
-  const int next_h0 = dpp_move<kDppWaveShl1, 0xF>(h0 ? 1 : 0, 0);
-  // The run before the first head closes there (a head on entry 0 closes the
-  // segment the lane to the left carried, written by that lane, or for lane 0
-  // the open segment of the previous step).
-  const bool lead_closes = own > 0 && (!h0 || (lane == 0 && t > s0));
-  const A lead_sum = from_left + lead;
-  // The trailing run closes when the next lane opens on a head.
-  const bool tail_closes = next_h0 != 0;
-  const A tail_sum = own > 0 ? s : lead_sum;
-  if (!direct) {
-    ring[lead_closes ? o0 - rg.base : spare] = static_cast<double>(lead_sum);
-    ring[tail_closes ? o0 + own - rg.base : spare] = static_cast<double>(tail_sum);
+    ring[own > 0 ? ring_slot(o0 + static_cast<uint32_t>(own)) : spare] = static_cast<double>(s);
+    __hip_atomic_fetch_add(&ring[ring_slot(o0)], static_cast<double>(own > 0 ? lead : s),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
   } else {
-    if (lead_closes) partial[o0] = static_cast<double>(lead_sum);
-    if (tail_closes) partial[o0 + own] = static_cast<double>(tail_sum);
-    rg.base = end;
-    rg.lo = end;
+    const A carry_in = static_cast<A>(ring[ring_slot(open)]);
+#pragma unroll
+    for (int j = 0; j < kStreamEpl; ++j) {
+      const uint32_t c = (word_of(cur.c, j / 2) >> (16 * (j & 1))) & 0x7FFFu;
+      A prod = cur.v[j] * sx[c];
+      if constexpr (!FULL) prod = e + j < s1 ? prod : A(0);
+      const bool hj = (hmask >> j) & 1u;
+      if (hj && seen > 0) partial[o0 + seen] = static_cast<double>(s);
+      lead = hj && seen == 0 ? s : lead;
+      s = __builtin_fma(s, PassMask<A>::value(hj ? 0u : PassMask<A>::kOne), prod);
+      seen += hj ? 1u : 0u;
+    }
+    // Segmented scan of the trailing runs: what reaches each lane from the left.
+    A sc = s;
+    uint32_t pass = own > 0 ? 0u : PassMask<A>::kOne;
+    pass_scan_wave(sc, pass);
+    sc = __builtin_fma(carry_in, PassMask<A>::value(pass), sc);
+    const A from_left = dpp_move<kDppWaveShr1, 0xF>(sc, carry_in);  // lane 0: the open segment
+    const int next_h0 = dpp_move<kDppWaveShl1, 0xF>(h0 ? 1 : 0, 0);
+    // The run before the first head closes there (a head on entry 0 closes
+    // the segment the lane to the left carried, written by that lane, or for
+    // lane 0 the open segment of the previous step).
+    if (own > 0 && (!h0 || (lane == 0 && t > s0)))
+      partial[o0] = static_cast<double>(from_left + lead);
+    // The trailing run closes when the next lane opens on a head.
+    if (next_h0 != 0) partial[o0 + own] = static_cast<double>(own > 0 ? s : from_left + s);
+    const A open_sum = read_lane(sc, kWave - 1);
+    if (lane == 0) ring[ring_slot(end)] = static_cast<double>(open_sum);
+    lo = end;
   }
-  st.sum = read_lane(sc, kWave - 1);
-  st.ord = end;
+  open = end;
 }
 
 template <typename V>
@@ -753,7 +815,7 @@ __global__ __launch_bounds__(kLdsBlock) void spmv_stream_kernel(
     double* __restrict__ partial, double* __restrict__ carry) {
   using A = typename acc_of<V>::type;
   __shared__ A sx[kStreamPanelBytes / sizeof(A)];
-  __shared__ double sring[kStreamWaves * kStreamRing];
+  __shared__ double sring[kStreamWaves * (kStreamRing + kWave)];
   const uint32_t first = wg_table[2 * blockIdx.x];
   const uint32_t count = wg_table[2 * blockIdx.x + 1];
   const int64_t q = slice_info[4 * static_cast<int64_t>(first) + 3] & ~kStreamClosed;
@@ -790,13 +852,11 @@ __global__ __launch_bounds__(kLdsBlock) void spmv_stream_kernel(
   __syncthreads();
   if (!active) return;  // whole waves leave together
   const bool closed = (si[3] & kStreamClosed) != 0;
-  lds_f64* ring = (lds_f64*)(sring + wave * kStreamRing);  // address-space cast
-  StreamOpen<A> st;
-  st.ord = si[2];
-  st.sum = A(0);
-  StreamRing rg;
-  rg.base = si[2];
-  rg.lo = si[2];
+  lds_f64* ring = (lds_f64*)(sring + wave * (kStreamRing + kWave));  // address-space cast
+  uint32_t open = si[2];
+  uint32_t lo = si[2];
+  // The segment open at the slice's start gets no first write in this slice.
+  if (lane == 0) ring[ring_slot(open)] = 0.0;
   // Full steps with the next one (or two) in flight; the last, partial step
   // of the slice after the loop.
   int64_t t = s0;
@@ -804,49 +864,68 @@ __global__ __launch_bounds__(kLdsBlock) void spmv_stream_kernel(
     // FP64 steps hold twice the registers: one step in flight, two buffers.
     while (t + kStreamTile < s1) {
       stream_claim(b0);
-      stream_load(scol, sval, t + kStreamTile, lane, b1);
-      stream_step<true, V, A>(b0, sx, t, s0, s1, lane, ring, rg, partial, st);
+      {
+        const StepHeads hd = stream_heads<true, V>(b0, t, s0, s1, lane, ring, lo, partial, open);
+        stream_load(scol, sval, t + kStreamTile, lane, b1);
+        stream_values<true, V, A>(b0, hd, sx, t, s0, s1, lane, ring, lo, partial, open);
+      }
       t += kStreamTile;
       if (t + kStreamTile >= s1) {
         b0 = b1;
         break;
       }
       stream_claim(b1);
-      stream_load(scol, sval, t + kStreamTile, lane, b0);
-      stream_step<true, V, A>(b1, sx, t, s0, s1, lane, ring, rg, partial, st);
+      {
+        const StepHeads hd = stream_heads<true, V>(b1, t, s0, s1, lane, ring, lo, partial, open);
+        stream_load(scol, sval, t + kStreamTile, lane, b0);
+        stream_values<true, V, A>(b1, hd, sx, t, s0, s1, lane, ring, lo, partial, open);
+      }
       t += kStreamTile;
     }
   } else {
     // Two steps in flight; the three buffers rotate roles.
     while (t + kStreamTile < s1) {
       stream_claim(b0);
-      if (t + 2 * kStreamTile < s1) stream_load(scol, sval, t + 2 * kStreamTile, lane, b2);
-      stream_step<true, V, A>(b0, sx, t, s0, s1, lane, ring, rg, partial, st);
+      {
+        const StepHeads hd = stream_heads<true, V>(b0, t, s0, s1, lane, ring, lo, partial, open);
+        if (t + 2 * kStreamTile < s1) stream_load(scol, sval, t + 2 * kStreamTile, lane, b2);
+        stream_values<true, V, A>(b0, hd, sx, t, s0, s1, lane, ring, lo, partial, open);
+      }
       t += kStreamTile;
       if (t + kStreamTile >= s1) {
         b0 = b1;
         break;
       }
       stream_claim(b1);
-      if (t + 2 * kStreamTile < s1) stream_load(scol, sval, t + 2 * kStreamTile, lane, b0);
-      stream_step<true, V, A>(b1, sx, t, s0, s1, lane, ring, rg, partial, st);
+      {
+        const StepHeads hd = stream_heads<true, V>(b1, t, s0, s1, lane, ring, lo, partial, open);
+        if (t + 2 * kStreamTile < s1) stream_load(scol, sval, t + 2 * kStreamTile, lane, b0);
+        stream_values<true, V, A>(b1, hd, sx, t, s0, s1, lane, ring, lo, partial, open);
+      }
       t += kStreamTile;
       if (t + kStreamTile >= s1) {
         b0 = b2;
         break;
       }
       stream_claim(b2);
-      if (t + 2 * kStreamTile < s1) stream_load(scol, sval, t + 2 * kStreamTile, lane, b1);
-      stream_step<true, V, A>(b2, sx, t, s0, s1, lane, ring, rg, partial, st);
+      {
+        const StepHeads hd = stream_heads<true, V>(b2, t, s0, s1, lane, ring, lo, partial, open);
+        if (t + 2 * kStreamTile < s1) stream_load(scol, sval, t + 2 * kStreamTile, lane, b1);
+        stream_values<true, V, A>(b2, hd, sx, t, s0, s1, lane, ring, lo, partial, open);
+      }
       t += kStreamTile;
     }
   }
   stream_claim(b0);
-  stream_step<false, V, A>(b0, sx, t, s0, s1, lane, ring, rg, partial, st);
-  stream_flush(ring, rg, st.ord, lane, partial);
+  {
+    const StepHeads hd = stream_heads<false, V>(b0, t, s0, s1, lane, ring, lo, partial, open);
+    stream_values<false, V, A>(b0, hd, sx, t, s0, s1, lane, ring, lo, partial, open);
+  }
+  stream_flush(ring, lo, open, lane, partial);
   if (lane == 0) {
-    if (closed) partial[st.ord] = static_cast<double>(st.sum);
-    else carry[slice] = static_cast<double>(st.sum);
+    const double v = ring[ring_slot(open)];
+    if (closed) partial[open] = v;
+    else carry[slice] = v;
   }
 }
 
