@@ -5,9 +5,10 @@ Headline workload (one MI355X): the single large matrix -- n = 2^22 rows and
 columns, row lengths Zipf(s=1.5) clipped to [1, 10^4] (about 6.4e8 nnz),
 sorted distinct uniform int32 columns, N(0,1) values, FP64, alpha=1, beta=0.
 One step = one compat SpMV y = alpha*A*x + beta*y over the whole matrix
-through the column-panel plan (its analysis runs once, before timing; the
-learned selector's direct-kernel pick is reported beside it), inputs resident
-in HBM.  `value` is algorithmic bytes per SpMV / time:
+through the column-panel plan with stream staging (its analysis runs once,
+before timing; the L2-staged plan and the learned selector's direct-kernel
+pick are reported beside it), inputs resident in HBM.  `value` is
+algorithmic bytes per SpMV / time:
 
   B = nnz*(s_v + s_i) + (rows+1)*s_o + cols*s_x + rows*s_y*(1 + [beta != 0])
 
@@ -372,8 +373,9 @@ def bench_gather_ceiling(ctx, extra, bytes_per_entry, n=1 << 26):
 def bench_dense_rows(ctx, extra, rows=1 << 14, cols=1 << 22, per_row=20_000):
     """Column-panel SpMV on dense rows: rows x cols with per_row uniformly
     random columns each (FP64, int32 columns), where the default analysis
-    stages 160 KiB x panels in LDS; the direct vec64 kernel and the L2-staged
-    plan are timed beside it.  Same algorithmic bytes as the headline."""
+    stages 160 KiB x panels in LDS; the direct vec64 kernel, the L2-staged
+    plan and the stream-staged plan are timed beside it.  Same algorithmic
+    bytes as the headline."""
     m = G.make(G.UNIFORM_FIXED, rows, cols, seed=G.SEED_BASE + 9, values=G.VALUES_NORMAL,
                per_row=per_row)
     a = H.clone_compressed(ctx, H.CSR, m.rows, m.cols, m.offsets, m.indices, m.values, H.F64, 4)
@@ -387,16 +389,16 @@ def bench_dense_rows(ctx, extra, rows=1 << 14, cols=1 << 22, per_row=20_000):
     r = time_ops(ctx, lambda: H.csr_spmv(ctx, 1.0, a, xd, 0.0, yd, alg=H.ALG_VECTOR[64], ws=ws), 5, 2)
     out["direct_vec64_gbs"] = round(by / r["median_ms"] / 1e6, 1)
     for tag, opts in (("panel_l2_gbs", H.panel_options(staging=H.STAGING_L2)),
-                      ("panel_auto_lds_gbs", None)):
+                      ("panel_auto_lds_gbs", None),
+                      ("panel_stream_gbs", H.panel_options(staging=H.STAGING_STREAM))):
         H.csr_spmv_analyze(ctx, a, opts)
         r = time_ops(ctx, lambda: H.csr_spmv(ctx, 1.0, a, xd, 0.0, yd, alg=H.ALG_PANEL, ws=ws), 10, 2)
         out[tag] = round(by / r["median_ms"] / 1e6, 1)
-        if opts is None:
-            out["panel_auto_lds_ms"] = round(r["median_ms"], 3)
+        out[tag.replace("_gbs", "_ms")] = round(r["median_ms"], 3)
     H.plan_clear(ctx)
     extra["dense_rows_2^14x2^22_20k"] = out
     log(f"dense rows: direct {out['direct_vec64_gbs']} GB/s, L2 panels {out['panel_l2_gbs']}, "
-        f"LDS panels {out['panel_auto_lds_gbs']} GB/s")
+        f"LDS panels {out['panel_auto_lds_gbs']}, stream panels {out['panel_stream_gbs']} GB/s")
 
 
 def bench_triangular(ctx, extra, n=1 << 18, per_row=8, band=2000):

@@ -376,12 +376,14 @@ void TestConversions(Gpu& g) {
 }
 
 // Column-panel analysis on the golden matrix: panels of 1 column, every row
-// listed (long_row_min 1); the result still meets the golden vector.
-void TestColumnPanel(Gpu& g) {
-  Case("column-panel plan (double)");
+// listed (long_row_min 1), with the default and the stream staging; the
+// result still meets the golden vector.
+void TestColumnPanel(Gpu& g, uint8_t staging) {
+  Case(staging == ASC_HIP_STAGING_STREAM ? "column-panel plan, stream staging (double)"
+                                         : "column-panel plan (double)");
   auto a = CloneCsr<double>(g, kOffsets4x5, kIndices4x5, kValues4x5, 4, 5);
   const asc::CsrView<const double> cv = *a.view();
-  asc_hip_panel_options opt{1, 1, ASC_HIP_ALG_AUTO, ASC_HIP_ALG_AUTO, {}};
+  asc_hip_panel_options opt{1, 1, ASC_HIP_ALG_AUTO, ASC_HIP_ALG_AUTO, staging, {}};
   auto need = asc::HipCsrSpmvAnalysisSize(g.ctx, cv, &opt);
   CHECK(need.ok());
   auto analysis = asc::HipBuffer::Allocate(g.device, *need);
@@ -601,7 +603,8 @@ int main(int argc, char** argv) {
     TestStandard<float>(g);
     TestConversions<double>(g);
     TestConversions<float>(g);
-    TestColumnPanel(g);
+    TestColumnPanel(g, ASC_HIP_STAGING_AUTO);
+    TestColumnPanel(g, ASC_HIP_STAGING_STREAM);
     TestFinalize(g);
     TestLevelOne<double>(g);
     TestLevelOne<float>(g);
