@@ -133,9 +133,11 @@ struct PanelShape {
 // from the matrix's offsets (read back after the context's stream drains).
 // A validation-only context has no device to read: every row counts.
 asc_hip_status count_listed_rows(asc_hip_context* ctx, const asc_hip_sparse_desc* a,
-                                 int64_t long_min, int64_t* nlong, int64_t* nnz_long) {
+                                 int64_t long_min, int64_t panels, int64_t* nlong,
+                                 int64_t* nnz_long, int64_t* segs_bound) {
   *nlong = a->rows;
   *nnz_long = a->nnz;
+  *segs_bound = a->nnz;
   if (ctx->device < 0 || a->rows == 0) return ok();
   std::vector<int64_t> off(static_cast<size_t>(a->rows) + 1);
   DeviceGuard guard(ctx->device);
@@ -144,16 +146,19 @@ asc_hip_status count_listed_rows(asc_hip_context* ctx, const asc_hip_sparse_desc
                                 hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) return fail_hip(e, "reading the matrix offsets");
-  int64_t n = 0, entries = 0;
+  // A row's entries in one panel form one segment: at most min(len, panels).
+  int64_t n = 0, entries = 0, segs = 0;
   for (int64_t r = 0; r < a->rows; ++r) {
     const int64_t len = off[r + 1] - off[r];
     if (len >= long_min) {
       ++n;
       entries += len;
+      segs += std::min(len, panels);
     }
   }
   *nlong = n;
   *nnz_long = entries;
+  *segs_bound = segs;
   return ok();
 }
 
@@ -167,7 +172,12 @@ asc_hip_status panel_shape(asc_hip_context* ctx, const asc_hip_sparse_desc* a,
   if (a->element_kind == ASC_HIP_C64 || a->element_kind == ASC_HIP_C128)
     return fail(ASC_HIP_UNSUPPORTED, "compat SpMV is defined for real storage");
   const int64_t want_cols = opt ? opt->panel_columns : 0;
-  const int64_t long_min = opt && opt->long_row_min ? opt->long_row_min : kDefaultLongRowMin;
+  const uint8_t staging_req = opt ? opt->staging : ASC_HIP_STAGING_AUTO;
+  // Stream staging lists every nonempty row by default: its x comes from
+  // LDS, so even short rows gain, and the combine walks rows in groups.
+  const int64_t long_min = opt && opt->long_row_min ? opt->long_row_min
+                           : staging_req == ASC_HIP_STAGING_STREAM ? 1
+                                                                   : kDefaultLongRowMin;
   if (want_cols < 0 || long_min < 0)
     return fail(ASC_HIP_INVALID_ARGUMENT, "panel width and long-row length must be nonnegative");
   if (a->rows > kU32Max || std::min(a->nnz, a->cols) > kU32Max)
@@ -189,18 +199,16 @@ asc_hip_status panel_shape(asc_hip_context* ctx, const asc_hip_sparse_desc* a,
     panels = (panels + 7) / 8 * 8;
     if (panels > kMaxPanels)
       return fail(ASC_HIP_INVALID_ARGUMENT, "panel width gives more than 1024 panels");
-    int64_t nlong = 0, nnz_long = 0;
-    if ((st = count_listed_rows(ctx, a, long_min, &nlong, &nnz_long)).code) return st;
-    int64_t pairs = 0;
-    if (!checked_mul(panels, nlong, &pairs) || pairs > std::numeric_limits<int32_t>::max() ||
-        nnz_long + panels * asc_hip::kStreamTile > static_cast<int64_t>(kU32Max))
-      return fail(ASC_HIP_UNSUPPORTED,
-                  "the stream form numbers its entries and (panel, row) pairs in 32 bits");
+    int64_t nlong = 0, nnz_long = 0, segs = 0;
+    if ((st = count_listed_rows(ctx, a, long_min, panels, &nlong, &nnz_long, &segs)).code)
+      return st;
+    if (nnz_long + panels * asc_hip::kStreamTile > static_cast<int64_t>(kU32Max))
+      return fail(ASC_HIP_UNSUPPORTED, "the stream form numbers its entries in 32 bits");
     out->panels = panels;
     out->panel_cols = a->cols > 0 ? (a->cols - 1) / panels + 1 : 1;
     out->long_min = long_min;
     out->stream = true;
-    out->layout = asc_hip::stream_layout(a->rows, nlong, nnz_long, panels,
+    out->layout = asc_hip::stream_layout(a->rows, nlong, nnz_long, segs, panels,
                                          elem_size(a->element_kind));
     out->bytes = out->layout.total;
     return ok();

@@ -58,20 +58,24 @@ struct PanelPlan {
   int64_t chunks = 1;   // LDS form: row chunks per panel (work-groups = panels * chunks)
   // Stream form (built by panel_stream_analysis_launch): the listed rows'
   // entries copied panel-major, so the SpMV streams them contiguously and
-  // gathers x from an LDS copy of the panel.  Panel q's entries occupy
-  // [slice starts of q ...), each panel padded to a multiple of kStreamTile
-  // entries.  scol holds the column within the panel, with kStreamHead set on
-  // the first entry of every (panel, row) segment; sval the value.  Nonempty
-  // segments are numbered panel-major; seg_map[q*nlong + i] is the ordinal
-  // of the (panel q, listed row i) segment, or kStreamNoSegment when empty.  A slice (kStreamSlice entries at most, one wave) is described by
-  // slice_info[4k..4k+3] = {start, end, segment of the first entry,
-  // panel | kStreamClosed when its last segment ends inside it}; wg_table[2w..]
-  // = {first slice, slice count} of work-group w (one panel, XCD-ordered).
-  // spartial[ordinal] holds each segment's sum, written by the SpMV.
+  // gathers x from an LDS copy of the panel.  A listed row's entries in one
+  // panel form a segment; segments are numbered panel-major (rows ascending
+  // inside a panel).  Panel q's entries are padded to a multiple of
+  // kStreamTile; scol holds the column within the panel, with kStreamHead set
+  // on each segment's first entry, sval the value.  A slice (kStreamSlice
+  // entries at most, one wave) is described by slice_info[4k..4k+3] =
+  // {start, end, segment open before the first entry, panel | kStreamClosed
+  // when its last segment ends inside it}; wg_table[2w..] = {first slice,
+  // slice count} of work-group w (one panel, XCD-ordered).  spartial[ordinal]
+  // holds each segment's sum; seg_lrow[ordinal] its listed row modulo
+  // kStreamGroupRows, and ctable[q*(groups+1) + g] the first ordinal of panel
+  // q whose listed row is at least g*kStreamGroupRows.
   bool stream = false;
   const uint16_t* scol = nullptr;
   const void* sval = nullptr;
-  const uint32_t* seg_map = nullptr;
+  const uint16_t* seg_lrow = nullptr;
+  const uint32_t* ctable = nullptr;
+  int64_t groups = 0;
   const uint32_t* slice_info = nullptr;
   const uint32_t* wg_table = nullptr;
   double* spartial = nullptr;
@@ -99,7 +103,8 @@ static_assert(kStreamPanelBytes + int64_t(kStreamWaves) * (kStreamRing + 64) * 8
               "x panel and segment rings share one work-group's LDS");
 constexpr uint16_t kStreamHead = 0x8000;
 constexpr uint32_t kStreamClosed = 0x80000000u;
-constexpr uint32_t kStreamNoSegment = 0xFFFFFFFFu;
+// Listed rows per combine wave (one FP64 LDS accumulator each).
+constexpr int kStreamGroupRows = 1024;
 // Widest stream panel: the column field keeps 15 bits beside the head flag.
 constexpr int64_t kStreamMaxPanelCols = 32768;
 
@@ -145,18 +150,20 @@ hipError_t panel_analysis_launch(const Compressed& a, int64_t panel_cols, int64_
                                  hipStream_t stream);
 
 // Stream-form layout for a matrix whose listed rows (nlong of them, nnz_long
-// entries) are known: the byte offsets of every array in the analysis buffer.
+// entries, at most segs_cap (panel, row) segments) are known: the byte
+// offsets of every array in the analysis buffer.
 struct StreamLayout {
-  int64_t nlong = 0, nnz_long = 0, panels = 0, pairs = 0;
+  int64_t nlong = 0, nnz_long = 0, panels = 0, groups = 0;
   int64_t entries_cap = 0, slices_cap = 0, segs_cap = 0;
-  size_t long_rows = 0, partial = 0, scol = 0, sval = 0, ords = 0, slice_info = 0,
-         wg_table = 0;  // kept (ords: the segment map)
-  size_t row_begin = 0, splits = 0, lens = 0, begins = 0, small = 0,
-         slice_stage = 0, scan = 0, list_scratch = 0;  // analysis only
+  size_t long_rows = 0, partial = 0, scol = 0, sval = 0, seg_lrow = 0, ctable = 0,
+         slice_info = 0, wg_table = 0;  // kept
+  size_t row_begin = 0, cnt = 0, sbase = 0, sstart = 0, spanel = 0, srow = 0, slen = 0, sorder = 0,
+         lens_s = 0, lrow_s = 0, dest = 0, pseg = 0, small = 0, slice_stage = 0, scan = 0,
+         sort = 0, list_scratch = 0;  // analysis only
   size_t kept_bytes = 0, total = 0;
 };
-StreamLayout stream_layout(int64_t rows, int64_t nlong, int64_t nnz_long, int64_t panels,
-                           size_t value_bytes);
+StreamLayout stream_layout(int64_t rows, int64_t nlong, int64_t nnz_long, int64_t segs_cap,
+                           int64_t panels, size_t value_bytes);
 // Builds the stream form into `base` (laid out by `L`); fills the stream
 // fields of `pp` (whose panels/panel_cols/long_min are set).  Synchronous.
 hipError_t panel_stream_analysis_launch(const Compressed& a, const StreamLayout& L, void* base,

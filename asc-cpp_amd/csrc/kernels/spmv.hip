@@ -948,89 +948,179 @@ __global__ __launch_bounds__(kBlock) void spmv_stream_fixup_kernel(
   partial[target] += sum;
 }
 
-// Row epilogue of the stream form: listed row i adds its segments' sums in
-// panel order (seg_map[q*nlong + i]: the (panel q, row i) segment, or
-// kStreamNoSegment).
+// Row epilogue of the stream form.  Wave g owns listed rows [g*R, g*R + R)
+// (R = kStreamGroupRows) with one FP64 accumulator per row in LDS.  In each
+// panel the segments of those rows form one contiguous ordinal range,
+// ctable[q*(groups+1) + g] .. [q*(groups+1) + g + 1], and each row has at
+// most one segment there; the wave walks the panels in order, adding every
+// segment's sum to its row, so a row's sum is formed in panel order (no
+// barriers, no atomics between waves: the result is the same every run).
 template <typename V>
-__global__ __launch_bounds__(kBlock) void spmv_stream_combine_kernel(
+__global__ __launch_bounds__(kLdsBlock) void spmv_stream_combine_kernel(
     KParams p, const uint32_t* __restrict__ long_rows, int64_t nlong, int64_t panels,
-    const uint32_t* __restrict__ seg_map, const double* __restrict__ partial) {
+    const uint32_t* __restrict__ ctable, int64_t groups, const uint16_t* __restrict__ seg_lrow,
+    const double* __restrict__ partial) {
   using Acc = typename AccSel<V, false>::type;
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  if (i >= nlong) return;
-  // Panels in blocks of 8 (panel counts are multiples of 8): a block's sums
-  // load together while the next block's map entries are in flight.
-  constexpr int kB = 8;
-  uint32_t m[kB];
-#pragma unroll
-  for (int k = 0; k < kB; ++k) m[k] = seg_map[k * nlong + i];
-  double s = 0.0;
+  __shared__ double sacc[kStreamWaves * kStreamGroupRows];
+  const int wave = threadIdx.x / kWave;
+  const int lane = threadIdx.x % kWave;
+  const int64_t g = static_cast<int64_t>(blockIdx.x) * kStreamWaves + wave;
+  if (g >= groups) return;  // no barrier follows
+  lds_f64* acc = (lds_f64*)(sacc + wave * kStreamGroupRows);  // address-space cast
+  for (int k = lane; k < kStreamGroupRows; k += kWave) acc[k] = 0.0;
+  const int64_t stride = groups + 1;
+  constexpr int kB = 8;  // panels (a multiple of 8) per batch of loads
   for (int64_t q0 = 0; q0 < panels; q0 += kB) {
-    double v[kB];
+    uint32_t lo[kB], n[kB];
 #pragma unroll
-    for (int k = 0; k < kB; ++k) v[k] = m[k] != kStreamNoSegment ? partial[m[k]] : 0.0;
-    if (q0 + kB < panels) {
+    for (int j = 0; j < kB; ++j) {
+      lo[j] = ctable[(q0 + j) * stride + g];
+      n[j] = ctable[(q0 + j) * stride + g + 1] - lo[j];
+    }
+    // Up to two segments per lane per panel load together; longer ranges
+    // (more than 128 of the group's rows in one panel) finish after.
+    double v[kB][2];
+    uint16_t r[kB][2];
 #pragma unroll
-      for (int k = 0; k < kB; ++k) m[k] = seg_map[(q0 + kB + k) * nlong + i];
+    for (int j = 0; j < kB; ++j) {
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const uint32_t k = static_cast<uint32_t>(lane + u * kWave);
+        v[j][u] = k < n[j] ? partial[lo[j] + k] : 0.0;
+        r[j][u] = k < n[j] ? seg_lrow[lo[j] + k] : static_cast<uint16_t>(0);
+      }
     }
 #pragma unroll
-    for (int k = 0; k < kB; ++k) s += v[k];
+    for (int j = 0; j < kB; ++j) {
+#pragma unroll
+      for (int u = 0; u < 2; ++u)
+        if (static_cast<uint32_t>(lane + u * kWave) < n[j])
+          __hip_atomic_fetch_add(&acc[r[j][u]], v[j][u], __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_WAVEFRONT);
+      for (uint32_t k = lane + 2 * kWave; k < n[j]; k += kWave)
+        __hip_atomic_fetch_add(&acc[seg_lrow[lo[j] + k]], partial[lo[j] + k], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WAVEFRONT);
+    }
   }
-  finish_row<V>(p, long_rows[i], Acc::from_f64(s));
+  const int64_t base = g * kStreamGroupRows;
+  for (int k = lane; k < kStreamGroupRows && base + k < nlong; k += kWave)
+    finish_row<V>(p, long_rows[base + k], Acc::from_f64(acc[k]));
 }
 
 // ---- stream-form analysis -------------------------------------------------
-// Segment lengths from the split table: lens[q*nlong + i] = entries of listed
-// row i in panel q.
-__global__ __launch_bounds__(kBlock) void stream_len_kernel(
-    const uint32_t* __restrict__ splits, int64_t nlong, int64_t pairs,
-    uint32_t* __restrict__ lens) {
-  const int64_t idx = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  if (idx >= pairs) return;
-  lens[idx] = splits[idx] - (idx >= nlong ? splits[idx - nlong] : 0u);
-}
+// A listed row's entries in one panel are a contiguous run of its sorted
+// columns: a segment.  G lanes walk each listed row, flag the entries that
+// start a segment (the row's first entry, or a panel change), and either
+// count them (cnt) or record them (start, panel, listed row) at the row's
+// base ordinal plus the heads before them.  Segments are in row order here;
+// a stable sort by panel then gives the stream order.
+constexpr int kSegLanes = 16;
 
-__global__ __launch_bounds__(kBlock) void stream_nonempty_kernel(uint32_t* __restrict__ lens,
-                                                                 int64_t pairs) {
-  const int64_t idx = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  if (idx < pairs) lens[idx] = lens[idx] > 0 ? 1u : 0u;
-}
-
-// small[q] = first raw stream position of panel q (q < panels),
-// small[panels] = total entries, small[panels + 1] = nonempty segments.
-__global__ __launch_bounds__(kBlock) void stream_totals_kernel(
-    const uint32_t* __restrict__ splits, const uint32_t* __restrict__ begins,
-    const uint32_t* __restrict__ ords, int64_t nlong, int64_t panels,
-    uint32_t* __restrict__ small) {
-  const int64_t q = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  const int64_t pairs = nlong * panels;
-  if (q < panels) small[q] = begins[q * nlong];
-  if (q == panels) {
-    const int64_t last = pairs - 1;
-    const uint32_t len = splits[last] - (last >= nlong ? splits[last - nlong] : 0u);
-    small[panels] = begins[last] + len;
-    small[panels + 1] = ords[last] + (len > 0 ? 1u : 0u);
+template <typename I, bool FILL>
+__global__ __launch_bounds__(kBlock) void stream_segments_kernel(
+    const int64_t* __restrict__ offsets, const I* __restrict__ col,
+    const uint32_t* __restrict__ long_rows, int64_t nlong, uint32_t panel_cols,
+    uint32_t* __restrict__ cnt, const uint32_t* __restrict__ base, uint32_t* __restrict__ sstart,
+    uint32_t* __restrict__ spanel, uint32_t* __restrict__ srow) {
+  const int64_t i = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) / kSegLanes;
+  const int sub = threadIdx.x % kSegLanes;
+  const int shift = (threadIdx.x % kWave) / kSegLanes * kSegLanes;  // the group's lanes in the wave
+  const bool live = i < nlong;  // whole groups agree
+  int64_t b = 0, e = 0;
+  if (live) {
+    const int64_t r = long_rows[i];
+    b = offsets[r];
+    e = offsets[r + 1];
   }
+  uint32_t count = 0;
+  const uint32_t at = FILL && live ? base[i] : 0u;
+  // Every lane of the wave iterates while any group still has entries.
+  int64_t len = e - b;
+  int64_t rounds = (len + kSegLanes - 1) / kSegLanes;
+  // One shuffle per step, taken by every lane (a shuffle inside a
+  // conditional would read inactive lanes as 0).
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    const int64_t other = __shfl_xor(rounds, off);
+    rounds = rounds > other ? rounds : other;
+  }
+  for (int64_t t = 0; t < rounds; ++t) {
+    const int64_t k = b + t * kSegLanes + sub;
+    bool head = false;
+    uint32_t q = 0;
+    if (k < e) {
+      q = static_cast<uint32_t>(static_cast<int64_t>(col[k])) / panel_cols;
+      head = k == b || static_cast<uint32_t>(static_cast<int64_t>(col[k - 1])) / panel_cols != q;
+    }
+    const uint64_t m = (__ballot(head) >> shift) & ((1ull << kSegLanes) - 1ull);
+    if (FILL && head) {
+      const uint32_t sid = at + count + static_cast<uint32_t>(__popcll(m & ((1ull << sub) - 1ull)));
+      sstart[sid] = static_cast<uint32_t>(k);
+      spanel[sid] = q;
+      srow[sid] = static_cast<uint32_t>(i);
+    }
+    count += static_cast<uint32_t>(__popcll(m));
+  }
+  if (!FILL && live && sub == 0) cnt[i] = count;
 }
 
-// One thread per (panel, listed row) segment: copies its entries into the
-// panel's stream and flags the first.
+// Segment lengths (row order): up to the next segment of the same row, or
+// the row's end.
+__global__ __launch_bounds__(kBlock) void stream_seg_len_kernel(
+    const int64_t* __restrict__ offsets, const uint32_t* __restrict__ long_rows,
+    const uint32_t* __restrict__ sstart, const uint32_t* __restrict__ srow, int64_t nseg,
+    uint32_t* __restrict__ slen) {
+  const int64_t sid = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (sid >= nseg) return;
+  const uint32_t i = srow[sid];
+  const uint32_t end = sid + 1 < nseg && srow[sid + 1] == i
+                           ? sstart[sid + 1]
+                           : static_cast<uint32_t>(offsets[long_rows[i] + 1]);
+  slen[sid] = end - sstart[sid];
+}
+
+// Segment fields in stream order (o: ordinal, sorder[o]: row-order segment).
+__global__ __launch_bounds__(kBlock) void stream_sorted_kernel(
+    const uint32_t* __restrict__ sorder, const uint32_t* __restrict__ slen,
+    const uint32_t* __restrict__ srow, int64_t nseg, uint32_t* __restrict__ lens_s,
+    uint32_t* __restrict__ lrow_s) {
+  const int64_t o = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (o >= nseg) return;
+  const uint32_t sid = sorder[o];
+  lens_s[o] = slen[sid];
+  lrow_s[o] = srow[sid];
+}
+
+// small[q] = first raw stream position of panel q, small[panels] = entries.
+__global__ __launch_bounds__(kBlock) void stream_panel_starts_kernel(
+    const int64_t* __restrict__ pseg, const uint32_t* __restrict__ dest, int64_t panels,
+    int64_t nseg, uint32_t total, uint32_t* __restrict__ small) {
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (q > panels) return;
+  const int64_t o = q < panels ? pseg[q] : nseg;
+  small[q] = o < nseg ? dest[o] : total;
+}
+
+// One thread per segment (stream order): copies its entries into the
+// panel's stream (column within the panel, head flag on the first; values
+// at their lane-interleaved positions) and records its row within its
+// group of kStreamGroupRows listed rows.
 template <typename I, typename VB>
 __global__ __launch_bounds__(kBlock) void stream_scatter_kernel(
-    const I* __restrict__ col, const VB* __restrict__ val, const int64_t* __restrict__ row_begin,
-    const uint32_t* __restrict__ splits, const uint32_t* __restrict__ begins,
-    const uint32_t* __restrict__ shift, int64_t nlong, int64_t pairs, int64_t panel_cols,
-    uint16_t* __restrict__ scol, VB* __restrict__ sval) {
-  const int64_t idx = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  if (idx >= pairs) return;
-  const int64_t q = idx / nlong;
-  const int64_t i = idx - q * nlong;
-  const uint32_t sb = q > 0 ? splits[idx - nlong] : 0u;
-  const uint32_t len = splits[idx] - sb;
-  if (len == 0) return;
-  const int64_t src = row_begin[i] + sb;
-  const int64_t dst = static_cast<int64_t>(begins[idx]) + shift[q];
+    const I* __restrict__ col, const VB* __restrict__ val, const uint32_t* __restrict__ sorder,
+    const uint32_t* __restrict__ sstart, const uint32_t* __restrict__ spanel,
+    const uint32_t* __restrict__ lens_s, const uint32_t* __restrict__ lrow_s,
+    const uint32_t* __restrict__ dest, const uint32_t* __restrict__ shift, int64_t nseg,
+    int64_t panel_cols, uint16_t* __restrict__ scol, VB* __restrict__ sval,
+    uint16_t* __restrict__ seg_lrow) {
+  const int64_t o = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (o >= nseg) return;
+  const uint32_t sid = sorder[o];
+  const int64_t src = sstart[sid];
+  const int64_t q = spanel[sid];
+  const uint32_t len = lens_s[o];
+  const int64_t dst = static_cast<int64_t>(dest[o]) + shift[q];
   const int64_t col0 = q * panel_cols;
+  seg_lrow[o] = static_cast<uint16_t>(lrow_s[o] % kStreamGroupRows);
   for (uint32_t k = 0; k < len; ++k) {
     const uint32_t c = static_cast<uint32_t>(static_cast<int64_t>(col[src + k]) - col0);
     scol[dst + k] = static_cast<uint16_t>(c | (k == 0 ? kStreamHead : 0u));
@@ -1042,23 +1132,22 @@ __global__ __launch_bounds__(kBlock) void stream_scatter_kernel(
 // panel | kStreamClosed on the panel's last slice}: the segment open before
 // the slice's first entry, and whether the slice's last segment ends in it.
 __global__ __launch_bounds__(kBlock) void stream_slice_kernel(
-    const uint32_t* __restrict__ stage, int64_t slices, const uint32_t* __restrict__ begins,
-    const uint32_t* __restrict__ ords, int64_t nlong, const uint16_t* __restrict__ scol,
+    const uint32_t* __restrict__ stage, int64_t slices, const int64_t* __restrict__ pseg,
+    const uint32_t* __restrict__ dest, const uint16_t* __restrict__ scol,
     uint32_t* __restrict__ slice_info) {
   const int64_t k = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (k >= slices) return;
   const uint32_t start = stage[4 * k], end = stage[4 * k + 1], raw = stage[4 * k + 2];
   const uint32_t w = stage[4 * k + 3];
   const int64_t q = w & ~kStreamClosed;
-  const uint32_t* b = begins + q * nlong;
-  // Largest i with b[i] <= raw: that segment is nonempty and holds entry raw.
-  int64_t lo = 0, hi = nlong - 1;
+  // Last ordinal o of panel q with dest[o] <= raw: the segment holding raw.
+  int64_t lo = pseg[q], hi = pseg[q + 1] - 1;
   while (lo < hi) {
     const int64_t mid = (lo + hi + 1) >> 1;
-    if (b[mid] <= raw) lo = mid;
+    if (dest[mid] <= raw) lo = mid;
     else hi = mid - 1;
   }
-  const uint32_t ord = ords[q * nlong + lo] - (b[lo] == raw ? 1u : 0u);
+  const uint32_t ord = static_cast<uint32_t>(lo) - (dest[lo] == raw ? 1u : 0u);
   const bool closed = (w & kStreamClosed) != 0 || (scol[end] & kStreamHead) != 0;
   slice_info[4 * k] = start;
   slice_info[4 * k + 1] = end;
@@ -1066,13 +1155,23 @@ __global__ __launch_bounds__(kBlock) void stream_slice_kernel(
   slice_info[4 * k + 3] = static_cast<uint32_t>(q) | (closed ? kStreamClosed : 0u);
 }
 
-// Segment map: the ordinal of each nonempty (panel, row) segment, else
-// kStreamNoSegment (`flags` holds the nonempty flags).
-__global__ __launch_bounds__(kBlock) void stream_map_kernel(const uint32_t* __restrict__ flags,
-                                                            int64_t pairs,
-                                                            uint32_t* __restrict__ ords) {
+// Combine table: ctable[q*(groups+1) + g] = first ordinal of panel q whose
+// listed row is at least g*kStreamGroupRows.
+__global__ __launch_bounds__(kBlock) void stream_ctable_kernel(
+    const int64_t* __restrict__ pseg, const uint32_t* __restrict__ lrow_s, int64_t panels,
+    int64_t groups, uint32_t* __restrict__ ctable) {
   const int64_t idx = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  if (idx < pairs && flags[idx] == 0) ords[idx] = kStreamNoSegment;
+  const int64_t stride = groups + 1;
+  if (idx >= panels * stride) return;
+  const int64_t q = idx / stride, g = idx - q * stride;
+  const uint32_t bound = static_cast<uint32_t>(g * kStreamGroupRows);
+  int64_t lo = pseg[q], hi = pseg[q + 1];
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (lrow_s[mid] < bound) lo = mid + 1;
+    else hi = mid;
+  }
+  ctable[idx] = static_cast<uint32_t>(lo);
 }
 
 // ---------------------------------------------------------------------------
@@ -1466,16 +1565,23 @@ hipError_t launch_stream_forked(const SpmvArgs& a, const KParams& kp, void* ws, 
                                 const SideStream& side) {
   const PanelPlan& pp = a.panel;
   double* carry = static_cast<double*>(ws);
-  hipError_t e = hipEventRecord(side.fork, s);
+  // Rows below long_min (at the default of 1: only empty rows) are finished
+  // by the short-row pass on the side stream; with every row listed there is
+  // nothing to fork.
+  const bool shorts = pp.nlong < a.a.outer;
+  hipError_t e = shorts ? hipEventRecord(side.fork, s) : hipSuccess;
   if (e != hipSuccess) return e;
   spmv_stream_kernel<V><<<static_cast<unsigned>(pp.wgs), kLdsBlock, 0, s>>>(
       static_cast<const typename acc_of<V>::type*>(kp.x), kp.incx, a.a.inner, pp.panel_cols,
       pp.scol, static_cast<const V*>(pp.sval), pp.slice_info, pp.wg_table, pp.spartial, carry);
   e = hipGetLastError();
   // The side stream's work is joined below whatever happens after the fork.
-  hipError_t es = hipStreamWaitEvent(side.stream, side.fork, 0);
-  if (es == hipSuccess) es = launch_short_rows<V, I>(a, kp, pp.long_min, side.stream);
-  const hipError_t ej = hipEventRecord(side.join, side.stream);
+  hipError_t es = hipSuccess, ej = hipSuccess;
+  if (shorts) {
+    es = hipStreamWaitEvent(side.stream, side.fork, 0);
+    if (es == hipSuccess) es = launch_short_rows<V, I>(a, kp, pp.long_min, side.stream);
+    ej = hipEventRecord(side.join, side.stream);
+  }
   if (e == hipSuccess) {
     spmv_stream_fixup_kernel<<<static_cast<unsigned>((pp.slices + kBlock - 1) / kBlock), kBlock, 0,
                                s>>>(pp.slice_info, pp.slices, carry, pp.spartial);
@@ -1483,11 +1589,12 @@ hipError_t launch_stream_forked(const SpmvArgs& a, const KParams& kp, void* ws, 
   }
   if (e == hipSuccess) {
     spmv_stream_combine_kernel<V>
-        <<<static_cast<unsigned>((pp.nlong + kBlock - 1) / kBlock), kBlock, 0, s>>>(
-            kp, pp.long_rows, pp.nlong, pp.panels, pp.seg_map, pp.spartial);
+        <<<static_cast<unsigned>((pp.groups + kStreamWaves - 1) / kStreamWaves), kLdsBlock, 0, s>>>(
+            kp, pp.long_rows, pp.nlong, pp.panels, pp.ctable, pp.groups, pp.seg_lrow, pp.spartial);
     e = hipGetLastError();
   }
-  const hipError_t ew = ej == hipSuccess ? hipStreamWaitEvent(s, side.join, 0) : ej;
+  const hipError_t ew =
+      !shorts ? hipSuccess : ej == hipSuccess ? hipStreamWaitEvent(s, side.join, 0) : ej;
   if (e != hipSuccess) return e;
   if (es != hipSuccess) return es;
   return ew;
@@ -1588,8 +1695,10 @@ hipError_t launch_typed(const SpmvArgs& a, const KParams& kp, void* ws,
               pp.slice_info, pp.slices, carry, pp.spartial);
           e = hipGetLastError();
           if (e != hipSuccess) return e;
-          spmv_stream_combine_kernel<V><<<blocks_for(pp.nlong, kBlock), kBlock, 0, s>>>(
-              kp, pp.long_rows, pp.nlong, pp.panels, pp.seg_map, pp.spartial);
+          spmv_stream_combine_kernel<V>
+              <<<static_cast<unsigned>((pp.groups + kStreamWaves - 1) / kStreamWaves), kLdsBlock,
+                 0, s>>>(kp, pp.long_rows, pp.nlong, pp.panels, pp.ctable, pp.groups,
+                         pp.seg_lrow, pp.spartial);
           return hipGetLastError();
         }
       }
@@ -1728,17 +1837,17 @@ hipError_t panel_analysis_launch(const Compressed& a, int64_t panel_cols, int64_
   return hipGetLastError();
 }
 
-StreamLayout stream_layout(int64_t rows, int64_t nlong, int64_t nnz_long, int64_t panels,
-                           size_t value_bytes) {
+StreamLayout stream_layout(int64_t rows, int64_t nlong, int64_t nnz_long, int64_t segs_cap,
+                           int64_t panels, size_t value_bytes) {
   StreamLayout L;
   L.nlong = nlong;
   L.nnz_long = nnz_long;
   L.panels = panels;
-  L.pairs = nlong * panels;
+  L.groups = (nlong + kStreamGroupRows - 1) / kStreamGroupRows;
   L.entries_cap = nnz_long + panels * kStreamTile;
   L.slices_cap = L.entries_cap / kStreamSlice + panels + 1;
-  L.segs_cap = nnz_long < L.pairs ? nnz_long : L.pairs;
-  const size_t pairs = static_cast<size_t>(L.pairs > 0 ? L.pairs : 1);
+  L.segs_cap = segs_cap;
+  const size_t segs = static_cast<size_t>(segs_cap > 0 ? segs_cap : 1);
   const size_t n = static_cast<size_t>(nlong > 0 ? nlong : 1);
   size_t off = 0;
   auto take = [&](size_t bytes) {
@@ -1747,43 +1856,55 @@ StreamLayout stream_layout(int64_t rows, int64_t nlong, int64_t nnz_long, int64_
     return at;
   };
   L.long_rows = take(sizeof(uint32_t) * static_cast<size_t>(rows > 0 ? rows : 1));
-  L.partial = take(sizeof(double) * static_cast<size_t>(L.segs_cap > 0 ? L.segs_cap : 1));
+  L.partial = take(sizeof(double) * segs);
   L.scol = take(sizeof(uint16_t) * static_cast<size_t>(L.entries_cap));
   L.sval = take(value_bytes * static_cast<size_t>(L.entries_cap));
-  L.ords = take(sizeof(uint32_t) * pairs);  // the segment map
+  L.seg_lrow = take(sizeof(uint16_t) * segs);
+  L.ctable = take(sizeof(uint32_t) * static_cast<size_t>(panels * (L.groups + 1)));
   L.slice_info = take(4 * sizeof(uint32_t) * static_cast<size_t>(L.slices_cap));
   L.wg_table = take(2 * sizeof(uint32_t) * static_cast<size_t>(L.slices_cap));
   L.kept_bytes = off;
-  L.row_begin = take(sizeof(int64_t) * static_cast<size_t>(rows > 0 ? rows : 1));
-  L.splits = take(sizeof(uint32_t) * pairs);
-  L.lens = take(sizeof(uint32_t) * pairs);
-  L.begins = take(sizeof(uint32_t) * pairs);
+  L.row_begin = take(sizeof(int64_t) * n);
+  L.cnt = take(sizeof(uint32_t) * n);
+  L.sbase = take(sizeof(uint32_t) * n);
+  L.sstart = take(sizeof(uint32_t) * segs);
+  L.spanel = take(sizeof(uint32_t) * segs);
+  L.srow = take(sizeof(uint32_t) * segs);
+  L.slen = take(sizeof(uint32_t) * segs);
+  L.sorder = take(sizeof(uint32_t) * segs);
+  L.lens_s = take(sizeof(uint32_t) * segs);
+  L.lrow_s = take(sizeof(uint32_t) * segs);
+  L.dest = take(sizeof(uint32_t) * segs);
+  L.pseg = take(sizeof(int64_t) * static_cast<size_t>(panels + 1));
   L.small = take(sizeof(uint32_t) * static_cast<size_t>(2 * panels + 4));
   L.slice_stage = take(4 * sizeof(uint32_t) * static_cast<size_t>(L.slices_cap));
-  L.scan = take(scan_u32_scratch_bytes(L.pairs));
+  const int64_t scan_n = segs_cap > nlong ? segs_cap : nlong;
+  L.scan = take(scan_u32_scratch_bytes(scan_n));
+  L.sort = take(sort_by_key_workspace_bytes(segs_cap, panels));
   L.list_scratch = take(panel_analysis_scratch_bytes(rows));
   L.total = off;
-  (void)n;
   return L;
 }
 
 hipError_t panel_stream_analysis_launch(const Compressed& a, const StreamLayout& L, void* base,
                                         PanelPlan* pp, hipStream_t s) {
   char* w = static_cast<char*>(base);
-  uint32_t* long_rows = reinterpret_cast<uint32_t*>(w + L.long_rows);
+  auto at = [&](size_t o) { return reinterpret_cast<uint32_t*>(w + o); };
+  uint32_t* long_rows = at(L.long_rows);
   double* partial = reinterpret_cast<double*>(w + L.partial);
   uint16_t* scol = reinterpret_cast<uint16_t*>(w + L.scol);
   void* sval = w + L.sval;
-  uint32_t* slice_info = reinterpret_cast<uint32_t*>(w + L.slice_info);
-  uint32_t* wg_table = reinterpret_cast<uint32_t*>(w + L.wg_table);
-  int64_t* row_begin = reinterpret_cast<int64_t*>(w + L.row_begin);
-  uint32_t* splits = reinterpret_cast<uint32_t*>(w + L.splits);
-  uint32_t* lens = reinterpret_cast<uint32_t*>(w + L.lens);
-  uint32_t* begins = reinterpret_cast<uint32_t*>(w + L.begins);
-  uint32_t* ords = reinterpret_cast<uint32_t*>(w + L.ords);
-  uint32_t* small = reinterpret_cast<uint32_t*>(w + L.small);
-  uint32_t* stage = reinterpret_cast<uint32_t*>(w + L.slice_stage);
+  uint16_t* seg_lrow = reinterpret_cast<uint16_t*>(w + L.seg_lrow);
+  uint32_t* ctable = at(L.ctable);
+  uint32_t* slice_info = at(L.slice_info);
+  uint32_t* wg_table = at(L.wg_table);
+  uint32_t *cnt = at(L.cnt), *sbase = at(L.sbase), *sstart = at(L.sstart), *spanel = at(L.spanel),
+           *srow = at(L.srow), *slen = at(L.slen), *sorder = at(L.sorder), *lens_s = at(L.lens_s),
+           *lrow_s = at(L.lrow_s), *dest = at(L.dest), *small = at(L.small),
+           *stage = at(L.slice_stage);
+  int64_t* pseg = reinterpret_cast<int64_t*>(w + L.pseg);
   const int64_t panels = pp->panels;
+  const uint32_t pcols = static_cast<uint32_t>(pp->panel_cols);
   const size_t vb = elem_bytes(a.elem);
   pp->stream = true;
   pp->long_rows = long_rows;
@@ -1791,7 +1912,9 @@ hipError_t panel_stream_analysis_launch(const Compressed& a, const StreamLayout&
   pp->splits = nullptr;
   pp->scol = scol;
   pp->sval = sval;
-  pp->seg_map = ords;
+  pp->seg_lrow = seg_lrow;
+  pp->ctable = ctable;
+  pp->groups = L.groups;
   pp->slice_info = slice_info;
   pp->wg_table = wg_table;
   pp->spartial = partial;
@@ -1800,32 +1923,56 @@ hipError_t panel_stream_analysis_launch(const Compressed& a, const StreamLayout&
   pp->stream_bytes = L.kept_bytes;
   pp->slices = 0;
   pp->wgs = 0;
+  // Listed rows (the row-begin and split outputs of the listing are unused
+  // here: one panel of the whole width keeps the split table to one entry
+  // per row).
   int64_t nlong = 0;
-  hipError_t e = panel_analysis_launch(a, pp->panel_cols, panels, pp->long_min, long_rows,
-                                       row_begin, splits, w + L.list_scratch, &nlong, s);
+  hipError_t e = panel_analysis_launch(a, a.inner > 0 ? a.inner : 1, 1, pp->long_min, long_rows,
+                                       reinterpret_cast<int64_t*>(w + L.row_begin), cnt,
+                                       w + L.list_scratch, &nlong, s);
   if (e != hipSuccess) return e;
   if (nlong != L.nlong) return hipErrorInvalidValue;  // the structure changed since sizing
   pp->nlong = nlong;
   if (nlong == 0) return hipSuccess;
-  const int64_t pairs = L.pairs;
-  const unsigned pb = static_cast<unsigned>((pairs + kBlock - 1) / kBlock);
-  // Segment lengths, their stream positions, nonempty-segment ordinals.
-  stream_len_kernel<<<pb, kBlock, 0, s>>>(splits, nlong, pairs, lens);
+  // Segments in row order: count, scan, record, measure.
+  const unsigned lb = static_cast<unsigned>((nlong * kSegLanes + kBlock - 1) / kBlock);
+#define ASC_SEGS(I, FILL)                                                                      \
+  stream_segments_kernel<I, FILL><<<lb, kBlock, 0, s>>>(                                     \
+      a.offsets, static_cast<const I*>(a.indices), long_rows, nlong, pcols, cnt, sbase, sstart, \
+      spanel, srow)
+  if (a.index_bytes == 4) ASC_SEGS(int32_t, false);
+  else ASC_SEGS(int64_t, false);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  if ((e = scan_u32_exclusive(lens, begins, pairs, w + L.scan, s)) != hipSuccess) return e;
-  stream_nonempty_kernel<<<pb, kBlock, 0, s>>>(lens, pairs);
+  if ((e = scan_u32_exclusive(cnt, sbase, nlong, w + L.scan, s)) != hipSuccess) return e;
+  uint32_t hcount[2] = {0, 0};
+  e = hipMemcpyAsync(&hcount[0], sbase + nlong - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(&hcount[1], cnt + nlong - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return e;
+  const int64_t nseg = static_cast<int64_t>(hcount[0]) + hcount[1];
+  if (nseg > L.segs_cap) return hipErrorInvalidValue;
+  if (a.index_bytes == 4) ASC_SEGS(int32_t, true);
+  else ASC_SEGS(int64_t, true);
+#undef ASC_SEGS
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  if ((e = scan_u32_exclusive(lens, ords, pairs, w + L.scan, s)) != hipSuccess) return e;
-  stream_totals_kernel<<<static_cast<unsigned>((panels + 1 + kBlock - 1) / kBlock), kBlock, 0,
-                         s>>>(splits, begins, ords, nlong, panels, small);
+  const unsigned sb = static_cast<unsigned>((nseg + kBlock - 1) / kBlock);
+  stream_seg_len_kernel<<<sb, kBlock, 0, s>>>(a.offsets, long_rows, sstart, srow, nseg, slen);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  std::vector<uint32_t> hs(static_cast<size_t>(panels + 2));
+  // Stream order: a stable sort of the segments by panel.
+  if ((e = sort_by_key_launch(spanel, nseg, panels, sorder, pseg, w + L.sort, s)) != hipSuccess)
+    return e;
+  stream_sorted_kernel<<<sb, kBlock, 0, s>>>(sorder, slen, srow, nseg, lens_s, lrow_s);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = scan_u32_exclusive(lens_s, dest, nseg, w + L.scan, s)) != hipSuccess) return e;
+  stream_panel_starts_kernel<<<static_cast<unsigned>((panels + 1 + kBlock - 1) / kBlock), kBlock, 0,
+                               s>>>(pseg, dest, panels, nseg, static_cast<uint32_t>(L.nnz_long),
+                                    small);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  std::vector<uint32_t> hs(static_cast<size_t>(panels + 1));
   e = hipMemcpyAsync(hs.data(), small, sizeof(uint32_t) * hs.size(), hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) return e;
-  if (hs[panels] != static_cast<uint32_t>(L.nnz_long) ||
-      static_cast<int64_t>(hs[panels + 1]) > L.segs_cap)
-    return hipErrorInvalidValue;
   // Padded panel starts, slices and work-groups (host: panels are few).
   std::vector<uint32_t> shift(static_cast<size_t>(panels));
   std::vector<uint32_t> hstage;
@@ -1833,7 +1980,7 @@ hipError_t panel_stream_analysis_launch(const Compressed& a, const StreamLayout&
   int64_t padded = 0, slices = 0;
   for (int64_t q = 0; q < panels; ++q) {
     const int64_t raw = hs[q];
-    const int64_t len = static_cast<int64_t>(q + 1 < panels ? hs[q + 1] : hs[panels]) - raw;
+    const int64_t len = static_cast<int64_t>(hs[q + 1]) - raw;
     shift[q] = static_cast<uint32_t>(padded - raw);
     const int64_t ns = (len + kStreamSlice - 1) / kStreamSlice;
     for (int64_t j = 0; j < ns; ++j) {
@@ -1845,14 +1992,16 @@ hipError_t panel_stream_analysis_launch(const Compressed& a, const StreamLayout&
       hstage.push_back(static_cast<uint32_t>(q) | (j + 1 == ns ? kStreamClosed : 0u));
     }
     for (int64_t j = 0; j < ns; j += kStreamWaves) {
-      const int64_t cnt = ns - j < kStreamWaves ? ns - j : kStreamWaves;
+      const int64_t cnt2 = ns - j < kStreamWaves ? ns - j : kStreamWaves;
       xcd_wgs[q % kXcds].push_back(static_cast<uint32_t>(slices + j));
-      xcd_wgs[q % kXcds].push_back(static_cast<uint32_t>(cnt));
+      xcd_wgs[q % kXcds].push_back(static_cast<uint32_t>(cnt2));
     }
     slices += ns;
     padded += (len + kStreamTile - 1) / kStreamTile * kStreamTile;
   }
-  if (slices > L.slices_cap || padded > L.entries_cap) return hipErrorInvalidValue;
+  if (hs[panels] != static_cast<uint32_t>(L.nnz_long) || slices > L.slices_cap ||
+      padded > L.entries_cap)
+    return hipErrorInvalidValue;
   // Work-group b runs on XCD b % 8: give it a work-group of a panel that XCD
   // stages (q % 8), so each XCD's panels stay in its L2; when an XCD's list
   // runs out, the longest remaining list fills in.
@@ -1880,21 +2029,21 @@ hipError_t panel_stream_analysis_launch(const Compressed& a, const StreamLayout&
   uint32_t* dshift = small + panels + 2;
   e = hipMemcpyAsync(dshift, shift.data(), sizeof(uint32_t) * shift.size(),
                      hipMemcpyHostToDevice, s);
-  if (e == hipSuccess)
+  if (e == hipSuccess && !hstage.empty())
     e = hipMemcpyAsync(stage, hstage.data(), sizeof(uint32_t) * hstage.size(),
                        hipMemcpyHostToDevice, s);
-  if (e == hipSuccess)
+  if (e == hipSuccess && !hwg.empty())
     e = hipMemcpyAsync(wg_table, hwg.data(), sizeof(uint32_t) * hwg.size(),
                        hipMemcpyHostToDevice, s);
-  // Zeroed stream (padding reads as column 0, value 0) and partial slots
-  // (empty segments are never written and stay zero).
+  // Zeroed stream (padding reads as column 0, value 0).
   if (e == hipSuccess) e = fill_launch(scol, 0, sizeof(uint16_t) * static_cast<size_t>(padded), s);
   if (e == hipSuccess) e = fill_launch(sval, 0, vb * static_cast<size_t>(padded), s);
   if (e != hipSuccess) return e;
-#define ASC_SCATTER(I, VB)                                                                   \
-  stream_scatter_kernel<I, VB><<<pb, kBlock, 0, s>>>(                                      \
-      static_cast<const I*>(a.indices), static_cast<const VB*>(a.values), row_begin, splits, \
-      begins, dshift, nlong, pairs, pp->panel_cols, scol, static_cast<VB*>(sval))
+#define ASC_SCATTER(I, VB)                                                                     \
+  stream_scatter_kernel<I, VB><<<sb, kBlock, 0, s>>>(                                        \
+      static_cast<const I*>(a.indices), static_cast<const VB*>(a.values), sorder, sstart,    \
+      spanel, lens_s, lrow_s, dest, dshift, nseg, pp->panel_cols, scol,                     \
+      static_cast<VB*>(sval), seg_lrow)
   if (a.index_bytes == 4) {
     if (vb == 8) ASC_SCATTER(int32_t, uint64_t);
     else if (vb == 4) ASC_SCATTER(int32_t, uint32_t);
@@ -1908,10 +2057,12 @@ hipError_t panel_stream_analysis_launch(const Compressed& a, const StreamLayout&
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if (slices > 0) {
     stream_slice_kernel<<<static_cast<unsigned>((slices + kBlock - 1) / kBlock), kBlock, 0, s>>>(
-        stage, slices, begins, ords, nlong, scol, slice_info);
+        stage, slices, pseg, dest, scol, slice_info);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
-  stream_map_kernel<<<pb, kBlock, 0, s>>>(lens, pairs, ords);
+  const int64_t centries = panels * (L.groups + 1);
+  stream_ctable_kernel<<<static_cast<unsigned>((centries + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+      pseg, lrow_s, panels, L.groups, ctable);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   // The host tables above must outlive their copies.
   if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;

@@ -352,9 +352,12 @@ enum {
    * (panel, row) partial sums in the analysis buffer, so SpMVs through one
    * analysis must not overlap (one context is never used concurrently).
    * Plain accumulation only (ASC_HIP_ACC_TWOSUM is ASC_HIP_UNSUPPORTED).
-   * The size query counts the listed rows, so it reads the matrix's offsets
-   * and waits for the context's stream (a validation-only context returns
-   * the bound for every row listed). */
+   * long_row_min defaults to 1 here: every nonempty row is listed (empty
+   * rows, and rows below an explicit long_row_min, run in the vector pass).
+   * The size query counts the listed rows and bounds their (panel, row)
+   * segments, so it reads the matrix's offsets and waits for the context's
+   * stream (a validation-only context returns the bound for every row
+   * listed). */
   ASC_HIP_STAGING_STREAM = 3
 };
 

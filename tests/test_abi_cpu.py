@@ -248,16 +248,16 @@ def test_panel_analysis_validation(vctx):
     assert size(f32, H.panel_options(50000, staging=H.STAGING_LDS)) == H.INVALID_ARGUMENT
     assert size(big, H.panel_options(0, staging=4)) == H.INVALID_ARGUMENT
     # stream staging: panels of at most 128 KiB of x and 32768 columns; without a
-    # device every row counts as listed (the bound), and (panel, row) pairs
-    # and entries are numbered in 32 bits
+    # device every row counts as listed (the bound), and entries are numbered
+    # in 32 bits
     assert size(big, H.panel_options(0, staging=H.STAGING_STREAM)) == H.OK
     stream_bound = need.value
-    assert stream_bound > 1000 * 616 * 4  # a segment-map entry per (panel, row) at least
+    assert stream_bound > (5000 + 616 * 512) * 10  # the padded stream copy at least
     assert size(big, H.panel_options(16384, staging=H.STAGING_STREAM)) == H.OK
     assert size(big, H.panel_options(16385, staging=H.STAGING_STREAM)) == H.INVALID_ARGUMENT
     assert size(f32, H.panel_options(32768, staging=H.STAGING_STREAM)) == H.OK
     assert size(f32, H.panel_options(32769, staging=H.STAGING_STREAM)) == H.INVALID_ARGUMENT
-    many = dev_csr(rows=5_000_000, cols=10_000_000, nnz=5_000_000)  # 616 panels x 5M rows
+    many = dev_csr(rows=5_000_000, cols=10_000_000, nnz=1 << 32)  # entries past 32 bits
     assert size(many, H.panel_options(0, staging=H.STAGING_STREAM)) == H.UNSUPPORTED
     assert size(dev_csr()) == H.OK  # 4 columns: still 8 (tiny) panels
     assert size(big, H.panel_options(-1)) == H.INVALID_ARGUMENT
