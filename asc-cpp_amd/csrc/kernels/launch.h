@@ -89,22 +89,29 @@ struct PanelPlan {
 // LDS bytes one work-group stages an x panel in (gfx950: 160 KiB per CU).
 constexpr int64_t kLdsPanelBytes = int64_t(160) << 10;
 // Stream form geometry: entries per lane per step, entries per wave step,
-// entries per slice (one wave's share), slices per work-group.
+// entries per slice (one wave's share), slices (streaming waves) per
+// work-group, and the work-group: those waves plus one writer wave.
 constexpr int kStreamEpl = 8;
 constexpr int kStreamTile = 64 * kStreamEpl;
-constexpr int kStreamSlice = 16 * kStreamTile;
-constexpr int kStreamWaves = 16;
-// Segment sums a wave keeps in LDS before writing them out (a power of two),
-// one spare slot per lane beside them, and the LDS left for the x panel.
-constexpr int kStreamRing = 128;
+constexpr int kStreamSlice = 32 * kStreamTile;
+constexpr int kStreamWaves = 7;
+constexpr int kStreamBlock = 64 * (kStreamWaves + 1);
+// Step buffers a streaming wave rotates through: kStreamDepth - 1 steps stay
+// in flight while one is reduced.
+constexpr int kStreamDepth = 4;
+// Segment sums a wave keeps in LDS for the writer wave (a power of two), one
+// spare slot per lane beside them, and the LDS left for the x panel.
+constexpr int kStreamRing = 512;
 constexpr int64_t kStreamPanelBytes = int64_t(128) << 10;
-static_assert(kStreamPanelBytes + int64_t(kStreamWaves) * (kStreamRing + 64) * 8 <=
+static_assert(kStreamPanelBytes + int64_t(kStreamWaves) * ((kStreamRing + 64) * 8 + 3 * 4) <=
                   kLdsPanelBytes,
-              "x panel and segment rings share one work-group's LDS");
+              "x panel, segment rings and ring counters share one work-group's LDS");
 constexpr uint16_t kStreamHead = 0x8000;
 constexpr uint32_t kStreamClosed = 0x80000000u;
-// Listed rows per combine wave (one FP64 LDS accumulator each).
+// Listed rows per combine wave (one FP64 LDS accumulator each), and combine
+// waves per 1024-thread work-group.
 constexpr int kStreamGroupRows = 1024;
+constexpr int kCombineWaves = 16;
 // Widest stream panel: the column field keeps 15 bits beside the head flag.
 constexpr int64_t kStreamMaxPanelCols = 32768;
 

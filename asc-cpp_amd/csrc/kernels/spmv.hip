@@ -656,27 +656,53 @@ __device__ __forceinline__ void count_wave(int& n) {
 }
 
 // Segment sums accumulate in a per-wave circular LDS ring of kStreamRing
-// slots (slot = ordinal mod kStreamRing) and leave for HBM as contiguous
-// bursts, so the stream's loads never wait behind scattered stores.  Ordinals
-// [lo, open) are finished and not yet written; `open` is still accumulating.
+// slots (slot = ordinal mod kStreamRing).  A streaming wave never stores to
+// HBM in its loop: stores and loads share one wait counter on this target, and
+// a store pending beside the prefetched loads would make every wait for a
+// step's data also wait for the steps behind it.  Instead the work-group's
+// writer wave moves finished sums from the rings to HBM.  Per streaming wave,
+// three LDS words hand ordinals over:
+//   pub:  ordinals below it are final in the ring (set by the streaming wave);
+//   con:  ordinals below it are written out, their slots free (writer wave);
+//   done: the streaming wave has published its last ordinal.
+// Live ordinals [con, end] of a wave never span more than a ring.
 typedef __attribute__((address_space(3))) double lds_f64;
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
 
 __device__ __forceinline__ uint32_t ring_slot(uint32_t ordinal) {
   return ordinal & (kStreamRing - 1u);
 }
+// The writer stores whole 128-byte lines of partial sums (16 ordinals) while
+// a streaming wave runs: a line written in two parts at different times can
+// reach HBM as two partial-line writes, each a read-modify-write.
+constexpr uint32_t kLineOrdinals = 16;
 
-// Writes the finished ordinals [lo, upto) and advances lo.  Before a slice's
-// first finished segment `upto` may still lie below lo (a slice opening on a
-// head never writes the ordinal before it).
-__device__ __forceinline__ void stream_flush(lds_f64* ring, uint32_t& lo, uint32_t upto,
-                                             int lane, double* __restrict__ partial) {
-  const int32_t n = static_cast<int32_t>(upto - lo);
-  if (n > 0) {
-    for (int32_t k = lane; k < n; k += kWave)
-      __builtin_nontemporal_store(static_cast<double>(ring[ring_slot(lo + k)]),
-                                  partial + static_cast<int64_t>(lo) + k);
-    lo = upto;
-  }
+struct RingCtl {
+  lds_u32* pub;
+  lds_u32* con;
+  lds_u32* done;
+};
+
+__device__ __forceinline__ uint32_t lds_get(lds_u32* p) {
+  return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(
+      static_cast<int>(*static_cast<volatile lds_u32*>(p))));
+}
+// Lane 0 writes (one LDS word; all lanes hold the same value).
+__device__ __forceinline__ void lds_put(lds_u32* p, uint32_t v, int lane) {
+  if (lane == 0) *static_cast<volatile lds_u32*>(p) = v;
+}
+// Completes this wave's earlier LDS operations before its later ones issue
+// (lgkmcnt(0) only: vector-memory loads stay in flight).
+__device__ __forceinline__ void lds_complete() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0xC07F);
+  asm volatile("" ::: "memory");
+}
+// Completes this wave's vector-memory operations (vmcnt(0)).
+__device__ __forceinline__ void vmem_complete() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0x0F70);
+  asm volatile("" ::: "memory");
 }
 
 // One step: kStreamTile entries from t, kStreamEpl consecutive per lane
@@ -689,9 +715,9 @@ __device__ __forceinline__ void stream_flush(lds_f64* ring, uint32_t& lo, uint32
 //   - the run before the first head (or the whole lane, without heads) is
 //     added to the slot of o0 with an LDS atomic, after all writes.
 // The open segment keeps accumulating in its slot from step to step.  A step
-// that finishes at least a ring of segments (about 4 entries per segment or
-// fewer) writes them to HBM directly instead, with a segmented scan of the
-// lanes' runs.
+// whose segments would not fit in the ring even once the writer has emptied
+// it (more than a ring of segments in 512 entries) writes them to HBM
+// directly instead, with a segmented scan of the lanes' runs.
 struct StepHeads {
   uint32_t hmask;  // bit j: entry j opens a segment
   int own;         // heads in the lane
@@ -700,15 +726,13 @@ struct StepHeads {
   bool direct;     // the step writes its segments straight to HBM
 };
 
-// The step's heads and ordinals, and room in the ring (flushing finished
-// segments before the next step's loads are issued, so the stores drain
-// while those loads are in flight).
+// The step's heads and ordinals.  Publishes the ordinals the previous steps
+// finished, and waits for the writer while the step would wrap the ring onto
+// ordinals it has not written out yet (con_c caches the writer's progress).
 template <bool FULL, typename V>
 __device__ __forceinline__ StepHeads stream_heads(const StreamStep<V>& cur, int64_t t,
-                                                  int64_t s0, int64_t s1, int lane,
-                                                  lds_f64* ring, uint32_t& lo,
-                                                  double* __restrict__ partial,
-                                                  uint32_t open) {
+                                                  int64_t s1, int lane, const RingCtl& ctl,
+                                                  uint32_t open, uint32_t& con_c) {
   const int64_t e = t + kStreamEpl * lane;
   uint32_t hmask = 0;  // bit j: entry j opens a segment
 #pragma unroll
@@ -723,17 +747,31 @@ __device__ __forceinline__ StepHeads stream_heads(const StreamStep<V>& cur, int6
   count_wave(incl);
   const uint32_t total = static_cast<uint32_t>(__builtin_amdgcn_readlane(incl, kWave - 1));
   const uint32_t o0 = open + static_cast<uint32_t>(incl - own);
-  const bool h0 = (hmask & 1u) != 0;
-  // A slice that opens on a head leaves the ordinal before it to the
-  // previous slice.
-  if (t == s0) lo = open + static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(h0 ? 1 : 0));
-  // Live ordinals [lo, end] share the ring: flush the finished ones when the
-  // step would wrap onto them.
+  // One wave's LDS operations take effect in issue order, so the ring updates
+  // of the previous steps land before this publication (the compiler is kept
+  // from moving them past it).
+  asm volatile("" ::: "memory");
+  lds_put(ctl.pub, open, lane);
   const uint32_t end = open + total;
+  // Slots in use run from the oldest ordinal the writer still owns (con; a
+  // slice that opens on a head starts with con one past open) or the open
+  // segment, whichever is older, to end.  With pub at open the writer always
+  // catches up to open's line (it keeps back less than a line); a step that
+  // still does not fit goes direct.
+  const auto span = [&](uint32_t c) {
+    return end - (static_cast<int32_t>(c - open) < 0 ? c : open);
+  };
   bool direct = false;
-  if (end - lo >= static_cast<uint32_t>(kStreamRing)) {
-    stream_flush(ring, lo, open, lane, partial);
-    direct = end - lo >= static_cast<uint32_t>(kStreamRing);
+  if (span(con_c) >= static_cast<uint32_t>(kStreamRing)) {
+    const uint32_t line = open & ~(kLineOrdinals - 1u);
+    for (;;) {
+      con_c = lds_get(ctl.con);
+      if (span(con_c) < static_cast<uint32_t>(kStreamRing) ||
+          static_cast<int32_t>(con_c - line) >= 0)
+        break;
+      __builtin_amdgcn_s_sleep(1);
+    }
+    direct = span(con_c) >= static_cast<uint32_t>(kStreamRing);
   }
   StepHeads hd;
   hd.hmask = hmask;
@@ -747,8 +785,9 @@ __device__ __forceinline__ StepHeads stream_heads(const StreamStep<V>& cur, int6
 template <bool FULL, typename V, typename A>
 __device__ __forceinline__ void stream_values(const StreamStep<V>& cur, const StepHeads& hd,
                                               const A* sx, int64_t t, int64_t s0, int64_t s1,
-                                              int lane, lds_f64* ring, uint32_t& lo,
-                                              double* __restrict__ partial, uint32_t& open) {
+                                              int lane, lds_f64* ring, const RingCtl& ctl,
+                                              double* __restrict__ partial, uint32_t& open,
+                                              uint32_t& con_c) {
   const int64_t e = t + kStreamEpl * lane;
   const uint32_t hmask = hd.hmask;
   const int own = hd.own;
@@ -774,6 +813,13 @@ __device__ __forceinline__ void stream_values(const StreamStep<V>& cur, const St
     __hip_atomic_fetch_add(&ring[ring_slot(o0)], static_cast<double>(own > 0 ? lead : s),
                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
   } else {
+    // The writer has written all it will of this wave's ring (up to con, the
+    // line of open): this wave writes the rest of [con, open) and the step's
+    // ordinals itself, and hands the ring back at end.
+    const int32_t kept = static_cast<int32_t>(open - con_c);
+    for (int32_t k = lane; k < kept; k += kWave)
+      partial[con_c + static_cast<uint32_t>(k)] =
+          static_cast<double>(ring[ring_slot(con_c + static_cast<uint32_t>(k))]);
     const A carry_in = static_cast<A>(ring[ring_slot(open)]);
 #pragma unroll
     for (int j = 0; j < kStreamEpl; ++j) {
@@ -795,20 +841,78 @@ __device__ __forceinline__ void stream_values(const StreamStep<V>& cur, const St
     const int next_h0 = dpp_move<kDppWaveShl1, 0xF>(h0 ? 1 : 0, 0);
     // The run before the first head closes there (a head on entry 0 closes
     // the segment the lane to the left carried, written by that lane, or for
-    // lane 0 the open segment of the previous step).
+    // lane 0 the open segment of the previous step; a slice that opens on a
+    // head leaves that segment to the previous slice).
     if (own > 0 && (!h0 || (lane == 0 && t > s0)))
       partial[o0] = static_cast<double>(from_left + lead);
     // The trailing run closes when the next lane opens on a head.
     if (next_h0 != 0) partial[o0 + own] = static_cast<double>(own > 0 ? s : from_left + s);
     const A open_sum = read_lane(sc, kWave - 1);
     if (lane == 0) ring[ring_slot(end)] = static_cast<double>(open_sum);
-    lo = end;
+    // These stores complete here, so no store is pending at the next step's
+    // wait for its data.
+    vmem_complete();
+    lds_complete();
+    lds_put(ctl.con, end, lane);  // before pub: the writer reads pub first
+    lds_put(ctl.pub, end, lane);
+    con_c = end;
   }
   open = end;
 }
 
+// The writer wave: moves each streaming wave's published ordinals from its
+// ring to HBM as contiguous bursts until every streaming wave is done.
+__device__ __forceinline__ void stream_writer(int waves, int lane, const double* sring,
+                                              lds_u32* pub, lds_u32* con, lds_u32* done,
+                                              double* __restrict__ partial) {
+  uint32_t finished = 0;
+  const uint32_t all = (1u << waves) - 1u;
+  const int w_of = lane < kStreamWaves ? lane : 0;
+  while (finished != all) {
+    // Lane w reads wave w's done word, then (after it returns) pub and con:
+    // pub is final once done was seen set.
+    const uint32_t dv = *static_cast<volatile lds_u32*>(done + w_of);
+    lds_complete();
+    const uint32_t pv = *static_cast<volatile lds_u32*>(pub + w_of);
+    const uint32_t cv = *static_cast<volatile lds_u32*>(con + w_of);
+    bool worked = false;
+    for (int w = 0; w < waves; ++w) {
+      if ((finished >> w) & 1u) continue;
+      const uint32_t d = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(dv), w));
+      const uint32_t p = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(pv), w));
+      const uint32_t c = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(cv), w));
+      // Whole lines only until the wave is done (pub is then final).
+      const uint32_t upto = d != 0 ? p : p & ~(kLineOrdinals - 1u);
+      if (static_cast<int32_t>(upto - c) > 0) {
+        const lds_f64* ring = (const lds_f64*)(sring + w * (kStreamRing + kWave));
+        const uint32_t n = upto - c;
+        // Four 64-ordinal chunks per pass: their LDS reads go out together,
+        // then their stores (one LDS round trip per 256 ordinals).
+        for (uint32_t k0 = 0; k0 < n; k0 += 4 * kWave) {
+          double v[4];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const uint32_t k = k0 + static_cast<uint32_t>(lane + j * kWave);
+            v[j] = k < n ? static_cast<double>(ring[ring_slot(c + k)]) : 0.0;
+          }
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const uint32_t k = k0 + static_cast<uint32_t>(lane + j * kWave);
+            if (k < n) __builtin_nontemporal_store(v[j], partial + static_cast<int64_t>(c) + k);
+          }
+        }
+        lds_complete();  // the slots are read before they are handed back
+        lds_put(con + w, upto, lane);
+        worked = true;
+      }
+      if (d != 0) finished |= 1u << w;
+    }
+    if (!worked) __builtin_amdgcn_s_sleep(2);
+  }
+}
+
 template <typename V>
-__global__ __launch_bounds__(kLdsBlock) void spmv_stream_kernel(
+__global__ __launch_bounds__(kStreamBlock) void spmv_stream_kernel(
     const typename acc_of<V>::type* __restrict__ x, int64_t incx, int64_t cols,
     int64_t panel_cols, const uint16_t* __restrict__ scol, const V* __restrict__ sval,
     const uint32_t* __restrict__ slice_info, const uint32_t* __restrict__ wg_table,
@@ -816,117 +920,110 @@ __global__ __launch_bounds__(kLdsBlock) void spmv_stream_kernel(
   using A = typename acc_of<V>::type;
   __shared__ A sx[kStreamPanelBytes / sizeof(A)];
   __shared__ double sring[kStreamWaves * (kStreamRing + kWave)];
+  __shared__ uint32_t sctl[3 * kStreamWaves];
   const uint32_t first = wg_table[2 * blockIdx.x];
   const uint32_t count = wg_table[2 * blockIdx.x + 1];
   const int64_t q = slice_info[4 * static_cast<int64_t>(first) + 3] & ~kStreamClosed;
-  const int wave = threadIdx.x / kWave;
+  // Wave-uniform as far as the compiler can see, so the slice bounds and the
+  // step loop's exit stay scalar (a divergent exit would merge every step
+  // buffer at the loop latch).
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / kWave));
   const int lane = threadIdx.x % kWave;
-  const bool active = wave < static_cast<int>(count);
+  const bool active = wave < static_cast<int>(count);  // the writer wave never is
   const int64_t slice = static_cast<int64_t>(first) + (active ? wave : 0);
   const uint32_t* si = slice_info + 4 * slice;
   const int64_t s0 = si[0], s1 = si[1];
-  // The slice's first step loads while the panel of x is staged.  Slices end
-  // inside their panel's padded range, so a step's loads stay in the
-  // allocation even past the slice's end.
-  StreamStep<V> b0, b1, b2;
+  lds_u32* pub = (lds_u32*)(sctl);  // address-space casts
+  lds_u32* con = (lds_u32*)(sctl + kStreamWaves);
+  lds_u32* done = (lds_u32*)(sctl + 2 * kStreamWaves);
+  // The first steps load while the panel of x is staged.  A wave loads
+  // kStreamDepth - 1 steps ahead, clamped to its slice's last step (the
+  // repeated loads of that step hit in cache and are never used).
+  const int64_t s_last = s0 + (s1 - 1 - s0) / kStreamTile * kStreamTile;
+  static_assert(kStreamDepth == 4, "the step rotation below uses four buffers");
+  StreamStep<V> b0, b1, b2, b3;
+  uint32_t first_col = 0;  // the slice's first entry (its head flag)
   if (active) {
+    first_col = scol[s0];
     stream_load(scol, sval, s0, lane, b0);
-    if (sizeof(V) < 8 && s0 + kStreamTile < s1)
-      stream_load(scol, sval, s0 + kStreamTile, lane, b1);
+    stream_load(scol, sval, min(s0 + kStreamTile, s_last), lane, b1);
+    stream_load(scol, sval, min(s0 + 2 * kStreamTile, s_last), lane, b2);
+    lds_put(pub + wave, si[2], lane);
+    lds_put(con + wave, si[2], lane);
+    lds_put(done + wave, 0u, lane);
   }
   const int64_t col0 = q * panel_cols;
   const int64_t width = cols - col0 < panel_cols ? cols - col0 : panel_cols;
   const A* __restrict__ xp = x + col0 * incx;
   int64_t k = threadIdx.x;
-  for (; k + 3 * kLdsBlock < width; k += 4 * kLdsBlock) {
+  for (; k + 3 * kStreamBlock < width; k += 4 * kStreamBlock) {
     const A x0 = xp[k * incx];
-    const A x1 = xp[(k + kLdsBlock) * incx];
-    const A x2 = xp[(k + 2 * kLdsBlock) * incx];
-    const A x3 = xp[(k + 3 * kLdsBlock) * incx];
+    const A x1 = xp[(k + kStreamBlock) * incx];
+    const A x2 = xp[(k + 2 * kStreamBlock) * incx];
+    const A x3 = xp[(k + 3 * kStreamBlock) * incx];
     sx[k] = x0;
-    sx[k + kLdsBlock] = x1;
-    sx[k + 2 * kLdsBlock] = x2;
-    sx[k + 3 * kLdsBlock] = x3;
+    sx[k + kStreamBlock] = x1;
+    sx[k + 2 * kStreamBlock] = x2;
+    sx[k + 3 * kStreamBlock] = x3;
   }
-  for (; k < width; k += kLdsBlock) sx[k] = xp[k * incx];
+  for (; k < width; k += kStreamBlock) sx[k] = xp[k * incx];
   __syncthreads();
+  if (wave == kStreamWaves) {
+    stream_writer(static_cast<int>(count < kStreamWaves ? count : kStreamWaves), lane, sring, pub,
+                  con, done, partial);
+    return;
+  }
   if (!active) return;  // whole waves leave together
   const bool closed = (si[3] & kStreamClosed) != 0;
   lds_f64* ring = (lds_f64*)(sring + wave * (kStreamRing + kWave));  // address-space cast
+  const RingCtl ctl{pub + wave, con + wave, done + wave};
   uint32_t open = si[2];
-  uint32_t lo = si[2];
-  // The segment open at the slice's start gets no first write in this slice.
+  // The segment open at the slice's start gets no first write in this slice;
+  // a slice that opens on a head leaves that segment to the previous slice,
+  // so the writer starts after it.
   if (lane == 0) ring[ring_slot(open)] = 0.0;
-  // Full steps with the next one (or two) in flight; the last, partial step
-  // of the slice after the loop.
+  uint32_t con_c = open + ((first_col & kStreamHead) != 0 ? 1u : 0u);
+  lds_put(con + wave, con_c, lane);  // before any publication
+  // Full steps, each claiming its buffer and sending the load kStreamDepth-1
+  // steps ahead into the buffer the previous step used; the last, partial
+  // step of the slice after the loop.
   int64_t t = s0;
-  if constexpr (sizeof(V) == 8) {
-    // FP64 steps hold twice the registers: one step in flight, two buffers.
-    while (t + kStreamTile < s1) {
-      stream_claim(b0);
-      {
-        const StepHeads hd = stream_heads<true, V>(b0, t, s0, s1, lane, ring, lo, partial, open);
-        stream_load(scol, sval, t + kStreamTile, lane, b1);
-        stream_values<true, V, A>(b0, hd, sx, t, s0, s1, lane, ring, lo, partial, open);
-      }
-      t += kStreamTile;
-      if (t + kStreamTile >= s1) {
-        b0 = b1;
-        break;
-      }
-      stream_claim(b1);
-      {
-        const StepHeads hd = stream_heads<true, V>(b1, t, s0, s1, lane, ring, lo, partial, open);
-        stream_load(scol, sval, t + kStreamTile, lane, b0);
-        stream_values<true, V, A>(b1, hd, sx, t, s0, s1, lane, ring, lo, partial, open);
-      }
-      t += kStreamTile;
-    }
-  } else {
-    // Two steps in flight; the three buffers rotate roles.
-    while (t + kStreamTile < s1) {
-      stream_claim(b0);
-      {
-        const StepHeads hd = stream_heads<true, V>(b0, t, s0, s1, lane, ring, lo, partial, open);
-        if (t + 2 * kStreamTile < s1) stream_load(scol, sval, t + 2 * kStreamTile, lane, b2);
-        stream_values<true, V, A>(b0, hd, sx, t, s0, s1, lane, ring, lo, partial, open);
-      }
-      t += kStreamTile;
-      if (t + kStreamTile >= s1) {
-        b0 = b1;
-        break;
-      }
-      stream_claim(b1);
-      {
-        const StepHeads hd = stream_heads<true, V>(b1, t, s0, s1, lane, ring, lo, partial, open);
-        if (t + 2 * kStreamTile < s1) stream_load(scol, sval, t + 2 * kStreamTile, lane, b0);
-        stream_values<true, V, A>(b1, hd, sx, t, s0, s1, lane, ring, lo, partial, open);
-      }
-      t += kStreamTile;
-      if (t + kStreamTile >= s1) {
-        b0 = b2;
-        break;
-      }
-      stream_claim(b2);
-      {
-        const StepHeads hd = stream_heads<true, V>(b2, t, s0, s1, lane, ring, lo, partial, open);
-        if (t + 2 * kStreamTile < s1) stream_load(scol, sval, t + 2 * kStreamTile, lane, b1);
-        stream_values<true, V, A>(b2, hd, sx, t, s0, s1, lane, ring, lo, partial, open);
-      }
-      t += kStreamTile;
-    }
+  StreamStep<V> last;
+#define ASC_STREAM_STEP(CUR, NEXT)                                                          \
+  if (t + kStreamTile >= s1) {                                                              \
+    last = CUR;                                                                             \
+    goto tail;                                                                              \
+  }                                                                                         \
+  stream_claim(CUR);                                                                        \
+  {                                                                                         \
+    const StepHeads hd = stream_heads<true, V>(CUR, t, s1, lane, ctl, open, con_c);         \
+    stream_load(scol, sval, min(t + 3 * kStreamTile, s_last), lane, NEXT);                  \
+    stream_values<true, V, A>(CUR, hd, sx, t, s0, s1, lane, ring, ctl, partial, open, con_c); \
+  }                                                                                         \
+  t += kStreamTile;
+  for (;;) {
+    ASC_STREAM_STEP(b0, b3)
+    ASC_STREAM_STEP(b1, b0)
+    ASC_STREAM_STEP(b2, b1)
+    ASC_STREAM_STEP(b3, b2)
   }
-  stream_claim(b0);
+#undef ASC_STREAM_STEP
+tail:
+  stream_claim(last);
   {
-    const StepHeads hd = stream_heads<false, V>(b0, t, s0, s1, lane, ring, lo, partial, open);
-    stream_values<false, V, A>(b0, hd, sx, t, s0, s1, lane, ring, lo, partial, open);
+    const StepHeads hd = stream_heads<false, V>(last, t, s1, lane, ctl, open, con_c);
+    stream_values<false, V, A>(last, hd, sx, t, s0, s1, lane, ring, ctl, partial, open, con_c);
   }
-  stream_flush(ring, lo, open, lane, partial);
+  lds_complete();
+  lds_put(ctl.pub, open, lane);
+  lds_complete();
+  lds_put(ctl.done, 1u, lane);
   if (lane == 0) {
     const double v = ring[ring_slot(open)];
     if (closed) partial[open] = v;
     else carry[slice] = v;
   }
+  vmem_complete();  // the unused prefetches land before the wave ends
 }
 
 // Adds the parts of segments that cross slice ends: the first slice of each
@@ -961,10 +1058,10 @@ __global__ __launch_bounds__(kLdsBlock) void spmv_stream_combine_kernel(
     const uint32_t* __restrict__ ctable, int64_t groups, const uint16_t* __restrict__ seg_lrow,
     const double* __restrict__ partial) {
   using Acc = typename AccSel<V, false>::type;
-  __shared__ double sacc[kStreamWaves * kStreamGroupRows];
+  __shared__ double sacc[kCombineWaves * kStreamGroupRows];
   const int wave = threadIdx.x / kWave;
   const int lane = threadIdx.x % kWave;
-  const int64_t g = static_cast<int64_t>(blockIdx.x) * kStreamWaves + wave;
+  const int64_t g = static_cast<int64_t>(blockIdx.x) * kCombineWaves + wave;
   if (g >= groups) return;  // no barrier follows
   lds_f64* acc = (lds_f64*)(sacc + wave * kStreamGroupRows);  // address-space cast
   for (int k = lane; k < kStreamGroupRows; k += kWave) acc[k] = 0.0;
@@ -1571,7 +1668,7 @@ hipError_t launch_stream_forked(const SpmvArgs& a, const KParams& kp, void* ws, 
   const bool shorts = pp.nlong < a.a.outer;
   hipError_t e = shorts ? hipEventRecord(side.fork, s) : hipSuccess;
   if (e != hipSuccess) return e;
-  spmv_stream_kernel<V><<<static_cast<unsigned>(pp.wgs), kLdsBlock, 0, s>>>(
+  spmv_stream_kernel<V><<<static_cast<unsigned>(pp.wgs), kStreamBlock, 0, s>>>(
       static_cast<const typename acc_of<V>::type*>(kp.x), kp.incx, a.a.inner, pp.panel_cols,
       pp.scol, static_cast<const V*>(pp.sval), pp.slice_info, pp.wg_table, pp.spartial, carry);
   e = hipGetLastError();
@@ -1589,7 +1686,7 @@ hipError_t launch_stream_forked(const SpmvArgs& a, const KParams& kp, void* ws, 
   }
   if (e == hipSuccess) {
     spmv_stream_combine_kernel<V>
-        <<<static_cast<unsigned>((pp.groups + kStreamWaves - 1) / kStreamWaves), kLdsBlock, 0, s>>>(
+        <<<static_cast<unsigned>((pp.groups + kCombineWaves - 1) / kCombineWaves), kLdsBlock, 0, s>>>(
             kp, pp.long_rows, pp.nlong, pp.panels, pp.ctable, pp.groups, pp.seg_lrow, pp.spartial);
     e = hipGetLastError();
   }
@@ -1685,7 +1782,7 @@ hipError_t launch_typed(const SpmvArgs& a, const KParams& kp, void* ws,
           return hipErrorInvalidValue;  // the stream form accumulates plainly
         } else {
           double* carry = static_cast<double*>(ws);
-          spmv_stream_kernel<V><<<static_cast<unsigned>(pp.wgs), kLdsBlock, 0, s>>>(
+          spmv_stream_kernel<V><<<static_cast<unsigned>(pp.wgs), kStreamBlock, 0, s>>>(
               static_cast<const typename acc_of<V>::type*>(kp.x), kp.incx, a.a.inner,
               pp.panel_cols, pp.scol, static_cast<const V*>(pp.sval), pp.slice_info,
               pp.wg_table, pp.spartial, carry);
@@ -1696,7 +1793,7 @@ hipError_t launch_typed(const SpmvArgs& a, const KParams& kp, void* ws,
           e = hipGetLastError();
           if (e != hipSuccess) return e;
           spmv_stream_combine_kernel<V>
-              <<<static_cast<unsigned>((pp.groups + kStreamWaves - 1) / kStreamWaves), kLdsBlock,
+              <<<static_cast<unsigned>((pp.groups + kCombineWaves - 1) / kCombineWaves), kLdsBlock,
                  0, s>>>(kp, pp.long_rows, pp.nlong, pp.panels, pp.ctable, pp.groups,
                          pp.seg_lrow, pp.spartial);
           return hipGetLastError();
@@ -1844,7 +1941,7 @@ StreamLayout stream_layout(int64_t rows, int64_t nlong, int64_t nnz_long, int64_
   L.nnz_long = nnz_long;
   L.panels = panels;
   L.groups = (nlong + kStreamGroupRows - 1) / kStreamGroupRows;
-  L.entries_cap = nnz_long + panels * kStreamTile;
+  L.entries_cap = nnz_long + panels * kStreamTile;  // entries plus per-panel padding
   L.slices_cap = L.entries_cap / kStreamSlice + panels + 1;
   L.segs_cap = segs_cap;
   const size_t segs = static_cast<size_t>(segs_cap > 0 ? segs_cap : 1);
