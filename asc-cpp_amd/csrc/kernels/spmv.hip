@@ -1066,37 +1066,61 @@ __global__ __launch_bounds__(kLdsBlock) void spmv_stream_combine_kernel(
   lds_f64* acc = (lds_f64*)(sacc + wave * kStreamGroupRows);  // address-space cast
   for (int k = lane; k < kStreamGroupRows; k += kWave) acc[k] = 0.0;
   const int64_t stride = groups + 1;
-  constexpr int kB = 8;  // panels (a multiple of 8) per batch of loads
-  for (int64_t q0 = 0; q0 < panels; q0 += kB) {
+  // Lane l loads the ordinal range of panel qq + l once per 64 panels; the
+  // panels then go in batches of 8 (panels is a multiple of 8), each batch's
+  // sums and row tags loading while the previous batch is added.
+  constexpr int kB = 8;
+  struct Batch {
     uint32_t lo[kB], n[kB];
-#pragma unroll
-    for (int j = 0; j < kB; ++j) {
-      lo[j] = ctable[(q0 + j) * stride + g];
-      n[j] = ctable[(q0 + j) * stride + g + 1] - lo[j];
-    }
-    // Up to two segments per lane per panel load together; longer ranges
-    // (more than 128 of the group's rows in one panel) finish after.
     double v[kB][2];
     uint16_t r[kB][2];
+  };
+  const auto load = [&](Batch& b, uint32_t lo_r, uint32_t hi_r, int j0) {
+#pragma unroll
+    for (int j = 0; j < kB; ++j) {
+      b.lo[j] = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(lo_r), j0 + j));
+      b.n[j] = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(hi_r), j0 + j)) -
+               b.lo[j];
+    }
 #pragma unroll
     for (int j = 0; j < kB; ++j) {
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
         const uint32_t k = static_cast<uint32_t>(lane + u * kWave);
-        v[j][u] = k < n[j] ? partial[lo[j] + k] : 0.0;
-        r[j][u] = k < n[j] ? seg_lrow[lo[j] + k] : static_cast<uint16_t>(0);
+        b.v[j][u] = k < b.n[j] ? partial[b.lo[j] + k] : 0.0;
+        b.r[j][u] = k < b.n[j] ? seg_lrow[b.lo[j] + k] : static_cast<uint16_t>(0);
       }
     }
+  };
+  // Up to two segments per lane per panel come from the batch; longer ranges
+  // (more than 128 of the group's rows in one panel) finish here.
+  const auto add = [&](const Batch& b) {
 #pragma unroll
     for (int j = 0; j < kB; ++j) {
 #pragma unroll
       for (int u = 0; u < 2; ++u)
-        if (static_cast<uint32_t>(lane + u * kWave) < n[j])
-          __hip_atomic_fetch_add(&acc[r[j][u]], v[j][u], __ATOMIC_RELAXED,
+        if (static_cast<uint32_t>(lane + u * kWave) < b.n[j])
+          __hip_atomic_fetch_add(&acc[b.r[j][u]], b.v[j][u], __ATOMIC_RELAXED,
                                  __HIP_MEMORY_SCOPE_WAVEFRONT);
-      for (uint32_t k = lane + 2 * kWave; k < n[j]; k += kWave)
-        __hip_atomic_fetch_add(&acc[seg_lrow[lo[j] + k]], partial[lo[j] + k], __ATOMIC_RELAXED,
+      for (uint32_t k = lane + 2 * kWave; k < b.n[j]; k += kWave)
+        __hip_atomic_fetch_add(&acc[seg_lrow[b.lo[j] + k]], partial[b.lo[j] + k], __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_WAVEFRONT);
+    }
+  };
+  for (int64_t qq = 0; qq < panels; qq += kWave) {
+    const int64_t q = qq + lane;
+    const uint32_t lo_r = q < panels ? ctable[q * stride + g] : 0u;
+    const uint32_t hi_r = q < panels ? ctable[q * stride + g + 1] : 0u;
+    const int64_t left = panels - qq;
+    const int nb = static_cast<int>((left < kWave ? left : kWave) / kB);
+    Batch A, B;
+    load(A, lo_r, hi_r, 0);
+    for (int bi = 0; bi < nb; bi += 2) {
+      if (bi + 1 < nb) load(B, lo_r, hi_r, (bi + 1) * kB);
+      add(A);
+      if (bi + 1 >= nb) break;
+      if (bi + 2 < nb) load(A, lo_r, hi_r, (bi + 2) * kB);
+      add(B);
     }
   }
   const int64_t base = g * kStreamGroupRows;
