@@ -112,6 +112,8 @@ constexpr uint32_t kStreamClosed = 0x80000000u;
 // waves per 1024-thread work-group.
 constexpr int kStreamGroupRows = 1024;
 constexpr int kCombineWaves = 16;
+// Combine work-groups wanted at least (one per CU of an MI355X).
+constexpr int64_t kCombineTargetBlocks = 256;
 // Widest stream panel: the column field keeps 15 bits beside the head flag.
 constexpr int64_t kStreamMaxPanelCols = 32768;
 

@@ -1045,31 +1045,40 @@ __global__ __launch_bounds__(kBlock) void spmv_stream_fixup_kernel(
   partial[target] += sum;
 }
 
-// Row epilogue of the stream form.  Wave g owns listed rows [g*R, g*R + R)
-// (R = kStreamGroupRows) with one FP64 accumulator per row in LDS.  In each
-// panel the segments of those rows form one contiguous ordinal range,
-// ctable[q*(groups+1) + g] .. [q*(groups+1) + g + 1], and each row has at
-// most one segment there; the wave walks the panels in order, adding every
-// segment's sum to its row, so a row's sum is formed in panel order (no
-// barriers, no atomics between waves: the result is the same every run).
+// Row epilogue of the stream form.  Group g is listed rows [g*R, g*R + R)
+// (R = kStreamGroupRows).  In each panel the segments of those rows form one
+// contiguous ordinal range, ctable[q*(groups+1) + g] .. [q*(groups+1) + g + 1],
+// and each row has at most one segment there.  A work-group takes gpb
+// adjacent groups (adjacent ranges share cache lines), 16 / gpb waves per
+// group; each wave takes a contiguous run of its group's panels (a multiple
+// of 8) and adds every segment's sum to its row in an LDS accumulator of its
+// own, panel by panel, and the rows then sum their waves' accumulators in
+// wave order.  The order is fixed, so the result is the same every run.
 template <typename V>
 __global__ __launch_bounds__(kLdsBlock) void spmv_stream_combine_kernel(
     KParams p, const uint32_t* __restrict__ long_rows, int64_t nlong, int64_t panels,
     const uint32_t* __restrict__ ctable, int64_t groups, const uint16_t* __restrict__ seg_lrow,
-    const double* __restrict__ partial) {
+    const double* __restrict__ partial, int gpb) {
   using Acc = typename AccSel<V, false>::type;
+  static_assert(kLdsBlock == kCombineWaves * kWave, "one accumulator per combine wave");
   __shared__ double sacc[kCombineWaves * kStreamGroupRows];
-  const int wave = threadIdx.x / kWave;
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / kWave));
   const int lane = threadIdx.x % kWave;
-  const int64_t g = static_cast<int64_t>(blockIdx.x) * kCombineWaves + wave;
-  if (g >= groups) return;  // no barrier follows
+  const int wpg = kCombineWaves / gpb;  // waves per group
+  const int64_t g = static_cast<int64_t>(blockIdx.x) * gpb + wave / wpg;
+  const int part = wave % wpg;
   lds_f64* acc = (lds_f64*)(sacc + wave * kStreamGroupRows);  // address-space cast
   for (int k = lane; k < kStreamGroupRows; k += kWave) acc[k] = 0.0;
   const int64_t stride = groups + 1;
-  // Lane l loads the ordinal range of panel qq + l once per 64 panels; the
-  // panels then go in batches of 8 (panels is a multiple of 8), each batch's
-  // sums and row tags loading while the previous batch is added.
+  // This wave's panels [qa, qb): runs of a multiple of 8 (panels is one);
+  // none past the last group.
   constexpr int kB = 8;
+  const int64_t per = ((panels + wpg - 1) / wpg + kB - 1) / kB * kB;
+  const int64_t qa = g < groups && part * per < panels ? part * per : panels;
+  const int64_t qb = qa + per < panels ? qa + per : panels;
+  // Lane l loads the ordinal range of panel qq + l once per 64 panels; the
+  // panels then go in batches of 8, each batch's sums and row tags loading
+  // while the previous batch is added.
   struct Batch {
     uint32_t lo[kB], n[kB];
     double v[kB][2];
@@ -1107,11 +1116,11 @@ __global__ __launch_bounds__(kLdsBlock) void spmv_stream_combine_kernel(
                                __HIP_MEMORY_SCOPE_WAVEFRONT);
     }
   };
-  for (int64_t qq = 0; qq < panels; qq += kWave) {
+  for (int64_t qq = qa; qq < qb; qq += kWave) {
     const int64_t q = qq + lane;
-    const uint32_t lo_r = q < panels ? ctable[q * stride + g] : 0u;
-    const uint32_t hi_r = q < panels ? ctable[q * stride + g + 1] : 0u;
-    const int64_t left = panels - qq;
+    const uint32_t lo_r = q < qb ? ctable[q * stride + g] : 0u;
+    const uint32_t hi_r = q < qb ? ctable[q * stride + g + 1] : 0u;
+    const int64_t left = qb - qq;
     const int nb = static_cast<int>((left < kWave ? left : kWave) / kB);
     Batch A, B;
     load(A, lo_r, hi_r, 0);
@@ -1123,9 +1132,22 @@ __global__ __launch_bounds__(kLdsBlock) void spmv_stream_combine_kernel(
       add(B);
     }
   }
-  const int64_t base = g * kStreamGroupRows;
-  for (int k = lane; k < kStreamGroupRows && base + k < nlong; k += kWave)
-    finish_row<V>(p, long_rows[base + k], Acc::from_f64(acc[k]));
+  __syncthreads();
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * gpb * kStreamGroupRows;
+  for (int k = threadIdx.x; k < gpb * kStreamGroupRows && base + k < nlong; k += kLdsBlock) {
+    const int gi = k / kStreamGroupRows, r = k % kStreamGroupRows;
+    double sum = 0.0;
+    for (int w = gi * wpg; w < gi * wpg + wpg; ++w) sum += sacc[w * kStreamGroupRows + r];
+    finish_row<V>(p, long_rows[base + k], Acc::from_f64(sum));
+  }
+}
+
+// Groups per combine work-group: the most (up to 16, one wave each) that
+// still give a work-group per CU.
+int combine_groups_per_block(int64_t groups) {
+  int gpb = kCombineWaves;
+  while (gpb > 1 && (groups + gpb - 1) / gpb < kCombineTargetBlocks) gpb /= 2;
+  return gpb;
 }
 
 // ---- stream-form analysis -------------------------------------------------
@@ -1709,9 +1731,11 @@ hipError_t launch_stream_forked(const SpmvArgs& a, const KParams& kp, void* ws, 
     e = hipGetLastError();
   }
   if (e == hipSuccess) {
+    const int gpb = combine_groups_per_block(pp.groups);
     spmv_stream_combine_kernel<V>
-        <<<static_cast<unsigned>((pp.groups + kCombineWaves - 1) / kCombineWaves), kLdsBlock, 0, s>>>(
-            kp, pp.long_rows, pp.nlong, pp.panels, pp.ctable, pp.groups, pp.seg_lrow, pp.spartial);
+        <<<static_cast<unsigned>((pp.groups + gpb - 1) / gpb), kLdsBlock, 0, s>>>(
+            kp, pp.long_rows, pp.nlong, pp.panels, pp.ctable, pp.groups, pp.seg_lrow, pp.spartial,
+            gpb);
     e = hipGetLastError();
   }
   const hipError_t ew =
@@ -1816,10 +1840,11 @@ hipError_t launch_typed(const SpmvArgs& a, const KParams& kp, void* ws,
               pp.slice_info, pp.slices, carry, pp.spartial);
           e = hipGetLastError();
           if (e != hipSuccess) return e;
+          const int gpb = combine_groups_per_block(pp.groups);
           spmv_stream_combine_kernel<V>
-              <<<static_cast<unsigned>((pp.groups + kCombineWaves - 1) / kCombineWaves), kLdsBlock,
-                 0, s>>>(kp, pp.long_rows, pp.nlong, pp.panels, pp.ctable, pp.groups,
-                         pp.seg_lrow, pp.spartial);
+              <<<static_cast<unsigned>((pp.groups + gpb - 1) / gpb), kLdsBlock, 0, s>>>(
+                  kp, pp.long_rows, pp.nlong, pp.panels, pp.ctable, pp.groups, pp.seg_lrow,
+                  pp.spartial, gpb);
           return hipGetLastError();
         }
       }

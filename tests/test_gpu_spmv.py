@@ -397,6 +397,38 @@ def test_stream_plan_parity(ctx, name, dtype):
     H.plan_clear(ctx)
 
 
+@pytest.mark.parametrize("groups", [300, 600, 1100, 2100, 4200])
+def test_stream_plan_combine_shapes(ctx, groups):
+    # The combine gives a work-group 16 / w adjacent groups of 1024 listed
+    # rows, w waves splitting each group's panels, with w the fewest that
+    # still yield a work-group per CU: 300 groups run one group per
+    # work-group, 600 / 1100 / 2100 / 4200 run two, four, eight and sixteen
+    # (the 2^22-row Zipf matrix, 4096 groups, runs sixteen).
+    rng = np.random.default_rng(groups)
+    rows, cols = groups * 1024, 1 << 17
+    r = np.repeat(np.arange(rows, dtype=np.int64), rng.integers(1, 5, rows))
+    key = np.unique(r * cols + rng.integers(0, cols, r.size))  # sorted, distinct per row
+    idx = key % cols
+    offsets = np.concatenate([[0], np.cumsum(np.bincount(key // cols, minlength=rows))]).astype(np.int64)
+    vals = rng.standard_normal(idx.size)
+    x = G.vector(cols, 73)
+    y0 = G.vector(rows, 74)
+    lens = np.diff(offsets)
+    absax = O.abs_spmv(offsets, idx, vals, x)
+    a = H.clone_compressed(ctx, H.CSR, rows, cols, offsets, idx, vals, H.F64, index_bytes=4)
+    H.csr_spmv_analyze(ctx, a, H.panel_options(staging=H.STAGING_STREAM))
+    assert H.plan_get(ctx, a) == H.ALG_PANEL
+    for alpha, beta in ((1.0, 0.0), (-1.5, 0.25)):
+        yref = y0.copy()
+        O.compat_spmv(offsets, idx, vals, alpha, x, beta, yref)
+        print(f"stream combine shape: {groups} groups alpha {alpha} beta {beta}", flush=True)
+        y = run_compat(ctx, a, x, y0, alpha, beta, H.ALG_PANEL)
+        bound_check(y, yref, lens, absax, alpha, beta, y0, EPS[np.float64])
+        again = run_compat(ctx, a, x, y0, alpha, beta, H.ALG_PANEL)
+        np.testing.assert_array_equal(again.view(np.uint8), y.view(np.uint8))
+    H.plan_clear(ctx)
+
+
 def test_stream_plan_strides_standard_and_empty_rows(ctx):
     rng = np.random.default_rng(19)
     rows, cols = 7000, 50000
