@@ -351,6 +351,14 @@ def _stream_cases():
         o, i, v = _explicit(rows, cols, lens, 63)
         return o, i, v, rows, cols
 
+    def one_per_row():
+        # every entry its own segment: each full step finishes 512 segments,
+        # more than a wave's ring holds, so the steps write them directly
+        rng = np.random.default_rng(64)
+        rows, cols = 60000, 8 * 16384
+        idx = rng.integers(0, cols, rows).astype(np.int64)
+        return np.arange(rows + 1, dtype=np.int64), idx, rng.standard_normal(rows), rows, cols
+
     return {
         "zipf_2e16": (gm(lambda: G.make(G.ZIPF, 1 << 16, 1 << 16, seed=23, values=G.VALUES_NORMAL)),
                       [(0, 0), (4099, 8), (1 << 13, 1), (300, 64)]),
@@ -361,6 +369,7 @@ def _stream_cases():
                                          band=40, fill=0.5)), [(2048, 4), (37, 30)]),
         "dense_rows": (dense_rows, [(0, 0), (0, 1), (5000, 100)]),
         "half_empty": (half_empty, [(0, 1), (1 << 12, 100), (777, 1)]),
+        "one_per_row": (one_per_row, [(0, 0), (5000, 1)]),
     }
 
 
