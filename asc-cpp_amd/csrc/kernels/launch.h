@@ -141,9 +141,9 @@ struct SideStream {
 
 // Workspace bytes the algorithm needs for this operand (0 for most).
 size_t spmv_workspace_bytes(const SpmvArgs& args);
-// With `side`, the stream-form panel plan runs its short rows on the side
-// stream, beside the panel kernels; the main stream waits for them before it
-// returns.
+// With `side`, the stream-form panel plan runs its rows below long_min (if
+// any) on the side stream, beside the stream kernels; the main stream waits
+// for them before it returns.
 hipError_t spmv_launch(const SpmvArgs& args, void* workspace,
                        hipStream_t stream, const SideStream* side = nullptr);
 
