@@ -159,6 +159,9 @@ struct KParams {
   double alpha;
   double beta;
   int form;  // SpmvForm
+  // 16- or 32-bit values on a 2-value boundary and int32 indices on an
+  // 8-byte boundary: the vector kernel loads entries in pairs.
+  int paired;
 };
 
 // Fused epilogue for one finished row.
@@ -242,6 +245,49 @@ __device__ __forceinline__ Acc group_sum(const KParams& p, int64_t b, int64_t e,
   return acc;
 }
 
+// 16- or 32-bit values with int32 indices (p.paired): lane l takes the entry
+// pairs (2l, 2l+1) from the row's first even entry, one load for the two
+// values and one 8-byte load for the two indices, half the load instructions
+// of the per-entry loop; an odd first or last entry goes to lane 0.
+template <int G, typename Acc, typename V>
+__device__ __forceinline__ Acc group_sum_pairs(const KParams& p, int64_t b, int64_t e, int lane) {
+  using A = typename acc_of<V>::type;
+  static_assert(sizeof(V) == 2 || sizeof(V) == 4, "pairs of 16- or 32-bit values");
+  typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
+  typedef int32_t i32x2_t __attribute__((ext_vector_type(2)));
+  const V* __restrict__ val = static_cast<const V*>(p.values);
+  const int32_t* __restrict__ col = static_cast<const int32_t*>(p.indices);
+  const A* __restrict__ x = static_cast<const A*>(p.x);
+  Acc acc = Acc::zero();
+  Acc acc2 = Acc::zero();
+  int64_t b2 = b;
+  if ((b & 1) && b < e) {
+    if (lane == 0) acc.add_product(widen<A>(val[b]), x[static_cast<int64_t>(col[b]) * p.incx]);
+    b2 = b + 1;
+  }
+  for (int64_t k = b2 + 2 * lane; k + 1 < e; k += 2 * G) {
+    const i32x2_t cc = __builtin_nontemporal_load(reinterpret_cast<const i32x2_t*>(col + k));
+    V v0, v1;
+    if constexpr (sizeof(V) == 2) {
+      const uint32_t w = __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(val + k));
+      v0.bits = static_cast<uint16_t>(w & 0xFFFFu);
+      v1.bits = static_cast<uint16_t>(w >> 16);
+    } else {
+      const u32x2_t w = __builtin_nontemporal_load(reinterpret_cast<const u32x2_t*>(val + k));
+      v0 = __uint_as_float(w.x);
+      v1 = __uint_as_float(w.y);
+    }
+    acc.add_product(widen<A>(v0), x[static_cast<int64_t>(cc.x) * p.incx]);
+    acc2.add_product(widen<A>(v1), x[static_cast<int64_t>(cc.y) * p.incx]);
+  }
+  if (((e - b2) & 1) && e > b2 && lane == 0)
+    acc.add_product(widen<A>(val[e - 1]), x[static_cast<int64_t>(col[e - 1]) * p.incx]);
+  acc.merge_after(acc2);
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) acc.merge_after(acc.shfl_xor(off, G));
+  return acc;
+}
+
 // Rows of at least `skip_from` entries are left to the panel kernels
 // (INT64_MAX: none are).
 template <int G, typename V, typename I, bool TWOSUM>
@@ -254,7 +300,14 @@ __global__ __launch_bounds__(kBlock) void spmv_vector_kernel(KParams p, int64_t 
   const int64_t b = ld_matrix<(G >= 16)>(p.offsets + row);
   const int64_t e = ld_matrix<(G >= 16)>(p.offsets + row + 1);
   if (e - b >= skip_from) return;
-  const Acc acc = group_sum<G, V, I, Acc>(p, b, e, lane);
+  Acc acc = Acc::zero();
+  // Pairs halve the lanes a row keeps busy: only up to 16 lanes per row.
+  if constexpr (sizeof(V) <= 4 && sizeof(I) == 4 && G <= 16) {
+    if (p.paired) acc = group_sum_pairs<G, Acc, V>(p, b, e, lane);  // uniform branch
+    else acc = group_sum<G, V, I, Acc>(p, b, e, lane);
+  } else {
+    acc = group_sum<G, V, I, Acc>(p, b, e, lane);
+  }
   if (lane == 0) finish_row<V>(p, row, acc);
 }
 
@@ -2232,6 +2285,12 @@ hipError_t spmv_launch(const SpmvArgs& a, void* ws, hipStream_t s, const SideStr
   kp.alpha = a.alpha;
   kp.beta = a.beta;
   kp.form = static_cast<int>(a.form);
+  {
+    const size_t vb = elem_bytes(a.a.elem);
+    kp.paired = vb <= 4 && a.a.index_bytes == 4 &&
+                reinterpret_cast<uintptr_t>(a.a.values) % (2 * vb) == 0 &&
+                reinterpret_cast<uintptr_t>(a.a.indices) % 8 == 0;
+  }
   switch (a.a.elem) {
     case Elem::F64: return launch_by_index<double, false>(a, kp, ws, s, side);
     case Elem::F32:
