@@ -122,3 +122,38 @@ def bf16_rne(v):
         bits = np.array([f], np.float32).view(np.uint32)[0]
         out[i] = sign | (int(bits) >> 16)
     return out
+
+
+@pytest.mark.parametrize("alg", [H.ALG_VECTOR[8], H.ALG_VECTOR[16]])
+def test_paired_loads_and_misaligned_values(ctx, alg):
+    # 16-bit values start on a 4-byte boundary in a clone, so the vector
+    # kernel loads them in pairs; the same values rebound one element off
+    # that boundary take the per-entry loop.  Both sum the same products
+    # (only the order differs), within the FP32 plain-mode bound.
+    m = G.make(G.ZIPF, 1 << 14, 1 << 14, seed=31, values=G.VALUES_NORMAL)
+    a64 = H.clone_compressed(ctx, H.CSR, m.rows, m.cols, m.offsets, m.indices, m.values, H.F64, 4)
+    abf = H.convert_values(ctx, a64, H.BF16)
+    bits = ctx.download(H._ptr_buf(ctx, abf.desc.values), np.uint16, m.nnz)
+    src = ctx.upload(bits)
+    shifted = ctx.alloc(2 * (m.nnz + 2))
+    ctx.copy(shifted.ptr + 2, src.ptr, 2 * m.nnz)
+    ctx.synchronize()
+    odd = H.rebind_values(abf, shifted.ptr + 2)
+    x = G.vector(m.cols, 32).astype(np.float32)
+    xb = ctx.upload(x)
+    xd = H.vector(xb, m.cols, H.F32, writable=False)
+    ys = []
+    for a in (abf, odd):
+        yb = ctx.alloc(4 * m.rows)
+        H.csr_spmv(ctx, 1.0, a, xd, 0.0, H.vector(yb, m.rows, H.F32), alg=alg)
+        ys.append(ctx.download(yb, np.float32, m.rows).astype(np.float64))
+    vals = (bits.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+    absax = O.abs_spmv(m.offsets, m.indices, vals, x.astype(np.float64))
+    lens = np.maximum(np.diff(m.offsets), 1)
+    bound = 2 * 4.0 * lens * 2.0 ** -24 * absax
+    bad = np.nonzero(~(np.abs(ys[0] - ys[1]) <= bound))[0]
+    assert bad.size == 0, f"{bad.size} rows differ beyond the bound, first {bad[:5]}"
+    ref = np.zeros(m.rows)
+    O.compat_spmv(m.offsets, m.indices, vals, 1.0, x.astype(np.float64), 0.0, ref)
+    for y in ys:
+        assert np.all(np.abs(y - ref) <= 4.0 * lens * 2.0 ** -24 * absax + 1e-300)
