@@ -11,7 +11,7 @@ Workload: seeded matrices from six generators (uniform-random, Zipf
 power-law, banded, 8x8 block-diagonal, stencil, random bipartite), rows
 log-uniform in [2^12, 2^22], FP64 values, int32 columns.
 
-Usage on a GPU box:
+Usage on a machine with an MI355X:
     python -m asc_sparse_hip.selector collect --split train --start 0 --count 500 --out f.npz
     python -m asc_sparse_hip.selector train --data a.npz b.npz --out model.json
     python -m asc_sparse_hip.selector evaluate --model model.json --data held.npz

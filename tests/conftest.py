@@ -27,7 +27,7 @@ def ctx():
     import asc_sparse_hip as H
     problem = _gpu_problem()
     if problem:
-        pytest.fail(f"no usable HIP device ({problem}): GPU tests must run on a GPU box")
+        pytest.fail(f"no usable HIP device ({problem}): GPU tests need an MI355X")
     c = H.Context(0)
     yield c
     c.close()
