@@ -66,6 +66,22 @@ int oracle_compat_spmv_f32(int64_t rows, const int64_t* offsets,
   COMPAT_SPMV_BODY(float)
 }
 
+/* The same loops over int32 column indices (the device clones' narrow
+ * width), so large checks need no widened copy of the indices. */
+int oracle_compat_spmv_f64_i32(int64_t rows, const int64_t* offsets,
+                               const int32_t* indices, const double* values,
+                               double alpha, const double* x, int64_t incx,
+                               double beta, double* y, int64_t incy) {
+  COMPAT_SPMV_BODY(double)
+}
+
+int oracle_compat_spmv_f32_i32(int64_t rows, const int64_t* offsets,
+                               const int32_t* indices, const float* values,
+                               float alpha, const float* x, int64_t incx,
+                               float beta, float* y, int64_t incy) {
+  COMPAT_SPMV_BODY(float)
+}
+
 /* ------------------------------------------------------------------------ */
 /* Standardized SpMV: include/asc/sparse/blas.h:1607-1667, walking stored   */
 /* entries with ForEachStored (:1241-1256): outer ascending, then position. */
@@ -150,17 +166,35 @@ int oracle_std_spmm_f32(int format, int transpose, int64_t rows, int64_t cols,
   STD_SPMM_BODY(float)
 }
 
-/* Row-wise (|A||x|)_i, accumulated in long double; test helper only. */
+int oracle_std_spmm_f64_i32(int format, int transpose, int64_t rows, int64_t cols,
+                            const int64_t* offsets, const int32_t* indices,
+                            const double* values, double alpha, const double* b,
+                            int b_layout, int64_t ldb, int64_t k, double* c,
+                            int c_layout, int64_t ldc) {
+  STD_SPMM_BODY(double)
+}
+
+/* Row-wise (|A||x|)_i = sum_k |v_k| * |x_{c_k}|, accumulated in long double
+ * (64-bit significand) and rounded once to double; test helper only. */
+#define ABS_SPMV_BODY                                                        \
+  for (int64_t r = 0; r < rows; ++r) {                                       \
+    long double s = 0.0L;                                                    \
+    for (int64_t p = offsets[r]; p < offsets[r + 1]; ++p) {                  \
+      s += fabsl((long double)values[p]) * fabsl((long double)x[indices[p]]); \
+    }                                                                        \
+    out[r] = (double)s;                                                      \
+  }
+
 void oracle_abs_spmv_f64(int64_t rows, const int64_t* offsets,
                          const int64_t* indices, const double* values,
                          const double* x, double* out) {
-  for (int64_t r = 0; r < rows; ++r) {
-    long double s = 0.0L;
-    for (int64_t p = offsets[r]; p < offsets[r + 1]; ++p) {
-      s += fabsl((long double)values[p]) * fabsl((long double)x[indices[p]]);
-    }
-    out[r] = (double)s;
-  }
+  ABS_SPMV_BODY
+}
+
+void oracle_abs_spmv_f64_i32(int64_t rows, const int64_t* offsets,
+                             const int32_t* indices, const double* values,
+                             const double* x, double* out) {
+  ABS_SPMV_BODY
 }
 
 /* ------------------------------------------------------------------------ */

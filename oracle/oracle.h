@@ -35,6 +35,16 @@
  *                           and oracle/_ref.
  *   oracle_coo_finalize_*   golden vectors of coordinate_test.cc and a
  *                           Python sorted()-based brute force.
+ *   oracle_abs_spmv_*       (sets every plain-mode tolerance) exact integer
+ *                           products, where the result must be exact; the
+ *                           correctly rounded exact sum of |v||x| (Python
+ *                           Fractions) within one unit in the last place on
+ *                           random doubles; operands with cancelling signs,
+ *                           where a dropped absolute value shows; and a
+ *                           rectangular shape, where a transposed operand
+ *                           shows.
+ *   *_i32 variants          the same loop bodies over int32 indices; equal
+ *                           bit for bit to the int64 form on random inputs.
  */
 #ifndef ASC_SPARSE_HIP_ORACLE_H_
 #define ASC_SPARSE_HIP_ORACLE_H_
@@ -79,6 +89,15 @@ int oracle_compat_spmv_f32(int64_t rows, const int64_t* offsets,
                            const int64_t* indices, const float* values,
                            float alpha, const float* x, int64_t incx,
                            float beta, float* y, int64_t incy);
+/* The same with int32 column indices (the device clones' narrow width). */
+int oracle_compat_spmv_f64_i32(int64_t rows, const int64_t* offsets,
+                               const int32_t* indices, const double* values,
+                               double alpha, const double* x, int64_t incx,
+                               double beta, double* y, int64_t incy);
+int oracle_compat_spmv_f32_i32(int64_t rows, const int64_t* offsets,
+                               const int32_t* indices, const float* values,
+                               float alpha, const float* x, int64_t incx,
+                               float beta, float* y, int64_t incy);
 
 /* Standardized SpMV y += alpha*op(A)*x, blas.h:1607-1667 (loop :1241-1256). */
 int oracle_std_spmv_f64(int format, int transpose, int64_t rows, int64_t cols,
@@ -101,12 +120,20 @@ int oracle_std_spmm_f32(int format, int transpose, int64_t rows, int64_t cols,
                         const float* values, float alpha, const float* b,
                         int b_layout, int64_t ldb, int64_t k, float* c,
                         int c_layout, int64_t ldc);
+int oracle_std_spmm_f64_i32(int format, int transpose, int64_t rows, int64_t cols,
+                            const int64_t* offsets, const int32_t* indices,
+                            const double* values, double alpha, const double* b,
+                            int b_layout, int64_t ldb, int64_t k, double* c,
+                            int c_layout, int64_t ldc);
 
 /* Per-row (|A||x|)_i in long double, for the plain-mode error bound
  * |y_gpu - y_ref| <= 4*len_i*eps*(|A||x|)_i.  Not a reference function. */
 void oracle_abs_spmv_f64(int64_t rows, const int64_t* offsets,
                          const int64_t* indices, const double* values,
                          const double* x, double* out);
+void oracle_abs_spmv_f64_i32(int64_t rows, const int64_t* offsets,
+                             const int32_t* indices, const double* values,
+                             const double* x, double* out);
 
 /*
  * Canonical COO (interleaved int64 (row, col) pairs) to CSR (format 0) or

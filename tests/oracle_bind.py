@@ -31,6 +31,11 @@ def lib():
         L.oracle_std_spmm_f32.argtypes = [C.c_int, C.c_int, I64, I64, VP, VP, VP, C.c_float, VP, C.c_int, I64, I64, VP, C.c_int, I64]
         L.oracle_abs_spmv_f64.argtypes = [I64, VP, VP, VP, VP, VP]
         L.oracle_abs_spmv_f64.restype = None
+        L.oracle_abs_spmv_f64_i32.argtypes = [I64, VP, VP, VP, VP, VP]
+        L.oracle_abs_spmv_f64_i32.restype = None
+        L.oracle_compat_spmv_f64_i32.argtypes = L.oracle_compat_spmv_f64.argtypes
+        L.oracle_compat_spmv_f32_i32.argtypes = L.oracle_compat_spmv_f32.argtypes
+        L.oracle_std_spmm_f64_i32.argtypes = L.oracle_std_spmm_f64.argtypes
         L.oracle_coo_to_compressed_faithful.argtypes = [C.c_int, I64, I64, I64, VP, VP, C.c_int, VP, VP, VP]
         L.oracle_coo_to_compressed_fast.argtypes = [C.c_int, I64, I64, I64, VP, VP, C.c_int, VP, VP, VP]
         L.oracle_transpose_faithful.argtypes = [C.c_int, I64, I64, VP, VP, VP, C.c_int, VP, VP, VP]
@@ -60,14 +65,25 @@ def i64(a):
     return np.ascontiguousarray(a, dtype=np.int64)
 
 
+def _idx(indices):
+    """int32 indices pass as they are (the *_i32 entry points), any other
+    integer type as int64."""
+    indices = np.asarray(indices)
+    if indices.dtype == np.int32 and indices.flags.c_contiguous:
+        return indices, True
+    return i64(indices), False
+
+
 def compat_spmv(offsets, indices, values, alpha, x, beta, y, incx=1, incy=1):
-    """y (modified in place) = alpha*A*x (+ beta*y when beta != 0)."""
-    offsets, indices = i64(offsets), i64(indices)
+    """y (modified in place) = alpha*A*x (+ beta*y when beta != 0).  Row r
+    reads indices[offsets[r]:offsets[r+1]], so `offsets` may be any row
+    range's slice of the full offsets over the full indices and values."""
+    offsets = i64(offsets)
+    indices, narrow = _idx(indices)
     rows = offsets.shape[0] - 1
-    if values.dtype == np.float64:
-        lib().oracle_compat_spmv_f64(rows, _p(offsets), _p(indices), _p(values), alpha, _p(x), incx, beta, _p(y), incy)
-    else:
-        lib().oracle_compat_spmv_f32(rows, _p(offsets), _p(indices), _p(values), alpha, _p(x), incx, beta, _p(y), incy)
+    sfx = ("f64" if values.dtype == np.float64 else "f32") + ("_i32" if narrow else "")
+    getattr(lib(), "oracle_compat_spmv_" + sfx)(rows, _p(offsets), _p(indices), _p(values), alpha,
+                                                _p(x), incx, beta, _p(y), incy)
     return y
 
 
@@ -79,17 +95,69 @@ def std_spmv(fmt, op, rows, cols, offsets, indices, values, alpha, x, y, incx=1,
 
 
 def std_spmm(fmt, op, rows, cols, offsets, indices, values, alpha, b, b_layout, ldb, k, c, c_layout, ldc):
-    offsets, indices = i64(offsets), i64(indices)
-    f = lib().oracle_std_spmm_f64 if values.dtype == np.float64 else lib().oracle_std_spmm_f32
+    offsets, indices = i64(offsets), np.asarray(indices)
+    if indices.dtype == np.int32 and values.dtype == np.float64:
+        indices, _ = _idx(indices)
+        f = lib().oracle_std_spmm_f64_i32
+    else:
+        indices = i64(indices)
+        f = lib().oracle_std_spmm_f64 if values.dtype == np.float64 else lib().oracle_std_spmm_f32
     f(fmt, op, rows, cols, _p(offsets), _p(indices), _p(values), alpha, _p(b), b_layout, ldb, k,
       _p(c), c_layout, ldc)
     return c
 
 
-def abs_spmv(offsets, indices, values, x):
-    offsets, indices = i64(offsets), i64(indices)
+def abs_spmv(offsets, indices, values, x, out=None):
+    """(|A||x|)_i per row (FP64 values and x); `offsets` may be a row
+    range's slice, as for compat_spmv."""
+    offsets = i64(offsets)
+    indices, narrow = _idx(indices)
+    if out is None:
+        out = np.empty(offsets.shape[0] - 1)
+    f = lib().oracle_abs_spmv_f64_i32 if narrow else lib().oracle_abs_spmv_f64
+    f(out.shape[0], _p(offsets), _p(indices), _p(values), _p(x), _p(out))
+    return out
+
+
+def row_chunks(offsets, parts):
+    """Row ranges [r0, r1) of about equal entry counts (for running the
+    serial oracle over independent rows on several host threads)."""
+    rows = offsets.shape[0] - 1
+    nnz = int(offsets[-1])
+    cuts = np.searchsorted(offsets, np.linspace(0, nnz, parts + 1)[1:-1]).tolist()
+    bounds = sorted(set([0] + [min(max(c, 0), rows) for c in cuts] + [rows]))
+    return list(zip(bounds[:-1], bounds[1:]))
+
+
+def threads():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def parallel_rows(offsets, fn, parts=None):
+    """Calls fn(r0, r1) for row chunks on a thread pool (the oracle's C
+    loops release the GIL), and re-raises the first failure."""
+    from concurrent.futures import ThreadPoolExecutor
+    n = parts or 4 * threads()
+    with ThreadPoolExecutor(threads()) as ex:
+        for f in [ex.submit(fn, r0, r1) for r0, r1 in row_chunks(offsets, n)]:
+            f.result()
+
+
+def compat_spmv_rows(offsets, indices, values, alpha, x, beta, y):
+    """compat_spmv over every row, split over host threads (unit strides)."""
+    def part(r0, r1):
+        compat_spmv(offsets[r0:r1 + 1], indices, values, alpha, x, beta, y[r0:r1])
+    parallel_rows(offsets, part)
+    return y
+
+
+def abs_spmv_rows(offsets, indices, values, x):
+    """abs_spmv over every row, split over host threads."""
     out = np.empty(offsets.shape[0] - 1)
-    lib().oracle_abs_spmv_f64(out.shape[0], _p(offsets), _p(indices), _p(values), _p(x), _p(out))
+
+    def part(r0, r1):
+        abs_spmv(offsets[r0:r1 + 1], indices, values, x, out[r0:r1])
+    parallel_rows(offsets, part)
     return out
 
 

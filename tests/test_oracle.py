@@ -91,6 +91,92 @@ def test_compat_spmv_unfused_rounding():
     assert y[0] == 2.0 ** -29
 
 
+# ---- |A||x|: the quantity every plain-mode tolerance scales ---------------------------
+
+def _sparsity(rng, rows, cols, density):
+    mask = rng.random((rows, cols)) < density
+    off = np.concatenate([[0], np.cumsum(mask.sum(1))]).astype(np.int64)
+    idx = np.nonzero(mask)[1].astype(np.int64)
+    return mask, off, idx
+
+
+def test_abs_spmv_exact_integers():
+    # Integer operands of mixed sign: every product and partial sum is an
+    # integer below 2^53, so the result is exact and must equal the integer
+    # sum of |a_ij| * |x_j|.
+    rng = np.random.default_rng(21)
+    rows, cols = 300, 170  # rectangular: a transposed operand fails
+    mask, off, idx = _sparsity(rng, rows, cols, 0.2)
+    dense = np.where(mask, rng.integers(-1000, 1000, (rows, cols)), 0)
+    val = dense[mask].astype(np.float64)
+    x = rng.integers(-1000, 1000, cols).astype(np.float64)
+    out = O.abs_spmv(off, idx, val, x)
+    expect = np.abs(dense).astype(np.int64) @ np.abs(x).astype(np.int64)
+    np.testing.assert_array_equal(out, expect.astype(np.float64))
+
+
+def test_abs_spmv_correctly_rounded_within_one_ulp():
+    # Random doubles over many binades: the exact rational sum of |v||x|,
+    # rounded once (float(Fraction) rounds to nearest), must agree with the
+    # oracle to one unit in the last place.
+    rng = np.random.default_rng(22)
+    rows, cols = 120, 90
+    mask, off, idx = _sparsity(rng, rows, cols, 0.3)
+    val = rng.standard_normal(idx.size) * np.exp2(rng.integers(-30, 30, idx.size))
+    x = rng.uniform(-1, 1, cols) * np.exp2(rng.integers(-20, 20, cols))
+    out = O.abs_spmv(off, idx, val, x)
+    for r in range(rows):
+        exact = sum((abs(Fraction(float(val[p]))) * abs(Fraction(float(x[idx[p]])))
+                     for p in range(off[r], off[r + 1])), Fraction(0))
+        ref = float(exact)
+        assert out[r] >= 0.0
+        assert abs(out[r] - ref) <= np.spacing(ref), (r, out[r], ref)
+
+
+def test_abs_spmv_is_not_the_plain_product():
+    # Cancelling signs: A x is exactly zero while |A||x| is not, so a dropped
+    # absolute value (on v or on x) gives a different answer.
+    off = np.array([0, 2, 4], dtype=np.int64)
+    idx = np.array([0, 1, 0, 1], dtype=np.int64)
+    val = np.array([3.0, 3.0, -2.0, 2.0])
+    x = np.array([1.5, -1.5])
+    np.testing.assert_array_equal(O.abs_spmv(off, idx, val, x), [9.0, 6.0])
+    y = np.zeros(2)
+    O.compat_spmv(off, idx, val, 1.0, x, 0.0, y)
+    np.testing.assert_array_equal(y, [0.0, -6.0])
+
+
+def test_narrow_index_variants_equal_wide():
+    # The *_i32 entry points run the same loops over int32 indices; chunked
+    # calls over row ranges (the threaded helpers) give the same bits.
+    rng = np.random.default_rng(23)
+    rows, cols = 3000, 2500
+    mask, off, idx = _sparsity(rng, rows, cols, 0.01)
+    val = rng.standard_normal(idx.size)
+    x = rng.uniform(-1, 1, cols)
+    y64, y32, yt = (G := rng.uniform(-1, 1, rows)).copy(), G.copy(), G.copy()
+    O.compat_spmv(off, idx, val, -1.25, x, 0.5, y64)
+    O.compat_spmv(off, idx.astype(np.int32), val, -1.25, x, 0.5, y32)
+    O.compat_spmv_rows(off, idx.astype(np.int32), val, -1.25, x, 0.5, yt)
+    np.testing.assert_array_equal(y64.view(np.uint64), y32.view(np.uint64))
+    np.testing.assert_array_equal(y64.view(np.uint64), yt.view(np.uint64))
+    f64 = O.compat_spmv(off, idx, val.astype(np.float32), np.float32(2.0), x.astype(np.float32),
+                        0.0, np.zeros(rows, np.float32))
+    f32 = O.compat_spmv(off, idx.astype(np.int32), val.astype(np.float32), np.float32(2.0),
+                        x.astype(np.float32), 0.0, np.zeros(rows, np.float32))
+    np.testing.assert_array_equal(f64.view(np.uint32), f32.view(np.uint32))
+    a64 = O.abs_spmv(off, idx, val, x)
+    np.testing.assert_array_equal(a64, O.abs_spmv(off, idx.astype(np.int32), val, x))
+    np.testing.assert_array_equal(a64, O.abs_spmv_rows(off, idx.astype(np.int32), val, x))
+    k = 5
+    b = rng.uniform(-1, 1, (cols, k))
+    c0 = rng.uniform(-1, 1, (rows, k))
+    cw, cn = c0.copy(), c0.copy()
+    O.std_spmm(0, 0, rows, cols, off, idx, val, 0.5, b, 1, k, k, cw, 1, k)
+    O.std_spmm(0, 0, rows, cols, off, idx.astype(np.int32), val, 0.5, b, 1, k, k, cn, 1, k)
+    np.testing.assert_array_equal(cw.view(np.uint64), cn.view(np.uint64))
+
+
 # ---- standard SpMV / SpMM ----------------------------------------------------------
 
 def test_standard_golden():
