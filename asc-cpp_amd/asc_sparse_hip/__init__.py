@@ -17,6 +17,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_DIR = os.path.join(os.path.dirname(_HERE), "lib")
 LIB_PATH = os.path.join(LIB_DIR, "libasc_sparse_hip.so")
 GEN_PATH = os.path.join(LIB_DIR, "libasc_sparse_gen.so")
+PROBE_PATH = os.path.join(LIB_DIR, "libasc_sparse_probe.so")
 
 # ---- enums (values of include/asc_sparse_hip.h) ---------------------------
 OK, INVALID_ARGUMENT, SHAPE, INDEX, OVERFLOW, INVALID_STATE, ALLOCATION = 0, 1, 2, 3, 4, 5, 6
@@ -37,7 +38,7 @@ ALG_NAMES = {2: "merge", 3: "vec1", 4: "vec2", 5: "vec4", 6: "vec8", 7: "vec16",
              8: "vec32", 9: "vec64", 1: "ordered", 10: "panel"}
 TUNABLE_ALGS = (2, 3, 4, 5, 6, 7, 8, 9)
 ACC_PLAIN, ACC_TWOSUM = 0, 1
-INJECT_ALLOCATION, INJECT_EVENT = 0, 1
+INJECT_ALLOCATION, INJECT_EVENT, INJECT_STREAM_STALL = 0, 1, 2
 
 ELEM_BYTES = {F32: 4, F64: 8, C64: 8, C128: 16, BF16: 2, F16: 2}
 NP_STORAGE = {F32: np.float32, F64: np.float64, BF16: np.uint16, F16: np.float16}
@@ -706,30 +707,13 @@ class PanelOptions(C.Structure):
 _SIGS4 = {
     "asc_sparse_hip_csr_spmv_analysis_size": [_VP, P(SparseDesc), P(PanelOptions), P(C.c_size_t)],
     "asc_sparse_hip_csr_spmv_analyze": [_VP, P(SparseDesc), P(PanelOptions), _VP, C.c_size_t],
+    "asc_sparse_hip_csr_spmv_refresh_values": [_VP, P(SparseDesc)],
 }
 _SIGS.update(_SIGS4)
 if _lib is not None:
     for _n, _a in _SIGS4.items():
         getattr(_lib, _n).argtypes = _a
         getattr(_lib, _n).restype = Status
-
-
-GATHER_PROBE_BLOCKS = 2048
-_SIGS_PROBE = {
-    "asc_sparse_hip_gather_probe": [_VP, _VP, _VP, C.c_int64, _VP, C.c_int64, _VP, C.c_size_t],
-}
-_SIGS.update(_SIGS_PROBE)
-if _lib is not None:
-    for _n, _a in _SIGS_PROBE.items():
-        getattr(_lib, _n).argtypes = _a
-        getattr(_lib, _n).restype = Status
-
-
-def gather_probe(ctx: Context, indices, values, count: int, x, x_extent: int, partials):
-    """Enqueues the gather-rate probe over device buffers (see the C header)."""
-    ptr = lambda b: b.ptr if hasattr(b, "ptr") else b
-    check(ctx.lib.asc_sparse_hip_gather_probe(ctx.handle, ptr(indices), ptr(values), count, ptr(x),
-                                              x_extent, ptr(partials), GATHER_PROBE_BLOCKS))
 
 
 def panel_options(panel_columns: int = 0, long_row_min: int = 0, short_alg: int = ALG_AUTO,
@@ -759,6 +743,12 @@ def csr_spmv_analyze(ctx: Context, a: DeviceSparse, opts: PanelOptions | None = 
         ctx.synchronize()  # queued work may still read the replaced plan
         previous.free()
     return buffer
+
+
+def csr_spmv_refresh_values(ctx: Context, a: DeviceSparse):
+    """Copies a's current values into its stream-staged panel analysis (the
+    values-only refresh after an in-place change or a rebind)."""
+    check(ctx.lib.asc_sparse_hip_csr_spmv_refresh_values(ctx.handle, C.byref(a.desc)))
 
 
 DUPLICATE_REJECT, DUPLICATE_SUM = 0, 1

@@ -22,6 +22,11 @@ struct asc_hip_context {
   // A second stream (default priority, library-owned) and fork/join events:
   // work of one call that can run beside the call's main kernels goes there.
   asc_hip::SideStream side{};
+  // Device word a kernel sets when it gives up a bounded wait
+  // (asc_hip::kFaultStreamWait); read and cleared by
+  // asc_sparse_hip_context_synchronize, which then fails with
+  // ASC_HIP_INTERNAL.
+  uint32_t* fault_word = nullptr;
   uint8_t determinism = ASC_HIP_DETERMINISTIC;
   // Bytes of B per SpMM column pass; <= 0: one pass.
   int64_t spmm_panel_bytes = 0;
@@ -98,5 +103,6 @@ asc_hip::Compressed as_compressed(const asc_hip_sparse_desc& d, bool row_view_of
 // Host-forced fault injection (test hooks, see asc_sparse_hip_test_inject).
 bool test_hook_consume_allocation_failure();
 bool test_hook_consume_event_failure();
+bool test_hook_consume_stream_stall();
 
 }  // namespace asc_hip_abi

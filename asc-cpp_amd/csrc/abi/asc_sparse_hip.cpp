@@ -354,6 +354,7 @@ asc_hip::Compressed as_compressed(const asc_hip_sparse_desc& d, bool row_view_of
 
 std::atomic<int32_t> g_fail_allocation{-1};
 std::atomic<int32_t> g_fail_event{-1};
+std::atomic<int32_t> g_stream_stall{-1};
 
 bool consume(std::atomic<int32_t>& countdown) {
   int32_t v = countdown.load();
@@ -365,6 +366,7 @@ bool consume(std::atomic<int32_t>& countdown) {
 
 bool test_hook_consume_allocation_failure() { return consume(g_fail_allocation); }
 bool test_hook_consume_event_failure() { return consume(g_fail_event); }
+bool test_hook_consume_stream_stall() { return consume(g_stream_stall); }
 
 }  // namespace asc_hip_abi
 
@@ -377,6 +379,7 @@ const char* asc_sparse_hip_last_message(void) { return g_message.c_str(); }
 asc_hip_status asc_sparse_hip_test_inject(uint32_t what, int32_t countdown) {
   if (what == ASC_HIP_INJECT_ALLOCATION) g_fail_allocation.store(countdown);
   else if (what == ASC_HIP_INJECT_EVENT) g_fail_event.store(countdown);
+  else if (what == ASC_HIP_INJECT_STREAM_STALL) g_stream_stall.store(countdown);
   else return fail(ASC_HIP_INVALID_ARGUMENT, "unknown injection point");
   return ok();
 }
@@ -435,13 +438,16 @@ asc_hip_status asc_sparse_hip_context_create(int32_t device, void* stream,
   e = hipStreamCreateWithFlags(&ctx->side.stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->side.fork, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->side.join, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&ctx->fault_word), sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMemset(ctx->fault_word, 0, sizeof(uint32_t));
   if (e != hipSuccess) {
+    if (ctx->fault_word) (void)hipFree(ctx->fault_word);
     if (ctx->side.join) (void)hipEventDestroy(ctx->side.join);
     if (ctx->side.fork) (void)hipEventDestroy(ctx->side.fork);
     if (ctx->side.stream) (void)hipStreamDestroy(ctx->side.stream);
     if (ctx->owns_stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
-    return fail_hip(e, "creating the side stream");
+    return fail_hip(e, "creating the side stream and fault word");
   }
   *out = ctx;
   return ok();
@@ -462,6 +468,7 @@ asc_hip_status asc_sparse_hip_context_destroy(asc_hip_context* ctx) {
     }
     if (ctx->side.fork) (void)hipEventDestroy(ctx->side.fork);
     if (ctx->side.join) (void)hipEventDestroy(ctx->side.join);
+    if (ctx->fault_word) (void)hipFree(ctx->fault_word);
   }
   ctx->magic = 0;
   delete ctx;
@@ -481,8 +488,18 @@ asc_hip_status asc_sparse_hip_context_synchronize(asc_hip_context* ctx) {
   if (st.code) return st;
   if ((st = unavailable_if_validation_only(ctx)).code) return st;
   DeviceGuard guard(ctx->device);
-  const hipError_t e = hipStreamSynchronize(ctx->stream);
-  return e == hipSuccess ? ok() : fail_hip(e, "hipStreamSynchronize");
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return fail_hip(e, "hipStreamSynchronize");
+  // A kernel that gave up a bounded wait left its mark: report it once.
+  uint32_t fault = 0;
+  e = hipMemcpy(&fault, ctx->fault_word, sizeof(fault), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && fault != 0) e = hipMemset(ctx->fault_word, 0, sizeof(uint32_t));
+  if (e != hipSuccess) return fail_hip(e, "reading the context's fault word");
+  if (fault & asc_hip::kFaultStreamWait)
+    return fail(ASC_HIP_INTERNAL,
+                "a stream-staged panel SpMV gave up waiting between its waves; the outputs of "
+                "the SpMVs since the last synchronization are not valid");
+  return ok();
 }
 
 asc_hip_status asc_sparse_hip_allocate(asc_hip_context* ctx, size_t bytes,

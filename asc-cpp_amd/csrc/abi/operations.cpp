@@ -82,10 +82,11 @@ uint8_t resolve_alg(asc_hip_context* ctx, const asc_hip_sparse_desc& a,
         p.rows == a.rows && p.cols == a.cols && p.nnz == a.nnz &&
         p.element_kind == a.element_kind) {
       if (p.algorithm == ASC_HIP_ALG_PANEL) {
-        // A stream-form analysis copied other values: AUTO runs merge-path
-        // (an explicit ASC_HIP_ALG_PANEL is refused by the caller).
-        if (req == ASC_HIP_ALG_AUTO && p.panel.stream && p.panel.values_src != a.values)
-          return ASC_HIP_ALG_MERGE;
+        // The stream form runs on its copy of the values, which an in-place
+        // change of the caller's values would leave stale: AUTO (the
+        // reference's read-the-current-values semantics) never takes it and
+        // runs merge-path; only an explicit ASC_HIP_ALG_PANEL does.
+        if (req == ASC_HIP_ALG_AUTO && p.panel.stream) return ASC_HIP_ALG_MERGE;
         *panel = p.panel;
         *has_panel = true;
       }
@@ -182,6 +183,9 @@ asc_hip_status panel_shape(asc_hip_context* ctx, const asc_hip_sparse_desc* a,
     return fail(ASC_HIP_INVALID_ARGUMENT, "panel width and long-row length must be nonnegative");
   if (a->rows > kU32Max || std::min(a->nnz, a->cols) > kU32Max)
     return fail(ASC_HIP_UNSUPPORTED, "column-panel analysis stores 32-bit row ids and positions");
+  // The stream form records each segment's first entry as a 32-bit position.
+  if (staging_req == ASC_HIP_STAGING_STREAM && a->nnz > kU32Max)
+    return fail(ASC_HIP_UNSUPPORTED, "the stream form records 32-bit entry positions");
   const uint8_t staging = opt ? opt->staging : ASC_HIP_STAGING_AUTO;
   if (staging > ASC_HIP_STAGING_STREAM)
     return fail(ASC_HIP_INVALID_ARGUMENT, "unknown panel staging");
@@ -271,15 +275,15 @@ bool panel_overlaps(const asc_hip::PanelPlan& pp, Span out) {
          overlaps(span_of(pp.splits, 4 * n * static_cast<size_t>(pp.panels)), out);
 }
 
-// The stream form runs on its copy of the analysed values, with plain
-// accumulation.
+// The stream form runs on its copy of the analysed (or last refreshed)
+// values, with plain accumulation.
 asc_hip_status check_stream_plan(const asc_hip_sparse_desc& a, const asc_hip::PanelPlan& pp,
                                  const asc_hip_spmv_options* opt) {
   if (!pp.stream) return ok();
   if (a.values != pp.values_src)
     return fail(ASC_HIP_INVALID_ARGUMENT,
-                "the stream-staged panel analysis holds a copy of the analysed values; "
-                "analyze the matrix again after rebinding its values");
+                "the stream-staged panel analysis holds a copy of other values; refresh it "
+                "(asc_sparse_hip_csr_spmv_refresh_values) or analyze the matrix again");
   if (opt && opt->accumulate == ASC_HIP_ACC_TWOSUM)
     return fail(ASC_HIP_UNSUPPORTED, "the stream-staged panel plan accumulates plainly");
   return ok();
@@ -485,6 +489,10 @@ asc_hip_status run_spmv(asc_hip_context* ctx, SpmvPlan& plan, void* ws) {
     const hipError_t e = run_transpose(&plan, w, ctx->stream);
     if (e != hipSuccess) return finish_launch(ctx, e, "transpose");
   }
+  plan.args.fault = ctx->fault_word;
+  if (plan.args.alg == asc_hip::SpmvAlg::Panel && plan.args.panel.stream &&
+      test_hook_consume_stream_stall())
+    plan.args.spin_limit = 0;
   const hipError_t e =
       asc_hip::spmv_launch(plan.args, w + transpose_prefix(plan), ctx->stream, &ctx->side);
   return finish_launch(ctx, e, "SpMV launch");
@@ -846,6 +854,35 @@ asc_hip_status asc_sparse_hip_csr_spmv_analyze(asc_hip_context* ctx,
   pp.short_alg = static_cast<asc_hip::SpmvAlg>(short_alg);
   pp.long_alg = static_cast<asc_hip::SpmvAlg>(long_alg);
   return record_panel_plan(ctx, a, pp);
+}
+
+asc_hip_status asc_sparse_hip_csr_spmv_refresh_values(asc_hip_context* ctx,
+                                                      const asc_hip_sparse_desc* a) {
+  asc_hip_status st = check_context(ctx);
+  if (st.code) return st;
+  if ((st = check_device_sparse(ctx, a, "matrix")).code) return st;
+  if (a->format != ASC_HIP_CSR)
+    return fail(ASC_HIP_INVALID_ARGUMENT, "column-panel plans are built for CSR matrices");
+  if ((st = unavailable_if_validation_only(ctx)).code) return st;
+  std::lock_guard<std::mutex> lock(ctx->plan_mutex);
+  for (auto& p : ctx->plans) {
+    if (p.offsets == a->outer_offsets && p.indices == a->inner_indices && p.rows == a->rows &&
+        p.cols == a->cols && p.nnz == a->nnz && p.element_kind == a->element_kind) {
+      if (p.algorithm != ASC_HIP_ALG_PANEL || !p.panel.stream)
+        return fail(ASC_HIP_INVALID_STATE,
+                    "the matrix has no stream-staged panel analysis in this context");
+      if (overlaps(span_of(p.panel.stream_base, p.panel.stream_bytes), sparse_spans(*a).values))
+        return fail(ASC_HIP_INVALID_ARGUMENT, "the values overlap the panel analysis");
+      DeviceGuard guard(ctx->device);
+      if (guard.error != hipSuccess) return fail_hip(guard.error, "hipSetDevice");
+      const hipError_t e = asc_hip::panel_stream_refresh_launch(
+          p.panel, a->values, elem_size(a->element_kind), ctx->stream);
+      if ((st = finish_launch(ctx, e, "stream panel value refresh")).code) return st;
+      p.panel.values_src = a->values;
+      return ok();
+    }
+  }
+  return fail(ASC_HIP_INVALID_STATE, "the matrix has no stream-staged panel analysis in this context");
 }
 
 asc_hip_status asc_sparse_hip_standard_spmv_workspace_size(

@@ -84,6 +84,13 @@ struct PanelPlan {
   const void* values_src = nullptr;  // the values the copy was taken from
   size_t stream_bytes = 0;           // bytes of the stream arrays (overlap checks)
   const void* stream_base = nullptr;
+  // Per segment in ordinal order: its first entry in the CSR arrays, its
+  // first (padded) stream position and its length, so a values-only refresh
+  // can copy new values into the stream without redoing the analysis.
+  const uint32_t* seg_src = nullptr;
+  const uint32_t* seg_dst = nullptr;
+  const uint32_t* seg_len = nullptr;
+  int64_t nseg = 0;
 };
 
 // LDS bytes one work-group stages an x panel in (gfx950: 160 KiB per CU).
@@ -102,6 +109,13 @@ constexpr int kStreamDepth = 4;
 // Segment sums a wave keeps in LDS for the writer wave (a power of two), one
 // spare slot per lane beside them, and the LDS left for the x panel.
 constexpr int kStreamRing = 512;
+// Bound on each wait between the stream kernel's waves (sleeps of about
+// 64 clocks; about half a second): a wave that reaches it stops, records
+// kFaultStreamWait in the context's fault word and leaves, so a protocol
+// error ends the kernel instead of hanging it.  0 is a test setting: the
+// writer wave leaves at once and every wait gives up at its first check.
+constexpr uint32_t kStreamSpinLimit = 1u << 24;
+constexpr uint32_t kFaultStreamWait = 1u;
 constexpr int64_t kStreamPanelBytes = int64_t(128) << 10;
 static_assert(kStreamPanelBytes + int64_t(kStreamWaves) * ((kStreamRing + 64) * 8 + 3 * 4) <=
                   kLdsPanelBytes,
@@ -129,6 +143,10 @@ struct SpmvArgs {
   SpmvAlg alg;
   bool twosum;  // reduced-precision compensated accumulation
   PanelPlan panel;  // used when alg == Panel
+  // Stream form: where a wave that gives up waiting records it (may be
+  // null), and the wait bound.
+  uint32_t* fault = nullptr;
+  uint32_t spin_limit = kStreamSpinLimit;
 };
 
 // A stream beside the main one and two events: `fork` recorded on the main
@@ -165,7 +183,7 @@ struct StreamLayout {
   int64_t nlong = 0, nnz_long = 0, panels = 0, groups = 0;
   int64_t entries_cap = 0, slices_cap = 0, segs_cap = 0;
   size_t long_rows = 0, partial = 0, scol = 0, sval = 0, seg_lrow = 0, ctable = 0,
-         slice_info = 0, wg_table = 0;  // kept
+         slice_info = 0, wg_table = 0, seg_src = 0, seg_dst = 0, seg_len = 0;  // kept
   size_t row_begin = 0, cnt = 0, sbase = 0, sstart = 0, spanel = 0, srow = 0, slen = 0, sorder = 0,
          lens_s = 0, lrow_s = 0, dest = 0, pseg = 0, small = 0, slice_stage = 0, scan = 0,
          sort = 0, list_scratch = 0;  // analysis only
@@ -177,6 +195,10 @@ StreamLayout stream_layout(int64_t rows, int64_t nlong, int64_t nnz_long, int64_
 // fields of `pp` (whose panels/panel_cols/long_min are set).  Synchronous.
 hipError_t panel_stream_analysis_launch(const Compressed& a, const StreamLayout& L, void* base,
                                         PanelPlan* pp, hipStream_t stream);
+// Copies `values` (the matrix's current values, same element kind as the
+// analysed ones) into the stream of a stream-form plan, segment by segment.
+hipError_t panel_stream_refresh_launch(const PanelPlan& pp, const void* values,
+                                       size_t value_bytes, hipStream_t stream);
 
 // Stable sort of the indices 0..n-1 by uint32 key (< key_bound): out_index
 // gets the sorted order, out_key_offsets[k] (k = 0..key_bound) the first
@@ -271,13 +293,6 @@ struct BatchLaunch {
   int index_bytes;
 };
 hipError_t batched_spmv_launch(const BatchLaunch& launch, hipStream_t stream);
-
-// ---- gather-rate probe -------------------------------------------------------
-// Work-groups of the probe launch (one FP64 partial each).
-constexpr int64_t kGatherProbeBlocks = 2048;
-hipError_t gather_probe_launch(const int32_t* indices, const double* values, int64_t n,
-                               const double* x, int64_t x_extent, double* partials,
-                               hipStream_t stream);
 
 // ---- selector: features and MLP --------------------------------------------
 size_t features_workspace_bytes();

@@ -777,7 +777,15 @@ struct StepHeads {
   uint32_t o0;     // segment open at the lane's start
   uint32_t end;    // segment open after the step
   bool direct;     // the step writes its segments straight to HBM
+  bool gave_up;    // the wait for the writer reached its bound
 };
+
+// Records a wave that gave up waiting (any lane; every writer stores the
+// same value).
+__device__ __forceinline__ void stream_fault(uint32_t* fault) {
+  if (fault != nullptr)
+    __hip_atomic_fetch_or(fault, kFaultStreamWait, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // The step's heads and ordinals.  Publishes the ordinals the previous steps
 // finished, and waits for the writer while the step would wrap the ring onto
@@ -785,7 +793,8 @@ struct StepHeads {
 template <bool FULL, typename V>
 __device__ __forceinline__ StepHeads stream_heads(const StreamStep<V>& cur, int64_t t,
                                                   int64_t s1, int lane, const RingCtl& ctl,
-                                                  uint32_t open, uint32_t& con_c) {
+                                                  uint32_t open, uint32_t& con_c,
+                                                  uint32_t spin_limit) {
   const int64_t e = t + kStreamEpl * lane;
   uint32_t hmask = 0;  // bit j: entry j opens a segment
 #pragma unroll
@@ -814,14 +823,20 @@ __device__ __forceinline__ StepHeads stream_heads(const StreamStep<V>& cur, int6
   const auto span = [&](uint32_t c) {
     return end - (static_cast<int32_t>(c - open) < 0 ? c : open);
   };
-  bool direct = false;
+  bool direct = false, gave_up = false;
   if (span(con_c) >= static_cast<uint32_t>(kStreamRing)) {
     const uint32_t line = open & ~(kLineOrdinals - 1u);
-    for (;;) {
+    // con_c and the count are wave-uniform (readfirstlane), so the whole
+    // wave leaves the loop together.
+    for (uint32_t spins = 0;; ++spins) {
       con_c = lds_get(ctl.con);
       if (span(con_c) < static_cast<uint32_t>(kStreamRing) ||
           static_cast<int32_t>(con_c - line) >= 0)
         break;
+      if (spins >= spin_limit) {
+        gave_up = true;
+        break;
+      }
       __builtin_amdgcn_s_sleep(1);
     }
     direct = span(con_c) >= static_cast<uint32_t>(kStreamRing);
@@ -832,6 +847,7 @@ __device__ __forceinline__ StepHeads stream_heads(const StreamStep<V>& cur, int6
   hd.o0 = o0;
   hd.end = end;
   hd.direct = __builtin_amdgcn_readfirstlane(direct ? 1 : 0) != 0;  // wave-uniform
+  hd.gave_up = __builtin_amdgcn_readfirstlane(gave_up ? 1 : 0) != 0;
   return hd;
 }
 
@@ -917,10 +933,15 @@ __device__ __forceinline__ void stream_values(const StreamStep<V>& cur, const St
 // ring to HBM as contiguous bursts until every streaming wave is done.
 __device__ __forceinline__ void stream_writer(int waves, int lane, const double* sring,
                                               lds_u32* pub, lds_u32* con, lds_u32* done,
-                                              double* __restrict__ partial) {
+                                              double* __restrict__ partial,
+                                              uint32_t spin_limit, uint32_t* fault) {
   uint32_t finished = 0;
   const uint32_t all = (1u << waves) - 1u;
   const int w_of = lane < kStreamWaves ? lane : 0;
+  if (spin_limit == 0) return;  // test setting: no writer
+  // Passes in a row that found nothing to write: bounded like the streaming
+  // waves' waits (a streaming wave always reaches done, or gives up).
+  uint32_t idle = 0;
   while (finished != all) {
     // Lane w reads wave w's done word, then (after it returns) pub and con:
     // pub is final once done was seen set.
@@ -960,7 +981,15 @@ __device__ __forceinline__ void stream_writer(int waves, int lane, const double*
       }
       if (d != 0) finished |= 1u << w;
     }
-    if (!worked) __builtin_amdgcn_s_sleep(2);
+    if (worked) {
+      idle = 0;
+    } else {
+      if (++idle > spin_limit) {
+        if (lane == 0) stream_fault(fault);
+        return;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
   }
 }
 
@@ -969,7 +998,8 @@ __global__ __launch_bounds__(kStreamBlock) void spmv_stream_kernel(
     const typename acc_of<V>::type* __restrict__ x, int64_t incx, int64_t cols,
     int64_t panel_cols, const uint16_t* __restrict__ scol, const V* __restrict__ sval,
     const uint32_t* __restrict__ slice_info, const uint32_t* __restrict__ wg_table,
-    double* __restrict__ partial, double* __restrict__ carry) {
+    double* __restrict__ partial, double* __restrict__ carry, uint32_t spin_limit,
+    uint32_t* __restrict__ fault) {
   using A = typename acc_of<V>::type;
   __shared__ A sx[kStreamPanelBytes / sizeof(A)];
   __shared__ double sring[kStreamWaves * (kStreamRing + kWave)];
@@ -1023,7 +1053,7 @@ __global__ __launch_bounds__(kStreamBlock) void spmv_stream_kernel(
   __syncthreads();
   if (wave == kStreamWaves) {
     stream_writer(static_cast<int>(count < kStreamWaves ? count : kStreamWaves), lane, sring, pub,
-                  con, done, partial);
+                  con, done, partial, spin_limit, fault);
     return;
   }
   if (!active) return;  // whole waves leave together
@@ -1049,7 +1079,9 @@ __global__ __launch_bounds__(kStreamBlock) void spmv_stream_kernel(
   }                                                                                         \
   stream_claim(CUR);                                                                        \
   {                                                                                         \
-    const StepHeads hd = stream_heads<true, V>(CUR, t, s1, lane, ctl, open, con_c);         \
+    const StepHeads hd =                                                                    \
+        stream_heads<true, V>(CUR, t, s1, lane, ctl, open, con_c, spin_limit);              \
+    if (hd.gave_up) goto gave_up;                                                           \
     stream_load(scol, sval, min(t + 3 * kStreamTile, s_last), lane, NEXT);                  \
     stream_values<true, V, A>(CUR, hd, sx, t, s0, s1, lane, ring, ctl, partial, open, con_c); \
   }                                                                                         \
@@ -1064,7 +1096,9 @@ __global__ __launch_bounds__(kStreamBlock) void spmv_stream_kernel(
 tail:
   stream_claim(last);
   {
-    const StepHeads hd = stream_heads<false, V>(last, t, s1, lane, ctl, open, con_c);
+    const StepHeads hd =
+        stream_heads<false, V>(last, t, s1, lane, ctl, open, con_c, spin_limit);
+    if (hd.gave_up) goto gave_up;
     stream_values<false, V, A>(last, hd, sx, t, s0, s1, lane, ring, ctl, partial, open, con_c);
   }
   lds_complete();
@@ -1077,6 +1111,14 @@ tail:
     else carry[slice] = v;
   }
   vmem_complete();  // the unused prefetches land before the wave ends
+  return;
+gave_up:
+  // No further ring or HBM writes: record the fault, let the writer finish
+  // what was published, and leave once the prefetches have landed.
+  if (lane == 0) stream_fault(fault);
+  lds_complete();
+  lds_put(ctl.done, 1u, lane);
+  vmem_complete();
 }
 
 // Adds the parts of segments that cross slice ends: the first slice of each
@@ -1307,7 +1349,8 @@ __global__ __launch_bounds__(kBlock) void stream_scatter_kernel(
     const uint32_t* __restrict__ lens_s, const uint32_t* __restrict__ lrow_s,
     const uint32_t* __restrict__ dest, const uint32_t* __restrict__ shift, int64_t nseg,
     int64_t panel_cols, uint16_t* __restrict__ scol, VB* __restrict__ sval,
-    uint16_t* __restrict__ seg_lrow) {
+    uint16_t* __restrict__ seg_lrow, uint32_t* __restrict__ seg_src,
+    uint32_t* __restrict__ seg_dst, uint32_t* __restrict__ seg_len) {
   const int64_t o = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (o >= nseg) return;
   const uint32_t sid = sorder[o];
@@ -1317,11 +1360,29 @@ __global__ __launch_bounds__(kBlock) void stream_scatter_kernel(
   const int64_t dst = static_cast<int64_t>(dest[o]) + shift[q];
   const int64_t col0 = q * panel_cols;
   seg_lrow[o] = static_cast<uint16_t>(lrow_s[o] % kStreamGroupRows);
+  seg_src[o] = static_cast<uint32_t>(src);
+  seg_dst[o] = static_cast<uint32_t>(dst);
+  seg_len[o] = len;
   for (uint32_t k = 0; k < len; ++k) {
     const uint32_t c = static_cast<uint32_t>(static_cast<int64_t>(col[src + k]) - col0);
     scol[dst + k] = static_cast<uint16_t>(c | (k == 0 ? kStreamHead : 0u));
     sval[stream_value_pos<VB>(dst + k)] = val[src + k];
   }
+}
+
+// Values-only refresh of a stream-form plan: one thread per segment copies
+// the segment's current values to its lane-interleaved stream positions.
+template <typename VB>
+__global__ __launch_bounds__(kBlock) void stream_refresh_kernel(
+    const VB* __restrict__ val, const uint32_t* __restrict__ seg_src,
+    const uint32_t* __restrict__ seg_dst, const uint32_t* __restrict__ seg_len, int64_t nseg,
+    VB* __restrict__ sval) {
+  const int64_t o = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (o >= nseg) return;
+  const int64_t src = seg_src[o];
+  const int64_t dst = seg_dst[o];
+  const uint32_t len = seg_len[o];
+  for (uint32_t k = 0; k < len; ++k) sval[stream_value_pos<VB>(dst + k)] = val[src + k];
 }
 
 // Completes slice_info from the host's staging table {start, end, raw start,
@@ -1769,7 +1830,8 @@ hipError_t launch_stream_forked(const SpmvArgs& a, const KParams& kp, void* ws, 
   if (e != hipSuccess) return e;
   spmv_stream_kernel<V><<<static_cast<unsigned>(pp.wgs), kStreamBlock, 0, s>>>(
       static_cast<const typename acc_of<V>::type*>(kp.x), kp.incx, a.a.inner, pp.panel_cols,
-      pp.scol, static_cast<const V*>(pp.sval), pp.slice_info, pp.wg_table, pp.spartial, carry);
+      pp.scol, static_cast<const V*>(pp.sval), pp.slice_info, pp.wg_table, pp.spartial, carry,
+      a.spin_limit, a.fault);
   e = hipGetLastError();
   // The side stream's work is joined below whatever happens after the fork.
   hipError_t es = hipSuccess, ej = hipSuccess;
@@ -1886,7 +1948,7 @@ hipError_t launch_typed(const SpmvArgs& a, const KParams& kp, void* ws,
           spmv_stream_kernel<V><<<static_cast<unsigned>(pp.wgs), kStreamBlock, 0, s>>>(
               static_cast<const typename acc_of<V>::type*>(kp.x), kp.incx, a.a.inner,
               pp.panel_cols, pp.scol, static_cast<const V*>(pp.sval), pp.slice_info,
-              pp.wg_table, pp.spartial, carry);
+              pp.wg_table, pp.spartial, carry, a.spin_limit, a.fault);
           e = hipGetLastError();
           if (e != hipSuccess) return e;
           spmv_stream_fixup_kernel<<<blocks_for(pp.slices, kBlock), kBlock, 0, s>>>(
@@ -2062,6 +2124,9 @@ StreamLayout stream_layout(int64_t rows, int64_t nlong, int64_t nnz_long, int64_
   L.ctable = take(sizeof(uint32_t) * static_cast<size_t>(panels * (L.groups + 1)));
   L.slice_info = take(4 * sizeof(uint32_t) * static_cast<size_t>(L.slices_cap));
   L.wg_table = take(2 * sizeof(uint32_t) * static_cast<size_t>(L.slices_cap));
+  L.seg_src = take(sizeof(uint32_t) * segs);
+  L.seg_dst = take(sizeof(uint32_t) * segs);
+  L.seg_len = take(sizeof(uint32_t) * segs);
   L.kept_bytes = off;
   L.row_begin = take(sizeof(int64_t) * n);
   L.cnt = take(sizeof(uint32_t) * n);
@@ -2120,6 +2185,10 @@ hipError_t panel_stream_analysis_launch(const Compressed& a, const StreamLayout&
   pp->values_src = a.values;
   pp->stream_base = base;
   pp->stream_bytes = L.kept_bytes;
+  pp->seg_src = at(L.seg_src);
+  pp->seg_dst = at(L.seg_dst);
+  pp->seg_len = at(L.seg_len);
+  pp->nseg = 0;
   pp->slices = 0;
   pp->wgs = 0;
   // Listed rows (the row-begin and split outputs of the listing are unused
@@ -2242,7 +2311,7 @@ hipError_t panel_stream_analysis_launch(const Compressed& a, const StreamLayout&
   stream_scatter_kernel<I, VB><<<sb, kBlock, 0, s>>>(                                        \
       static_cast<const I*>(a.indices), static_cast<const VB*>(a.values), sorder, sstart,    \
       spanel, lens_s, lrow_s, dest, dshift, nseg, pp->panel_cols, scol,                     \
-      static_cast<VB*>(sval), seg_lrow)
+      static_cast<VB*>(sval), seg_lrow, at(L.seg_src), at(L.seg_dst), at(L.seg_len))
   if (a.index_bytes == 4) {
     if (vb == 8) ASC_SCATTER(int32_t, uint64_t);
     else if (vb == 4) ASC_SCATTER(int32_t, uint32_t);
@@ -2267,7 +2336,24 @@ hipError_t panel_stream_analysis_launch(const Compressed& a, const StreamLayout&
   if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
   pp->slices = slices;
   pp->wgs = static_cast<int64_t>(hwg.size() / 2);
+  pp->nseg = nseg;
   return hipSuccess;
+}
+
+hipError_t panel_stream_refresh_launch(const PanelPlan& pp, const void* values,
+                                       size_t value_bytes, hipStream_t s) {
+  if (!pp.stream || pp.nseg == 0) return hipSuccess;
+  const unsigned sb = static_cast<unsigned>((pp.nseg + kBlock - 1) / kBlock);
+#define ASC_REFRESH(VB)                                                                    \
+  stream_refresh_kernel<VB><<<sb, kBlock, 0, s>>>(static_cast<const VB*>(values), pp.seg_src, \
+                                                  pp.seg_dst, pp.seg_len, pp.nseg,          \
+                                                  static_cast<VB*>(const_cast<void*>(pp.sval)))
+  if (value_bytes == 8) ASC_REFRESH(uint64_t);
+  else if (value_bytes == 4) ASC_REFRESH(uint32_t);
+  else if (value_bytes == 2) ASC_REFRESH(uint16_t);
+  else return hipErrorInvalidValue;
+#undef ASC_REFRESH
+  return hipGetLastError();
 }
 
 hipError_t spmv_launch(const SpmvArgs& a, void* ws, hipStream_t s, const SideStream* side) {

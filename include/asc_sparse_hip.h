@@ -206,6 +206,9 @@ asc_hip_status asc_sparse_hip_context_create(int32_t device, void* stream,
 asc_hip_status asc_sparse_hip_context_destroy(asc_hip_context* context);
 asc_hip_status asc_sparse_hip_context_stream(asc_hip_context* context,
                                              void** out_stream);
+/* Waits for the context's stream.  Also the point where a kernel that gave
+ * up a bounded wait is reported (once): ASC_HIP_INTERNAL, and the outputs of
+ * the operations since the previous synchronization are not valid. */
 asc_hip_status asc_sparse_hip_context_synchronize(asc_hip_context* context);
 
 /* Message of the calling thread's most recent failed call ("" if none). */
@@ -320,7 +323,8 @@ asc_hip_status asc_sparse_hip_csr_spmv(
 /*
  * Column-panel analysis for a trusted device CSR matrix.  No reference
  * counterpart: it is the analysis step of this library's ASC_HIP_ALG_PANEL
- * SpMV, which reads the caller's CSR arrays in place.  Rows with at least
+ * SpMV.  With L2 or LDS staging that SpMV reads the caller's CSR arrays in
+ * place; with stream staging it reads a copy (below).  Rows with at least
  * `long_row_min` entries are listed; the columns are cut into a multiple of 8
  * panels of about `panel_columns` columns, and the listed rows are summed
  * panel by panel, so that their x gathers hit a resident panel instead of
@@ -345,10 +349,14 @@ enum {
    * 16-bit column within the panel and the value per entry) into the
    * analysis buffer, and the SpMV streams that copy contiguously, gathering x
    * from an LDS copy of each panel (at most 128 KiB of x and 32768 columns
-   * per panel).  The copy holds A's values as they were at analysis: after
-   * the values change (in place, or through asc_sparse_hip_rebind_values) the
-   * matrix needs a new analysis; a SpMV whose values pointer differs from the
-   * analysed one fails with ASC_HIP_INVALID_ARGUMENT.  The SpMV keeps its
+   * per panel).  The copy holds A's values as they were at analysis (or at
+   * the last asc_sparse_hip_csr_spmv_refresh_values), so this plan is an
+   * explicit opt-in: only ASC_HIP_ALG_PANEL runs it, and ASC_HIP_ALG_AUTO on
+   * a matrix with a stream-staged plan runs merge-path on the current values.
+   * After the values change in place, refresh the copy before the next
+   * ASC_HIP_ALG_PANEL SpMV; a SpMV whose values pointer differs from the
+   * copied one (a rebind) fails with ASC_HIP_INVALID_ARGUMENT until a refresh
+   * or a new analysis.  Requires nnz <= UINT32_MAX.  The SpMV keeps its
    * (panel, row) partial sums in the analysis buffer, so SpMVs through one
    * analysis must not overlap (one context is never used concurrently).
    * Plain accumulation only (ASC_HIP_ACC_TWOSUM is ASC_HIP_UNSUPPORTED).
@@ -376,6 +384,23 @@ asc_hip_status asc_sparse_hip_csr_spmv_analysis_size(
 asc_hip_status asc_sparse_hip_csr_spmv_analyze(
     asc_hip_context* context, const asc_hip_sparse_desc* matrix,
     const asc_hip_panel_options* options, void* analysis, size_t analysis_bytes);
+
+/*
+ * Values-only refresh of a stream-staged panel analysis: enqueues a copy of
+ * `matrix`'s current values (same structure and element kind as the analysed
+ * matrix; the values pointer may differ, as after a rebind) into the
+ * analysis's stream copy, and records that pointer as the copied one.  No
+ * reference counterpart: the reference's SpMV always reads the current
+ * values (reference_blas.cc:23-33), which ASC_HIP_ALG_AUTO keeps; this call
+ * lets an explicit ASC_HIP_ALG_PANEL caller keep the stream plan across
+ * value updates (the RebindValues workflow, compressed.h:472-516) at the
+ * cost of one pass over the values (nnz * value bytes read and written,
+ * 12 bytes per segment of map read), instead of a new analysis.
+ * ASC_HIP_INVALID_STATE when the context holds no stream-staged analysis for
+ * the matrix.  Does not synchronize.
+ */
+asc_hip_status asc_sparse_hip_csr_spmv_refresh_values(asc_hip_context* context,
+                                                      const asc_hip_sparse_desc* matrix);
 
 /*
  * Standardized SpMV  y += alpha*op(A)*x  for CSR or CSC, real F32/F64
@@ -683,26 +708,6 @@ asc_hip_status asc_sparse_hip_batch_tune(asc_hip_context* context, int64_t count
                                          int32_t reps, asc_hip_batch_classes* out,
                                          float* class_ms);
 
-/* ---- gather-rate probe ---------------------------------------------------- */
-
-/* Work-groups of the probe launch: the number of partials it writes. */
-#define ASC_HIP_GATHER_PROBE_BLOCKS 2048
-
-/*
- * Measurement probe, no reference counterpart: the ceiling behind the SpMV
- * rooflines.  For p in [0, count) it streams indices[p] (int32, clamped on
- * the device to [0, x_extent)) and values[p], gathers x[indices[p]] through
- * the caches as the SpMV kernels do, and sums values[p] * x[indices[p]]; one
- * FP64 partial per work-group goes to `partials` (device, at least
- * ASC_HIP_GATHER_PROBE_BLOCKS values, overlapping no input).  Timing it over
- * random indices with x of a given size gives the gather rate a
- * random-column SpMV can reach when x is resident at that cache level.
- */
-asc_hip_status asc_sparse_hip_gather_probe(asc_hip_context* context, const int32_t* indices,
-                                           const double* values, int64_t count, const double* x,
-                                           int64_t x_extent, double* partials,
-                                           size_t partials_count);
-
 /* ---- variant selection: autotuner, plan cache, features, learned MLP ------ */
 
 /*
@@ -793,7 +798,12 @@ asc_hip_status asc_sparse_hip_select(asc_hip_context* context,
  * record fail as if HIP had failed; -1 disarms.  Mirrors the reference's
  * TrackingMemoryResource::FailOnAllocation (tests/sparse/test_resources.h:
  * 47-53) and its CUDA runtime test hooks (src/core/cuda/runtime.cc:53-70). */
-enum { ASC_HIP_INJECT_ALLOCATION = 0, ASC_HIP_INJECT_EVENT = 1 };
+/* ASC_HIP_INJECT_STREAM_STALL (no reference counterpart) makes the matching
+ * stream-staged panel SpMV run with its inter-wave waits bounded at zero and
+ * no writer wave, so that a wait gives up as a protocol fault would: the
+ * kernel ends normally, its outputs are not valid, and the next
+ * asc_sparse_hip_context_synchronize fails with ASC_HIP_INTERNAL. */
+enum { ASC_HIP_INJECT_ALLOCATION = 0, ASC_HIP_INJECT_EVENT = 1, ASC_HIP_INJECT_STREAM_STALL = 2 };
 asc_hip_status asc_sparse_hip_test_inject(uint32_t what, int32_t countdown);
 
 #ifdef __cplusplus

@@ -436,18 +436,3 @@ def test_span_byte_counts_that_overflow_are_refused(vctx):
     st = vctx.lib.asc_sparse_hip_standard_spmm(vctx.handle, H.OP_N, H.Scalar(1.0, 0.0), C.byref(a),
                                                C.byref(b), C.byref(c), None, 0)
     assert st.code == H.OVERFLOW
-
-
-def test_gather_probe_validation(vctx):
-    lib = vctx.lib
-    probe = lambda idx, val, n, x, xn, out, m: lib.asc_sparse_hip_gather_probe(
-        vctx.handle, idx, val, n, x, xn, out, m).code
-    i, v, x, out = 0x100000, 0x200000, 0x300000, 0x400000
-    assert probe(i, v, 10, x, 100, out, H.GATHER_PROBE_BLOCKS) == H.UNAVAILABLE  # valid
-    assert probe(i, v, -1, x, 100, out, H.GATHER_PROBE_BLOCKS) == H.SHAPE
-    assert probe(i, v, 10, x, 0, out, H.GATHER_PROBE_BLOCKS) == H.SHAPE
-    assert probe(i, v, 10, x, 2 ** 33, out, H.GATHER_PROBE_BLOCKS) == H.UNSUPPORTED
-    assert probe(i, v, 10, x, 100, out, H.GATHER_PROBE_BLOCKS - 1) == H.INVALID_ARGUMENT
-    assert probe(i, v, 10, x, 100, None, H.GATHER_PROBE_BLOCKS) == H.INVALID_ARGUMENT
-    assert probe(i, v, 10, x, 100, v + 8, H.GATHER_PROBE_BLOCKS) == H.INVALID_ARGUMENT  # overlaps values
-    assert probe(None, None, 0, None, 0, out, H.GATHER_PROBE_BLOCKS) == H.UNAVAILABLE  # empty is fine

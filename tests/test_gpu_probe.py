@@ -1,11 +1,12 @@
-"""The gather-rate probe computes what it states: sum over p of
+"""The measurement probes compute what they state.  Gather: sum over p of
 values[p] * x[min(indices[p], x_extent - 1)] (partials summed on the host,
 checked to a summation-order bound), including indices past the extent,
-which it clamps."""
+which it clamps.  Stream read: the wrapping sum of every 32-bit word.
+Stream copy: the bytes, unchanged."""
 import numpy as np
 import pytest
 
-import asc_sparse_hip as H
+from asc_sparse_hip import probe as P
 
 pytestmark = pytest.mark.gpu
 
@@ -17,10 +18,25 @@ def test_probe_sum(ctx, n, xn):
     val = rng.uniform(-1, 1, n)
     x = rng.uniform(-1, 1, xn)
     bi, bv, bx = ctx.upload(idx), ctx.upload(val), ctx.upload(x)
-    out = ctx.alloc(8 * H.GATHER_PROBE_BLOCKS)
+    out = ctx.alloc(8 * P.GATHER_PROBE_BLOCKS)
     ctx.fill(out, 0)
     print(f"probe n {n} x {xn}", flush=True)
-    H.gather_probe(ctx, bi, bv, n, bx, xn, out)
-    got = ctx.download(out, np.float64, H.GATHER_PROBE_BLOCKS).sum()
+    P.gather_probe(ctx, bi, bv, n, bx, xn, out)
+    got = ctx.download(out, np.float64, P.GATHER_PROBE_BLOCKS).sum()
     prods = val * x[np.minimum(idx, xn - 1)]
     assert abs(got - prods.sum()) <= 4 * n * 2.0 ** -53 * np.abs(prods).sum() + 1e-300
+
+
+@pytest.mark.parametrize("words", [1, 1000, 1 << 20, (1 << 22) + 7])
+def test_stream_probes(ctx, words):
+    rng = np.random.default_rng(words)
+    data = rng.integers(0, 2 ** 32, 4 * words, dtype=np.uint64).astype(np.uint32)
+    src, dst = ctx.upload(data), ctx.alloc(data.nbytes)
+    out = ctx.alloc(4 * P.STREAM_PROBE_BLOCKS)
+    ctx.fill(out, 0)
+    print(f"stream probes, {words} words", flush=True)
+    P.stream_read(ctx, src, data.nbytes, out)
+    got = int(ctx.download(out, np.uint32, P.STREAM_PROBE_BLOCKS).astype(np.uint64).sum()) % 2 ** 32
+    assert got == int(data.astype(np.uint64).sum()) % 2 ** 32
+    P.stream_copy(ctx, src, dst, data.nbytes)
+    np.testing.assert_array_equal(ctx.download(dst, np.uint32, data.size), data)

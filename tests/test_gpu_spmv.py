@@ -397,8 +397,12 @@ def test_stream_plan_parity(ctx, name, dtype):
                   f"alpha {alpha} beta {beta}", flush=True)
             first = None
             for alg in (H.ALG_PANEL, H.ALG_AUTO, H.ALG_PANEL):
+                # AUTO never takes the stream plan (merge-path on the current
+                # values); both ALG_PANEL calls give identical bits.
                 y = run_compat(ctx, a, x, y_in, alpha, beta, alg)
                 bound_check(y, yref, lens, absax, alpha, beta, np.nan_to_num(y_in), EPS[dtype])
+                if alg == H.ALG_AUTO:
+                    continue
                 if first is None:
                     first = y
                 else:  # fixed decomposition: identical bits every call
@@ -513,6 +517,13 @@ def test_stream_plan_reduced_precision_and_refusals(ctx):
     print("stream plan, rebound values through AUTO", flush=True)
     y = run_compat(ctx, b, x, np.zeros(m.rows), 1.0, 0.0, H.ALG_AUTO)
     bound_check(y, yref, lens, absnew, 1.0, 0.0, np.zeros(m.rows), EPS[np.float64])
+    # a values-only refresh takes the rebound values into the copy
+    H.csr_spmv_refresh_values(ctx, b)
+    print("stream plan, refreshed after a rebind", flush=True)
+    y = run_compat(ctx, b, x, np.zeros(m.rows), 1.0, 0.0, H.ALG_PANEL)
+    bound_check(y, yref, lens, absnew, 1.0, 0.0, np.zeros(m.rows), EPS[np.float64])
+    with pytest.raises(H.AscHipError):  # the copy now follows b's values, not a's
+        run_compat(ctx, a, x, np.zeros(m.rows), 1.0, 0.0, H.ALG_PANEL)
     H.csr_spmv_analyze(ctx, b, H.panel_options(1500, 32, staging=H.STAGING_STREAM))
     y = run_compat(ctx, b, x, np.zeros(m.rows), 1.0, 0.0, H.ALG_PANEL)
     bound_check(y, yref, lens, absnew, 1.0, 0.0, np.zeros(m.rows), EPS[np.float64])
@@ -522,4 +533,88 @@ def test_stream_plan_reduced_precision_and_refusals(ctx):
     bound = H.csr_spmv_analysis_size(vctx, a, H.panel_options(0, 64, staging=H.STAGING_STREAM))
     vctx.close()
     assert need < bound
+    H.plan_clear(ctx)
+
+
+@pytest.mark.parametrize("vkind", [H.F64, H.BF16])
+def test_stream_plan_in_place_value_update(ctx, vkind):
+    """Values changed in place after a stream-staged analysis: AUTO (the
+    reference's read-the-current-values semantics, reference_blas.cc:23-33)
+    gives the new product at once; the explicit plan runs its copy until
+    asc_sparse_hip_csr_spmv_refresh_values, then gives the new product."""
+    m = G.make(G.ZIPF, 1 << 15, 1 << 16, seed=47, values=G.VALUES_NORMAL)
+    a64 = H.clone_compressed(ctx, H.CSR, m.rows, m.cols, m.offsets, m.indices, m.values, H.F64, 4)
+    a = a64 if vkind == H.F64 else H.convert_values(ctx, a64, vkind)
+    x = G.vector(m.cols, 6)
+    lens = np.diff(m.offsets)
+    H.csr_spmv_analyze(ctx, a, H.panel_options(staging=H.STAGING_STREAM))
+    new_vals = np.random.default_rng(48).standard_normal(m.nnz)
+    eps, tol = (EPS[np.float64], None) if vkind == H.F64 else (None, 1e-2)
+
+    def check(y, vals):
+        yref = np.zeros(m.rows)
+        O.compat_spmv(m.offsets, m.indices, vals, 1.0, x, 0.0, yref)
+        absax = O.abs_spmv(m.offsets, m.indices, vals, x)
+        if tol is None:
+            bound_check(y, yref, lens, absax, 1.0, 0.0, np.zeros(m.rows), eps)
+        else:
+            assert np.max(np.abs(y - yref)) / np.max(absax) <= tol
+
+    old = m.values
+    check(run_compat(ctx, a, x, np.zeros(m.rows), 1.0, 0.0, H.ALG_PANEL), old)
+    # overwrite the values in place (same pointer)
+    if vkind == H.F64:
+        ctx.copy(a.desc.values, new_vals.ctypes.data, new_vals.nbytes)
+    else:
+        nb = ctx.upload(new_vals)
+        fresh = H.convert_values(ctx, H.rebind_values(a64, nb), vkind)
+        ctx.copy(a.desc.values, fresh.desc.values, 2 * m.nnz)
+    ctx.synchronize()
+    print(f"in-place update, kind {vkind}: AUTO reads the current values", flush=True)
+    check(run_compat(ctx, a, x, np.zeros(m.rows), 1.0, 0.0, H.ALG_AUTO), new_vals)
+    print(f"in-place update, kind {vkind}: explicit plan until refreshed", flush=True)
+    check(run_compat(ctx, a, x, np.zeros(m.rows), 1.0, 0.0, H.ALG_PANEL), old)
+    H.csr_spmv_refresh_values(ctx, a)
+    check(run_compat(ctx, a, x, np.zeros(m.rows), 1.0, 0.0, H.ALG_PANEL), new_vals)
+    H.plan_clear(ctx)
+    with pytest.raises(H.AscHipError) as e:  # nothing to refresh without a stream plan
+        H.csr_spmv_refresh_values(ctx, a)
+    assert e.value.code == H.INVALID_STATE
+
+
+def test_stream_plan_bounded_wait_reports_internal(ctx):
+    """A stream-staged SpMV whose inter-wave wait gives up (forced from the
+    host: no writer wave, waits bounded at zero, on a matrix whose rings
+    fill within a slice) ends normally and the next synchronization fails
+    with ASC_HIP_INTERNAL; the following SpMV is correct again.  The same
+    layout gives up in every work-group of the CPU model
+    (tests/test_stream_protocol.py)."""
+    rng = np.random.default_rng(12)
+    rows, cols = 40000, 16384
+    idx = np.sort(np.stack([rng.choice(cols, 5, replace=False) for _ in range(rows)]), 1).reshape(-1)
+    offsets = np.arange(0, 5 * rows + 1, 5, dtype=np.int64)
+    vals = rng.standard_normal(idx.size)
+    x = G.vector(cols, 8)
+    a = H.clone_compressed(ctx, H.CSR, rows, cols, offsets, idx, vals, H.F64, 4)
+    H.csr_spmv_analyze(ctx, a, H.panel_options(staging=H.STAGING_STREAM))
+    xb, yb = ctx.upload(x), ctx.alloc(8 * rows)
+    xd, yd = H.vector(xb, cols, H.F64, writable=False), H.vector(yb, rows, H.F64)
+    ctx.synchronize()
+    print("stream plan with a forced stall", flush=True)
+    H.test_inject(H.INJECT_STREAM_STALL, 0)
+    try:
+        H.csr_spmv(ctx, 1.0, a, xd, 0.0, yd, alg=H.ALG_PANEL)
+        with pytest.raises(H.AscHipError) as e:
+            ctx.synchronize()
+        assert e.value.code == H.INTERNAL
+    finally:
+        H.test_inject(H.INJECT_STREAM_STALL, -1)
+    ctx.synchronize()  # reported once
+    print("stream plan after the stall", flush=True)
+    H.csr_spmv(ctx, 1.0, a, xd, 0.0, yd, alg=H.ALG_PANEL)
+    ctx.synchronize()
+    yref = np.zeros(rows)
+    O.compat_spmv(offsets, idx, vals, 1.0, x, 0.0, yref)
+    bound_check(ctx.download(yb, np.float64, rows), yref, np.diff(offsets),
+                O.abs_spmv(offsets, idx, vals, x), 1.0, 0.0, np.zeros(rows), EPS[np.float64])
     H.plan_clear(ctx)

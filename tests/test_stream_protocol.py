@@ -20,6 +20,13 @@ It checks that every wait ends, that the slots a step writes never hold
 ordinals the writer has not written, and that every segment ordinal is
 written to HBM exactly once (by the writer, a direct step or a closed
 slice's end).
+
+Every wait of the kernel is bounded (kStreamSpinLimit sleeps): a streaming
+wave that reaches the bound records a fault, stops writing and reports done;
+a writer that finds nothing to write that many passes in a row records a
+fault and leaves.  The model runs the same rules with a spin bound and shows
+that a work-group whose writer is gone (the kernel's test setting, bound 0)
+ends with the fault recorded and no ordinal written twice.
 """
 import random
 
@@ -87,8 +94,11 @@ def stream_layout(offsets, cols_idx, cols, long_min, width):
     return wgs, nseg
 
 
-def run_workgroup(hp, slices, rng, written):
+def run_workgroup(hp, slices, rng, written, spin_limit=100_000, writer=True):
+    """Replays one work-group; returns True when a wait gave up (the
+    kernel's fault word)."""
     lds = {}
+    fault = False
     waves = []
     for w, (s0, s1, o, closed) in enumerate(slices):
         lds["pub", w] = lds["con", w] = o
@@ -97,7 +107,8 @@ def run_workgroup(hp, slices, rng, written):
     for w, st in enumerate(waves):  # after the barrier
         st["con_c"] = (st["open"] + (1 if hp[st["s0"]] else 0)) & M32
         lds["con", w] = st["con_c"]
-    finished, live = set(), set(range(len(waves))) | {"writer"}
+    finished, live = set(), set(range(len(waves))) | ({"writer"} if writer else set())
+    idle = 0
 
     def write(o):
         written[o] = written.get(o, 0) + 1
@@ -108,6 +119,7 @@ def run_workgroup(hp, slices, rng, written):
         assert steps < 5_000_000, "no progress"
         a = rng.choice(sorted(live, key=str))
         if a == "writer":
+            worked = False
             for w in range(len(waves)):
                 if w in finished:
                     continue
@@ -117,10 +129,18 @@ def run_workgroup(hp, slices, rng, written):
                     for k in range(s32(upto - c)):
                         write((c + k) & M32)
                     lds["con", w] = upto
+                    worked = True
                 if d:
                     finished.add(w)
             if len(finished) == len(waves):
                 live.discard("writer")
+            elif worked:
+                idle = 0
+            else:
+                idle += 1
+                if idle > spin_limit:  # bounded: record the fault and leave
+                    fault = True
+                    live.discard("writer")
             continue
         st = waves[a]
         t, s1, op = st["t"], st["s1"], st["open"]
@@ -134,8 +154,12 @@ def run_workgroup(hp, slices, rng, written):
         if span(st["con_c"]) >= RING:
             st["con_c"] = lds["con", a]
             if not (span(st["con_c"]) < RING or s32(st["con_c"] - ((op & ~(LINE - 1)) & M32)) >= 0):
+                if st["spins"] >= spin_limit:  # bounded: give up, write nothing more
+                    fault = True
+                    lds["done", a] = 1
+                    live.discard(a)
+                    continue
                 st["spins"] += 1
-                assert st["spins"] < 100_000, "a streaming wave waits forever"
                 continue
         st["spins"] = 0
         if span(st["con_c"]) >= RING:  # direct step
@@ -156,13 +180,14 @@ def run_workgroup(hp, slices, rng, written):
             if st["closed"]:
                 write(end)
             live.discard(a)
+    return fault
 
 
 def _check(offsets, idx, cols, long_min, width, seed):
     wgs, nseg = stream_layout(offsets, idx.astype(np.int64), cols, long_min, width)
     rng, written = random.Random(seed), {}
     for hp, slices in wgs:
-        run_workgroup(hp, slices, rng, written)
+        assert not run_workgroup(hp, slices, rng, written), "a wait reached its bound"
     bad = [o for o in range(nseg) if written.get(o, 0) != 1]
     assert not bad, f"{len(bad)} of {nseg} ordinals not written exactly once, first {bad[:5]}"
     assert set(written) <= set(range(nseg))
@@ -210,3 +235,38 @@ def test_model_catches_the_unbounded_wait():
     assert not ((end - con) & M32 < ring or con == o)  # the planted exit test: never true
     line = (o & ~(LINE - 1)) & M32
     assert s32(con - line) >= 0  # the kernel's exit test holds at once
+
+
+def _five_per_row(rows=40000, cols=16384, seed=12):
+    # One panel, rows of 5 entries: about 100 segments per 512-entry step, so
+    # each wave's ring fills after a few steps and needs the writer.
+    rng = np.random.default_rng(seed)
+    idx = np.sort(np.stack([rng.choice(cols, 5, replace=False) for _ in range(rows)]), 1)
+    return np.arange(0, 5 * rows + 1, 5, dtype=np.int64), idx.reshape(-1).astype(np.int64), cols
+
+
+def test_waits_give_up_without_a_writer():
+    # The kernel's test setting: no writer wave and every wait bounded at 0.
+    # Each streaming wave reaches a full ring, gives up, and the work-group
+    # ends with the fault recorded; nothing is written twice.
+    offsets, idx, cols = _five_per_row()
+    wgs, nseg = stream_layout(offsets, idx, cols, 1, 16384)
+    rng, written = random.Random(4), {}
+    for hp, slices in wgs:
+        assert run_workgroup(hp, slices, rng, written, spin_limit=0, writer=False)
+    assert all(v == 1 for v in written.values())
+    assert len(written) < nseg
+
+
+@pytest.mark.parametrize("limit", [0, 1, 3])
+def test_tight_bounds_end_with_a_fault_or_a_complete_result(limit):
+    # Whatever the interleaving, tight bounds end every wave and the writer:
+    # the work-group either completes (each ordinal once) or records a fault
+    # (no ordinal twice).
+    offsets, idx, cols = _five_per_row(rows=20000)
+    wgs, nseg = stream_layout(offsets, idx, cols, 1, 16384)
+    rng, written = random.Random(limit), {}
+    faults = [run_workgroup(hp, slices, rng, written, spin_limit=limit) for hp, slices in wgs]
+    assert all(v == 1 for v in written.values())
+    if not any(faults):
+        assert len(written) == nseg
