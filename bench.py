@@ -18,6 +18,15 @@ case (n = 2^20, 32 nnz/row, lognormal values: FP64 vs FP32/BF16/FP16
 storage, rows/s and error against the FP64 result); the batched case
 (10,000 matrices in one hipGraph-replayed persistent launch, SpMV/s).
 
+Every timed output is checked after its timing (the "guard" object): the
+headline and FP32 SpMVs on every row against the ordered kernel (the
+reference's summation order, itself compared bit for bit with the
+reference's own output on the CPU baseline's rows) within the plain-mode
+bound 4*len_i*eps*(|A||x|)_i; the ordered SpMM on sampled rows recomputed in
+the reference's order; the conversions' structure.  A failed check sets
+"error" in the line and the exit status to 3.  The roofline object also
+carries the measured HBM read and copy ceilings (libasc_sparse_probe.so).
+
 Prints one JSON line (the repository's benchmark contract).
 """
 import argparse
@@ -85,8 +94,27 @@ def measured_traffic(rows, nnz, alg):
     return prof["traffic_bytes_per_launch"], os.path.relpath(path, ROOT)
 
 
-def cpu_baseline(m, sample_rows, reps=3):
-    """Serial reference compat SpMV on the first `sample_rows` rows."""
+def host_facts():
+    """The GPU host's CPU, for the CPU baseline line."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model,
+            "allowed_cpus": len(os.sched_getaffinity(0))}
+
+
+def cpu_baseline(m, x, sample_rows, reps=20):
+    """Serial reference compat SpMV (oracle/_ref, the reference's own
+    sources compiled here; the oracle's restatement when that is absent) on
+    rows [0, sample_rows) of the same matrix, pinned to one host core.
+    Returns the line and the reference's y for those rows (the headline
+    guard compares the device's ordered kernel with it bit for bit)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_bind as O
     r = min(sample_rows, m.rows)
@@ -94,25 +122,69 @@ def cpu_baseline(m, sample_rows, reps=3):
     off = np.ascontiguousarray(m.offsets[: r + 1])
     idx = m.indices[:nnz].astype(np.int64)
     val = np.ascontiguousarray(m.values[:nnz])
-    x = G.vector(m.cols, 12345)
     y = np.zeros(r)
     kind = "port"
     fn = lambda: O.compat_spmv(off, idx, val, 1.0, x, 0.0, y)  # noqa: E731
     if O.ref_available():
         kind = "reference"
         fn = lambda: O.ref_compat_spmv(off, idx, val, 1.0, x, 0.0, y, cols=m.cols)  # noqa: E731
-    fn()
-    times = []
-    for _ in range(reps):
-        t0 = time.perf_counter()
+    facts = host_facts()
+    saved = os.sched_getaffinity(0)
+    core = min(saved)
+    os.sched_setaffinity(0, {core})  # this thread (the one calling the serial loop)
+    try:
         fn()
-        times.append(time.perf_counter() - t0)
+        times = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            times.append(time.perf_counter() - t0)
+    finally:
+        os.sched_setaffinity(0, saved)
     t = float(np.median(times))
     b = spmv_bytes(r, m.cols, nnz, 8, 8, 8, 8)  # the reference's int64 indices
-    return {"value": round(b / t / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": kind,
+    line = {"value": round(b / t / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": kind,
             "sample": f"rows [0, {r}) of the same matrix ({nnz} nnz, int64 indices as in the "
-                      f"reference), compat SpMV FP64 on one host core, median of {reps} after "
-                      f"1 warm-up, {t * 1e3:.1f} ms per SpMV"}
+                      f"reference), compat SpMV FP64 pinned to host core {core}, median of "
+                      f"{reps} after 1 warm-up ({t * 1e3:.1f} ms per SpMV, spread "
+                      f"{min(times) * 1e3:.1f}-{max(times) * 1e3:.1f} ms)",
+            "host": facts}
+    return line, y
+
+
+def abs_ax_rows(m, x, rows=None, parts=16):
+    """(|A||x|)_i in FP64 with numpy (the guard's error-bound scale), over
+    all rows or the rows listed."""
+    lens = np.diff(m.offsets)
+    if rows is not None:
+        out = np.empty(rows.shape[0])
+        for k, r in enumerate(rows):
+            b, e = int(m.offsets[r]), int(m.offsets[r + 1])
+            out[k] = float(np.sum(np.abs(m.values[b:e]) * np.abs(x[m.indices[b:e]])))
+        return out
+    out = np.zeros(m.rows)
+    cuts = np.searchsorted(m.offsets, np.linspace(0, m.nnz, parts + 1)).tolist()
+    cuts[0], cuts[-1] = 0, m.rows
+    for r0, r1 in zip(cuts[:-1], cuts[1:]):
+        if r1 <= r0:
+            continue
+        b, e = int(m.offsets[r0]), int(m.offsets[r1])
+        prod = np.abs(m.values[b:e]) * np.abs(x[m.indices[b:e]])
+        nz = np.nonzero(lens[r0:r1])[0]
+        if nz.size:
+            out[r0 + nz] = np.add.reduceat(prod, m.offsets[r0:r1][nz] - b)
+        del prod
+    return out
+
+
+def bound_violations(y, yref, lens, absax, eps):
+    """Rows where |y - yref| exceeds the plain-mode bound 4*len*eps*(|A||x|)_i
+    (alpha = 1, beta = 0), and the largest error/bound ratio."""
+    bound = 4.0 * np.maximum(lens, 1) * eps * absax
+    err = np.abs(y.astype(np.float64) - yref.astype(np.float64))
+    bad = ~(err <= bound)
+    ratio = float(np.max(np.where(bound > 0, err / np.where(bound > 0, bound, 1.0), 0.0)))
+    return int(np.count_nonzero(bad)), ratio
 
 
 def learned_pick(ctx, a):
@@ -189,6 +261,22 @@ def bench_large(ctx, args, extra):
     res = time_ops(ctx, lambda: H.csr_spmv(ctx, 1.0, a, xd, 0.0, yd, alg=alg, ws=ws),
                    args.steps, args.warmup)
     log(f"FP64 {name}: {res['loop_ms']:.3f} ms/step, {nbytes / res['loop_ms'] / 1e6:.0f} GB/s")
+    # Guard (after timing, outside the timed region): every row of the timed
+    # SpMV's output against the ordered kernel's (the reference's summation,
+    # bit for bit; checked against the reference's own output on the CPU
+    # baseline's rows below) within the plain-mode bound.
+    y_head = ctx.download(yb, np.float64, m.rows)
+    H.csr_spmv(ctx, 1.0, a, xd, 0.0, yd, alg=H.ALG_ORDERED, ws=ws)
+    y_ord = ctx.download(yb, np.float64, m.rows)
+    t0 = time.perf_counter()
+    absax = abs_ax_rows(m, x)
+    nbad, ratio = bound_violations(y_head, y_ord, lens, absax, 2.0 ** -53)
+    guard = {"headline": {"rows_checked": m.rows, "rows_over_bound": nbad,
+                          "max_err_over_bound": round(ratio, 4),
+                          "against": "ordered kernel (reference summation order)"}}
+    log(f"guard: headline {nbad} rows over the bound (max err/bound {ratio:.3f}, "
+        f"{time.perf_counter() - t0:.1f}s)")
+    del y_head
     if args.sweep or not args.quick:
         sweep = {}
         for v in H.TUNABLE_ALGS:
@@ -207,7 +295,14 @@ def bench_large(ctx, args, extra):
         extra["fp32_same_matrix"] = {"gbs": round(b32 / r32["median_ms"] / 1e6, 1),
                                      "ms": round(r32["median_ms"], 3), "algorithm": name}
         log(f"FP32 {name}: {r32['median_ms']:.3f} ms")
-        del a32, xb32, yb32
+        y32 = ctx.download(yb32, np.float32, m.rows)
+        H.csr_spmv(ctx, 1.0, a32, x32d, 0.0, y32d, alg=H.ALG_ORDERED, ws=ws)
+        y32o = ctx.download(yb32, np.float32, m.rows)
+        # |A32||x32| <= |A||x| (1 + 2^-24)^2 for the FP32-rounded operands.
+        nbad, ratio = bound_violations(y32, y32o, lens, absax * (1 + 2.0 ** -22), 2.0 ** -24)
+        guard["fp32"] = {"rows_checked": m.rows, "rows_over_bound": nbad,
+                         "max_err_over_bound": round(ratio, 4)}
+        del a32, xb32, yb32, y32, y32o
         # SpMM: C += A*B, B row-major 2^22 x 32 (U[-1,1]).
         k = 32
         bmat = ctx.alloc(8 * m.cols * k)
@@ -224,7 +319,23 @@ def bench_large(ctx, args, extra):
         extra["spmm_k32_row_major"] = {"gbs": round(bs / rs["median_ms"] / 1e6, 1),
                                        "ms": round(rs["median_ms"], 3)}
         log(f"SpMM k=32: {rs['median_ms']:.2f} ms")
-        del bmat, cmat, bh
+        # Guard: sampled rows (the longest among them) recomputed in the
+        # reference's order, C(r,:) += (1*v)*B(c,:) entry by entry, bit for
+        # bit, after one more call from zero.
+        ctx.fill(cmat, 0)
+        H.standard_spmm(ctx, H.OP_N, 1.0, a, bd, cd, sws)
+        cgot = ctx.download(cmat, np.float64, m.rows * k).reshape(m.rows, k)
+        bhost = bh.reshape(m.cols, k)
+        pick = np.concatenate([np.random.default_rng(5).integers(0, m.rows, 256),
+                               np.argsort(lens)[-8:]])
+        exact = 0
+        for r in pick:
+            acc = np.zeros(k)
+            for p in range(int(m.offsets[r]), int(m.offsets[r + 1])):
+                acc = acc + (1.0 * m.values[p]) * bhost[m.indices[p]]
+            exact += int(np.array_equal(acc.view(np.uint64), cgot[r].view(np.uint64)))
+        guard["spmm_k32"] = {"rows_checked": int(pick.size), "rows_bit_exact": exact}
+        del bmat, cmat, bh, cgot, bhost
         # Conversions on the same matrix.
         cws = H.Workspace(ctx)
         holder = {}
@@ -234,6 +345,11 @@ def bench_large(ctx, args, extra):
         rc = time_ops(ctx, to_csc, 2, 1)
         bc = 2 * m.nnz * 12 + (m.rows + 1) * 8 + (m.cols + 1) * 8
         extra["csr_to_csc"] = {"gbs": round(bc / rc["median_ms"] / 1e6, 1), "ms": round(rc["median_ms"], 3)}
+        # Guard: the column counts (the full transpose is checked bit for bit
+        # by the GPU tests at this size).
+        off_c = ctx.download(H._ptr_buf(ctx, holder["csc"].desc.outer_offsets), np.int64, m.cols + 1)
+        guard["csr_to_csc"] = {"column_counts_match": bool(
+            off_c[0] == 0 and np.array_equal(np.diff(off_c), np.bincount(m.indices, minlength=m.cols)))}
         holder.clear()
         del cws
         coo = H.convert_to_coordinate(ctx, a)
@@ -244,9 +360,57 @@ def bench_large(ctx, args, extra):
         bcoo = m.nnz * (16 + 8) + m.nnz * (4 + 8) + (m.rows + 1) * 8
         extra["coo_to_csr"] = {"gbs": round(bcoo / rr["median_ms"] / 1e6, 1), "ms": round(rr["median_ms"], 3)}
         log(f"CSR->CSC {rc['median_ms']:.2f} ms, COO->CSR {rr['median_ms']:.2f} ms")
+        off_r = ctx.download(H._ptr_buf(ctx, holder["csr"].desc.outer_offsets), np.int64, m.rows + 1)
+        guard["coo_to_csr"] = {"offsets_match": bool(np.array_equal(off_r, m.offsets))}
         holder.clear()
         del coo
-    return m, alg, res, nbytes
+        bench_shuffled_finalize(ctx, m, extra, guard)
+    return m, alg, res, nbytes, y_ord, guard, x
+
+
+def bench_shuffled_finalize(ctx, m, extra, guard):
+    """COO->CSR through the sort path: the same matrix's entries as raw
+    triplets in a scrambled order (entry i placed at (a*i + b) mod nnz, a
+    coprime with nnz, which moves every entry far from its neighbours),
+    finalized on the device (stable sort by (row, col), duplicates summed,
+    zeros kept).  Algorithmic bytes: triplets read (16 + 8 per entry) and the
+    canonical result written (16 + 8 per entry)."""
+    import math
+    n = m.nnz
+    t0 = time.perf_counter()
+    a_mul = int(n * 0.6180339887)
+    while math.gcd(a_mul, n) != 1:
+        a_mul += 1
+    pos = (np.arange(n, dtype=np.int64) * a_mul + 12345) % n
+    src = np.empty(n, dtype=np.int64)
+    src[pos] = np.arange(n, dtype=np.int64)  # the entry placed at each position
+    del pos
+    coords = np.empty((n, 2), dtype=np.int64)
+    coords[:, 0] = np.repeat(np.arange(m.rows, dtype=np.int32), np.diff(m.offsets))[src]
+    coords[:, 1] = m.indices[src]
+    vals = m.values[src]
+    del src
+    t = H.triplets(ctx, m.rows, m.cols, coords, vals, H.F64)
+    del coords, vals
+    log(f"shuffled COO of {n} entries built in {time.perf_counter() - t0:.1f}s")
+    ws = H.Workspace(ctx)
+    holder = {}
+
+    def fin():
+        holder["out"] = H.coo_finalize(ctx, t, H.DUPLICATE_SUM, H.ZERO_KEEP, ws)
+    res = time_ops(ctx, fin, 3, 1)
+    kept = int(holder["out"].desc.nnz)
+    by = n * 24 + kept * 24
+    extra["coo_finalize_shuffled_same_matrix"] = {
+        "entries": n, "unique": kept, "ms": round(res["median_ms"], 3),
+        "gbs": round(by / res["median_ms"] / 1e6, 1),
+        "entries_per_s": round(n / (res["median_ms"] * 1e-3))}
+    csr = H.convert_coordinate(ctx, holder["out"], H.CSR, 4)
+    off_r = ctx.download(H._ptr_buf(ctx, csr.desc.outer_offsets), np.int64, m.rows + 1)
+    guard["coo_finalize_shuffled"] = {"unique_equals_nnz": kept == n,
+                                      "offsets_match": bool(np.array_equal(off_r, m.offsets))}
+    log(f"COO finalize of the shuffled matrix: {res['median_ms']:.2f} ms")
+    del holder, csr, t, ws
 
 
 def bench_precision(ctx, extra):
@@ -351,20 +515,53 @@ def bench_finalize(ctx, extra, n=1 << 26, dim=1 << 22):
     log(f"COO finalize of {n} triplets ({kept} unique): {res['median_ms']:.2f} ms")
 
 
+def guard_failures(guard):
+    """Names of the output checks that failed."""
+    bad = []
+    for k, v in guard.items():
+        if v.get("rows_over_bound", 0) != 0:
+            bad.append(k)
+        if "rows_bit_exact" in v and v["rows_bit_exact"] != v["rows_checked"]:
+            bad.append(k)
+        if any(val is False for val in v.values()):
+            bad.append(k)
+    return sorted(set(bad))
+
+
+def bench_hbm_ceiling(ctx, nbytes=4 << 30):
+    """Measured HBM ceilings: a plain nontemporal 16-byte read of 4 GiB and a
+    copy of 2 GiB (read + written bytes), from libasc_sparse_probe.so."""
+    from asc_sparse_hip import probe as P
+    src = ctx.alloc(nbytes)
+    ctx.fill(src, 1)
+    part = ctx.alloc(4 * P.STREAM_PROBE_BLOCKS)
+    r = time_ops(ctx, lambda: P.stream_read(ctx, src, nbytes, part), 10, 3)
+    half = nbytes // 2
+    dst = ctx.alloc(half)
+    c = time_ops(ctx, lambda: P.stream_copy(ctx, src, dst, half), 10, 3)
+    out = {"read_gbs": round(nbytes / r["median_ms"] / 1e6, 1),
+           "copy_gbs": round(2 * half / c["median_ms"] / 1e6, 1),
+           "source": "libasc_sparse_probe.so stream probes, 4 GiB read / 2 GiB copy"}
+    log(f"HBM ceiling: read {out['read_gbs']} GB/s, copy {out['copy_gbs']} GB/s")
+    del src, dst, part
+    return out
+
+
 def bench_gather_ceiling(ctx, extra, bytes_per_entry, n=1 << 26):
     """The random-gather ceiling behind the SpMV roofline: n streamed
     (int32 index, FP64 value) pairs, each gathering x at a uniformly random
     position, for x of 256 KiB (L2-resident), 2 MiB (one column panel) and
     32 MiB (the headline's x, MALL-resident).  `spmv_ceiling_gbs` is that
     rate times the headline's algorithmic bytes per nonzero."""
+    from asc_sparse_hip import probe as P
     rng = np.random.default_rng(G.SEED_BASE + 13)
     bv = ctx.upload(np.ones(n))
-    out = ctx.alloc(8 * H.GATHER_PROBE_BLOCKS)
+    out = ctx.alloc(8 * P.GATHER_PROBE_BLOCKS)
     res = {}
     for xn, tag in ((1 << 15, "x_256KiB_l2"), (1 << 18, "x_2MiB_panel"), (1 << 22, "x_32MiB_mall")):
         bi = ctx.upload(rng.integers(0, xn, n).astype(np.int32))
         bx = ctx.upload(G.vector(xn, 3))
-        r = time_ops(ctx, lambda: H.gather_probe(ctx, bi, bv, n, bx, xn, out), 5, 2)
+        r = time_ops(ctx, lambda: P.gather_probe(ctx, bi, bv, n, bx, xn, out), 5, 2)
         rate = n / (r["median_ms"] * 1e-3)
         res[tag] = {"ggathers_per_s": round(rate / 1e9, 1),
                     "spmv_ceiling_gbs": round(rate * bytes_per_entry / 1e9, 1)}
@@ -483,7 +680,8 @@ def main():
                     help="auto (column-panel plan, learned pick reported), merge, ordered, vecN")
     ap.add_argument("--staging", default="stream", choices=sorted(STAGINGS),
                     help="x-panel staging of the column-panel plan the timed steps run")
-    ap.add_argument("--cpu-sample-rows", type=int, default=1 << 20)
+    ap.add_argument("--cpu-sample-rows", type=int, default=1 << 18)
+    ap.add_argument("--cpu-reps", type=int, default=20)
     ap.add_argument("--batch-count", type=int, default=10_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline line only")
@@ -500,17 +698,23 @@ def main():
         sys.exit(1)
     ctx = H.Context(0)
     extra = {}
-    m, alg, res, nbytes = bench_large(ctx, args, extra)
+    m, alg, res, nbytes, y_ord, guard, x = bench_large(ctx, args, extra)
     name = H.ALG_NAMES[alg]
     gbs = nbytes / (res["loop_ms"] * 1e-3) / 1e9
     kgbs = nbytes / (res["mean_ms"] * 1e-3) / 1e9
     cpu = None
     if not args.no_cpu:
         t0 = time.perf_counter()
-        cpu = cpu_baseline(m, args.cpu_sample_rows)
+        cpu, y_cpu = cpu_baseline(m, x, args.cpu_sample_rows, args.cpu_reps)
         log(f"cpu baseline {cpu['value']} GB/s ({time.perf_counter() - t0:.1f}s)")
+        # The device's ordered kernel against the reference's own output.
+        guard["ordered_vs_cpu_reference"] = {
+            "rows": int(y_cpu.shape[0]),
+            "bit_exact": bool(np.array_equal(y_ord[: y_cpu.shape[0]].view(np.uint64),
+                                             y_cpu.view(np.uint64)))}
     rows, nnz = m.rows, m.nnz
-    del m
+    del m, y_ord
+    ceiling = bench_hbm_ceiling(ctx)
     if not args.quick:
         bench_gather_ceiling(ctx, extra, nbytes / nnz)
         bench_precision(ctx, extra)
@@ -542,12 +746,20 @@ def main():
                    "bytes_per_spmv": nbytes, "hbm_frac_of_peak": round(gbs / HBM_PEAK_GBS, 4)},
         "roofline": {"bound": "hbm", "achieved": round(kgbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(kgbs / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "traffic_source": traffic_src},
+                     "traffic": traffic, "traffic_source": traffic_src,
+                     "measured_ceiling": ceiling,
+                     "frac_of_measured_read": round(kgbs / ceiling["read_gbs"], 4)},
         "cpu_baseline": cpu,
+        "guard": guard,
         "extra": extra,
     }
+    failed = guard_failures(guard)
+    if failed:
+        out["error"] = "output check failed: " + ", ".join(failed)
     print(json.dumps(out), flush=True)
     ctx.close()
+    if failed:
+        sys.exit(3)
 
 
 if __name__ == "__main__":
