@@ -126,6 +126,7 @@ _SIGS = {
     "asc_sparse_hip_convert_values_scaled": [_VP, P(SparseDesc), C.c_uint8, C.c_double, _VP, P(SparseDesc)],
     "asc_sparse_hip_csr_spmv_workspace_size": [_VP, P(SparseDesc), P(VectorDesc), P(VectorDesc), P(SpmvOptions), P(C.c_size_t)],
     "asc_sparse_hip_csr_spmv": [_VP, C.c_double, P(SparseDesc), P(VectorDesc), C.c_double, P(VectorDesc), _VP, C.c_size_t, P(SpmvOptions)],
+    "asc_sparse_hip_csr_spmv_noalloc": [_VP, C.c_double, P(SparseDesc), P(VectorDesc), C.c_double, P(VectorDesc), P(SpmvOptions)],
     "asc_sparse_hip_standard_spmv_workspace_size": [_VP, C.c_uint8, P(SparseDesc), P(VectorDesc), P(VectorDesc), P(SpmvOptions), P(C.c_size_t)],
     "asc_sparse_hip_standard_spmv": [_VP, C.c_uint8, Scalar, P(SparseDesc), P(VectorDesc), P(VectorDesc), _VP, C.c_size_t, P(SpmvOptions)],
     "asc_sparse_hip_standard_spmm_workspace_size": [_VP, C.c_uint8, P(SparseDesc), P(MatrixDesc), P(MatrixDesc), P(C.c_size_t)],
@@ -441,6 +442,17 @@ def csr_spmv(ctx: Context, alpha: float, a: DeviceSparse, x: VectorDesc, beta: f
     check(ctx.lib.asc_sparse_hip_csr_spmv(ctx.handle, alpha, C.byref(a.desc), C.byref(x), beta,
                                           C.byref(y), p, n, C.byref(opt)))
     return ws
+
+
+NOALLOC_SCRATCH_BYTES = 32 << 20
+
+
+def csr_spmv_noalloc(ctx: Context, alpha: float, a: DeviceSparse, x: VectorDesc, beta: float,
+                     y: VectorDesc, alg: int = ALG_AUTO, acc: int = ACC_PLAIN):
+    """The allocation-free compat SpMV (the library's own scratch, no workspace)."""
+    opt = options(alg, acc)
+    check(ctx.lib.asc_sparse_hip_csr_spmv_noalloc(ctx.handle, alpha, C.byref(a.desc), C.byref(x),
+                                                  beta, C.byref(y), C.byref(opt)))
 
 
 def standard_spmv(ctx: Context, transpose: int, alpha: float, a: DeviceSparse, x: VectorDesc,

@@ -27,6 +27,9 @@ struct asc_hip_context {
   // asc_sparse_hip_context_synchronize, which then fails with
   // ASC_HIP_INTERNAL.
   uint32_t* fault_word = nullptr;
+  // This device's copy of the allocation-free entry point's scratch array
+  // (found at context creation; null for a validation-only context).
+  void* noalloc_scratch = nullptr;
   uint8_t determinism = ASC_HIP_DETERMINISTIC;
   // Bytes of B per SpMM column pass; <= 0: one pass.
   int64_t spmm_panel_bytes = 0;
@@ -104,5 +107,8 @@ asc_hip::Compressed as_compressed(const asc_hip_sparse_desc& d, bool row_view_of
 bool test_hook_consume_allocation_failure();
 bool test_hook_consume_event_failure();
 bool test_hook_consume_stream_stall();
+// Address of the current device's copy of the allocation-free scratch array
+// (resolved once per device; call with `device` current).
+asc_hip_status noalloc_scratch(int device, void** out);
 
 }  // namespace asc_hip_abi

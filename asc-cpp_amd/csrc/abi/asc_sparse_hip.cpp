@@ -440,6 +440,18 @@ asc_hip_status asc_sparse_hip_context_create(int32_t device, void* stream,
   if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->side.join, hipEventDisableTiming);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&ctx->fault_word), sizeof(uint32_t));
   if (e == hipSuccess) e = hipMemset(ctx->fault_word, 0, sizeof(uint32_t));
+  if (e == hipSuccess) {
+    const asc_hip_status sst = noalloc_scratch(device, &ctx->noalloc_scratch);
+    if (sst.code != ASC_HIP_OK) {
+      (void)hipFree(ctx->fault_word);
+      (void)hipEventDestroy(ctx->side.join);
+      (void)hipEventDestroy(ctx->side.fork);
+      (void)hipStreamDestroy(ctx->side.stream);
+      if (ctx->owns_stream) (void)hipStreamDestroy(ctx->stream);
+      delete ctx;
+      return sst;
+    }
+  }
   if (e != hipSuccess) {
     if (ctx->fault_word) (void)hipFree(ctx->fault_word);
     if (ctx->side.join) (void)hipEventDestroy(ctx->side.join);

@@ -500,6 +500,35 @@ asc_hip_status run_spmv(asc_hip_context* ctx, SpmvPlan& plan, void* ws) {
 
 }  // namespace
 
+// Device scratch of asc_sparse_hip_csr_spmv_noalloc: one copy per GPU the
+// library runs on, held until the process ends.  In host code the array's
+// name is not a device address: each device's copy is found once, with that
+// device current, by hipGetSymbolAddress, and only that pointer is used.
+static __device__ __align__(256) unsigned char scratch[ASC_HIP_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+}  // namespace
+
+namespace asc_hip_abi {
+asc_hip_status noalloc_scratch(int device, void** out) {
+  *out = nullptr;
+  if (device < 0 || device >= kMaxScratchDevices)
+    return fail(ASC_HIP_UNSUPPORTED, "device ordinal beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_ptr[device] == nullptr) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return fail_hip(e, "hipGetSymbolAddress of the allocation-free scratch");
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return ok();
+}
+}  // namespace asc_hip_abi
+
 extern "C" {
 
 // ---- staging ---------------------------------------------------------------
@@ -751,6 +780,31 @@ asc_hip_status asc_sparse_hip_csr_spmv(asc_hip_context* ctx, double alpha,
   plan.args.alpha = alpha;
   plan.args.beta = beta;
   return run_spmv(ctx, plan, ws);
+}
+
+// ---- allocation-free compat SpMV ------------------------------------------
+
+asc_hip_status asc_sparse_hip_csr_spmv_noalloc(asc_hip_context* ctx, double alpha,
+                                               const asc_hip_sparse_desc* a,
+                                               const asc_hip_vector_desc* x, double beta,
+                                               const asc_hip_vector_desc* y,
+                                               const asc_hip_spmv_options* opt) {
+  SpmvPlan plan;
+  asc_hip_status st = prepare_compat_spmv(ctx, a, x, y, opt, &plan);
+  if (st.code) return st;
+  if (plan.total > ASC_HIP_NOALLOC_SCRATCH_BYTES)
+    return fail(ASC_HIP_UNSUPPORTED,
+                "this SpMV needs " + std::to_string(plan.total) +
+                    " bytes of workspace, beyond the allocation-free scratch (" +
+                    std::to_string(ASC_HIP_NOALLOC_SCRATCH_BYTES) +
+                    " bytes): use asc_sparse_hip_csr_spmv with a workspace");
+  if ((st = unavailable_if_validation_only(ctx)).code) return st;
+  if (plan.noop) return ok();
+  if (ctx->noalloc_scratch == nullptr)
+    return fail(ASC_HIP_INVALID_STATE, "the allocation-free scratch is not set up");
+  plan.args.alpha = alpha;
+  plan.args.beta = beta;
+  return run_spmv(ctx, plan, ctx->noalloc_scratch);
 }
 
 // ---- column-panel analysis -----------------------------------------------

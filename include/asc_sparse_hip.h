@@ -321,6 +321,29 @@ asc_hip_status asc_sparse_hip_csr_spmv(
     const asc_hip_spmv_options* options);
 
 /*
+ * Allocation-free compat CSR SpMV.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory or pass a workspace (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same operands, options, validation and results as asc_sparse_hip_csr_spmv
+ * (it runs the same kernels), but the workspace is the library's own device
+ * scratch array of ASC_HIP_NOALLOC_SCRATCH_BYTES per GPU, held until the
+ * process ends; calls whose algorithm needs more (asc_sparse_hip_csr_spmv_
+ * workspace_size) fail with ASC_HIP_UNSUPPORTED before anything is enqueued.
+ * Merge path needs about 16 bytes per 512 (rows + nnz) plus 1 MiB, so it
+ * serves rows + nnz up to about 10^9; the vector and ordered kernels need
+ * none; the stream-staged plan 8 bytes per slice of 16384 entries; the L2 and
+ * LDS plans 8 bytes per (panel, listed row).  The call allocates nothing and
+ * enqueues no memset.  Every call on one GPU shares that GPU's scratch, so
+ * calls that overlap in time on one GPU (two streams, two contexts, two host
+ * threads) race: order them on one stream.
+ */
+#define ASC_HIP_NOALLOC_SCRATCH_BYTES (32u << 20)
+asc_hip_status asc_sparse_hip_csr_spmv_noalloc(
+    asc_hip_context* context, double alpha, const asc_hip_sparse_desc* matrix,
+    const asc_hip_vector_desc* x, double beta, const asc_hip_vector_desc* y,
+    const asc_hip_spmv_options* options);
+
+/*
  * Column-panel analysis for a trusted device CSR matrix.  No reference
  * counterpart: it is the analysis step of this library's ASC_HIP_ALG_PANEL
  * SpMV.  With L2 or LDS staging that SpMV reads the caller's CSR arrays in
