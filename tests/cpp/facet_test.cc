@@ -1,7 +1,12 @@
-// Tests of the C++20 facet (include/asc_sparse_hip/hip.hpp), written the way
-// the reference tests its CUDA facet (tests/sparse_cuda/sparse_cuda_test.cc):
-// golden vectors restated from the reference's tests, element types float and
-// double, and the rejection matrix.
+// Tests of the C++20 facet (include/asc_sparse_hip/hip.hpp) and its core
+// provider (core_hip.hpp), written the way the reference tests its CUDA facet
+// (tests/sparse_cuda/sparse_cuda_test.cc): golden vectors restated from the
+// reference's tests, element types float and double, and the rejection
+// matrix.  The same source builds standalone (the facet's own vocabulary) and
+// inside the asc-cpp tree (ASC_SPARSE_HIP_IN_ASC_TREE: the reference's core and
+// sparse headers and its compiled core library), so it checks that the facet
+// is a drop-in on the reference's types: uploads go through the reference's
+// CopyBytes on a HIP ExecutionContext, arrays come from a MemoryResource&.
 //
 //   facet_test validate   no GPU needed: host-view validation and every
 //                         rejection on a validation-only context
@@ -9,6 +14,9 @@
 //
 // Exit status 0 when every check passes; each failing check prints a line.
 #include <cmath>
+#include <cstdint>
+#include <memory>
+#include <utility>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -49,6 +57,9 @@ void Case(const char* name) {
 
 using asc::ErrorCode;
 using asc::MemorySpace;
+using Csr = asc::SparseCompressedFormat;
+
+asc::MutableMemoryView NoWorkspace() { return asc::MutableMemoryView(nullptr, 0, MemorySpace::kDevice); }
 
 // tests/sparse/blas_test.cc:123-125 and sparse_cuda_test.cc:240-341.
 const std::vector<asc::nnz_t> kOffsets3x4{0, 2, 4, 5};
@@ -141,69 +152,98 @@ void TestRejectionMatrix() {
   CHECK_CODE(asc::HipCsrSpmvWorkspaceSize(*ctx, *untrusted, *x, *y).status(),
              ErrorCode::kInvalidArgument);
   // A trusted device view reaches the enqueue point: kUnavailable.
-  const auto trusted = asc::HipProviderAccess::TrustedDevice<const double, asc::SparseCompressedFormat::kCsr>(
-      reinterpret_cast<const void*>(0x20000), reinterpret_cast<const void*>(0x30000), fake, 3, 4, 5, 8);
+  const auto trusted =
+      asc::internal_sparse_compressed::ProviderAccess::MakeCanonicalView<const double, Csr::kCsr>(
+          reinterpret_cast<const asc::nnz_t*>(0x20000), reinterpret_cast<const asc::index_t*>(0x30000),
+          fake, {3, 4}, 5, MemorySpace::kDevice);
   auto ws = asc::HipCsrSpmvWorkspaceSize(*ctx, trusted, *x, *y);
   CHECK(ws.ok());
-  CHECK_CODE(asc::HipCsrSpmv(*ctx, 1.0, trusted, *x, 0.0, *y, {}, asc::HipSpmvAlgorithm::kVector8).status(),
+  CHECK_CODE(asc::HipCsrSpmv(*ctx, 1.0, trusted, *x, 0.0, *y, NoWorkspace(), asc::HipSpmvAlgorithm::kVector8).status(),
              ErrorCode::kUnavailable);
   // Merge-path needs its workspace.
-  CHECK_CODE(asc::HipCsrSpmv(*ctx, 1.0, trusted, *x, 0.0, *y, {}, asc::HipSpmvAlgorithm::kMergePath).status(),
+  CHECK_CODE(asc::HipCsrSpmv(*ctx, 1.0, trusted, *x, 0.0, *y, NoWorkspace(), asc::HipSpmvAlgorithm::kMergePath).status(),
              ErrorCode::kInvalidArgument);
   // Shape mismatches (blas.h:140-205).
   auto x5 = asc::HipStridedVectorView<const double>::Create(reinterpret_cast<const double*>(0x40000), 5, 1);
-  CHECK_CODE(asc::HipCsrSpmv(*ctx, 1.0, trusted, *x5, 0.0, *y, {}).status(), ErrorCode::kShape);
+  CHECK_CODE(asc::HipCsrSpmv(*ctx, 1.0, trusted, *x5, 0.0, *y, NoWorkspace()).status(), ErrorCode::kShape);
   // Output overlapping the matrix values.
   auto y_on_values = asc::HipStridedVectorView<double>::Create(fake, 3, 1);
-  CHECK_CODE(asc::HipCsrSpmv(*ctx, 1.0, trusted, *x, 0.0, *y_on_values, {}).status(),
+  CHECK_CODE(asc::HipCsrSpmv(*ctx, 1.0, trusted, *x, 0.0, *y_on_values, NoWorkspace()).status(),
              ErrorCode::kInvalidArgument);
   // Host matrices are not device operands.
   auto host = asc::CsrView<const double>::Create(kOffsets3x4.data(), kIndices3x4.data(),
                                                 kValues3x4.data(), shape, 5, MemorySpace::kHost);
-  CHECK_CODE(asc::HipCsrSpmv(*ctx, 1.0, *host, *x, 0.0, *y, {}).status(), ErrorCode::kMemoryAccess);
+  CHECK_CODE(asc::HipCsrSpmv(*ctx, 1.0, *host, *x, 0.0, *y, NoWorkspace()).status(), ErrorCode::kMemoryAccess);
   // Column-panel algorithm without an analysis.
-  CHECK_CODE(asc::HipCsrSpmv(*ctx, 1.0, trusted, *x, 0.0, *y, {}, asc::HipSpmvAlgorithm::kColumnPanel)
+  CHECK_CODE(asc::HipCsrSpmv(*ctx, 1.0, trusted, *x, 0.0, *y, NoWorkspace(),
+                             asc::HipSpmvAlgorithm::kColumnPanel)
                  .status(),
              ErrorCode::kInvalidArgument);
   // Standardized SpMV with a wrong output extent.
   auto vx = asc::SparseBlasVectorView<const double>::Create(
-      reinterpret_cast<const double*>(0x40000), 4, 1, {reinterpret_cast<const void*>(0x40000), 32, MemorySpace::kDevice});
+      reinterpret_cast<const double*>(0x40000), 4, 1,
+      asc::ConstMemoryView(reinterpret_cast<const void*>(0x40000), 32, MemorySpace::kDevice));
   auto vy = asc::SparseBlasVectorView<double>::Create(
-      reinterpret_cast<double*>(0x50000), 4, 1, {reinterpret_cast<const void*>(0x50000), 32, MemorySpace::kDevice});
+      reinterpret_cast<double*>(0x50000), 4, 1,
+      asc::ConstMemoryView(reinterpret_cast<const void*>(0x50000), 32, MemorySpace::kDevice));
   CHECK_CODE(asc::HipSpmv(*ctx, asc::SparseBlasTranspose::kNone, 1.0, trusted, *vx, *vy).status(),
              ErrorCode::kShape);
+  // Core provider: a HIP context comes only from its factory, and the
+  // sparse context takes only a HIP execution context.
+  CHECK_CODE(asc::ExecutionContext::Create(asc::Backend::kHip, asc::Device{asc::Backend::kHip, 0})
+                 .status(),
+             ErrorCode::kUnsupported);
+  CHECK_CODE(asc::CreateHipExecutionContext(-1).status(), ErrorCode::kInvalidArgument);
+  CHECK_CODE(asc::CreateHipExecutionContext(1 << 20).status(), ErrorCode::kUnavailable);
+  CHECK_CODE(asc::SparseHipContext::Create(asc::ExecutionContext::Serial()).status(),
+             ErrorCode::kInvalidArgument);
+  CHECK_CODE(asc::RecordHipEvent(asc::ExecutionContext::Serial()).status(),
+             ErrorCode::kInvalidArgument);
+  CHECK_CODE(asc::HipMemoryResource::Create(0, MemorySpace::kHost).status(),
+             ErrorCode::kInvalidArgument);
 }
 
 // ---- GPU ---------------------------------------------------------------------------
 
 struct Gpu {
   asc::SparseHipContext ctx;
-  asc::HipMemoryResource device;
+  std::unique_ptr<asc::HipMemoryResource> device;
 };
 
-bool Copy(Gpu& g, void* dst, const void* src, std::size_t bytes) {
-  auto e = asc::HipCopy(g.ctx, dst, src, bytes);
+asc::Buffer Allocate(Gpu& g, std::size_t bytes) {
+  auto b = asc::Buffer::Allocate(*g.device, bytes, 256);
+  CHECK(b.ok());
+  return b.ok() ? std::move(*b) : asc::Buffer(std::move(*asc::Buffer::Allocate(*g.device, 0)));
+}
+
+asc::MutableMemoryView Whole(asc::Buffer& b) {
+  return asc::MutableMemoryView(b.data(), b.size(), MemorySpace::kDevice);
+}
+
+// Host <-> device through the reference's CopyBytes on the HIP context.
+bool Copy(Gpu& g, void* dst, MemorySpace dst_space, const void* src, MemorySpace src_space,
+          std::size_t bytes) {
+  auto e = asc::CopyBytes(g.ctx.execution_context(), asc::MutableMemoryView(dst, bytes, dst_space),
+                          asc::ConstMemoryView(src, bytes, src_space), bytes);
   return e.ok() && e->Wait().ok();
 }
-
-template <typename E>
-asc::HipBuffer Upload(Gpu& g, const std::vector<E>& host) {
-  auto b = asc::HipBuffer::Allocate(g.device, sizeof(E) * host.size() + 8);
-  CHECK(b.ok());
-  if (!host.empty()) {
-    auto e = asc::HipCopy(g.ctx, b->data(), host.data(), sizeof(E) * host.size());
-    CHECK(e.ok() && e->Wait().ok());
-  }
-  return *b;
+bool FromDevice(Gpu& g, void* dst, const void* src, std::size_t bytes) {
+  return Copy(g, dst, MemorySpace::kHost, src, MemorySpace::kDevice, bytes);
 }
 
 template <typename E>
-std::vector<E> Download(Gpu& g, const asc::HipBuffer& b, std::size_t n) {
+asc::Buffer Upload(Gpu& g, const std::vector<E>& host) {
+  asc::Buffer b = Allocate(g, sizeof(E) * host.size() + 8);
+  if (!host.empty())
+    CHECK(Copy(g, b.data(), MemorySpace::kDevice, host.data(), MemorySpace::kHost,
+               sizeof(E) * host.size()));
+  return b;
+}
+
+template <typename E>
+std::vector<E> Download(Gpu& g, const asc::Buffer& b, std::size_t n) {
   std::vector<E> host(n);
-  if (n) {
-    auto e = asc::HipCopy(g.ctx, host.data(), b.data(), sizeof(E) * n);
-    CHECK(e.ok() && e->Wait().ok());
-  }
+  if (n) CHECK(FromDevice(g, host.data(), b.data(), sizeof(E) * n));
   return host;
 }
 
@@ -215,7 +255,7 @@ asc::HipCsrArray<E> CloneCsr(Gpu& g, const std::vector<asc::nnz_t>& off, const s
   auto host = asc::CsrView<const E>::Create(off.data(), idx.data(), val.data(), shape,
                                             static_cast<asc::nnz_t>(idx.size()), MemorySpace::kHost);
   CHECK(host.ok());
-  auto clone = asc::HipCloneCsr(g.ctx, *host, g.device, width);
+  auto clone = asc::HipCloneCsr(g.ctx, *host, *g.device, width);
   CHECK(clone.ok());
   CHECK(clone->completion.Wait().ok());
   return std::move(clone->array);
@@ -228,34 +268,75 @@ const asc::HipSpmvAlgorithm kAlgorithms[] = {
 
 // sparse_cuda_test.cc:240-341: strided device golden, alpha=2, beta=-0.5 and
 // alpha=1, beta=0 with NaN in y.
+// The HIP core provider on the reference's seams: CopyBytes round trip
+// through pinned and device memory, events, a Buffer's release.
+void TestCoreProvider(Gpu& g) {
+  Case("core provider: CopyBytes, events, resources");
+  auto pinned = asc::HipMemoryResource::Create(0, MemorySpace::kPinnedHost);
+  CHECK(pinned.ok());
+  if (!pinned.ok()) return;
+  auto p = asc::Buffer::Allocate(**pinned, 64, 64);
+  CHECK(p.ok() && p->valid() && p->size() == 64);
+  const std::vector<double> src{1.5, -2.25, 3.0, 4.0e300, -0.0, 6, 7, 8};
+  std::vector<double> back(8, 0.0);
+  asc::Buffer d = Allocate(g, 64);
+  CHECK(Copy(g, p->data(), MemorySpace::kPinnedHost, src.data(), MemorySpace::kHost, 64));
+  CHECK(Copy(g, d.data(), MemorySpace::kDevice, p->data(), MemorySpace::kPinnedHost, 64));
+  CHECK(FromDevice(g, back.data(), d.data(), 64));
+  CHECK(std::memcmp(back.data(), src.data(), 64) == 0);
+  // A device view declared as host memory is refused.
+  CHECK_CODE(asc::CopyBytes(g.ctx.execution_context(),
+                            asc::MutableMemoryView(back.data(), 64, MemorySpace::kHost),
+                            asc::ConstMemoryView(d.data(), 64, MemorySpace::kHost), 64)
+                 .status(),
+             ErrorCode::kMemoryAccess);
+  auto ev = asc::RecordHipEvent(g.ctx.execution_context());
+  CHECK(ev.ok() && ev->valid() && ev->Wait().ok() && ev->is_complete());
+  auto q = ev->Query();
+  CHECK(q.ok() && *q);
+  CHECK(g.ctx.execution_context().CanAccess(MemorySpace::kDevice));
+  CHECK(g.ctx.Synchronize().ok());
+}
+
+template <typename E, typename Matrix>
+void SpmvGoldenOn(Gpu& g, const Matrix& cv) {
+  auto xb = Upload<E>(g, {1, -2, 3, 4, -1});
+  for (auto alg : kAlgorithms) {
+    auto yb = Upload<E>(g, {10, 20, 30, 40});
+    auto x = asc::HipStridedVectorView<const E>::Create(static_cast<const E*>(xb.data()), 5, 1);
+    auto y = asc::HipStridedVectorView<E>::Create(static_cast<E*>(yb.data()), 4, 1);
+    auto need = asc::HipCsrSpmvWorkspaceSize(g.ctx, cv, *x, *y, alg);
+    CHECK(need.ok());
+    asc::Buffer ws = Allocate(g, *need + 256);
+    auto done = asc::HipCsrSpmv(g.ctx, E(2), cv, *x, E(-0.5), *y, Whole(ws), alg);
+    CHECK(done.ok() && done->Wait().ok());
+    const auto out = Download<E>(g, yb, 4);
+    CHECK(out[0] == E(-9) && out[1] == E(-26) && out[2] == E(7) && out[3] == E(-24));
+    auto nanb = Upload<E>(g, {NAN, NAN, NAN, NAN});
+    auto yn = asc::HipStridedVectorView<E>::Create(static_cast<E*>(nanb.data()), 4, 1);
+    done = asc::HipCsrSpmv(g.ctx, E(1), cv, *x, E(0), *yn, Whole(ws), alg);
+    CHECK(done.ok() && done->Wait().ok());
+    const auto out2 = Download<E>(g, nanb, 4);
+    CHECK(out2[0] == E(-2) && out2[1] == E(-8) && out2[2] == E(11) && out2[3] == E(-2));
+  }
+}
+
 template <typename E>
 void TestCsrSpmvGolden(Gpu& g) {
   Case(sizeof(E) == 8 ? "compat SpMV golden (double)" : "compat SpMV golden (float)");
-  for (auto width : {asc::HipIndexWidth::k64, asc::HipIndexWidth::k32}) {
-    auto a = CloneCsr<E>(g, kOffsets4x5, kIndices4x5, Cast<E>(kValues4x5), 4, 5, width);
-    auto view = a.view();
-    CHECK(view.ok() && view->canonical_structure_trusted());
-    const asc::CsrView<const E> cv = *view;
-    auto xb = Upload<E>(g, {1, -2, 3, 4, -1});
-    for (auto alg : kAlgorithms) {
-      auto yb = Upload<E>(g, {10, 20, 30, 40});
-      auto x = asc::HipStridedVectorView<const E>::Create(static_cast<const E*>(xb.data()), 5, 1);
-      auto y = asc::HipStridedVectorView<E>::Create(static_cast<E*>(yb.data()), 4, 1);
-      auto need = asc::HipCsrSpmvWorkspaceSize(g.ctx, cv, *x, *y, alg);
-      CHECK(need.ok());
-      auto ws = asc::HipBuffer::Allocate(g.device, *need + 256);
-      auto done = asc::HipCsrSpmv(g.ctx, E(2), cv, *x, E(-0.5), *y, ws->mutable_view(), alg);
-      CHECK(done.ok() && done->Wait().ok());
-      const auto out = Download<E>(g, yb, 4);
-      CHECK(out[0] == E(-9) && out[1] == E(-26) && out[2] == E(7) && out[3] == E(-24));
-      auto nanb = Upload<E>(g, {NAN, NAN, NAN, NAN});
-      auto yn = asc::HipStridedVectorView<E>::Create(static_cast<E*>(nanb.data()), 4, 1);
-      done = asc::HipCsrSpmv(g.ctx, E(1), cv, *x, E(0), *yn, ws->mutable_view(), alg);
-      CHECK(done.ok() && done->Wait().ok());
-      const auto out2 = Download<E>(g, nanb, 4);
-      CHECK(out2[0] == E(-2) && out2[1] == E(-8) && out2[2] == E(11) && out2[3] == E(-2));
-    }
-  }
+  auto a = CloneCsr<E>(g, kOffsets4x5, kIndices4x5, Cast<E>(kValues4x5), 4, 5);
+  const auto& ca = a;
+  auto view = ca.view();
+  CHECK(view.ok() && view->canonical_structure_trusted());
+  CHECK_CODE(ca.narrow_view().status(), ErrorCode::kInvalidState);
+  SpmvGoldenOn<E>(g, *view);
+  // The explicit 32-bit-index clone, through its facet-side view.
+  auto n = CloneCsr<E>(g, kOffsets4x5, kIndices4x5, Cast<E>(kValues4x5), 4, 5, asc::HipIndexWidth::k32);
+  const auto& cn = n;
+  CHECK_CODE(cn.view().status(), ErrorCode::kInvalidState);
+  auto nview = cn.narrow_view();
+  CHECK(nview.ok());
+  SpmvGoldenOn<E>(g, *nview);
 }
 
 // blas_test.cc:221-242: [1e20, 1, -1e20] . [1, 1, 1] is exactly 0 in float
@@ -263,12 +344,12 @@ void TestCsrSpmvGolden(Gpu& g) {
 void TestSummationOrder(Gpu& g) {
   Case("summation order (float, ordered)");
   auto a = CloneCsr<float>(g, {0, 3}, {0, 1, 2}, {1e20f, 1.0f, -1e20f}, 1, 3);
-  const asc::CsrView<const float> cv = *a.view();
+  const asc::CsrView<const float> cv = *std::as_const(a).view();
   auto xb = Upload<float>(g, {1, 1, 1});
   auto yb = Upload<float>(g, {-1});
   auto x = asc::HipStridedVectorView<const float>::Create(static_cast<const float*>(xb.data()), 3, 1);
   auto y = asc::HipStridedVectorView<float>::Create(static_cast<float*>(yb.data()), 1, 1);
-  auto done = asc::HipCsrSpmv(g.ctx, 1.0f, cv, *x, 0.0f, *y, {}, asc::HipSpmvAlgorithm::kOrdered);
+  auto done = asc::HipCsrSpmv(g.ctx, 1.0f, cv, *x, 0.0f, *y, NoWorkspace(), asc::HipSpmvAlgorithm::kOrdered);
   CHECK(done.ok() && done->Wait().ok());
   CHECK(Download<float>(g, yb, 1)[0] == 0.0f);
 }
@@ -278,19 +359,19 @@ template <typename E>
 void TestStandard(Gpu& g) {
   Case(sizeof(E) == 8 ? "standardized SpMV/SpMM golden (double)" : "standardized SpMV/SpMM golden (float)");
   auto a = CloneCsr<E>(g, {0, 1, 3}, {0, 0, 1}, {2, 1, 3}, 2, 2);
-  const asc::CsrView<const E> cv = *a.view();
+  const asc::CsrView<const E> cv = *std::as_const(a).view();
   auto xb = Upload<E>(g, {1, -1});
-  const asc::ConstMemoryView xs{xb.data(), 2 * sizeof(E), MemorySpace::kDevice};
+  const asc::ConstMemoryView xs(xb.data(), 2 * sizeof(E), MemorySpace::kDevice);
   auto x = asc::SparseBlasVectorView<const E>::Create(static_cast<const E*>(xb.data()), 2, 1, xs);
   for (auto [op, expect] : {std::pair{asc::SparseBlasTranspose::kNone, std::array<E, 2>{2, -2}},
                             std::pair{asc::SparseBlasTranspose::kTranspose, std::array<E, 2>{1, -3}}}) {
     auto yb = Upload<E>(g, {0, 0});
-    const asc::ConstMemoryView ys{yb.data(), 2 * sizeof(E), MemorySpace::kDevice};
+    const asc::ConstMemoryView ys(yb.data(), 2 * sizeof(E), MemorySpace::kDevice);
     auto y = asc::SparseBlasVectorView<E>::Create(static_cast<E*>(yb.data()), 2, 1, ys);
     auto need = asc::HipSpmvWorkspaceSize(g.ctx, op, cv, *x, *y);
     CHECK(need.ok());
-    auto ws = asc::HipBuffer::Allocate(g.device, *need + 256);
-    auto done = asc::HipSpmv(g.ctx, op, E(1), cv, *x, *y, ws->mutable_view());
+    asc::Buffer ws = Allocate(g, *need + 256);
+    auto done = asc::HipSpmv(g.ctx, op, E(1), cv, *x, *y, Whole(ws));
     CHECK(done.ok() && done->Wait().ok());
     const auto out = Download<E>(g, yb, 2);
     CHECK(out[0] == expect[0] && out[1] == expect[1]);
@@ -298,8 +379,8 @@ void TestStandard(Gpu& g) {
   // Row-major SpMM: C += A*B with B = [[1,2],[3,4]] gives [[2,4],[10,14]].
   auto bb = Upload<E>(g, {1, 2, 3, 4});
   auto cb = Upload<E>(g, {0, 0, 0, 0});
-  const asc::ConstMemoryView bs{bb.data(), 4 * sizeof(E), MemorySpace::kDevice};
-  const asc::ConstMemoryView cs{cb.data(), 4 * sizeof(E), MemorySpace::kDevice};
+  const asc::ConstMemoryView bs(bb.data(), 4 * sizeof(E), MemorySpace::kDevice);
+  const asc::ConstMemoryView cs(cb.data(), 4 * sizeof(E), MemorySpace::kDevice);
   auto b = asc::SparseBlasMatrixView<const E>::Create(static_cast<const E*>(bb.data()), 2, 2,
                                                       asc::SparseBlasLayout::kRowMajor, 2, bs);
   auto c = asc::SparseBlasMatrixView<E>::Create(static_cast<E*>(cb.data()), 2, 2,
@@ -320,57 +401,57 @@ template <typename E>
 void TestConversions(Gpu& g) {
   Case(sizeof(E) == 8 ? "conversions golden (double)" : "conversions golden (float)");
   auto a = CloneCsr<E>(g, kOffsets3x4, kIndices3x4, Cast<E>(kValues3x4), 3, 4);
-  const asc::CsrView<const E> cv = *a.view();
+  const asc::CsrView<const E> cv = *std::as_const(a).view();
   auto need = asc::HipTransposeWorkspaceSize(g.ctx, cv);
   CHECK(need.ok());
-  auto ws = asc::HipBuffer::Allocate(g.device, *need + 256);
-  auto csc = asc::HipConvertToCsc(g.ctx, cv, g.device, ws->mutable_view());
+  asc::Buffer ws = Allocate(g, *need + 256);
+  auto csc = asc::HipConvertToCsc(g.ctx, cv, *g.device, Whole(ws));
   CHECK(csc.ok() && csc->completion.Wait().ok());
   {
-    auto view = csc->array.view();
+    auto view = std::as_const(csc->array).view();
     CHECK(view.ok());
     std::vector<asc::nnz_t> o(5);
     std::vector<asc::index_t> i(5);
     std::vector<E> v(5);
-    CHECK(Copy(g, o.data(), view->outer_offsets(), 5 * 8));
-    CHECK(Copy(g, i.data(), view->inner_indices(), 5 * 8));
-    CHECK(Copy(g, v.data(), view->values(), 5 * sizeof(E)));
+    CHECK(FromDevice(g, o.data(), view->outer_offsets(), 5 * 8));
+    CHECK(FromDevice(g, i.data(), view->inner_indices(), 5 * 8));
+    CHECK(FromDevice(g, v.data(), view->values(), 5 * sizeof(E)));
     CHECK((o == std::vector<asc::nnz_t>{0, 1, 3, 4, 5}));
     CHECK((i == std::vector<asc::index_t>{1, 0, 2, 1, 0}));
     CHECK((v == std::vector<E>{4, 2, 3, 5, -1}));
     // CSC -> CSR returns the original arrays.
     const asc::CscView<const E> ccv = *view;
-    auto back = asc::HipConvertToCsr(g.ctx, ccv, g.device, ws->mutable_view());
+    auto back = asc::HipConvertToCsr(g.ctx, ccv, *g.device, Whole(ws));
     CHECK(back.ok() && back->completion.Wait().ok());
-    auto bv = back->array.view();
+    auto bv = std::as_const(back->array).view();
     std::vector<asc::nnz_t> bo(4);
     std::vector<asc::index_t> bi(5);
     std::vector<E> bvals(5);
-    CHECK(Copy(g, bo.data(), bv->outer_offsets(), 4 * 8));
-    CHECK(Copy(g, bi.data(), bv->inner_indices(), 5 * 8));
-    CHECK(Copy(g, bvals.data(), bv->values(), 5 * sizeof(E)));
+    CHECK(FromDevice(g, bo.data(), bv->outer_offsets(), 4 * 8));
+    CHECK(FromDevice(g, bi.data(), bv->inner_indices(), 5 * 8));
+    CHECK(FromDevice(g, bvals.data(), bv->values(), 5 * sizeof(E)));
     CHECK(bo == kOffsets3x4);
     CHECK(bi == kIndices3x4);
     CHECK(bvals == Cast<E>(kValues3x4));
   }
   // CSR -> COO -> CSR.
-  auto coo = asc::HipConvertToCoordinate(g.ctx, cv, g.device);
+  auto coo = asc::HipConvertToCoordinate(g.ctx, cv, *g.device);
   CHECK(coo.ok() && coo->completion.Wait().ok());
   auto coov = coo->array.view();
   CHECK(coov.ok());
   std::vector<asc::index_t> pairs(10);
-  CHECK(Copy(g, pairs.data(), coov->coordinates(), 10 * 8));
+  CHECK(FromDevice(g, pairs.data(), coov->coordinates(), 10 * 8));
   CHECK((pairs == std::vector<asc::index_t>{0, 1, 0, 3, 1, 0, 1, 2, 2, 1}));
   auto cneed = asc::HipConvertCoordinateWorkspaceSize(g.ctx, *coov, asc::SparseCompressedFormat::kCsr);
   CHECK(cneed.ok());
-  auto cws = asc::HipBuffer::Allocate(g.device, *cneed + 256);
-  auto csr = asc::HipConvertToCsr(g.ctx, *coov, g.device, cws->mutable_view());
+  asc::Buffer cws = Allocate(g, *cneed + 256);
+  auto csr = asc::HipConvertToCsr(g.ctx, *coov, *g.device, Whole(cws));
   CHECK(csr.ok() && csr->completion.Wait().ok());
-  auto rv = csr->array.view();
+  auto rv = std::as_const(csr->array).view();
   std::vector<asc::nnz_t> ro(4);
   std::vector<asc::index_t> ri(5);
-  CHECK(Copy(g, ro.data(), rv->outer_offsets(), 4 * 8));
-  CHECK(Copy(g, ri.data(), rv->inner_indices(), 5 * 8));
+  CHECK(FromDevice(g, ro.data(), rv->outer_offsets(), 4 * 8));
+  CHECK(FromDevice(g, ri.data(), rv->inner_indices(), 5 * 8));
   CHECK(ro == kOffsets3x4);
   CHECK(ri == kIndices3x4);
 }
@@ -382,24 +463,37 @@ void TestColumnPanel(Gpu& g, uint8_t staging) {
   Case(staging == ASC_HIP_STAGING_STREAM ? "column-panel plan, stream staging (double)"
                                          : "column-panel plan (double)");
   auto a = CloneCsr<double>(g, kOffsets4x5, kIndices4x5, kValues4x5, 4, 5);
-  const asc::CsrView<const double> cv = *a.view();
+  const asc::CsrView<const double> cv = *std::as_const(a).view();
   asc_hip_panel_options opt{1, 1, ASC_HIP_ALG_AUTO, ASC_HIP_ALG_AUTO, staging, {}};
   auto need = asc::HipCsrSpmvAnalysisSize(g.ctx, cv, &opt);
   CHECK(need.ok());
-  auto analysis = asc::HipBuffer::Allocate(g.device, *need);
-  CHECK(asc::HipCsrSpmvAnalyze(g.ctx, cv, analysis->mutable_view(), &opt).ok());
+  asc::Buffer analysis = Allocate(g, *need);
+  CHECK(asc::HipCsrSpmvAnalyze(g.ctx, cv, Whole(analysis), &opt).ok());
   auto xb = Upload<double>(g, {1, -2, 3, 4, -1});
   auto yb = Upload<double>(g, {10, 20, 30, 40});
   auto x = asc::HipStridedVectorView<const double>::Create(static_cast<const double*>(xb.data()), 5, 1);
   auto y = asc::HipStridedVectorView<double>::Create(static_cast<double*>(yb.data()), 4, 1);
   auto wneed = asc::HipCsrSpmvWorkspaceSize(g.ctx, cv, *x, *y, asc::HipSpmvAlgorithm::kColumnPanel);
   CHECK(wneed.ok());
-  auto ws = asc::HipBuffer::Allocate(g.device, *wneed + 256);
-  auto done = asc::HipCsrSpmv(g.ctx, 2.0, cv, *x, -0.5, *y, ws->mutable_view(),
+  asc::Buffer ws = Allocate(g, *wneed + 256);
+  auto done = asc::HipCsrSpmv(g.ctx, 2.0, cv, *x, -0.5, *y, Whole(ws),
                               asc::HipSpmvAlgorithm::kColumnPanel);
   CHECK(done.ok() && done->Wait().ok());
   const auto out = Download<double>(g, yb, 4);
   CHECK(out[0] == -9 && out[1] == -26 && out[2] == 7 && out[3] == -24);
+  if (staging == ASC_HIP_STAGING_STREAM) {
+    // Values changed in place: the stream plan's copy is refreshed, then the
+    // plan gives the doubled product (-18, -52, 14, -48 with beta 0).
+    std::vector<double> doubled;
+    for (double v : kValues4x5) doubled.push_back(2 * v);
+    CHECK(Copy(g, const_cast<double*>(cv.values()), MemorySpace::kDevice, doubled.data(),
+               MemorySpace::kHost, doubled.size() * sizeof(double)));
+    auto refreshed = asc::HipCsrSpmvRefreshValues(g.ctx, cv);
+    CHECK(refreshed.ok() && refreshed->Wait().ok());
+    done = asc::HipCsrSpmv(g.ctx, 2.0, cv, *x, 0.0, *y, Whole(ws), asc::HipSpmvAlgorithm::kColumnPanel);
+    CHECK(done.ok() && done->Wait().ok());
+    CHECK((Download<double>(g, yb, 4) == std::vector<double>{-8, -32, 44, -8}));
+  }
   CHECK(asc_sparse_hip_plan_clear(g.ctx.native()).code == ASC_HIP_OK);
 }
 
@@ -410,7 +504,7 @@ template <typename E>
 void TestRebind(Gpu& g) {
   Case(sizeof(E) == 8 ? "rebound values (double)" : "rebound values (float)");
   auto a = CloneCsr<E>(g, kOffsets4x5, kIndices4x5, Cast<E>(kValues4x5), 4, 5);
-  const asc::CsrView<const E> cv = *a.view();
+  const asc::CsrView<const E> cv = *std::as_const(a).view();
   std::vector<E> doubled;
   for (double v : kValues4x5) doubled.push_back(E(2 * v));
   auto vb = Upload<E>(g, doubled);
@@ -420,8 +514,8 @@ void TestRebind(Gpu& g) {
   asc_hip_panel_options opt{1, 1, ASC_HIP_ALG_AUTO, ASC_HIP_ALG_AUTO, {}};
   auto need = asc::HipCsrSpmvAnalysisSize(g.ctx, cv, &opt);
   CHECK(need.ok());
-  auto analysis = asc::HipBuffer::Allocate(g.device, *need);
-  CHECK(asc::HipCsrSpmvAnalyze(g.ctx, cv, analysis->mutable_view(), &opt).ok());
+  asc::Buffer analysis = Allocate(g, *need);
+  CHECK(asc::HipCsrSpmvAnalyze(g.ctx, cv, Whole(analysis), &opt).ok());
   auto xb = Upload<E>(g, {1, -2, 3, 4, -1});
   auto x = asc::HipStridedVectorView<const E>::Create(static_cast<const E*>(xb.data()), 5, 1);
   for (auto alg : {asc::HipSpmvAlgorithm::kOrdered, asc::HipSpmvAlgorithm::kMergePath,
@@ -430,8 +524,8 @@ void TestRebind(Gpu& g) {
     auto y = asc::HipStridedVectorView<E>::Create(static_cast<E*>(yb.data()), 4, 1);
     auto wneed = asc::HipCsrSpmvWorkspaceSize(g.ctx, *rebound, *x, *y, alg);
     CHECK(wneed.ok());
-    auto ws = asc::HipBuffer::Allocate(g.device, *wneed + 256);
-    auto done = asc::HipCsrSpmv(g.ctx, E(1), *rebound, *x, E(0), *y, ws->mutable_view(), alg);
+    asc::Buffer ws = Allocate(g, *wneed + 256);
+    auto done = asc::HipCsrSpmv(g.ctx, E(1), *rebound, *x, E(0), *y, Whole(ws), alg);
     CHECK(done.ok() && done->Wait().ok());
     CHECK((Download<E>(g, yb, 4) == std::vector<E>{-4, -16, 22, -4}));
   }
@@ -439,15 +533,14 @@ void TestRebind(Gpu& g) {
   // allocation (a size the runtime keeps as requested): the host rebind
   // cannot see the allocation, the operation refuses it and leaves y as it was.
   constexpr std::size_t kBlock = std::size_t{1} << 22;
-  auto block = asc::HipBuffer::Allocate(g.device, kBlock);
-  CHECK(block.ok());
-  if (!block.ok()) return;
-  const E* tail = reinterpret_cast<const E*>(static_cast<const char*>(block->data()) + kBlock) - 7;
+  asc::Buffer block = Allocate(g, kBlock);
+  if (block.data() == nullptr) return;
+  const E* tail = reinterpret_cast<const E*>(static_cast<const char*>(block.data()) + kBlock) - 7;
   auto too_short = cv.RebindValues(tail);
   CHECK(too_short.ok());
   auto yb = Upload<E>(g, {1, 2, 3, 4});
   auto y = asc::HipStridedVectorView<E>::Create(static_cast<E*>(yb.data()), 4, 1);
-  CHECK_CODE(asc::HipCsrSpmv(g.ctx, E(1), *too_short, *x, E(0), *y, asc::MutableMemoryView{},
+  CHECK_CODE(asc::HipCsrSpmv(g.ctx, E(1), *too_short, *x, E(0), *y, NoWorkspace(),
                              asc::HipSpmvAlgorithm::kOrdered)
                  .status(),
              ErrorCode::kMemoryAccess);
@@ -464,29 +557,31 @@ void TestFinalize(Gpu& g) {
   auto vb = Upload<double>(g, {5, 2, -1, 0, 7});
   const auto* c = static_cast<const asc::index_t*>(cb.data());
   const auto* v = static_cast<const double*>(vb.data());
-  auto sum = asc::HipFinalizeCoordinates(g.ctx, c, v, 3, 4, 5, ASC_HIP_DUPLICATE_SUM,
-                                         ASC_HIP_ZERO_KEEP, g.device);
+  using asc::DuplicatePolicy;
+  using asc::ExplicitZeroPolicy;
+  auto sum = asc::HipFinalizeCoordinates(g.ctx, c, v, 3, 4, 5, DuplicatePolicy::kSum,
+                                         ExplicitZeroPolicy::kKeep, *g.device);
   CHECK(sum.ok());
   if (sum.ok()) {
     auto view = sum->view();
     CHECK(view.ok() && view->nnz() == 4);
     std::vector<asc::index_t> pairs(8);
     std::vector<double> vals(4);
-    CHECK(Copy(g, pairs.data(), view->coordinates(), 8 * 8));
-    CHECK(Copy(g, vals.data(), view->values(), 4 * 8));
+    CHECK(FromDevice(g, pairs.data(), view->coordinates(), 8 * 8));
+    CHECK(FromDevice(g, vals.data(), view->values(), 4 * 8));
     CHECK((pairs == std::vector<asc::index_t>{0, 1, 0, 3, 1, 0, 2, 1}));
     CHECK((vals == std::vector<double>{7, 2, 0, 4}));
   }
-  CHECK_CODE(asc::HipFinalizeCoordinates(g.ctx, c, v, 3, 4, 5, ASC_HIP_DUPLICATE_REJECT,
-                                         ASC_HIP_ZERO_KEEP, g.device)
+  CHECK_CODE(asc::HipFinalizeCoordinates(g.ctx, c, v, 3, 4, 5, DuplicatePolicy::kReject,
+                                         ExplicitZeroPolicy::kKeep, *g.device)
                  .status(),
              ErrorCode::kInvalidArgument);
-  auto drop = asc::HipFinalizeCoordinates(g.ctx, c, v, 3, 4, 5, ASC_HIP_DUPLICATE_SUM,
-                                          ASC_HIP_ZERO_DROP, g.device);
+  auto drop = asc::HipFinalizeCoordinates(g.ctx, c, v, 3, 4, 5, DuplicatePolicy::kSum,
+                                          ExplicitZeroPolicy::kDrop, *g.device);
   CHECK(drop.ok() && drop->view()->nnz() == 3);
   // A coordinate outside the shape.
-  CHECK_CODE(asc::HipFinalizeCoordinates(g.ctx, c, v, 2, 4, 5, ASC_HIP_DUPLICATE_SUM,
-                                         ASC_HIP_ZERO_KEEP, g.device)
+  CHECK_CODE(asc::HipFinalizeCoordinates(g.ctx, c, v, 2, 4, 5, DuplicatePolicy::kSum,
+                                         ExplicitZeroPolicy::kKeep, *g.device)
                  .status(),
              ErrorCode::kIndex);
 }
@@ -499,17 +594,17 @@ void TestLevelOne(Gpu& g) {
   const std::vector<asc::index_t> idx{0, 2};
   const std::vector<E> vals{1, -2};
   auto host = asc::SparseBlasIndexedVectorView<const E>::Create(
-      idx.data(), vals.data(), 2, 3, {idx.data(), 16, MemorySpace::kHost},
-      {vals.data(), 2 * sizeof(E), MemorySpace::kHost});
+      idx.data(), vals.data(), 2, 3, asc::ConstMemoryView(idx.data(), 16, MemorySpace::kHost),
+      asc::ConstMemoryView(vals.data(), 2 * sizeof(E), MemorySpace::kHost));
   CHECK(host.ok());
-  auto clone = asc::HipCloneIndexedVector(g.ctx, *host, g.device);
+  auto clone = asc::HipCloneIndexedVector(g.ctx, *host, *g.device);
   CHECK(clone.ok() && clone->completion.Wait().ok());
   auto sp = clone->array.view();
   const asc::SparseBlasIndexedVectorView<const E> csp = *sp;
   auto xb = Upload<E>(g, {3, 9, 2});
   auto rb = Upload<E>(g, {7});
-  const asc::ConstMemoryView xs{xb.data(), 3 * sizeof(E), MemorySpace::kDevice};
-  const asc::ConstMemoryView rs{rb.data(), sizeof(E), MemorySpace::kDevice};
+  const asc::ConstMemoryView xs(xb.data(), 3 * sizeof(E), MemorySpace::kDevice);
+  const asc::ConstMemoryView rs(rb.data(), sizeof(E), MemorySpace::kDevice);
   auto x = asc::SparseBlasVectorView<E>::Create(static_cast<E*>(xb.data()), 3, 1, xs);
   auto cx = asc::SparseBlasVectorView<const E>::Create(static_cast<const E*>(xb.data()), 3, 1, xs);
   auto r = asc::SparseBlasVectorView<E>::Create(static_cast<E*>(rb.data()), 1, 1, rs);
@@ -525,7 +620,7 @@ void TestLevelOne(Gpu& g) {
   CHECK(gz.ok() && gz->Wait().ok());
   CHECK((Download<E>(g, xb, 3) == std::vector<E>{0, 9, 0}));
   std::vector<E> gathered(2);
-  CHECK(Copy(g, gathered.data(), sp->values(), 2 * sizeof(E)));
+  CHECK(FromDevice(g, gathered.data(), sp->values(), 2 * sizeof(E)));
   CHECK((gathered == std::vector<E>{5, -2}));
   auto sc = asc::HipSparseScatter(g.ctx, csp, *x);
   CHECK(sc.ok() && sc->Wait().ok());
@@ -534,8 +629,8 @@ void TestLevelOne(Gpu& g) {
   CHECK(ga.ok() && ga->Wait().ok());
   // An untrusted device view is refused (blas.h:1231-1239 for the host form).
   auto untrusted = asc::SparseBlasIndexedVectorView<const E>::Create(
-      sp->indices(), sp->values(), 2, 3, {sp->indices(), 16, MemorySpace::kDevice},
-      {sp->values(), 2 * sizeof(E), MemorySpace::kDevice});
+      sp->indices(), sp->values(), 2, 3, asc::ConstMemoryView(sp->indices(), 16, MemorySpace::kDevice),
+      asc::ConstMemoryView(sp->values(), 2 * sizeof(E), MemorySpace::kDevice));
   CHECK(untrusted.ok());
   CHECK_CODE(asc::HipSparseAxpy(g.ctx, E(1), *untrusted, *x).status(), ErrorCode::kInvalidArgument);
 }
@@ -545,22 +640,22 @@ template <typename E>
 void TestTriangular(Gpu& g) {
   Case(sizeof(E) == 8 ? "triangular solve golden (double)" : "triangular solve golden (float)");
   auto a = CloneCsr<E>(g, {0, 1, 3}, {0, 0, 1}, {2, 1, 3}, 2, 2);
-  const asc::CsrView<const E> cv = *a.view();
+  const asc::CsrView<const E> cv = *std::as_const(a).view();
   auto plan = asc::HipTriangularPlan<E, asc::SparseCompressedFormat::kCsr>::Analyze(
       g.ctx, cv, asc::SparseBlasTriangle::kLower, asc::SparseBlasDiagonal::kNonUnit,
-      asc::SparseBlasTranspose::kNone, g.device);
+      asc::SparseBlasTranspose::kNone, *g.device);
   CHECK(plan.ok());
   if (!plan.ok()) return;
   CHECK(plan->levels() == 2);
   auto xb = Upload<E>(g, {4, -1});
-  const asc::ConstMemoryView xs{xb.data(), 2 * sizeof(E), MemorySpace::kDevice};
+  const asc::ConstMemoryView xs(xb.data(), 2 * sizeof(E), MemorySpace::kDevice);
   auto x = asc::SparseBlasVectorView<E>::Create(static_cast<E*>(xb.data()), 2, 1, xs);
   auto done = plan->Solve(g.ctx, E(1), *x);
   CHECK(done.ok() && done->Wait().ok());
   CHECK((Download<E>(g, xb, 2) == std::vector<E>{2, -1}));
   // Several right-hand sides, row-major: [[4, 2], [-1, 10]] -> [[2, 1], [-1, 3]].
   auto bb = Upload<E>(g, {4, 2, -1, 10});
-  const asc::ConstMemoryView bs{bb.data(), 4 * sizeof(E), MemorySpace::kDevice};
+  const asc::ConstMemoryView bs(bb.data(), 4 * sizeof(E), MemorySpace::kDevice);
   auto bm = asc::SparseBlasMatrixView<E>::Create(static_cast<E*>(bb.data()), 2, 2,
                                                  asc::SparseBlasLayout::kRowMajor, 2, bs);
   done = plan->Solve(g.ctx, E(1), *bm);
@@ -568,10 +663,10 @@ void TestTriangular(Gpu& g) {
   CHECK((Download<E>(g, bb, 4) == std::vector<E>{2, 1, -1, 3}));
   // A singular non-unit diagonal is refused.
   auto s = CloneCsr<E>(g, {0, 1, 2}, {0, 1}, {1, 0}, 2, 2);
-  const asc::CsrView<const E> sv = *s.view();
+  const asc::CsrView<const E> sv = *std::as_const(s).view();
   CHECK_CODE((asc::HipTriangularPlan<E, asc::SparseCompressedFormat::kCsr>::Analyze(
                   g.ctx, sv, asc::SparseBlasTriangle::kLower, asc::SparseBlasDiagonal::kNonUnit,
-                  asc::SparseBlasTranspose::kNone, g.device)
+                  asc::SparseBlasTranspose::kNone, *g.device)
                   .status()),
              ErrorCode::kInvalidArgument);
 }
@@ -593,9 +688,11 @@ int main(int argc, char** argv) {
     auto ctx = asc::SparseHipContext::Create(*exec);
     CHECK(ctx.ok());
     if (!ctx.ok()) return 3;
-    auto res = asc::HipMemoryResource::Create(*ctx, MemorySpace::kDevice);
+    auto res = asc::HipMemoryResource::Create(0, MemorySpace::kDevice);
     CHECK(res.ok());
-    Gpu g{std::move(*ctx), *res};
+    if (!res.ok()) return 3;
+    Gpu g{std::move(*ctx), std::move(*res)};
+    TestCoreProvider(g);
     TestCsrSpmvGolden<double>(g);
     TestCsrSpmvGolden<float>(g);
     TestSummationOrder(g);
