@@ -129,8 +129,8 @@ _SIGS = {
     "asc_sparse_hip_csr_spmv_noalloc": [_VP, C.c_double, P(SparseDesc), P(VectorDesc), C.c_double, P(VectorDesc), P(SpmvOptions)],
     "asc_sparse_hip_standard_spmv_workspace_size": [_VP, C.c_uint8, P(SparseDesc), P(VectorDesc), P(VectorDesc), P(SpmvOptions), P(C.c_size_t)],
     "asc_sparse_hip_standard_spmv": [_VP, C.c_uint8, Scalar, P(SparseDesc), P(VectorDesc), P(VectorDesc), _VP, C.c_size_t, P(SpmvOptions)],
-    "asc_sparse_hip_standard_spmm_workspace_size": [_VP, C.c_uint8, P(SparseDesc), P(MatrixDesc), P(MatrixDesc), P(C.c_size_t)],
-    "asc_sparse_hip_standard_spmm": [_VP, C.c_uint8, Scalar, P(SparseDesc), P(MatrixDesc), P(MatrixDesc), _VP, C.c_size_t],
+    "asc_sparse_hip_standard_spmm_workspace_size": [_VP, C.c_uint8, P(SparseDesc), P(MatrixDesc), P(MatrixDesc), P(SpmvOptions), P(C.c_size_t)],
+    "asc_sparse_hip_standard_spmm": [_VP, C.c_uint8, Scalar, P(SparseDesc), P(MatrixDesc), P(MatrixDesc), _VP, C.c_size_t, P(SpmvOptions)],
     "asc_sparse_hip_convert_coordinate_workspace_size": [_VP, P(SparseDesc), C.c_uint8, C.c_uint8, P(C.c_size_t)],
     "asc_sparse_hip_convert_coordinate": [_VP, P(SparseDesc), C.c_uint8, C.c_uint8, _VP, _VP, _VP, P(SparseDesc), _VP, C.c_size_t],
     "asc_sparse_hip_convert_compressed_workspace_size": [_VP, P(SparseDesc), P(C.c_size_t)],
@@ -471,14 +471,18 @@ def standard_spmv(ctx: Context, transpose: int, alpha: float, a: DeviceSparse, x
 
 
 def standard_spmm(ctx: Context, transpose: int, alpha: float, a: DeviceSparse, b: MatrixDesc,
-                  c: MatrixDesc, ws: Workspace | None = None):
+                  c: MatrixDesc, ws: Workspace | None = None, alg: int = ALG_AUTO):
+    """C += alpha*op(A)*B; alg ALG_AUTO (plain form) or ALG_ORDERED (bit-exact)."""
+    opt = options(alg)
     need = C.c_size_t()
     check(ctx.lib.asc_sparse_hip_standard_spmm_workspace_size(
-        ctx.handle, transpose, C.byref(a.desc), C.byref(b), C.byref(c), C.byref(need)))
+        ctx.handle, transpose, C.byref(a.desc), C.byref(b), C.byref(c), C.byref(opt),
+        C.byref(need)))
     ws = ws or Workspace(ctx)
     p, n = ws.get(need.value)
     check(ctx.lib.asc_sparse_hip_standard_spmm(ctx.handle, transpose, Scalar(alpha, 0.0),
-                                               C.byref(a.desc), C.byref(b), C.byref(c), p, n))
+                                               C.byref(a.desc), C.byref(b), C.byref(c), p, n,
+                                               C.byref(opt)))
     return ws
 
 
@@ -805,19 +809,6 @@ def coo_finalize(ctx: Context, t: DeviceSparse, duplicate_policy: int, zero_poli
     check(ctx.lib.asc_sparse_hip_coo_finalize(ctx.handle, C.byref(d), duplicate_policy, zero_policy,
                                               bc.ptr, bv.ptr, C.byref(out), p, n))
     return DeviceSparse(out, [bc, bv])
-
-
-_SIGS6 = {"asc_sparse_hip_context_set_spmm_panel_bytes": [_VP, C.c_int64]}
-_SIGS.update(_SIGS6)
-if _lib is not None:
-    for _n, _a in _SIGS6.items():
-        getattr(_lib, _n).argtypes = _a
-        getattr(_lib, _n).restype = Status
-
-
-def set_spmm_panel_bytes(ctx: Context, nbytes: int):
-    """Bytes of B per SpMM column pass (0: automatic, < 0: one pass)."""
-    check(ctx.lib.asc_sparse_hip_context_set_spmm_panel_bytes(ctx.handle, nbytes))
 
 
 LEVEL1_DOT, LEVEL1_AXPY, LEVEL1_GATHER, LEVEL1_GATHER_ZERO, LEVEL1_SCATTER = 0, 1, 2, 3, 4

@@ -1,7 +1,7 @@
 """ctypes binding of the measurement probes (include/asc_sparse_hip_probe.h,
-lib/libasc_sparse_probe.so): the gather-rate and HBM stream ceilings the
-benchmark reports beside its rooflines.  Not part of the Sparse BLAS
-library."""
+lib/libasc_sparse_probe.so): the ceilings the benchmark reports beside its
+rooflines (x-element and B-row gather rates, HBM read and copy).  Not part of
+the Sparse BLAS library."""
 from __future__ import annotations
 
 import ctypes as C
@@ -10,7 +10,8 @@ import os
 from . import PROBE_PATH, Status, check, load
 
 GATHER_PROBE_BLOCKS = 2048
-STREAM_PROBE_BLOCKS = 4096
+ROW_GATHER_PROBE_BLOCKS = 8192
+STREAM_PROBE_BLOCKS = 16384
 
 _lib = None
 
@@ -26,6 +27,9 @@ def _load():
         lib.asc_sparse_hip_gather_probe.argtypes = [vp, vp, vp, C.c_int64, vp, C.c_int64, vp,
                                                     C.c_size_t]
         lib.asc_sparse_hip_gather_probe.restype = Status
+        lib.asc_sparse_hip_row_gather_probe.argtypes = [vp, vp, C.c_int64, vp, C.c_int64, vp,
+                                                        C.c_size_t]
+        lib.asc_sparse_hip_row_gather_probe.restype = Status
         lib.asc_sparse_hip_stream_probe.argtypes = [vp, vp, vp, C.c_int64, vp, C.c_size_t]
         lib.asc_sparse_hip_stream_probe.restype = Status
         _lib = lib
@@ -41,6 +45,13 @@ def gather_probe(ctx, indices, values, count: int, x, x_extent: int, partials):
     check(_load().asc_sparse_hip_gather_probe(ctx.handle, _ptr(indices), _ptr(values), count,
                                               _ptr(x), x_extent, _ptr(partials),
                                               GATHER_PROBE_BLOCKS))
+
+
+def row_gather_probe(ctx, indices, count: int, table, table_rows: int, partials):
+    """Enqueues the 256-byte row gather probe over device buffers."""
+    check(_load().asc_sparse_hip_row_gather_probe(ctx.handle, _ptr(indices), count, _ptr(table),
+                                                  table_rows, _ptr(partials),
+                                                  ROW_GATHER_PROBE_BLOCKS))
 
 
 def stream_read(ctx, src, nbytes: int, partials):

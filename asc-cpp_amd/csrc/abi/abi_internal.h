@@ -31,8 +31,6 @@ struct asc_hip_context {
   // (found at context creation; null for a validation-only context).
   void* noalloc_scratch = nullptr;
   uint8_t determinism = ASC_HIP_DETERMINISTIC;
-  // Bytes of B per SpMM column pass; <= 0: one pass.
-  int64_t spmm_panel_bytes = 0;
   std::mutex plan_mutex;  // guards the tuned-plan cache below
   struct Plan {
     const void* offsets;

@@ -64,7 +64,6 @@ struct SpmvPlan {
   size_t t_offsets = 0, t_indices = 0, t_values = 0, t_ws = 0, spmv_ws = 0;
   size_t total = 0;
   bool noop = false;
-  int64_t spmm_panel_cols = 0;  // SpMM column passes
 };
 
 // The algorithm a call runs: the explicit choice, else the plan cache's entry
@@ -96,15 +95,18 @@ uint8_t resolve_alg(asc_hip_context* ctx, const asc_hip_sparse_desc& a,
   return req == ASC_HIP_ALG_AUTO ? ASC_HIP_ALG_MERGE : req;
 }
 
-// SpMM column passes gather B one slice of spmm_panel_bytes at a time.
-// Measured on the 640M-entry Zipf matrix with a 1 GiB B (k = 32), one pass
-// is fastest (slices of 256/128/64 MiB: +4%/+11%/+30% time), so passes are
-// off unless the context asks for them.
-int64_t spmm_panel_cols(const asc_hip_context* ctx, int64_t inner, int64_t k, size_t esz) {
-  if (k <= 0 || inner <= 0 || ctx->spmm_panel_bytes <= 0) return 0;
-  const int64_t row_bytes = k * static_cast<int64_t>(esz);
-  const int64_t cols = ctx->spmm_panel_bytes / row_bytes;
-  return cols > 0 ? cols : 1;
+// The plain SpMM gathers B one slice of kSpmmPassBytes at a time, one pass
+// over the matrix per slice, so that the slice stays in the 256 MiB
+// memory-side cache while its rows are gathered.  Measured on the 640M-entry
+// Zipf matrix with a 1 GiB B (k = 32, FP64): one pass 27.2 ms, 512/256/128
+// MiB slices 25.1/24.5/24.8 ms.  The ordered kernels run in one pass (slices
+// of 256/128/64 MiB made them 4%/11%/30% slower).
+constexpr int64_t kSpmmPassBytes = int64_t(256) << 20;
+
+void spmm_plain_plan(asc_hip::SpmmArgs* m, size_t esz) {
+  m->panel_cols = 0;
+  if (m->ordered || !asc_hip::spmm_plain_applies(*m) || m->k <= 0 || m->a.inner <= 0) return;
+  m->panel_cols = std::max<int64_t>(1, kSpmmPassBytes / (m->k * static_cast<int64_t>(esz)));
 }
 
 asc_hip_status missing_panel_analysis() {
@@ -975,12 +977,43 @@ asc_hip_status asc_sparse_hip_standard_spmv(asc_hip_context* ctx, uint8_t transp
 
 namespace {
 
+// The SpMM operands of a call, in kernel form (B and C pointers included, so
+// that the plain kernel's alignment conditions are known at sizing time).
+void spmm_args(const SpmvPlan& plan, const asc_hip_matrix_desc& b, const asc_hip_matrix_desc& c,
+               bool ordered, asc_hip::SpmmArgs* m) {
+  *m = asc_hip::SpmmArgs{};
+  m->a = plan.args.a;
+  m->b = b.data;
+  m->ldb = b.leading_dimension;
+  m->b_row_major = b.layout == ASC_HIP_ROW_MAJOR;
+  m->c = const_cast<void*>(c.data);
+  m->ldc = c.leading_dimension;
+  m->c_row_major = c.layout == ASC_HIP_ROW_MAJOR;
+  m->k = b.cols;
+  m->alpha = 0.0;
+  m->ordered = ordered;
+}
+
+asc_hip_status spmm_form(const asc_hip_spmv_options* opt, bool* ordered) {
+  const uint8_t alg = opt ? opt->algorithm : ASC_HIP_ALG_AUTO;
+  if (alg != ASC_HIP_ALG_AUTO && alg != ASC_HIP_ALG_ORDERED)
+    return fail(ASC_HIP_INVALID_ARGUMENT,
+                "SpMM takes ASC_HIP_ALG_AUTO (plain) or ASC_HIP_ALG_ORDERED");
+  if (opt && opt->accumulate != ASC_HIP_ACC_PLAIN)
+    return fail(ASC_HIP_UNSUPPORTED, "SpMM accumulates in the storage precision only");
+  *ordered = alg == ASC_HIP_ALG_ORDERED;
+  return ok();
+}
+
 asc_hip_status prepare_spmm(asc_hip_context* ctx, uint8_t transpose,
                             const asc_hip_sparse_desc* a, const asc_hip_matrix_desc* b,
-                            const asc_hip_matrix_desc* c, SpmvPlan* plan) {
+                            const asc_hip_matrix_desc* c, const asc_hip_spmv_options* opt,
+                            SpmvPlan* plan, asc_hip::SpmmArgs* m) {
   asc_hip_status st = check_context(ctx);
   if (st.code) return st;
   if (transpose > ASC_HIP_OP_H) return fail(ASC_HIP_INVALID_ARGUMENT, "unknown transpose");
+  bool ordered = false;
+  if ((st = spmm_form(opt, &ordered)).code) return st;
   if ((st = check_device_sparse(ctx, a, "matrix")).code) return st;
   if (a->format == ASC_HIP_COO) return fail(ASC_HIP_INVALID_ARGUMENT, "SpMM requires CSR or CSC");
   if (a->element_kind != ASC_HIP_F32 && a->element_kind != ASC_HIP_F64)
@@ -994,12 +1027,9 @@ asc_hip_status prepare_spmm(asc_hip_context* ctx, uint8_t transpose,
   if (overlaps(matrix_span(*b), cs) || sparse_overlaps(*a, cs))
     return fail(ASC_HIP_INVALID_ARGUMENT, "Sparse matrix-matrix input and output overlap");
   if ((st = plan_row_view(*a, transpose, plan)).code) return st;
-  asc_hip::SpmmArgs m;
-  m.a = plan->args.a;
-  m.k = b->cols;
-  m.panel_cols = spmm_panel_cols(ctx, m.a.inner, m.k, elem_size(a->element_kind));
-  plan->spmm_panel_cols = m.panel_cols;
-  plan->spmv_ws = align256(asc_hip::spmm_workspace_bytes(m));
+  spmm_args(*plan, *b, *c, ordered, m);
+  spmm_plain_plan(m, elem_size(a->element_kind));
+  plan->spmv_ws = align256(asc_hip::spmm_workspace_bytes(*m));
   plan->total = transpose_prefix(*plan) + plan->spmv_ws;
   plan->noop = rows == 0 || b->cols == 0;
   return ok();
@@ -1009,10 +1039,12 @@ asc_hip_status prepare_spmm(asc_hip_context* ctx, uint8_t transpose,
 
 asc_hip_status asc_sparse_hip_standard_spmm_workspace_size(
     asc_hip_context* ctx, uint8_t transpose, const asc_hip_sparse_desc* a,
-    const asc_hip_matrix_desc* b, const asc_hip_matrix_desc* c, size_t* out) {
+    const asc_hip_matrix_desc* b, const asc_hip_matrix_desc* c,
+    const asc_hip_spmv_options* opt, size_t* out) {
   if (out == nullptr) return fail(ASC_HIP_INVALID_ARGUMENT, "output pointer is null");
   SpmvPlan plan;
-  asc_hip_status st = prepare_spmm(ctx, transpose, a, b, c, &plan);
+  asc_hip::SpmmArgs m;
+  asc_hip_status st = prepare_spmm(ctx, transpose, a, b, c, opt, &plan, &m);
   if (st.code) return st;
   *out = plan.total;
   return ok();
@@ -1023,15 +1055,16 @@ asc_hip_status asc_sparse_hip_standard_spmm(asc_hip_context* ctx, uint8_t transp
                                             const asc_hip_sparse_desc* a,
                                             const asc_hip_matrix_desc* b,
                                             const asc_hip_matrix_desc* c, void* ws,
-                                            size_t ws_bytes) {
+                                            size_t ws_bytes, const asc_hip_spmv_options* opt) {
   SpmvPlan plan;
-  asc_hip_status st = prepare_spmm(ctx, transpose, a, b, c, &plan);
+  asc_hip::SpmmArgs m;
+  asc_hip_status st = prepare_spmm(ctx, transpose, a, b, c, opt, &plan, &m);
   if (st.code) return st;
-  // The column passes' split table is an optimization: a workspace that only
-  // holds the required part runs the SpMM in one pass.
+  // The passes' split table is an optimization: a workspace that only holds
+  // the required part runs the SpMM in one pass.
   const size_t required = transpose_prefix(plan);
   if (ws_bytes < plan.total || ws == nullptr) {
-    plan.spmm_panel_cols = 0;
+    m.panel_cols = 0;
     plan.total = required;
   }
   if ((st = check_workspace(ctx, ws, ws_bytes, plan.total,
@@ -1044,29 +1077,12 @@ asc_hip_status asc_sparse_hip_standard_spmm(asc_hip_context* ctx, uint8_t transp
     const hipError_t e = run_transpose(&plan, static_cast<char*>(ws), ctx->stream);
     if (e != hipSuccess) return finish_launch(ctx, e, "transpose");
   }
-  asc_hip::SpmmArgs m;
-  m.a = plan.args.a;
-  m.b = b->data;
-  m.ldb = b->leading_dimension;
-  m.b_row_major = b->layout == ASC_HIP_ROW_MAJOR;
-  m.c = const_cast<void*>(c->data);
-  m.ldc = c->leading_dimension;
-  m.c_row_major = c->layout == ASC_HIP_ROW_MAJOR;
-  m.k = b->cols;
+  m.a = plan.args.a;  // the transposed operand when op(A) needed one
   m.alpha = alpha.real;
-  m.panel_cols = plan.spmm_panel_cols;
   return finish_launch(ctx,
                        asc_hip::spmm_launch(m, static_cast<char*>(ws) + transpose_prefix(plan),
                                             ctx->stream),
                        "SpMM launch");
-}
-
-asc_hip_status asc_sparse_hip_context_set_spmm_panel_bytes(asc_hip_context* ctx,
-                                                           int64_t bytes) {
-  asc_hip_status st = check_context(ctx);
-  if (st.code) return st;
-  ctx->spmm_panel_bytes = bytes;
-  return ok();
 }
 
 // ---- conversions ---------------------------------------------------------------

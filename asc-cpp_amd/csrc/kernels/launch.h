@@ -253,11 +253,18 @@ struct SpmmArgs {
   bool c_row_major;
   int64_t k;  // right-hand sides
   double alpha;
+  // true: every C(i,j) receives its terms in stored order without FMA (bit-
+  // exact with the serial reference); false: the plain kernel where it
+  // applies (row-major B and C, k <= 32, k/ldb/ldc multiples of 4, 4-element
+  // aligned B and C), fused multiply-adds in a fixed per-row order.
+  bool ordered = true;
   // Columns of op(A) (rows of B) per pass; 0 or >= inner: one pass.  Each
   // pass gathers only from its slice of B, which stays memory-side cached.
   int64_t panel_cols = 0;
 };
-// Workspace: the passes' split table (0 bytes for one pass).
+// Whether the plain SpMM kernel takes these operands (else the ordered ones).
+bool spmm_plain_applies(const SpmmArgs& args);
+// Workspace: the passes' split table (none for one pass).
 size_t spmm_workspace_bytes(const SpmmArgs& args);
 hipError_t spmm_launch(const SpmmArgs& args, void* workspace, hipStream_t stream);
 

@@ -1,14 +1,15 @@
 // Standardized SpMM  C += alpha*op(A)*B  for gfx950 (blas.h:1692-1750).
 //
-// The ABI hands this kernel a compressed operand whose outer dimension is
-// the output row (op(A) already in row form).  G lanes share one output
-// row, lane l owning output columns l, l+G, ...; every lane walks the row's
-// stored entries in order and updates its C(i,j) with
-//     C(i,j) = C(i,j) + (alpha*v)*B(c,j)
-// unfused, which is the serial reference's per-entry arithmetic
-// (blas.h:1736-1748), so the result is bit-exact with it.  For the
-// row-major k=32 block a G=32 group reads each gathered B row as one
-// contiguous 256-byte segment and writes C rows the same way.
+// The ABI hands these kernels a compressed operand whose outer dimension is
+// the output row (op(A) already in row form).  Two forms:
+//  - ordered: G lanes share one output row, lane l owning output columns
+//    l, l+G, ...; every lane walks the row's stored entries in order and
+//    updates its C(i,j) with
+//        C(i,j) = C(i,j) + (alpha*v)*B(c,j)
+//    unfused, which is the serial reference's per-entry arithmetic
+//    (blas.h:1736-1748), so the result is bit-exact with it;
+//  - plain: per-row register accumulation with fused multiply-adds and one
+//    C update per row (below), within the plain bound of the serial result.
 #include "device_common.h"
 #include "launch.h"
 
@@ -294,6 +295,152 @@ __global__ __launch_bounds__(kBlock) void spmm_wide_kernel(MParams p) {
   }
 }
 
+// ---- plain mode (any association of each C(i,j)'s terms) ------------------
+//
+// Row-major B and C, k <= 32 with k, ldb and ldc multiples of 4 and B and C
+// aligned to 4 elements.  An 8-lane group covers one output row: lane q owns
+// columns 4q..4q+3 and gathers them as one 4-wide vector per stored entry,
+// so a group reads a whole B row (k = 32) per instruction.  Every lane
+// accumulates in registers with fused multiply-adds and adds alpha times its
+// sums to C at the end.
+//
+// A block takes kBlock consecutive rows and sorts them, in LDS, into short
+// rows (fewer than kPlainLongRow entries in the pass) and long rows:
+//   short: one 8-lane group per row, kPlainUnroll B rows in flight per group;
+//   long:  one wave per row, its eight groups taking every eighth entry
+//          (kPlainWideBatches batches of eight B rows in flight) and summing
+//          their partial sums by cross-lane exchange at the end.
+// Which group or wave takes a row depends on LDS atomics, but the arithmetic
+// of a row does not, so results repeat bit for bit from run to run.
+// Rows with no entries in the pass leave C untouched.
+constexpr int kPlainLongRow = 64;
+constexpr int kPlainUnroll = 4;
+constexpr int kPlainWideBatches = 4;
+
+template <typename T>
+using vec4_of = T __attribute__((ext_vector_type(4)));
+
+template <typename T>
+__device__ __forceinline__ void fma4(vec4_of<T>& acc, T v, const vec4_of<T>& b) {
+  acc.x = __builtin_fma(v, b.x, acc.x);
+  acc.y = __builtin_fma(v, b.y, acc.y);
+  acc.z = __builtin_fma(v, b.z, acc.z);
+  acc.w = __builtin_fma(v, b.w, acc.w);
+}
+
+template <typename T>
+__device__ __forceinline__ void plain_store(const MParams& p, int64_t row, int64_t jq,
+                                           vec4_of<T> acc) {
+  typedef vec4_of<T> v4;
+  T* c = static_cast<T*>(p.c) + row * p.ldc + jq;
+  const T alpha = static_cast<T>(p.alpha);
+  v4 old = *reinterpret_cast<const v4*>(c);
+  fma4<T>(old, alpha, acc);
+  *reinterpret_cast<v4*>(c) = old;
+}
+
+template <typename T, typename I>
+__global__ __launch_bounds__(kBlock) void spmm_plain_kernel(MParams p) {
+  typedef vec4_of<T> v4;
+  __shared__ int64_t s_begin[kBlock];
+  __shared__ int64_t s_end[kBlock];
+  __shared__ uint16_t s_short[kBlock];
+  __shared__ uint16_t s_long[kBlock];
+  __shared__ uint32_t s_count[2];
+  const int tid = threadIdx.x;
+  const int64_t row0 = xcd_remap(blockIdx.x, gridDim.x) * kBlock;
+  if (tid < 2) s_count[tid] = 0;
+  __syncthreads();
+  if (row0 + tid < p.rows) {
+    int64_t begin, end;
+    pass_range(p, row0 + tid, &begin, &end);
+    s_begin[tid] = begin;
+    s_end[tid] = end;
+    if (end - begin >= kPlainLongRow)
+      s_long[atomicAdd(&s_count[1], 1u)] = static_cast<uint16_t>(tid);
+    else if (end > begin)
+      s_short[atomicAdd(&s_count[0], 1u)] = static_cast<uint16_t>(tid);
+  }
+  __syncthreads();
+  const int q = tid & 7;
+  const int64_t jq = 4 * q;
+  const bool qvalid = jq < p.k;
+  const T* __restrict__ val = static_cast<const T*>(p.values);
+  const I* __restrict__ col = static_cast<const I*>(p.indices);
+  const T* __restrict__ b = static_cast<const T*>(p.b) + jq;
+  const v4 zero = v4{T(0), T(0), T(0), T(0)};
+
+  // Long rows: wave w takes the block's long rows w, w + 4, ...
+  const uint32_t nlong = s_count[1];
+  const int g = (tid % kWave) >> 3;  // entry slot within the wave
+  for (uint32_t i = tid / kWave; i < nlong; i += kBlock / kWave) {
+    const int r = s_long[i];
+    const int64_t begin = s_begin[r], end = s_end[r];
+    v4 acc = zero;
+    constexpr int U = kPlainWideBatches;
+    for (int64_t e0 = begin; e0 < end; e0 += 8 * U) {
+      int64_t c[U];
+      T v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t e = e0 + 8 * u + g;
+        c[u] = e < end ? static_cast<int64_t>(col[e]) : 0;
+        v[u] = e < end ? val[e] : T(0);
+      }
+      v4 bv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        bv[u] = (e0 + 8 * u + g < end && qvalid) ? *reinterpret_cast<const v4*>(b + c[u] * p.ldb)
+                                                 : zero;
+#pragma unroll
+      for (int u = 0; u < U; ++u) fma4<T>(acc, v[u], bv[u]);
+    }
+    // The eight slots' sums of the same columns, in a fixed tree.
+#pragma unroll
+    for (int m = 8; m < kWave; m <<= 1) {
+      acc.x += __shfl_xor(acc.x, m);
+      acc.y += __shfl_xor(acc.y, m);
+      acc.z += __shfl_xor(acc.z, m);
+      acc.w += __shfl_xor(acc.w, m);
+    }
+    if (g == 0 && qvalid) plain_store<T>(p, row0 + r, jq, acc);
+  }
+
+  // Short rows: group gi takes the block's short rows gi, gi + 32, ...
+  const uint32_t nshort = s_count[0];
+  if (!qvalid) return;
+  for (uint32_t i = tid >> 3; i < nshort; i += kBlock / 8) {
+    const int r = s_short[i];
+    const int64_t begin = s_begin[r], end = s_end[r];
+    v4 acc = zero;
+    constexpr int U = kPlainUnroll;
+    for (int64_t e0 = begin; e0 < end; e0 += U) {
+      int64_t c[U];
+      T v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const bool in = e0 + u < end;
+        c[u] = in ? static_cast<int64_t>(col[e0 + u]) : 0;
+        v[u] = in ? val[e0 + u] : T(0);
+      }
+      v4 bv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        bv[u] = e0 + u < end ? *reinterpret_cast<const v4*>(b + c[u] * p.ldb) : zero;
+#pragma unroll
+      for (int u = 0; u < U; ++u) fma4<T>(acc, v[u], bv[u]);
+    }
+    plain_store<T>(p, row0 + r, jq, acc);
+  }
+}
+
+template <typename T, typename I>
+hipError_t launch_plain(const MParams& p, hipStream_t s) {
+  const unsigned grid = static_cast<unsigned>((p.rows + kBlock - 1) / kBlock);
+  spmm_plain_kernel<T, I><<<grid, kBlock, 0, s>>>(p);
+  return hipGetLastError();
+}
+
 template <typename T, typename I>
 hipError_t launch_typed(const MParams& p, hipStream_t s) {
   const uintptr_t vec_bytes = 4 * sizeof(T);
@@ -331,6 +478,14 @@ hipError_t launch_typed(const MParams& p, hipStream_t s) {
 
 }  // namespace
 
+bool spmm_plain_applies(const SpmmArgs& a) {
+  const size_t esz = a.a.elem == Elem::F64 ? 8 : 4;
+  const uintptr_t vb = 4 * esz;
+  return a.b_row_major && a.c_row_major && a.k <= 32 && a.k % 4 == 0 && a.ldb % 4 == 0 &&
+         a.ldc % 4 == 0 && reinterpret_cast<uintptr_t>(a.b) % vb == 0 &&
+         reinterpret_cast<uintptr_t>(a.c) % vb == 0;
+}
+
 namespace {
 
 // Column passes and the bytes of their split table (0 for one pass).
@@ -346,14 +501,18 @@ int64_t pass_count(const SpmmArgs& a) {
   return w > 0 && w < a.a.inner ? (a.a.inner - 1) / w + 1 : 1;
 }
 
+size_t align_up(size_t n) { return (n + 255) & ~size_t(255); }
+
+size_t split_bytes(const SpmmArgs& a) {
+  const int64_t passes = pass_count(a);
+  return passes <= 1 ? 0
+                     : align_up(sizeof(uint32_t) * static_cast<size_t>(passes - 1) *
+                                static_cast<size_t>(a.a.outer));
+}
+
 }  // namespace
 
-size_t spmm_workspace_bytes(const SpmmArgs& a) {
-  const int64_t passes = pass_count(a);
-  if (passes <= 1) return 0;
-  return (sizeof(uint32_t) * static_cast<size_t>(passes - 1) * static_cast<size_t>(a.a.outer) +
-          255) & ~size_t(255);
-}
+size_t spmm_workspace_bytes(const SpmmArgs& a) { return split_bytes(a); }
 
 hipError_t spmm_launch(const SpmmArgs& a, void* workspace, hipStream_t s) {
   if (a.a.outer == 0 || a.k == 0) return hipSuccess;
@@ -371,8 +530,12 @@ hipError_t spmm_launch(const SpmmArgs& a, void* workspace, hipStream_t s) {
   p.k = a.k;
   p.alpha = a.alpha;
   p.passes = pass_count(a);
+  p.pass = 0;
   p.splits = static_cast<const uint32_t*>(workspace);
   const bool i32 = a.a.index_bytes == 4;
+  const bool f64 = a.a.elem == Elem::F64;
+  if (!f64 && a.a.elem != Elem::F32) return hipErrorInvalidValue;
+  hipError_t e;
   if (p.passes > 1) {
     const unsigned blocks = static_cast<unsigned>((p.rows + kBlock - 1) / kBlock);
     uint32_t* splits = static_cast<uint32_t*>(workspace);
@@ -382,16 +545,17 @@ hipError_t spmm_launch(const SpmmArgs& a, void* workspace, hipStream_t s) {
     else
       spmm_split_kernel<int64_t><<<blocks, kBlock, 0, s>>>(
           p.offsets, static_cast<const int64_t*>(p.indices), p.rows, pass_width(a), p.passes, splits);
-    const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    if ((e = hipGetLastError()) != hipSuccess) return e;
   }
+  const bool plain = !a.ordered && spmm_plain_applies(a);
   for (int64_t q = 0; q < p.passes; ++q) {
     p.pass = q;
-    hipError_t e = hipErrorInvalidValue;
-    if (a.a.elem == Elem::F64)
-      e = i32 ? launch_typed<double, int32_t>(p, s) : launch_typed<double, int64_t>(p, s);
-    else if (a.a.elem == Elem::F32)
-      e = i32 ? launch_typed<float, int32_t>(p, s) : launch_typed<float, int64_t>(p, s);
+    if (plain)
+      e = f64 ? (i32 ? launch_plain<double, int32_t>(p, s) : launch_plain<double, int64_t>(p, s))
+              : (i32 ? launch_plain<float, int32_t>(p, s) : launch_plain<float, int64_t>(p, s));
+    else
+      e = f64 ? (i32 ? launch_typed<double, int32_t>(p, s) : launch_typed<double, int64_t>(p, s))
+              : (i32 ? launch_typed<float, int32_t>(p, s) : launch_typed<float, int64_t>(p, s));
     if (e != hipSuccess) return e;
   }
   return hipSuccess;

@@ -13,6 +13,7 @@
 using namespace asc_hip_abi;
 
 static_assert(ASC_HIP_GATHER_PROBE_BLOCKS == asc_hip::kGatherProbeBlocks, "probe grid");
+static_assert(ASC_HIP_ROW_GATHER_PROBE_BLOCKS == asc_hip::kRowGatherProbeBlocks, "probe grid");
 static_assert(ASC_HIP_STREAM_PROBE_BLOCKS == asc_hip::kStreamProbeBlocks, "probe grid");
 
 extern "C" {
@@ -48,6 +49,42 @@ asc_hip_status asc_sparse_hip_gather_probe(asc_hip_context* ctx, const int32_t* 
                        asc_hip::gather_probe_launch(indices, values, count, x, x_extent, partials,
                                                     ctx->stream),
                        "gather probe");
+}
+
+asc_hip_status asc_sparse_hip_row_gather_probe(asc_hip_context* ctx, const int32_t* indices,
+                                               int64_t count, const double* table,
+                                               int64_t table_rows, double* partials,
+                                               size_t partials_count) {
+  constexpr size_t kRowBytes = 256;
+  asc_hip_status st = check_context(ctx);
+  if (st.code) return st;
+  if (count < 0 || table_rows < 0) return fail(ASC_HIP_SHAPE, "negative probe extent");
+  if (table_rows > static_cast<int64_t>(UINT32_MAX) + 1)
+    return fail(ASC_HIP_UNSUPPORTED, "the probe gathers from at most 2^32 rows");
+  if (count > 0 && table_rows == 0) return fail(ASC_HIP_SHAPE, "gathers need a nonempty table");
+  if (partials == nullptr ||
+      partials_count < static_cast<size_t>(asc_hip::kRowGatherProbeBlocks))
+    return fail(ASC_HIP_INVALID_ARGUMENT,
+                "partials must hold ASC_HIP_ROW_GATHER_PROBE_BLOCKS values");
+  const size_t pb = sizeof(double) * static_cast<size_t>(asc_hip::kRowGatherProbeBlocks);
+  if ((st = check_device_pointer(ctx, partials, "partials", pb)).code) return st;
+  const size_t n = static_cast<size_t>(count);
+  if (count > 0) {
+    if (reinterpret_cast<uintptr_t>(table) % 32 != 0)
+      return fail(ASC_HIP_INVALID_ARGUMENT, "the table must be 32-byte aligned");
+    const size_t tb = kRowBytes * static_cast<size_t>(table_rows);
+    if ((st = check_device_pointer(ctx, indices, "indices", 4 * n)).code) return st;
+    if ((st = check_device_pointer(ctx, table, "table", tb)).code) return st;
+    const Span out = span_of(partials, pb);
+    if (overlaps(out, span_of(indices, 4 * n)) || overlaps(out, span_of(table, tb)))
+      return fail(ASC_HIP_INVALID_ARGUMENT, "partials overlap an input");
+  }
+  if ((st = unavailable_if_validation_only(ctx)).code) return st;
+  DeviceGuard guard(ctx->device);
+  return finish_launch(ctx,
+                       asc_hip::row_gather_probe_launch(indices, count, table, table_rows,
+                                                        partials, ctx->stream),
+                       "row gather probe");
 }
 
 asc_hip_status asc_sparse_hip_stream_probe(asc_hip_context* ctx, const void* src, void* dst,

@@ -22,8 +22,9 @@ Every timed output is checked after its timing (the "guard" object): the
 headline and FP32 SpMVs on every row against the ordered kernel (the
 reference's summation order, itself compared bit for bit with the
 reference's own output on the CPU baseline's rows) within the plain-mode
-bound 4*len_i*eps*(|A||x|)_i; the ordered SpMM on sampled rows recomputed in
-the reference's order; the conversions' structure.  A failed check sets
+bound 4*len_i*eps*(|A||x|)_i; the plain and ordered SpMM on sampled rows
+against a recomputation in the reference's order (within the plain bound and
+bit for bit); the conversions' structure.  A failed check sets
 "error" in the line and the exit status to 3.  The roofline object also
 carries the measured HBM read and copy ceilings (libasc_sparse_probe.so).
 
@@ -203,6 +204,16 @@ def learned_pick(ctx, a):
     return alg.value, [round(float(s), 4) for s in scores]
 
 
+def calibration_copy(ctx):
+    """A plain 1 GiB device copy: HBM counters read it as 1 GiB fetched and
+    1 GiB written, which scales the counters of the kernels profiled beside it."""
+    ca, cb = ctx.alloc(1 << 30), ctx.alloc(1 << 30)
+    ctx.fill(ca, 1)
+    ctx.copy(cb.ptr, ca.ptr, 1 << 30)
+    ctx.synchronize()
+    del ca, cb
+
+
 def bench_large(ctx, args, extra):
     t0 = time.perf_counter()
     m = G.zipf_large(args.rows)
@@ -213,13 +224,7 @@ def bench_large(ctx, args, extra):
     xd, yd = H.vector(xb, m.cols, H.F64, writable=False), H.vector(yb, m.rows, H.F64)
     ws = H.Workspace(ctx)
     if args.calibrate:
-        # A plain copy of known size: HBM counters read it as ~1 GiB fetched
-        # and 1 GiB written, which scales the SpMV kernel's counters.
-        ca, cb = ctx.alloc(1 << 30), ctx.alloc(1 << 30)
-        ctx.fill(ca, 1)
-        ctx.copy(cb.ptr, ca.ptr, 1 << 30)
-        ctx.synchronize()
-        del ca, cb
+        calibration_copy(ctx)
     nbytes = spmv_bytes(m.rows, m.cols, m.nnz, 8, 4, 8, 8)
     lens = np.diff(m.offsets)
     if args.alg in ("auto", "panel"):
@@ -303,69 +308,105 @@ def bench_large(ctx, args, extra):
         guard["fp32"] = {"rows_checked": m.rows, "rows_over_bound": nbad,
                          "max_err_over_bound": round(ratio, 4)}
         del a32, xb32, yb32, y32, y32o
-        # SpMM: C += A*B, B row-major 2^22 x 32 (U[-1,1]).
-        k = 32
-        bmat = ctx.alloc(8 * m.cols * k)
-        bh = G.vector(m.cols * k, 777)
-        ctx.copy(bmat.ptr, bh.ctypes.data, bh.nbytes)
-        cmat = ctx.alloc(8 * m.rows * k)
-        ctx.fill(cmat, 0)
-        ctx.synchronize()
-        bd = H.dense(bmat, m.cols, k, k, H.ROW_MAJOR, H.F64, writable=False)
-        cd = H.dense(cmat, m.rows, k, k, H.ROW_MAJOR, H.F64)
-        sws = H.Workspace(ctx)
-        rs = time_ops(ctx, lambda: H.standard_spmm(ctx, H.OP_N, 1.0, a, bd, cd, sws), 2, 1)
-        bs = m.nnz * 12 + (m.rows + 1) * 8 + m.cols * k * 8 + 2 * m.rows * k * 8
-        extra["spmm_k32_row_major"] = {"gbs": round(bs / rs["median_ms"] / 1e6, 1),
-                                       "ms": round(rs["median_ms"], 3)}
-        log(f"SpMM k=32: {rs['median_ms']:.2f} ms")
-        # Guard: sampled rows (the longest among them) recomputed in the
-        # reference's order, C(r,:) += (1*v)*B(c,:) entry by entry, bit for
-        # bit, after one more call from zero.
-        ctx.fill(cmat, 0)
-        H.standard_spmm(ctx, H.OP_N, 1.0, a, bd, cd, sws)
-        cgot = ctx.download(cmat, np.float64, m.rows * k).reshape(m.rows, k)
-        bhost = bh.reshape(m.cols, k)
-        pick = np.concatenate([np.random.default_rng(5).integers(0, m.rows, 256),
-                               np.argsort(lens)[-8:]])
-        exact = 0
-        for r in pick:
-            acc = np.zeros(k)
-            for p in range(int(m.offsets[r]), int(m.offsets[r + 1])):
-                acc = acc + (1.0 * m.values[p]) * bhost[m.indices[p]]
-            exact += int(np.array_equal(acc.view(np.uint64), cgot[r].view(np.uint64)))
-        guard["spmm_k32"] = {"rows_checked": int(pick.size), "rows_bit_exact": exact}
-        del bmat, cmat, bh, cgot, bhost
-        # Conversions on the same matrix.
-        cws = H.Workspace(ctx)
-        holder = {}
-
-        def to_csc():
-            holder["csc"] = H.convert_compressed(ctx, a, cws)
-        rc = time_ops(ctx, to_csc, 2, 1)
-        bc = 2 * m.nnz * 12 + (m.rows + 1) * 8 + (m.cols + 1) * 8
-        extra["csr_to_csc"] = {"gbs": round(bc / rc["median_ms"] / 1e6, 1), "ms": round(rc["median_ms"], 3)}
-        # Guard: the column counts (the full transpose is checked bit for bit
-        # by the GPU tests at this size).
-        off_c = ctx.download(H._ptr_buf(ctx, holder["csc"].desc.outer_offsets), np.int64, m.cols + 1)
-        guard["csr_to_csc"] = {"column_counts_match": bool(
-            off_c[0] == 0 and np.array_equal(np.diff(off_c), np.bincount(m.indices, minlength=m.cols)))}
-        holder.clear()
-        del cws
-        coo = H.convert_to_coordinate(ctx, a)
-
-        def to_csr():
-            holder["csr"] = H.convert_coordinate(ctx, coo, H.CSR, 4)
-        rr = time_ops(ctx, to_csr, 3, 1)
-        bcoo = m.nnz * (16 + 8) + m.nnz * (4 + 8) + (m.rows + 1) * 8
-        extra["coo_to_csr"] = {"gbs": round(bcoo / rr["median_ms"] / 1e6, 1), "ms": round(rr["median_ms"], 3)}
-        log(f"CSR->CSC {rc['median_ms']:.2f} ms, COO->CSR {rr['median_ms']:.2f} ms")
-        off_r = ctx.download(H._ptr_buf(ctx, holder["csr"].desc.outer_offsets), np.int64, m.rows + 1)
-        guard["coo_to_csr"] = {"offsets_match": bool(np.array_equal(off_r, m.offsets))}
-        holder.clear()
-        del coo
+        bench_spmm(ctx, m, a, lens, extra, guard)
+        bench_conversions(ctx, m, a, extra, guard)
         bench_shuffled_finalize(ctx, m, extra, guard)
     return m, alg, res, nbytes, y_ord, guard, x
+
+
+def bench_spmm(ctx, m, a, lens, extra, guard, steps=3):
+    """SpMM C += A*B on the large matrix, B row-major 2^22 x 32 (U[-1,1]): the
+    plain and ordered forms timed, sampled rows checked, and the B-row gather
+    ceiling measured beside them."""
+    k = 32
+    bmat = ctx.alloc(8 * m.cols * k)
+    bh = G.vector(m.cols * k, 777)
+    ctx.copy(bmat.ptr, bh.ctypes.data, bh.nbytes)
+    cmat = ctx.alloc(8 * m.rows * k)
+    ctx.fill(cmat, 0)
+    ctx.synchronize()
+    bd = H.dense(bmat, m.cols, k, k, H.ROW_MAJOR, H.F64, writable=False)
+    cd = H.dense(cmat, m.rows, k, k, H.ROW_MAJOR, H.F64)
+    sws = H.Workspace(ctx)
+    bs = m.nnz * 12 + (m.rows + 1) * 8 + m.cols * k * 8 + 2 * m.rows * k * 8
+    spmm_ms = {}
+    for form, alg in (("plain", H.ALG_AUTO), ("ordered", H.ALG_ORDERED)):
+        rs = time_ops(ctx, lambda alg=alg: H.standard_spmm(ctx, H.OP_N, 1.0, a, bd, cd, sws,
+                                                           alg=alg), 3, 1)
+        spmm_ms[form] = rs["median_ms"]
+        log(f"SpMM k=32 {form}: {rs['median_ms']:.2f} ms")
+    gathered = m.nnz * k * 8  # every stored entry gathers one 256-byte row of B
+    ceiling = bench_row_gather_ceiling(ctx)
+    extra["spmm_k32_row_major"] = {
+        "gbs": round(bs / spmm_ms["plain"] / 1e6, 1), "ms": round(spmm_ms["plain"], 3),
+        "form": "plain (FMA, 256 MiB column passes)",
+        "ordered_gbs": round(bs / spmm_ms["ordered"] / 1e6, 1),
+        "ordered_ms": round(spmm_ms["ordered"], 3),
+        "b_row_gathers_gb": round(gathered / 1e9, 1),
+        "b_row_gather_gbs": round(gathered / spmm_ms["plain"] / 1e6, 1),
+        "b_row_gather_ceiling": ceiling,
+        "frac_of_gather_ceiling_256MiB": round(
+            gathered / spmm_ms["plain"] / 1e6 / ceiling["table_256MiB_gbs"], 3)}
+    # Guard: sampled rows (the longest among them) recomputed in the
+    # reference's order, C(r,:) += (1*v)*B(c,:) entry by entry, after one
+    # more call of each form from zero: the ordered form bit for bit, the
+    # plain form within 4*(len+1)*eps*(|A||B|)(r,:).
+    bhost = bh.reshape(m.cols, k)
+    pick = np.concatenate([np.random.default_rng(5).integers(0, m.rows, 256),
+                           np.argsort(lens)[-8:]])
+    serial, magnitude = {}, {}
+    for r in pick:
+        acc, mag = np.zeros(k), np.zeros(k)
+        for p in range(int(m.offsets[r]), int(m.offsets[r + 1])):
+            acc = acc + (1.0 * m.values[p]) * bhost[m.indices[p]]
+            mag = mag + abs(m.values[p]) * np.abs(bhost[m.indices[p]])
+        serial[r], magnitude[r] = acc, mag
+    spmm_guard = {"rows_checked": int(pick.size)}
+    for form, alg in (("plain", H.ALG_AUTO), ("ordered", H.ALG_ORDERED)):
+        ctx.fill(cmat, 0)
+        H.standard_spmm(ctx, H.OP_N, 1.0, a, bd, cd, sws, alg=alg)
+        cgot = ctx.download(cmat, np.float64, m.rows * k).reshape(m.rows, k)
+        if form == "ordered":
+            spmm_guard["ordered_rows_bit_exact"] = sum(
+                int(np.array_equal(serial[r].view(np.uint64), cgot[r].view(np.uint64)))
+                for r in pick)
+        else:
+            spmm_guard["plain_rows_within_bound"] = sum(
+                int(np.all(np.abs(cgot[r] - serial[r]) <=
+                           4 * (lens[r] + 1) * 2.0 ** -53 * magnitude[r])) for r in pick)
+    guard["spmm_k32"] = spmm_guard
+    del bmat, cmat, bh, cgot, bhost
+
+
+def bench_conversions(ctx, m, a, extra, guard, steps=2):
+    """CSR->CSC and COO->CSR on the large matrix, structure checked."""
+    cws = H.Workspace(ctx)
+    holder = {}
+
+    def to_csc():
+        holder["csc"] = H.convert_compressed(ctx, a, cws)
+    rc = time_ops(ctx, to_csc, steps, 1)
+    bc = 2 * m.nnz * 12 + (m.rows + 1) * 8 + (m.cols + 1) * 8
+    extra["csr_to_csc"] = {"gbs": round(bc / rc["median_ms"] / 1e6, 1), "ms": round(rc["median_ms"], 3)}
+    # Guard: the column counts (the full transpose is checked bit for bit
+    # by the GPU tests at this size).
+    off_c = ctx.download(H._ptr_buf(ctx, holder["csc"].desc.outer_offsets), np.int64, m.cols + 1)
+    guard["csr_to_csc"] = {"column_counts_match": bool(
+        off_c[0] == 0 and np.array_equal(np.diff(off_c), np.bincount(m.indices, minlength=m.cols)))}
+    holder.clear()
+    del cws
+    coo = H.convert_to_coordinate(ctx, a)
+
+    def to_csr():
+        holder["csr"] = H.convert_coordinate(ctx, coo, H.CSR, 4)
+    rr = time_ops(ctx, to_csr, steps + 1, 1)
+    bcoo = m.nnz * (16 + 8) + m.nnz * (4 + 8) + (m.rows + 1) * 8
+    extra["coo_to_csr"] = {"gbs": round(bcoo / rr["median_ms"] / 1e6, 1), "ms": round(rr["median_ms"], 3)}
+    log(f"CSR->CSC {rc['median_ms']:.2f} ms, COO->CSR {rr['median_ms']:.2f} ms")
+    off_r = ctx.download(H._ptr_buf(ctx, holder["csr"].desc.outer_offsets), np.int64, m.rows + 1)
+    guard["coo_to_csr"] = {"offsets_match": bool(np.array_equal(off_r, m.offsets))}
+    holder.clear()
+    del coo
 
 
 def bench_shuffled_finalize(ctx, m, extra, guard):
@@ -521,8 +562,9 @@ def guard_failures(guard):
     for k, v in guard.items():
         if v.get("rows_over_bound", 0) != 0:
             bad.append(k)
-        if "rows_bit_exact" in v and v["rows_bit_exact"] != v["rows_checked"]:
-            bad.append(k)
+        for count in ("rows_bit_exact", "ordered_rows_bit_exact", "plain_rows_within_bound"):
+            if count in v and v[count] != v["rows_checked"]:
+                bad.append(k)
         if any(val is False for val in v.values()):
             bad.append(k)
     return sorted(set(bad))
@@ -568,6 +610,29 @@ def bench_gather_ceiling(ctx, extra, bytes_per_entry, n=1 << 26):
         del bi, bx
     extra["gather_ceiling"] = res
     log("gather ceiling: " + ", ".join(f"{k} {v['ggathers_per_s']} G/s" for k, v in res.items()))
+
+
+def bench_row_gather_ceiling(ctx, n=1 << 26):
+    """The B-row gather ceiling behind the SpMM roofline: n random 256-byte
+    row gathers (32 FP64, an 8-lane group per row as the plain SpMM kernel
+    gathers) from tables of 2 MiB (L2-resident), 256 MiB (one column pass of
+    the plain SpMM, MALL-sized) and 1 GiB (the whole 2^22 x 32 B), from
+    libasc_sparse_probe.so; rates in gathered bytes per second."""
+    from asc_sparse_hip import probe as P
+    rng = np.random.default_rng(G.SEED_BASE + 14)
+    out = ctx.alloc(8 * P.ROW_GATHER_PROBE_BLOCKS)
+    res = {}
+    for mib in (2, 256, 1024):
+        rows = (mib << 20) // 256
+        bi = ctx.upload(rng.integers(0, rows, n).astype(np.int32))
+        bt = ctx.alloc(mib << 20)
+        ctx.fill(bt, 0)
+        r = time_ops(ctx, lambda: P.row_gather_probe(ctx, bi, n, bt, rows, out), 5, 2)
+        res[f"table_{mib}MiB_gbs"] = round(n * 256 / r["median_ms"] / 1e6, 1)
+        del bi, bt
+    res["source"] = "libasc_sparse_probe.so row gather probe, 2^26 random rows per table"
+    log("B-row gather ceiling: " + ", ".join(f"{k} {v}" for k, v in res.items() if k != "source"))
+    return res
 
 
 def bench_dense_rows(ctx, extra, rows=1 << 14, cols=1 << 22, per_row=20_000):
@@ -670,6 +735,29 @@ def bench_batched(ctx, extra, count):
     extra["batched_10k"] = res
 
 
+def bench_section(ctx, args):
+    """One section on the large matrix with one timed call per form, for
+    counter runs (rocprofv3 --pmc) of its kernels."""
+    t0 = time.perf_counter()
+    m = G.zipf_large(args.rows)
+    log(f"generated n={m.rows} nnz={m.nnz} in {time.perf_counter() - t0:.1f}s")
+    a = H.clone_compressed(ctx, H.CSR, m.rows, m.cols, m.offsets, m.indices, m.values, H.F64, 4)
+    if args.calibrate:
+        calibration_copy(ctx)
+    extra, guard = {}, {}
+    if args.only == "spmm":
+        bench_spmm(ctx, m, a, np.diff(m.offsets), extra, guard, steps=1)
+    else:
+        bench_conversions(ctx, m, a, extra, guard, steps=1)
+    bad = guard_failures(guard)
+    out = {"section": args.only, "extra": extra, "guard": guard}
+    if bad:
+        out["error"] = "output check failed: " + ", ".join(bad)
+    print(json.dumps(out), flush=True)
+    if bad:
+        sys.exit(3)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -688,6 +776,9 @@ def main():
     ap.add_argument("--sweep", action="store_true", help="time every SpMV variant")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run one 1 GiB device copy (counter calibration)")
+    ap.add_argument("--only", choices=("spmm", "convert"),
+                    help="profiling aid: run one section on the large matrix (one timed call "
+                         "per form) and print its own JSON line instead of the benchmark line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
@@ -698,6 +789,9 @@ def main():
         sys.exit(1)
     ctx = H.Context(0)
     extra = {}
+    if args.only:
+        bench_section(ctx, args)
+        return
     m, alg, res, nbytes, y_ord, guard, x = bench_large(ctx, args, extra)
     name = H.ALG_NAMES[alg]
     gbs = nbytes / (res["loop_ms"] * 1e-3) / 1e9

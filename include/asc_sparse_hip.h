@@ -562,28 +562,32 @@ asc_hip_status asc_sparse_hip_triangular_solve(asc_hip_context* context, uint8_t
 
 /*
  * Standardized SpMM  C += alpha*op(A)*B  against a dense block in either
- * layout (blas.h:1692-1750).  Every C(i,j) is accumulated in stored order
- * without FMA, so the result equals the serial reference bit for bit.
- * Replaces StandardSpmmErased (providers/cuda.h:245-248).
+ * layout (blas.h:1692-1750).  Replaces StandardSpmmErased
+ * (providers/cuda.h:245-248); `options` (may be NULL) picks the form:
+ *  - ASC_HIP_ALG_ORDERED: every C(i,j) is accumulated in stored order
+ *    without FMA, so the result equals the serial reference bit for bit;
+ *  - ASC_HIP_ALG_AUTO (NULL): the plain form.  For row-major B and C with
+ *    k <= 32, k and both leading dimensions multiples of 4 and B and C
+ *    aligned to 4 elements, each C(i,j) sums its terms in registers with
+ *    fused multiply-adds and is updated once, within
+ *    4*len_i*eps*|alpha|*(|A||B|)(i,j) of the serial result; B is gathered
+ *    in slices of 256 MiB, one pass over the matrix per slice, when the
+ *    workspace holds the passes' split table ((passes-1)*rows uint32; a
+ *    smaller workspace runs one pass).  Other shapes run the ordered form.
+ *    Results repeat bit for bit from run to run in both forms.
+ * `options->accumulate` must be ASC_HIP_ACC_PLAIN; other algorithms are
+ * ASC_HIP_INVALID_ARGUMENT.
  */
 asc_hip_status asc_sparse_hip_standard_spmm_workspace_size(
     asc_hip_context* context, uint8_t transpose,
     const asc_hip_sparse_desc* matrix, const asc_hip_matrix_desc* b,
-    const asc_hip_matrix_desc* c, size_t* out_bytes);
+    const asc_hip_matrix_desc* c, const asc_hip_spmv_options* options,
+    size_t* out_bytes);
 asc_hip_status asc_sparse_hip_standard_spmm(
     asc_hip_context* context, uint8_t transpose, asc_hip_scalar alpha,
     const asc_hip_sparse_desc* matrix, const asc_hip_matrix_desc* b,
-    const asc_hip_matrix_desc* c, void* workspace, size_t workspace_bytes);
-
-/* SpMM column passes (no reference counterpart): B is gathered one slice of
- * `bytes` at a time (one pass over A per slice, in column order, so every
- * C(i,j) still receives its terms in stored order and stays bit-exact), which
- * keeps the slice in the memory-side cache.  0 (the default) or a negative
- * value runs one pass.  The passes need a split table of (passes-1)*rows uint32
- * in the SpMM workspace (asc_sparse_hip_standard_spmm_workspace_size counts
- * it); a call given less workspace than that runs in one pass. */
-asc_hip_status asc_sparse_hip_context_set_spmm_panel_bytes(asc_hip_context* context,
-                                                           int64_t bytes);
+    const asc_hip_matrix_desc* c, void* workspace, size_t workspace_bytes,
+    const asc_hip_spmv_options* options);
 
 /* ---- conversions --------------------------------------------------------- */
 

@@ -736,38 +736,52 @@ template <typename Element, SparseCompressedFormat Format>
 
 // ---- standardized SpMM C += alpha*op(A)*B (blas.h:1692-1750) ----------------
 
+// kPlain: register accumulation with fused multiply-adds, within the plain
+// bound of the serial result (row-major B and C with k <= 32, k and both
+// leading dimensions multiples of 4, 4-element aligned; other shapes run
+// kOrdered); kOrdered: the serial reference's per-entry updates, bit for bit.
+enum class HipSpmmForm : std::uint8_t {
+  kPlain = ASC_HIP_ALG_AUTO,
+  kOrdered = ASC_HIP_ALG_ORDERED,
+};
+
 template <typename Element, typename Matrix>
   requires internal_sparse_hip::kSupportedElement<Element>
 [[nodiscard]] Result<std::size_t> HipSpmmWorkspaceSize(SparseHipContext& context,
                                                        SparseBlasTranspose transpose,
                                                        const Matrix& matrix,
                                                        SparseBlasMatrixView<const Element> input,
-                                                       SparseBlasMatrixView<Element> output) {
+                                                       SparseBlasMatrixView<Element> output,
+                                                       HipSpmmForm form = HipSpmmForm::kPlain) {
   const asc_hip_sparse_desc a = internal_sparse_hip::Describe(matrix);
   const asc_hip_matrix_desc b = internal_sparse_hip::Describe(input);
   const asc_hip_matrix_desc c = internal_sparse_hip::Describe(output);
+  const asc_hip_spmv_options opt{static_cast<std::uint8_t>(form), ASC_HIP_ACC_PLAIN, {}};
   std::size_t bytes = 0;
   const Status s = internal_sparse_hip::ToStatus(asc_sparse_hip_standard_spmm_workspace_size(
-      context.native(), internal_sparse_hip::Op(transpose), &a, &b, &c, &bytes));
+      context.native(), internal_sparse_hip::Op(transpose), &a, &b, &c, &opt, &bytes));
   if (!s.ok()) return s;
   return bytes;
 }
 
-// CudaSpmm (providers/cuda.h:1648-1658).
+// CudaSpmm (providers/cuda.h:1648-1658).  The workspace is optional: with
+// less than HipSpmmWorkspaceSize reports, the plain form runs in one pass.
 template <typename Element, SparseCompressedFormat Format>
   requires internal_sparse_hip::kSupportedElement<Element>
 [[nodiscard]] Result<CompletionEvent> HipSpmm(
     SparseHipContext& context, SparseBlasTranspose transpose, Element alpha,
     CompressedSparseView<const Element, Format> matrix, SparseBlasMatrixView<const Element> input,
     SparseBlasMatrixView<Element> output,
-    MutableMemoryView workspace = internal_sparse_hip::NoWorkspace()) {
+    MutableMemoryView workspace = internal_sparse_hip::NoWorkspace(),
+    HipSpmmForm form = HipSpmmForm::kPlain) {
   const asc_hip_sparse_desc a = internal_sparse_hip::Describe(matrix);
   const asc_hip_matrix_desc b = internal_sparse_hip::Describe(input);
   const asc_hip_matrix_desc c = internal_sparse_hip::Describe(output);
   const asc_hip_scalar s{static_cast<double>(alpha), 0.0};
+  const asc_hip_spmv_options opt{static_cast<std::uint8_t>(form), ASC_HIP_ACC_PLAIN, {}};
   return internal_sparse_hip::Enqueued(
       context, asc_sparse_hip_standard_spmm(context.native(), internal_sparse_hip::Op(transpose), s,
-                                            &a, &b, &c, workspace.data(), workspace.size()));
+                                            &a, &b, &c, workspace.data(), workspace.size(), &opt));
 }
 
 // ---- conversions (compressed.h:1675-1837) ----------------------------------

@@ -19,7 +19,8 @@ extern "C" {
 
 /* Work-groups of the probe launches: the number of partials they write. */
 #define ASC_HIP_GATHER_PROBE_BLOCKS 2048
-#define ASC_HIP_STREAM_PROBE_BLOCKS 4096
+#define ASC_HIP_ROW_GATHER_PROBE_BLOCKS 8192
+#define ASC_HIP_STREAM_PROBE_BLOCKS 16384
 
 /*
  * Gather rate: for p in [0, count) streams indices[p] (int32, clamped on the
@@ -34,6 +35,21 @@ asc_hip_status asc_sparse_hip_gather_probe(asc_hip_context* context, const int32
                                            const double* values, int64_t count, const double* x,
                                            int64_t x_extent, double* partials,
                                            size_t partials_count);
+
+/*
+ * Row gather rate: for p in [0, count) streams indices[p] (int32, clamped on
+ * the device to [0, table_rows)) and gathers row indices[p] of `table`
+ * (table_rows rows of 32 FP64, 256 bytes each, 32-byte aligned), an 8-lane
+ * group per row as the plain SpMM kernel gathers B; every gathered element
+ * is summed, one FP64 partial per work-group to `partials` (device, at least
+ * ASC_HIP_ROW_GATHER_PROBE_BLOCKS values, overlapping no input).  Timing it
+ * over random indices with tables of a given size gives the B-row gather
+ * rate a random-column SpMM with k = 32 can reach at that cache level.
+ */
+asc_hip_status asc_sparse_hip_row_gather_probe(asc_hip_context* context, const int32_t* indices,
+                                               int64_t count, const double* table,
+                                               int64_t table_rows, double* partials,
+                                               size_t partials_count);
 
 /*
  * HBM stream rate: reads `bytes` (a multiple of 16, 16-byte aligned) of

@@ -394,6 +394,29 @@ void TestStandard(Gpu& g) {
   done = asc::HipSpmm(g.ctx, asc::SparseBlasTranspose::kNone, E(0), cv, *b, *c);
   CHECK(done.ok() && done->Wait().ok());
   CHECK(Download<E>(g, cb, 4) == out);
+  // The ordered form adds the same exact products again: [[4,8],[20,28]].
+  done = asc::HipSpmm(g.ctx, asc::SparseBlasTranspose::kNone, E(1), cv, *b, *c, NoWorkspace(),
+                      asc::HipSpmmForm::kOrdered);
+  CHECK(done.ok() && done->Wait().ok());
+  const auto twice = Download<E>(g, cb, 4);
+  CHECK(twice[0] == 4 && twice[1] == 8 && twice[2] == 20 && twice[3] == 28);
+  // A 4-column row-major block takes the plain kernel: B = [[1..4],[5..8]],
+  // C = A*B = [[2,4,6,8],[16,20,24,28]].
+  auto b4b = Upload<E>(g, {1, 2, 3, 4, 5, 6, 7, 8});
+  auto c4b = Upload<E>(g, {0, 0, 0, 0, 0, 0, 0, 0});
+  const asc::ConstMemoryView b4s(b4b.data(), 8 * sizeof(E), MemorySpace::kDevice);
+  const asc::ConstMemoryView c4s(c4b.data(), 8 * sizeof(E), MemorySpace::kDevice);
+  auto b4 = asc::SparseBlasMatrixView<const E>::Create(static_cast<const E*>(b4b.data()), 2, 4,
+                                                       asc::SparseBlasLayout::kRowMajor, 4, b4s);
+  auto c4 = asc::SparseBlasMatrixView<E>::Create(static_cast<E*>(c4b.data()), 2, 4,
+                                                 asc::SparseBlasLayout::kRowMajor, 4, c4s);
+  CHECK(b4.ok() && c4.ok());
+  auto need4 = asc::HipSpmmWorkspaceSize(g.ctx, asc::SparseBlasTranspose::kNone, cv, *b4, *c4);
+  CHECK(need4.ok());
+  done = asc::HipSpmm(g.ctx, asc::SparseBlasTranspose::kNone, E(1), cv, *b4, *c4);
+  CHECK(done.ok() && done->Wait().ok());
+  const auto out4 = Download<E>(g, c4b, 8);
+  CHECK(out4 == (std::vector<E>{2, 4, 6, 8, 16, 20, 24, 28}));
 }
 
 // compressed_conversion_test.cc:26-34,305-400: CSR <-> CSC <-> COO, bit-exact.

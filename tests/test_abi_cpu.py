@@ -434,5 +434,34 @@ def test_span_byte_counts_that_overflow_are_refused(vctx):
     b = H.MatrixDesc(0x200000, 4, 2, 1 << 62, H.COLUMN_MAJOR, H.F64, H.DEVICE, 0)
     c = H.MatrixDesc(0x300000, 3, 2, 3, H.COLUMN_MAJOR, H.F64, H.DEVICE, 1)
     st = vctx.lib.asc_sparse_hip_standard_spmm(vctx.handle, H.OP_N, H.Scalar(1.0, 0.0), C.byref(a),
-                                               C.byref(b), C.byref(c), None, 0)
+                                               C.byref(b), C.byref(c), None, 0, None)
     assert st.code == H.OVERFLOW
+
+
+def test_spmm_form_options(vctx):
+    """SpMM takes ALG_AUTO (plain) or ALG_ORDERED with plain accumulation;
+    other algorithms and compensated accumulation are refused before any
+    operand is looked at; NULL options are AUTO."""
+    a = dev_csr()
+    b = H.MatrixDesc(0x200000, 4, 2, 2, H.ROW_MAJOR, H.F64, H.DEVICE, 0)
+    c = H.MatrixDesc(0x300000, 3, 2, 2, H.ROW_MAJOR, H.F64, H.DEVICE, 1)
+    need = C.c_size_t()
+
+    def size_code(opt):
+        return vctx.lib.asc_sparse_hip_standard_spmm_workspace_size(
+            vctx.handle, H.OP_N, C.byref(a), C.byref(b), C.byref(c),
+            None if opt is None else C.byref(opt), C.byref(need)).code
+
+    def call_code(opt):
+        return vctx.lib.asc_sparse_hip_standard_spmm(
+            vctx.handle, H.OP_N, H.Scalar(1.0, 0.0), C.byref(a), C.byref(b), C.byref(c), None, 0,
+            None if opt is None else C.byref(opt)).code
+
+    for opt in (None, H.options(H.ALG_AUTO), H.options(H.ALG_ORDERED)):
+        assert size_code(opt) == H.OK
+        assert call_code(opt) == H.UNAVAILABLE  # validation-only context
+    for alg in (H.ALG_MERGE, H.ALG_VECTOR[8], H.ALG_PANEL, 200):
+        assert size_code(H.options(alg)) == H.INVALID_ARGUMENT
+        assert call_code(H.options(alg)) == H.INVALID_ARGUMENT
+    assert size_code(H.options(H.ALG_AUTO, H.ACC_TWOSUM)) == H.UNSUPPORTED
+    assert call_code(H.options(H.ALG_ORDERED, H.ACC_TWOSUM)) == H.UNSUPPORTED

@@ -2,7 +2,8 @@
 values[p] * x[min(indices[p], x_extent - 1)] (partials summed on the host,
 checked to a summation-order bound), including indices past the extent,
 which it clamps.  Stream read: the wrapping sum of every 32-bit word.
-Stream copy: the bytes, unchanged."""
+Row gather: the sum of every element of table row min(indices[p],
+table_rows - 1) over p.  Stream copy: the bytes, unchanged."""
 import numpy as np
 import pytest
 
@@ -40,3 +41,18 @@ def test_stream_probes(ctx, words):
     assert got == int(data.astype(np.uint64).sum()) % 2 ** 32
     P.stream_copy(ctx, src, dst, data.nbytes)
     np.testing.assert_array_equal(ctx.download(dst, np.uint32, data.size), data)
+
+
+@pytest.mark.parametrize("n,rows", [(1, 1), (7, 3), (1000, 17), (300_001, 1 << 12)])
+def test_row_gather_probe_sum(ctx, n, rows):
+    rng = np.random.default_rng(n + rows)
+    idx = rng.integers(0, rows + 3, n).astype(np.int32)  # a few past the end: clamped
+    table = rng.uniform(-1, 1, (rows, 32))
+    bi, bt = ctx.upload(idx), ctx.upload(table)
+    out = ctx.alloc(8 * P.ROW_GATHER_PROBE_BLOCKS)
+    ctx.fill(out, 0)
+    print(f"row gather probe n {n} rows {rows}", flush=True)
+    P.row_gather_probe(ctx, bi, n, bt, rows, out)
+    got = ctx.download(out, np.float64, P.ROW_GATHER_PROBE_BLOCKS).sum()
+    gathered = table[np.minimum(idx, rows - 1)]
+    assert abs(got - gathered.sum()) <= 4 * 32 * n * 2.0 ** -53 * np.abs(gathered).sum() + 1e-300

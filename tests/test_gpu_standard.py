@@ -36,7 +36,7 @@ def run_spmv(ctx, op, alpha, a, x, y0, alg):
     return ctx.download(yb, dt, len(y0))
 
 
-def run_spmm(ctx, op, alpha, a, b, c0, layout):
+def run_spmm(ctx, op, alpha, a, b, c0, layout, alg=H.ALG_ORDERED, ws=None):
     kind = a.desc.element_kind
     dt = np.float64 if kind == H.F64 else np.float32
     nb, k = b.shape
@@ -47,9 +47,29 @@ def run_spmm(ctx, op, alpha, a, b, c0, layout):
         bb, cc, ldb, ldc = b.T.reshape(-1), c0.T.reshape(-1), nb, nc
     bbuf, cbuf = ctx.upload(bb.astype(dt)), ctx.upload(cc.astype(dt))
     H.standard_spmm(ctx, op, alpha, a, H.dense(bbuf, nb, k, ldb, layout, kind, writable=False),
-                    H.dense(cbuf, nc, k, ldc, layout, kind))
+                    H.dense(cbuf, nc, k, ldc, layout, kind), ws=ws, alg=alg)
     out = ctx.download(cbuf, dt, nc * k)
     return (out.reshape(nc, k) if layout == H.ROW_MAJOR else out.reshape(k, nc).T), (bb, cc, ldb, ldc)
+
+
+def op_rows(op, rows, cols, off, idx, val):
+    """op(A) of a CSR matrix as (outer, offsets, indices, values) rows."""
+    if op == H.OP_N:
+        return rows, off, idx, val
+    t_off, t_idx, t_val = O.transpose(0, rows, cols, off, idx, val)
+    return cols, t_off, t_idx, t_val
+
+
+def plain_spmm_bound(op, rows, cols, off, idx, val, alpha, b, c0, eps):
+    """Plain-form bound on |C - C_serial|: 4*(len_i+1)*eps*(|alpha|(|A||B|)_ij + |C0_ij|),
+    with |A||B| summed by a sparse product of absolute values (a magnitude, so
+    its own rounding is immaterial)."""
+    import scipy.sparse as sp
+    n, o, i, v = op_rows(op, rows, cols, off, np.asarray(idx, np.int64), val)
+    abs_a = sp.csr_matrix((np.abs(v.astype(np.float64)), i, o), shape=(n, b.shape[0]))
+    lens = np.diff(np.asarray(o, np.int64))
+    mag = abs(alpha) * (abs_a @ np.abs(b.astype(np.float64))) + np.abs(c0.astype(np.float64))
+    return 4 * (lens[:, None] + 1) * eps * mag
 
 
 def test_standard_golden(ctx):
@@ -113,8 +133,12 @@ def test_standard_spmv_random(ctx, fmt, op, dtype):
 @pytest.mark.parametrize("fmt", [H.CSR, H.CSC])
 @pytest.mark.parametrize("op", [H.OP_N, H.OP_T])
 @pytest.mark.parametrize("layout", [H.ROW_MAJOR, H.COLUMN_MAJOR])
-@pytest.mark.parametrize("k", [1, 5, 32, 70])
-def test_standard_spmm_bit_exact(ctx, fmt, op, layout, k):
+@pytest.mark.parametrize("k", [1, 4, 5, 32, 70])
+@pytest.mark.parametrize("alg", [H.ALG_ORDERED, H.ALG_AUTO])
+def test_standard_spmm(ctx, fmt, op, layout, k, alg):
+    """Ordered form: bit-exact with the oracle.  Plain form: within the plain
+    bound where its kernel applies (row-major, k in {4, 32}); every other
+    shape runs the ordered kernels, so AUTO is bit-exact there too."""
     m = G.make(G.ZIPF, 3000, 2500, seed=41, values=G.VALUES_NORMAL, zipf_max=800)
     off, idx, val = m.offsets, m.indices.astype(np.int64), m.values
     rows, cols = m.rows, m.cols
@@ -125,76 +149,99 @@ def test_standard_spmm_bit_exact(ctx, fmt, op, layout, k):
     b = rng.uniform(-1, 1, (nin, k))
     c0 = rng.uniform(-1, 1, (nout, k))
     a = device_matrix(ctx, fmt, rows, cols, off, idx, val, 4)
-    print(f"spmm fmt {fmt} op {op} layout {layout} k {k}", flush=True)
-    got, (bb, cc, ldb, ldc) = run_spmm(ctx, op, 1.5, a, b, c0, layout)
+    print(f"spmm fmt {fmt} op {op} layout {layout} k {k} alg {alg}", flush=True)
+    got, (bb, cc, ldb, ldc) = run_spmm(ctx, op, 1.5, a, b, c0, layout, alg)
     ref = O.std_spmm(0 if fmt == H.CSR else 1, op, rows, cols, off, idx, val, 1.5, bb.copy(), layout,
                      ldb, k, cc.copy(), layout, ldc)
     ref = ref.reshape(nout, k) if layout == H.ROW_MAJOR else ref.reshape(k, nout).T
-    np.testing.assert_array_equal(got.view(np.uint64), np.ascontiguousarray(ref).view(np.uint64))
+    plain = alg == H.ALG_AUTO and layout == H.ROW_MAJOR and k % 4 == 0 and k <= 32
+    if not plain:
+        np.testing.assert_array_equal(got.view(np.uint64), np.ascontiguousarray(ref).view(np.uint64))
+        return
+    # op(A) in CSR row form, for the bound
+    if fmt == H.CSR:
+        bound = plain_spmm_bound(op, rows, cols, off, idx, val, 1.5, b, c0, EPS[np.float64])
+    else:
+        t_off, t_idx, t_val = O.transpose(0, cols, rows, off, idx, val)  # CSC -> CSR of A
+        bound = plain_spmm_bound(op, rows, cols, t_off, t_idx, t_val, 1.5, b, c0, EPS[np.float64])
+    err = np.abs(got - ref)
+    assert np.all(err <= bound), float(np.max(err / np.maximum(bound, 1e-300)))
+    assert np.any(err > 0) or k == 4  # the plain kernel ran (FMA rounding differs somewhere)
 
 
-def test_spmm_k32_row_major_large(ctx):
-    # The benchmark's SpMM shape (row-major 32-column block) on a Zipf matrix.
+@pytest.mark.parametrize("alg", [H.ALG_ORDERED, H.ALG_AUTO])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_spmm_k32_row_major_large(ctx, alg, dtype):
+    # The benchmark's SpMM shape (row-major 32-column block) on a Zipf matrix
+    # with rows from 1 to 10^4 entries (both plain row classes).
     m = G.make(G.ZIPF, 1 << 15, 1 << 15, seed=8, values=G.VALUES_NORMAL)
     k = 32
     rng = np.random.default_rng(1)
-    b = rng.uniform(-1, 1, (m.cols, k))
-    c0 = np.zeros((m.rows, k))
-    a = device_matrix(ctx, H.CSR, m.rows, m.cols, m.offsets, m.indices, m.values, 4)
-    got, (bb, cc, ldb, ldc) = run_spmm(ctx, H.OP_N, 1.0, a, b, c0, H.ROW_MAJOR)
-    ref = O.std_spmm(0, 0, m.rows, m.cols, m.offsets, m.indices, m.values, 1.0, bb.copy(), 1, ldb, k,
+    b = rng.uniform(-1, 1, (m.cols, k)).astype(dtype)
+    c0 = rng.uniform(-1, 1, (m.rows, k)).astype(dtype)
+    val = m.values.astype(dtype)
+    a = device_matrix(ctx, H.CSR, m.rows, m.cols, m.offsets, m.indices, val, 4)
+    got, (bb, cc, ldb, ldc) = run_spmm(ctx, H.OP_N, -0.5, a, b, c0, H.ROW_MAJOR, alg)
+    ref = O.std_spmm(0, 0, m.rows, m.cols, m.offsets, m.indices, val, -0.5, bb.copy(), 1, ldb, k,
                      cc.copy(), 1, ldc).reshape(m.rows, k)
-    np.testing.assert_array_equal(got.view(np.uint64), ref.view(np.uint64))
+    if alg == H.ALG_ORDERED:
+        np.testing.assert_array_equal(got.view(np.uint8), ref.view(np.uint8))
+    else:
+        bound = plain_spmm_bound(H.OP_N, m.rows, m.cols, m.offsets, m.indices, val, -0.5, b, c0,
+                                 EPS[dtype])
+        err = np.abs(got.astype(np.float64) - ref.astype(np.float64))
+        assert np.all(err <= bound), float(np.max(err / np.maximum(bound, 1e-300)))
 
 
-@pytest.mark.parametrize("panel_bytes", [-1, 8 * 32 * 7, 8 * 32 * 1000, 1])
+# 256 MiB slices of B: FP64, 32 columns -> 2^20 rows of B per pass.
+PASS_ROWS = (256 << 20) // (8 * 32)
+
+
 @pytest.mark.parametrize("op", [H.OP_N, H.OP_T])
-def test_spmm_column_passes_bit_exact(ctx, panel_bytes, op):
-    """SpMM in column passes (slices of B) keeps every C(i,j)'s terms in
-    stored order: bit-exact with the oracle for any slice width, including
-    one column per pass and the transposed (workspace) form."""
-    m = G.make(G.ZIPF, 4000, 3000, seed=43, values=G.VALUES_NORMAL, zipf_max=900)
-    k = 32
-    nin, nout = (m.cols, m.rows) if op == H.OP_N else (m.rows, m.cols)
-    rng = np.random.default_rng(3)
+def test_spmm_plain_column_passes(ctx, op):
+    """B larger than one 256 MiB slice (2.5 * 2^20 rows, three passes) with the
+    passes' split table in the workspace, and the same call without a
+    workspace (one pass): both within the plain bound of the serial result,
+    for A and for its transpose."""
+    k, short, long_ = 32, 1 << 14, int(2.5 * PASS_ROWS)
+    rng = np.random.default_rng(11)
+    if op == H.OP_N:   # 2^14 x 2.6M, ~48 entries a row spread over every pass
+        rows, cols = short, long_
+        lens = rng.integers(0, 97, rows)
+        lens[:64] = 3000  # long rows too
+    else:              # 2.6M x 2^14, transposed into 2^14 rows of op(A)
+        rows, cols = long_, short
+        lens = rng.integers(0, 2, rows)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    idx = np.concatenate([np.sort(rng.choice(cols, n, replace=False)) for n in lens]).astype(np.int32)
+    val = rng.standard_normal(len(idx))
+    nin, nout = (cols, rows) if op == H.OP_N else (rows, cols)
     b = rng.uniform(-1, 1, (nin, k))
     c0 = rng.uniform(-1, 1, (nout, k))
-    a = device_matrix(ctx, H.CSR, m.rows, m.cols, m.offsets, m.indices, m.values, 4)
-    print(f"spmm passes panel_bytes {panel_bytes} op {op}", flush=True)
-    H.set_spmm_panel_bytes(ctx, panel_bytes)
-    try:
-        got, (bb, cc, ldb, ldc) = run_spmm(ctx, op, -0.75, a, b, c0, H.ROW_MAJOR)
-    finally:
-        H.set_spmm_panel_bytes(ctx, 0)
-    ref = O.std_spmm(0, op, m.rows, m.cols, m.offsets, m.indices.astype(np.int64), m.values, -0.75,
-                     bb.copy(), 1, ldb, k, cc.copy(), 1, ldc).reshape(nout, k)
-    np.testing.assert_array_equal(got.view(np.uint64), ref.view(np.uint64))
-
-
-def test_spmm_column_passes_without_workspace_run_one_pass(ctx):
-    """Column passes need a split table in the workspace; a call given none
-    runs in one pass, with the same (bit-exact) result."""
-    m = G.make(G.ZIPF, 2000, 1500, seed=44, values=G.VALUES_NORMAL, zipf_max=600)
-    k = 32
-    rng = np.random.default_rng(4)
-    b = rng.uniform(-1, 1, (m.cols, k))
-    c0 = rng.uniform(-1, 1, (m.rows, k))
-    a = device_matrix(ctx, H.CSR, m.rows, m.cols, m.offsets, m.indices, m.values, 4)
-    bb, cb = ctx.upload(b), ctx.upload(c0)
-    bd = H.dense(bb, m.cols, k, k, H.ROW_MAJOR, H.F64, writable=False)
-    cd = H.dense(cb, m.rows, k, k, H.ROW_MAJOR, H.F64)
-    H.set_spmm_panel_bytes(ctx, 8 * k * 100)
-    try:
-        need = C.c_size_t()
-        H.check(ctx.lib.asc_sparse_hip_standard_spmm_workspace_size(
-            ctx.handle, H.OP_N, C.byref(a.desc), C.byref(bd), C.byref(cd), C.byref(need)))
-        assert need.value > 0  # the passes' split table
-        H.check(ctx.lib.asc_sparse_hip_standard_spmm(ctx.handle, H.OP_N, H.Scalar(1.25, 0.0),
+    a = device_matrix(ctx, H.CSR, rows, cols, off, idx, val, 4)
+    ref = O.std_spmm(0, op, rows, cols, off, idx.astype(np.int64), val, 0.75, b.reshape(-1).copy(), 1,
+                     k, k, c0.reshape(-1).copy(), 1, k).reshape(nout, k)
+    bound = plain_spmm_bound(op, rows, cols, off, idx, val, 0.75, b, c0, EPS[np.float64])
+    bb, cb = ctx.upload(b.reshape(-1)), ctx.upload(c0.reshape(-1))
+    bd = H.dense(bb, nin, k, k, H.ROW_MAJOR, H.F64, writable=False)
+    cd = H.dense(cb, nout, k, k, H.ROW_MAJOR, H.F64)
+    need = C.c_size_t()
+    H.check(ctx.lib.asc_sparse_hip_standard_spmm_workspace_size(
+        ctx.handle, op, C.byref(a.desc), C.byref(bd), C.byref(cd), None, C.byref(need)))
+    prefix = C.c_size_t()  # the transposed operand's part (op T), passes excluded
+    H.check(ctx.lib.asc_sparse_hip_standard_spmm_workspace_size(
+        ctx.handle, op, C.byref(a.desc), C.byref(bd), C.byref(cd),
+        C.byref(H.options(H.ALG_ORDERED)), C.byref(prefix)))
+    assert need.value >= prefix.value + 2 * nout * 4  # two split rows of uint32
+    for label, give in (("passes", need.value), ("one pass", prefix.value)):
+        print(f"spmm plain passes op {op} {label}", flush=True)
+        H.check(ctx.lib.asc_sparse_hip_copy(ctx.handle, cb.ptr, c0.ctypes.data, c0.nbytes))
+        ctx.synchronize()
+        ws = H.Workspace(ctx)
+        p, n = ws.get(give)
+        H.check(ctx.lib.asc_sparse_hip_standard_spmm(ctx.handle, op, H.Scalar(0.75, 0.0),
                                                      C.byref(a.desc), C.byref(bd), C.byref(cd),
-                                                     None, 0))
-    finally:
-        H.set_spmm_panel_bytes(ctx, 0)
-    got = ctx.download(cb, np.float64, m.rows * k).reshape(m.rows, k)
-    ref = O.std_spmm(0, 0, m.rows, m.cols, m.offsets, m.indices.astype(np.int64), m.values, 1.25,
-                     b.copy(), 1, k, k, c0.copy(), 1, k).reshape(m.rows, k)
-    np.testing.assert_array_equal(got.view(np.uint64), ref.view(np.uint64))
+                                                     p, n, None))
+        got = ctx.download(cb, np.float64, nout * k).reshape(nout, k)
+        err = np.abs(got - ref)
+        assert np.all(err <= bound), (label, float(np.max(err / np.maximum(bound, 1e-300))))

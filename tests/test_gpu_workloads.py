@@ -118,25 +118,61 @@ def test_cfg2_spmv_fp32_every_row(ctx, cfg2):
     H.plan_clear(ctx)
 
 
-def test_cfg2_spmm_k32_ordered_every_entry(ctx, cfg2):
-    m, a, _ = cfg2
+@pytest.fixture(scope="module")
+def cfg2_spmm(cfg2):
+    """B (2^22 x 32, row-major), C0 and the serial result C0 + 0.5*A*B of the
+    oracle over row ranges."""
+    m, _, _ = cfg2
     k = 32
     b = G.vector(m.cols * k, 777).reshape(m.cols, k)
     c0 = G.vector(m.rows * k, 778).reshape(m.rows, k)
-    bb, cb = ctx.upload(b), ctx.upload(c0)
-    say("SpMM k=32 row-major, ordered: device")
-    H.standard_spmm(ctx, H.OP_N, 0.5, a, H.dense(bb, m.cols, k, k, H.ROW_MAJOR, H.F64, writable=False),
-                    H.dense(cb, m.rows, k, k, H.ROW_MAJOR, H.F64))
-    got = ctx.download(cb, np.float64, m.rows * k).reshape(m.rows, k)
-    del bb, cb
-    say("SpMM k=32 row-major, ordered: oracle over row ranges")
-    ref = c0
+    say("SpMM k=32 row-major: oracle over row ranges")
+    ref = c0.copy()
 
     def part(r0, r1):
         O.std_spmm(0, 0, r1 - r0, m.cols, m.offsets[r0:r1 + 1], m.indices, m.values, 0.5, b, 1, k, k,
                    ref[r0:r1], 1, k)
     O.parallel_rows(m.offsets, part)
+    return b, c0, ref
+
+
+def _device_spmm(ctx, m, a, b, c0, alg):
+    k = b.shape[1]
+    bb, cb = ctx.upload(b), ctx.upload(c0)
+    H.standard_spmm(ctx, H.OP_N, 0.5, a, H.dense(bb, m.cols, k, k, H.ROW_MAJOR, H.F64, writable=False),
+                    H.dense(cb, m.rows, k, k, H.ROW_MAJOR, H.F64), alg=alg)
+    got = ctx.download(cb, np.float64, m.rows * k).reshape(m.rows, k)
+    bb.free()
+    cb.free()
+    return got
+
+
+def test_cfg2_spmm_k32_ordered_every_entry(ctx, cfg2, cfg2_spmm):
+    m, a, _ = cfg2
+    b, c0, ref = cfg2_spmm
+    say("SpMM k=32 row-major, ordered: device")
+    got = _device_spmm(ctx, m, a, b, c0, H.ALG_ORDERED)
     np.testing.assert_array_equal(got.view(np.uint64), ref.view(np.uint64))
+
+
+def test_cfg2_spmm_k32_plain_every_entry(ctx, cfg2, cfg2_spmm):
+    """The plain form on the full matrix (B is 1 GiB: four 256 MiB column
+    passes), every C(i,j) within 4*(len_i+1)*eps*(0.5*(|A||B|)_ij + |C0_ij|)
+    of the serial result, with (|A||B|)_ij bounded above by |A|*m, m_c the
+    largest |B(c,j)| of row c of B."""
+    m, a, _ = cfg2
+    b, c0, ref = cfg2_spmm
+    say("SpMM k=32 row-major, plain: device")
+    got = _device_spmm(ctx, m, a, b, c0, H.ALG_AUTO)
+    say("SpMM k=32 row-major, plain: bound")
+    rowmax = np.abs(b).max(axis=1)
+    mag = O.abs_spmv_rows(m.offsets, m.indices, m.values, rowmax)
+    lens = np.diff(m.offsets).astype(np.float64)
+    bound = 4 * (lens[:, None] + 1) * EPS64 * (0.5 * mag[:, None] + np.abs(c0))
+    err = np.abs(got - ref)
+    worst = float(np.max(err / np.maximum(bound, 1e-300)))
+    print(f"SpMM plain: max error / bound {worst:.3e}", flush=True)
+    assert np.all(err <= bound), worst
 
 
 def _row_of_entry(m):

@@ -33,3 +33,20 @@ def test_stream_probe_validation(vctx):
     assert probe(src, src + 1024, 1 << 16, None, 0) == H.INVALID_ARGUMENT  # overlap
     assert probe(src, None, 1 << 16, src + 16, P.STREAM_PROBE_BLOCKS) == H.INVALID_ARGUMENT
     assert probe(None, None, 0, None, 0) == H.UNAVAILABLE  # empty is fine
+
+
+def test_row_gather_probe_validation(vctx):
+    lib = P._load()
+    probe = lambda idx, n, t, rows, out, m: lib.asc_sparse_hip_row_gather_probe(
+        vctx.handle, idx, n, t, rows, out, m).code
+    i, t, out = 0x100000, 0x200000, 0x900000
+    B = P.ROW_GATHER_PROBE_BLOCKS
+    assert probe(i, 10, t, 100, out, B) == H.UNAVAILABLE  # valid
+    assert probe(i, -1, t, 100, out, B) == H.SHAPE
+    assert probe(i, 10, t, 0, out, B) == H.SHAPE
+    assert probe(i, 10, t, 2 ** 33, out, B) == H.UNSUPPORTED
+    assert probe(i, 10, t, 100, out, B - 1) == H.INVALID_ARGUMENT
+    assert probe(i, 10, t, 100, None, B) == H.INVALID_ARGUMENT
+    assert probe(i, 10, t + 8, 100, out, B) == H.INVALID_ARGUMENT  # table not 32-byte aligned
+    assert probe(i, 10, t, 100, t + 256, B) == H.INVALID_ARGUMENT  # partials overlap the table
+    assert probe(None, 0, None, 0, out, B) == H.UNAVAILABLE  # empty is fine
