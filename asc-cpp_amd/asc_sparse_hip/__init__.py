@@ -617,6 +617,15 @@ class Selector(C.Structure):
                 ("clip", C.c_float), ("reserved", C.c_int32)]
 
 
+class Selection(C.Structure):
+    _fields_ = [("algorithm", C.c_uint8), ("staging", C.c_uint8), ("direct_algorithm", C.c_uint8),
+                ("fell_back", C.c_uint8), ("reserved", C.c_uint8 * 4)]
+
+
+SELECT_AMORTIZE = 100  # SpMVs over which a plan's analysis time is charged
+SELECT_DIRECT_OUTPUTS, SELECT_PLAN_OUTPUTS = 8, 11
+
+
 _SIGS2 = {
     "asc_sparse_hip_plan_set": [_VP, P(SparseDesc), C.c_uint8],
     "asc_sparse_hip_plan_get": [_VP, P(SparseDesc), P(C.c_uint8)],
@@ -632,7 +641,7 @@ _SIGS2 = {
                                 C.c_float, C.c_int64],
     "asc_sparse_hip_normalize_features": [_VP, _VP, _VP, _VP, C.c_int64, _VP],
     "asc_sparse_hip_select_workspace_size": [_VP, P(Selector), P(C.c_size_t)],
-    "asc_sparse_hip_select": [_VP, P(SparseDesc), P(Selector), _VP, C.c_size_t, P(C.c_uint8), P(C.c_float)],
+    "asc_sparse_hip_select": [_VP, P(SparseDesc), P(Selector), _VP, C.c_size_t, P(Selection), P(C.c_float)],
 }
 _SIGS.update(_SIGS2)
 if _lib is not None:  # signatures added after an early load()

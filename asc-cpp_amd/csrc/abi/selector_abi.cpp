@@ -253,20 +253,21 @@ asc_hip_status asc_sparse_hip_select_workspace_size(asc_hip_context* ctx,
   asc_hip_status st = check_context(ctx);
   if (st.code) return st;
   if (sel == nullptr || out == nullptr) return fail(ASC_HIP_INVALID_ARGUMENT, "null argument");
-  if (sel->outputs < 1 || sel->outputs > ASC_HIP_ALG_VECTOR64 - ASC_HIP_ALG_MERGE + 1)
-    return fail(ASC_HIP_INVALID_ARGUMENT, "selector outputs must cover merge..vector64");
+  if (sel->outputs != ASC_HIP_SELECT_DIRECT_OUTPUTS && sel->outputs != ASC_HIP_SELECT_PLAN_OUTPUTS)
+    return fail(ASC_HIP_INVALID_ARGUMENT,
+                "selector outputs are the 8 direct kernels, or those and the 3 panel plans");
   *out = select_ws(sel->outputs);
   return ok();
 }
 
 asc_hip_status asc_sparse_hip_select(asc_hip_context* ctx, const asc_hip_sparse_desc* a,
                                      const asc_hip_selector* sel, void* ws, size_t ws_bytes,
-                                     uint8_t* out_alg, float* out_scores) {
+                                     asc_hip_selection* out, float* out_scores) {
   size_t need = 0;
   asc_hip_status st = asc_sparse_hip_select_workspace_size(ctx, sel, &need);
   if (st.code) return st;
   if ((st = check_csr_for_features(ctx, a)).code) return st;
-  if (out_alg == nullptr) return fail(ASC_HIP_INVALID_ARGUMENT, "output pointer is null");
+  if (out == nullptr) return fail(ASC_HIP_INVALID_ARGUMENT, "output pointer is null");
   if ((st = check_device_pointer(ctx, sel->device_model, "selector model")).code) return st;
   if ((st = check_workspace(ctx, ws, ws_bytes, need, operand_spans_for_select(*a))).code) return st;
   if ((st = unavailable_if_validation_only(ctx)).code) return st;
@@ -293,19 +294,32 @@ asc_hip_status asc_sparse_hip_select(asc_hip_context* ctx, const asc_hip_sparse_
     e = hipMemcpyAsync(host + 32, scores, sizeof(float) * k, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) return finish_launch(ctx, e, "selector inference");
-  // Predicted log-time regrets: pick the smallest, unless unsure.
+  // Predicted log-cost regrets: pick the smallest, unless unsure.
   bool in_range = true;
   for (int f = 0; f < 16; ++f)
     if (!(std::fabs(host[f]) <= sel->clip)) in_range = false;
-  int bestk = 0;
-  for (int j = 1; j < k; ++j)
+  int bestk = 0, best_direct = 0;
+  for (int j = 1; j < k; ++j) {
     if (host[32 + j] < host[32 + bestk]) bestk = j;
-  uint8_t alg = static_cast<uint8_t>(ASC_HIP_ALG_MERGE + bestk);
-  if (!in_range || host[32 + 0] - host[32 + bestk] < sel->margin) alg = ASC_HIP_ALG_MERGE;
+    if (j < ASC_HIP_SELECT_DIRECT_OUTPUTS && host[32 + j] < host[32 + best_direct]) best_direct = j;
+  }
+  const bool unsure = !in_range || host[32 + 0] - host[32 + bestk] < sel->margin;
+  asc_hip_selection pick{};
+  pick.fell_back = unsure ? 1 : 0;
+  pick.direct_algorithm =
+      static_cast<uint8_t>(unsure ? ASC_HIP_ALG_MERGE : ASC_HIP_ALG_MERGE + best_direct);
+  if (!unsure && bestk >= ASC_HIP_SELECT_DIRECT_OUTPUTS) {
+    // Outputs 8, 9, 10: the L2-, LDS- and stream-staged column-panel plans.
+    pick.algorithm = ASC_HIP_ALG_PANEL;
+    pick.staging = static_cast<uint8_t>(ASC_HIP_STAGING_L2 + (bestk - ASC_HIP_SELECT_DIRECT_OUTPUTS));
+  } else {
+    pick.algorithm = pick.direct_algorithm;
+    pick.staging = 0;
+  }
   if (out_scores)
     for (int j = 0; j < k; ++j) out_scores[j] = host[32 + j];
-  *out_alg = alg;
-  return asc_sparse_hip_plan_set(ctx, a, alg);
+  *out = pick;
+  return asc_sparse_hip_plan_set(ctx, a, pick.direct_algorithm);
 }
 
 }  // extern "C"

@@ -6,8 +6,9 @@ columns, row lengths Zipf(s=1.5) clipped to [1, 10^4] (about 6.4e8 nnz),
 sorted distinct uniform int32 columns, N(0,1) values, FP64, alpha=1, beta=0.
 One step = one compat SpMV y = alpha*A*x + beta*y over the whole matrix
 through the column-panel plan with stream staging (its analysis runs once,
-before timing; the L2-staged plan and the learned selector's direct-kernel
-pick are reported beside it), inputs resident in HBM.  `value` is
+before timing; the L2-staged plan, the learned selector's choice among the
+direct kernels and the plans, and its best direct kernel are reported beside
+it), inputs resident in HBM.  `value` is
 algorithmic bytes per SpMV / time:
 
   B = nnz*(s_v + s_i) + (rows+1)*s_o + cols*s_x + rows*s_y*(1 + [beta != 0])
@@ -189,19 +190,14 @@ def bound_violations(y, yref, lens, absax, eps):
 
 
 def learned_pick(ctx, a):
-    """Runs the committed learned selector on the device matrix."""
+    """Runs the committed learned selector on the device matrix: returns the
+    Selection (a direct kernel, or a column-panel plan and its staging) and
+    the network's per-output scores."""
     if not os.path.exists(S.DEFAULT_MODEL):
-        return H.ALG_MERGE, None
+        return None, None
     model = S.load_model()
-    sel, keep = S.device_selector(ctx, model)
-    need = C.c_size_t()
-    H.check(ctx.lib.asc_sparse_hip_select_workspace_size(ctx.handle, C.byref(sel), C.byref(need)))
-    ws = ctx.alloc(need.value)
-    alg = C.c_uint8()
-    scores = (C.c_float * model["k"])()
-    H.check(ctx.lib.asc_sparse_hip_select(ctx.handle, C.byref(a.desc), C.byref(sel), ws.ptr,
-                                          ws.nbytes, C.byref(alg), scores))
-    return alg.value, [round(float(s), 4) for s in scores]
+    pick, scores = S.select(ctx, model, a)
+    return pick, {name: round(v, 4) for name, v in zip(model.get("outputs", S.OUTPUT_NAMES), scores)}
 
 
 def calibration_copy(ctx):
@@ -228,13 +224,19 @@ def bench_large(ctx, args, extra):
     nbytes = spmv_bytes(m.rows, m.cols, m.nnz, 8, 4, 8, 8)
     lens = np.diff(m.offsets)
     if args.alg in ("auto", "panel"):
-        # The direct kernel the learned selector picks (no analysis step) ...
-        direct, scores = learned_pick(ctx, a)
+        # What the learned selector picks: its choice, and the direct kernel
+        # the plan cache holds meanwhile, timed ...
+        pick, scores = learned_pick(ctx, a)
+        direct = pick.direct_algorithm if pick is not None else H.ALG_MERGE
         rd = time_ops(ctx, lambda: H.csr_spmv(ctx, 1.0, a, xd, 0.0, yd, alg=direct, ws=ws), 5, 2)
-        extra["learned_select"] = {"algorithm": H.ALG_NAMES[direct], "scores": scores,
-                                   "gbs": round(nbytes / rd["median_ms"] / 1e6, 1),
-                                   "ms": round(rd["median_ms"], 3)}
-        log(f"learned pick {H.ALG_NAMES[direct]}: {rd['median_ms']:.3f} ms")
+        chosen = (f"panel_{STAGING_NAMES[pick.staging]}" if pick is not None and
+                  pick.algorithm == H.ALG_PANEL else H.ALG_NAMES[direct])
+        extra["learned_select"] = {"choice": chosen, "scores": scores,
+                                   "fell_back": bool(pick.fell_back) if pick is not None else None,
+                                   "direct_algorithm": H.ALG_NAMES[direct],
+                                   "direct_gbs": round(nbytes / rd["median_ms"] / 1e6, 1),
+                                   "direct_ms": round(rd["median_ms"], 3)}
+        log(f"learned pick {chosen} (direct {H.ALG_NAMES[direct]}: {rd['median_ms']:.3f} ms)")
         # ... the column-panel plan with the other staging, for comparison ...
         staging = STAGINGS[args.staging]
         other = H.STAGING_STREAM if staging != H.STAGING_STREAM else H.STAGING_AUTO

@@ -797,11 +797,19 @@ asc_hip_status asc_sparse_hip_normalize_features(asc_hip_context* context,
                                                  float* out);
 
 /* A trained selector: device_model = [mean 16][stddev 16][MLP params].
- * Output j predicts log(time / best time) of algorithm ASC_HIP_ALG_MERGE+j.
+ * `outputs` is 8 (the direct kernels) or 11 (the direct kernels and the
+ * column-panel plans).  Output j predicts the log of the variant's cost
+ * over the best variant's:
+ *   j = 0..7:  SpMV time of ASC_HIP_ALG_MERGE + j (merge, vec1 .. vec64);
+ *   j = 8..10: ASC_HIP_ALG_PANEL with staging ASC_HIP_STAGING_L2, _LDS,
+ *              _STREAM (default panel widths): SpMV time plus the analysis
+ *              time spread over ASC_HIP_SELECT_AMORTIZE SpMVs.
  * Selection takes the smallest prediction unless unsure -- some normalized
  * feature outside [-clip, clip], or the merge-path prediction within
- * `margin` of the best -- in which case it falls back to merge-path.  The
- * choice is stored in the plan cache. */
+ * `margin` of the best -- in which case it falls back to merge-path. */
+#define ASC_HIP_SELECT_AMORTIZE 100
+#define ASC_HIP_SELECT_DIRECT_OUTPUTS 8
+#define ASC_HIP_SELECT_PLAN_OUTPUTS 11
 typedef struct asc_hip_selector {
   const float* device_model;
   int32_t outputs;
@@ -810,13 +818,28 @@ typedef struct asc_hip_selector {
   int32_t reserved;
 } asc_hip_selector;
 
+/* The choice.  A direct pick (algorithm ASC_HIP_ALG_MERGE..VECTOR64) is
+ * stored in the plan cache, so ASC_HIP_ALG_AUTO runs it.  A plan pick
+ * (algorithm ASC_HIP_ALG_PANEL, `staging` set) needs caller memory: the
+ * caller sizes and runs asc_sparse_hip_csr_spmv_analyze with that staging,
+ * which registers the plan; until then the plan cache holds
+ * `direct_algorithm`, the best-predicted direct kernel (merge-path when
+ * unsure). */
+typedef struct asc_hip_selection {
+  uint8_t algorithm;
+  uint8_t staging;          /* ASC_HIP_STAGING_* for ASC_HIP_ALG_PANEL, else 0 */
+  uint8_t direct_algorithm; /* what the plan cache holds after the call */
+  uint8_t fell_back;        /* 1 when unsure: merge-path */
+  uint8_t reserved[4];
+} asc_hip_selection;
+
 asc_hip_status asc_sparse_hip_select_workspace_size(asc_hip_context* context,
                                                     const asc_hip_selector* selector,
                                                     size_t* out_bytes);
 asc_hip_status asc_sparse_hip_select(asc_hip_context* context,
                                      const asc_hip_sparse_desc* matrix,
                                      const asc_hip_selector* selector, void* workspace,
-                                     size_t workspace_bytes, uint8_t* out_algorithm,
+                                     size_t workspace_bytes, asc_hip_selection* out,
                                      float* out_scores);
 
 /* ---- host-forced fault injection (tests) -------------------------------- */

@@ -1,7 +1,8 @@
 """GPU checks of the variant machinery: features, MLP kernels (against the
 FP64 numpy oracle), training, the exhaustive autotuner, selection with its
-merge-path fallback, and -- on the committed MI355X dataset -- the learned
-selector's regret against exhaustive autotuning and against no network.
+merge-path fallback and its column-panel plan outputs, and -- on the
+committed MI355X dataset -- the learned selector's regret against exhaustive
+autotuning over every variant and against no network.
 """
 import glob
 import os
@@ -83,31 +84,59 @@ def test_select_api_sets_plan_and_falls_back(ctx):
     model = S.train(ctx, feats, times, steps=50, log=lambda *a, **k: None)
     m = G.uniform16(4096)
     a = H.clone_compressed(ctx, H.CSR, m.rows, m.cols, m.offsets, m.indices, m.values, H.F64, 4)
-    import ctypes as C
     for clip, margin, expect_merge in ((1e-3, 0.0, True), (1e30, 1e30, True), (1e30, -1e30, False)):
         model["clip"], model["margin"] = clip, margin
-        sel, keep = S.device_selector(ctx, model)
-        need = C.c_size_t()
-        H.check(ctx.lib.asc_sparse_hip_select_workspace_size(ctx.handle, C.byref(sel), C.byref(need)))
-        ws = ctx.alloc(need.value)
-        alg = C.c_uint8()
-        scores = (C.c_float * 8)()
-        H.check(ctx.lib.asc_sparse_hip_select(ctx.handle, C.byref(a.desc), C.byref(sel), ws.ptr,
-                                              ws.nbytes, C.byref(alg), scores))
+        pick, scores = S.select(ctx, model, a)
         if expect_merge:
-            assert alg.value == H.ALG_MERGE
+            assert pick.algorithm == H.ALG_MERGE and pick.fell_back == 1
         else:
-            assert alg.value == H.ALG_MERGE + int(np.argmin(list(scores)))
-        assert H.plan_get(ctx, a) == alg.value
+            assert pick.algorithm == H.ALG_MERGE + int(np.argmin(scores)) and pick.fell_back == 0
+        assert pick.staging == 0 and pick.direct_algorithm == pick.algorithm
+        assert H.plan_get(ctx, a) == pick.algorithm
+
+
+def test_select_with_plan_outputs(ctx):
+    """An 11-output selector whose targets make a plan the best variant picks
+    ALG_PANEL with that plan's staging, and leaves the best direct kernel in
+    the plan cache; the caller's analysis with that staging then registers
+    the plan."""
+    rng = np.random.default_rng(4)
+    m = G.uniform16(4096)
+    a = H.clone_compressed(ctx, H.CSR, m.rows, m.cols, m.offsets, m.indices, m.values, H.F64, 4)
+    # Training inputs around this matrix's own features, so that its
+    # normalized features lie inside the training range.
+    f0 = H.features(ctx, a)
+    feats = (f0[None, :] * (1 + 0.05 * rng.standard_normal((96, 16)))).astype(np.float32)
+    cost = np.exp(rng.uniform(0.5, 1.5, (96, 11)))
+    cost[:, 2] = 1.2   # output 2 (vec2): the best direct kernel
+    cost[:, 10] = 1.0  # the stream plan: the best variant
+    model = S.train(ctx, feats, cost, steps=400, log=lambda *a, **k: None)
+    assert model["k"] == 11
+    model["clip"], model["margin"] = 1e30, -1e30
+    pick, scores = S.select(ctx, model, a)
+    assert len(scores) == 11 and int(np.argmin(scores)) == 10
+    assert pick.algorithm == H.ALG_PANEL and pick.staging == H.STAGING_STREAM
+    assert pick.direct_algorithm == H.ALG_MERGE + int(np.argmin(scores[:8])) == H.ALG_VECTOR[2]
+    assert H.plan_get(ctx, a) == pick.direct_algorithm
+    H.csr_spmv_analyze(ctx, a, H.panel_options(staging=pick.staging))
+    assert H.plan_get(ctx, a) == H.ALG_PANEL
+    H.plan_clear(ctx)
 
 
 @pytest.mark.skipif(not glob.glob(os.path.join(DATA_DIR, "heldout*.npz")) or
                     not os.path.exists(S.DEFAULT_MODEL), reason="no committed selector dataset/model")
 def test_learned_selector_regret_on_heldout(ctx):
+    """On the 500 held-out matrices, against the exhaustive autotuner over all
+    eleven variants (direct kernels and column-panel plans with amortized
+    analysis): the selector's geometric-mean regret is at most 10%, below
+    merge-path alone, the best single variant and the exhaustive autotuner
+    over the direct kernels only."""
     d = S.load_data(sorted(glob.glob(os.path.join(DATA_DIR, "heldout*.npz"))))
     model = S.load_model()
-    r = S.evaluate(ctx, model, d["features"], d["times"])
+    r = S.evaluate(ctx, model, d["features"], S.costs(d))
     print(r)
+    assert r["outputs"] == model["k"] == 11
     assert r["regret_selector"] <= 0.10
     assert r["regret_selector"] < r["regret_merge_only"]
     assert r["regret_selector"] <= r["regret_best_single"]
+    assert r["regret_selector"] < r["regret_direct_autotuner"]

@@ -67,3 +67,24 @@ def test_specs_are_deterministic_and_cover_six_families():
     for i in range(60):
         r = S.spec("train", i)["rows"]
         assert S.ROWS_LO // 2 <= r <= S.ROWS_HI * 1.1
+
+
+def test_costs_amortize_plan_analysis_and_targets_cap_refusals():
+    d = {"times": np.array([[2.0, 4.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0]], np.float32),
+         "plan_spmv": np.array([[0.5, np.inf, 0.25]], np.float32),
+         "plan_analysis": np.array([[100.0, np.inf, 50.0]], np.float32)}
+    c = S.costs(d)
+    assert c.shape == (1, S.K) and S.K == 11
+    # plan cost = SpMV + analysis / 100
+    assert c[0, 8] == pytest.approx(0.5 + 1.0) and c[0, 10] == pytest.approx(0.25 + 0.5)
+    assert np.isinf(c[0, 9])
+    t = S.targets(c)
+    assert t[0, 10] == pytest.approx(0.0)           # the best variant
+    assert t[0, 2] == pytest.approx(np.log(1.0 / 0.75))
+    assert t[0, 9] == S.TARGET_CAP                  # refused plan: capped
+    assert S.OUTPUT_NAMES[8:] == ["panel_l2", "panel_lds", "panel_stream"]
+
+
+def test_regret_caps_refused_picks():
+    cost = np.array([[1.0, np.inf]])
+    assert S.regret(cost, np.array([1])) == pytest.approx(1e6 - 1.0)
