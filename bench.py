@@ -458,8 +458,11 @@ def bench_shuffled_finalize(ctx, m, extra, guard):
 
 def bench_precision(ctx, extra):
     """n = 2^20, 32 nnz/row: FP64 vs FP32/BF16/FP16 storage (FP32 x, y, FP32
-    accumulation plain or compensated).  MALL is flushed before each timed
-    call (the BF16 working set, 222 MB, would otherwise stay cached)."""
+    accumulation plain or compensated), each through its fastest form (direct
+    kernels and, for plain accumulation, the column-panel plans; analysis
+    outside the timed region, as for the headline).  MALL is flushed before
+    each timed call (the BF16 working set, 222 MB, would otherwise stay
+    cached).  Every result's error against the ordered FP64 SpMV is reported."""
     m = G.precision_matrix(1 << 20)
     n = m.rows
     a64 = H.clone_compressed(ctx, H.CSR, n, n, m.offsets, m.indices, m.values, H.F64, 4)
@@ -480,23 +483,60 @@ def bench_precision(ctx, extra):
     yb64 = ctx.alloc(8 * n)
     yd64 = H.vector(yb64, n, H.F64)
 
-    def cold_pick(a, xd, yd, scale=1.0):
-        """The direct variant fastest under the timed conditions (MALL
-        flushed before each call): a warm back-to-back autotune can favour a
-        variant that loses once the matrix comes from HBM."""
-        best, best_ms = None, None
+    # Column-panel plans at their defaults, and the L2/LDS plans with every
+    # row listed (the default lists rows of at least 128 entries, so on this
+    # 32-entry-per-row matrix it leaves every row to the direct kernel).
+    plans = ((H.STAGING_L2, 0, "panel_l2"), (H.STAGING_L2, 1, "panel_l2_all_rows"),
+             (H.STAGING_LDS, 0, "panel_lds"), (H.STAGING_LDS, 1, "panel_lds_all_rows"),
+             (H.STAGING_STREAM, 0, "panel_stream"))
+
+    def timed(a, xd, yd, alg, scale=1.0, acc=H.ACC_PLAIN, steps=3, warmup=1):
+        return time_ops(ctx, lambda: H.csr_spmv(ctx, 1.0 / scale, a, xd, 0.0, yd, alg=alg, acc=acc,
+                                                ws=ws),
+                        steps, warmup, flush=lambda: ctx.fill(flush, 0))["median_ms"]
+
+    def cold_pick(a, xd, yd, scale=1.0, acc=H.ACC_PLAIN):
+        """The fastest form under the timed conditions (MALL flushed before
+        each call): every direct variant and, for plain accumulation, the
+        three column-panel plans (each analysed once, untimed).  Leaves the
+        winning plan analysed; returns (algorithm, name)."""
+        best, best_ms = (H.ALG_MERGE, "merge"), None
         for alg in H.TUNABLE_ALGS:
-            t = time_ops(ctx, lambda: H.csr_spmv(ctx, 1.0 / scale, a, xd, 0.0, yd, alg=alg, ws=ws),
-                         3, 1, flush=lambda: ctx.fill(flush, 0))["median_ms"]
+            t = timed(a, xd, yd, alg, scale, acc)
             if best_ms is None or t < best_ms:
-                best, best_ms = alg, t
+                best, best_ms = (alg, H.ALG_NAMES[alg]), t
+        if acc != H.ACC_PLAIN:
+            return best
+        best_plan = None
+        for staging, lmin, name in plans:
+            try:
+                H.csr_spmv_analyze(ctx, a, H.panel_options(staging=staging, long_row_min=lmin))
+            except H.AscHipError:
+                continue
+            t = timed(a, xd, yd, H.ALG_PANEL, scale)
+            if t < best_ms:
+                best, best_ms, best_plan = (H.ALG_PANEL, name), t, (staging, lmin)
+        if best_plan is not None:
+            H.csr_spmv_analyze(ctx, a, H.panel_options(staging=best_plan[0], long_row_min=best_plan[1]))
         return best
+
+    def spmv_gbs(sv, sx, ms):
+        by = spmv_bytes(n, n, m.nnz, sv, 4, sx, sx)
+        return round(by / ms / 1e6, 1), round(by / ms / 1e6 / HBM_PEAK_GBS, 4)
+
     best64 = cold_pick(a64, xd64, yd64)
-    r = time_ops(ctx, lambda: H.csr_spmv(ctx, 1.0, a64, xd64, 0.0, yd64, alg=best64, ws=ws),
+    r = time_ops(ctx, lambda: H.csr_spmv(ctx, 1.0, a64, xd64, 0.0, yd64, alg=best64[0], ws=ws),
                  7, 2, flush=lambda: ctx.fill(flush, 0))
     fp64_rows = n / (r["median_ms"] * 1e-3)
-    out["fp64"] = {"rows_per_s": fp64_rows, "ms": round(r["median_ms"], 4),
-                   "algorithm": H.ALG_NAMES[best64]}
+    y = ctx.download(yb64, np.float64, n)
+    gbs, frac = spmv_gbs(8, 8, r["median_ms"])
+    out["fp64"] = {"rows_per_s": fp64_rows, "ms": round(r["median_ms"], 4), "algorithm": best64[1],
+                   "gbs": gbs, "frac_of_hbm_peak": frac,
+                   "rel_error_vs_ordered_fp64": float(np.max(np.abs(y - y64)) / norm)}
+    if best64[0] == H.ALG_PANEL:
+        a64.analysis.free()
+        a64.analysis = None
+        H.plan_clear(ctx)
     xb32 = ctx.upload(x64.astype(np.float32))
     x32d = H.vector(xb32, n, H.F32, writable=False)
     vmax = float(np.max(np.abs(m.values)))
@@ -508,18 +548,24 @@ def bench_precision(ctx, extra):
         ak = H.convert_values(ctx, a64, kind, scale)
         yb = ctx.alloc(4 * n)
         yd = H.vector(yb, n, H.F32)
-        best = cold_pick(ak, x32d, yd, scale)
         for acc, aname in ((H.ACC_PLAIN, "plain"), (H.ACC_TWOSUM, "twosum")):
-            rr = time_ops(ctx, lambda: H.csr_spmv(ctx, 1.0 / scale, ak, x32d, 0.0, yd, alg=best,
+            best = cold_pick(ak, x32d, yd, scale, acc)
+            rr = time_ops(ctx, lambda: H.csr_spmv(ctx, 1.0 / scale, ak, x32d, 0.0, yd, alg=best[0],
                                                   acc=acc, ws=ws), 7, 2,
                           flush=lambda: ctx.fill(flush, 0))
             y = ctx.download(yb, np.float32, n).astype(np.float64)
             err = float(np.max(np.abs(y - y64)) / norm)
             rows_s = n / (rr["median_ms"] * 1e-3)
+            gbs, frac = spmv_gbs(H.ELEM_BYTES[kind], 4, rr["median_ms"])
             out[f"{kname}_{aname}"] = {"rows_per_s": rows_s, "ms": round(rr["median_ms"], 4),
                                        "speedup_vs_fp64": round(rows_s / fp64_rows, 3),
-                                       "rel_error_vs_fp64": err, "algorithm": H.ALG_NAMES[best],
+                                       "rel_error_vs_fp64": err, "algorithm": best[1],
+                                       "gbs": gbs, "frac_of_hbm_peak": frac,
                                        "value_scale": scale}
+            if best[0] == H.ALG_PANEL:
+                ak.analysis.free()
+                ak.analysis = None
+                H.plan_clear(ctx)
         del ak, yb
     log("precision: " + ", ".join(f"{k} {v.get('speedup_vs_fp64', 1.0)}x err "
                                   f"{v.get('rel_error_vs_fp64', 0):.2e}" for k, v in out.items()))
@@ -738,19 +784,23 @@ def bench_batched(ctx, extra, count):
 
 
 def bench_section(ctx, args):
-    """One section on the large matrix with one timed call per form, for
-    counter runs (rocprofv3 --pmc) of its kernels."""
-    t0 = time.perf_counter()
-    m = G.zipf_large(args.rows)
-    log(f"generated n={m.rows} nnz={m.nnz} in {time.perf_counter() - t0:.1f}s")
-    a = H.clone_compressed(ctx, H.CSR, m.rows, m.cols, m.offsets, m.indices, m.values, H.F64, 4)
+    """One section (SpMM or the conversions on the large matrix, one timed
+    call per form; or the precision case), for counter runs (rocprofv3 --pmc)
+    of its kernels."""
+    extra, guard = {}, {}
     if args.calibrate:
         calibration_copy(ctx)
-    extra, guard = {}, {}
-    if args.only == "spmm":
-        bench_spmm(ctx, m, a, np.diff(m.offsets), extra, guard, steps=1)
+    if args.only == "precision":
+        bench_precision(ctx, extra)
     else:
-        bench_conversions(ctx, m, a, extra, guard, steps=1)
+        t0 = time.perf_counter()
+        m = G.zipf_large(args.rows)
+        log(f"generated n={m.rows} nnz={m.nnz} in {time.perf_counter() - t0:.1f}s")
+        a = H.clone_compressed(ctx, H.CSR, m.rows, m.cols, m.offsets, m.indices, m.values, H.F64, 4)
+        if args.only == "spmm":
+            bench_spmm(ctx, m, a, np.diff(m.offsets), extra, guard, steps=1)
+        else:
+            bench_conversions(ctx, m, a, extra, guard, steps=1)
     bad = guard_failures(guard)
     out = {"section": args.only, "extra": extra, "guard": guard}
     if bad:
@@ -778,9 +828,9 @@ def main():
     ap.add_argument("--sweep", action="store_true", help="time every SpMV variant")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run one 1 GiB device copy (counter calibration)")
-    ap.add_argument("--only", choices=("spmm", "convert"),
-                    help="profiling aid: run one section on the large matrix (one timed call "
-                         "per form) and print its own JSON line instead of the benchmark line")
+    ap.add_argument("--only", choices=("spmm", "convert", "precision"),
+                    help="profiling aid: run one section (the large-matrix ones with one timed "
+                         "call per form) and print its own JSON line instead of the benchmark line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
