@@ -248,7 +248,9 @@ __device__ __forceinline__ Acc group_sum(const KParams& p, int64_t b, int64_t e,
 // 16- or 32-bit values with int32 indices (p.paired): lane l takes the entry
 // pairs (2l, 2l+1) from the row's first even entry, one load for the two
 // values and one 8-byte load for the two indices, half the load instructions
-// of the per-entry loop; an odd first or last entry goes to lane 0.
+// of the per-entry loop; an odd first or last entry goes to lane 0.  Loads
+// follow the per-entry loop's policy: nontemporal from 16 lanes per row up,
+// cached below (where neighbouring rows share the lines).
 template <int G, typename Acc, typename V>
 __device__ __forceinline__ Acc group_sum_pairs(const KParams& p, int64_t b, int64_t e, int lane) {
   using A = typename acc_of<V>::type;
@@ -265,15 +267,16 @@ __device__ __forceinline__ Acc group_sum_pairs(const KParams& p, int64_t b, int6
     if (lane == 0) acc.add_product(widen<A>(val[b]), x[static_cast<int64_t>(col[b]) * p.incx]);
     b2 = b + 1;
   }
+  constexpr bool kNt = G >= 16;
   for (int64_t k = b2 + 2 * lane; k + 1 < e; k += 2 * G) {
-    const i32x2_t cc = __builtin_nontemporal_load(reinterpret_cast<const i32x2_t*>(col + k));
+    const i32x2_t cc = ld_matrix<kNt>(reinterpret_cast<const i32x2_t*>(col + k));
     V v0, v1;
     if constexpr (sizeof(V) == 2) {
-      const uint32_t w = __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(val + k));
+      const uint32_t w = ld_matrix<kNt>(reinterpret_cast<const uint32_t*>(val + k));
       v0.bits = static_cast<uint16_t>(w & 0xFFFFu);
       v1.bits = static_cast<uint16_t>(w >> 16);
     } else {
-      const u32x2_t w = __builtin_nontemporal_load(reinterpret_cast<const u32x2_t*>(val + k));
+      const u32x2_t w = ld_matrix<kNt>(reinterpret_cast<const u32x2_t*>(val + k));
       v0 = __uint_as_float(w.x);
       v1 = __uint_as_float(w.y);
     }
