@@ -15,7 +15,9 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_DIR = os.path.join(os.path.dirname(_HERE), "lib")
-LIB_PATH = os.path.join(LIB_DIR, "libasc_sparse_hip.so")
+# ASC_SPARSE_HIP_LIBRARY names another build of the C ABI library (the host
+# sanitizer build of tests/test_sanitized_abi.py); the default is the in-tree one.
+LIB_PATH = os.environ.get("ASC_SPARSE_HIP_LIBRARY") or os.path.join(LIB_DIR, "libasc_sparse_hip.so")
 GEN_PATH = os.path.join(LIB_DIR, "libasc_sparse_gen.so")
 PROBE_PATH = os.path.join(LIB_DIR, "libasc_sparse_probe.so")
 

@@ -5,6 +5,8 @@
 
 #include <atomic>
 #include <cstdint>
+#include <cstdlib>
+#include <initializer_list>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -70,6 +72,23 @@ struct Span {
   uintptr_t begin = 0;
   uintptr_t end = 0;
 };
+// The operand spans of one call, held inline so that validation allocates
+// nothing on the success path (no call names more than kMax).
+struct SpanList {
+  static constexpr size_t kMax = 12;
+  Span items[kMax];
+  size_t count = 0;
+  SpanList() = default;
+  SpanList(std::initializer_list<Span> spans) {
+    for (const Span& s : spans) push(s);
+  }
+  void push(Span s) {
+    if (count == kMax) std::abort();  // a call with more operands than kMax: a bug here
+    items[count++] = s;
+  }
+  const Span* begin() const { return items; }
+  const Span* end() const { return items + count; }
+};
 bool overlaps(Span a, Span b);
 Span span_of(const void* p, size_t bytes);
 Span vector_span(const asc_hip_vector_desc& v);
@@ -96,7 +115,7 @@ asc_hip_status check_matrix(const asc_hip_context* ctx, const asc_hip_matrix_des
                             uint8_t kind, int64_t rows, int64_t cols, bool writable,
                             const char* what);
 asc_hip_status check_workspace(const asc_hip_context* ctx, void* ws, size_t ws_bytes,
-                               size_t needed, const std::vector<Span>& operands);
+                               size_t needed, const SpanList& operands);
 asc_hip_status unavailable_if_validation_only(const asc_hip_context* ctx);
 asc_hip_status finish_launch(asc_hip_context* ctx, hipError_t e, const char* what);
 asc_hip::Compressed as_compressed(const asc_hip_sparse_desc& d, bool row_view_of_csc);

@@ -279,7 +279,7 @@ asc_hip_status check_vector(const asc_hip_context* ctx, const asc_hip_vector_des
 }
 
 asc_hip_status check_workspace(const asc_hip_context* ctx, void* ws, size_t ws_bytes,
-                               size_t needed, const std::vector<Span>& operands) {
+                               size_t needed, const SpanList& operands) {
   if (needed == 0) return ok();
   if (ws == nullptr || ws_bytes < needed)
     return fail(ASC_HIP_INVALID_ARGUMENT,

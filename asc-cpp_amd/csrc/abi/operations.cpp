@@ -451,10 +451,10 @@ asc_hip_status prepare_standard_spmv(asc_hip_context* ctx, uint8_t transpose,
   return ok();
 }
 
-std::vector<Span> operand_spans(const asc_hip_sparse_desc& a, std::initializer_list<Span> more) {
+SpanList operand_spans(const asc_hip_sparse_desc& a, std::initializer_list<Span> more) {
   const SparseSpans s = sparse_spans(a);
-  std::vector<Span> v{s.first, s.second, s.values};
-  v.insert(v.end(), more.begin(), more.end());
+  SpanList v{s.first, s.second, s.values};
+  for (const Span& m : more) v.push(m);
   return v;
 }
 
@@ -502,34 +502,6 @@ asc_hip_status run_spmv(asc_hip_context* ctx, SpmvPlan& plan, void* ws) {
 
 }  // namespace
 
-// Device scratch of asc_sparse_hip_csr_spmv_noalloc: one copy per GPU the
-// library runs on, held until the process ends.  In host code the array's
-// name is not a device address: each device's copy is found once, with that
-// device current, by hipGetSymbolAddress, and only that pointer is used.
-static __device__ __align__(256) unsigned char scratch[ASC_HIP_NOALLOC_SCRATCH_BYTES];
-
-namespace {
-constexpr int kMaxScratchDevices = 64;
-std::mutex g_scratch_mutex;
-void* g_scratch_ptr[kMaxScratchDevices] = {};
-}  // namespace
-
-namespace asc_hip_abi {
-asc_hip_status noalloc_scratch(int device, void** out) {
-  *out = nullptr;
-  if (device < 0 || device >= kMaxScratchDevices)
-    return fail(ASC_HIP_UNSUPPORTED, "device ordinal beyond the scratch table");
-  std::lock_guard<std::mutex> lock(g_scratch_mutex);
-  if (g_scratch_ptr[device] == nullptr) {
-    void* p = nullptr;
-    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
-    if (e != hipSuccess) return fail_hip(e, "hipGetSymbolAddress of the allocation-free scratch");
-    g_scratch_ptr[device] = p;
-  }
-  *out = g_scratch_ptr[device];
-  return ok();
-}
-}  // namespace asc_hip_abi
 
 extern "C" {
 

@@ -32,7 +32,7 @@ asc_hip_status check_csr_for_features(asc_hip_context* ctx, const asc_hip_sparse
   return ok();
 }
 
-std::vector<Span> operand_spans_for_select(const asc_hip_sparse_desc& a) {
+SpanList operand_spans_for_select(const asc_hip_sparse_desc& a) {
   const SparseSpans s = sparse_spans(a);
   return {s.first, s.second, s.values};
 }
