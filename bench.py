@@ -154,6 +154,59 @@ def cpu_baseline(m, x, sample_rows, reps=20):
     return line, y
 
 
+def pinned_median(fn, reps):
+    """Median wall time of fn() on one host core (after one warm-up)."""
+    saved = os.sched_getaffinity(0)
+    core = min(saved)
+    os.sched_setaffinity(0, {core})
+    try:
+        fn()
+        times = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            times.append(time.perf_counter() - t0)
+    finally:
+        os.sched_setaffinity(0, saved)
+    return float(np.median(times)), core
+
+
+def cpu_conversions(m, sample_rows, reps=3):
+    """CPU baselines of CSR->CSC, labelled by algorithm: the O(nnz + cols)
+    counting-sort restatement (oracle) on rows [0, sample_rows) of the large
+    matrix, and the reference's own FromCompressed (oracle/_ref, its
+    O(cols * (rows + nnz)) scan) beside the restatement on the 4096 x 4096,
+    16-entries-per-row matrix, where the scan finishes.  One host core each;
+    int64 indices as in the reference."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind as O
+    out = {}
+    r = min(sample_rows, m.rows)
+    nnz = int(m.offsets[r])
+    off = np.ascontiguousarray(m.offsets[: r + 1])
+    idx = m.indices[:nnz].astype(np.int64)
+    val = np.ascontiguousarray(m.values[:nnz])
+    t, core = pinned_median(lambda: O.transpose(0, r, m.cols, off, idx, val), reps)
+    by = (r + 1) * 8 + 2 * nnz * 16 + (m.cols + 1) * 8
+    out["restatement_sample"] = {
+        "algorithm": "O(nnz + cols) stable counting sort (the oracle's restatement)",
+        "sample": f"rows [0, {r}) of the large matrix ({nnz} nnz) -> CSC with {m.cols} columns",
+        "ms": round(t * 1e3, 1), "gbs": round(by / t / 1e9, 3),
+        "entries_per_s": round(nnz / t), "core": core, "reps": reps}
+    small = G.uniform16(4096)
+    so, si, sv = small.offsets, small.indices.astype(np.int64), small.values
+    t, _ = pinned_median(lambda: O.transpose(0, small.rows, small.cols, so, si, sv), reps)
+    out["restatement_4096"] = {"algorithm": "O(nnz + cols) restatement", "nnz": int(small.nnz),
+                               "ms": round(t * 1e3, 3), "entries_per_s": round(small.nnz / t)}
+    if O.ref_available():
+        t, _ = pinned_median(lambda: O.ref_transpose(0, small.rows, small.cols, so, si, sv), reps)
+        out["reference_4096"] = {
+            "algorithm": "the reference's FromCompressed, O(cols * (rows + nnz)) "
+                         "(compressed.h:1151-1221, compiled from its sources)",
+            "nnz": int(small.nnz), "ms": round(t * 1e3, 1), "entries_per_s": round(small.nnz / t)}
+    return out
+
+
 def abs_ax_rows(m, x, rows=None, parts=16):
     """(|A||x|)_i in FP64 with numpy (the guard's error-bound scale), over
     all rows or the rows listed."""
@@ -267,6 +320,15 @@ def bench_large(ctx, args, extra):
     name = H.ALG_NAMES[alg]
     res = time_ops(ctx, lambda: H.csr_spmv(ctx, 1.0, a, xd, 0.0, yd, alg=alg, ws=ws),
                    args.steps, args.warmup)
+    # Enqueue + wait on the host (the reference's second timing figure):
+    # call, then synchronize, per SpMV.
+    host = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        H.csr_spmv(ctx, 1.0, a, xd, 0.0, yd, alg=alg, ws=ws)
+        ctx.synchronize()
+        host.append(time.perf_counter() - t0)
+    res["host_enqueue_wait_ms"] = float(np.median(host)) * 1e3
     log(f"FP64 {name}: {res['loop_ms']:.3f} ms/step, {nbytes / res['loop_ms'] / 1e6:.0f} GB/s")
     # Guard (after timing, outside the timed region): every row of the timed
     # SpMV's output against the ordered kernel's (the reference's summation,
@@ -853,6 +915,11 @@ def main():
         t0 = time.perf_counter()
         cpu, y_cpu = cpu_baseline(m, x, args.cpu_sample_rows, args.cpu_reps)
         log(f"cpu baseline {cpu['value']} GB/s ({time.perf_counter() - t0:.1f}s)")
+        if not args.quick:
+            t0 = time.perf_counter()
+            extra["cpu_conversions"] = cpu_conversions(m, args.cpu_sample_rows)
+            log(f"cpu conversions ({time.perf_counter() - t0:.1f}s): " +
+                ", ".join(f"{k} {v['ms']} ms" for k, v in extra["cpu_conversions"].items()))
         # The device's ordered kernel against the reference's own output.
         guard["ordered_vs_cpu_reference"] = {
             "rows": int(y_cpu.shape[0]),
@@ -878,6 +945,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(res["loop_ms"], 4),
+        "host_enqueue_wait_ms": round(res["host_enqueue_wait_ms"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -894,7 +962,8 @@ def main():
                      "unit": "GB/s", "frac": round(kgbs / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "traffic_source": traffic_src,
                      "measured_ceiling": ceiling,
-                     "frac_of_measured_read": round(kgbs / ceiling["read_gbs"], 4)},
+                     "frac_of_measured_read": round(kgbs / ceiling["read_gbs"], 4),
+                     "frac_of_measured_copy": round(kgbs / ceiling["copy_gbs"], 4)},
         "cpu_baseline": cpu,
         "guard": guard,
         "extra": extra,
