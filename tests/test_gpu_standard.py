@@ -193,6 +193,63 @@ def test_spmm_k32_row_major_large(ctx, alg, dtype):
         assert np.all(err <= bound), float(np.max(err / np.maximum(bound, 1e-300)))
 
 
+@pytest.mark.parametrize("index_bytes", [4, 8])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("k,ld_pad", [(4, 0), (8, 4), (12, 0), (16, 8), (28, 4), (32, 0)])
+def test_spmm_plain_shapes(ctx, index_bytes, dtype, k, ld_pad):
+    """The plain kernel over int32 and int64 indices, FP64 and FP32, k a
+    multiple of 4 up to 32, and leading dimensions beyond k (row-major B and
+    C with ld = k + ld_pad): within the plain bound, the padding columns of C
+    untouched, the result identical on a second run."""
+    m = G.make(G.ZIPF, 2500, 1800, seed=47, values=G.VALUES_NORMAL, zipf_max=700)
+    val = m.values.astype(dtype)
+    rng = np.random.default_rng(k + ld_pad)
+    ld = k + ld_pad
+    b = rng.uniform(-1, 1, (m.cols, ld)).astype(dtype)
+    c0 = rng.uniform(-1, 1, (m.rows, ld)).astype(dtype)
+    a = device_matrix(ctx, H.CSR, m.rows, m.cols, m.offsets, m.indices, val, index_bytes)
+    kind = KIND[dtype]
+    bb = ctx.upload(b.reshape(-1))
+    outs = []
+    print(f"spmm plain shapes {index_bytes} {dtype.__name__} k {k} ld {ld}", flush=True)
+    for _ in range(2):
+        cb = ctx.upload(c0.reshape(-1))
+        H.standard_spmm(ctx, H.OP_N, -1.25, a, H.dense(bb, m.cols, k, ld, H.ROW_MAJOR, kind, writable=False),
+                        H.dense(cb, m.rows, k, ld, H.ROW_MAJOR, kind), alg=H.ALG_AUTO)
+        outs.append(ctx.download(cb, dtype, m.rows * ld).reshape(m.rows, ld))
+    got = outs[0]
+    np.testing.assert_array_equal(outs[1].view(np.uint8), got.view(np.uint8))  # repeatable
+    np.testing.assert_array_equal(got[:, k:], c0[:, k:])  # padding untouched
+    ref = O.std_spmm(0, 0, m.rows, m.cols, m.offsets, m.indices.astype(np.int64), val, -1.25,
+                     b.reshape(-1).copy(), 1, ld, k, c0.reshape(-1).copy(), 1, ld).reshape(m.rows, ld)
+    bound = plain_spmm_bound(H.OP_N, m.rows, m.cols, m.offsets, m.indices, val, -1.25, b[:, :k],
+                             c0[:, :k], EPS[dtype])
+    err = np.abs(got[:, :k].astype(np.float64) - ref[:, :k].astype(np.float64))
+    assert np.all(err <= bound), float(np.max(err / np.maximum(bound, 1e-300)))
+
+
+def test_spmm_plain_falls_back_to_ordered_on_misaligned_blocks(ctx):
+    """B (or C) not aligned to 4 elements, or a leading dimension that is not
+    a multiple of 4, runs the ordered kernels: bit-exact with the oracle."""
+    m = G.make(G.ZIPF, 1500, 1200, seed=48, values=G.VALUES_NORMAL, zipf_max=500)
+    a = device_matrix(ctx, H.CSR, m.rows, m.cols, m.offsets, m.indices, m.values, 4)
+    rng = np.random.default_rng(9)
+    k = 8
+    for case, (b_off, ld) in {"B one element off": (1, 8), "ld 9": (0, 9)}.items():
+        b = rng.uniform(-1, 1, (m.cols, ld))
+        c0 = rng.uniform(-1, 1, (m.rows, ld))
+        bflat = np.concatenate([np.zeros(b_off), b.reshape(-1)])
+        bb, cb = ctx.upload(bflat), ctx.upload(c0.reshape(-1))
+        print(f"spmm fallback: {case}", flush=True)
+        H.standard_spmm(ctx, H.OP_N, 0.5, a,
+                        H.dense(bb.ptr + 8 * b_off, m.cols, k, ld, H.ROW_MAJOR, H.F64, writable=False),
+                        H.dense(cb, m.rows, k, ld, H.ROW_MAJOR, H.F64), alg=H.ALG_AUTO)
+        got = ctx.download(cb, np.float64, m.rows * ld).reshape(m.rows, ld)
+        ref = O.std_spmm(0, 0, m.rows, m.cols, m.offsets, m.indices.astype(np.int64), m.values, 0.5,
+                         b.reshape(-1).copy(), 1, ld, k, c0.reshape(-1).copy(), 1, ld).reshape(m.rows, ld)
+        np.testing.assert_array_equal(got.view(np.uint64), ref.view(np.uint64), err_msg=case)
+
+
 # 256 MiB slices of B: FP64, 32 columns -> 2^20 rows of B per pass.
 PASS_ROWS = (256 << 20) // (8 * 32)
 
