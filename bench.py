@@ -847,13 +847,15 @@ def bench_batched(ctx, extra, count):
 
 def bench_section(ctx, args):
     """One section (SpMM or the conversions on the large matrix, one timed
-    call per form; or the precision case), for counter runs (rocprofv3 --pmc)
-    of its kernels."""
+    call per form; or the precision or batched case), for counter runs
+    (rocprofv3 --pmc) of its kernels."""
     extra, guard = {}, {}
     if args.calibrate:
         calibration_copy(ctx)
     if args.only == "precision":
         bench_precision(ctx, extra)
+    elif args.only == "batched":
+        bench_batched(ctx, extra, args.batch_count)
     else:
         t0 = time.perf_counter()
         m = G.zipf_large(args.rows)
@@ -890,7 +892,7 @@ def main():
     ap.add_argument("--sweep", action="store_true", help="time every SpMV variant")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run one 1 GiB device copy (counter calibration)")
-    ap.add_argument("--only", choices=("spmm", "convert", "precision"),
+    ap.add_argument("--only", choices=("spmm", "convert", "precision", "batched"),
                     help="profiling aid: run one section (the large-matrix ones with one timed "
                          "call per form) and print its own JSON line instead of the benchmark line")
     args = ap.parse_args()
