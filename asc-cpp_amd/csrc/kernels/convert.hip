@@ -771,6 +771,7 @@ hipError_t csr_to_coo_launch(const Compressed& a, int64_t* coords, void* out_val
 }
 
 size_t transpose_workspace_bytes(const Compressed& src) {
+  if (transpose2_applies(src)) return transpose2_workspace_bytes(src);
   const SortLayout L = sort_layout(src.nnz, src.inner, elem_bytes(src.elem));
   return L.total + align256(sizeof(uint32_t) * src.nnz);
 }
@@ -783,6 +784,8 @@ hipError_t transpose_launch(const Compressed& src, int64_t* out_offsets,
     // No entries: every output offset is zero.
     return fill_launch(out_offsets, 0, sizeof(int64_t) * (src.inner + 1), s);
   }
+  if (transpose2_applies(src))
+    return transpose2_launch(src, out_offsets, out_indices, out_index_bytes, out_values, workspace, s);
   switch (elem_bytes(src.elem)) {
     case 8: return transpose_typed<uint64_t>(src, out_offsets, out_indices, out_index_bytes, out_values, ws, s);
     case 4: return transpose_typed<uint32_t>(src, out_offsets, out_indices, out_index_bytes, out_values, ws, s);

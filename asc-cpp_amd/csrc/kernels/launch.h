@@ -329,6 +329,15 @@ hipError_t transpose_launch(const Compressed& src, int64_t* out_offsets,
                             void* out_values, void* workspace,
                             hipStream_t stream);
 
+// The two-pass transpose (transpose.hip) behind transpose_launch for inner
+// extents of 2^9..2^22 (digits of at most 11 bits), nnz and outer below 2^32
+// and at most 16 row segments; transpose2_applies says whether it runs.
+bool transpose2_applies(const Compressed& src);
+size_t transpose2_workspace_bytes(const Compressed& src);
+hipError_t transpose2_launch(const Compressed& src, int64_t* out_offsets, void* out_indices,
+                             int out_index_bytes, void* out_values, void* workspace,
+                             hipStream_t stream);
+
 // Canonical COO (interleaved int64 (row, col)) -> CSR (to_csc false) or CSC.
 size_t coo_convert_workspace_bytes(int64_t rows, int64_t cols, int64_t nnz,
                                    bool to_csc, Elem elem);

@@ -136,3 +136,85 @@ def test_compressed_to_coordinate_random_round_trip(ctx):
         np.testing.assert_array_equal(vals.view(np.uint64), rv.view(np.uint64))
         back = H.convert_coordinate(ctx, coo, H.CSR, 4)
         check_compressed(ctx, back, m.offsets, m.indices, m.values)
+
+
+# ---- the two-pass transpose (inner extents 2^9..2^22) -----------------------
+# Column extents of 2^9..2^22 take the two-pass radix transpose: rows derived
+# in the tile, (high digit, row) packed in 32 bits with the row's top bits
+# recovered from row-segment split points when they do not fit, column
+# offsets read from the second pass's scanned counts.  These cases aim at
+# those parts: dropped row bits (2 and 4 of them), tiles spanning millions of
+# empty rows, one column spread over many tiles, narrow values.
+
+def sparse_rows_csr(rows, cols, nonempty, per_row_max, seed, dense_col=None, vdtype=np.float64):
+    """CSR with `nonempty` random rows of 1..per_row_max distinct random
+    columns (plus `dense_col` in every row when given); values random bits."""
+    rng = np.random.default_rng(seed)
+    which = np.sort(rng.choice(rows, size=nonempty, replace=False)) if nonempty < rows else np.arange(rows)
+    lens = rng.integers(1, per_row_max + 1, size=which.shape[0])
+    rid = np.repeat(which, lens)
+    cid = rng.integers(0, cols, size=rid.shape[0])
+    if dense_col is not None:
+        rid = np.concatenate([rid, which])
+        cid = np.concatenate([cid, np.full(which.shape[0], dense_col)])
+    order = np.lexsort((cid, rid))
+    rid, cid = rid[order], cid[order]
+    keep = np.ones(rid.shape[0], dtype=bool)
+    keep[1:] = (rid[1:] != rid[:-1]) | (cid[1:] != cid[:-1])
+    rid, cid = rid[keep], cid[keep]
+    off = np.zeros(rows + 1, dtype=np.int64)
+    np.add.at(off, rid + 1, 1)
+    off = np.cumsum(off)
+    nbytes = np.dtype(vdtype).itemsize
+    vals = rng.integers(0, 1 << (8 * nbytes - 1), size=rid.shape[0], dtype=np.int64).astype(
+        {8: np.uint64, 4: np.uint32, 2: np.uint16}[nbytes]).view(vdtype)
+    return off, cid.astype(np.int64), vals
+
+
+def check_transpose_both_ways(ctx, rows, cols, off, idx, vals, kind, ib):
+    ro, ri, rv = O.transpose(0, rows, cols, off, idx, vals)
+    csr = H.clone_compressed(ctx, H.CSR, rows, cols, off, idx, vals, kind, ib)
+    csc = H.convert_compressed(ctx, csr)
+    check_compressed(ctx, csc, ro, ri, rv)
+    back = H.convert_compressed(ctx, csc)
+    check_compressed(ctx, back, off, idx, vals)
+
+
+@pytest.mark.parametrize("rows,cols,nonempty,per_row", [
+    ((1 << 22) + 17, (1 << 21) + 3, 2_000_000, 4),  # 23 row bits + 11-bit digit: 2 bits dropped
+    ((1 << 24) + 5, (1 << 21) + 9, 2_500_000, 3),   # 25 row bits: 4 bits dropped (16 segments)
+])
+@pytest.mark.parametrize("ib", [4, 8])
+def test_two_pass_row_segments(ctx, rows, cols, nonempty, per_row, ib):
+    off, idx, vals = sparse_rows_csr(rows, cols, nonempty, per_row, seed=rows % 1009)
+    print(f"two-pass segments rows {rows} cols {cols} nnz {vals.shape[0]} ib {ib}", flush=True)
+    check_transpose_both_ways(ctx, rows, cols, off, idx, vals, H.F64, ib)
+
+
+@pytest.mark.parametrize("kind", [H.F64, H.F32, H.BF16])
+def test_two_pass_dense_column(ctx, kind):
+    # Column 777777 is in every row: its low-digit run spans ten second-pass tiles.
+    rows, cols = 300_000, 1 << 20
+    vd = {H.F64: np.float64, H.F32: np.float32, H.BF16: np.uint16}[kind]
+    off, idx, vals = sparse_rows_csr(rows, cols, rows, 4, seed=5, dense_col=777_777, vdtype=vd)
+    print(f"two-pass dense column kind {kind} nnz {vals.shape[0]}", flush=True)
+    for ib in (4, 8):
+        check_transpose_both_ways(ctx, rows, cols, off, idx, vals, kind, ib)
+
+
+def test_two_pass_empty_row_gaps(ctx):
+    # 20000 non-empty rows among 5M: tiles span millions of empty rows.
+    rows, cols = 5_000_000, 4096
+    off, idx, vals = sparse_rows_csr(rows, cols, 20_000, 5, seed=9)
+    print(f"two-pass empty-row gaps nnz {vals.shape[0]}", flush=True)
+    check_transpose_both_ways(ctx, rows, cols, off, idx, vals, H.F64, 4)
+
+
+@pytest.mark.parametrize("cols", [257, 300, 511, 512, 513, 1 << 22])
+def test_two_pass_extent_edges(ctx, cols):
+    # Smallest and largest inner extents of the two-pass form, and the
+    # digit-width steps in between.
+    rows = 120_000
+    off, idx, vals = sparse_rows_csr(rows, cols, rows - 1000, 6, seed=cols)
+    print(f"two-pass extent cols {cols} nnz {vals.shape[0]}", flush=True)
+    check_transpose_both_ways(ctx, rows, cols, off, idx, vals, H.F64, 8)
