@@ -64,6 +64,7 @@ struct SpmvPlan {
   size_t t_offsets = 0, t_indices = 0, t_values = 0, t_ws = 0, spmv_ws = 0;
   size_t total = 0;
   bool noop = false;
+  asc_hip::PanelPlan spmm_panel{};  // the plan an ASC_HIP_ALG_PANEL SpMM runs on
 };
 
 // The algorithm a call runs: the explicit choice, else the plan cache's entry
@@ -966,15 +967,32 @@ void spmm_args(const SpmvPlan& plan, const asc_hip_matrix_desc& b, const asc_hip
   m->ordered = ordered;
 }
 
-asc_hip_status spmm_form(const asc_hip_spmv_options* opt, bool* ordered) {
+asc_hip_status spmm_form(const asc_hip_spmv_options* opt, bool* ordered, bool* panel) {
   const uint8_t alg = opt ? opt->algorithm : ASC_HIP_ALG_AUTO;
-  if (alg != ASC_HIP_ALG_AUTO && alg != ASC_HIP_ALG_ORDERED)
+  if (alg != ASC_HIP_ALG_AUTO && alg != ASC_HIP_ALG_ORDERED && alg != ASC_HIP_ALG_PANEL)
     return fail(ASC_HIP_INVALID_ARGUMENT,
-                "SpMM takes ASC_HIP_ALG_AUTO (plain) or ASC_HIP_ALG_ORDERED");
+                "SpMM takes ASC_HIP_ALG_AUTO (plain), ASC_HIP_ALG_PANEL or ASC_HIP_ALG_ORDERED");
   if (opt && opt->accumulate != ASC_HIP_ACC_PLAIN)
     return fail(ASC_HIP_UNSUPPORTED, "SpMM accumulates in the storage precision only");
   *ordered = alg == ASC_HIP_ALG_ORDERED;
+  *panel = alg == ASC_HIP_ALG_PANEL;
   return ok();
+}
+
+// The stream-staged column-panel analysis the context holds for `a` (the
+// plain SpMM's ASC_HIP_ALG_PANEL form runs on its segment tables).
+bool find_stream_plan(asc_hip_context* ctx, const asc_hip_sparse_desc& a,
+                      asc_hip::PanelPlan* out) {
+  std::lock_guard<std::mutex> lock(ctx->plan_mutex);
+  for (const auto& p : ctx->plans) {
+    if (p.offsets == a.outer_offsets && p.indices == a.inner_indices && p.rows == a.rows &&
+        p.cols == a.cols && p.nnz == a.nnz && p.element_kind == a.element_kind &&
+        p.algorithm == ASC_HIP_ALG_PANEL && p.panel.stream && p.panel.seg_src != nullptr) {
+      *out = p.panel;
+      return true;
+    }
+  }
+  return false;
 }
 
 asc_hip_status prepare_spmm(asc_hip_context* ctx, uint8_t transpose,
@@ -984,8 +1002,8 @@ asc_hip_status prepare_spmm(asc_hip_context* ctx, uint8_t transpose,
   asc_hip_status st = check_context(ctx);
   if (st.code) return st;
   if (transpose > ASC_HIP_OP_H) return fail(ASC_HIP_INVALID_ARGUMENT, "unknown transpose");
-  bool ordered = false;
-  if ((st = spmm_form(opt, &ordered)).code) return st;
+  bool ordered = false, panel = false;
+  if ((st = spmm_form(opt, &ordered, &panel)).code) return st;
   if ((st = check_device_sparse(ctx, a, "matrix")).code) return st;
   if (a->format == ASC_HIP_COO) return fail(ASC_HIP_INVALID_ARGUMENT, "SpMM requires CSR or CSC");
   if (a->element_kind != ASC_HIP_F32 && a->element_kind != ASC_HIP_F64)
@@ -1000,6 +1018,22 @@ asc_hip_status prepare_spmm(asc_hip_context* ctx, uint8_t transpose,
     return fail(ASC_HIP_INVALID_ARGUMENT, "Sparse matrix-matrix input and output overlap");
   if ((st = plan_row_view(*a, transpose, plan)).code) return st;
   spmm_args(*plan, *b, *c, ordered, m);
+  if (panel) {
+    if (transpose != ASC_HIP_OP_N || a->format != ASC_HIP_CSR)
+      return fail(ASC_HIP_UNSUPPORTED, "ASC_HIP_ALG_PANEL SpMM takes op(A) = A of a CSR matrix");
+    if (!asc_hip::spmm_plain_applies(*m))
+      return fail(ASC_HIP_UNSUPPORTED,
+                  "ASC_HIP_ALG_PANEL SpMM needs the plain form's operands: row-major B and C, "
+                  "k <= 32, k and both leading dimensions multiples of 4, B and C aligned to "
+                  "4 elements");
+    if (!find_stream_plan(ctx, *a, &plan->spmm_panel))
+      return fail(ASC_HIP_INVALID_ARGUMENT,
+                  "ASC_HIP_ALG_PANEL SpMM needs a stream-staged column-panel analysis of this "
+                  "matrix (asc_sparse_hip_csr_spmv_analyze with ASC_HIP_STAGING_STREAM)");
+    if (panel_overlaps(plan->spmm_panel, cs))
+      return fail(ASC_HIP_INVALID_ARGUMENT, "SpMM output overlaps the column-panel analysis");
+    m->panel = &plan->spmm_panel;
+  }
   spmm_plain_plan(m, elem_size(a->element_kind));
   plan->spmv_ws = align256(asc_hip::spmm_workspace_bytes(*m));
   plan->total = transpose_prefix(*plan) + plan->spmv_ws;
@@ -1033,11 +1067,11 @@ asc_hip_status asc_sparse_hip_standard_spmm(asc_hip_context* ctx, uint8_t transp
   asc_hip_status st = prepare_spmm(ctx, transpose, a, b, c, opt, &plan, &m);
   if (st.code) return st;
   // The passes' split table is an optimization: a workspace that only holds
-  // the required part runs the SpMM in one pass.
-  const size_t required = transpose_prefix(plan);
+  // the required part (the transposed operand, the panel form's partial
+  // sums) runs the SpMM in one pass.
   if (ws_bytes < plan.total || ws == nullptr) {
     m.panel_cols = 0;
-    plan.total = required;
+    plan.total = transpose_prefix(plan) + align256(asc_hip::spmm_workspace_bytes(m));
   }
   if ((st = check_workspace(ctx, ws, ws_bytes, plan.total,
                             operand_spans(*a, {matrix_span(*b), matrix_span(*c)}))).code)

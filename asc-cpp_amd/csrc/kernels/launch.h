@@ -261,10 +261,17 @@ struct SpmmArgs {
   // Columns of op(A) (rows of B) per pass; 0 or >= inner: one pass.  Each
   // pass gathers only from its slice of B, which stays memory-side cached.
   int64_t panel_cols = 0;
+  // Plain form with a stream-staged column-panel plan of op(A) (its segment
+  // tables: each listed row's entries in one panel are a segment): the
+  // listed rows go segment by segment, B gathered from the panel's slice in
+  // an XCD's L2, partial sums combined per row in panel order; the other
+  // rows run the plain kernel.  Null: the plain kernel takes every row.
+  const PanelPlan* panel = nullptr;
 };
 // Whether the plain SpMM kernel takes these operands (else the ordered ones).
 bool spmm_plain_applies(const SpmmArgs& args);
-// Workspace: the passes' split table (none for one pass).
+// Workspace: the passes' split table (none for one pass), and with a panel
+// plan the segments' partial sums and the combine's per-part row sums.
 size_t spmm_workspace_bytes(const SpmmArgs& args);
 hipError_t spmm_launch(const SpmmArgs& args, void* workspace, hipStream_t stream);
 

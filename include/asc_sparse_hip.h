@@ -574,7 +574,19 @@ asc_hip_status asc_sparse_hip_triangular_solve(asc_hip_context* context, uint8_t
  *    in slices of 256 MiB, one pass over the matrix per slice, when the
  *    workspace holds the passes' split table ((passes-1)*rows uint32; a
  *    smaller workspace runs one pass).  Other shapes run the ordered form.
- *    Results repeat bit for bit from run to run in both forms.
+ *  - ASC_HIP_ALG_PANEL: the plain form with a stream-staged column-panel
+ *    analysis of this CSR matrix (asc_sparse_hip_csr_spmv_analyze with
+ *    ASC_HIP_STAGING_STREAM; ASC_HIP_INVALID_ARGUMENT without one), op(A) = A
+ *    and the plain form's operands (else ASC_HIP_UNSUPPORTED).  The rows the
+ *    analysis lists go segment by segment (a row's entries in one panel),
+ *    gathering B from the panel's slice, which one XCD's L2 keeps; their
+ *    partial sums, kept in the workspace (32 columns * element bytes per
+ *    segment, plus 4 per listed row), are added in panel order, so the
+ *    result is within the same bound as the plain form and repeats bit for
+ *    bit.  The other rows run the plain kernel and equal its result.  The
+ *    SpMM reads the matrix's current indices and values, not the
+ *    analysis's copy.
+ *    Results repeat bit for bit from run to run in every form.
  * `options->accumulate` must be ASC_HIP_ACC_PLAIN; other algorithms are
  * ASC_HIP_INVALID_ARGUMENT.
  */

@@ -460,8 +460,21 @@ def test_spmm_form_options(vctx):
     for opt in (None, H.options(H.ALG_AUTO), H.options(H.ALG_ORDERED)):
         assert size_code(opt) == H.OK
         assert call_code(opt) == H.UNAVAILABLE  # validation-only context
-    for alg in (H.ALG_MERGE, H.ALG_VECTOR[8], H.ALG_PANEL, 200):
+    for alg in (H.ALG_MERGE, H.ALG_VECTOR[8], 200):
         assert size_code(H.options(alg)) == H.INVALID_ARGUMENT
         assert call_code(H.options(alg)) == H.INVALID_ARGUMENT
     assert size_code(H.options(H.ALG_AUTO, H.ACC_TWOSUM)) == H.UNSUPPORTED
     assert call_code(H.options(H.ALG_ORDERED, H.ACC_TWOSUM)) == H.UNSUPPORTED
+    # ALG_PANEL (the plain form over a stream-staged column-panel analysis):
+    # refused for operands the plain form does not take (k = 2 here), for
+    # op(A) = A^T, and without an analysis of the matrix.
+    assert size_code(H.options(H.ALG_PANEL)) == H.UNSUPPORTED
+    b4 = H.MatrixDesc(0x200000, 4, 4, 4, H.ROW_MAJOR, H.F64, H.DEVICE, 0)
+    c4 = H.MatrixDesc(0x300000, 3, 4, 4, H.ROW_MAJOR, H.F64, H.DEVICE, 1)
+    for op, code in ((H.OP_N, H.INVALID_ARGUMENT), (H.OP_T, H.UNSUPPORTED)):
+        cc = c4 if op == H.OP_N else H.MatrixDesc(0x300000, 4, 4, 4, H.ROW_MAJOR, H.F64, H.DEVICE, 1)
+        bb = b4 if op == H.OP_N else H.MatrixDesc(0x200000, 3, 4, 4, H.ROW_MAJOR, H.F64, H.DEVICE, 0)
+        st = vctx.lib.asc_sparse_hip_standard_spmm_workspace_size(
+            vctx.handle, op, C.byref(a), C.byref(bb), C.byref(cc),
+            C.byref(H.options(H.ALG_PANEL)), C.byref(need))
+        assert st.code == code, (op, st.code)

@@ -1,0 +1,115 @@
+"""GPU parity of the plain SpMM over a stream-staged column-panel analysis
+(ASC_HIP_ALG_PANEL): the analysis's listed rows go segment by segment (one
+segment per row and panel, B gathered from the panel's slice), the other rows
+through the plain kernel.
+
+Every result is checked against the serial oracle (the reference's
+standardized SpMM, blas.h:1692-1750) within the plain-form bound
+4*(len_i+1)*eps*(|alpha|(|A||B|)_ij + |C0_ij|), and against itself on a
+second run (the combine adds the panels in a fixed order).  Rows the analysis
+does not list take the plain kernel and equal its output bit for bit.
+"""
+import numpy as np
+import pytest
+
+import asc_sparse_hip as H
+from asc_sparse_hip import gen as G
+import oracle_bind as O
+from test_gpu_standard import EPS, KIND, plain_spmm_bound
+
+pytestmark = pytest.mark.gpu
+
+
+def spmm(ctx, a, bbuf, b_shape, c0, k, ld, alpha, alg, dtype):
+    kind = KIND[dtype]
+    cb = ctx.upload(c0.reshape(-1))
+    H.standard_spmm(ctx, H.OP_N, alpha, a, H.dense(bbuf, b_shape, k, ld, H.ROW_MAJOR, kind, writable=False),
+                    H.dense(cb, c0.shape[0], k, ld, H.ROW_MAJOR, kind), alg=alg)
+    return ctx.download(cb, dtype, c0.size).reshape(c0.shape)
+
+
+CASES = [
+    # (rows, cols, zipf_max, long_row_min, panel_columns, k, ld, dtype, index_bytes)
+    (1 << 15, 1 << 15, 10_000, 1024, 0, 32, 32, np.float64, 4),
+    (1 << 15, 1 << 15, 10_000, 64, 2048, 32, 32, np.float64, 8),
+    (1 << 15, 1 << 15, 10_000, 1, 0, 16, 20, np.float32, 4),
+    (5000, 70_000, 3000, 16, 97, 20, 24, np.float64, 4),
+    (5000, 70_000, 3000, 128, 4096, 4, 4, np.float32, 8),
+    (20_000, 20_000, 12, 8, 1000, 28, 28, np.float64, 4),
+]
+
+
+@pytest.mark.parametrize("rows,cols,zmax,long_min,pcols,k,ld,dtype,ib", CASES)
+def test_spmm_panel_parity(ctx, rows, cols, zmax, long_min, pcols, k, ld, dtype, ib):
+    m = G.make(G.ZIPF, rows, cols, seed=rows % 1013 + long_min, values=G.VALUES_NORMAL, zipf_max=zmax)
+    val = m.values.astype(dtype)
+    rng = np.random.default_rng(k * 7 + long_min)
+    b = rng.uniform(-1, 1, (cols, ld)).astype(dtype)
+    c0 = rng.uniform(-1, 1, (rows, ld)).astype(dtype)
+    a = H.clone_compressed(ctx, H.CSR, rows, cols, m.offsets, m.indices, val, KIND[dtype], ib)
+    plan = H.csr_spmv_analyze(ctx, a, H.panel_options(panel_columns=pcols, long_row_min=long_min,
+                                                      staging=H.STAGING_STREAM))
+    bbuf = ctx.upload(b.reshape(-1))
+    print(f"spmm panel rows {rows} cols {cols} long_min {long_min} panel {pcols} k {k} ld {ld} "
+          f"{dtype.__name__} ib {ib} nnz {m.nnz}", flush=True)
+    got = spmm(ctx, a, bbuf, cols, c0, k, ld, -0.75, H.ALG_PANEL, dtype)
+    again = spmm(ctx, a, bbuf, cols, c0, k, ld, -0.75, H.ALG_PANEL, dtype)
+    np.testing.assert_array_equal(again.view(np.uint8), got.view(np.uint8))  # repeatable
+    np.testing.assert_array_equal(got[:, k:], c0[:, k:])  # padding columns untouched
+    ref = O.std_spmm(0, 0, rows, cols, m.offsets, m.indices.astype(np.int64), val, -0.75,
+                     b.reshape(-1).copy(), 1, ld, k, c0.reshape(-1).copy(), 1, ld).reshape(rows, ld)
+    bound = plain_spmm_bound(H.OP_N, rows, cols, m.offsets, m.indices, val, -0.75, b[:, :k], c0[:, :k],
+                             EPS[dtype])
+    err = np.abs(got[:, :k].astype(np.float64) - ref[:, :k].astype(np.float64))
+    assert np.all(err <= bound), float(np.max(err / np.maximum(bound, 1e-300)))
+    # Rows below long_row_min ran the plain kernel: the same bits as the
+    # plain form without the analysis.
+    plain = spmm(ctx, a, bbuf, cols, c0, k, ld, -0.75, H.ALG_AUTO, dtype)
+    short = np.diff(m.offsets) < long_min
+    np.testing.assert_array_equal(got[short].view(np.uint8), plain[short].view(np.uint8))
+    del plan
+
+
+def test_spmm_panel_nothing_listed_equals_plain(ctx):
+    # An analysis listing no row (long_row_min above every row length) leaves
+    # every row to the plain kernel.
+    m = G.make(G.ZIPF, 3000, 3000, seed=4, values=G.VALUES_NORMAL, zipf_max=50)
+    rng = np.random.default_rng(2)
+    b = rng.uniform(-1, 1, (3000, 8))
+    c0 = rng.uniform(-1, 1, (3000, 8))
+    a = H.clone_compressed(ctx, H.CSR, 3000, 3000, m.offsets, m.indices, m.values, H.F64, 4)
+    plan = H.csr_spmv_analyze(ctx, a, H.panel_options(long_row_min=100, staging=H.STAGING_STREAM))
+    bbuf = ctx.upload(b.reshape(-1))
+    got = spmm(ctx, a, bbuf, 3000, c0, 8, 8, 1.5, H.ALG_PANEL, np.float64)
+    plain = spmm(ctx, a, bbuf, 3000, c0, 8, 8, 1.5, H.ALG_AUTO, np.float64)
+    np.testing.assert_array_equal(got.view(np.uint64), plain.view(np.uint64))
+    del plan
+
+
+def test_spmm_panel_refusals(ctx):
+    m = G.make(G.ZIPF, 2000, 2000, seed=5, values=G.VALUES_NORMAL, zipf_max=500)
+    a = H.clone_compressed(ctx, H.CSR, 2000, 2000, m.offsets, m.indices, m.values, H.F64, 4)
+    b = ctx.upload(np.zeros(2000 * 8))
+    c = ctx.upload(np.zeros(2000 * 8))
+    bd = H.dense(b, 2000, 8, 8, H.ROW_MAJOR, H.F64, writable=False)
+    cd = H.dense(c, 2000, 8, 8, H.ROW_MAJOR, H.F64)
+    # No analysis of this matrix.
+    with pytest.raises(H.AscHipError) as e:
+        H.standard_spmm(ctx, H.OP_N, 1.0, a, bd, cd, alg=H.ALG_PANEL)
+    assert e.value.code == H.INVALID_ARGUMENT
+    # An L2-staged analysis has no segment tables.
+    plan = H.csr_spmv_analyze(ctx, a, H.panel_options(staging=H.STAGING_L2))
+    with pytest.raises(H.AscHipError) as e:
+        H.standard_spmm(ctx, H.OP_N, 1.0, a, bd, cd, alg=H.ALG_PANEL)
+    assert e.value.code == H.INVALID_ARGUMENT
+    del plan
+    plan = H.csr_spmv_analyze(ctx, a, H.panel_options(staging=H.STAGING_STREAM))
+    # op(A) = A^T, and a column-major block, are not the panel form's.
+    with pytest.raises(H.AscHipError) as e:
+        H.standard_spmm(ctx, H.OP_T, 1.0, a, bd, cd, alg=H.ALG_PANEL)
+    assert e.value.code == H.UNSUPPORTED
+    with pytest.raises(H.AscHipError) as e:
+        H.standard_spmm(ctx, H.OP_N, 1.0, a, H.dense(b, 2000, 8, 2000, H.COLUMN_MAJOR, H.F64, writable=False),
+                        cd, alg=H.ALG_PANEL)
+    assert e.value.code == H.UNSUPPORTED
+    del plan
