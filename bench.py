@@ -14,7 +14,7 @@ algorithmic bytes per SpMV / time:
   B = nnz*(s_v + s_i) + (rows+1)*s_o + cols*s_x + rows*s_y*(1 + [beta != 0])
 
 Extra lines (same JSON object, "extra"): FP32 storage; SpMM with a 32-column
-row-major block; CSR->CSC and COO->CSR on the same matrix; the precision
+row-major block (plain, column-panel and ordered forms); CSR->CSC and COO->CSR on the same matrix; the precision
 case (n = 2^20, 32 nnz/row, lognormal values: FP64 vs FP32/BF16/FP16
 storage, rows/s and error against the FP64 result); the batched case
 (10,000 matrices in one hipGraph-replayed persistent launch, SpMV/s).
@@ -378,10 +378,19 @@ def bench_large(ctx, args, extra):
     return m, alg, res, nbytes, y_ord, guard, x
 
 
+# SpMM panel form: a stream-staged column-panel analysis with panels of 16384
+# columns (B's panel slice, 16384 rows x 256 bytes = 4 MiB, one XCD's L2) and
+# rows of at least 4096 entries listed (68% of the Zipf matrix's entries);
+# measured on that matrix at k = 32: 8192/16384-column panels 20.2/18.1 ms,
+# listed-row minimum 1024/2048/4096 with 16384 columns 19.9/18.6/18.1 ms.
+SPMM_PANEL_COLUMNS = 16384
+SPMM_PANEL_LONG_MIN = 4096
+
+
 def bench_spmm(ctx, m, a, lens, extra, guard, steps=3):
     """SpMM C += A*B on the large matrix, B row-major 2^22 x 32 (U[-1,1]): the
-    plain and ordered forms timed, sampled rows checked, and the B-row gather
-    ceiling measured beside them."""
+    plain, panel and ordered forms timed, sampled rows checked, and the B-row
+    gather ceiling measured beside them."""
     k = 32
     bmat = ctx.alloc(8 * m.cols * k)
     bh = G.vector(m.cols * k, 777)
@@ -394,7 +403,16 @@ def bench_spmm(ctx, m, a, lens, extra, guard, steps=3):
     sws = H.Workspace(ctx)
     bs = m.nnz * 12 + (m.rows + 1) * 8 + m.cols * k * 8 + 2 * m.rows * k * 8
     spmm_ms = {}
-    for form, alg in (("plain", H.ALG_AUTO), ("ordered", H.ALG_ORDERED)):
+    # The panel form's analysis (once, outside the timed region; it replaces
+    # the matrix's SpMV plan, which is not timed again).
+    t0 = time.perf_counter()
+    H.csr_spmv_analyze(ctx, a, H.panel_options(panel_columns=SPMM_PANEL_COLUMNS,
+                                                long_row_min=SPMM_PANEL_LONG_MIN,
+                                                staging=H.STAGING_STREAM))
+    ctx.synchronize()
+    panel_analysis_ms = (time.perf_counter() - t0) * 1e3
+    forms = (("plain", H.ALG_AUTO), ("panel", H.ALG_PANEL), ("ordered", H.ALG_ORDERED))
+    for form, alg in forms:
         rs = time_ops(ctx, lambda alg=alg: H.standard_spmm(ctx, H.OP_N, 1.0, a, bd, cd, sws,
                                                            alg=alg), 3, 1)
         spmm_ms[form] = rs["median_ms"]
@@ -404,6 +422,14 @@ def bench_spmm(ctx, m, a, lens, extra, guard, steps=3):
     extra["spmm_k32_row_major"] = {
         "gbs": round(bs / spmm_ms["plain"] / 1e6, 1), "ms": round(spmm_ms["plain"], 3),
         "form": "plain (FMA, 256 MiB column passes)",
+        "panel_ms": round(spmm_ms["panel"], 3),
+        "panel_gbs": round(bs / spmm_ms["panel"] / 1e6, 1),
+        "panel_b_row_gather_gbs": round(gathered / spmm_ms["panel"] / 1e6, 1),
+        "panel_form": {"panel_columns": SPMM_PANEL_COLUMNS, "long_row_min": SPMM_PANEL_LONG_MIN,
+                       "listed_rows": int(np.sum(lens >= SPMM_PANEL_LONG_MIN)),
+                       "listed_nnz_share": round(float(lens[lens >= SPMM_PANEL_LONG_MIN].sum())
+                                                 / m.nnz, 4),
+                       "analysis_ms": round(panel_analysis_ms, 1)},
         "ordered_gbs": round(bs / spmm_ms["ordered"] / 1e6, 1),
         "ordered_ms": round(spmm_ms["ordered"], 3),
         "b_row_gathers_gb": round(gathered / 1e9, 1),
@@ -414,7 +440,7 @@ def bench_spmm(ctx, m, a, lens, extra, guard, steps=3):
     # Guard: sampled rows (the longest among them) recomputed in the
     # reference's order, C(r,:) += (1*v)*B(c,:) entry by entry, after one
     # more call of each form from zero: the ordered form bit for bit, the
-    # plain form within 4*(len+1)*eps*(|A||B|)(r,:).
+    # plain and panel forms within 4*(len+1)*eps*(|A||B|)(r,:).
     bhost = bh.reshape(m.cols, k)
     pick = np.concatenate([np.random.default_rng(5).integers(0, m.rows, 256),
                            np.argsort(lens)[-8:]])
@@ -426,7 +452,7 @@ def bench_spmm(ctx, m, a, lens, extra, guard, steps=3):
             mag = mag + abs(m.values[p]) * np.abs(bhost[m.indices[p]])
         serial[r], magnitude[r] = acc, mag
     spmm_guard = {"rows_checked": int(pick.size)}
-    for form, alg in (("plain", H.ALG_AUTO), ("ordered", H.ALG_ORDERED)):
+    for form, alg in forms:
         ctx.fill(cmat, 0)
         H.standard_spmm(ctx, H.OP_N, 1.0, a, bd, cd, sws, alg=alg)
         cgot = ctx.download(cmat, np.float64, m.rows * k).reshape(m.rows, k)
@@ -435,10 +461,13 @@ def bench_spmm(ctx, m, a, lens, extra, guard, steps=3):
                 int(np.array_equal(serial[r].view(np.uint64), cgot[r].view(np.uint64)))
                 for r in pick)
         else:
-            spmm_guard["plain_rows_within_bound"] = sum(
+            spmm_guard[f"{form}_rows_within_bound"] = sum(
                 int(np.all(np.abs(cgot[r] - serial[r]) <=
                            4 * (lens[r] + 1) * 2.0 ** -53 * magnitude[r])) for r in pick)
     guard["spmm_k32"] = spmm_guard
+    H.plan_clear(ctx)
+    a.analysis.free()
+    a.analysis = None
     del bmat, cmat, bh, cgot, bhost
 
 
@@ -672,7 +701,8 @@ def guard_failures(guard):
     for k, v in guard.items():
         if v.get("rows_over_bound", 0) != 0:
             bad.append(k)
-        for count in ("rows_bit_exact", "ordered_rows_bit_exact", "plain_rows_within_bound"):
+        for count in ("rows_bit_exact", "ordered_rows_bit_exact", "plain_rows_within_bound",
+                      "panel_rows_within_bound"):
             if count in v and v[count] != v["rows_checked"]:
                 bad.append(k)
         if any(val is False for val in v.values()):
