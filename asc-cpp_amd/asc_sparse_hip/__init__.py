@@ -489,16 +489,28 @@ def standard_spmm(ctx: Context, transpose: int, alpha: float, a: DeviceSparse, b
     return ws
 
 
+def _output_buffers(ctx: Context, into: DeviceSparse | None, sizes: list) -> list:
+    """Fresh output buffers of `sizes` bytes, or the (offsets, indices,
+    values) buffers of an earlier result `into` when each is large enough."""
+    if into is None:
+        return [ctx.alloc(n) for n in sizes]
+    bufs = into.buffers[:3]
+    if len(bufs) != 3 or any(b.nbytes < n for b, n in zip(bufs, sizes)):
+        raise ValueError("the output buffers to reuse are smaller than the conversion's output")
+    return bufs
+
+
 def convert_coordinate(ctx: Context, coo: DeviceSparse, out_format: int, index_bytes: int = 8,
-                       ws: Workspace | None = None) -> DeviceSparse:
+                       ws: Workspace | None = None, into: DeviceSparse | None = None) -> DeviceSparse:
+    """COO -> CSR/CSC into fresh device buffers, or into those of an earlier
+    result `into` (no allocation: the C ABI's caller-provided outputs)."""
     d = coo.desc
     need = C.c_size_t()
     check(ctx.lib.asc_sparse_hip_convert_coordinate_workspace_size(
         ctx.handle, C.byref(d), out_format, index_bytes, C.byref(need)))
     outer = d.rows if out_format == CSR else d.cols
-    bo = ctx.alloc(8 * (outer + 1))
-    bi = ctx.alloc(max(index_bytes * d.nnz, 1))
-    bv = ctx.alloc(max(ELEM_BYTES[d.element_kind] * d.nnz, 1))
+    bo, bi, bv = _output_buffers(ctx, into, [8 * (outer + 1), max(index_bytes * d.nnz, 1),
+                                             max(ELEM_BYTES[d.element_kind] * d.nnz, 1)])
     ws = ws or Workspace(ctx)
     p, n = ws.get(need.value)
     out = SparseDesc()
@@ -508,15 +520,17 @@ def convert_coordinate(ctx: Context, coo: DeviceSparse, out_format: int, index_b
     return DeviceSparse(out, [bo, bi, bv])
 
 
-def convert_compressed(ctx: Context, a: DeviceSparse, ws: Workspace | None = None) -> DeviceSparse:
+def convert_compressed(ctx: Context, a: DeviceSparse, ws: Workspace | None = None,
+                       into: DeviceSparse | None = None) -> DeviceSparse:
+    """CSR <-> CSC into fresh device buffers, or into those of an earlier
+    result `into` (no allocation: the C ABI's caller-provided outputs)."""
     d = a.desc
     need = C.c_size_t()
     check(ctx.lib.asc_sparse_hip_convert_compressed_workspace_size(ctx.handle, C.byref(d),
                                                                    C.byref(need)))
     outer = d.cols if d.format == CSR else d.rows
-    bo = ctx.alloc(8 * (outer + 1))
-    bi = ctx.alloc(max(d.index_bytes * d.nnz, 1))
-    bv = ctx.alloc(max(ELEM_BYTES[d.element_kind] * d.nnz, 1))
+    bo, bi, bv = _output_buffers(ctx, into, [8 * (outer + 1), max(d.index_bytes * d.nnz, 1),
+                                             max(ELEM_BYTES[d.element_kind] * d.nnz, 1)])
     ws = ws or Workspace(ctx)
     p, n = ws.get(need.value)
     out = SparseDesc()
