@@ -743,6 +743,10 @@ template <typename Element, SparseCompressedFormat Format>
 enum class HipSpmmForm : std::uint8_t {
   kPlain = ASC_HIP_ALG_AUTO,
   kOrdered = ASC_HIP_ALG_ORDERED,
+  // The plain form over the matrix's stream-staged column-panel analysis
+  // (HipCsrSpmvAnalyze with ASC_HIP_STAGING_STREAM): CSR, op N and the plain
+  // kernel's operands only.
+  kPanel = ASC_HIP_ALG_PANEL,
 };
 
 template <typename Element, typename Matrix>

@@ -516,6 +516,27 @@ void TestColumnPanel(Gpu& g, uint8_t staging) {
     done = asc::HipCsrSpmv(g.ctx, 2.0, cv, *x, 0.0, *y, Whole(ws), asc::HipSpmvAlgorithm::kColumnPanel);
     CHECK(done.ok() && done->Wait().ok());
     CHECK((Download<double>(g, yb, 4) == std::vector<double>{-8, -32, 44, -8}));
+    // The SpMM's column-panel form on the same analysis reads the current
+    // (doubled) values: C = 2A*B with B's rows (1,2,3,4), (0,1,0,-1),
+    // (2,0,1,0), (1,1,1,1), (0,0,2,1), every product exact.
+    auto bb = Upload<double>(g, {1, 2, 3, 4, 0, 1, 0, -1, 2, 0, 1, 0, 1, 1, 1, 1, 0, 0, 2, 1});
+    auto cb = Upload<double>(g, std::vector<double>(16, 0.0));
+    const asc::ConstMemoryView bs(bb.data(), 20 * sizeof(double), MemorySpace::kDevice);
+    const asc::ConstMemoryView cs(cb.data(), 16 * sizeof(double), MemorySpace::kDevice);
+    auto b = asc::SparseBlasMatrixView<const double>::Create(
+        static_cast<const double*>(bb.data()), 5, 4, asc::SparseBlasLayout::kRowMajor, 4, bs);
+    auto c = asc::SparseBlasMatrixView<double>::Create(static_cast<double*>(cb.data()), 4, 4,
+                                                       asc::SparseBlasLayout::kRowMajor, 4, cs);
+    CHECK(b.ok() && c.ok());
+    auto mneed = asc::HipSpmmWorkspaceSize(g.ctx, asc::SparseBlasTranspose::kNone, cv, *b, *c,
+                                           asc::HipSpmmForm::kPanel);
+    CHECK(mneed.ok());
+    asc::Buffer mws = Allocate(g, *mneed + 256);
+    auto spmm = asc::HipSpmm(g.ctx, asc::SparseBlasTranspose::kNone, 1.0, cv, *b, *c, Whole(mws),
+                             asc::HipSpmmForm::kPanel);
+    CHECK(spmm.ok() && spmm->Wait().ok());
+    CHECK((Download<double>(g, cb, 16) ==
+           std::vector<double>{2, 6, 10, 14, 0, 8, 0, -8, 26, 28, 68, 66, 0, -4, 24, 16}));
   }
   CHECK(asc_sparse_hip_plan_clear(g.ctx.native()).code == ASC_HIP_OK);
 }

@@ -7,7 +7,8 @@
   plan, merge path and the L2-staged plan within the plain-mode bound
   |y - y_ref| <= 4*len_i*eps*|alpha|*(|A||x|)_i + 4*eps*|beta*y0_i|, in FP64
   and FP32.  The ordered SpMM with a 2^22 x 32 row-major block is checked bit
-  for bit on every entry.  CSR->CSC is checked through properties that fix the
+  for bit on every entry, the plain and column-panel forms within the plain
+  bound on every entry.  CSR->CSC is checked through properties that fix the
   canonical transpose (per-column counts, rows strictly increasing within each
   column, per-column checksums of row ids and value bits, sampled entries
   found with their exact bits) and transposing back gives the original bit
@@ -172,6 +173,34 @@ def test_cfg2_spmm_k32_plain_every_entry(ctx, cfg2, cfg2_spmm):
     err = np.abs(got - ref)
     worst = float(np.max(err / np.maximum(bound, 1e-300)))
     print(f"SpMM plain: max error / bound {worst:.3e}", flush=True)
+    assert np.all(err <= bound), worst
+
+
+def test_cfg2_spmm_k32_panel_every_entry(ctx, cfg2, cfg2_spmm):
+    """The column-panel form on the full matrix (stream-staged analysis,
+    16384-column panels, rows of at least 4096 entries listed: 12.9M
+    segments): every C(i,j) within the plain bound, repeatable bit for bit."""
+    m, a, _ = cfg2
+    b, c0, ref = cfg2_spmm
+    say("SpMM k=32 row-major, panel form: analysis")
+    H.csr_spmv_analyze(ctx, a, H.panel_options(panel_columns=16384, long_row_min=4096,
+                                                staging=H.STAGING_STREAM))
+    say("SpMM k=32 row-major, panel form: device, twice")
+    got = _device_spmm(ctx, m, a, b, c0, H.ALG_PANEL)
+    again = _device_spmm(ctx, m, a, b, c0, H.ALG_PANEL)
+    a.analysis.free()
+    a.analysis = None
+    H.plan_clear(ctx)
+    np.testing.assert_array_equal(again.view(np.uint64), got.view(np.uint64))
+    del again
+    say("SpMM k=32 row-major, panel form: bound")
+    rowmax = np.abs(b).max(axis=1)
+    mag = O.abs_spmv_rows(m.offsets, m.indices, m.values, rowmax)
+    lens = np.diff(m.offsets).astype(np.float64)
+    bound = 4 * (lens[:, None] + 1) * EPS64 * (0.5 * mag[:, None] + np.abs(c0))
+    err = np.abs(got - ref)
+    worst = float(np.max(err / np.maximum(bound, 1e-300)))
+    print(f"SpMM panel: max error / bound {worst:.3e}", flush=True)
     assert np.all(err <= bound), worst
 
 
