@@ -461,7 +461,14 @@ constexpr int kSegLanes = 16;  // lanes a segment (one DPP row)
 constexpr int kSegStep = 16;  // entries a group gathers at once
 constexpr int kSegBlock = 256;
 constexpr int kCombineBlock = 1024;
-constexpr int kCombineParts = 4;
+// The combine splits each group's panels into `parts` ranges (one
+// work-group each, at most kCombineMaxParts), chosen so that the work-groups
+// (groups x column halves x parts, one 1024-thread work-group per CU for its
+// 128 KiB of LDS) come to about two per CU of an MI355X: on the 640M-entry
+// Zipf matrix (50 groups, 100 units) 4 parts 1.20 ms, 5 parts 1.00 ms,
+// 8 parts 1.22 ms, 16 parts 1.10 ms.
+constexpr int kCombineMaxParts = 16;
+constexpr int64_t kCombineTargetWgs = 2 * 256;
 // Chunks per panel: about one segment per 16-lane group (16 segments a
 // 256-thread work-group), so that an XCD's work-groups are short-lived and
 // dispatched in panel order, and the XCD's L2 holds one panel's slice of B
@@ -600,19 +607,25 @@ __global__ __launch_bounds__(kSegBlock) void spmm_seg_kernel(
 // 128 KiB of LDS): a 32-column partial row is combined in two halves.
 constexpr int kCombineCols = 16;
 
+// Panel ranges per combine group for `units` (groups x column halves).
+int64_t combine_parts(int64_t units) {
+  const int64_t p = (kCombineTargetWgs + units / 2) / (units > 0 ? units : 1);
+  return p < 1 ? 1 : p > kCombineMaxParts ? kCombineMaxParts : p;
+}
+
 template <typename T, int W>
 __global__ __launch_bounds__(kCombineBlock) void spmm_panel_combine_kernel(
     const T* __restrict__ partial, const uint16_t* __restrict__ seg_lrow,
     const uint32_t* __restrict__ ctable, int64_t groups, int64_t panels, int64_t nlong,
-    T* __restrict__ parts) {
+    int64_t nparts, T* __restrict__ parts) {
   constexpr int H = W / kCombineCols;  // column halves
   __shared__ T acc[kStreamGroupRows * kCombineCols];
-  const int64_t part = blockIdx.x % kCombineParts;
-  const int64_t gh = blockIdx.x / kCombineParts;
+  const int64_t part = blockIdx.x % nparts;
+  const int64_t gh = blockIdx.x / nparts;
   const int64_t g = gh / H, jc = (gh % H) * kCombineCols;
   for (int i = threadIdx.x; i < kStreamGroupRows * kCombineCols; i += kCombineBlock) acc[i] = T(0);
   __syncthreads();
-  const int64_t qa = panels * part / kCombineParts, qb = panels * (part + 1) / kCombineParts;
+  const int64_t qa = panels * part / nparts, qb = panels * (part + 1) / nparts;
   for (int64_t q = qa; q < qb; ++q) {
     const uint64_t lo = ctable[q * (groups + 1) + g], hi = ctable[q * (groups + 1) + g + 1];
     for (uint64_t i = lo * kCombineCols + threadIdx.x; i < hi * kCombineCols; i += kCombineBlock) {
@@ -631,15 +644,15 @@ __global__ __launch_bounds__(kCombineBlock) void spmm_panel_combine_kernel(
 
 template <typename T, int W>
 __global__ __launch_bounds__(kBlock) void spmm_panel_finish_kernel(
-    const T* __restrict__ parts, int64_t nlong, const uint32_t* __restrict__ long_rows,
-    T* __restrict__ c, int64_t ldc, int64_t j0, int64_t kk, double alpha) {
+    const T* __restrict__ parts, int64_t nlong, int64_t nparts,
+    const uint32_t* __restrict__ long_rows, T* __restrict__ c, int64_t ldc, int64_t j0, int64_t kk,
+    double alpha) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (i >= nlong * W) return;
   const int64_t li = i / W, j = i % W;
   if (j >= kk) return;
   T sum = T(0);
-#pragma unroll
-  for (int part = 0; part < kCombineParts; ++part) sum += parts[(part * nlong + li) * W + j];
+  for (int64_t part = 0; part < nparts; ++part) sum += parts[(part * nlong + li) * W + j];
   T* cp = c + static_cast<int64_t>(long_rows[li]) * ldc + j0 + j;
   *cp = __builtin_fma(static_cast<T>(alpha), sum, *cp);
 }
@@ -661,7 +674,8 @@ hipError_t launch_panel_rows(const SpmmArgs& a, const PanelPlan& pp, char* ws, h
   const unsigned seg_grid =
       static_cast<unsigned>((pp.panels + kXcds - 1) / kXcds * kXcds * chunks);
   // Combine work-groups: each group's column halves times the panel parts.
-  const unsigned comb_grid = static_cast<unsigned>(groups * (width / kCombineCols) * kCombineParts);
+  const int64_t nparts = combine_parts(groups * (width / kCombineCols));
+  const unsigned comb_grid = static_cast<unsigned>(groups * (width / kCombineCols) * nparts);
   const unsigned fin_grid = static_cast<unsigned>((pp.nlong * width + kBlock - 1) / kBlock);
   // 32-bit B offsets (24-bit row index and row pitch) when they reach all of B.
   const int64_t row_bytes = a.ldb * static_cast<int64_t>(sizeof(T));
@@ -682,14 +696,14 @@ hipError_t launch_panel_rows(const SpmmArgs& a, const PanelPlan& pp, char* ws, h
 #undef ASC_SEG
     if (width == 32) {
       spmm_panel_combine_kernel<T, 32><<<comb_grid, kCombineBlock, 0, s>>>(
-          partial, pp.seg_lrow, pp.ctable, groups, pp.panels, pp.nlong, parts);
+          partial, pp.seg_lrow, pp.ctable, groups, pp.panels, pp.nlong, nparts, parts);
       spmm_panel_finish_kernel<T, 32><<<fin_grid, kBlock, 0, s>>>(
-          parts, pp.nlong, pp.long_rows, static_cast<T*>(a.c), a.ldc, j0, kk, a.alpha);
+          parts, pp.nlong, nparts, pp.long_rows, static_cast<T*>(a.c), a.ldc, j0, kk, a.alpha);
     } else {
       spmm_panel_combine_kernel<T, 16><<<comb_grid, kCombineBlock, 0, s>>>(
-          partial, pp.seg_lrow, pp.ctable, groups, pp.panels, pp.nlong, parts);
+          partial, pp.seg_lrow, pp.ctable, groups, pp.panels, pp.nlong, nparts, parts);
       spmm_panel_finish_kernel<T, 16><<<fin_grid, kBlock, 0, s>>>(
-          parts, pp.nlong, pp.long_rows, static_cast<T*>(a.c), a.ldc, j0, kk, a.alpha);
+          parts, pp.nlong, nparts, pp.long_rows, static_cast<T*>(a.c), a.ldc, j0, kk, a.alpha);
     }
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -774,12 +788,14 @@ size_t split_bytes(const SpmmArgs& a) {
 }
 
 // Workspace of the panel path: one partial row per segment, then the
-// combine's per-part row sums (kCombineParts x listed rows).
+// combine's per-part row sums (parts x listed rows).
 size_t panel_bytes(const SpmmArgs& a) {
   if (a.panel == nullptr) return 0;
   const size_t esz = a.a.elem == Elem::F64 ? 8 : 4;
-  return align_up(esz * seg_cols(a.k) * static_cast<size_t>(a.panel->nseg)) +
-         align_up(esz * seg_cols(a.k) * kCombineParts * static_cast<size_t>(a.panel->nlong));
+  const int64_t width = seg_cols(a.k);
+  const size_t nparts = static_cast<size_t>(combine_parts(a.panel->groups * (width / kCombineCols)));
+  return align_up(esz * width * static_cast<size_t>(a.panel->nseg)) +
+         align_up(esz * width * nparts * static_cast<size_t>(a.panel->nlong));
 }
 
 }  // namespace
