@@ -381,8 +381,8 @@ def bench_large(ctx, args, extra):
 # SpMM panel form: a stream-staged column-panel analysis with panels of 16384
 # columns (B's panel slice, 16384 rows x 256 bytes = 4 MiB, one XCD's L2) and
 # rows of at least 4096 entries listed (68% of the Zipf matrix's entries);
-# measured on that matrix at k = 32: 8192/16384-column panels 20.2/18.1 ms,
-# listed-row minimum 1024/2048/4096 with 16384 columns 19.9/18.6/18.1 ms.
+# measured on that matrix at k = 32: 8192/16384-column panels 20.3/17.5 ms,
+# listed-row minimum 2048/4096 with 16384 columns 18.3/17.5 ms.
 SPMM_PANEL_COLUMNS = 16384
 SPMM_PANEL_LONG_MIN = 4096
 
