@@ -474,8 +474,10 @@ def standard_spmv(ctx: Context, transpose: int, alpha: float, a: DeviceSparse, x
 
 def standard_spmm(ctx: Context, transpose: int, alpha: float, a: DeviceSparse, b: MatrixDesc,
                   c: MatrixDesc, ws: Workspace | None = None, alg: int = ALG_AUTO):
-    """C += alpha*op(A)*B; alg ALG_AUTO (plain form), ALG_PANEL (plain form over the
-    context's stream-staged column-panel analysis of A) or ALG_ORDERED (bit-exact)."""
+    """C += alpha*op(A)*B; alg ALG_AUTO (plain form; the panel form when the
+    context holds a stream-staged analysis of A listing only long rows),
+    ALG_PANEL (plain form over the context's stream-staged column-panel
+    analysis of A) or ALG_ORDERED (bit-exact)."""
     opt = options(alg)
     need = C.c_size_t()
     check(ctx.lib.asc_sparse_hip_standard_spmm_workspace_size(

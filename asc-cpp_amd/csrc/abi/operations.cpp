@@ -65,6 +65,7 @@ struct SpmvPlan {
   size_t total = 0;
   bool noop = false;
   asc_hip::PanelPlan spmm_panel{};  // the plan an ASC_HIP_ALG_PANEL SpMM runs on
+  bool spmm_panel_optional = false;  // AUTO took it: dropped when the workspace is short
 };
 
 // The algorithm a call runs: the explicit choice, else the plan cache's entry
@@ -995,6 +996,19 @@ bool find_stream_plan(asc_hip_context* ctx, const asc_hip_sparse_desc& a,
   return false;
 }
 
+// Whether ASC_HIP_ALG_AUTO takes the SpMM's panel form on a stream-staged
+// analysis: only when its listed rows are long against the panel count, so
+// that their (row, panel) segments hold several entries each.  On the
+// 640M-entry Zipf matrix (256 panels, k = 32) the panel form took 17.5 /
+// 18.3 / 19.9 ms with rows of at least 16 / 8 / 4 entries per panel listed,
+// against 24.5 ms for the plain form; an analysis listing every row (the
+// stream form's default) gives segments of one or two entries.
+constexpr int64_t kSpmmAutoPanelEntries = 8;
+
+bool spmm_panel_pays(const asc_hip::PanelPlan& pp) {
+  return pp.nlong > 0 && pp.long_min >= kSpmmAutoPanelEntries * pp.panels;
+}
+
 asc_hip_status prepare_spmm(asc_hip_context* ctx, uint8_t transpose,
                             const asc_hip_sparse_desc* a, const asc_hip_matrix_desc* b,
                             const asc_hip_matrix_desc* c, const asc_hip_spmv_options* opt,
@@ -1033,6 +1047,12 @@ asc_hip_status prepare_spmm(asc_hip_context* ctx, uint8_t transpose,
     if (panel_overlaps(plan->spmm_panel, cs))
       return fail(ASC_HIP_INVALID_ARGUMENT, "SpMM output overlaps the column-panel analysis");
     m->panel = &plan->spmm_panel;
+  } else if (!ordered && transpose == ASC_HIP_OP_N && a->format == ASC_HIP_CSR &&
+             asc_hip::spmm_plain_applies(*m) && find_stream_plan(ctx, *a, &plan->spmm_panel) &&
+             spmm_panel_pays(plan->spmm_panel) && !panel_overlaps(plan->spmm_panel, cs)) {
+    // AUTO on a matrix whose stream-staged analysis lists long rows only.
+    m->panel = &plan->spmm_panel;
+    plan->spmm_panel_optional = true;
   }
   spmm_plain_plan(m, elem_size(a->element_kind));
   plan->spmv_ws = align256(asc_hip::spmm_workspace_bytes(*m));
@@ -1066,12 +1086,20 @@ asc_hip_status asc_sparse_hip_standard_spmm(asc_hip_context* ctx, uint8_t transp
   asc_hip::SpmmArgs m;
   asc_hip_status st = prepare_spmm(ctx, transpose, a, b, c, opt, &plan, &m);
   if (st.code) return st;
-  // The passes' split table is an optimization: a workspace that only holds
-  // the required part (the transposed operand, the panel form's partial
-  // sums) runs the SpMM in one pass.
+  // The passes' split table is an optimization, and so is the panel form
+  // when AUTO took it: a workspace too small for them runs the plain kernel
+  // over every row, then in one pass.  An explicit ASC_HIP_ALG_PANEL keeps
+  // its partial sums as the required part.
+  const auto needed = [&] {
+    return transpose_prefix(plan) + align256(asc_hip::spmm_workspace_bytes(m));
+  };
+  if ((ws_bytes < plan.total || ws == nullptr) && plan.spmm_panel_optional) {
+    m.panel = nullptr;
+    plan.total = needed();
+  }
   if (ws_bytes < plan.total || ws == nullptr) {
     m.panel_cols = 0;
-    plan.total = transpose_prefix(plan) + align256(asc_hip::spmm_workspace_bytes(m));
+    plan.total = needed();
   }
   if ((st = check_workspace(ctx, ws, ws_bytes, plan.total,
                             operand_spans(*a, {matrix_span(*b), matrix_span(*c)}))).code)

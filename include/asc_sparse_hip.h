@@ -566,7 +566,8 @@ asc_hip_status asc_sparse_hip_triangular_solve(asc_hip_context* context, uint8_t
  * (providers/cuda.h:245-248); `options` (may be NULL) picks the form:
  *  - ASC_HIP_ALG_ORDERED: every C(i,j) is accumulated in stored order
  *    without FMA, so the result equals the serial reference bit for bit;
- *  - ASC_HIP_ALG_AUTO (NULL): the plain form.  For row-major B and C with
+ *  - ASC_HIP_ALG_AUTO (NULL): the plain form (below ASC_HIP_ALG_PANEL for
+ *    when AUTO takes the panel form).  For row-major B and C with
  *    k <= 32, k and both leading dimensions multiples of 4 and B and C
  *    aligned to 4 elements, each C(i,j) sums its terms in registers with
  *    fused multiply-adds and is updated once, within
@@ -585,7 +586,12 @@ asc_hip_status asc_sparse_hip_triangular_solve(asc_hip_context* context, uint8_t
  *    result is within the same bound as the plain form and repeats bit for
  *    bit.  The other rows run the plain kernel and equal its result.  The
  *    SpMM reads the matrix's current indices and values, not the
- *    analysis's copy.
+ *    analysis's copy.  ASC_HIP_ALG_AUTO takes this form by itself when the
+ *    context holds such an analysis whose listed rows have at least 8
+ *    entries per panel (long_row_min >= 8 * panels) and the workspace holds
+ *    the partial sums (a smaller workspace runs the plain kernel over every
+ *    row); an analysis listing every row, the stream form's default, leaves
+ *    AUTO on the plain kernel.
  *    Results repeat bit for bit from run to run in every form.
  * `options->accumulate` must be ASC_HIP_ACC_PLAIN; other algorithms are
  * ASC_HIP_INVALID_ARGUMENT.

@@ -47,9 +47,11 @@ def test_spmm_panel_parity(ctx, rows, cols, zmax, long_min, pcols, k, ld, dtype,
     b = rng.uniform(-1, 1, (cols, ld)).astype(dtype)
     c0 = rng.uniform(-1, 1, (rows, ld)).astype(dtype)
     a = H.clone_compressed(ctx, H.CSR, rows, cols, m.offsets, m.indices, val, KIND[dtype], ib)
+    bbuf = ctx.upload(b.reshape(-1))
+    # The plain form before any analysis (AUTO may take the panel form after).
+    plain = spmm(ctx, a, bbuf, cols, c0, k, ld, -0.75, H.ALG_AUTO, dtype)
     plan = H.csr_spmv_analyze(ctx, a, H.panel_options(panel_columns=pcols, long_row_min=long_min,
                                                       staging=H.STAGING_STREAM))
-    bbuf = ctx.upload(b.reshape(-1))
     print(f"spmm panel rows {rows} cols {cols} long_min {long_min} panel {pcols} k {k} ld {ld} "
           f"{dtype.__name__} ib {ib} nnz {m.nnz}", flush=True)
     got = spmm(ctx, a, bbuf, cols, c0, k, ld, -0.75, H.ALG_PANEL, dtype)
@@ -64,10 +66,53 @@ def test_spmm_panel_parity(ctx, rows, cols, zmax, long_min, pcols, k, ld, dtype,
     assert np.all(err <= bound), float(np.max(err / np.maximum(bound, 1e-300)))
     # Rows below long_row_min ran the plain kernel: the same bits as the
     # plain form without the analysis.
-    plain = spmm(ctx, a, bbuf, cols, c0, k, ld, -0.75, H.ALG_AUTO, dtype)
     short = np.diff(m.offsets) < long_min
     np.testing.assert_array_equal(got[short].view(np.uint8), plain[short].view(np.uint8))
     del plan
+
+
+def _spmm_raw(ctx, a, bbuf, rows, cols, c0, k, alg, ws, ws_bytes):
+    """One SpMM through the C ABI with the caller's workspace pointer and size."""
+    cb = ctx.upload(c0.reshape(-1))
+    bd = H.dense(bbuf, cols, k, k, H.ROW_MAJOR, H.F64, writable=False)
+    cd = H.dense(cb, rows, k, k, H.ROW_MAJOR, H.F64)
+    H.check(ctx.lib.asc_sparse_hip_standard_spmm(ctx.handle, H.OP_N, H.Scalar(1.25, 0.0),
+                                                 H.C.byref(a.desc), H.C.byref(bd), H.C.byref(cd),
+                                                 ws, ws_bytes, H.C.byref(H.options(alg))))
+    return ctx.download(cb, np.float64, c0.size).reshape(c0.shape)
+
+
+def test_spmm_auto_takes_panel_form_when_rows_are_long(ctx):
+    # AUTO runs the panel form on a stream-staged analysis whose listed rows
+    # hold at least 8 entries per panel (here 1024 >= 8 * 8 panels): the
+    # same bits as ALG_PANEL.  A workspace too small for its partial sums
+    # runs the plain form instead, and so does an analysis listing every row.
+    rows = cols = 1 << 15
+    m = G.make(G.ZIPF, rows, cols, seed=17, values=G.VALUES_NORMAL, zipf_max=10_000)
+    a = H.clone_compressed(ctx, H.CSR, rows, cols, m.offsets, m.indices, m.values, H.F64, 4)
+    rng = np.random.default_rng(23)
+    k = 32
+    b = rng.uniform(-1, 1, (cols, k))
+    c0 = rng.uniform(-1, 1, (rows, k))
+    bbuf = ctx.upload(b.reshape(-1))
+    plain = spmm(ctx, a, bbuf, cols, c0, k, k, 1.25, H.ALG_AUTO, np.float64)
+    plan = H.csr_spmv_analyze(ctx, a, H.panel_options(long_row_min=1024, staging=H.STAGING_STREAM))
+    panel = spmm(ctx, a, bbuf, cols, c0, k, k, 1.25, H.ALG_PANEL, np.float64)
+    auto = spmm(ctx, a, bbuf, cols, c0, k, k, 1.25, H.ALG_AUTO, np.float64)
+    np.testing.assert_array_equal(auto.view(np.uint64), panel.view(np.uint64))
+    assert not np.array_equal(auto.view(np.uint64), plain.view(np.uint64))  # long rows differ
+    small = _spmm_raw(ctx, a, bbuf, rows, cols, c0, k, H.ALG_AUTO, None, 0)
+    np.testing.assert_array_equal(small.view(np.uint64), plain.view(np.uint64))
+    # An explicit ALG_PANEL needs its partial sums' workspace.
+    with pytest.raises(H.AscHipError) as e:
+        _spmm_raw(ctx, a, bbuf, rows, cols, c0, k, H.ALG_PANEL, None, 0)
+    assert e.value.code == H.INVALID_ARGUMENT
+    del plan
+    plan = H.csr_spmv_analyze(ctx, a, H.panel_options(long_row_min=1, staging=H.STAGING_STREAM))
+    every = spmm(ctx, a, bbuf, cols, c0, k, k, 1.25, H.ALG_AUTO, np.float64)
+    np.testing.assert_array_equal(every.view(np.uint64), plain.view(np.uint64))
+    del plan
+    H.plan_clear(ctx)
 
 
 def test_spmm_panel_nothing_listed_equals_plain(ctx):
