@@ -534,59 +534,119 @@ __global__ __launch_bounds__(kBlock) void finalize_bounds_kernel(
   }
 }
 
-__global__ __launch_bounds__(kBlock) void finalize_head_kernel(
-    const uint32_t* __restrict__ rows_sorted, const int64_t* __restrict__ cols_sorted,
-    int64_t n, uint32_t* __restrict__ head, uint32_t* __restrict__ duplicate) {
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
-  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) {
-    const bool h = i == 0 || rows_sorted[i] != rows_sorted[i - 1] ||
-                   cols_sorted[i] != cols_sorted[i - 1];
-    head[i] = h ? 1u : 0u;
-    if (!h) *duplicate = 1u;
-  }
+// Compaction of the sorted triplets, in two passes over them: a count of
+// the surviving runs whose head lies in each tile of kFinTile entries, then
+// (after a scan of the tile counts) each tile's runs written at their place.
+// Both passes find heads and sum runs the same way, so they agree.  Thread t
+// takes entries t, t + kBlock, ... of its tile (kFinPer rounds), so loads
+// and stores are coalesced; positions inside a tile follow the rounds' ballots.
+constexpr int kFinPer = 8;
+constexpr int64_t kFinTile = int64_t(kBlock) * kFinPer;
+
+// The run starting at head i, summed left to right in insertion order (no
+// FMA), and whether it survives the zero policy (exact zeros, -0.0
+// included, are dropped; NaN is kept).
+template <typename T>
+__device__ __forceinline__ bool finalize_run(const uint32_t* __restrict__ rows,
+                                             const uint32_t* __restrict__ cols,
+                                             const T* __restrict__ vals, int64_t n, int64_t i,
+                                             uint32_t r, uint32_t c, int drop_zeros, T* sum) {
+#pragma clang fp contract(off)
+  T v = vals[i];
+  for (int64_t j = i + 1; j < n && rows[j] == r && cols[j] == c; ++j) v = v + vals[j];
+  *sum = v;
+  return !(drop_zeros && v == T(0));
 }
 
-// One thread per run head: the run's sum in insertion order (no FMA), and
-// whether it survives the zero policy (exact zeros, -0.0 included, are
-// dropped; NaN is kept).
+// Entry i's coordinates, and whether it heads a run (differs from i - 1).
+__device__ __forceinline__ bool finalize_head(const uint32_t* __restrict__ rows,
+                                              const uint32_t* __restrict__ cols, int64_t i,
+                                              uint32_t* r, uint32_t* c) {
+  *r = rows[i];
+  *c = cols[i];
+  return i == 0 || *r != rows[i - 1] || *c != cols[i - 1];
+}
+
+// Pass 1: per tile, the surviving runs that start in it; any entry that is
+// not a head marks a duplicate coordinate.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void finalize_sum_kernel(
-    const uint32_t* __restrict__ head, const T* __restrict__ vals, int64_t n, int drop_zeros,
-    T* __restrict__ sums, uint32_t* __restrict__ keep) {
-#pragma clang fp contract(off)
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
-  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) {
-    if (!head[i]) {
-      keep[i] = 0u;
+__global__ __launch_bounds__(kBlock) void finalize_count_kernel(
+    const uint32_t* __restrict__ rows, const uint32_t* __restrict__ cols,
+    const T* __restrict__ vals, int64_t n, int drop_zeros, uint32_t* __restrict__ counts,
+    uint32_t* __restrict__ duplicate) {
+  __shared__ uint32_t s_tmp[kWaves];
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kFinTile + threadIdx.x;
+  uint32_t kept = 0;
+  bool dup = false;
+#pragma unroll
+  for (int u = 0; u < kFinPer; ++u) {
+    const int64_t i = base + int64_t(u) * kBlock;
+    if (i >= n) break;
+    uint32_t r, c;
+    if (!finalize_head(rows, cols, i, &r, &c)) {
+      dup = true;
       continue;
     }
-    T v = vals[i];
-    for (int64_t j = i + 1; j < n && !head[j]; ++j) v = v + vals[j];
-    sums[i] = v;
-    keep[i] = (drop_zeros && v == T(0)) ? 0u : 1u;
+    T v;
+    if (finalize_run(rows, cols, vals, n, i, r, c, drop_zeros, &v)) ++kept;
   }
+  if (dup) *duplicate = 1u;  // any writer: same value
+  uint32_t total;
+  block_exclusive_scan(kept, s_tmp, &total);
+  if (threadIdx.x == 0) counts[blockIdx.x] = total;
 }
 
+// Pass 2: each surviving run written at its tile's scanned offset plus the
+// runs before it in the tile (round by round, wave by wave, lane by lane:
+// entry order); the last tile records the total.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void finalize_write_kernel(
-    const uint32_t* __restrict__ keep, const uint32_t* __restrict__ pos,
-    const uint32_t* __restrict__ rows_sorted, const int64_t* __restrict__ cols_sorted,
-    const T* __restrict__ sums, int64_t n, int64_t* __restrict__ out_coords,
-    T* __restrict__ out_vals, uint32_t* __restrict__ count) {
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
-  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) {
-    if (keep[i]) {
-      const int64_t d = pos[i];
-      out_coords[2 * d] = rows_sorted[i];
-      out_coords[2 * d + 1] = cols_sorted[i];
-      out_vals[d] = sums[i];
-    }
-    if (i == n - 1) *count = pos[i] + keep[i];
+__global__ __launch_bounds__(kBlock) void finalize_compact_kernel(
+    const uint32_t* __restrict__ rows, const uint32_t* __restrict__ cols,
+    const T* __restrict__ vals, int64_t n, int drop_zeros, const uint32_t* __restrict__ offsets,
+    int64_t* __restrict__ out_coords, T* __restrict__ out_vals, uint32_t* __restrict__ count) {
+  __shared__ uint32_t s_cnt[kFinPer][kWaves];
+  const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (kWave - lane));
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kFinTile + threadIdx.x;
+  T sum[kFinPer];
+  uint32_t rr[kFinPer], cc[kFinPer], rank[kFinPer];
+  uint32_t keep = 0;
+#pragma unroll
+  for (int u = 0; u < kFinPer; ++u) {
+    const int64_t i = base + int64_t(u) * kBlock;
+    sum[u] = T(0);
+    rr[u] = cc[u] = 0u;
+    bool k = false;
+    if (i < n && finalize_head(rows, cols, i, &rr[u], &cc[u]))
+      k = finalize_run(rows, cols, vals, n, i, rr[u], cc[u], drop_zeros, &sum[u]);
+    const uint64_t m = __ballot(k);
+    rank[u] = static_cast<uint32_t>(__popcll(m & below));
+    if (lane == 0) s_cnt[u][wave] = static_cast<uint32_t>(__popcll(m));
+    if (k) keep |= 1u << u;
   }
+  __syncthreads();
+  int64_t d = offsets[blockIdx.x];
+#pragma unroll
+  for (int u = 0; u < kFinPer; ++u) {
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+      before += w < wave ? s_cnt[u][w] : 0u;
+      all += s_cnt[u][w];
+    }
+    if (keep & (1u << u)) {
+      const int64_t p = d + before + rank[u];
+      out_coords[2 * p] = rr[u];
+      out_coords[2 * p + 1] = cc[u];
+      out_vals[p] = sum[u];
+    }
+    d += all;
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *count = static_cast<uint32_t>(d);
 }
 
 struct FinalizeLayout {
-  size_t sort_a, sort_b, rows_a, vals_a, cols_b, vals_b, head, keep, pos, sums, scan, flags, total;
+  size_t sort_a, sort_b, rows_a, vals_a, cols_b, vals_b, tcount, toff, scan, flags, total;
 };
 
 FinalizeLayout finalize_layout(int64_t n, int64_t rows, int64_t cols, size_t vbytes) {
@@ -600,15 +660,15 @@ FinalizeLayout finalize_layout(int64_t n, int64_t rows, int64_t cols, size_t vby
   };
   L.sort_a = take(sort_layout(n, cols, vbytes).total);
   L.sort_b = take(sort_layout(n, rows, vbytes).total);
-  L.rows_a = take(8 * un);
+  L.rows_a = take(4 * un);
   L.vals_a = take(vbytes * un);
-  L.cols_b = take(8 * un);
+  L.cols_b = take(4 * un);
   L.vals_b = take(vbytes * un);
-  L.head = take(4 * un);
-  L.keep = take(4 * un);
-  L.pos = take(4 * un);
-  L.sums = take(vbytes * un);
-  L.scan = take(scan_workspace_bytes(n));
+  const int64_t tiles = (n + kFinTile - 1) / kFinTile;
+  const size_t ut = static_cast<size_t>(tiles > 0 ? tiles : 1);
+  L.tcount = take(4 * ut);
+  L.toff = take(4 * ut);
+  L.scan = take(scan_workspace_bytes(tiles));
   L.flags = take(16);
   L.total = off;
   return L;
@@ -636,26 +696,28 @@ hipError_t finalize_typed(const int64_t* coords, const T* vals, int64_t rows, in
     *out_failure = 1;  // coordinate out of range
     return hipSuccess;
   }
-  // Sort by column, then (stably) by row.
+  // Sort by column, then (stably) by row.  Rows and columns are below 2^32,
+  // so the sorts carry them as 32-bit words.
   char* wa = ws + L.sort_a;
   char* wb = ws + L.sort_b;
-  int64_t* rows_a = reinterpret_cast<int64_t*>(ws + L.rows_a);
+  uint32_t* rows_a = reinterpret_cast<uint32_t*>(ws + L.rows_a);
   VB* vals_a = reinterpret_cast<VB*>(ws + L.vals_a);
-  int64_t* cols_b = reinterpret_cast<int64_t*>(ws + L.cols_b);
+  uint32_t* cols_b = reinterpret_cast<uint32_t*>(ws + L.cols_b);
   VB* vals_b = reinterpret_cast<VB*>(ws + L.vals_b);
-  if ((e = stable_sort<int64_t, int64_t, VB>(coords + 1, 2, coords, 2, vals, n, cols, rows_a, 8,
+  if ((e = stable_sort<int64_t, int64_t, VB>(coords + 1, 2, coords, 2, vals, n, cols, rows_a, 4,
                                              vals_a, nullptr, wa, s)) != hipSuccess)
     return e;
   const uint32_t* cols_a = sorted_keys_in(wa, n, cols, sizeof(VB));
-  if ((e = stable_sort<int64_t, uint32_t, VB>(rows_a, 1, cols_a, 1, vals_a, n, rows, cols_b, 8,
-                                              vals_b, nullptr, wb, s)) != hipSuccess)
+  if ((e = stable_sort<uint32_t, uint32_t, VB>(rows_a, 1, cols_a, 1, vals_a, n, rows, cols_b, 4,
+                                               vals_b, nullptr, wb, s)) != hipSuccess)
     return e;
   const uint32_t* rows_b = sorted_keys_in(wb, n, rows, sizeof(VB));
-  uint32_t* head = reinterpret_cast<uint32_t*>(ws + L.head);
-  uint32_t* keep = reinterpret_cast<uint32_t*>(ws + L.keep);
-  uint32_t* pos = reinterpret_cast<uint32_t*>(ws + L.pos);
-  T* sums = reinterpret_cast<T*>(ws + L.sums);
-  finalize_head_kernel<<<g, kBlock, 0, s>>>(rows_b, cols_b, n, head, flags + 1);
+  const unsigned tiles = blocks_for(n, kFinTile);
+  uint32_t* tcount = reinterpret_cast<uint32_t*>(ws + L.tcount);
+  uint32_t* toff = reinterpret_cast<uint32_t*>(ws + L.toff);
+  const int drop = drop_zeros ? 1 : 0;
+  finalize_count_kernel<T><<<tiles, kBlock, 0, s>>>(rows_b, cols_b, vals_b, n, drop, tcount,
+                                                    flags + 1);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if (!sum_duplicates) {
     if ((e = hipMemcpyAsync(host_flags + 1, flags + 1, 4, hipMemcpyDeviceToHost, s)) != hipSuccess)
@@ -666,12 +728,11 @@ hipError_t finalize_typed(const int64_t* coords, const T* vals, int64_t rows, in
       return hipSuccess;
     }
   }
-  finalize_sum_kernel<T><<<g, kBlock, 0, s>>>(head, vals_b, n, drop_zeros ? 1 : 0, sums, keep);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
-  if ((e = scan_exclusive(keep, pos, n, reinterpret_cast<uint32_t*>(ws + L.scan), s)) != hipSuccess)
+  if ((e = scan_exclusive(tcount, toff, tiles, reinterpret_cast<uint32_t*>(ws + L.scan), s)) !=
+      hipSuccess)
     return e;
-  finalize_write_kernel<T><<<g, kBlock, 0, s>>>(keep, pos, rows_b, cols_b, sums, n, out_coords,
-                                                out_vals, flags + 2);
+  finalize_compact_kernel<T><<<tiles, kBlock, 0, s>>>(rows_b, cols_b, vals_b, n, drop, toff,
+                                                      out_coords, out_vals, flags + 2);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if ((e = hipMemcpyAsync(host_flags + 2, flags + 2, 4, hipMemcpyDeviceToHost, s)) != hipSuccess)
     return e;

@@ -902,6 +902,12 @@ def bench_section(ctx, args):
         bench_precision(ctx, extra)
     elif args.only == "batched":
         bench_batched(ctx, extra, args.batch_count)
+    elif args.only == "finalize":
+        t0 = time.perf_counter()
+        m = G.zipf_large(args.rows)
+        log(f"generated n={m.rows} nnz={m.nnz} in {time.perf_counter() - t0:.1f}s")
+        bench_shuffled_finalize(ctx, m, extra, guard)
+        bench_finalize(ctx, extra)
     else:
         t0 = time.perf_counter()
         m = G.zipf_large(args.rows)
@@ -938,7 +944,7 @@ def main():
     ap.add_argument("--sweep", action="store_true", help="time every SpMV variant")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run one 1 GiB device copy (counter calibration)")
-    ap.add_argument("--only", choices=("spmm", "convert", "precision", "batched"),
+    ap.add_argument("--only", choices=("spmm", "convert", "finalize", "precision", "batched"),
                     help="profiling aid: run one section (the large-matrix ones with one timed "
                          "call per form) and print its own JSON line instead of the benchmark line")
     args = ap.parse_args()
