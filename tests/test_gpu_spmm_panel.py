@@ -158,3 +158,30 @@ def test_spmm_panel_refusals(ctx):
                         cd, alg=H.ALG_PANEL)
     assert e.value.code == H.UNSUPPORTED
     del plan
+
+
+def test_spmm_panel_b_beyond_32bit_offsets(ctx):
+    # 2^24 columns of a 32-column FP64 B (4 GiB: 1024 panels of 16384
+    # columns, the stream analysis's most), so the segment kernel takes its
+    # 64-bit B offsets.  Rows of 2000-3000 entries spread over every panel; a
+    # few short rows take the plain kernel.
+    rows, cols, k = 512, 1 << 24, 32
+    rng = np.random.default_rng(91)
+    lens = np.where(np.arange(rows) % 8 == 0, rng.integers(1, 40, rows), rng.integers(2000, 3000, rows))
+    offsets = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    indices = np.concatenate([np.sort(rng.choice(cols, int(n), replace=False)) for n in lens])
+    values = rng.standard_normal(int(offsets[-1]))
+    a = H.clone_compressed(ctx, H.CSR, rows, cols, offsets, indices, values, H.F64, 4)
+    b = rng.uniform(-1, 1, (cols, k))
+    c0 = rng.uniform(-1, 1, (rows, k))
+    print(f"spmm panel, B of {b.nbytes / 1e9:.1f} GB, nnz {offsets[-1]}", flush=True)
+    bbuf = ctx.upload(b.reshape(-1))
+    plan = H.csr_spmv_analyze(ctx, a, H.panel_options(long_row_min=1000, staging=H.STAGING_STREAM))
+    got = spmm(ctx, a, bbuf, cols, c0, k, k, 0.5, H.ALG_PANEL, np.float64)
+    ref = O.std_spmm(0, 0, rows, cols, offsets, indices.astype(np.int64), values, 0.5,
+                     b.reshape(-1).copy(), 1, k, k, c0.reshape(-1).copy(), 1, k).reshape(rows, k)
+    bound = plain_spmm_bound(H.OP_N, rows, cols, offsets, indices, values, 0.5, b, c0, EPS[np.float64])
+    err = np.abs(got - ref)
+    assert np.all(err <= bound), float(np.max(err / np.maximum(bound, 1e-300)))
+    del plan, bbuf
+    H.plan_clear(ctx)
