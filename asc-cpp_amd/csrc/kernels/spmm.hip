@@ -441,15 +441,15 @@ __global__ __launch_bounds__(kBlock) void spmm_plain_kernel(MParams p) {
 // The plan cuts op(A)'s columns into panels; a listed row's entries in panel
 // q form one segment, contiguous in the CSR arrays (seg_src, seg_len), and
 // segments are numbered panel by panel (ctable gives each panel's range).
-// Per slice of G (16 or 32) columns of B and C:
-//   spmm_seg_kernel: a G-lane group takes one segment at a time, lane j
-//     gathering column j0+j of each entry's B row (a whole 256-byte row for
-//     k = 32, so a panel's slice of B is contiguous), FMA accumulation in
-//     stored order, one partial row per segment.  Work-group w runs on XCD
-//     w mod 8 and takes a chunk of panel q (q mod 8 = w mod 8, panels in
-//     turn), so an XCD's work-groups gather from one panel's B slice at a
-//     time (8192 rows x 32 FP64 columns = 2 MiB with the plan's panel width
-//     of 8192), which its L2 keeps;
+// Per slice of W (16 or 32) columns of B and C:
+//   spmm_seg_kernel: a 16-lane group takes one segment at a time, lane j
+//     gathering columns j0 + (W/16)*j .. of each entry's B row (a whole
+//     256-byte row for k = 32, so a panel's slice of B is contiguous), FMA
+//     accumulation in stored order, one partial row per segment.  Work-group
+//     w runs on XCD w mod 8 and takes a chunk of panel q (q mod 8 = w mod 8,
+//     panels in turn), so an XCD's work-groups gather from one panel's B
+//     slice at a time (16384 rows x 32 FP64 columns = 4 MiB at the stream
+//     analysis's widest FP64 panel), which its L2 keeps;
 //   spmm_panel_combine_kernel: per group of kStreamGroupRows listed rows,
 //     16 of the slice's columns and part of the panels, the panels' partial
 //     rows added in panel order into LDS accumulators (a row has at most one
