@@ -185,3 +185,22 @@ def test_spmm_panel_b_beyond_32bit_offsets(ctx):
     assert np.all(err <= bound), float(np.max(err / np.maximum(bound, 1e-300)))
     del plan, bbuf
     H.plan_clear(ctx)
+
+
+def test_spmm_panel_alpha_zero_leaves_c(ctx):
+    # Standard SpMM with alpha == 0 is a no-op (blas.h:1730-1732): C keeps its
+    # bits, NaN values in A included, in the panel form and under AUTO.
+    m = G.make(G.ZIPF, 4096, 4096, seed=8, values=G.VALUES_NORMAL, zipf_max=4000)
+    vals = m.values.copy()
+    vals[::7] = np.nan
+    a = H.clone_compressed(ctx, H.CSR, 4096, 4096, m.offsets, m.indices, vals, H.F64, 4)
+    rng = np.random.default_rng(3)
+    b = rng.uniform(-1, 1, (4096, 32))
+    c0 = rng.uniform(-1, 1, (4096, 32))
+    bbuf = ctx.upload(b.reshape(-1))
+    plan = H.csr_spmv_analyze(ctx, a, H.panel_options(long_row_min=1024, staging=H.STAGING_STREAM))
+    for alg in (H.ALG_PANEL, H.ALG_AUTO):
+        got = spmm(ctx, a, bbuf, 4096, c0, 32, 32, 0.0, alg, np.float64)
+        np.testing.assert_array_equal(got.view(np.uint64), c0.view(np.uint64))
+    del plan
+    H.plan_clear(ctx)
