@@ -101,13 +101,15 @@ uint8_t resolve_alg(asc_hip_context* ctx, const asc_hip_sparse_desc& a,
 // over the matrix per slice, so that the slice stays in the 256 MiB
 // memory-side cache while its rows are gathered.  Measured on the 640M-entry
 // Zipf matrix with a 1 GiB B (k = 32, FP64): one pass 27.2 ms, 512/256/128
-// MiB slices 25.1/24.5/24.8 ms.  The ordered kernels run in one pass (slices
-// of 256/128/64 MiB made them 4%/11%/30% slower).
+// MiB slices 25.1/24.5/24.8 ms.  The ordered form on the same operands runs
+// row by row with 16 lanes a row and takes the same passes (one pass
+// 36.6 ms, 256 MiB slices 31.2 ms); the ordered kernels for other operands
+// run in one pass (slices of 256/128/64 MiB made them 4%/11%/30% slower).
 constexpr int64_t kSpmmPassBytes = int64_t(256) << 20;
 
 void spmm_plain_plan(asc_hip::SpmmArgs* m, size_t esz) {
   m->panel_cols = 0;
-  if (m->ordered || !asc_hip::spmm_plain_applies(*m) || m->k <= 0 || m->a.inner <= 0) return;
+  if (!asc_hip::spmm_plain_applies(*m) || m->k <= 0 || m->a.inner <= 0) return;
   m->panel_cols = std::max<int64_t>(1, kSpmmPassBytes / (m->k * static_cast<int64_t>(esz)));
 }
 
@@ -1047,12 +1049,15 @@ asc_hip_status prepare_spmm(asc_hip_context* ctx, uint8_t transpose,
     if (panel_overlaps(plan->spmm_panel, cs))
       return fail(ASC_HIP_INVALID_ARGUMENT, "SpMM output overlaps the column-panel analysis");
     m->panel = &plan->spmm_panel;
-  } else if (!ordered && transpose == ASC_HIP_OP_N && a->format == ASC_HIP_CSR &&
+  } else if (transpose == ASC_HIP_OP_N && a->format == ASC_HIP_CSR &&
              asc_hip::spmm_plain_applies(*m) && find_stream_plan(ctx, *a, &plan->spmm_panel) &&
              spmm_panel_pays(plan->spmm_panel) && !panel_overlaps(plan->spmm_panel, cs)) {
-    // AUTO on a matrix whose stream-staged analysis lists long rows only.
+    // AUTO or ORDERED on a matrix whose stream-staged analysis lists long
+    // rows only.  The ordered form walks the panels in order and needs no
+    // workspace (the same bits as without the analysis); AUTO's panel form
+    // keeps partial sums in the workspace and is dropped when it is short.
     m->panel = &plan->spmm_panel;
-    plan->spmm_panel_optional = true;
+    plan->spmm_panel_optional = !ordered;
   }
   spmm_plain_plan(m, elem_size(a->element_kind));
   plan->spmv_ws = align256(asc_hip::spmm_workspace_bytes(*m));

@@ -40,6 +40,7 @@ struct MParams {
   int64_t pass;
   // Rows of at least this many entries are left to the panel path.
   int64_t skip_min;
+  int64_t inner;  // columns of op(A) (rows of B)
 };
 
 // The row's entries in this pass; with one pass, the whole row.
@@ -214,6 +215,8 @@ __global__ __launch_bounds__(kBlock) void spmm_wide_kernel(MParams p) {
   const int64_t blk = xcd_remap(blockIdx.x, gridDim.x);
   const int64_t row = blk * kRowsPerBlock + w;
   if (row >= p.rows) return;  // whole wave
+  // Rows a column-panel plan lists run segment by segment elsewhere.
+  if (p.offsets[row + 1] - p.offsets[row] >= p.skip_min) return;
   const int lane = threadIdx.x % kWave;
   const int g = lane >> 3;
   const int64_t jq = 4 * (lane & 7);
@@ -603,6 +606,147 @@ __global__ __launch_bounds__(kSegBlock) void spmm_seg_kernel(
   }
 }
 
+// Ordered form over a column-panel plan: one launch per panel, in panel
+// order.  A 16-lane group takes a segment of the panel, loads its row's C
+// columns, and continues each C(r, j)'s serial chain C = C + (alpha*v)*B(c, j)
+// (no FMA) over the segment's entries in stored order; a row's segments run
+// in panel order, which is its stored order, so every chain is the serial
+// reference's, bit for bit.  The segment's row: its group by binary search
+// over the panel's combine table, then its tag.
+template <typename T, int CPL>
+__device__ __forceinline__ void ordered_add(ColPack<T, CPL>& acc, T av, const ColPack<T, CPL>& b) {
+#pragma clang fp contract(off)
+  acc.x = acc.x + av * b.x;
+  if constexpr (CPL == 2) acc.y = acc.y + av * b.y;
+}
+
+// The step's first n entries added in stored order (every lane takes part in
+// each broadcast; the adds stop at the segment's end).
+template <typename T, int CPL, int U>
+__device__ __forceinline__ void seg_ordered_accumulate(T av, const ColPack<T, CPL> (&bv)[kSegStep],
+                                                       ColPack<T, CPL>& acc, int64_t n) {
+  if constexpr (U < kSegStep) {
+    const T avu = row_bcast<U>(av);
+    if (U < n) ordered_add<T, CPL>(acc, avu, bv[U]);
+    seg_ordered_accumulate<T, CPL, U + 1>(av, bv, acc, n);
+  }
+}
+
+template <typename T, typename I, int CPL, bool SMALL>
+__global__ __launch_bounds__(kSegBlock) void spmm_seg_ordered_kernel(
+    const I* __restrict__ col, const T* __restrict__ val, const T* __restrict__ b, int64_t ldb,
+    T* __restrict__ c, int64_t ldc, int64_t j0, int64_t kk, double alpha,
+    const uint32_t* __restrict__ seg_src, const uint32_t* __restrict__ seg_len,
+    const uint16_t* __restrict__ seg_lrow, const uint32_t* __restrict__ ctable,
+    const uint32_t* __restrict__ long_rows, int64_t groups, int64_t q) {
+#pragma clang fp contract(off)
+  constexpr int G = kSegLanes;
+  const uint32_t* __restrict__ ct = ctable + q * (groups + 1);
+  const uint64_t o0 = ct[0], o1 = ct[groups];
+  const int j = threadIdx.x % G;
+  const bool on = CPL * j < kk;  // kk is a multiple of CPL
+  const T a = static_cast<T>(alpha);
+  const char* __restrict__ bj = reinterpret_cast<const char*>(b + j0 + CPL * j);
+  const uint32_t row_bytes = static_cast<uint32_t>(ldb * static_cast<int64_t>(sizeof(T)));
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * (kSegBlock / G);
+  for (uint64_t o = o0 + static_cast<uint64_t>(blockIdx.x) * (kSegBlock / G) + threadIdx.x / G;
+       o < o1; o += stride) {
+    int64_t lo = 0, hi = groups;  // the last group whose first ordinal is at most o
+    while (hi - lo > 1) {
+      const int64_t mid = (lo + hi) / 2;
+      if (ct[mid] <= o) lo = mid;
+      else hi = mid;
+    }
+    const int64_t row = long_rows[lo * kStreamGroupRows + seg_lrow[o]];
+    T* cr = c + row * ldc + j0 + CPL * j;
+    ColPack<T, CPL> acc{};
+    if (on) {
+      acc.x = cr[0];
+      if constexpr (CPL == 2) acc.y = cr[1];
+    }
+    const int64_t src = seg_src[o];
+    const int64_t len = seg_len[o];
+    int cc = -1;
+    T av = T(0);
+    if (j < len) {
+      cc = static_cast<int>(ld_stream(col + src + j));
+      av = a * ld_stream(val + src + j);
+    }
+    for (int64_t t = 0; t < len; t += kSegStep) {
+      ColPack<T, CPL> bv[kSegStep];
+      seg_gather<T, CPL, SMALL, 0>(bj, row_bytes, cc, on, bv);
+      const T avcur = av;
+      const int64_t tn = t + kSegStep + j;
+      cc = -1;
+      av = T(0);
+      if (tn < len) {
+        cc = static_cast<int>(ld_stream(col + src + tn));
+        av = a * ld_stream(val + src + tn);
+      }
+      seg_ordered_accumulate<T, CPL, 0>(avcur, bv, acc, len - t);
+    }
+    if (on) {
+      cr[0] = acc.x;
+      if constexpr (CPL == 2) cr[1] = acc.y;
+    }
+  }
+}
+
+// Ordered form, row by row: a 16-lane group per row (of the pass's entries
+// when B is gathered in column passes), the same steps and serial chains as
+// the segment kernel above.  Rows of at least skip_min entries are left to a
+// column-panel plan.
+template <typename T, typename I, int CPL, bool SMALL>
+__global__ __launch_bounds__(kSegBlock) void spmm_rows_ordered_kernel(MParams p, int64_t j0,
+                                                                      int64_t kk) {
+#pragma clang fp contract(off)
+  constexpr int G = kSegLanes;
+  const int64_t row = (xcd_remap(blockIdx.x, gridDim.x) * kSegBlock + threadIdx.x) / G;
+  if (row >= p.rows) return;  // whole group
+  if (p.offsets[row + 1] - p.offsets[row] >= p.skip_min) return;
+  int64_t begin, end;
+  pass_range(p, row, &begin, &end);
+  if (begin == end) return;
+  const int j = threadIdx.x % G;
+  const bool on = CPL * j < kk;  // kk is a multiple of CPL
+  const T a = static_cast<T>(p.alpha);
+  const I* __restrict__ col = static_cast<const I*>(p.indices);
+  const T* __restrict__ val = static_cast<const T*>(p.values);
+  const char* __restrict__ bj =
+      reinterpret_cast<const char*>(static_cast<const T*>(p.b) + j0 + CPL * j);
+  const uint32_t row_bytes = static_cast<uint32_t>(p.ldb * static_cast<int64_t>(sizeof(T)));
+  T* cr = static_cast<T*>(p.c) + row * p.ldc + j0 + CPL * j;
+  ColPack<T, CPL> acc{};
+  if (on) {
+    acc.x = cr[0];
+    if constexpr (CPL == 2) acc.y = cr[1];
+  }
+  const int64_t len = end - begin;
+  int cc = -1;
+  T av = T(0);
+  if (j < len) {
+    cc = static_cast<int>(ld_stream(col + begin + j));
+    av = a * ld_stream(val + begin + j);
+  }
+  for (int64_t t = 0; t < len; t += kSegStep) {
+    ColPack<T, CPL> bv[kSegStep];
+    seg_gather<T, CPL, SMALL, 0>(bj, row_bytes, cc, on, bv);
+    const T avcur = av;
+    const int64_t tn = t + kSegStep + j;
+    cc = -1;
+    av = T(0);
+    if (tn < len) {
+      cc = static_cast<int>(ld_stream(col + begin + tn));
+      av = a * ld_stream(val + begin + tn);
+    }
+    seg_ordered_accumulate<T, CPL, 0>(avcur, bv, acc, len - t);
+  }
+  if (on) {
+    cr[0] = acc.x;
+    if constexpr (CPL == 2) cr[1] = acc.y;
+  }
+}
+
 // Columns a combine work-group accumulates (kStreamGroupRows x 16 FP64 =
 // 128 KiB of LDS): a 32-column partial row is combined in two halves.
 constexpr int kCombineCols = 16;
@@ -711,6 +855,65 @@ hipError_t launch_panel_rows(const SpmmArgs& a, const PanelPlan& pp, char* ws, h
   return hipSuccess;
 }
 
+// Work-groups of one ordered panel launch (each 16-lane group walks the
+// panel's segments at this stride).
+constexpr unsigned kSegOrderedGrid = 1024;
+
+// The ordered form's listed rows: one launch per panel, in panel order (a
+// row's later segments continue its chains), no workspace.
+template <typename T, typename I>
+hipError_t launch_panel_ordered(const SpmmArgs& a, const PanelPlan& pp, hipStream_t s) {
+  const int64_t width = seg_cols(a.k);
+  const int64_t row_bytes = a.ldb * static_cast<int64_t>(sizeof(T));
+  const bool small = a.a.inner < (int64_t(1) << 24) && row_bytes < (int64_t(1) << 24) &&
+                     a.a.inner * row_bytes < (int64_t(1) << 32);
+  for (int64_t j0 = 0; j0 < a.k; j0 += width) {
+    const int64_t kk = a.k - j0 < width ? a.k - j0 : width;
+    for (int64_t q = 0; q < pp.panels; ++q) {
+#define ASC_OSEG(CPL, SMALL)                                                                   \
+  spmm_seg_ordered_kernel<T, I, CPL, SMALL><<<kSegOrderedGrid, kSegBlock, 0, s>>>(             \
+      static_cast<const I*>(a.a.indices), static_cast<const T*>(a.a.values),                  \
+      static_cast<const T*>(a.b), a.ldb, static_cast<T*>(a.c), a.ldc, j0, kk, a.alpha,        \
+      pp.seg_src, pp.seg_len, pp.seg_lrow, pp.ctable, pp.long_rows, pp.groups, q)
+      if (width == 32) {
+        if (small) ASC_OSEG(2, true); else ASC_OSEG(2, false);
+      } else {
+        if (small) ASC_OSEG(1, true); else ASC_OSEG(1, false);
+      }
+#undef ASC_OSEG
+      const hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return e;
+    }
+  }
+  return hipSuccess;
+}
+
+// The row-by-row ordered kernel (plain-form operands: row-major B and C,
+// k <= 32 in one slice of 16 or 32 columns).
+template <typename T, typename I>
+hipError_t launch_rows_ordered(const MParams& p, hipStream_t s) {
+  const int64_t width = p.k > 16 ? 32 : 16;
+  const int64_t row_bytes = p.ldb * static_cast<int64_t>(sizeof(T));
+  const int64_t inner = p.inner;
+  const bool small = inner < (int64_t(1) << 24) && row_bytes < (int64_t(1) << 24) &&
+                     inner * row_bytes < (int64_t(1) << 32);
+  const unsigned grid = static_cast<unsigned>((p.rows + kSegBlock / kSegLanes - 1) /
+                                              (kSegBlock / kSegLanes));
+  for (int64_t j0 = 0; j0 < p.k; j0 += width) {
+    const int64_t kk = p.k - j0 < width ? p.k - j0 : width;
+    if (width == 32) {
+      if (small) spmm_rows_ordered_kernel<T, I, 2, true><<<grid, kSegBlock, 0, s>>>(p, j0, kk);
+      else spmm_rows_ordered_kernel<T, I, 2, false><<<grid, kSegBlock, 0, s>>>(p, j0, kk);
+    } else {
+      if (small) spmm_rows_ordered_kernel<T, I, 1, true><<<grid, kSegBlock, 0, s>>>(p, j0, kk);
+      else spmm_rows_ordered_kernel<T, I, 1, false><<<grid, kSegBlock, 0, s>>>(p, j0, kk);
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 template <typename T, typename I>
 hipError_t launch_plain(const MParams& p, hipStream_t s) {
   const unsigned grid = static_cast<unsigned>((p.rows + kBlock - 1) / kBlock);
@@ -790,7 +993,7 @@ size_t split_bytes(const SpmmArgs& a) {
 // Workspace of the panel path: one partial row per segment, then the
 // combine's per-part row sums (parts x listed rows).
 size_t panel_bytes(const SpmmArgs& a) {
-  if (a.panel == nullptr) return 0;
+  if (a.panel == nullptr || a.ordered) return 0;  // the ordered form needs none
   const size_t esz = a.a.elem == Elem::F64 ? 8 : 4;
   const int64_t width = seg_cols(a.k);
   const size_t nparts = static_cast<size_t>(combine_parts(a.panel->groups * (width / kCombineCols)));
@@ -821,11 +1024,16 @@ hipError_t spmm_launch(const SpmmArgs& a, void* workspace, hipStream_t s) {
   p.pass = 0;
   p.splits = static_cast<const uint32_t*>(workspace);
   const bool panel = a.panel != nullptr && !a.ordered && spmm_plain_applies(a);
-  p.skip_min = panel ? a.panel->long_min : INT64_MAX;
+  const bool opanel = a.panel != nullptr && a.ordered && spmm_plain_applies(a);
+  p.skip_min = panel || opanel ? a.panel->long_min : INT64_MAX;
+  p.inner = a.a.inner;
   const bool i32 = a.a.index_bytes == 4;
   const bool f64 = a.a.elem == Elem::F64;
   if (!f64 && a.a.elem != Elem::F32) return hipErrorInvalidValue;
   const bool plain = !a.ordered && spmm_plain_applies(a);
+  // The ordered form on the plain form's operands runs row by row with 16
+  // lanes a row (the same chains as the wide kernel).
+  const bool rows_ordered = a.ordered && spmm_plain_applies(a);
   // The row-by-row kernels over every row (or, with a panel plan, the rows
   // it does not list).
   const auto run_rows = [&](hipStream_t st) -> hipError_t {
@@ -848,6 +1056,11 @@ hipError_t spmm_launch(const SpmmArgs& a, void* workspace, hipStream_t s) {
       if (plain)
         e = f64 ? (i32 ? launch_plain<double, int32_t>(p, st) : launch_plain<double, int64_t>(p, st))
                 : (i32 ? launch_plain<float, int32_t>(p, st) : launch_plain<float, int64_t>(p, st));
+      else if (rows_ordered)
+        e = f64 ? (i32 ? launch_rows_ordered<double, int32_t>(p, st)
+                       : launch_rows_ordered<double, int64_t>(p, st))
+                : (i32 ? launch_rows_ordered<float, int32_t>(p, st)
+                       : launch_rows_ordered<float, int64_t>(p, st));
       else
         e = f64 ? (i32 ? launch_typed<double, int32_t>(p, st) : launch_typed<double, int64_t>(p, st))
                 : (i32 ? launch_typed<float, int32_t>(p, st) : launch_typed<float, int64_t>(p, st));
@@ -855,6 +1068,15 @@ hipError_t spmm_launch(const SpmmArgs& a, void* workspace, hipStream_t s) {
     }
     return hipSuccess;
   };
+  if (opanel && a.panel->nlong > 0) {
+    const hipError_t e =
+        f64 ? (i32 ? launch_panel_ordered<double, int32_t>(a, *a.panel, s)
+                   : launch_panel_ordered<double, int64_t>(a, *a.panel, s))
+            : (i32 ? launch_panel_ordered<float, int32_t>(a, *a.panel, s)
+                   : launch_panel_ordered<float, int64_t>(a, *a.panel, s));
+    if (e != hipSuccess) return e;
+    return run_rows(s);
+  }
   if (!panel || a.panel->nlong == 0) return run_rows(s);
   char* pws = static_cast<char*>(workspace) + split_bytes(a);
   const auto run_listed = [&](hipStream_t st) -> hipError_t {

@@ -426,8 +426,8 @@ def bench_spmm(ctx, m, a, lens, extra, guard, steps=3):
         ctx.fill(cmat, 0)
         H.standard_spmm(ctx, H.OP_N, 1.0, a, bd, cd, sws, alg=alg)
         cgot = ctx.download(cmat, np.float64, m.rows * k).reshape(m.rows, k)
-        if form == "ordered":
-            spmm_guard["ordered_rows_bit_exact"] = sum(
+        if form.startswith("ordered"):
+            spmm_guard[f"{form}_rows_bit_exact"] = sum(
                 int(np.array_equal(serial[r].view(np.uint64), cgot[r].view(np.uint64)))
                 for r in pick)
         else:
@@ -453,6 +453,8 @@ def bench_spmm(ctx, m, a, lens, extra, guard, steps=3):
     ctx.synchronize()
     panel_analysis_ms = (time.perf_counter() - t0) * 1e3
     got_panel = run_form("panel", H.ALG_PANEL)
+    # The ordered form on the same analysis: the panels in order, the same bits.
+    run_form("ordered_panel", H.ALG_ORDERED)
     # AUTO now takes the panel form: the same bits on the sampled rows.
     ctx.fill(cmat, 0)
     H.standard_spmm(ctx, H.OP_N, 1.0, a, bd, cd, sws, alg=H.ALG_AUTO)
@@ -474,6 +476,7 @@ def bench_spmm(ctx, m, a, lens, extra, guard, steps=3):
                        "analysis_ms": round(panel_analysis_ms, 1)},
         "ordered_gbs": round(bs / spmm_ms["ordered"] / 1e6, 1),
         "ordered_ms": round(spmm_ms["ordered"], 3),
+        "ordered_panel_ms": round(spmm_ms["ordered_panel"], 3),
         "b_row_gathers_gb": round(gathered / 1e9, 1),
         "b_row_gather_gbs": round(gathered / spmm_ms["plain"] / 1e6, 1),
         "b_row_gather_ceiling": ceiling,
@@ -717,8 +720,8 @@ def guard_failures(guard):
     for k, v in guard.items():
         if v.get("rows_over_bound", 0) != 0:
             bad.append(k)
-        for count in ("rows_bit_exact", "ordered_rows_bit_exact", "plain_rows_within_bound",
-                      "panel_rows_within_bound"):
+        for count in ("rows_bit_exact", "ordered_rows_bit_exact", "ordered_panel_rows_bit_exact",
+                      "plain_rows_within_bound", "panel_rows_within_bound"):
             if count in v and v[count] != v["rows_checked"]:
                 bad.append(k)
         if any(val is False for val in v.values()):

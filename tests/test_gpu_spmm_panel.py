@@ -204,3 +204,42 @@ def test_spmm_panel_alpha_zero_leaves_c(ctx):
         np.testing.assert_array_equal(got.view(np.uint64), c0.view(np.uint64))
     del plan
     H.plan_clear(ctx)
+
+
+ORDERED_CASES = [
+    # (rows, cols, zipf_max, long_row_min, panel_columns, k, ld, dtype, index_bytes)
+    (1 << 15, 1 << 15, 10_000, 1024, 0, 32, 32, np.float64, 4),
+    (1 << 15, 1 << 15, 10_000, 512, 2048, 20, 24, np.float64, 8),
+    (5000, 70_000, 3000, 1000, 97, 16, 16, np.float32, 4),
+    (5000, 70_000, 3000, 2000, 0, 4, 8, np.float64, 4),
+]
+
+
+@pytest.mark.parametrize("rows,cols,zmax,long_min,pcols,k,ld,dtype,ib", ORDERED_CASES)
+def test_spmm_ordered_over_panels_bit_exact(ctx, rows, cols, zmax, long_min, pcols, k, ld, dtype, ib):
+    # ORDERED on a stream-staged analysis listing long rows walks the panels in
+    # order, continuing each C(i,j)'s serial chain: bit for bit the serial
+    # reference (and the ordered form without the analysis), signed zeros and
+    # the padding included.
+    m = G.make(G.ZIPF, rows, cols, seed=rows % 997 + k, values=G.VALUES_NORMAL, zipf_max=zmax)
+    val = m.values.astype(dtype)
+    val[::11] = 0.0  # zero products: -0.0 + 0 must stay the serial result
+    rng = np.random.default_rng(k + long_min)
+    b = rng.uniform(-1, 1, (cols, ld)).astype(dtype)
+    c0 = rng.uniform(-1, 1, (rows, ld)).astype(dtype)
+    c0[::5, :] = -0.0
+    a = H.clone_compressed(ctx, H.CSR, rows, cols, m.offsets, m.indices, val, KIND[dtype], ib)
+    bbuf = ctx.upload(b.reshape(-1))
+    before = spmm(ctx, a, bbuf, cols, c0, k, ld, -0.75, H.ALG_ORDERED, dtype)
+    plan = H.csr_spmv_analyze(ctx, a, H.panel_options(panel_columns=pcols, long_row_min=long_min,
+                                                      staging=H.STAGING_STREAM))
+    print(f"ordered over panels rows {rows} cols {cols} long_min {long_min} panel {pcols} k {k} "
+          f"{dtype.__name__} ib {ib}", flush=True)
+    got = spmm(ctx, a, bbuf, cols, c0, k, ld, -0.75, H.ALG_ORDERED, dtype)
+    ref = O.std_spmm(0, 0, rows, cols, m.offsets, m.indices.astype(np.int64), val, -0.75,
+                     b.reshape(-1).copy(), 1, ld, k, c0.reshape(-1).copy(), 1, ld).reshape(rows, ld)
+    bits = np.uint64 if dtype == np.float64 else np.uint32
+    np.testing.assert_array_equal(got.view(bits), ref.view(bits))
+    np.testing.assert_array_equal(got.view(bits), before.view(bits))
+    del plan
+    H.plan_clear(ctx)

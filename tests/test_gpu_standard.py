@@ -259,7 +259,8 @@ def test_spmm_plain_column_passes(ctx, op):
     """B larger than one 256 MiB slice (2.5 * 2^20 rows, three passes) with the
     passes' split table in the workspace, and the same call without a
     workspace (one pass): both within the plain bound of the serial result,
-    for A and for its transpose."""
+    for A and for its transpose; the ordered form bit for bit with both
+    workspaces."""
     k, short, long_ = 32, 1 << 14, int(2.5 * PASS_ROWS)
     rng = np.random.default_rng(11)
     if op == H.OP_N:   # 2^14 x 2.6M, ~48 entries a row spread over every pass
@@ -285,10 +286,12 @@ def test_spmm_plain_column_passes(ctx, op):
     need = C.c_size_t()
     H.check(ctx.lib.asc_sparse_hip_standard_spmm_workspace_size(
         ctx.handle, op, C.byref(a.desc), C.byref(bd), C.byref(cd), None, C.byref(need)))
-    prefix = C.c_size_t()  # the transposed operand's part (op T), passes excluded
+    # The transposed operand's part (op T) without the passes' split table:
+    # the query for a column-major B, which no column pass serves.
+    prefix = C.c_size_t()
+    bd_cm = H.dense(bb, nin, k, nin, H.COLUMN_MAJOR, H.F64, writable=False)
     H.check(ctx.lib.asc_sparse_hip_standard_spmm_workspace_size(
-        ctx.handle, op, C.byref(a.desc), C.byref(bd), C.byref(cd),
-        C.byref(H.options(H.ALG_ORDERED)), C.byref(prefix)))
+        ctx.handle, op, C.byref(a.desc), C.byref(bd_cm), C.byref(cd), None, C.byref(prefix)))
     assert need.value >= prefix.value + 2 * nout * 4  # two split rows of uint32
     for label, give in (("passes", need.value), ("one pass", prefix.value)):
         print(f"spmm plain passes op {op} {label}", flush=True)
@@ -302,3 +305,12 @@ def test_spmm_plain_column_passes(ctx, op):
         got = ctx.download(cb, np.float64, nout * k).reshape(nout, k)
         err = np.abs(got - ref)
         assert np.all(err <= bound), (label, float(np.max(err / np.maximum(bound, 1e-300))))
+        # The ordered form takes the same passes (each C(i,j)'s chain goes on
+        # pass after pass in stored order): the serial result bit for bit.
+        H.check(ctx.lib.asc_sparse_hip_copy(ctx.handle, cb.ptr, c0.ctypes.data, c0.nbytes))
+        ctx.synchronize()
+        H.check(ctx.lib.asc_sparse_hip_standard_spmm(ctx.handle, op, H.Scalar(0.75, 0.0),
+                                                     C.byref(a.desc), C.byref(bd), C.byref(cd),
+                                                     p, n, C.byref(H.options(H.ALG_ORDERED))))
+        got = ctx.download(cb, np.float64, nout * k).reshape(nout, k)
+        np.testing.assert_array_equal(got.view(np.uint64), ref.view(np.uint64))
