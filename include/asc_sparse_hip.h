@@ -565,7 +565,12 @@ asc_hip_status asc_sparse_hip_triangular_solve(asc_hip_context* context, uint8_t
  * layout (blas.h:1692-1750).  Replaces StandardSpmmErased
  * (providers/cuda.h:245-248); `options` (may be NULL) picks the form:
  *  - ASC_HIP_ALG_ORDERED: every C(i,j) is accumulated in stored order
- *    without FMA, so the result equals the serial reference bit for bit;
+ *    without FMA, so the result equals the serial reference bit for bit.
+ *    On the plain form's operands (below) it gathers B in the same column
+ *    passes when the workspace holds their split table (one pass
+ *    otherwise), and when the context holds a stream-staged analysis of
+ *    this CSR matrix listing only long rows (as for AUTO, below) it takes
+ *    those rows panel by panel; neither changes a bit of the result;
  *  - ASC_HIP_ALG_AUTO (NULL): the plain form (below ASC_HIP_ALG_PANEL for
  *    when AUTO takes the panel form).  For row-major B and C with
  *    k <= 32, k and both leading dimensions multiples of 4 and B and C
