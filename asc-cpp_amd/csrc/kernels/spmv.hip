@@ -1476,6 +1476,75 @@ __global__ __launch_bounds__(kBlock) void spmv_ordered_kernel(KParams p) {
   }
 }
 
+// Ordered, 16 lanes a row (rows averaging kOrderedGroupMin entries or
+// more): the lanes load 16 entries of the row and form their terms at once,
+// then every lane adds the 16 terms to the running sum one after another
+// (each broadcast across the group by DPP), so the sum keeps the serial
+// order of the one-lane kernel above, bit for bit, while the row's loads
+// and gathers go out 16 at a time.  The next step's entries load before the
+// current step's additions.
+constexpr int kOrderedLanes = 16;
+constexpr int64_t kOrderedGroupMin = 8;
+
+template <int U, typename T>
+__device__ __forceinline__ T ordered_bcast(T v) {
+  return __builtin_amdgcn_update_dpp(T(0), v, 0x150 + U, 0xF, 0xF, false);
+}
+
+template <typename T, int U>
+__device__ __forceinline__ void ordered_add_step(T term, int64_t n, T* acc) {
+#pragma clang fp contract(off)
+  if constexpr (U < kOrderedLanes) {
+    const T t = ordered_bcast<U>(term);
+    if (U < n) *acc = *acc + t;
+    ordered_add_step<T, U + 1>(term, n, acc);
+  }
+}
+
+template <typename T, typename I>
+__global__ __launch_bounds__(kBlock) void spmv_ordered16_kernel(KParams p) {
+#pragma clang fp contract(off)
+  constexpr int G = kOrderedLanes;
+  const int64_t row = (xcd_remap(blockIdx.x, gridDim.x) * kBlock + threadIdx.x) / G;
+  if (row >= p.rows) return;  // whole group
+  const int j = threadIdx.x % G;
+  const T* __restrict__ val = static_cast<const T*>(p.values);
+  const I* __restrict__ col = static_cast<const I*>(p.indices);
+  const T* __restrict__ x = static_cast<const T*>(p.x);
+  T* y = static_cast<T*>(p.y) + row * p.incy;
+  const int64_t b = p.offsets[row];
+  const int64_t e = p.offsets[row + 1];
+  const T alpha = static_cast<T>(p.alpha);
+  // The term of entry k: v*x (compat, reference_blas.cc:22-38) or
+  // (alpha*v)*x (standard, blas.h:1653-1665).
+  const auto term_of = [&](int64_t k) {
+    if (k >= e) return T(0);
+    const T xv = x[static_cast<int64_t>(col[k]) * p.incx];
+    const T v = p.form == 0 ? val[k] : alpha * val[k];
+    return v * xv;
+  };
+  T acc = p.form == 0 ? T(0) : *y;
+  T term = term_of(b + j);
+  for (int64_t t = b; t < e; t += G) {
+    const T cur = term;
+    term = term_of(t + G + j);
+    ordered_add_step<T, 0>(cur, e - t, &acc);
+  }
+  if (j == 0) {
+    if (p.form == 0) {
+      T result = alpha * acc;
+      const T beta = static_cast<T>(p.beta);
+      if (beta != T(0)) {
+        const T scaled = beta * *y;
+        result = result + scaled;
+      }
+      *y = result;
+    } else {
+      *y = acc;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // merge-path, wave-level and persistent
 //
@@ -1874,7 +1943,10 @@ hipError_t launch_typed(const SpmvArgs& a, const KParams& kp, void* ws,
     case SpmvAlg::Ordered: {
       if constexpr (!TWOSUM && (std::is_same<V, double>::value ||
                                 std::is_same<V, float>::value)) {
-        spmv_ordered_kernel<V, I><<<blocks_for(rows, kBlock), kBlock, 0, s>>>(kp);
+        if (a.a.nnz >= kOrderedGroupMin * rows)
+          spmv_ordered16_kernel<V, I><<<blocks_for(rows, kBlock / kOrderedLanes), kBlock, 0, s>>>(kp);
+        else
+          spmv_ordered_kernel<V, I><<<blocks_for(rows, kBlock), kBlock, 0, s>>>(kp);
         return hipGetLastError();
       }
       return hipErrorInvalidValue;
