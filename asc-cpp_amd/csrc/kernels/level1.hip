@@ -33,18 +33,42 @@ __global__ __launch_bounds__(kDotBlock) void dot_plain_kernel(
   if (threadIdx.x == 0) *out = s_part[0];
 }
 
+// Lane u's value (u uniform over the wave), for every lane.
+__device__ __forceinline__ float lane_value(float t, int u) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(t), u));
+}
+__device__ __forceinline__ double lane_value(double t, int u) {
+  const int64_t b = __double_as_longlong(t);
+  const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(b), u));
+  const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(b >> 32), u));
+  return __longlong_as_double(static_cast<int64_t>((static_cast<uint64_t>(hi) << 32) | lo));
+}
+
 template <typename T>
-__global__ void dot_ordered_kernel(const int64_t* __restrict__ idx, const T* __restrict__ v,
-                                   const T* __restrict__ x, int64_t inc, int64_t nnz,
-                                   T* __restrict__ out) {
+__global__ __launch_bounds__(kWave) void dot_ordered_kernel(const int64_t* __restrict__ idx,
+                                                            const T* __restrict__ v,
+                                                            const T* __restrict__ x, int64_t inc,
+                                                            int64_t nnz, T* __restrict__ out) {
 #pragma clang fp contract(off)
-  if (threadIdx.x != 0) return;
+  // One wave: its lanes form 64 terms v[p]*x[idx[p]] at once (the next 64
+  // loading meanwhile), and the running sum takes them one after another in
+  // entry order, each read from its lane: the serial sum, bit for bit.
+  const int lane = static_cast<int>(threadIdx.x);
+  const auto term_of = [&](int64_t p) { return p < nnz ? v[p] * x[idx[p] * inc] : T(0); };
   T r = T(0);
-  for (int64_t p = 0; p < nnz; ++p) {
-    const T term = v[p] * x[idx[p] * inc];
-    r = r + term;
+  T term = term_of(lane);
+  for (int64_t p0 = 0; p0 < nnz; p0 += kWave) {
+    const T cur = term;
+    term = term_of(p0 + kWave + lane);
+    if (nnz - p0 >= kWave) {
+#pragma unroll
+      for (int u = 0; u < kWave; ++u) r = r + lane_value(cur, u);
+    } else {
+      const int n = static_cast<int>(nnz - p0);
+      for (int u = 0; u < n; ++u) r = r + lane_value(cur, u);
+    }
   }
-  *out = r;
+  if (lane == 0) *out = r;
 }
 
 template <typename T>
